@@ -1,0 +1,352 @@
+"""Pins the CPU oracle against libzfp fixtures, closed forms and brute force.
+
+The reference has no tests or golden vectors of its own, so every oracle
+function is pinned here by something independent of its own formula.
+"""
+import hashlib
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from bcmc_amd import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FIX = np.load(os.path.join(ROOT, "tests", "golden", "zfp_fixtures.npz"))
+META = json.loads(bytes(FIX["meta"]).decode())
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    oracle.build()
+
+
+# ---------------------------------------------------------------- tri table
+def test_tri_table_hash_and_histogram():
+    # js/tri_table.js:1-181; sha256 of ','.join(ints) and the per-case vertex
+    # count histogram of the reference table.
+    ints = oracle.tri_table_ints()
+    assert len(ints) == 4096
+    h = hashlib.sha256(",".join(map(str, ints)).encode()).hexdigest()[:16]
+    assert h == "83857ee79d1da622"
+    counts = oracle.tri_counts()
+    hist = {int(k): int(v) for k, v in zip(*np.unique(counts, return_counts=True))}
+    assert hist == {0: 2, 3: 16, 6: 50, 9: 80, 12: 76, 15: 32}
+    assert counts.sum() == 2460
+
+
+def test_kernel_tri_table_matches():
+    # The packed constant-memory table of the HIP kernels decodes to the same rows.
+    txt = open(os.path.join(ROOT, "webgpu-bcmc_amd", "csrc", "tri_table.h")).read()
+    edges_txt = txt[txt.index("BCMC_TRI_EDGES_INIT"):txt.index("BCMC_TRI_COUNT_INIT")]
+    count_txt = txt[txt.index("BCMC_TRI_COUNT_INIT"):]
+    edges = [int(v, 16) for v in re.findall(r"0x([0-9a-f]+)ull", edges_txt)]
+    counts = [int(v) for v in re.findall(r"\b(\d+),", count_txt)]
+    assert len(edges) == 256 and len(counts) == 256
+    ints = oracle.tri_table_ints()
+    for c in range(256):
+        row = [(edges[c] >> (4 * k)) & 15 for k in range(counts[c])]
+        assert row == ints[16 * c:16 * c + counts[c]]
+        assert ints[16 * c + counts[c]] == -1
+
+
+# ---------------------------------------------------------------- zfp decode
+@pytest.mark.parametrize("name", sorted(META["cases"]))
+def test_decode_matches_libzfp(name):
+    m = META["cases"][name]
+    vol = oracle.decode_volume(FIX["stream_" + name], m["maxbits"], m["dims"])
+    dec = FIX["decoded_" + name]
+    nz, ny, nx = dec.shape
+    assert np.array_equal(vol[:nz, :ny, :nx].view(np.uint32), dec.view(np.uint32))
+
+
+@pytest.mark.parametrize("name", sorted(META["cases"]))
+def test_encoder_matches_libzfp(name):
+    m = META["cases"][name]
+    s = synth.encode(FIX["field_" + name], m["maxbits"])
+    assert s.tobytes() == FIX["stream_" + name].tobytes()
+    assert hashlib.sha256(s.tobytes()).hexdigest() == m["sha256"]
+
+
+def test_decode_closed_forms():
+    # All-zero stream: flag bit 0 -> +0.0; constant block at a high rate is exact.
+    assert np.all(oracle.decode_block(np.zeros(64, np.uint8), 512, 0) == 0)
+    vol = np.full((4, 4, 4), 0.375, np.float32)
+    s = synth.encode(vol, 32 * 64)
+    assert np.all(oracle.decode_block(s, 32 * 64, 0) == np.float32(0.375))
+    # Fixed-rate addressing: block k occupies bits [k*maxbits, (k+1)*maxbits).
+    f = synth.field(synth.GAUSS, 5, (8, 8, 8), 3, (2, 4, 0.5, 1))
+    for rate in (1, 2, 4, 8):
+        s = synth.encode(f, 64 * rate)
+        for k in range(8):
+            one = s[k * 8 * rate:(k + 1) * 8 * rate]
+            assert np.array_equal(oracle.decode_block(one, 64 * rate, 0),
+                                  oracle.decode_block(s, 64 * rate, k))
+
+
+# ---------------------------------------------------------------- ranges / active
+def _ranges_bruteforce(vol_padded, dims):
+    pz, py, px = vol_padded.shape
+    nbx, nby, nbz = px // 4, py // 4, pz // 4
+    out = np.zeros((nbz * nby * nbx, 2), np.float32)
+    for bz in range(nbz):
+        for by in range(nby):
+            for bx in range(nbx):
+                sub = vol_padded[bz * 4:min(bz * 4 + 4, dims[2]), by * 4:min(by * 4 + 4, dims[1]),
+                                 bx * 4:min(bx * 4 + 4, dims[0])]
+                out[bx + nbx * (by + nby * bz)] = (sub.min(), sub.max())
+    return out
+
+
+@pytest.mark.parametrize("name", ["partial_13x10x7_r4", "partial_20x12x36_r4", "smooth16_r8"])
+def test_ranges_bruteforce(name):
+    m = META["cases"][name]
+    s = FIX["stream_" + name]
+    vol = oracle.decode_volume(s, m["maxbits"], m["dims"])
+    r = oracle.block_ranges(s, m["maxbits"], m["dims"])
+    assert np.array_equal(r, _ranges_bruteforce(vol, m["dims"]))
+
+
+def _one_sided(vol, z0, z1, y0, y1, x0, x1, iso):
+    sub = vol[z0:z1, y0:y1, x0:x1]
+    return bool(np.all(sub <= iso) or np.all(sub > iso))
+
+
+def test_active_semantics():
+    # Semantic pins of compute_block_active.comp: (1) a block whose own range
+    # contains iso is active; (2) a block is inactive only if it and every
+    # in-bounds neighbour lie strictly on one common side of iso.
+    f = synth.field(synth.GAUSS, 3, (32, 28, 24), 6, (3, 8, 0.5, 1))
+    s = synth.encode(f, 4 * 64)
+    dims = (32, 28, 24)
+    r = oracle.block_ranges(s, 256, dims)
+    nb = (8, 7, 6)
+    for iso in (0.1, 0.3, 0.55):
+        a = oracle.block_active(r, dims, iso)
+        isof = np.float32(iso)
+        for b in range(len(a)):
+            lo, hi = r[b]
+            if lo <= isof <= hi:
+                assert a[b] == 1
+            bx, by, bz = b % nb[0], (b // nb[0]) % nb[1], b // (nb[0] * nb[1])
+            nbrs = [(bx + i, by + j, bz + k) for k in (-1, 0, 1) for j in (-1, 0, 1)
+                    for i in (-1, 0, 1)
+                    if 0 <= bx + i < nb[0] and 0 <= by + j < nb[1] and 0 <= bz + k < nb[2]]
+            ids = [x + nb[0] * (y + nb[1] * z) for x, y, z in nbrs]
+            above = all(r[i][0] > isof for i in ids)
+            below = all(r[i][1] < isof for i in ids)
+            assert a[b] == (0 if (above or below) else 1)
+
+
+# ---------------------------------------------------------------- scan / compact
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 511, 512, 513, 262143, 262145])
+def test_scan_compact_closed_form(n):
+    rng = np.random.default_rng(n)
+    flags = (rng.uniform(0, 1, n) < 0.3).astype(np.uint32)
+    off, total = oracle.exclusive_scan(flags)
+    ref = np.concatenate([[0], np.cumsum(flags.astype(np.uint64))[:-1]]).astype(np.uint32)
+    assert np.array_equal(off, ref) and total == int(flags.sum())
+    assert np.array_equal(oracle.compact(flags, off), np.flatnonzero(flags).astype(np.uint32))
+
+
+def test_scan_wraps_u32():
+    d = np.full(5, 0x80000000, np.uint32)
+    off, total = oracle.exclusive_scan(d)
+    assert list(off) == [0, 0x80000000, 0, 0x80000000, 0] and total == 0x80000000
+
+
+# ---------------------------------------------------------------- sort
+def _sort_closed_form(keys, reverse):
+    """Order derived from radix_sort_chunk.comp (stable), merge_sorted_chunks.comp
+    (equal keys from the later run first) and reverse_buffer.comp over the
+    UINT_MAX-padded aligned array; the slice read back per
+    radix_sort_by_key.js:382-391."""
+    n = len(keys)
+    a = oracle.sort_aligned_size(n)
+    k = np.full(a, 0xFFFFFFFF, np.uint64)
+    k[:n] = keys
+    pos = np.arange(a)
+    vals = np.full(a, 0xFFFFFFFF, np.int64)
+    vals[:n] = np.arange(n)
+    asc = sorted(range(a), key=lambda j: (k[j], -(pos[j] // 64), pos[j] % 64))
+    order = asc[::-1] if reverse else asc
+    sl = order[a - n:] if reverse else order[:n]
+    return np.array([vals[j] for j in sl], np.int64)
+
+
+@pytest.mark.parametrize("n", [1, 5, 64, 65, 130, 300, 1000])
+@pytest.mark.parametrize("reverse", [False, True])
+@pytest.mark.parametrize("keyset", ["distinct", "few", "equal", "with_max"])
+def test_sort_matches_closed_form(n, reverse, keyset):
+    rng = np.random.default_rng(n + 7 * reverse)
+    if keyset == "distinct":
+        keys = rng.permutation(n * 3)[:n].astype(np.uint32)
+    elif keyset == "few":
+        keys = rng.integers(0, 4, n).astype(np.uint32)
+    elif keyset == "equal":
+        keys = np.full(n, 100000, np.uint32)
+    else:
+        keys = rng.integers(0, 3, n).astype(np.uint32)
+        keys[rng.uniform(0, 1, n) < 0.3] = 0xFFFFFFFF
+    got, _ = oracle.sort_by_key(keys, np.arange(n, dtype=np.uint32), reverse)
+    want = _sort_closed_form(keys, reverse)
+    assert np.array_equal(got.astype(np.int64) & 0xFFFFFFFF, want & 0xFFFFFFFF)
+    if keyset == "distinct":
+        ref = np.argsort(keys, kind="stable")
+        assert np.array_equal(got, (ref[::-1] if reverse else ref).astype(np.uint32))
+
+
+def test_sort_single_chunk_ties_reverse_input_order():
+    # n <= 64: one chunk, no merges; reverse gives equal keys in reverse input order.
+    keys = np.array([5, 3, 5, 3, 5], np.uint32)
+    got, _ = oracle.sort_by_key(keys, np.arange(5, dtype=np.uint32), True)
+    assert list(got) == [4, 2, 0, 3, 1]
+
+
+# ---------------------------------------------------------------- LRU
+def _needed_sets(total, steps, rng):
+    out = []
+    for _ in range(steps):
+        f = np.zeros(total, np.uint32)
+        lo = rng.integers(0, total)
+        f[lo:lo + rng.integers(1, total // 3)] = 1
+        f[rng.uniform(0, 1, total) < 0.05] = 1
+        out.append(f)
+    return out
+
+
+@pytest.mark.parametrize("initial", [32, 200, 1000])
+def test_lru_invariants(initial):
+    total = 1000
+    rng = np.random.default_rng(initial)
+    lru = oracle.LRU(initial, 256, total)
+    age, avail, item, slot = lru.state()
+    assert lru.size == (initial + 31) // 32 * 32
+    assert np.array_equal(age, np.arange(lru.size) + 2) and np.all(avail == 1) and np.all(item == -1)
+    for needed in _needed_sets(total, 12, rng):
+        _, _, item_b, slot_b = lru.state()
+        size_b = lru.size
+        cached_before = set(np.flatnonzero(slot_b >= 0))
+        n_new, new_items = lru.update(needed)
+        age, avail, item, slot = lru.state()
+        want_new = sorted(set(np.flatnonzero(needed)) - cached_before)
+        assert n_new == len(want_new) and list(new_items) == want_new
+        # every needed item is cached; maps are mutually inverse
+        assert np.all(slot[needed == 1] >= 0)
+        for s in range(lru.size):
+            if item[s] >= 0:
+                assert slot[item[s]] == s
+        for i in np.flatnonzero(slot >= 0):
+            assert item[slot[i]] == i
+        # hits and new items have age 0 and are unavailable
+        assert np.all(age[slot[needed == 1]] == 0) and np.all(avail[slot[needed == 1]] == 0)
+        # grow formula (lru_cache.js:518-522)
+        if lru.size != size_b:
+            n_avail = int(np.sum([1 for s in range(size_b)
+                                  if item_b[s] < 0 or needed[item_b[s]] == 0]))
+            want = min(size_b + -(-(n_new - n_avail) * 3 // 2), total)
+            assert lru.size == (want + 31) // 32 * 32
+
+
+def test_lru_first_misses_take_highest_slots():
+    lru = oracle.LRU(64, 256, 100)
+    f = np.zeros(100, np.uint32)
+    f[[3, 7, 9]] = 1
+    n, ids = lru.update(f)
+    _, _, item, slot = lru.state()
+    # initial ages 2+s, oldest = highest slot
+    assert n == 3 and list(slot[[3, 7, 9]]) == [63, 62, 61]
+
+
+def test_lru_evicts_oldest_available():
+    lru = oracle.LRU(32, 256, 64)
+    for i in range(64):
+        f = np.zeros(64, np.uint32)
+        f[i] = 1
+        lru.update(f)
+    age, avail, item, slot = lru.state()
+    # 64 single-item queries through 32 slots: the last 32 items remain cached.
+    assert set(item) == set(range(32, 64))
+
+
+# ---------------------------------------------------------------- marching cubes
+def _mc_case(dims, kind, seed, rate, count, params):
+    f = synth.field(kind, seed, dims, count, params)
+    s = synth.encode(f, 64 * rate)
+    return f, s
+
+
+@pytest.mark.parametrize("dims,rate,iso", [((32, 32, 32), 4, 0.3), ((24, 16, 40), 8, 0.5),
+                                           ((16, 16, 16), 2, 0.45)])
+def test_mc_count_equals_bruteforce(dims, rate, iso):
+    # For dims = 0 mod 4, the block/ghost algorithm emits exactly the vertices
+    # of a full-volume marching cubes over all cells with the same table.
+    _, s = _mc_case(dims, synth.GAUSS, 11, rate, 8, (3, 10, 0.5, 1))
+    mc = oracle.MC(s, 64 * rate, dims, initial_cache=64)
+    v = mc.compute_surface(iso)
+    vol = oracle.decode_volume(s, 64 * rate, dims)
+    assert v == oracle.bruteforce_count(vol, iso) and v > 0
+
+
+def test_mc_vertex_positions():
+    dims, rate, iso = (32, 24, 20), 4, 0.4
+    _, s = _mc_case(dims, synth.GAUSS, 12, rate, 8, (3, 10, 0.5, 1))
+    mc = oracle.MC(s, 64 * rate, dims, initial_cache=32)
+    v = mc.compute_surface(iso)
+    res = mc.results()
+    p32, p64 = res["verts_f32"], res["verts_f64"]
+    assert p32.shape == (v, 4) and v > 0
+    assert np.max(np.abs(p32[:, :3].astype(np.float64) - p64)) <= 1e-5
+    # every vertex lies on a cell edge: two coordinates are k + 0.5
+    frac = np.abs(p64 - np.floor(p64) - 0.5) < 1e-12
+    assert np.all(frac.sum(axis=1) >= 2)
+    # the field interpolated linearly along that edge equals iso at the vertex
+    vol = oracle.decode_volume(s, 64 * rate, dims)
+    bid = p32[:, 3].view(np.uint32)
+    nb = [(d + 3) // 4 for d in dims]
+    g = p64 - 0.5 + 4 * np.stack([bid % nb[0], (bid // nb[0]) % nb[1], bid // (nb[0] * nb[1])], 1)
+    ax = np.argmin(frac, axis=1)
+    lo = np.floor(g).astype(np.int64)
+    hi = lo.copy()
+    hi[np.arange(v), ax] += 1
+    fa = vol[lo[:, 2], lo[:, 1], lo[:, 0]].astype(np.float64)
+    fb = vol[hi[:, 2], hi[:, 1], hi[:, 0]].astype(np.float64)
+    t = g[np.arange(v), ax] - lo[np.arange(v), ax]
+    interp = fa + t * (fb - fa)
+    flat = np.abs(fa - fb) < 1e-3
+    assert np.all(np.abs(interp[~flat] - iso) < 1e-6)
+    # blocks appear in ascending id order (A.8)
+    assert np.all(np.diff(bid.astype(np.int64)) >= 0)
+
+
+def test_mc_lists_and_offsets():
+    dims, rate = (32, 32, 32), 4
+    _, s = _mc_case(dims, synth.GAUSS, 13, rate, 6, (4, 10, 0.5, 1))
+    mc = oracle.MC(s, 64 * rate, dims, initial_cache=64)
+    r = oracle.block_ranges(s, 256, dims)
+    for iso in (0.2, 0.35, 0.5):
+        v = mc.compute_surface(iso)
+        res = mc.results()
+        flags = oracle.block_active(r, dims, iso)
+        assert np.array_equal(res["flags"], flags)
+        assert np.array_equal(res["active_ids"], np.flatnonzero(flags))
+        off = res["vert_offsets"]
+        counts = np.diff(np.append(off.astype(np.int64), v))
+        assert np.all(counts > 0) and np.all(np.diff(res["with_verts"].astype(np.int64)) > 0)
+        bid = res["verts_f32"][:, 3].view(np.uint32)
+        # vertices of block h are contiguous at off[h]
+        for h in range(0, len(off), max(1, len(off) // 17)):
+            b = res["active_ids"][res["with_verts"][h]]
+            assert np.all(bid[off[h]:off[h] + counts[h]] == b)
+
+
+def test_mc_no_active_skips_cache_update():
+    dims = (16, 16, 16)
+    _, s = _mc_case(dims, synth.GAUSS, 14, 4, 4, (3, 6, 0.5, 1))
+    mc = oracle.MC(s, 256, dims, initial_cache=32)
+    age0 = mc.lru_state()[0].copy()
+    assert mc.compute_surface(2.0) == 0 and mc.stats["num_active"] == 0
+    assert np.array_equal(mc.lru_state()[0], age0)
