@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Benchmark of the block-compressed marching-cubes path on one MI355X.
+
+Metric: iso-queries/s (computeSurface calls per second) of the plain per-query
+pipeline on the 512^3 turbulence-like workload: sum of 512 random Fourier
+modes (|k| in [1,64], amplitude |k|^-5/6, seed 1) normalised to [0,1], zfp
+fixed rate 4 bits/value, LRU cache initially 25% of the blocks, a sequential
+sweep of 8 isovalues linspace(0.2, 0.8).  One step = cache reset + the 8-query
+sweep.  The compressed volume is resident in HBM before timing starts.
+
+Also reported: the ZFP decode rate of the cache-decode kernel and of the
+load-time range pass, the HBM roofline fraction of the dominant kernel, p50/p99
+query latency, and the CPU oracle (the reference method restated in C, OpenMP)
+timed on this host on a bounded sample of the same sweep.
+
+Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "webgpu-bcmc_amd", "python"))
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def stage_bytes(s, rate, n_blocks):
+    """Algorithmic HBM bytes per stage of one query (per-stage model of the
+    reference's kernels)."""
+    a, d, h, v = s["numActiveBlocks"], s["numBlocksDecompressed"], s["numBlocksWithVertices"], \
+        s["numVertices"]
+    return {
+        "computeActiveBlocks": 8 * n_blocks + 4 * n_blocks + 4 * a,
+        "decompression": d * (8 * rate + 4 + 4 + 256),
+        "computeBlockHasVertices": a * (256 + 244 + 4 + 4),
+        "compactBlocksWithVertices": 4 * a + 12 * h,
+        "computeVertices": h * (256 + 244 + 8 + 4) + 16 * v,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="fourier512_r4")
+    ap.add_argument("--cpu-queries", type=int, default=2,
+                    help="queries of the sweep timed on the CPU oracle (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the path shards by z-slab or by volume; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import bcmc_amd
+    from bcmc_amd import synth
+
+    w = synth.WORKLOADS[args.workload]
+    dims, rate = w["dims"], w["rate"]
+    n_blocks = int(np.prod([(d + 3) // 4 for d in dims]))
+    t0 = time.time()
+    stream, sha = synth.make_stream(w)
+    log(f"bench: generated {args.workload} {dims} rate {rate}: {stream.nbytes} B sha256 {sha[:16]} "
+        f"in {time.time() - t0:.1f}s")
+
+    dev = bcmc_amd.Device(0)
+    info = dev.info()
+    cache0 = n_blocks // 4
+    isos = np.linspace(0.2, 0.8, 8).astype(np.float32)
+
+    # load-time range pass (full-volume decode) timed by itself
+    sbuf = dev.buffer_from(np.concatenate([stream, np.zeros(16, np.uint8)]))
+    bcmc_amd.lib().bcmc_ctx_enable_timing(dev.h, 1)
+    range_ms = []
+    for _ in range(5):
+        bcmc_amd.block_ranges(dev, sbuf, 64 * rate, dims)
+        range_ms.append(bcmc_amd.lib().bcmc_ctx_last_kernel_ms(dev.h))
+    bcmc_amd.lib().bcmc_ctx_enable_timing(dev.h, 0)
+    range_t = float(np.median(range_ms[1:]))
+    del sbuf
+
+    mc = bcmc_amd.CompressedMarchingCubes(dev)
+    mc.setCompressedVolume(stream, rate, dims, (1, 1, 1), cache0)
+
+    def step(records):
+        mc.lruCache.reset()
+        for iso in isos:
+            t = time.perf_counter()
+            nv = mc.computeSurface(float(iso))
+            dt = time.perf_counter() - t
+            if records is not None:
+                rec = dict(mc.lastPerf)
+                rec["wall_ms"] = dt * 1e3
+                rec["iso"] = float(iso)
+                records.append(rec)
+        return nv
+
+    for i in range(args.warmup):
+        step(None)
+        log(f"bench: warmup step {i + 1}/{args.warmup} done")
+    dev.sync()
+    records = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        step(records)
+    dev.sync()
+    elapsed = time.perf_counter() - t_start
+    log(f"bench: {args.steps} steps x {len(isos)} queries in {elapsed:.3f}s")
+    queries = args.steps * len(isos)
+    qps = queries / elapsed
+    lat = np.array([r["wall_ms"] for r in records])
+
+    # per-stage device time and algorithmic bytes over the timed queries
+    stages = ["computeActiveBlocks", "cacheUpdate", "decompression", "computeBlockHasVertices",
+              "compactBlocksWithVertices", "computeVertices"]
+    st_ms = {k: float(sum(r[k] for r in records)) for k in stages}
+    st_bytes = {k: 0 for k in stages}
+    for r in records:
+        for k, b in stage_bytes(r, rate, n_blocks).items():
+            st_bytes[k] += b
+    dom = max((k for k in stages if k != "cacheUpdate"), key=lambda k: st_ms[k])
+    gbps = {k: (st_bytes[k] / (st_ms[k] * 1e-3) / 1e9 if st_ms[k] > 0 else 0.0)
+            for k in stages if k != "cacheUpdate"}
+    decode_bytes_range = n_blocks * (8 * rate + 8)
+
+    out = {
+        "metric": "iso-queries/s (plain computeSurface, 1 GPU)",
+        "value": round(qps, 3),
+        "unit": "iso-queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {dims[0]}x{dims[1]}x{dims[2]} Fourier-mode field, "
+                               f"zfp rate {rate}, 8-isovalue sweep 0.2..0.8, LRU 25% initial",
+                   "queries_per_step": len(isos), "blocks": n_blocks,
+                   "stream_sha256": sha, "device": info["name"], "cus": info["num_cus"]},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                       "p99": round(float(np.percentile(lat, 99)), 3),
+                       "mean": round(float(lat.mean()), 3)},
+        "stage_ms_per_query": {k: round(v / queries, 4) for k, v in st_ms.items()},
+        "stage_GBps": {k: round(v, 1) for k, v in gbps.items()},
+        "zfp_decode": {
+            "cache_decode_GBps": round(gbps["decompression"], 1),
+            "cache_decode_frac_of_peak": round(gbps["decompression"] / HBM_PEAK_GBPS, 4),
+            "blocks_decoded_per_query": round(sum(r["numBlocksDecompressed"] for r in records) / queries, 1),
+            "range_pass_ms": round(range_t, 4),
+            "range_pass_GBps": round(decode_bytes_range / (range_t * 1e-3) / 1e9, 1),
+            "range_pass_decoded_equiv_GBps": round(256 * n_blocks / (range_t * 1e-3) / 1e9, 1),
+        },
+        "roofline": {"bound": "hbm", "kernel": dom,
+                     "achieved": round(gbps[dom], 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(gbps[dom] / HBM_PEAK_GBPS, 4), "traffic": None},
+        "counts_per_query": {k: round(float(np.mean([r[k] for r in records])), 1) for k in
+                             ("numActiveBlocks", "numBlocksDecompressed", "numBlocksWithVertices",
+                              "numVertices")},
+        "cpu_baseline": None,
+    }
+
+    if args.cpu_queries > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        t = time.time()
+        ref = oracle.MC(stream, 64 * rate, dims, initial_cache=cache0, threads=threads)
+        t_load = time.time() - t
+        tq = time.time()
+        for iso in isos[:args.cpu_queries]:
+            ref.compute_surface(float(iso))
+        t_q = time.time() - tq
+        out["cpu_baseline"] = {
+            "value": round(args.cpu_queries / t_q, 4), "unit": "iso-queries/s", "cores": threads,
+            "kind": "port",
+            "sample": f"first {args.cpu_queries} isovalues of the same sweep after a cold LRU "
+                      f"(same stream); C oracle, OpenMP {threads} threads; load-time range "
+                      f"pass {t_load:.2f}s not included"}
+        log(f"bench: cpu oracle {args.cpu_queries} queries in {t_q:.2f}s with {threads} threads")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

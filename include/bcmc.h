@@ -1,0 +1,219 @@
+/* bcmc.h - C ABI of the MI355X (gfx950) block-compressed marching-cubes library.
+ *
+ * Drop-in boundary for the hot path of Twinklebear/webgpu-bcmc.  The reference
+ * exposes JS classes over a WebGPU device; each entry point below replaces one
+ * of their methods (cited as file:line) and is what a Node N-API addon or any
+ * other FFI binds.  Conventions:
+ *   - every call returns BCMC_OK (0) or a negative bcmc_status; the message of
+ *     the last failure on the calling thread is in bcmc_last_error();
+ *   - a context owns one HIP device and one stream; all work on a context runs
+ *     in issue order, like the reference's single GPUQueue;
+ *   - buffers are device allocations (the reference's GPUBuffer); objects own
+ *     their internal buffers; buffers returned to the caller are the caller's
+ *     to free (as compressed_marching_cubes.js:693 destroys newItemIDs);
+ *   - no exceptions cross the ABI; objects are not thread-safe (one object,
+ *     one caller at a time, as with the reference's async methods).
+ */
+#ifndef BCMC_H
+#define BCMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define BCMC_API __attribute__((visibility("default")))
+#else
+#define BCMC_API
+#endif
+
+typedef enum {
+    BCMC_OK = 0,
+    BCMC_ERR_INVALID = -1,   /* bad argument (alert() in the reference) */
+    BCMC_ERR_HIP = -2,       /* HIP runtime error */
+    BCMC_ERR_NOMEM = -3,     /* device or host allocation failed */
+    BCMC_ERR_OVERFLOW = -4,  /* a count exceeds what the output can hold */
+    BCMC_ERR_STATE = -5,     /* call order violated (e.g. no volume set) */
+    BCMC_ERR_NO_DEVICE = -6  /* no gfx950 device */
+} bcmc_status;
+
+typedef struct bcmc_ctx bcmc_ctx; /* GPUDevice + GPUQueue */
+typedef struct bcmc_buf bcmc_buf; /* GPUBuffer */
+typedef struct bcmc_lru bcmc_lru; /* LRUCache */
+typedef struct bcmc_mc bcmc_mc;   /* CompressedMarchingCubes */
+
+BCMC_API const char *bcmc_last_error(void);
+BCMC_API const char *bcmc_version(void);
+
+/* ---- context (navigator.gpu.requestAdapter/requestDevice, js/bcmc.js:1-20) */
+BCMC_API int bcmc_ctx_create(int device, bcmc_ctx **out);
+BCMC_API int bcmc_ctx_destroy(bcmc_ctx *ctx);
+/* queue.onSubmittedWorkDone(): wait for all work issued on the context. */
+BCMC_API int bcmc_ctx_sync(bcmc_ctx *ctx);
+/* Number of compute units and the device name (for reports). */
+BCMC_API int bcmc_ctx_info(bcmc_ctx *ctx, int *num_cus, char *name, size_t name_len);
+
+/* ---- buffers (device.createBuffer / mapAsync / copyBufferToBuffer) -------- */
+BCMC_API int bcmc_buf_alloc(bcmc_ctx *ctx, uint64_t bytes, bcmc_buf **out);
+/* Wrap caller-owned device memory (not freed by bcmc_buf_free). */
+BCMC_API int bcmc_buf_wrap(bcmc_ctx *ctx, void *device_ptr, uint64_t bytes, bcmc_buf **out);
+BCMC_API int bcmc_buf_free(bcmc_buf *buf);
+BCMC_API int bcmc_buf_upload(bcmc_buf *buf, uint64_t offset, const void *src, uint64_t bytes);
+BCMC_API int bcmc_buf_download(bcmc_buf *buf, uint64_t offset, void *dst, uint64_t bytes);
+BCMC_API int bcmc_buf_fill_u32(bcmc_buf *buf, uint64_t offset, uint32_t value, uint64_t count);
+BCMC_API uint64_t bcmc_buf_size(const bcmc_buf *buf);
+BCMC_API void *bcmc_buf_device_ptr(const bcmc_buf *buf);
+
+/* ---- ExclusiveScanPipeline / ExclusiveScanner (js/exclusive_scan.js) ------ */
+/* getAlignedSize (exclusive_scan.js:95-97): n rounded up to 512. */
+BCMC_API uint32_t bcmc_scan_aligned_size(uint32_t n);
+/* ExclusiveScanner.scan (exclusive_scan.js:178-295): in-place exclusive scan of
+ * the first n u32 of `data` (wrapping), total = sum of those n.  If n <
+ * aligned_n, data[n] receives the total, as in the reference (:271-276).
+ * aligned_n must equal bcmc_scan_aligned_size(aligned_n) (:116-119). */
+BCMC_API int bcmc_scan_exclusive_u32(bcmc_ctx *ctx, bcmc_buf *data, uint32_t aligned_n, uint32_t n,
+                                     uint32_t *total);
+
+/* ---- StreamCompact / StreamCompactor (js/stream_compact.js:56-165) -------- */
+/* out[offsets[i]] = i for every i < n with flags[i] != 0 (stream_compact.comp). */
+BCMC_API int bcmc_compact_ids(bcmc_ctx *ctx, uint32_t n, bcmc_buf *flags, bcmc_buf *offsets,
+                              bcmc_buf *out);
+/* Fused single-pass scan + compact of 0/1 flags: out = ascending ids of the
+ * non-zero flags; *count = their number.  (No reference counterpart: it is the
+ * scan + compactActiveIDs pair of compressed_marching_cubes.js:815-825,704-707
+ * as one pass.) */
+BCMC_API int bcmc_compact_flags(bcmc_ctx *ctx, uint32_t n, bcmc_buf *flags, bcmc_buf *out,
+                                uint32_t *count);
+
+/* ---- RadixSorter (js/radix_sort_by_key.js) --------------------------------- */
+/* getAlignedSize (radix_sort_by_key.js:155-158): nextPow2(ceil(n/64))*64. */
+BCMC_API uint32_t bcmc_sort_aligned_size(uint32_t n);
+/* RadixSorter.sort (radix_sort_by_key.js:161-400): sorts the first n (key,
+ * value) u32 pairs; keys/vals hold bcmc_sort_aligned_size(n) entries.  On
+ * return vals[0..n) holds exactly what the reference leaves there (ascending
+ * keys, or descending with reverse != 0, with the reference's order among equal
+ * keys); the contents of keys are unspecified, as in the reference. */
+BCMC_API int bcmc_sort_u32_by_key(bcmc_ctx *ctx, bcmc_buf *keys, bcmc_buf *vals, uint32_t n,
+                                  int reverse);
+
+/* ---- LRUCache (js/lru_cache.js) -------------------------------------------- */
+/* new LRUCache(device, scan, compact, initialSize, elementSize, totalElements)
+ * (lru_cache.js:4-305).  Slot storage for alignTo(total, 32) elements is
+ * reserved up front so the cache grows without reallocation; the logical size
+ * follows the reference exactly. */
+BCMC_API int bcmc_lru_create(bcmc_ctx *ctx, uint32_t initial_size, uint32_t elem_bytes,
+                             uint32_t total, bcmc_lru **out);
+BCMC_API int bcmc_lru_destroy(bcmc_lru *lru);
+/* LRUCache.update (lru_cache.js:312-843).  needed: `total` u32 flags (0/1).
+ * Returns the number of new items and, when non-zero, a new buffer with their
+ * ids in ascending order (caller frees); *new_items = NULL when zero. */
+BCMC_API int bcmc_lru_update(bcmc_lru *lru, bcmc_buf *needed, uint32_t *num_new,
+                             bcmc_buf **new_items);
+/* LRUCache.reset (lru_cache.js:846-903). */
+BCMC_API int bcmc_lru_reset(bcmc_lru *lru);
+typedef struct {
+    uint32_t cache_size;      /* this.cacheSize (slots) */
+    uint32_t total_elements;  /* this.totalElements */
+    uint32_t element_size;    /* this.elementSize (bytes) */
+    uint32_t slots_available; /* this.displayNumSlotsAvailable after the last update */
+    uint64_t reserved_bytes;  /* physical slot storage */
+} bcmc_lru_info;
+BCMC_API int bcmc_lru_get_info(bcmc_lru *lru, bcmc_lru_info *info);
+/* Slot state for inspection: age[C], available[C], item[C] (-1 = empty) and
+ * item_slot[total] (-1 = not cached); any pointer may be NULL. */
+BCMC_API int bcmc_lru_read_state(bcmc_lru *lru, uint32_t *age, uint32_t *available, int32_t *item,
+                                 int32_t *item_slot);
+/* The cache contents (this.cache): cache_size * element_size bytes. */
+BCMC_API bcmc_buf *bcmc_lru_cache_buffer(bcmc_lru *lru);
+
+/* ---- CompressedMarchingCubes (js/compressed_marching_cubes.js) ------------- */
+typedef enum {
+    /* 16 B per vertex: float x, y, z in block-local voxel units (the
+     * reference's lerp + voxel + 0.5, compute_block_vertices.comp:141-164)
+     * and the u32 block id; global position = 4*block_pos + local. */
+    BCMC_VERTS_LOCAL_F32 = 0,
+    /* 12 B per vertex: global float xyz = 4*block_pos + local. */
+    BCMC_VERTS_GLOBAL_F32 = 1,
+    /* 8 B per vertex: the reference's uvec2(compress_position(p), block_id)
+     * (block_vertex_position.comp:9-17), for its renderer. */
+    BCMC_VERTS_PACKED_U32X2 = 2
+} bcmc_vertex_format;
+
+typedef struct {
+    /* perfTracker keys of compressed_marching_cubes.js:593-610, milliseconds of
+     * device time per stage (stages the reference runs separately but this
+     * library fuses report 0 in the fused-away key). */
+    double computeActiveBlocks, cacheUpdate, decompression, compactActiveIDs,
+        computeBlockHasVertices, compactBlocksWithVertices, computeVertices;
+    double total_ms; /* wall time of the call */
+    uint32_t numActiveBlocks, numBlocksDecompressed, numBlocksWithVertices, cacheSize;
+    uint64_t numVertices;
+} bcmc_perf;
+
+/* new CompressedMarchingCubes(device) (compressed_marching_cubes.js:1-364). */
+BCMC_API int bcmc_mc_create(bcmc_ctx *ctx, bcmc_mc **out);
+BCMC_API int bcmc_mc_destroy(bcmc_mc *mc);
+/* setCompressedVolume(volume, compressionRate, volumeDims, volumeScale)
+ * (compressed_marching_cubes.js:432-520, ranges :522-578).  `stream` is a
+ * headerless zfp fixed-rate float32 stream; rate in bits per value (64*rate
+ * must be an integer >= 9).  initial_cache_slots = 0 selects the reference's
+ * ceil(0.05 * totalBlocks) (:443-448). */
+BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *mc, const void *stream, uint64_t bytes,
+                                           double rate, const uint32_t dims[3],
+                                           const float scale[3], uint32_t initial_cache_slots);
+BCMC_API int bcmc_mc_set_vertex_format(bcmc_mc *mc, bcmc_vertex_format fmt);
+/* computeSurface(isovalue, perfTracker) (compressed_marching_cubes.js:580-809).
+ * The isovalue is used as FP32 (:613-619).  perf may be NULL. */
+BCMC_API int bcmc_mc_compute_surface(bcmc_mc *mc, float isovalue, bcmc_perf *perf,
+                                     uint64_t *num_vertices);
+/* Public fields of the reference object. */
+typedef struct {
+    uint32_t total_blocks, num_active_blocks, num_blocks_with_vertices, new_decompressed;
+    uint64_t num_vertices, vertex_capacity;
+    uint32_t dims[3], padded_dims[3], max_bits;
+    float scale[3];
+    bcmc_vertex_format format;
+} bcmc_mc_info;
+BCMC_API int bcmc_mc_get_info(bcmc_mc *mc, bcmc_mc_info *info);
+/* this.vertexBuffer: num_vertices entries in the selected format. */
+BCMC_API bcmc_buf *bcmc_mc_vertex_buffer(bcmc_mc *mc);
+BCMC_API bcmc_lru *bcmc_mc_lru(bcmc_mc *mc);
+/* Device arrays of the last query, for inspection: block ranges (float2[N]),
+ * active flags (u32[N]), active ids (u32[A]), decoded ids (u32[D]), indices
+ * into the active list of blocks with vertices (u32[H]) and their vertex
+ * offsets (u64[H]). */
+typedef enum {
+    BCMC_ARRAY_RANGES = 0,
+    BCMC_ARRAY_ACTIVE_FLAGS = 1,
+    BCMC_ARRAY_ACTIVE_IDS = 2,
+    BCMC_ARRAY_NEW_ITEMS = 3,
+    BCMC_ARRAY_WITH_VERTS = 4,
+    BCMC_ARRAY_VERT_OFFSETS = 5
+} bcmc_mc_array;
+BCMC_API bcmc_buf *bcmc_mc_array_buffer(bcmc_mc *mc, bcmc_mc_array which);
+/* reportMemoryUse() (compressed_marching_cubes.js:366-430): total bytes of
+ * device memory held by the object (out), with the cache share (cache_out). */
+BCMC_API int bcmc_mc_memory_use(bcmc_mc *mc, uint64_t *total_out, uint64_t *cache_out);
+
+/* ---- ZFP decode (shaders/zfp_decompress.comp, zfp_decompress_block.comp) -- */
+/* Decode blocks ids[0..n) of a fixed-rate stream into out[i*64 .. i*64+64)
+ * (float32, x fastest).  ids may be NULL for blocks 0..n-1. */
+BCMC_API int bcmc_zfp_decode_blocks(bcmc_ctx *ctx, bcmc_buf *stream, uint32_t max_bits,
+                                    bcmc_buf *ids, uint32_t n, bcmc_buf *out);
+/* Block value ranges (zfp_compute_block_range.comp): float2 per block. */
+BCMC_API int bcmc_zfp_block_ranges(bcmc_ctx *ctx, bcmc_buf *stream, uint32_t max_bits,
+                                   const uint32_t dims[3], bcmc_buf *ranges);
+
+/* With timing enabled, bcmc_zfp_decode_blocks and bcmc_zfp_block_ranges wait
+ * for their kernel and record its device time (HIP events on the context
+ * stream); bcmc_ctx_last_kernel_ms returns it in ms (0 if none recorded). */
+BCMC_API int bcmc_ctx_enable_timing(bcmc_ctx *ctx, int enable);
+BCMC_API double bcmc_ctx_last_kernel_ms(bcmc_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,50 @@
+"""LRUCache on the GPU vs the oracle: every slot's age, availability and item,
+and every item's slot, bit-exact after each update (js/lru_cache.js)."""
+import numpy as np
+import pytest
+
+import bcmc_amd
+import oracle
+from gpu_common import dev, say  # noqa: F401
+
+pytestmark = pytest.mark.gpu
+
+
+def _needed(total, rng, frac):
+    f = np.zeros(total, np.uint32)
+    lo = int(rng.integers(0, total))
+    f[lo:lo + int(rng.integers(1, max(2, int(total * frac))))] = 1
+    f[rng.uniform(0, 1, total) < frac / 4] = 1
+    return f
+
+
+@pytest.mark.parametrize("total,initial,steps", [(100, 32, 10), (1000, 64, 15), (5000, 200, 12),
+                                                 (20000, 20000, 6), (70000, 3500, 8)])
+def test_lru_sequence_matches_oracle(dev, total, initial, steps):
+    say("lru", total, initial, steps)
+    rng = np.random.default_rng(total + initial)
+    gpu = bcmc_amd.LRUCache(dev, None, None, initial, 256, total)
+    cpu = oracle.LRU(initial, 256, total)
+    for step in range(steps):
+        needed = _needed(total, rng, 0.3)
+        n_g, ids_g = gpu.update(dev.buffer_from(needed))
+        n_c, ids_c = cpu.update(needed)
+        assert n_g == n_c, step
+        if n_c:
+            assert np.array_equal(ids_g.download(np.uint32, n_g), ids_c)
+        for a, b, what in zip(gpu.state(), cpu.state(), ("age", "avail", "item", "item_slot")):
+            assert np.array_equal(a, b), (step, what)
+        if step == steps // 2:
+            gpu.reset()
+            cpu.reset()
+
+
+def test_lru_no_new_items_is_age_only(dev):
+    gpu = bcmc_amd.LRUCache(dev, None, None, 32, 256, 64)
+    cpu = oracle.LRU(32, 256, 64)
+    f = np.zeros(64, np.uint32)
+    f[:5] = 1
+    for _ in range(3):
+        assert gpu.update(dev.buffer_from(f))[0] == cpu.update(f)[0]
+    for a, b in zip(gpu.state(), cpu.state()):
+        assert np.array_equal(a, b)

@@ -1,0 +1,135 @@
+"""CompressedMarchingCubes on the GPU vs the oracle (js/compressed_marching_cubes.js).
+
+Per query: active flags and ids, decoded ids, cache state, blocks with
+vertices, vertex offsets and counts are bit-exact; vertex positions are
+bit-exact with the oracle's FP32 evaluation and within 1e-5 voxel units of
+its FP64 interpolation.
+"""
+import numpy as np
+import pytest
+
+import bcmc_amd
+import oracle
+from bcmc_amd import synth
+from gpu_common import dev, say  # noqa: F401
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5  # voxel units, block-local FP32 positions vs FP64
+
+
+def _volume(dims, kind, seed, rate, count, params):
+    f = synth.field(kind, seed, dims, count, params)
+    return synth.encode(f, 64 * rate)
+
+
+def _check_query(mc, ref, iso):
+    nv = mc.computeSurface(iso)
+    nv_ref = ref.compute_surface(iso)
+    r = ref.results()
+    st = ref.stats
+    assert nv == nv_ref
+    info = mc.info()
+    assert info["num_active_blocks"] == st["num_active"]
+    assert info["new_decompressed"] == st["num_decoded"]
+    assert info["num_blocks_with_vertices"] == st["num_with_verts"]
+    assert np.array_equal(mc.array(bcmc_amd.ARRAY_ACTIVE_FLAGS), r["flags"])
+    assert np.array_equal(mc.array(bcmc_amd.ARRAY_ACTIVE_IDS), r["active_ids"])
+    assert np.array_equal(mc.array(bcmc_amd.ARRAY_NEW_ITEMS), r["new_items"])
+    assert np.array_equal(mc.array(bcmc_amd.ARRAY_WITH_VERTS), r["with_verts"])
+    offs = mc.array(bcmc_amd.ARRAY_VERT_OFFSETS, np.uint64)
+    assert np.array_equal(offs.astype(np.uint32), r["vert_offsets"])
+    for a, b, what in zip(mc.lruCache.state(), ref.lru_state(), ("age", "avail", "item", "slot")):
+        assert np.array_equal(a, b), what
+    return nv, r
+
+
+@pytest.mark.parametrize("dims,rate,cache,isos", [
+    ((32, 32, 32), 4, 64, [0.3, 0.5, 0.45]),
+    ((40, 24, 52), 8, 32, [0.2, 0.6]),
+    ((13, 10, 7), 2, 0, [0.4]),
+    ((64, 64, 64), 2, 256, [0.25, 0.35, 0.5, 0.65, 0.35]),
+])
+def test_pipeline_matches_oracle(dev, dims, rate, cache, isos):
+    say("mc", dims, rate, cache, isos)
+    s = _volume(dims, synth.GAUSS, 31, rate, 10, (3, 12, 0.5, 1))
+    mc = bcmc_amd.CompressedMarchingCubes(dev)
+    mc.setCompressedVolume(s, rate, dims, (1, 1, 1), cache)
+    ref = oracle.MC(s, 64 * rate, dims, initial_cache=cache if cache else None)
+    r_g = mc.array(bcmc_amd.ARRAY_RANGES, np.float32).reshape(-1, 2)
+    assert np.array_equal(r_g.view(np.uint32), ref.ranges().view(np.uint32))
+    for iso in isos:
+        nv, r = _check_query(mc, ref, iso)
+        v = mc.vertices()
+        assert v.shape == (nv, 4)
+        assert np.array_equal(v.view(np.uint32), r["verts_f32"].view(np.uint32))
+        if nv:
+            assert np.max(np.abs(v[:, :3].astype(np.float64) - r["verts_f64"])) <= TOL
+
+
+def test_turbulent_sweep_with_eviction(dev):
+    dims, rate = (96, 80, 64), 4
+    say("mc fourier sweep", dims)
+    f = synth.field(synth.FOURIER, 1, dims, 128, (1, 24, 0, 0))
+    s = synth.encode(f, 64 * rate)
+    n = oracle.nblocks(dims)
+    mc = bcmc_amd.CompressedMarchingCubes(dev)
+    mc.setCompressedVolume(s, rate, dims, (1, 1, 1), n // 4)
+    ref = oracle.MC(s, 64 * rate, dims, initial_cache=n // 4)
+    for iso in np.linspace(0.2, 0.8, 8).astype(np.float32):
+        nv, r = _check_query(mc, ref, float(iso))
+        v = mc.vertices()
+        assert np.array_equal(v.view(np.uint32), r["verts_f32"].view(np.uint32))
+    mc.lruCache.reset()
+    ref_l = oracle.MC(s, 64 * rate, dims, initial_cache=n // 4)
+    # after reset the oracle state differs in history; compare counts only
+    assert mc.computeSurface(0.5) == ref_l.compute_surface(0.5)
+
+
+def test_vertex_formats(dev):
+    dims, rate, iso = (48, 40, 36), 4, 0.45
+    s = _volume(dims, synth.BLOBS, 5, rate, 6, (4, 12, 0.5, 1))
+    ref = oracle.MC(s, 64 * rate, dims)
+    nv = ref.compute_surface(iso)
+    loc = ref.results()["verts_f32"]
+    bid = loc[:, 3].view(np.uint32)
+    nb = [(d + 3) // 4 for d in dims]
+    bpos = np.stack([bid % nb[0], (bid // nb[0]) % nb[1], bid // (nb[0] * nb[1])], 1)
+    for fmt in (bcmc_amd.VERTS_GLOBAL_F32, bcmc_amd.VERTS_PACKED_U32X2):
+        mc = bcmc_amd.CompressedMarchingCubes(dev)
+        mc.setCompressedVolume(s, rate, dims)
+        mc.setVertexFormat(fmt)
+        assert mc.computeSurface(iso) == nv
+        v = mc.vertices()
+        if fmt == bcmc_amd.VERTS_GLOBAL_F32:
+            want = (4 * bpos).astype(np.float32) + loc[:, :3]
+            assert np.array_equal(v.view(np.uint32), want.view(np.uint32))
+        else:
+            q = (((loc[:, :3] - np.float32(0.5)) * np.float32(0.25)) * np.float32(1023)).astype(np.uint32)
+            packed = ((q[:, 0] & 0x3FF) << 20) | ((q[:, 1] & 0x3FF) << 10) | (q[:, 2] & 0x3FF)
+            assert np.array_equal(v[:, 0], packed) and np.array_equal(v[:, 1], bid)
+
+
+def test_empty_and_full_isovalues(dev):
+    dims = (24, 24, 24)
+    s = _volume(dims, synth.GAUSS, 8, 4, 4, (3, 8, 0.5, 1))
+    mc = bcmc_amd.CompressedMarchingCubes(dev)
+    mc.setCompressedVolume(s, 4, dims)
+    ref = oracle.MC(s, 256, dims)
+    for iso in (2.0, -1.0, 0.5, 2.0):
+        _check_query(mc, ref, iso)
+
+
+def test_perf_tracker_keys(dev):
+    dims = (32, 32, 32)
+    s = _volume(dims, synth.GAUSS, 9, 4, 6, (3, 8, 0.5, 1))
+    mc = bcmc_amd.CompressedMarchingCubes(dev)
+    mc.setCompressedVolume(s, 4, dims)
+    perf = {}
+    mc.computeSurface(0.4, perf)
+    mc.computeSurface(0.5, perf)
+    for k in bcmc_amd.CompressedMarchingCubes.PERF_KEYS:
+        assert len(perf[k]) == 2
+    assert perf["totalBlocks"] == 512
+    mem = mc.reportMemoryUse()
+    assert mem["total"] > mem["cache"] > 0

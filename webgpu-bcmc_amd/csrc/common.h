@@ -1,0 +1,250 @@
+// Shared host/device helpers for the gfx950 kernels of the BCMC library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "bcmc.h"
+
+// ---------------------------------------------------------------------------
+// Host error handling
+// ---------------------------------------------------------------------------
+namespace bcmc {
+
+void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+const char *last_error();
+
+}  // namespace bcmc
+
+#define BCMC_HIP_CHECK(expr)                                                               \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            bcmc::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),        \
+                            __FILE__, __LINE__);                                           \
+            return BCMC_ERR_HIP;                                                           \
+        }                                                                                  \
+    } while (0)
+
+#define BCMC_CHECK_ARG(cond, msg)                                                          \
+    do {                                                                                   \
+        if (!(cond)) {                                                                     \
+            bcmc::set_error("invalid argument: %s", msg);                                  \
+            return BCMC_ERR_INVALID;                                                       \
+        }                                                                                  \
+    } while (0)
+
+#define BCMC_TRY(expr)                                                                     \
+    do {                                                                                   \
+        int rc_ = (expr);                                                                  \
+        if (rc_ != BCMC_OK)                                                                \
+            return rc_;                                                                    \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Host objects
+// ---------------------------------------------------------------------------
+struct bcmc_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    char name[128] = {0};
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_kernel_ms = 0.0;
+    // small pinned staging area for counters read back after a query
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
+};
+
+struct bcmc_buf {
+    bcmc_ctx *ctx = nullptr;
+    void *ptr = nullptr;
+    uint64_t bytes = 0;
+    bool owned = true;
+};
+
+namespace bcmc {
+
+// Device allocation helpers (hipMalloc, with an injected-failure hook for the
+// failure-handling tests: BCMC_FAIL_ALLOC_AFTER=k fails the k-th allocation).
+int dev_alloc(void **p, size_t bytes);
+void dev_free(void *p);
+int make_buf(bcmc_ctx *ctx, uint64_t bytes, bcmc_buf **out);
+
+// ---------------------------------------------------------------------------
+// Single-pass scan state (decoupled look-back).  One per call site; the
+// kernels reset it themselves, so launches need no memset (graph friendly).
+//   ctrl[0] tile counter, ctrl[1] finished-workgroup counter, ctrl[2] epoch
+//   status[t]: bits 63..42 epoch, 41..40 flag (1 aggregate, 2 inclusive),
+//              39..0 value
+// ---------------------------------------------------------------------------
+struct ScanSite {
+    uint32_t *ctrl = nullptr;
+    unsigned long long *status = nullptr;
+    uint32_t max_tiles = 0;
+};
+
+int scan_site_alloc(ScanSite *s, uint64_t max_elems, uint32_t tile);
+void scan_site_free(ScanSite *s);
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr uint32_t kScanTile = kScanThreads * kScanItems;  // 4096 elements
+
+__host__ __device__ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Persistent grid for device-sized loops: enough workgroups to fill the chip.
+inline uint32_t persistent_grid(const bcmc_ctx *ctx, uint32_t per_cu) {
+    return (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * per_cu;
+}
+
+}  // namespace bcmc
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+#define BCMC_DEV __device__ __forceinline__
+
+namespace bcmc {
+
+BCMC_DEV uint32_t lane_id() { return __lane_id(); }
+
+// Inclusive wave64 prefix sum using DPP row shifts and the GFX9 row
+// broadcasts (lane 15 -> row 1, lane 31 -> rows 2-3).
+BCMC_DEV uint32_t wave_incl_scan(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return v;
+}
+
+BCMC_DEV uint32_t wave_read_lane(uint32_t v, int lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+
+BCMC_DEV uint32_t wave_sum(uint32_t v) { return wave_read_lane(wave_incl_scan(v), 63); }
+
+// Agent-scope atomics (coherent across the XCDs' L2 caches).
+BCMC_DEV unsigned long long ld_acquire_u64(const unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+BCMC_DEV void st_release_u64(unsigned long long *p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+BCMC_DEV uint32_t ld_acquire_u32(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+BCMC_DEV void st_release_u32(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+BCMC_DEV uint32_t atomic_add_u32(uint32_t *p, uint32_t v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+constexpr unsigned long long kValueMask = (1ull << 40) - 1;
+constexpr uint32_t kFlagAggregate = 1, kFlagInclusive = 2;
+
+BCMC_DEV unsigned long long pack_status(uint32_t epoch, uint32_t flag, unsigned long long value) {
+    return ((unsigned long long)(epoch & 0x3FFFFF) << 42) | ((unsigned long long)flag << 40) |
+           (value & kValueMask);
+}
+
+// Workgroup-wide exclusive scan of one u32 per thread.  Returns the thread's
+// exclusive prefix; *total receives the workgroup sum (all threads).
+template <int NT>
+BCMC_DEV uint32_t block_excl_scan(uint32_t v, uint32_t *lds_waves /* NT/64 + 1 */, uint32_t *total) {
+    constexpr int NW = NT / 64;
+    const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
+    const uint32_t incl = wave_incl_scan(v);
+    if (lane == 63)
+        lds_waves[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        const uint32_t t = lds_waves[w];
+        before += (w < (int)wave) ? t : 0u;
+        all += t;
+    }
+    __syncthreads();
+    *total = all;
+    return before + incl - v;
+}
+
+// Decoupled look-back across tiles.  Called by all threads of the workgroup
+// after the tile aggregate is known; returns the exclusive prefix of the tile
+// (64-bit; status values are exact below 2^40).  The last tile to finish its
+// lookback publishes an inclusive status that later tiles consume.
+template <int NT>
+BCMC_DEV unsigned long long tile_lookback(const ScanSite &site, uint32_t tile, uint32_t epoch,
+                                          uint32_t aggregate, unsigned long long *lds_prefix) {
+    if (threadIdx.x == 0) {
+        unsigned long long prefix = 0;
+        if (tile == 0) {
+            st_release_u64(&site.status[0], pack_status(epoch, kFlagInclusive, aggregate));
+        } else {
+            st_release_u64(&site.status[tile], pack_status(epoch, kFlagAggregate, aggregate));
+            int32_t t = (int32_t)tile - 1;
+            while (t >= 0) {
+                unsigned long long s = ld_acquire_u64(&site.status[t]);
+                const uint32_t ep = (uint32_t)(s >> 42), flag = (uint32_t)(s >> 40) & 3u;
+                if (ep != (epoch & 0x3FFFFF) || flag == 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                prefix += s & kValueMask;
+                if (flag == kFlagInclusive)
+                    break;
+                --t;
+            }
+            st_release_u64(&site.status[tile], pack_status(epoch, kFlagInclusive, prefix + aggregate));
+        }
+        *lds_prefix = prefix;
+    }
+    __syncthreads();
+    const unsigned long long p = *lds_prefix;
+    __syncthreads();
+    return p;
+}
+
+// Dynamic tile index for persistent workgroups (broadcast through LDS).
+BCMC_DEV uint32_t next_tile(const ScanSite &site, uint32_t *lds_tile) {
+    if (threadIdx.x == 0)
+        *lds_tile = atomic_add_u32(&site.ctrl[0], 1u);
+    __syncthreads();
+    const uint32_t t = *lds_tile;
+    __syncthreads();
+    return t;
+}
+
+// Called once per workgroup when it has no more tiles: the last workgroup of
+// the grid resets the tile counter and advances the epoch (and clears the
+// status words when the 22-bit epoch wraps), so the next launch starts clean.
+BCMC_DEV void scan_site_finish(const ScanSite &site, uint32_t epoch) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t done = atomic_add_u32(&site.ctrl[1], 1u);
+        if (done == gridDim.x - 1) {
+            st_release_u32(&site.ctrl[0], 0u);
+            st_release_u32(&site.ctrl[1], 0u);
+            uint32_t next = (epoch + 1) & 0x3FFFFF;
+            if (next == 0) {
+                for (uint32_t t = 0; t < site.max_tiles; ++t)
+                    st_release_u64(&site.status[t], 0ull);
+                next = 1;
+            }
+            st_release_u32(&site.ctrl[2], next);
+        }
+    }
+}
+
+BCMC_DEV uint32_t scan_site_epoch(const ScanSite &site) { return ld_acquire_u32(&site.ctrl[2]); }
+
+}  // namespace bcmc

@@ -1,0 +1,180 @@
+// Internal launchers and the single-pass scan kernel template shared by the
+// library's translation units.
+#pragma once
+
+#include "common.h"
+
+namespace bcmc {
+
+// ---------------------------------------------------------------------------
+// Device-resident per-query counters (one struct per pipeline object).  All
+// sizes that the reference reads back to the host between stages
+// (compressed_marching_cubes.js:580-809, lru_cache.js:312-843) live here and
+// are consumed by later kernels directly; the host reads the struct once at
+// the end of a query.
+// ---------------------------------------------------------------------------
+struct QState {
+    float iso;
+    uint32_t num_active;       // A
+    uint32_t num_new;          // D: LRU misses
+    uint32_t num_avail;        // S: available slots (after growth)
+    uint32_t cache_size;       // C: logical LRU size before this update
+    uint32_t cache_size_next;  // C after this update (growth)
+    uint32_t num_with;         // H: active blocks with vertices
+    uint32_t overflow;         // emit skipped: V > vertex capacity
+    unsigned long long num_verts;     // V
+    unsigned long long vert_capacity; // vertices the output buffer holds
+    uint32_t avail_display;    // S - D (displayNumSlotsAvailable)
+    uint32_t pad[5];
+};
+
+// ---------------------------------------------------------------------------
+// Single-pass (decoupled look-back) scan over a device-sized range with one or
+// two u32 channels.  Op provides:
+//   uint32_t size() const                 number of elements (may read memory)
+//   void begin() const                    once, by thread 0 of workgroup 0
+//   void value(uint32_t i, uint32_t v[NCH]) const
+//   void emit(uint32_t i, const uint32_t v[NCH], const unsigned long long excl[NCH]) const
+//   void total(const unsigned long long t[NCH]) const  (one thread, once)
+// Each wave scans a contiguous 1024-element run of its tile in 16 coalesced
+// rounds with DPP; wave totals combine through LDS; tiles chain through the
+// look-back status words of site[c].
+// ---------------------------------------------------------------------------
+template <int NCH, class Op>
+__global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanSite s0, ScanSite s1, Op op) {
+    constexpr int NW = kScanThreads / 64;
+    __shared__ uint32_t lds_w[NCH][NW + 1];
+    __shared__ unsigned long long lds_prefix[NCH];
+    __shared__ uint32_t lds_tile;
+    const ScanSite *sites[2] = {&s0, &s1};
+    const uint32_t n = op.size();
+    const uint32_t ntiles = div_up(n, kScanTile);
+    uint32_t epoch[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+        epoch[c] = scan_site_epoch(*sites[c]);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        op.begin();
+    if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+        unsigned long long z[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            z[c] = 0;
+        op.total(z);
+    }
+    const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
+    for (;;) {
+        const uint32_t tile = next_tile(s0, &lds_tile);
+        if (tile >= ntiles)
+            break;
+        const uint32_t wbase = tile * kScanTile + wave * (64 * kScanItems);
+        uint32_t vals[kScanItems][NCH], excl[kScanItems][NCH], run[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            run[c] = 0;
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            const uint32_t i = wbase + j * 64 + lane;
+            uint32_t v[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; ++c)
+                v[c] = 0;
+            if (i < n)
+                op.value(i, v);
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                const uint32_t inc = wave_incl_scan(v[c]);
+                excl[j][c] = run[c] + inc - v[c];
+                run[c] += wave_read_lane(inc, 63);
+                vals[j][c] = v[c];
+            }
+        }
+        unsigned long long prefix[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            if (lane == 0)
+                lds_w[c][wave] = run[c];
+        }
+        __syncthreads();
+        uint32_t before[NCH], agg[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            before[c] = 0;
+            agg[c] = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                const uint32_t t = lds_w[c][w];
+                before[c] += w < (int)wave ? t : 0u;
+                agg[c] += t;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            prefix[c] = tile_lookback<kScanThreads>(*sites[c], tile, epoch[c], agg[c], &lds_prefix[c]);
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            const uint32_t i = wbase + j * 64 + lane;
+            if (i < n) {
+                unsigned long long e[NCH];
+#pragma unroll
+                for (int c = 0; c < NCH; ++c)
+                    e[c] = prefix[c] + before[c] + excl[j][c];
+                op.emit(i, vals[j], e);
+            }
+        }
+        if (tile == ntiles - 1 && threadIdx.x == 0) {
+            unsigned long long t[NCH];
+#pragma unroll
+            for (int c = 0; c < NCH; ++c)
+                t[c] = prefix[c] + agg[c];
+            op.total(t);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+        scan_site_finish(*sites[c], epoch[c]);
+}
+
+template <int NCH, class Op>
+inline void launch_scan(bcmc_ctx *ctx, const ScanSite &s0, const ScanSite &s1, const Op &op,
+                        uint64_t n_max) {
+    uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(n_max, kScanTile),
+                                                             persistent_grid(ctx, 4)));
+    scan_kernel<NCH, Op><<<grid, kScanThreads, 0, ctx->stream>>>(s0, s1, op);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers implemented in the .hip files
+// ---------------------------------------------------------------------------
+void launch_block_ranges(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords, uint32_t maxbits,
+                         const uint32_t dims[3], float2 *ranges);
+void launch_decode_to_cache(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords,
+                            uint32_t maxbits, const uint32_t *ids, const uint32_t *n_dev,
+                            uint32_t n_max, const int32_t *item_slot, float *cache,
+                            uint32_t capacity);
+void launch_decode_dense(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords, uint32_t maxbits,
+                         const uint32_t *ids, uint32_t n, float *out);
+
+// Radix sort of (key, value) pairs, see sort.hip.
+struct SortScratch {
+    uint32_t *keys[2] = {nullptr, nullptr};
+    uint32_t *vals[2] = {nullptr, nullptr};
+    uint32_t *hist = nullptr;   // 256 * max_tiles
+    uint32_t max_elems = 0, max_tiles = 0;
+    ScanSite site;
+};
+int sort_scratch_alloc(SortScratch *s, uint32_t max_elems);
+void sort_scratch_free(SortScratch *s);
+
+// Orders the `count` available cache slots for LRU assignment exactly as the
+// reference's RadixSorter.sort(ages, slot ids, S, reverse=true) does, writing
+// slot ids to out[0..count).  count is read from device memory.
+void launch_lru_order(bcmc_ctx *ctx, SortScratch *s, const uint32_t *avail_ids,
+                      const uint32_t *age, const uint32_t *count_dev, uint32_t count_max,
+                      uint32_t *out);
+// RadixSorter.sort semantics for host-known n over an aligned buffer.
+void launch_sort_by_key(bcmc_ctx *ctx, SortScratch *s, uint32_t *keys, uint32_t *vals, uint32_t n,
+                        int reverse);
+
+}  // namespace bcmc

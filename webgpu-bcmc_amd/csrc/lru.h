@@ -1,0 +1,55 @@
+// Device state of the decoded-block LRU cache (see lru.hip).
+#pragma once
+
+#include "kernels.h"
+
+namespace bcmc {
+
+// Kernel-side view (passed by value).
+struct LruView {
+    uint32_t *age = nullptr;      // [capacity]
+    uint32_t *avail = nullptr;    // [capacity]
+    int32_t *item = nullptr;      // [capacity], -1 = empty
+    int32_t *item_slot = nullptr; // [aligned_total], -1 = not cached (cachedItemSlots)
+    uint32_t *avail_ids = nullptr;// [capacity] compacted available slots
+    uint32_t *order = nullptr;    // [capacity] available slots in eviction order
+    uint32_t total = 0;           // totalElements
+    uint32_t capacity = 0;        // reserved slots
+};
+
+struct LruState {
+    LruView v;
+    uint8_t *data = nullptr;      // cache contents, capacity * elem_bytes
+    uint32_t *new_items = nullptr;// [aligned_total] ids to decode, ascending
+    uint32_t total = 0, aligned_total = 0, elem_bytes = 0, capacity = 0;
+    uint32_t cache_size = 0;      // host mirror of the logical size
+    ScanSite site_miss, site_avail;
+    SortScratch sort;
+};
+
+int lru_state_alloc(bcmc_ctx *ctx, LruState *L, uint32_t initial, uint32_t elem_bytes,
+                    uint32_t total);
+void lru_state_free(LruState *L);
+// LRUCache.update for a marching-cubes query: `flags` are the N active flags,
+// `active_ids` their ascending compaction with q->num_active entries; does
+// nothing when q->num_active == 0 (compressed_marching_cubes.js:630-641).
+void lru_enqueue_update_active(bcmc_ctx *ctx, LruState *L, const uint32_t *flags,
+                               const uint32_t *active_ids, QState *q);
+// LRUCache.update for arbitrary 0/1 `needed` flags over all items.
+void lru_enqueue_update_all(bcmc_ctx *ctx, LruState *L, const uint32_t *needed, QState *q);
+void lru_enqueue_reset(bcmc_ctx *ctx, LruState *L);
+int lru_copy_state(bcmc_ctx *ctx, LruState *L, uint32_t *age, uint32_t *available, int32_t *item,
+                   int32_t *item_slot);
+
+}  // namespace bcmc
+
+// C-ABI handle (include/bcmc.h).  A marching-cubes object hands out a
+// non-owning handle over its own cache state (owns = false).
+struct bcmc_lru {
+    bcmc_ctx *ctx = nullptr;
+    bcmc::LruState st;
+    bcmc::QState *q = nullptr;   // device counters
+    bcmc::QState *qh = nullptr;  // pinned host mirror
+    bcmc_buf cache_view;         // non-owning view of the cache contents
+    bool owns = true;
+};

@@ -1,0 +1,386 @@
+// Decoded-block LRU cache on gfx950 (js/lru_cache.js + shaders/lru_cache_*.comp,
+// lru_copy_available_slot_age.comp).
+//
+// One update runs as a chain of device-sized kernels with no host round trip:
+//   age+mark   one pass over the C slots; a slot ages by one, and a slot whose
+//              item is needed becomes {age 0, unavailable}, a slot whose item
+//              is cached but not needed becomes available
+//              (lru_cache_age_slots.comp + lru_cache_mark_new_items.comp; the
+//              item->slot map is a bijection on cached items, so visiting
+//              slots is the same as visiting cached items);
+//   misses     single-pass scan+compact of {needed and not cached} -> new item
+//              ids in ascending order (lru_cache.js:463-492);
+//   available  single-pass scan+compact of the availability flags (:501,
+//              :684-710);
+//   plan/grow  the reference's growth rule (:510-672) on the device: slots
+//              [C, C') get {100000, 1, -1} and are appended to the available
+//              list; storage for alignTo(total, 32) slots is reserved up front;
+//   order      the radix sort reproducing RadixSorter.sort(ages, ids, S,
+//              reverse) order (:717-769, sort.hip);
+//   assign     lru_cache_update.comp: evict the previous item of each chosen
+//              slot and install the new item.
+// Slot state is kept as three arrays (age, available, item) instead of the
+// reference's 12-byte struct.
+#include "common.h"
+#include "kernels.h"
+#include "lru.h"
+
+#include <algorithm>
+
+namespace bcmc {
+
+__global__ void lru_init_kernel(LruView v, uint32_t first, uint32_t end, uint32_t age0,
+                                int init_items) {
+    for (uint32_t s = first + blockIdx.x * blockDim.x + threadIdx.x; s < end;
+         s += gridDim.x * blockDim.x) {
+        v.age[s] = age0 == 0 ? 2 + s : age0;  // constructor ages 2+i (lru_cache.js:55-59)
+        v.avail[s] = 1;
+        v.item[s] = -1;
+    }
+    if (init_items)
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < v.total;
+             i += gridDim.x * blockDim.x)
+            v.item_slot[i] = -1;
+}
+
+// age + mark (lru_cache_age_slots.comp:45-52, lru_cache_mark_new_items.comp:80-98)
+__global__ void lru_age_mark_kernel(LruView v, const uint32_t *needed, const QState *q,
+                                    const uint32_t *gate) {
+    if (gate && *gate == 0)
+        return;
+    const uint32_t c = q->cache_size;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < c; s += gridDim.x * blockDim.x) {
+        uint32_t a = v.age[s] + 1;
+        const int32_t p = v.item[s];
+        if (p >= 0 && (uint32_t)p < v.total) {
+            if (needed[p] == 1) {
+                a = 0;
+                v.avail[s] = 0;
+            } else {
+                v.avail[s] = 1;
+            }
+        }
+        v.age[s] = a;
+    }
+}
+
+// misses over all items: needed (0/1) and not cached
+struct MissAllOp {
+    LruView v;
+    const uint32_t *needed;
+    uint32_t *new_items;
+    QState *q;
+    BCMC_DEV uint32_t size() const { return v.total; }
+    BCMC_DEV void begin() const {}
+    BCMC_DEV void value(uint32_t i, uint32_t (&x)[1]) const {
+        x[0] = (v.item_slot[i] < 0 && needed[i] != 0) ? 1u : 0u;
+    }
+    BCMC_DEV void emit(uint32_t i, const uint32_t (&x)[1], const unsigned long long (&e)[1]) const {
+        if (x[0])
+            new_items[e[0]] = i;
+    }
+    BCMC_DEV void total(const unsigned long long (&t)[1]) const { q->num_new = (uint32_t)t[0]; }
+};
+
+// misses over the active list of a marching-cubes query (only active blocks
+// can be needed); the list is ascending, so the result is too
+struct MissActiveOp {
+    LruView v;
+    const uint32_t *active_ids;
+    uint32_t *new_items;
+    QState *q;
+    BCMC_DEV uint32_t size() const { return q->num_active; }
+    BCMC_DEV void begin() const {}
+    BCMC_DEV void value(uint32_t k, uint32_t (&x)[1]) const {
+        x[0] = v.item_slot[active_ids[k]] < 0 ? 1u : 0u;
+    }
+    BCMC_DEV void emit(uint32_t k, const uint32_t (&x)[1], const unsigned long long (&e)[1]) const {
+        if (x[0])
+            new_items[e[0]] = active_ids[k];
+    }
+    BCMC_DEV void total(const unsigned long long (&t)[1]) const { q->num_new = (uint32_t)t[0]; }
+};
+
+// available slots (skipped when there is nothing to insert)
+struct AvailOp {
+    LruView v;
+    QState *q;
+    BCMC_DEV uint32_t size() const { return q->num_new ? q->cache_size : 0u; }
+    BCMC_DEV void begin() const {}
+    BCMC_DEV void value(uint32_t s, uint32_t (&x)[1]) const { x[0] = v.avail[s] != 0 ? 1u : 0u; }
+    BCMC_DEV void emit(uint32_t s, const uint32_t (&x)[1], const unsigned long long (&e)[1]) const {
+        if (x[0])
+            v.avail_ids[e[0]] = s;
+    }
+    BCMC_DEV void total(const unsigned long long (&t)[1]) const { q->num_avail = (uint32_t)t[0]; }
+};
+
+// lru_cache.js:510-522 on one thread: next size and total available slots.
+__global__ void lru_plan_kernel(LruView v, QState *q, const uint32_t *gate) {
+    const bool run = !(gate && *gate == 0) && q->num_new != 0;
+    const uint32_t c = q->cache_size;
+    uint32_t next = c, avail = run ? q->num_avail : 0u;
+    if (run && avail < q->num_new) {
+        const uint64_t want = (uint64_t)c + ((uint64_t)(q->num_new - avail) * 3 + 1) / 2;
+        uint64_t sz = std::min<uint64_t>(want, v.total);
+        sz = (sz + 31) / 32 * 32;
+        next = (uint32_t)std::min<uint64_t>(sz, v.capacity);
+        avail += next - c;
+    }
+    q->cache_size_next = next;
+    q->num_avail = avail;
+}
+
+// new slots [C, C'): {100000, 1, -1}, appended to the available list
+__global__ void lru_grow_kernel(LruView v, const QState *q) {
+    const uint32_t c = q->cache_size, c2 = q->cache_size_next;
+    if (c2 <= c)
+        return;
+    const uint32_t base = q->num_avail - (c2 - c);
+    for (uint32_t s = c + blockIdx.x * blockDim.x + threadIdx.x; s < c2;
+         s += gridDim.x * blockDim.x) {
+        v.age[s] = 100000;
+        v.avail[s] = 1;
+        v.item[s] = -1;
+        v.avail_ids[base + (s - c)] = s;
+    }
+}
+
+// lru_cache_update.comp:173-191; thread 0 also commits the new size.
+__global__ void lru_assign_kernel(LruView v, const uint32_t *new_items, QState *q) {
+    const uint32_t n = q->num_avail ? q->num_new : 0u;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) {
+        const uint32_t it = new_items[w], s = v.order[w];
+        if (s >= v.capacity || it >= v.total)
+            continue;  // cannot happen for a consistent state; never write out of bounds
+        const int32_t prev = v.item[s];
+        if (prev >= 0 && (uint32_t)prev < v.total)
+            v.item_slot[prev] = -1;
+        v.age[s] = 0;
+        v.item[s] = (int32_t)it;
+        v.avail[s] = 0;
+        v.item_slot[it] = (int32_t)s;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        q->avail_display = q->num_avail - n;
+        q->cache_size = q->cache_size_next;
+    }
+}
+
+static uint32_t grid_for(const bcmc_ctx *ctx, uint64_t n, uint32_t per_cu = 8) {
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(div_up(n, 256),
+                                                              persistent_grid(ctx, per_cu)));
+}
+
+int lru_state_alloc(bcmc_ctx *ctx, LruState *L, uint32_t initial, uint32_t elem_bytes,
+                    uint32_t total) {
+    L->total = total;
+    L->aligned_total = (total + 31) / 32 * 32;
+    L->elem_bytes = elem_bytes;
+    L->capacity = std::max<uint32_t>(L->aligned_total, (initial + 31) / 32 * 32);
+    L->cache_size = (initial + 31) / 32 * 32;
+    const size_t cap = L->capacity;
+    BCMC_TRY(dev_alloc((void **)&L->v.age, cap * 4));
+    BCMC_TRY(dev_alloc((void **)&L->v.avail, cap * 4));
+    BCMC_TRY(dev_alloc((void **)&L->v.item, cap * 4));
+    BCMC_TRY(dev_alloc((void **)&L->v.avail_ids, cap * 4));
+    BCMC_TRY(dev_alloc((void **)&L->v.order, cap * 4));
+    BCMC_TRY(dev_alloc((void **)&L->v.item_slot, (size_t)L->aligned_total * 4 + 4));
+    BCMC_TRY(dev_alloc((void **)&L->data, cap * elem_bytes));
+    BCMC_TRY(dev_alloc((void **)&L->new_items, (size_t)L->aligned_total * 4 + 4));
+    BCMC_TRY(scan_site_alloc(&L->site_miss, L->aligned_total, kScanTile));
+    BCMC_TRY(scan_site_alloc(&L->site_avail, cap, kScanTile));
+    BCMC_TRY(sort_scratch_alloc(&L->sort, L->capacity));
+    L->v.total = total;
+    L->v.capacity = L->capacity;
+    lru_init_kernel<<<grid_for(ctx, cap), 256, 0, ctx->stream>>>(L->v, 0, L->cache_size, 0, 1);
+    BCMC_HIP_CHECK(hipGetLastError());
+    return BCMC_OK;
+}
+
+void lru_state_free(LruState *L) {
+    dev_free(L->v.age);
+    dev_free(L->v.avail);
+    dev_free(L->v.item);
+    dev_free(L->v.avail_ids);
+    dev_free(L->v.order);
+    dev_free(L->v.item_slot);
+    dev_free(L->data);
+    dev_free(L->new_items);
+    scan_site_free(&L->site_miss);
+    scan_site_free(&L->site_avail);
+    sort_scratch_free(&L->sort);
+    *L = LruState{};
+}
+
+// Shared tail of an update once q->num_new is on the device.
+static void enqueue_tail(bcmc_ctx *ctx, LruState *L, QState *q, const uint32_t *gate) {
+    hipStream_t st = ctx->stream;
+    launch_scan<1>(ctx, L->site_avail, L->site_avail, AvailOp{L->v, q}, L->capacity);
+    lru_plan_kernel<<<1, 1, 0, st>>>(L->v, q, gate);
+    lru_grow_kernel<<<grid_for(ctx, L->capacity, 2), 256, 0, st>>>(L->v, q);
+    launch_lru_order(ctx, &L->sort, L->v.avail_ids, L->v.age, &q->num_avail, L->capacity,
+                     L->v.order);
+    lru_assign_kernel<<<grid_for(ctx, L->capacity, 2), 256, 0, st>>>(L->v, L->new_items, q);
+}
+
+void lru_enqueue_update_active(bcmc_ctx *ctx, LruState *L, const uint32_t *flags,
+                               const uint32_t *active_ids, QState *q) {
+    lru_age_mark_kernel<<<grid_for(ctx, L->capacity), 256, 0, ctx->stream>>>(L->v, flags, q,
+                                                                              &q->num_active);
+    launch_scan<1>(ctx, L->site_miss, L->site_miss, MissActiveOp{L->v, active_ids, L->new_items, q},
+                   L->aligned_total);
+    enqueue_tail(ctx, L, q, &q->num_active);
+}
+
+void lru_enqueue_update_all(bcmc_ctx *ctx, LruState *L, const uint32_t *needed, QState *q) {
+    lru_age_mark_kernel<<<grid_for(ctx, L->capacity), 256, 0, ctx->stream>>>(L->v, needed, q,
+                                                                              nullptr);
+    launch_scan<1>(ctx, L->site_miss, L->site_miss, MissAllOp{L->v, needed, L->new_items, q},
+                   L->aligned_total);
+    enqueue_tail(ctx, L, q, nullptr);
+}
+
+void lru_enqueue_reset(bcmc_ctx *ctx, LruState *L) {
+    lru_init_kernel<<<grid_for(ctx, std::max(L->capacity, L->aligned_total)), 256, 0, ctx->stream>>>(
+        L->v, 0, L->cache_size, 100000, 1);
+}
+
+}  // namespace bcmc
+
+using namespace bcmc;
+
+// ---------------------------------------------------------------------------
+// C ABI: LRUCache
+// ---------------------------------------------------------------------------
+
+static int lru_sync_state(bcmc_lru *l) {
+    BCMC_HIP_CHECK(hipMemcpyAsync(l->qh, l->q, sizeof(QState), hipMemcpyDeviceToHost,
+                                  l->ctx->stream));
+    BCMC_HIP_CHECK(hipStreamSynchronize(l->ctx->stream));
+    l->st.cache_size = l->qh->cache_size;
+    l->cache_view.bytes = (uint64_t)l->st.cache_size * l->st.elem_bytes;
+    return BCMC_OK;
+}
+
+extern "C" {
+
+BCMC_API int bcmc_lru_create(bcmc_ctx *ctx, uint32_t initial_size, uint32_t elem_bytes,
+                             uint32_t total, bcmc_lru **out) {
+    BCMC_CHECK_ARG(ctx && out, "null argument");
+    BCMC_CHECK_ARG(total > 0 && total < 0x7FFFFFE0u, "totalElements out of range");
+    BCMC_CHECK_ARG(elem_bytes > 0 && elem_bytes % 4 == 0, "elementSize must be a multiple of 4");
+    *out = nullptr;
+    bcmc_lru *l = new bcmc_lru;
+    l->ctx = ctx;
+    int rc = lru_state_alloc(ctx, &l->st, initial_size, elem_bytes, total);
+    if (rc == BCMC_OK)
+        rc = dev_alloc((void **)&l->q, sizeof(QState));
+    if (rc == BCMC_OK && hipHostMalloc(&l->qh, sizeof(QState), hipHostMallocDefault) != hipSuccess) {
+        set_error("pinned allocation failed");
+        rc = BCMC_ERR_NOMEM;
+    }
+    if (rc != BCMC_OK) {
+        lru_state_free(&l->st);
+        dev_free(l->q);
+        delete l;
+        return rc;
+    }
+    QState init{};
+    init.cache_size = l->st.cache_size;
+    init.cache_size_next = l->st.cache_size;
+    *l->qh = init;
+    BCMC_HIP_CHECK(hipMemcpyAsync(l->q, l->qh, sizeof(QState), hipMemcpyHostToDevice, ctx->stream));
+    BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    l->cache_view.ctx = ctx;
+    l->cache_view.ptr = l->st.data;
+    l->cache_view.bytes = (uint64_t)l->st.cache_size * elem_bytes;
+    l->cache_view.owned = false;
+    *out = l;
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_lru_destroy(bcmc_lru *lru) {
+    if (!lru)
+        return BCMC_OK;
+    if (!lru->owns) {
+        bcmc::set_error("this LRUCache belongs to a CompressedMarchingCubes object");
+        return BCMC_ERR_INVALID;
+    }
+    (void)hipStreamSynchronize(lru->ctx->stream);
+    lru_state_free(&lru->st);
+    dev_free(lru->q);
+    (void)hipHostFree(lru->qh);
+    delete lru;
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_lru_update(bcmc_lru *lru, bcmc_buf *needed, uint32_t *num_new,
+                             bcmc_buf **new_items) {
+    BCMC_CHECK_ARG(lru && needed && num_new, "null argument");
+    BCMC_CHECK_ARG(needed->bytes >= (uint64_t)lru->st.total * 4,
+                   "itemNeeded must hold totalElements u32 flags");
+    if (new_items)
+        *new_items = nullptr;
+    lru_enqueue_update_all(lru->ctx, &lru->st, (const uint32_t *)needed->ptr, lru->q);
+    BCMC_HIP_CHECK(hipGetLastError());
+    BCMC_TRY(lru_sync_state(lru));
+    *num_new = lru->qh->num_new;
+    if (*num_new && new_items) {
+        bcmc_buf *b = nullptr;
+        BCMC_TRY(make_buf(lru->ctx, (uint64_t)*num_new * 4, &b));
+        BCMC_HIP_CHECK(hipMemcpyAsync(b->ptr, lru->st.new_items, (size_t)*num_new * 4,
+                                      hipMemcpyDeviceToDevice, lru->ctx->stream));
+        BCMC_HIP_CHECK(hipStreamSynchronize(lru->ctx->stream));
+        *new_items = b;
+    }
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_lru_reset(bcmc_lru *lru) {
+    BCMC_CHECK_ARG(lru, "null argument");
+    lru_enqueue_reset(lru->ctx, &lru->st);
+    BCMC_HIP_CHECK(hipGetLastError());
+    BCMC_HIP_CHECK(hipStreamSynchronize(lru->ctx->stream));
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_lru_get_info(bcmc_lru *lru, bcmc_lru_info *info) {
+    BCMC_CHECK_ARG(lru && info, "null argument");
+    BCMC_TRY(lru_sync_state(lru));
+    info->cache_size = lru->st.cache_size;
+    info->total_elements = lru->st.total;
+    info->element_size = lru->st.elem_bytes;
+    info->slots_available = lru->qh->avail_display;
+    info->reserved_bytes = (uint64_t)lru->st.capacity * lru->st.elem_bytes;
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_lru_read_state(bcmc_lru *lru, uint32_t *age, uint32_t *available, int32_t *item,
+                                 int32_t *item_slot) {
+    BCMC_CHECK_ARG(lru, "null argument");
+    BCMC_TRY(lru_sync_state(lru));
+    return lru_copy_state(lru->ctx, &lru->st, age, available, item, item_slot);
+}
+
+BCMC_API bcmc_buf *bcmc_lru_cache_buffer(bcmc_lru *lru) { return lru ? &lru->cache_view : nullptr; }
+
+}  // extern "C"
+
+namespace bcmc {
+int lru_copy_state(bcmc_ctx *ctx, LruState *L, uint32_t *age, uint32_t *available, int32_t *item,
+                   int32_t *item_slot) {
+    const size_t c = L->cache_size;
+    if (age)
+        BCMC_HIP_CHECK(hipMemcpyAsync(age, L->v.age, c * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (available)
+        BCMC_HIP_CHECK(hipMemcpyAsync(available, L->v.avail, c * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (item)
+        BCMC_HIP_CHECK(hipMemcpyAsync(item, L->v.item, c * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (item_slot)
+        BCMC_HIP_CHECK(hipMemcpyAsync(item_slot, L->v.item_slot, (size_t)L->total * 4,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return BCMC_OK;
+}
+}  // namespace bcmc

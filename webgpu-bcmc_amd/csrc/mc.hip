@@ -1,0 +1,635 @@
+// CompressedMarchingCubes on gfx950 (js/compressed_marching_cubes.js).
+//
+// One query (computeSurface, :580-809) is a fixed chain of kernels on the
+// context's stream with a single host read-back at the end:
+//   active     block active test (compute_block_active.comp) fused with a
+//              single-pass scan + compaction of the flags -> active ids;
+//   lru        LRUCache.update over the active list (lru.hip);
+//   decode     zfp decode of the misses into their cache slots (zfp.hip);
+//   count      per active block: 5^3 LDS tile with the ghost layer
+//              (compute_vertex_values.comp:96-252) and the per-cell triangle
+//              count (compute_block_has_vertices.comp +
+//              compute_block_voxel_num_verts.comp), one wave per block;
+//   offsets    two-channel single-pass scan of the counts: the compacted list
+//              of blocks with vertices (indices into the active list) and
+//              their 64-bit vertex offsets;
+//   emit       per block with vertices: the same tile, a wave prefix sum of
+//              the per-cell counts, interpolation and the vertex writes
+//              (compute_block_vertices.comp:76-174).
+// The vertex buffer grows like the reference (x1.5, :774-785); when a query
+// needs more room the emit kernel writes nothing and is re-run once the
+// buffer has grown.
+#include "common.h"
+#include "kernels.h"
+#include "lru.h"
+#include "tri_table.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace bcmc {
+
+__constant__ unsigned long long kTriEdges[256] = BCMC_TRI_EDGES_INIT;
+__constant__ uint8_t kTriCount[256] = BCMC_TRI_COUNT_INIT;
+// edge_vertices (compute_block_vertices.comp:10-23) as two nibbles per edge
+__constant__ uint8_t kEdgeVerts[12] = {0x10, 0x21, 0x32, 0x03, 0x54, 0x56,
+                                       0x76, 0x47, 0x40, 0x51, 0x62, 0x73};
+
+// index_to_vertex (compute_vertex_values.comp:48-57): corner i at
+// (x, y, z) = (((i+1)>>1)&1, (i>>1)&1, i>>2)
+BCMC_DEV uint32_t corner_x(uint32_t i) { return ((i + 1) >> 1) & 1u; }
+BCMC_DEV uint32_t corner_y(uint32_t i) { return (i >> 1) & 1u; }
+BCMC_DEV uint32_t corner_z(uint32_t i) { return i >> 2; }
+
+struct McView {
+    const float *cache;
+    const int32_t *item_slot;
+    const uint32_t *flags;
+    uint32_t nb[3];
+    uint32_t capacity;  // cache slots (reads outside are skipped)
+};
+
+constexpr int kMcThreads = 256;  // 4 waves, one block each
+constexpr int kTileStride = 128;
+
+// compute_block_dims_with_ghost + load_block (compute_vertex_values.comp:96-252).
+// Tile layout x + 5y + 25z.  Returns the dims through d[3] (wave uniform).
+BCMC_DEV void load_tile(const McView &m, uint32_t bid, float *tile, int32_t *slots, uint32_t (&d)[3]) {
+    const uint32_t lane = lane_id();
+    const uint32_t bx = bid % m.nb[0], by = (bid / m.nb[0]) % m.nb[1], bz = bid / (m.nb[0] * m.nb[1]);
+    // neighbour activity: +x, +y, +z, +xy, +xz, +yz, +xyz
+    bool act = false;
+    if (lane < 7) {
+        const uint32_t code = lane < 3 ? (1u << lane) : (lane == 3 ? 3u : lane == 4 ? 5u : lane == 5 ? 6u : 7u);
+        const uint32_t nx = bx + (code & 1u), ny = by + ((code >> 1) & 1u), nz = bz + (code >> 2);
+        if (nx < m.nb[0] && ny < m.nb[1] && nz < m.nb[2])
+            act = m.flags[nx + m.nb[0] * (ny + m.nb[1] * nz)] != 0;
+    }
+    const uint64_t a = __ballot(act);
+    uint32_t dx = bx + 1 < m.nb[0] ? 5 : 4, dy = by + 1 < m.nb[1] ? 5 : 4, dz = bz + 1 < m.nb[2] ? 5 : 4;
+    if (dx == 5 && dy == 5 && dz == 5 && !(a & 64))
+        dx = dy = dz = 4;
+    if (dx == 5 && dy == 5 && !(a & 8))
+        dx = dy = 4;
+    if (dx == 5 && dz == 5 && !(a & 16))
+        dx = dz = 4;
+    if (dy == 5 && dz == 5 && !(a & 32))
+        dy = dz = 4;
+    if (dx == 5 && !(a & 1))
+        dx = 4;
+    if (dy == 5 && !(a & 2))
+        dy = 4;
+    if (dz == 5 && !(a & 4))
+        dz = 4;
+    d[0] = dx;
+    d[1] = dy;
+    d[2] = dz;
+    // cache slots of the (up to 8) blocks the tile touches
+    if (lane < 8) {
+        const uint32_t sx = lane & 1u, sy = (lane >> 1) & 1u, sz = lane >> 2;
+        const bool need = (!sx || dx == 5) && (!sy || dy == 5) && (!sz || dz == 5);
+        slots[lane] = need ? m.item_slot[(bx + sx) + m.nb[0] * ((by + sy) + m.nb[1] * (bz + sz))] : -1;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t t = lane; t < 125; t += 64) {
+        const uint32_t x = t % 5, y = (t / 5) % 5, z = t / 25;
+        float v = 0.0f;
+        if (x < dx && y < dy && z < dz) {
+            const int32_t s = slots[(x >> 2) + 2 * (y >> 2) + 4 * (z >> 2)];
+            if (s >= 0 && (uint32_t)s < m.capacity)
+                v = m.cache[(uint64_t)(uint32_t)s * 64 + (x & 3) + 4 * (y & 3) + 16 * (z & 3)];
+        }
+        tile[t] = v;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Cell of this lane: case index (bit i iff corner i <= iso) or -1 if the cell
+// is outside the tile (compute_block_has_vertices.comp:28-37).
+BCMC_DEV int cell_case(const float *tile, const uint32_t (&d)[3], float iso, uint32_t &base) {
+    const uint32_t lane = lane_id();
+    const uint32_t x = lane & 3u, y = (lane >> 2) & 3u, z = lane >> 4;
+    base = x + 5 * y + 25 * z;
+    if (!(x + 1 < d[0] && y + 1 < d[1] && z + 1 < d[2]))
+        return -1;
+    int c = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+        const float v = tile[base + corner_x(i) + 5 * corner_y(i) + 25 * corner_z(i)];
+        c |= (v <= iso) ? (1 << i) : 0;
+    }
+    return c;
+}
+
+// ---- active blocks ---------------------------------------------------------
+struct ActiveOp {
+    const float2 *ranges;
+    uint32_t *flags, *active_ids;
+    QState *q;
+    float iso;
+    uint32_t nb[3], n;
+    BCMC_DEV uint32_t size() const { return n; }
+    BCMC_DEV void begin() const {
+        q->iso = iso;
+        q->num_new = 0;
+        q->num_with = 0;
+        q->num_verts = 0;
+        q->overflow = 0;
+    }
+    // compute_block_active.comp:127-164
+    BCMC_DEV void value(uint32_t b, uint32_t (&v)[1]) const {
+        const float2 r = ranges[b];
+        uint32_t act = 0;
+        if (iso >= r.x && iso <= r.y) {
+            act = 1;
+        } else {
+            const int bx = (int)(b % nb[0]), by = (int)((b / nb[0]) % nb[1]),
+                      bz = (int)(b / (nb[0] * nb[1]));
+            for (int k = -1; k < 2 && !act; ++k)
+                for (int j = -1; j < 2 && !act; ++j)
+                    for (int i = -1; i < 2 && !act; ++i) {
+                        const int cx = bx + i, cy = by + j, cz = bz + k;
+                        if ((i | j | k) == 0 || cx < 0 || cy < 0 || cz < 0 || cx >= (int)nb[0] ||
+                            cy >= (int)nb[1] || cz >= (int)nb[2])
+                            continue;
+                        const float2 o = ranges[cx + nb[0] * (cy + nb[1] * cz)];
+                        const float lo = fminf(r.x, o.x), hi = fmaxf(r.y, o.y);
+                        if (iso >= lo && iso <= hi)
+                            act = 1;
+                    }
+        }
+        v[0] = act;
+    }
+    BCMC_DEV void emit(uint32_t b, const uint32_t (&v)[1], const unsigned long long (&e)[1]) const {
+        flags[b] = v[0];
+        if (v[0])
+            active_ids[e[0]] = b;
+    }
+    BCMC_DEV void total(const unsigned long long (&t)[1]) const { q->num_active = (uint32_t)t[0]; }
+};
+
+// ---- per-block vertex counts -------------------------------------------------
+__global__ __launch_bounds__(kMcThreads) void mc_count_kernel(McView m, const uint32_t *active_ids,
+                                                              const QState *q, uint32_t *counts) {
+    __shared__ float tiles[kMcThreads / 64][kTileStride];
+    __shared__ int32_t slots[kMcThreads / 64][8];
+    const uint32_t wave = threadIdx.x / 64, lane = lane_id();
+    const uint32_t n = q->num_active;
+    const float iso = q->iso;
+    for (uint32_t k = blockIdx.x * (kMcThreads / 64) + wave; k < n; k += gridDim.x * (kMcThreads / 64)) {
+        uint32_t d[3], base;
+        load_tile(m, active_ids[k], tiles[wave], slots[wave], d);
+        const int c = cell_case(tiles[wave], d, iso, base);
+        const uint32_t nv = c >= 0 ? kTriCount[c] : 0u;
+        const uint32_t total = wave_sum(nv);
+        if (lane == 0)
+            counts[k] = total;
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// ---- blocks with vertices + vertex offsets ---------------------------------
+struct CountScanOp {
+    const uint32_t *counts;
+    uint32_t *with_verts;
+    unsigned long long *vert_offsets;
+    QState *q;
+    BCMC_DEV uint32_t size() const { return q->num_active; }
+    BCMC_DEV void begin() const {}
+    BCMC_DEV void value(uint32_t k, uint32_t (&v)[2]) const {
+        const uint32_t c = counts[k];
+        v[0] = c ? 1u : 0u;
+        v[1] = c;
+    }
+    BCMC_DEV void emit(uint32_t k, const uint32_t (&v)[2], const unsigned long long (&e)[2]) const {
+        if (v[0]) {
+            with_verts[e[0]] = k;
+            vert_offsets[e[0]] = e[1];
+        }
+    }
+    BCMC_DEV void total(const unsigned long long (&t)[2]) const {
+        q->num_with = (uint32_t)t[0];
+        q->num_verts = t[1];
+    }
+};
+
+// ---- vertex emission -------------------------------------------------------
+struct EmitArgs {
+    McView m;
+    const uint32_t *active_ids, *with_verts;
+    const unsigned long long *vert_offsets;
+    const QState *q;
+    void *verts;
+    unsigned long long capacity;
+    int fmt;
+};
+
+__global__ __launch_bounds__(kMcThreads) void mc_emit_kernel(EmitArgs a) {
+    __shared__ float tiles[kMcThreads / 64][kTileStride];
+    __shared__ int32_t slots[kMcThreads / 64][8];
+    if (a.q->num_verts > a.capacity)
+        return;  // the host grows the buffer and runs this kernel again
+    const uint32_t wave = threadIdx.x / 64;
+    const uint32_t n = a.q->num_with;
+    const float iso = a.q->iso;
+    for (uint32_t h = blockIdx.x * (kMcThreads / 64) + wave; h < n; h += gridDim.x * (kMcThreads / 64)) {
+        const uint32_t bid = a.active_ids[a.with_verts[h]];
+        uint32_t d[3], base;
+        const float *tile = tiles[wave];
+        load_tile(a.m, bid, tiles[wave], slots[wave], d);
+        const int c = cell_case(tile, d, iso, base);
+        const uint32_t nv = c >= 0 ? kTriCount[c] : 0u;
+        const uint32_t excl = wave_incl_scan(nv) - nv;
+        const unsigned long long out0 = a.vert_offsets[h] + excl;
+        const uint32_t lane = lane_id();
+        const float vx = (float)(lane & 3u), vy = (float)((lane >> 2) & 3u), vz = (float)(lane >> 4);
+        const unsigned long long edges = c >= 0 ? kTriEdges[c] : 0ull;
+        const uint32_t bx = bid % a.m.nb[0], by = (bid / a.m.nb[0]) % a.m.nb[1],
+                       bz = bid / (a.m.nb[0] * a.m.nb[1]);
+        for (uint32_t j = 0; j < nv; ++j) {
+            const uint32_t e = (uint32_t)(edges >> (4 * j)) & 15u;
+            const uint32_t ev = kEdgeVerts[e], ca = ev & 15u, cb = ev >> 4;
+            const float fa = tile[base + corner_x(ca) + 5 * corner_y(ca) + 25 * corner_z(ca)];
+            const float fb = tile[base + corner_x(cb) + 5 * corner_y(cb) + 25 * corner_z(cb)];
+            // lerp_verts (compute_block_vertices.comp:62-74), then + voxel + 0.5
+            float t = 0.0f;
+            if (!(fabsf(fa - fb) < 0.001f))
+                t = (iso - fa) / (fb - fa);
+            const float ax = (float)corner_x(ca), ay = (float)corner_y(ca), az = (float)corner_z(ca);
+            const float px = (ax + t * ((float)corner_x(cb) - ax) + vx) + 0.5f;
+            const float py = (ay + t * ((float)corner_y(cb) - ay) + vy) + 0.5f;
+            const float pz = (az + t * ((float)corner_z(cb) - az) + vz) + 0.5f;
+            const unsigned long long o = out0 + j;
+            if (a.fmt == BCMC_VERTS_LOCAL_F32) {
+                reinterpret_cast<float4 *>(a.verts)[o] = make_float4(px, py, pz, __uint_as_float(bid));
+            } else if (a.fmt == BCMC_VERTS_GLOBAL_F32) {
+                float *g = reinterpret_cast<float *>(a.verts) + 3 * o;
+                g[0] = (float)(4 * bx) + px;
+                g[1] = (float)(4 * by) + py;
+                g[2] = (float)(4 * bz) + pz;
+            } else {
+                // compress_position (block_vertex_position.comp:9-17)
+                const uint32_t qx = (uint32_t)(((px - 0.5f) * 0.25f) * 1023.0f);
+                const uint32_t qy = (uint32_t)(((py - 0.5f) * 0.25f) * 1023.0f);
+                const uint32_t qz = (uint32_t)(((pz - 0.5f) * 0.25f) * 1023.0f);
+                const uint32_t packed = ((qx & 0x3ffu) << 20) | ((qy & 0x3ffu) << 10) | (qz & 0x3ffu);
+                reinterpret_cast<uint2 *>(a.verts)[o] = make_uint2(packed, bid);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+static uint32_t vertex_bytes(int fmt) {
+    return fmt == BCMC_VERTS_LOCAL_F32 ? 16u : fmt == BCMC_VERTS_GLOBAL_F32 ? 12u : 8u;
+}
+
+}  // namespace bcmc
+
+using namespace bcmc;
+
+// ---------------------------------------------------------------------------
+// C ABI: CompressedMarchingCubes
+// ---------------------------------------------------------------------------
+struct bcmc_mc {
+    bcmc_ctx *ctx = nullptr;
+    bool has_volume = false;
+    uint32_t dims[3] = {0, 0, 0}, pdims[3] = {0, 0, 0}, nb[3] = {0, 0, 0};
+    uint32_t total = 0, maxbits = 0;
+    float scale[3] = {1, 1, 1};
+    uint64_t *stream = nullptr;
+    uint64_t stream_bytes = 0, stream_words = 0;
+    float2 *ranges = nullptr;
+    uint32_t *flags = nullptr, *active_ids = nullptr, *counts = nullptr, *with_verts = nullptr;
+    unsigned long long *vert_offsets = nullptr;
+    bcmc_lru lru;  // non-owning handle over the cache state held here
+    QState *q = nullptr, *qh = nullptr;
+    ScanSite site_active, site_c0, site_c1;
+    void *verts = nullptr;
+    uint64_t vert_cap = 0;
+    int fmt = BCMC_VERTS_LOCAL_F32;
+    bcmc_buf vert_view, views[6];
+    hipEvent_t ev[7] = {};
+    uint32_t last_active = 0, last_with = 0, last_new = 0;
+    uint64_t last_verts = 0;
+};
+
+static void mc_free_volume(bcmc_mc *m) {
+    if (m->ctx)
+        (void)hipStreamSynchronize(m->ctx->stream);
+    dev_free(m->stream);
+    dev_free(m->ranges);
+    dev_free(m->flags);
+    dev_free(m->active_ids);
+    dev_free(m->counts);
+    dev_free(m->with_verts);
+    dev_free(m->vert_offsets);
+    dev_free(m->verts);
+    scan_site_free(&m->site_active);
+    scan_site_free(&m->site_c0);
+    scan_site_free(&m->site_c1);
+    lru_state_free(&m->lru.st);
+    m->stream = nullptr;
+    m->ranges = nullptr;
+    m->flags = m->active_ids = m->counts = m->with_verts = nullptr;
+    m->vert_offsets = nullptr;
+    m->verts = nullptr;
+    m->vert_cap = 0;
+    m->has_volume = false;
+}
+
+static void mc_refresh_views(bcmc_mc *m) {
+    auto set = [&](bcmc_buf &b, void *p, uint64_t bytes) {
+        b.ctx = m->ctx;
+        b.ptr = p;
+        b.bytes = bytes;
+        b.owned = false;
+    };
+    set(m->vert_view, m->verts, m->last_verts * vertex_bytes(m->fmt));
+    set(m->views[BCMC_ARRAY_RANGES], m->ranges, (uint64_t)m->total * 8);
+    set(m->views[BCMC_ARRAY_ACTIVE_FLAGS], m->flags, (uint64_t)m->total * 4);
+    set(m->views[BCMC_ARRAY_ACTIVE_IDS], m->active_ids, (uint64_t)m->last_active * 4);
+    set(m->views[BCMC_ARRAY_NEW_ITEMS], m->lru.st.new_items, (uint64_t)m->last_new * 4);
+    set(m->views[BCMC_ARRAY_WITH_VERTS], m->with_verts, (uint64_t)m->last_with * 4);
+    set(m->views[BCMC_ARRAY_VERT_OFFSETS], m->vert_offsets, (uint64_t)m->last_with * 8);
+    m->lru.cache_view.ctx = m->ctx;
+    m->lru.cache_view.ptr = m->lru.st.data;
+    m->lru.cache_view.bytes = (uint64_t)m->lru.st.cache_size * m->lru.st.elem_bytes;
+    m->lru.cache_view.owned = false;
+}
+
+static int mc_launch_emit(bcmc_mc *m) {
+    EmitArgs a;
+    a.m = McView{(const float *)m->lru.st.data, m->lru.st.v.item_slot, m->flags,
+                 {m->nb[0], m->nb[1], m->nb[2]}, m->lru.st.capacity};
+    a.active_ids = m->active_ids;
+    a.with_verts = m->with_verts;
+    a.vert_offsets = m->vert_offsets;
+    a.q = m->q;
+    a.verts = m->verts;
+    a.capacity = m->vert_cap;
+    a.fmt = m->fmt;
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(m->total, kMcThreads / 64),
+                                                                   persistent_grid(m->ctx, 8)));
+    mc_emit_kernel<<<grid, kMcThreads, 0, m->ctx->stream>>>(a);
+    BCMC_HIP_CHECK(hipGetLastError());
+    return BCMC_OK;
+}
+
+extern "C" {
+
+BCMC_API int bcmc_mc_create(bcmc_ctx *ctx, bcmc_mc **out) {
+    BCMC_CHECK_ARG(ctx && out, "null argument");
+    bcmc_mc *m = new bcmc_mc;
+    m->ctx = ctx;
+    if (dev_alloc((void **)&m->q, sizeof(QState)) != BCMC_OK ||
+        hipHostMalloc(&m->qh, sizeof(QState), hipHostMallocDefault) != hipSuccess) {
+        dev_free(m->q);
+        delete m;
+        set_error("allocation of the query counters failed");
+        return BCMC_ERR_NOMEM;
+    }
+    for (auto &e : m->ev)
+        BCMC_HIP_CHECK(hipEventCreate(&e));
+    m->lru.ctx = ctx;
+    m->lru.q = m->q;
+    m->lru.qh = m->qh;
+    m->lru.owns = false;
+    *out = m;
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_mc_destroy(bcmc_mc *m) {
+    if (!m)
+        return BCMC_OK;
+    mc_free_volume(m);
+    dev_free(m->q);
+    (void)hipHostFree(m->qh);
+    for (auto &e : m->ev)
+        (void)hipEventDestroy(e);
+    delete m;
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *m, const void *stream, uint64_t bytes,
+                                           double rate, const uint32_t dims[3],
+                                           const float scale[3], uint32_t initial_cache_slots) {
+    BCMC_CHECK_ARG(m && stream && dims, "null argument");
+    BCMC_CHECK_ARG(dims[0] && dims[1] && dims[2], "volume dims must be positive");
+    const double mb = 64.0 * rate;
+    BCMC_CHECK_ARG(mb >= 9.0 && mb <= 4096.0 && mb == std::floor(mb),
+                   "64*compressionRate must be an integer in [9, 4096]");
+    uint32_t nb[3];
+    for (int i = 0; i < 3; ++i)
+        nb[i] = (dims[i] + 3) / 4;
+    const uint64_t total = (uint64_t)nb[0] * nb[1] * nb[2];
+    BCMC_CHECK_ARG(total < (1ull << 31), "volume has too many blocks");
+    const uint32_t maxbits = (uint32_t)mb;
+    const uint64_t need = (total * maxbits + 7) / 8;
+    BCMC_CHECK_ARG(bytes >= need, "compressed stream shorter than the volume at this rate");
+    mc_free_volume(m);
+    bcmc_ctx *ctx = m->ctx;
+    for (int i = 0; i < 3; ++i) {
+        m->dims[i] = dims[i];
+        m->pdims[i] = nb[i] * 4;
+        m->nb[i] = nb[i];
+        m->scale[i] = scale ? scale[i] : 1.0f;
+    }
+    m->total = (uint32_t)total;
+    m->maxbits = maxbits;
+    // stream + zero padding (the reader may look one word past a block)
+    m->stream_bytes = bytes;
+    m->stream_words = (bytes + 7) / 8 + 2;
+    BCMC_TRY(dev_alloc((void **)&m->stream, m->stream_words * 8));
+    BCMC_HIP_CHECK(hipMemsetAsync(m->stream, 0, m->stream_words * 8, ctx->stream));
+    BCMC_HIP_CHECK(hipMemcpyAsync(m->stream, stream, bytes, hipMemcpyHostToDevice, ctx->stream));
+    const size_t n = m->total;
+    BCMC_TRY(dev_alloc((void **)&m->ranges, n * 8));
+    BCMC_TRY(dev_alloc((void **)&m->flags, n * 4));
+    BCMC_TRY(dev_alloc((void **)&m->active_ids, n * 4));
+    BCMC_TRY(dev_alloc((void **)&m->counts, n * 4));
+    BCMC_TRY(dev_alloc((void **)&m->with_verts, n * 4));
+    BCMC_TRY(dev_alloc((void **)&m->vert_offsets, n * 8));
+    BCMC_TRY(scan_site_alloc(&m->site_active, n, kScanTile));
+    BCMC_TRY(scan_site_alloc(&m->site_c0, n, kScanTile));
+    BCMC_TRY(scan_site_alloc(&m->site_c1, n, kScanTile));
+    const uint32_t init = initial_cache_slots ? initial_cache_slots
+                                              : (uint32_t)std::ceil((double)total * 0.05);
+    BCMC_TRY(lru_state_alloc(ctx, &m->lru.st, init, 256, m->total));
+    QState qi{};
+    qi.cache_size = m->lru.st.cache_size;
+    qi.cache_size_next = qi.cache_size;
+    *m->qh = qi;
+    BCMC_HIP_CHECK(hipMemcpyAsync(m->q, m->qh, sizeof(QState), hipMemcpyHostToDevice, ctx->stream));
+    // computeBlockRanges (:522-578): full-volume decode, min/max per block
+    launch_block_ranges(ctx, m->stream, m->stream_words, maxbits, m->dims, m->ranges);
+    BCMC_HIP_CHECK(hipGetLastError());
+    BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    m->has_volume = true;
+    m->last_active = m->last_with = m->last_new = 0;
+    m->last_verts = 0;
+    mc_refresh_views(m);
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_mc_set_vertex_format(bcmc_mc *m, bcmc_vertex_format fmt) {
+    BCMC_CHECK_ARG(m, "null argument");
+    BCMC_CHECK_ARG(fmt >= BCMC_VERTS_LOCAL_F32 && fmt <= BCMC_VERTS_PACKED_U32X2, "bad format");
+    if (fmt != m->fmt) {
+        if (m->ctx)
+            (void)hipStreamSynchronize(m->ctx->stream);
+        dev_free(m->verts);
+        m->verts = nullptr;
+        m->vert_cap = 0;
+        m->fmt = fmt;
+        m->last_verts = 0;
+    }
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_mc_compute_surface(bcmc_mc *m, float isovalue, bcmc_perf *perf,
+                                     uint64_t *num_vertices) {
+    BCMC_CHECK_ARG(m, "null argument");
+    if (!m->has_volume) {
+        set_error("computeSurface called before setCompressedVolume");
+        return BCMC_ERR_STATE;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    bcmc_ctx *ctx = m->ctx;
+    hipStream_t st = ctx->stream;
+    const bool timed = perf != nullptr;
+    if (timed)
+        BCMC_HIP_CHECK(hipEventRecord(m->ev[0], st));
+    // active blocks + compaction
+    ActiveOp aop{m->ranges, m->flags, m->active_ids, m->q, isovalue,
+                 {m->nb[0], m->nb[1], m->nb[2]}, m->total};
+    launch_scan<1>(ctx, m->site_active, m->site_active, aop, m->total);
+    if (timed)
+        BCMC_HIP_CHECK(hipEventRecord(m->ev[1], st));
+    // cache update
+    lru_enqueue_update_active(ctx, &m->lru.st, m->flags, m->active_ids, m->q);
+    if (timed)
+        BCMC_HIP_CHECK(hipEventRecord(m->ev[2], st));
+    // decode misses into their slots
+    launch_decode_to_cache(ctx, m->stream, m->stream_words, m->maxbits, m->lru.st.new_items,
+                           &m->q->num_new, m->total, m->lru.st.v.item_slot,
+                           (float *)m->lru.st.data, m->lru.st.capacity);
+    if (timed)
+        BCMC_HIP_CHECK(hipEventRecord(m->ev[3], st));
+    // per-block counts
+    McView mv{(const float *)m->lru.st.data, m->lru.st.v.item_slot, m->flags,
+              {m->nb[0], m->nb[1], m->nb[2]}, m->lru.st.capacity};
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(m->total, kMcThreads / 64),
+                                                                   persistent_grid(ctx, 8)));
+    mc_count_kernel<<<grid, kMcThreads, 0, st>>>(mv, m->active_ids, m->q, m->counts);
+    if (timed)
+        BCMC_HIP_CHECK(hipEventRecord(m->ev[4], st));
+    // blocks with vertices + offsets
+    CountScanOp cop{m->counts, m->with_verts, m->vert_offsets, m->q};
+    launch_scan<2>(ctx, m->site_c0, m->site_c1, cop, m->total);
+    if (timed)
+        BCMC_HIP_CHECK(hipEventRecord(m->ev[5], st));
+    BCMC_TRY(mc_launch_emit(m));
+    if (timed)
+        BCMC_HIP_CHECK(hipEventRecord(m->ev[6], st));
+    BCMC_HIP_CHECK(hipGetLastError());
+    BCMC_HIP_CHECK(hipMemcpyAsync(m->qh, m->q, sizeof(QState), hipMemcpyDeviceToHost, st));
+    BCMC_HIP_CHECK(hipStreamSynchronize(st));
+    const QState qs = *m->qh;
+    double emit_extra_ms = 0.0;
+    if (qs.num_verts > m->vert_cap) {
+        // grow like compressed_marching_cubes.js:774-785, then emit again
+        const uint64_t want = std::max<uint64_t>(qs.num_verts, (uint64_t)std::floor(m->vert_cap * 1.5));
+        dev_free(m->verts);
+        m->verts = nullptr;
+        m->vert_cap = 0;
+        BCMC_TRY(dev_alloc(&m->verts, want * vertex_bytes(m->fmt)));
+        m->vert_cap = want;
+        if (timed)
+            BCMC_HIP_CHECK(hipEventRecord(m->ev[5], st));
+        BCMC_TRY(mc_launch_emit(m));
+        if (timed)
+            BCMC_HIP_CHECK(hipEventRecord(m->ev[6], st));
+        BCMC_HIP_CHECK(hipStreamSynchronize(st));
+        (void)emit_extra_ms;
+    }
+    m->lru.st.cache_size = qs.cache_size;
+    m->last_active = qs.num_active;
+    m->last_with = qs.num_with;
+    m->last_new = qs.num_new;
+    m->last_verts = qs.num_verts;
+    mc_refresh_views(m);
+    if (num_vertices)
+        *num_vertices = qs.num_verts;
+    if (perf) {
+        float ms[6];
+        for (int i = 0; i < 6; ++i)
+            BCMC_HIP_CHECK(hipEventElapsedTime(&ms[i], m->ev[i], m->ev[i + 1]));
+        perf->computeActiveBlocks = ms[0];
+        perf->cacheUpdate = ms[1];
+        perf->decompression = ms[2];
+        perf->compactActiveIDs = 0.0;  // fused into computeActiveBlocks
+        perf->computeBlockHasVertices = ms[3];
+        perf->compactBlocksWithVertices = ms[4];
+        perf->computeVertices = ms[5];
+        perf->numActiveBlocks = qs.num_active;
+        perf->numBlocksDecompressed = qs.num_new;
+        perf->numBlocksWithVertices = qs.num_with;
+        perf->cacheSize = qs.cache_size;
+        perf->numVertices = qs.num_verts;
+        perf->total_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_mc_get_info(bcmc_mc *m, bcmc_mc_info *info) {
+    BCMC_CHECK_ARG(m && info, "null argument");
+    memset(info, 0, sizeof(*info));
+    info->total_blocks = m->total;
+    info->num_active_blocks = m->last_active;
+    info->num_blocks_with_vertices = m->last_with;
+    info->new_decompressed = m->last_new;
+    info->num_vertices = m->last_verts;
+    info->vertex_capacity = m->vert_cap;
+    for (int i = 0; i < 3; ++i) {
+        info->dims[i] = m->dims[i];
+        info->padded_dims[i] = m->pdims[i];
+        info->scale[i] = m->scale[i];
+    }
+    info->max_bits = m->maxbits;
+    info->format = (bcmc_vertex_format)m->fmt;
+    return BCMC_OK;
+}
+
+BCMC_API bcmc_buf *bcmc_mc_vertex_buffer(bcmc_mc *m) { return m ? &m->vert_view : nullptr; }
+BCMC_API bcmc_lru *bcmc_mc_lru(bcmc_mc *m) { return m && m->has_volume ? &m->lru : nullptr; }
+
+BCMC_API bcmc_buf *bcmc_mc_array_buffer(bcmc_mc *m, bcmc_mc_array which) {
+    if (!m || which < BCMC_ARRAY_RANGES || which > BCMC_ARRAY_VERT_OFFSETS)
+        return nullptr;
+    return &m->views[which];
+}
+
+BCMC_API int bcmc_mc_memory_use(bcmc_mc *m, uint64_t *total_out, uint64_t *cache_out) {
+    BCMC_CHECK_ARG(m, "null argument");
+    const uint64_t n = m->total;
+    const LruState &L = m->lru.st;
+    const uint64_t cache = (uint64_t)L.capacity * L.elem_bytes;
+    const uint64_t lru_meta = (uint64_t)L.capacity * 4 * 5 + (uint64_t)L.aligned_total * 8;
+    const uint64_t mc = m->stream_words * 8 + n * (8 + 4 + 4 + 4 + 4 + 8) +
+                        m->vert_cap * vertex_bytes(m->fmt);
+    if (total_out)
+        *total_out = mc + cache + lru_meta;
+    if (cache_out)
+        *cache_out = cache;
+    return BCMC_OK;
+}
+
+}  // extern "C"

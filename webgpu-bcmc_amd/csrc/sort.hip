@@ -1,0 +1,231 @@
+// Stable LSD radix sort of (u32 key, u32 value) pairs on gfx950, used for the
+// LRU slot ordering and the RadixSorter drop-in.
+//
+// The reference (js/radix_sort_by_key.js:161-400) sorts 64-element chunks
+// with 32 one-bit LDS split passes (radix_sort_chunk.comp), merges runs with
+// merge_sorted_chunks.comp, whose upper/lower bound pair places equal keys of
+// the later run first, reverses the value buffer (reverse_buffer.comp) and
+// reads back a slice (:382-391).  The resulting order is a total order:
+// ascending: key asc, ties by chunk (position/64) descending, then position
+// ascending; reversed: key desc, ties by chunk ascending, then position
+// descending, over the UINT_MAX-padded aligned array, of which the reference
+// keeps the first n (ascending) or the last n (reversed).  This file produces
+// exactly that order with a stable 8-bit LSD radix sort whose first pass
+// reads the input in that tie order (keys complemented for the descending
+// case).  Each pass: per-tile LDS histogram, a single-pass scan of the
+// digit-major histogram table, and a stable scatter that ranks equal digits
+// inside a tile with wave64 ballots.
+#include "common.h"
+#include "kernels.h"
+
+#include <algorithm>
+
+namespace bcmc {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 4;
+constexpr uint32_t kSortTile = kSortThreads * kSortItems;  // 1024
+
+// ---- element sources -------------------------------------------------------
+struct BufSource {  // passes >= 1
+    const uint32_t *keys, *vals;
+    const uint32_t *n_dev;
+    uint32_t n_host;
+    BCMC_DEV uint32_t size() const { return n_dev ? *n_dev : n_host; }
+    BCMC_DEV void get(uint32_t p, uint32_t &k, uint32_t &v) const {
+        k = keys[p];
+        v = vals[p];
+    }
+};
+
+// LRU: position p of the available-slot list in the reference's tie order
+// (64-chunks ascending, descending inside a chunk), key = ~age (descending).
+struct LruSource {
+    const uint32_t *avail_ids, *age;
+    const uint32_t *n_dev;
+    uint32_t capacity;
+    BCMC_DEV uint32_t size() const { return *n_dev; }
+    BCMC_DEV void get(uint32_t p, uint32_t &k, uint32_t &v) const {
+        const uint32_t n = *n_dev;
+        const uint32_t c0 = p & ~63u;
+        const uint32_t m = min(64u, n - c0);
+        const uint32_t j = c0 + (m - 1 - (p - c0));
+        v = avail_ids[j];
+        k = v < capacity ? ~age[v] : 0u;
+    }
+};
+
+// RadixSorter.sort: all `aligned` positions including UINT_MAX padding.
+struct GenericSource {
+    const uint32_t *keys, *vals;
+    uint32_t n, aligned, reverse;
+    BCMC_DEV uint32_t size() const { return aligned; }
+    BCMC_DEV void get(uint32_t p, uint32_t &k, uint32_t &v) const {
+        const uint32_t nch = aligned / 64, c = p / 64, i = p % 64;
+        const uint32_t j = reverse ? c * 64 + (63 - i) : (nch - 1 - c) * 64 + i;
+        const uint32_t kk = j < n ? keys[j] : 0xFFFFFFFFu;
+        v = j < n ? vals[j] : 0xFFFFFFFFu;
+        k = reverse ? ~kk : kk;
+    }
+};
+
+// ---- kernels ---------------------------------------------------------------
+template <class Src>
+__global__ __launch_bounds__(kSortThreads) void sort_hist_kernel(Src src, uint32_t shift,
+                                                                 uint32_t *hist) {
+    __shared__ uint32_t h[256];
+    const uint32_t n = src.size(), ntiles = div_up(n, kSortTile);
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        h[threadIdx.x] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j) {
+            const uint32_t p = tile * kSortTile + j * kSortThreads + threadIdx.x;
+            if (p < n) {
+                uint32_t k, v;
+                src.get(p, k, v);
+                atomicAdd(&h[(k >> shift) & 255u], 1u);
+            }
+        }
+        __syncthreads();
+        hist[threadIdx.x * ntiles + tile] = h[threadIdx.x];
+        __syncthreads();
+    }
+}
+
+struct HistScanOp {
+    uint32_t *hist;
+    const uint32_t *n_dev;
+    uint32_t n_host;
+    BCMC_DEV uint32_t size() const {
+        const uint32_t n = n_dev ? *n_dev : n_host;
+        return 256u * div_up(n, kSortTile);
+    }
+    BCMC_DEV void begin() const {}
+    BCMC_DEV void value(uint32_t i, uint32_t (&v)[1]) const { v[0] = hist[i]; }
+    BCMC_DEV void emit(uint32_t i, const uint32_t (&)[1], const unsigned long long (&e)[1]) const {
+        hist[i] = (uint32_t)e[0];
+    }
+    BCMC_DEV void total(const unsigned long long (&)[1]) const {}
+};
+
+// Stable scatter: rank = elements of the tile with the same digit and a
+// smaller position, found per round with 8 ballots and a per-digit LDS count
+// that the four waves of a round update in order.
+template <class Src>
+__global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(Src src, uint32_t shift,
+                                                                    const uint32_t *hist,
+                                                                    uint32_t *okeys, uint32_t *ovals,
+                                                                    uint32_t out_skip, uint32_t out_cap) {
+    __shared__ uint32_t cnt[256];
+    const uint32_t n = src.size(), ntiles = div_up(n, kSortTile);
+    const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        cnt[threadIdx.x] = hist[threadIdx.x * ntiles + tile];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j) {
+            const uint32_t p = tile * kSortTile + j * kSortThreads + threadIdx.x;
+            const bool valid = p < n;
+            uint32_t k = 0, v = 0;
+            if (valid)
+                src.get(p, k, v);
+            const uint32_t d = (k >> shift) & 255u;
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const uint64_t m = __ballot((d >> b) & 1u);
+                peers &= ((d >> b) & 1u) ? m : ~m;
+            }
+            const uint32_t rank = (uint32_t)__popcll(peers & lt);
+            const bool leader = valid && (peers & lt) == 0;
+            for (uint32_t w = 0; w < kSortThreads / 64; ++w) {
+                if (wave == w && valid) {
+                    const uint32_t dst = cnt[d] + rank;
+                    if (leader)
+                        cnt[d] += (uint32_t)__popcll(peers);
+                    if (dst >= out_skip && dst - out_skip < out_cap) {
+                        okeys[dst - out_skip] = k;
+                        ovals[dst - out_skip] = v;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
+template <class FirstSrc>
+static void sort_passes(bcmc_ctx *ctx, SortScratch *s, uint32_t n_max, const uint32_t *n_dev,
+                        uint32_t n_host, const FirstSrc &first_src, uint32_t *final_keys,
+                        uint32_t *final_vals, uint32_t out_skip) {
+    const uint32_t tiles = std::max<uint32_t>(1, div_up(n_max, kSortTile));
+    const uint32_t grid = std::min<uint32_t>(tiles, persistent_grid(ctx, 4));
+    hipStream_t st = ctx->stream;
+    HistScanOp hop{s->hist, n_dev, n_host};
+    for (int pass = 0; pass < 4; ++pass) {
+        const uint32_t shift = 8 * pass;
+        // pass 0: source -> buf0, 1: buf0 -> buf1, 2: buf1 -> buf0, 3: buf0 -> final
+        const int src_buf = (pass + 1) & 1;
+        const int dst_buf = pass & 1;
+        uint32_t *ok = pass == 3 ? final_keys : s->keys[dst_buf];
+        uint32_t *ov = pass == 3 ? final_vals : s->vals[dst_buf];
+        const uint32_t skip = pass == 3 ? out_skip : 0;
+        if (pass == 0) {
+            sort_hist_kernel<<<grid, kSortThreads, 0, st>>>(first_src, shift, s->hist);
+            launch_scan<1>(ctx, s->site, s->site, hop, 256ull * tiles);
+            sort_scatter_kernel<<<grid, kSortThreads, 0, st>>>(first_src, shift, s->hist, ok, ov, skip,
+                                                               s->max_elems);
+        } else {
+            BufSource b{s->keys[src_buf], s->vals[src_buf], n_dev, n_host};
+            sort_hist_kernel<<<grid, kSortThreads, 0, st>>>(b, shift, s->hist);
+            launch_scan<1>(ctx, s->site, s->site, hop, 256ull * tiles);
+            sort_scatter_kernel<<<grid, kSortThreads, 0, st>>>(b, shift, s->hist, ok, ov, skip,
+                                                               s->max_elems);
+        }
+    }
+}
+
+int sort_scratch_alloc(SortScratch *s, uint32_t max_elems) {
+    s->max_elems = std::max<uint32_t>(max_elems, 64);
+    s->max_tiles = div_up(s->max_elems, kSortTile);
+    for (int i = 0; i < 2; ++i) {
+        BCMC_TRY(dev_alloc((void **)&s->keys[i], (size_t)s->max_elems * 4));
+        BCMC_TRY(dev_alloc((void **)&s->vals[i], (size_t)s->max_elems * 4));
+    }
+    BCMC_TRY(dev_alloc((void **)&s->hist, (size_t)256 * s->max_tiles * 4));
+    BCMC_TRY(scan_site_alloc(&s->site, (uint64_t)256 * s->max_tiles, kScanTile));
+    return BCMC_OK;
+}
+
+void sort_scratch_free(SortScratch *s) {
+    for (int i = 0; i < 2; ++i) {
+        dev_free(s->keys[i]);
+        dev_free(s->vals[i]);
+        s->keys[i] = s->vals[i] = nullptr;
+    }
+    dev_free(s->hist);
+    s->hist = nullptr;
+    scan_site_free(&s->site);
+}
+
+void launch_lru_order(bcmc_ctx *ctx, SortScratch *s, const uint32_t *avail_ids,
+                      const uint32_t *age, const uint32_t *count_dev, uint32_t count_max,
+                      uint32_t *out) {
+    LruSource src{avail_ids, age, count_dev, count_max};
+    sort_passes(ctx, s, count_max, count_dev, 0, src, s->keys[1], out, 0);
+}
+
+void launch_sort_by_key(bcmc_ctx *ctx, SortScratch *s, uint32_t *keys, uint32_t *vals, uint32_t n,
+                        int reverse) {
+    uint32_t aligned = 64;  // nextPow2(ceil(n/64)) * 64
+    while (aligned < (uint64_t)((n + 63) / 64) * 64)
+        aligned *= 2;
+    GenericSource src{keys, vals, n, aligned, (uint32_t)(reverse != 0)};
+    // reversed: the reference keeps the last n of the descending order
+    sort_passes(ctx, s, aligned, nullptr, aligned, src, s->keys[1], s->vals[1],
+                reverse ? aligned - n : 0);
+}
+
+}  // namespace bcmc

@@ -174,17 +174,19 @@ static double field_final(const field_t *f, double smooth, uint64_t vi)
 
 static int nthreads(int t) { return t > 0 ? t : 1; }
 
-/* Pass 1 (and 2 with noise): global min/max. */
-static void field_minmax(field_t *f, int threads)
+/* Pass 1 (and 2 with noise): global min/max.  When `store` is non-null the
+ * smooth field is kept there (x fastest) so later passes need not recompute it. */
+static void field_minmax(field_t *f, int threads, double *store)
 {
     const uint32_t nx = f->dims[0], ny = f->dims[1], nz = f->dims[2];
     double lo = INFINITY, hi = -INFINITY;
 #pragma omp parallel num_threads(nthreads(threads))
     {
-        double *row = malloc(nx * sizeof(double));
+        double *tmp = malloc(nx * sizeof(double));
         double l = INFINITY, h = -INFINITY;
 #pragma omp for schedule(static)
         for (int64_t zy = 0; zy < (int64_t)ny * nz; ++zy) {
+            double *row = store ? store + (size_t)zy * nx : tmp;
             field_row(f, (uint32_t)(zy % ny), (uint32_t)(zy / ny), row);
             for (uint32_t x = 0; x < nx; ++x) {
                 l = row[x] < l ? row[x] : l;
@@ -196,7 +198,7 @@ static void field_minmax(field_t *f, int threads)
             lo = l < lo ? l : lo;
             hi = h > hi ? h : hi;
         }
-        free(row);
+        free(tmp);
     }
     f->fmin = lo;
     f->fmax = hi;
@@ -205,11 +207,13 @@ static void field_minmax(field_t *f, int threads)
         hi = -INFINITY;
 #pragma omp parallel num_threads(nthreads(threads))
         {
-            double *row = malloc(nx * sizeof(double));
+            double *tmp = malloc(nx * sizeof(double));
             double l = INFINITY, h = -INFINITY;
 #pragma omp for schedule(static)
             for (int64_t zy = 0; zy < (int64_t)ny * nz; ++zy) {
-                field_row(f, (uint32_t)(zy % ny), (uint32_t)(zy / ny), row);
+                double *row = store ? store + (size_t)zy * nx : tmp;
+                if (!store)
+                    field_row(f, (uint32_t)(zy % ny), (uint32_t)(zy / ny), row);
                 for (uint32_t x = 0; x < nx; ++x) {
                     const double v = (row[x] - f->fmin) / (f->fmax - f->fmin) +
                                      f->noise * noise_at(f->seed, (uint64_t)zy * nx + x);
@@ -222,7 +226,7 @@ static void field_minmax(field_t *f, int threads)
                 lo = l < lo ? l : lo;
                 hi = h > hi ? h : hi;
             }
-            free(row);
+            free(tmp);
         }
         f->gmin = lo;
         f->gmax = hi;
@@ -474,35 +478,43 @@ EXPORT int bcmc_synth_field(int kind, uint64_t seed, const uint32_t dims[3], uin
     field_t f;
     if (field_setup(&f, kind, seed, dims, count, params, noise))
         return -1;
-    field_minmax(&f, threads);
     const uint32_t nx = dims[0], ny = dims[1], nz = dims[2];
+    double *store = malloc((size_t)nx * ny * nz * sizeof(double));
+    field_minmax(&f, threads, store);
 #pragma omp parallel num_threads(nthreads(threads))
     {
-        double *row = malloc(nx * sizeof(double));
+        double *tmp = malloc(nx * sizeof(double));
 #pragma omp for schedule(static)
         for (int64_t zy = 0; zy < (int64_t)ny * nz; ++zy) {
-            field_row(&f, (uint32_t)(zy % ny), (uint32_t)(zy / ny), row);
+            double *row = store ? store + (size_t)zy * nx : tmp;
+            if (!store)
+                field_row(&f, (uint32_t)(zy % ny), (uint32_t)(zy / ny), row);
             for (uint32_t x = 0; x < nx; ++x)
                 out[(size_t)zy * nx + x] = (float)field_final(&f, row[x], (uint64_t)zy * nx + x);
         }
-        free(row);
+        free(tmp);
     }
+    free(store);
     field_free(&f);
     return 0;
 }
 
-/* Generate and encode a field one 4-slice slab at a time, so that only the
- * compressed stream (bcmc_synth_stream_bytes) is held in memory.  Produces the
- * same bytes as bcmc_synth_field followed by bcmc_synth_encode. */
+/* Generate and encode a field one 4-slice slab at a time.  The FP64 field is
+ * kept in host memory for volumes up to 2^31 voxels (16 GiB) and recomputed
+ * per pass beyond that.  Produces the same bytes as bcmc_synth_field followed
+ * by bcmc_synth_encode. */
 EXPORT int bcmc_synth_stream(int kind, uint64_t seed, const uint32_t dims[3], uint32_t count,
                              const double params[4], double noise, uint32_t maxbits, uint8_t *out,
                              int threads)
 {
+    const int store_ok = (uint64_t)dims[0] * dims[1] * dims[2] <= (1ull << 31);
     field_t f;
     if (maxbits < 9 || field_setup(&f, kind, seed, dims, count, params, noise))
         return -1;
-    field_minmax(&f, threads);
     const uint32_t nx = dims[0], ny = dims[1], nz = dims[2];
+    /* keep the FP64 field when it fits (saves recomputing it per pass) */
+    double *store = store_ok ? malloc((size_t)nx * ny * nz * sizeof(double)) : NULL;
+    field_minmax(&f, threads, store);
     const uint32_t nbx = (nx + 3) / 4, nby = (ny + 3) / 4, nbz = (nz + 3) / 4;
     const uint64_t bytes = bcmc_synth_stream_bytes(dims, maxbits);
     memset(out, 0, bytes);
@@ -511,22 +523,25 @@ EXPORT int bcmc_synth_stream(int kind, uint64_t seed, const uint32_t dims[3], ui
         const uint32_t z0 = bz * 4, sz = nz - z0 < 4 ? nz - z0 : 4;
 #pragma omp parallel num_threads(nthreads(threads))
         {
-            double *row = malloc(nx * sizeof(double));
+            double *tmp = malloc(nx * sizeof(double));
 #pragma omp for schedule(static)
             for (int64_t zy = 0; zy < (int64_t)ny * sz; ++zy) {
                 const uint32_t y = (uint32_t)(zy % ny), z = z0 + (uint32_t)(zy / ny);
-                field_row(&f, y, z, row);
                 const uint64_t vi = ((uint64_t)z * ny + y) * nx;
+                const double *row = store ? store + vi : tmp;
+                if (!store)
+                    field_row(&f, y, z, tmp);
                 for (uint32_t x = 0; x < nx; ++x)
                     slab[(size_t)zy * nx + x] = (float)field_final(&f, row[x], vi + x);
             }
-            free(row);
+            free(tmp);
         }
         const uint32_t sd[3] = {nx, ny, sz};
         const uint64_t b0 = (uint64_t)bz * nbx * nby;
         encode_range(slab, sd, maxbits, (uint64_t *)out, b0, b0 + (uint64_t)nbx * nby, b0, threads);
     }
     free(slab);
+    free(store);
     field_free(&f);
     return 0;
 }

@@ -1,0 +1,406 @@
+// ZFP fixed-rate float32 3-D block decoding on gfx950.
+//
+// Restates shaders/zfp_decompress.comp (bit reader :188-257, embedded
+// bit-plane decoder :265-294, negabinary + sequency permutation :101-185,
+// 259-263, 373-375, inverse lift :296-354, dequantisation :356-384) as one
+// lane per block.  The fixed-rate words of the 64 blocks of a wave are first
+// copied into LDS with coalesced loads; each lane then decodes its block from
+// LDS.  Group-tested runs of zero bits are consumed with one count-trailing-
+// zeros per newly significant coefficient instead of one bit at a time.
+// Bit offsets are 64-bit (the reference multiplies block_index*max_bits in
+// u32, zfp_decompress.comp:199).
+#include "common.h"
+#include "kernels.h"
+
+#include <algorithm>
+
+namespace bcmc {
+
+// zfp's sequency order (zfp_decompress.comp:101-185): raster index of the
+// i-th decoded coefficient, from "xyz" digit triples.
+static constexpr char kPermXYZ[] =
+    "000100010001011101110200020002111210201021120102012300030003"
+    "211121112022202220310301031130103013122212221311131113320302"
+    "032230203023222321312132231213123033303330322232223133313331"
+    "233323332333";
+
+__host__ __device__ constexpr int perm3d(int i) {
+    return (kPermXYZ[3 * i] - '0') + 4 * (kPermXYZ[3 * i + 1] - '0') +
+           16 * (kPermXYZ[3 * i + 2] - '0');
+}
+
+// LSB-first reader over u64 words (LDS or global).  buf holds nb unread bits;
+// the next word is loaded only when more than nb bits are needed, so a block
+// never reads past its own last bit.
+struct BitReader {
+    const uint64_t *w;
+    uint32_t idx;
+    uint32_t nb;
+    uint64_t buf;
+
+    BCMC_DEV void init(const uint64_t *words, uint32_t first_bit) {
+        w = words;
+        idx = 1;
+        const uint64_t w0 = words[0];
+        buf = first_bit ? (w0 >> first_bit) : w0;
+        nb = 64 - first_bit;
+    }
+    // next n bits (1 <= n <= 64) without consuming them; bits past n are junk
+    BCMC_DEV uint64_t peek(uint32_t n) const {
+        if (n <= nb)
+            return buf;
+        const uint64_t nx = w[idx];
+        return nb ? (buf | (nx << nb)) : nx;
+    }
+    BCMC_DEV void skip(uint32_t n) {  // n <= 64
+        if (n <= nb) {
+            buf = n == 64 ? 0ull : (buf >> n);
+            nb -= n;
+        } else {
+            const uint64_t nx = w[idx++];
+            const uint32_t r = n - nb;
+            buf = r == 64 ? 0ull : (nx >> r);
+            nb = 64 - r;
+        }
+    }
+    BCMC_DEV uint64_t read(uint32_t n) {  // n <= 64
+        if (n == 0)
+            return 0;
+        const uint64_t v = peek(n);
+        skip(n);
+        return n == 64 ? v : (v & ((1ull << n) - 1ull));
+    }
+    BCMC_DEV uint32_t bit() {
+        if (nb == 0) {
+            buf = w[idx++];
+            nb = 64;
+        }
+        const uint32_t b = (uint32_t)buf & 1u;
+        buf >>= 1;
+        --nb;
+        return b;
+    }
+};
+
+// Decode one block into v[64] (raster order); an all-zero block (flag bit 0)
+// decodes to +0.
+BCMC_DEV void decode_block(BitReader &rd, uint32_t maxbits, float (&v)[64]) {
+    if (rd.bit() == 0) {
+#pragma unroll
+        for (int i = 0; i < 64; ++i)
+            v[i] = 0.0f;
+        return;
+    }
+    const int emax = (int)rd.read(8) - 127;
+    uint32_t c[64];
+#pragma unroll
+    for (int i = 0; i < 64; ++i)
+        c[i] = 0;
+    uint32_t bits = maxbits - 9, n = 0;
+    for (uint32_t k = 32; bits != 0 && k-- > 0;) {
+        const uint32_t m = n < bits ? n : bits;
+        bits -= m;
+        uint64_t x = rd.read(m);
+        // group tests + unary runs (zfp_decompress.comp:282-287)
+        while (n < 64 && bits != 0) {
+            --bits;
+            if (rd.bit() == 0)
+                break;
+            const uint32_t lim = min(63u - n, bits);
+            if (lim) {
+                const uint64_t pk = rd.peek(lim) & ((1ull << lim) - 1ull);
+                if (pk) {
+                    const uint32_t p = (uint32_t)__builtin_ctzll(pk);
+                    rd.skip(p + 1);
+                    bits -= p + 1;
+                    n += p;
+                } else {
+                    rd.skip(lim);
+                    bits -= lim;
+                    n += lim;
+                }
+            }
+            x |= 1ull << n;
+            ++n;
+        }
+        // deposit bit plane k (zfp_decompress.comp:290-292); bits >= n are 0
+        if (x) {
+            const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                c[i] += ((lo >> i) & 1u) << k;
+            if (n > 16) {
+#pragma unroll
+                for (int i = 16; i < 32; ++i)
+                    c[i] += ((lo >> i) & 1u) << k;
+            }
+            if (n > 32) {
+#pragma unroll
+                for (int i = 32; i < 48; ++i)
+                    c[i] += ((hi >> (i - 32)) & 1u) << k;
+            }
+            if (n > 48) {
+#pragma unroll
+                for (int i = 48; i < 64; ++i)
+                    c[i] += ((hi >> (i - 32)) & 1u) << k;
+            }
+        }
+    }
+    // negabinary -> two's complement, sequency -> raster (register renaming)
+    int32_t q[64];
+#pragma unroll
+    for (int i = 0; i < 64; ++i)
+        q[perm3d(i)] = (int32_t)((c[i] ^ 0xaaaaaaaau) - 0xaaaaaaaau);
+    // inverse lifts along z, y, x (zfp_decompress.comp:296-354), wrapping int32
+    auto lift = [&](int i0, int s) {
+        uint32_t x = (uint32_t)q[i0], y = (uint32_t)q[i0 + s], z = (uint32_t)q[i0 + 2 * s],
+                 w = (uint32_t)q[i0 + 3 * s];
+        y += (uint32_t)((int32_t)w >> 1);
+        w -= (uint32_t)((int32_t)y >> 1);
+        y += w;
+        w <<= 1;
+        w -= y;
+        z += x;
+        x <<= 1;
+        x -= z;
+        y += z;
+        z <<= 1;
+        z -= y;
+        w += x;
+        x <<= 1;
+        x -= w;
+        q[i0] = (int32_t)x;
+        q[i0 + s] = (int32_t)y;
+        q[i0 + 2 * s] = (int32_t)z;
+        q[i0 + 3 * s] = (int32_t)w;
+    };
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+            lift(x + 4 * y, 16);
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int z = 0; z < 4; ++z)
+            lift(16 * z + x, 4);
+#pragma unroll
+    for (int z = 0; z < 4; ++z)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+            lift(4 * y + 16 * z, 1);
+    const float inv_w = __builtin_ldexpf(1.0f, emax - 30);
+#pragma unroll
+    for (int i = 0; i < 64; ++i)
+        v[i] = inv_w * (float)q[i];
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+constexpr int kZfpThreads = 256;  // 4 waves = 256 blocks per workgroup pass
+
+enum ZfpMode { kModeRanges = 0, kModeCache = 1, kModeDense = 2 };
+
+struct ZfpArgs {
+    const uint64_t *stream;
+    uint64_t nwords;
+    uint32_t maxbits;
+    const uint32_t *ids;       // block ids (cache/dense); null: identity
+    const uint32_t *n_dev;     // device count (nullable)
+    uint32_t n_host;
+    const int32_t *item_slot;  // cache mode: destination slot per block id
+    uint32_t capacity;         // cache mode: slots in `out` (others are skipped)
+    float *out;                // cache / dense output
+    float2 *ranges;            // ranges mode
+    uint32_t dims[3], nb[3];
+};
+
+// W = u64 words per block when 64 | maxbits (LDS-staged), 0 = generic.
+template <int W, int MODE>
+__global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
+    constexpr int kStage = W > 0 ? 64 * W + 1 : 1;
+    __shared__ uint64_t stage[kZfpThreads / 64][kStage];
+    const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
+    const uint32_t n = a.n_dev ? *a.n_dev : a.n_host;
+    const uint32_t waves_total = gridDim.x * (kZfpThreads / 64);
+    for (uint32_t chunk = blockIdx.x * (kZfpThreads / 64) + wave; (uint64_t)chunk * 64 < n;
+         chunk += waves_total) {
+        const uint32_t i = chunk * 64 + lane;
+        const bool live = i < n;
+        const uint32_t blk = live ? (a.ids ? a.ids[i] : i) : 0u;
+        BitReader rd;
+        if constexpr (W > 0) {
+            uint64_t *sw = stage[wave];
+            // coalesced copy of the wave's fixed-rate words into LDS
+            for (uint32_t j = lane; j < 64u * W; j += 64) {
+                const uint32_t e = chunk * 64 + j / W;
+                uint64_t word = 0;
+                if (e < n) {
+                    const uint64_t b = a.ids ? a.ids[e] : e;
+                    const uint64_t wi = b * W + j % W;
+                    word = wi < a.nwords ? a.stream[wi] : 0ull;
+                }
+                sw[j] = word;
+            }
+            if (lane == 0)
+                sw[64 * W] = 0;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            rd.init(sw + lane * W, 0);
+        } else {
+            const uint64_t bit0 = (uint64_t)blk * a.maxbits;
+            rd.init(a.stream + (bit0 >> 6), (uint32_t)(bit0 & 63));
+        }
+        float v[64];
+        if (live)
+            decode_block(rd, a.maxbits, v);
+        if constexpr (W > 0)
+            __builtin_amdgcn_wave_barrier();
+        if (!live)
+            continue;
+        if constexpr (MODE == kModeRanges) {
+            // zfp_compute_block_range.comp:42-60
+            const uint32_t bx = (blk % a.nb[0]) * 4, by = ((blk / a.nb[0]) % a.nb[1]) * 4,
+                           bz = (blk / (a.nb[0] * a.nb[1])) * 4;
+            const uint32_t sx = bx + 4 > a.dims[0] ? a.dims[0] - bx : 4;
+            const uint32_t sy = by + 4 > a.dims[1] ? a.dims[1] - by : 4;
+            const uint32_t sz = bz + 4 > a.dims[2] ? a.dims[2] - bz : 4;
+            float lo = 1e20f, hi = -1e20f;
+#pragma unroll
+            for (int j = 0; j < 64; ++j) {
+                const bool in = (uint32_t)(j & 3) < sx && (uint32_t)((j >> 2) & 3) < sy &&
+                                (uint32_t)(j >> 4) < sz;
+                const float x = v[j];
+                lo = (in && x < lo) ? x : lo;
+                hi = (in && x > hi) ? x : hi;
+            }
+            a.ranges[blk] = make_float2(lo, hi);
+        } else {
+            uint64_t dst = (uint64_t)i * 64;
+            if constexpr (MODE == kModeCache) {
+                const int32_t slot = a.item_slot[blk];
+                if (slot < 0 || (uint32_t)slot >= a.capacity)
+                    continue;
+                dst = (uint64_t)(uint32_t)slot * 64;
+            }
+            float4 *o = reinterpret_cast<float4 *>(a.out + dst);
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                o[j] = make_float4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]);
+        }
+    }
+}
+
+template <int MODE>
+static void launch_mode(const ZfpArgs &a, uint32_t grid, hipStream_t s) {
+    const uint32_t W = a.maxbits % 64 == 0 ? a.maxbits / 64 : 0;
+    switch (W) {
+    case 1: zfp_kernel<1, MODE><<<grid, kZfpThreads, 0, s>>>(a); break;
+    case 2: zfp_kernel<2, MODE><<<grid, kZfpThreads, 0, s>>>(a); break;
+    case 4: zfp_kernel<4, MODE><<<grid, kZfpThreads, 0, s>>>(a); break;
+    case 8: zfp_kernel<8, MODE><<<grid, kZfpThreads, 0, s>>>(a); break;
+    case 16: zfp_kernel<16, MODE><<<grid, kZfpThreads, 0, s>>>(a); break;
+    default: zfp_kernel<0, MODE><<<grid, kZfpThreads, 0, s>>>(a); break;
+    }
+}
+
+static ZfpArgs base_args(const uint64_t *stream, uint64_t nwords, uint32_t maxbits) {
+    ZfpArgs a{};
+    a.stream = stream;
+    a.nwords = nwords;
+    a.maxbits = maxbits;
+    return a;
+}
+
+void launch_block_ranges(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords, uint32_t maxbits,
+                         const uint32_t dims[3], float2 *ranges) {
+    ZfpArgs a = base_args(stream, nwords, maxbits);
+    for (int i = 0; i < 3; ++i) {
+        a.dims[i] = dims[i];
+        a.nb[i] = (dims[i] + 3) / 4;
+    }
+    a.n_host = a.nb[0] * a.nb[1] * a.nb[2];
+    a.ranges = ranges;
+    const uint32_t grid = div_up(a.n_host, kZfpThreads);
+    if (grid)
+        launch_mode<kModeRanges>(a, grid, ctx->stream);
+}
+
+void launch_decode_to_cache(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords,
+                            uint32_t maxbits, const uint32_t *ids, const uint32_t *n_dev,
+                            uint32_t n_max, const int32_t *item_slot, float *cache,
+                            uint32_t capacity) {
+    ZfpArgs a = base_args(stream, nwords, maxbits);
+    a.ids = ids;
+    a.n_dev = n_dev;
+    a.n_host = n_max;
+    a.item_slot = item_slot;
+    a.capacity = capacity;
+    a.out = cache;
+    const uint32_t grid = std::min(div_up(n_max, kZfpThreads), persistent_grid(ctx, 8));
+    if (grid)
+        launch_mode<kModeCache>(a, grid, ctx->stream);
+}
+
+void launch_decode_dense(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords, uint32_t maxbits,
+                         const uint32_t *ids, uint32_t n, float *out) {
+    ZfpArgs a = base_args(stream, nwords, maxbits);
+    a.ids = ids;
+    a.n_host = n;
+    a.out = out;
+    const uint32_t grid = div_up(n, kZfpThreads);
+    if (grid)
+        launch_mode<kModeDense>(a, grid, ctx->stream);
+}
+
+}  // namespace bcmc
+
+using namespace bcmc;
+
+// With timing enabled, the device time of the launch just issued is kept in
+// ctx->last_kernel_ms (HIP events on the context stream; waits for it).
+static int finish_timing(bcmc_ctx *ctx) {
+    if (!ctx->timing)
+        return BCMC_OK;
+    BCMC_HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));
+    BCMC_HIP_CHECK(hipEventSynchronize(ctx->ev1));
+    float ms = 0.0f;
+    BCMC_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_kernel_ms = ms;
+    return BCMC_OK;
+}
+
+extern "C" BCMC_API int bcmc_zfp_decode_blocks(bcmc_ctx *ctx, bcmc_buf *stream, uint32_t max_bits,
+                                               bcmc_buf *ids, uint32_t n, bcmc_buf *out) {
+    BCMC_CHECK_ARG(ctx && stream && out, "null argument");
+    BCMC_CHECK_ARG(max_bits >= 9, "max_bits must be >= 9");
+    BCMC_CHECK_ARG(out->bytes >= (uint64_t)n * 256, "output buffer too small");
+    BCMC_CHECK_ARG(!ids || ids->bytes >= (uint64_t)n * 4, "ids buffer too small");
+    // every block read must lie inside the stream (plus the 8-byte prefetch pad)
+    if (!ids)
+        BCMC_CHECK_ARG((uint64_t)n * max_bits <= stream->bytes * 8, "stream too short for n blocks");
+    if (ctx->timing)
+        BCMC_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
+    launch_decode_dense(ctx, (const uint64_t *)stream->ptr, stream->bytes / 8, max_bits,
+                        ids ? (const uint32_t *)ids->ptr : nullptr, n, (float *)out->ptr);
+    BCMC_HIP_CHECK(hipGetLastError());
+    return finish_timing(ctx);
+}
+
+extern "C" BCMC_API int bcmc_zfp_block_ranges(bcmc_ctx *ctx, bcmc_buf *stream, uint32_t max_bits,
+                                              const uint32_t dims[3], bcmc_buf *ranges) {
+    BCMC_CHECK_ARG(ctx && stream && ranges && dims, "null argument");
+    BCMC_CHECK_ARG(max_bits >= 9, "max_bits must be >= 9");
+    const uint64_t nb = (uint64_t)((dims[0] + 3) / 4) * ((dims[1] + 3) / 4) * ((dims[2] + 3) / 4);
+    BCMC_CHECK_ARG(nb > 0 && nb < (1ull << 32), "bad dims");
+    BCMC_CHECK_ARG(ranges->bytes >= nb * 8, "ranges buffer too small");
+    BCMC_CHECK_ARG(nb * max_bits <= stream->bytes * 8, "stream too short for the volume");
+    if (ctx->timing)
+        BCMC_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
+    launch_block_ranges(ctx, (const uint64_t *)stream->ptr, stream->bytes / 8, max_bits, dims,
+                        (float2 *)ranges->ptr);
+    BCMC_HIP_CHECK(hipGetLastError());
+    return finish_timing(ctx);
+}

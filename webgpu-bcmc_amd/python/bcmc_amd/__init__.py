@@ -1,0 +1,469 @@
+"""Python host side of the MI355X block-compressed marching-cubes library.
+
+Mirrors the reference's JS classes (Twinklebear/webgpu-bcmc js/*.js) over the
+C ABI of lib/libbcmc_amd.so (include/bcmc.h) with ctypes:
+
+  Device                   navigator.gpu device + queue      (js/bcmc.js:1-20)
+  Buffer                   GPUBuffer
+  ExclusiveScanPipeline    js/exclusive_scan.js:14-122
+  ExclusiveScanner         js/exclusive_scan.js:124-295
+  StreamCompact            js/stream_compact.js (alias StreamCompactor)
+  RadixSorter              js/radix_sort_by_key.js
+  LRUCache                 js/lru_cache.js
+  CompressedMarchingCubes  js/compressed_marching_cubes.js
+
+Every call goes to the HIP library; there is no CPU fallback.  Importing the
+module without the built library raises, and creating a Device without a
+gfx950 GPU raises BcmcError.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+LIB_PATH = os.path.join(_ROOT, "lib", "libbcmc_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_ROOT), "include", "bcmc.h")
+
+VERTS_LOCAL_F32, VERTS_GLOBAL_F32, VERTS_PACKED_U32X2 = 0, 1, 2
+ARRAY_RANGES, ARRAY_ACTIVE_FLAGS, ARRAY_ACTIVE_IDS, ARRAY_NEW_ITEMS, ARRAY_WITH_VERTS, \
+    ARRAY_VERT_OFFSETS = range(6)
+STATUS = {0: "OK", -1: "INVALID", -2: "HIP", -3: "NOMEM", -4: "OVERFLOW", -5: "STATE",
+          -6: "NO_DEVICE"}
+
+
+class BcmcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"bcmc error {code} ({STATUS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class _Perf(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in (
+        "computeActiveBlocks", "cacheUpdate", "decompression", "compactActiveIDs",
+        "computeBlockHasVertices", "compactBlocksWithVertices", "computeVertices", "total_ms")] + [
+        ("numActiveBlocks", ctypes.c_uint32), ("numBlocksDecompressed", ctypes.c_uint32),
+        ("numBlocksWithVertices", ctypes.c_uint32), ("cacheSize", ctypes.c_uint32),
+        ("numVertices", ctypes.c_uint64)]
+
+
+class _LruInfo(ctypes.Structure):
+    _fields_ = [("cache_size", ctypes.c_uint32), ("total_elements", ctypes.c_uint32),
+                ("element_size", ctypes.c_uint32), ("slots_available", ctypes.c_uint32),
+                ("reserved_bytes", ctypes.c_uint64)]
+
+
+class _McInfo(ctypes.Structure):
+    _fields_ = [("total_blocks", ctypes.c_uint32), ("num_active_blocks", ctypes.c_uint32),
+                ("num_blocks_with_vertices", ctypes.c_uint32), ("new_decompressed", ctypes.c_uint32),
+                ("num_vertices", ctypes.c_uint64), ("vertex_capacity", ctypes.c_uint64),
+                ("dims", ctypes.c_uint32 * 3), ("padded_dims", ctypes.c_uint32 * 3),
+                ("max_bits", ctypes.c_uint32), ("scale", ctypes.c_float * 3),
+                ("format", ctypes.c_int)]
+
+
+_lib = None
+
+
+def exported_symbols():
+    """Names declared BCMC_API in include/bcmc.h."""
+    import re
+    txt = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"BCMC_API\s+[\w\s\*]+?\b(bcmc_\w+)\s*\(", txt)))
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+    P = ctypes.POINTER
+    sig = {
+        "bcmc_last_error": (ctypes.c_char_p, []),
+        "bcmc_version": (ctypes.c_char_p, []),
+        "bcmc_ctx_create": (i32, [i32, P(vp)]),
+        "bcmc_ctx_destroy": (i32, [vp]),
+        "bcmc_ctx_sync": (i32, [vp]),
+        "bcmc_ctx_info": (i32, [vp, P(i32), ctypes.c_char_p, ctypes.c_size_t]),
+        "bcmc_ctx_enable_timing": (i32, [vp, i32]),
+        "bcmc_ctx_last_kernel_ms": (ctypes.c_double, [vp]),
+        "bcmc_buf_alloc": (i32, [vp, u64, P(vp)]),
+        "bcmc_buf_wrap": (i32, [vp, vp, u64, P(vp)]),
+        "bcmc_buf_free": (i32, [vp]),
+        "bcmc_buf_upload": (i32, [vp, u64, vp, u64]),
+        "bcmc_buf_download": (i32, [vp, u64, vp, u64]),
+        "bcmc_buf_fill_u32": (i32, [vp, u64, u32, u64]),
+        "bcmc_buf_size": (u64, [vp]),
+        "bcmc_buf_device_ptr": (vp, [vp]),
+        "bcmc_scan_aligned_size": (u32, [u32]),
+        "bcmc_scan_exclusive_u32": (i32, [vp, vp, u32, u32, P(u32)]),
+        "bcmc_compact_ids": (i32, [vp, u32, vp, vp, vp]),
+        "bcmc_compact_flags": (i32, [vp, u32, vp, vp, P(u32)]),
+        "bcmc_sort_aligned_size": (u32, [u32]),
+        "bcmc_sort_u32_by_key": (i32, [vp, vp, vp, u32, i32]),
+        "bcmc_lru_create": (i32, [vp, u32, u32, u32, P(vp)]),
+        "bcmc_lru_destroy": (i32, [vp]),
+        "bcmc_lru_update": (i32, [vp, vp, P(u32), P(vp)]),
+        "bcmc_lru_reset": (i32, [vp]),
+        "bcmc_lru_get_info": (i32, [vp, P(_LruInfo)]),
+        "bcmc_lru_read_state": (i32, [vp, vp, vp, vp, vp]),
+        "bcmc_lru_cache_buffer": (vp, [vp]),
+        "bcmc_mc_create": (i32, [vp, P(vp)]),
+        "bcmc_mc_destroy": (i32, [vp]),
+        "bcmc_mc_set_compressed_volume": (i32, [vp, vp, u64, ctypes.c_double, P(u32), P(ctypes.c_float), u32]),
+        "bcmc_mc_set_vertex_format": (i32, [vp, i32]),
+        "bcmc_mc_compute_surface": (i32, [vp, ctypes.c_float, P(_Perf), P(u64)]),
+        "bcmc_mc_get_info": (i32, [vp, P(_McInfo)]),
+        "bcmc_mc_vertex_buffer": (vp, [vp]),
+        "bcmc_mc_lru": (vp, [vp]),
+        "bcmc_mc_array_buffer": (vp, [vp, i32]),
+        "bcmc_mc_memory_use": (i32, [vp, P(u64), P(u64)]),
+        "bcmc_zfp_decode_blocks": (i32, [vp, vp, u32, vp, u32, vp]),
+        "bcmc_zfp_block_ranges": (i32, [vp, vp, u32, P(u32), vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise BcmcError(rc, lib().bcmc_last_error().decode(errors="replace"))
+
+
+def _u32arr(vals):
+    return (ctypes.c_uint32 * 3)(*[int(v) for v in vals])
+
+
+class Device:
+    """One GPU and its in-order queue (a bcmc_ctx)."""
+
+    def __init__(self, index=0):
+        h = ctypes.c_void_p()
+        _check(lib().bcmc_ctx_create(index, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bcmc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def sync(self):
+        _check(lib().bcmc_ctx_sync(self.h))
+
+    def info(self):
+        n = ctypes.c_int()
+        name = ctypes.create_string_buffer(128)
+        _check(lib().bcmc_ctx_info(self.h, ctypes.byref(n), name, 128))
+        return {"num_cus": n.value, "name": name.value.decode()}
+
+    def create_buffer(self, nbytes):
+        return Buffer.alloc(self, nbytes)
+
+    def buffer_from(self, array):
+        a = np.ascontiguousarray(array)
+        b = Buffer.alloc(self, max(a.nbytes, 4))
+        b.upload(a)
+        return b
+
+
+class Buffer:
+    """A device allocation (GPUBuffer)."""
+
+    def __init__(self, device, handle, owned=True):
+        self.device, self.h, self.owned = device, handle, owned
+
+    @classmethod
+    def alloc(cls, device, nbytes):
+        h = ctypes.c_void_p()
+        _check(lib().bcmc_buf_alloc(device.h, int(nbytes), ctypes.byref(h)))
+        return cls(device, h)
+
+    @property
+    def nbytes(self):
+        return int(lib().bcmc_buf_size(self.h))
+
+    def upload(self, array, offset=0):
+        a = np.ascontiguousarray(array)
+        _check(lib().bcmc_buf_upload(self.h, offset, a.ctypes.data, a.nbytes))
+
+    def download(self, dtype=np.uint8, count=None, offset=0):
+        dt = np.dtype(dtype)
+        if count is None:
+            count = (self.nbytes - offset) // dt.itemsize
+        out = np.empty(int(count), dt)
+        _check(lib().bcmc_buf_download(self.h, offset, out.ctypes.data, out.nbytes))
+        return out
+
+    def fill_u32(self, value, count, offset=0):
+        _check(lib().bcmc_buf_fill_u32(self.h, offset, value, count))
+
+    def destroy(self):
+        if self.owned and getattr(self, "h", None):
+            lib().bcmc_buf_free(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def _view(device, handle):
+    return Buffer(device, ctypes.c_void_p(handle), owned=False) if handle else None
+
+
+# ---------------------------------------------------------------------------
+# Primitives
+# ---------------------------------------------------------------------------
+class ExclusiveScanPipeline:
+    """js/exclusive_scan.js:14-122."""
+
+    def __init__(self, device):
+        self.device = device
+
+    def getAlignedSize(self, size):
+        return int(lib().bcmc_scan_aligned_size(size))
+
+    def prepareInput(self, cpu_array):
+        a = np.asarray(cpu_array, np.uint32)
+        aligned = self.getAlignedSize(a.size)
+        buf = Buffer.alloc(self.device, max(aligned, 1) * 4)
+        padded = np.zeros(aligned, np.uint32)
+        padded[:a.size] = a
+        buf.upload(padded)
+        return ExclusiveScanner(self, buf, aligned)
+
+    def prepareGPUInput(self, gpu_buffer, aligned_size, data_size=None):
+        if self.getAlignedSize(aligned_size) != aligned_size:
+            raise BcmcError(-1, "GPU input must be aligned to getAlignedSize")
+        return ExclusiveScanner(self, gpu_buffer, aligned_size)
+
+
+class ExclusiveScanner:
+    """js/exclusive_scan.js:124-295: scan(n) -> total."""
+
+    def __init__(self, pipeline, gpu_buffer, aligned_size):
+        self.pipeline, self.inputBuf, self.inputSize = pipeline, gpu_buffer, aligned_size
+
+    def scan(self, data_size):
+        total = ctypes.c_uint32()
+        _check(lib().bcmc_scan_exclusive_u32(self.pipeline.device.h, self.inputBuf.h,
+                                             self.inputSize, data_size, ctypes.byref(total)))
+        return total.value
+
+
+class StreamCompact:
+    """js/stream_compact.js:1-165."""
+
+    def __init__(self, device):
+        self.device = device
+
+    def compactActiveIDs(self, num_elements, is_active, offsets, output):
+        _check(lib().bcmc_compact_ids(self.device.h, num_elements, is_active.h, offsets.h, output.h))
+
+    def compactFlags(self, num_elements, flags, output):
+        """Fused single-pass scan + compaction; returns the count."""
+        n = ctypes.c_uint32()
+        _check(lib().bcmc_compact_flags(self.device.h, num_elements, flags.h, output.h,
+                                        ctypes.byref(n)))
+        return n.value
+
+
+StreamCompactor = StreamCompact
+
+
+class RadixSorter:
+    """js/radix_sort_by_key.js."""
+
+    def __init__(self, device):
+        self.device = device
+
+    def getAlignedSize(self, size):
+        return int(lib().bcmc_sort_aligned_size(size))
+
+    def sort(self, keys, values, size, reverse=False):
+        _check(lib().bcmc_sort_u32_by_key(self.device.h, keys.h, values.h, size, 1 if reverse else 0))
+
+
+class LRUCache:
+    """js/lru_cache.js: LRUCache(device, scan, compact, initialSize, elementSize, totalElements)."""
+
+    def __init__(self, device, scan_pipeline, stream_compact, initial_size, element_size,
+                 total_elements, _handle=None):
+        self.device = device
+        if _handle is not None:
+            self.h, self._owned = _handle, False
+        else:
+            h = ctypes.c_void_p()
+            _check(lib().bcmc_lru_create(device.h, initial_size, element_size, total_elements,
+                                         ctypes.byref(h)))
+            self.h, self._owned = h, True
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and getattr(self, "h", None):
+            lib().bcmc_lru_destroy(self.h)
+            self.h = None
+
+    def update(self, item_needed, perf_tracker=None):
+        n = ctypes.c_uint32()
+        ids = ctypes.c_void_p()
+        _check(lib().bcmc_lru_update(self.h, item_needed.h, ctypes.byref(n), ctypes.byref(ids)))
+        return n.value, (Buffer(self.device, ids) if ids.value else None)
+
+    def reset(self):
+        _check(lib().bcmc_lru_reset(self.h))
+
+    def info(self):
+        i = _LruInfo()
+        _check(lib().bcmc_lru_get_info(self.h, ctypes.byref(i)))
+        return {f: getattr(i, f) for f, _ in _LruInfo._fields_}
+
+    @property
+    def cacheSize(self):
+        return self.info()["cache_size"]
+
+    def state(self):
+        info = self.info()
+        c, t = info["cache_size"], info["total_elements"]
+        age = np.zeros(max(c, 1), np.uint32)
+        avail = np.zeros(max(c, 1), np.uint32)
+        item = np.zeros(max(c, 1), np.int32)
+        slot = np.zeros(max(t, 1), np.int32)
+        _check(lib().bcmc_lru_read_state(self.h, age.ctypes.data, avail.ctypes.data,
+                                         item.ctypes.data, slot.ctypes.data))
+        return age[:c], avail[:c], item[:c], slot[:t]
+
+    @property
+    def cache(self):
+        return _view(self.device, lib().bcmc_lru_cache_buffer(self.h))
+
+
+class CompressedMarchingCubes:
+    """js/compressed_marching_cubes.js."""
+
+    PERF_KEYS = ("computeActiveBlocks", "numActiveBlocks", "cacheUpdate", "numBlocksDecompressed",
+                 "decompression", "compactActiveIDs", "computeBlockHasVertices",
+                 "numBlocksWithVertices", "compactBlocksWithVertices", "numVertices",
+                 "computeVertices")
+
+    def __init__(self, device):
+        self.device = device
+        h = ctypes.c_void_p()
+        _check(lib().bcmc_mc_create(device.h, ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().bcmc_mc_destroy(self.h)
+            self.h = None
+
+    def setCompressedVolume(self, volume, compression_rate, volume_dims, volume_scale=(1, 1, 1),
+                            initial_cache_slots=0):
+        v = np.ascontiguousarray(np.frombuffer(memoryview(volume), np.uint8))
+        scale = (ctypes.c_float * 3)(*[float(s) for s in volume_scale])
+        _check(lib().bcmc_mc_set_compressed_volume(self.h, v.ctypes.data, v.nbytes,
+                                                   float(compression_rate), _u32arr(volume_dims),
+                                                   scale, int(initial_cache_slots)))
+
+    def setVertexFormat(self, fmt):
+        _check(lib().bcmc_mc_set_vertex_format(self.h, fmt))
+
+    def computeSurface(self, isovalue, perf_tracker=None):
+        """Returns the vertex count; fills perf_tracker (dict of lists) like the reference."""
+        perf = _Perf()
+        nv = ctypes.c_uint64()
+        _check(lib().bcmc_mc_compute_surface(self.h, float(isovalue), ctypes.byref(perf),
+                                             ctypes.byref(nv)))
+        self.lastPerf = {f: getattr(perf, f) for f, _ in _Perf._fields_}
+        if perf_tracker is not None:
+            if "computeActiveBlocks" not in perf_tracker:
+                perf_tracker["totalBlocks"] = self.info()["total_blocks"]
+                for k in self.PERF_KEYS:
+                    perf_tracker[k] = []
+            for k in self.PERF_KEYS:
+                perf_tracker[k].append(self.lastPerf[k])
+        return int(nv.value)
+
+    def info(self):
+        i = _McInfo()
+        _check(lib().bcmc_mc_get_info(self.h, ctypes.byref(i)))
+        d = {f: getattr(i, f) for f, _ in _McInfo._fields_}
+        for k in ("dims", "padded_dims", "scale"):
+            d[k] = list(d[k])
+        return d
+
+    @property
+    def numVertices(self):
+        return self.info()["num_vertices"]
+
+    @property
+    def numActiveBlocks(self):
+        return self.info()["num_active_blocks"]
+
+    @property
+    def numBlocksWithVertices(self):
+        return self.info()["num_blocks_with_vertices"]
+
+    @property
+    def totalBlocks(self):
+        return self.info()["total_blocks"]
+
+    @property
+    def newDecompressed(self):
+        return self.info()["new_decompressed"]
+
+    @property
+    def vertexBuffer(self):
+        return _view(self.device, lib().bcmc_mc_vertex_buffer(self.h))
+
+    @property
+    def lruCache(self):
+        h = lib().bcmc_mc_lru(self.h)
+        return LRUCache(self.device, None, None, 0, 0, 0, _handle=ctypes.c_void_p(h)) if h else None
+
+    def array(self, which, dtype=np.uint32):
+        b = _view(self.device, lib().bcmc_mc_array_buffer(self.h, which))
+        return b.download(dtype) if b is not None and b.nbytes else np.zeros(0, dtype)
+
+    def vertices(self):
+        info = self.info()
+        n, fmt = info["num_vertices"], info["format"]
+        b = self.vertexBuffer
+        if fmt == VERTS_LOCAL_F32:
+            return b.download(np.float32, 4 * n).reshape(-1, 4)
+        if fmt == VERTS_GLOBAL_F32:
+            return b.download(np.float32, 3 * n).reshape(-1, 3)
+        return b.download(np.uint32, 2 * n).reshape(-1, 2)
+
+    def reportMemoryUse(self):
+        t, c = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().bcmc_mc_memory_use(self.h, ctypes.byref(t), ctypes.byref(c)))
+        return {"total": t.value, "cache": c.value}
+
+
+# ---------------------------------------------------------------------------
+# Standalone decode helpers
+# ---------------------------------------------------------------------------
+def decode_blocks(device, stream_buf, max_bits, n, ids_buf=None):
+    out = Buffer.alloc(device, max(n, 1) * 256)
+    _check(lib().bcmc_zfp_decode_blocks(device.h, stream_buf.h, max_bits,
+                                        ids_buf.h if ids_buf is not None else None, n, out.h))
+    return out
+
+
+def block_ranges(device, stream_buf, max_bits, dims):
+    nb = int(np.prod([(d + 3) // 4 for d in dims]))
+    out = Buffer.alloc(device, nb * 8)
+    _check(lib().bcmc_zfp_block_ranges(device.h, stream_buf.h, max_bits, _u32arr(dims), out.h))
+    return out
