@@ -1,0 +1,57 @@
+"""The Node.js facade (the drop-in for the reference's JS classes) on the GPU,
+compared with the oracle: scan, compaction, sort, and a computeSurface sweep
+with cache slots, counts and vertices bit-exact."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from bcmc_amd import synth
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ADDON = os.path.join(ROOT, "webgpu-bcmc_amd", "lib", "bcmc_addon.node")
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+def test_facade_sweep(tmp_path):
+    assert os.path.exists(ADDON), "N-API addon not built"
+    dims, rate, cache = (40, 32, 36), 4, 96
+    f = synth.field(synth.GAUSS, 17, dims, 8, (3, 10, 0.5, 1))
+    s = synth.encode(f, 64 * rate)
+    rng = np.random.default_rng(5)
+    params = {"dims": dims, "rate": rate, "cache": cache, "isos": [0.3, 0.5, 0.42],
+              "scan_input": rng.integers(0, 100, 3000).tolist(),
+              "flags": (rng.uniform(0, 1, 5000) < 0.3).astype(int).tolist(),
+              "sort_keys": rng.integers(0, 6, 300).tolist()}
+    (tmp_path / "params.json").write_text(json.dumps(params))
+    (tmp_path / "stream.bin").write_bytes(s.tobytes())
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "facade_parity.js"), str(tmp_path)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads((tmp_path / "out.json").read_text())
+    want, total = oracle.exclusive_scan(np.array(params["scan_input"], np.uint32))
+    assert out["scan_total"] == total and out["scan"] == want.tolist()
+    assert out["compact"] == np.flatnonzero(params["flags"]).tolist()
+    keys = np.array(params["sort_keys"], np.uint32)
+    sv, _ = oracle.sort_by_key(keys, np.arange(len(keys), dtype=np.uint32), True)
+    assert out["sorted"] == sv.tolist()
+    ref = oracle.MC(s, 64 * rate, dims, initial_cache=cache)
+    for i, iso in enumerate(params["isos"]):
+        nv = ref.compute_surface(iso)
+        c = out["counts"][i]
+        assert c["numVertices"] == nv
+        assert c["numActiveBlocks"] == ref.stats["num_active"]
+        assert c["numBlocksWithVertices"] == ref.stats["num_with_verts"]
+        assert c["newDecompressed"] == ref.stats["num_decoded"]
+        assert c["cacheSize"] == ref.stats["cache_size"]
+        v = np.fromfile(tmp_path / f"verts_{i}.bin", np.float32).reshape(-1, 4)
+        assert np.array_equal(v.view(np.uint32), ref.results()["verts_f32"].view(np.uint32))
+        slots = np.fromfile(tmp_path / f"slots_{i}.bin", np.int32)
+        assert np.array_equal(slots, ref.lru_state()[3])
+    assert out["perfLen"] == 3 and "computeVertices" in out["perfKeys"]
+    assert out["after_reset_age0"] == 100000

@@ -1,0 +1,524 @@
+/* Node.js N-API addon over the C ABI in include/bcmc.h.
+ *
+ * Thin by design: each function converts JS values to plain C arguments, calls
+ * one bcmc_* entry point and throws a JS Error carrying bcmc_last_error() on a
+ * negative status.  Handles cross as N-API externals.  The JS facade
+ * (index.js) builds the reference's classes (CompressedMarchingCubes,
+ * LRUCache, ExclusiveScanPipeline/ExclusiveScanner, StreamCompact(or),
+ * RadixSorter) on top of these functions.
+ */
+#define NAPI_VERSION 6
+#include <node_api.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "bcmc.h"
+
+#define NAPI_CALL(env, call)                                                                    \
+    do {                                                                                        \
+        if ((call) != napi_ok) {                                                                \
+            napi_throw_error((env), NULL, "N-API call failed: " #call);                          \
+            return NULL;                                                                        \
+        }                                                                                       \
+    } while (0)
+
+#define BCMC_CALL(env, call)                                                                    \
+    do {                                                                                        \
+        int rc_ = (call);                                                                       \
+        if (rc_ != BCMC_OK) {                                                                   \
+            char msg_[640];                                                                     \
+            snprintf(msg_, sizeof(msg_), "bcmc error %d: %s", rc_, bcmc_last_error());          \
+            napi_throw_error((env), NULL, msg_);                                                \
+            return NULL;                                                                        \
+        }                                                                                       \
+    } while (0)
+
+#define GET_ARGS(n)                                                                             \
+    size_t argc = (n);                                                                          \
+    napi_value argv[(n) > 0 ? (n) : 1];                                                         \
+    NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL))
+
+static void *ext(napi_env env, napi_value v)
+{
+    void *p = NULL;
+    napi_valuetype t;
+    if (napi_typeof(env, v, &t) != napi_ok || t != napi_external)
+        return NULL;
+    napi_get_value_external(env, v, &p);
+    return p;
+}
+
+static napi_value mk_ext(napi_env env, void *p)
+{
+    napi_value r;
+    if (!p) {
+        napi_get_null(env, &r);
+        return r;
+    }
+    napi_create_external(env, p, NULL, NULL, &r);
+    return r;
+}
+
+static double num(napi_env env, napi_value v)
+{
+    double d = 0;
+    napi_get_value_double(env, v, &d);
+    return d;
+}
+
+static napi_value mk_num(napi_env env, double d)
+{
+    napi_value r;
+    napi_create_double(env, d, &r);
+    return r;
+}
+
+static void set_num(napi_env env, napi_value obj, const char *k, double d)
+{
+    napi_set_named_property(env, obj, k, mk_num(env, d));
+}
+
+/* typed array / buffer data and byte length */
+static int bytes_of(napi_env env, napi_value v, void **data, size_t *len)
+{
+    bool is_ta = false, is_buf = false;
+    napi_is_typedarray(env, v, &is_ta);
+    if (is_ta) {
+        napi_typedarray_type t;
+        size_t n, off;
+        napi_value ab;
+        void *d;
+        if (napi_get_typedarray_info(env, v, &t, &n, &d, &ab, &off) != napi_ok)
+            return -1;
+        size_t es = 1;
+        switch (t) {
+        case napi_int16_array: case napi_uint16_array: es = 2; break;
+        case napi_int32_array: case napi_uint32_array: case napi_float32_array: es = 4; break;
+        case napi_float64_array: case napi_bigint64_array: case napi_biguint64_array: es = 8; break;
+        default: es = 1;
+        }
+        *data = d;
+        *len = n * es;
+        return 0;
+    }
+    napi_is_buffer(env, v, &is_buf);
+    if (is_buf)
+        return napi_get_buffer_info(env, v, data, len) == napi_ok ? 0 : -1;
+    return -1;
+}
+
+static void *ext_arg(napi_env env, napi_value v, const char *what)
+{
+    void *p = ext(env, v);
+    if (!p) {
+        char m[128];
+        snprintf(m, sizeof(m), "expected a %s handle", what);
+        napi_throw_type_error(env, NULL, m);
+    }
+    return p;
+}
+
+#define EXT(i, T, what)                                                                         \
+    T *what = (T *)ext_arg(env, argv[i], #what);                                                \
+    if (!what)                                                                                  \
+        return NULL
+
+/* ---- context & buffers ---------------------------------------------------- */
+static napi_value ctx_create(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    bcmc_ctx *c = NULL;
+    BCMC_CALL(env, bcmc_ctx_create(argc > 0 ? (int)num(env, argv[0]) : 0, &c));
+    return mk_ext(env, c);
+}
+
+static napi_value ctx_destroy(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_ctx, ctx);
+    BCMC_CALL(env, bcmc_ctx_destroy(ctx));
+    return NULL;
+}
+
+static napi_value ctx_sync(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_ctx, ctx);
+    BCMC_CALL(env, bcmc_ctx_sync(ctx));
+    return NULL;
+}
+
+static napi_value ctx_info(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_ctx, ctx);
+    int cus = 0;
+    char name[128];
+    BCMC_CALL(env, bcmc_ctx_info(ctx, &cus, name, sizeof(name)));
+    napi_value o, s;
+    NAPI_CALL(env, napi_create_object(env, &o));
+    set_num(env, o, "numCUs", cus);
+    NAPI_CALL(env, napi_create_string_utf8(env, name, NAPI_AUTO_LENGTH, &s));
+    napi_set_named_property(env, o, "name", s);
+    return o;
+}
+
+static napi_value buf_alloc(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(2);
+    EXT(0, bcmc_ctx, ctx);
+    bcmc_buf *b = NULL;
+    BCMC_CALL(env, bcmc_buf_alloc(ctx, (uint64_t)num(env, argv[1]), &b));
+    return mk_ext(env, b);
+}
+
+static napi_value buf_free(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_buf, buf);
+    BCMC_CALL(env, bcmc_buf_free(buf));
+    return NULL;
+}
+
+static napi_value buf_size(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_buf, buf);
+    return mk_num(env, (double)bcmc_buf_size(buf));
+}
+
+/* bufUpload(buf, offset, typedArray) / bufDownload(buf, offset, typedArray) */
+static napi_value buf_upload(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(3);
+    EXT(0, bcmc_buf, buf);
+    void *d;
+    size_t n;
+    if (bytes_of(env, argv[2], &d, &n)) {
+        napi_throw_type_error(env, NULL, "expected a TypedArray or Buffer");
+        return NULL;
+    }
+    BCMC_CALL(env, bcmc_buf_upload(buf, (uint64_t)num(env, argv[1]), d, n));
+    return NULL;
+}
+
+static napi_value buf_download(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(3);
+    EXT(0, bcmc_buf, buf);
+    void *d;
+    size_t n;
+    if (bytes_of(env, argv[2], &d, &n)) {
+        napi_throw_type_error(env, NULL, "expected a TypedArray or Buffer");
+        return NULL;
+    }
+    BCMC_CALL(env, bcmc_buf_download(buf, (uint64_t)num(env, argv[1]), d, n));
+    return NULL;
+}
+
+/* ---- primitives ------------------------------------------------------------ */
+static napi_value scan_aligned_size(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    return mk_num(env, bcmc_scan_aligned_size((uint32_t)num(env, argv[0])));
+}
+
+static napi_value scan_exclusive(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(4);
+    EXT(0, bcmc_ctx, ctx);
+    EXT(1, bcmc_buf, buf);
+    uint32_t total = 0;
+    BCMC_CALL(env, bcmc_scan_exclusive_u32(ctx, buf, (uint32_t)num(env, argv[2]),
+                                           (uint32_t)num(env, argv[3]), &total));
+    return mk_num(env, total);
+}
+
+static napi_value compact_ids(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(5);
+    EXT(0, bcmc_ctx, ctx);
+    EXT(2, bcmc_buf, flags);
+    EXT(3, bcmc_buf, offsets);
+    EXT(4, bcmc_buf, out);
+    BCMC_CALL(env, bcmc_compact_ids(ctx, (uint32_t)num(env, argv[1]), flags, offsets, out));
+    return NULL;
+}
+
+static napi_value compact_flags(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(4);
+    EXT(0, bcmc_ctx, ctx);
+    EXT(2, bcmc_buf, flags);
+    EXT(3, bcmc_buf, out);
+    uint32_t count = 0;
+    BCMC_CALL(env, bcmc_compact_flags(ctx, (uint32_t)num(env, argv[1]), flags, out, &count));
+    return mk_num(env, count);
+}
+
+static napi_value sort_aligned_size(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    return mk_num(env, bcmc_sort_aligned_size((uint32_t)num(env, argv[0])));
+}
+
+static napi_value sort_by_key(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(5);
+    EXT(0, bcmc_ctx, ctx);
+    EXT(1, bcmc_buf, keys);
+    EXT(2, bcmc_buf, vals);
+    bool rev = false;
+    napi_get_value_bool(env, argv[4], &rev);
+    BCMC_CALL(env, bcmc_sort_u32_by_key(ctx, keys, vals, (uint32_t)num(env, argv[3]), rev ? 1 : 0));
+    return NULL;
+}
+
+/* ---- LRU ------------------------------------------------------------------- */
+static napi_value lru_create(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(4);
+    EXT(0, bcmc_ctx, ctx);
+    bcmc_lru *l = NULL;
+    BCMC_CALL(env, bcmc_lru_create(ctx, (uint32_t)num(env, argv[1]), (uint32_t)num(env, argv[2]),
+                                   (uint32_t)num(env, argv[3]), &l));
+    return mk_ext(env, l);
+}
+
+static napi_value lru_destroy(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_lru, lru);
+    BCMC_CALL(env, bcmc_lru_destroy(lru));
+    return NULL;
+}
+
+static napi_value lru_update(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(2);
+    EXT(0, bcmc_lru, lru);
+    EXT(1, bcmc_buf, needed);
+    uint32_t n = 0;
+    bcmc_buf *ids = NULL;
+    BCMC_CALL(env, bcmc_lru_update(lru, needed, &n, &ids));
+    napi_value arr;
+    NAPI_CALL(env, napi_create_array_with_length(env, 2, &arr));
+    napi_set_element(env, arr, 0, mk_num(env, n));
+    napi_set_element(env, arr, 1, mk_ext(env, ids));
+    return arr;
+}
+
+static napi_value lru_reset(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_lru, lru);
+    BCMC_CALL(env, bcmc_lru_reset(lru));
+    return NULL;
+}
+
+static napi_value lru_info(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_lru, lru);
+    bcmc_lru_info li;
+    BCMC_CALL(env, bcmc_lru_get_info(lru, &li));
+    napi_value o;
+    NAPI_CALL(env, napi_create_object(env, &o));
+    set_num(env, o, "cacheSize", li.cache_size);
+    set_num(env, o, "totalElements", li.total_elements);
+    set_num(env, o, "elementSize", li.element_size);
+    set_num(env, o, "displayNumSlotsAvailable", li.slots_available);
+    set_num(env, o, "reservedBytes", (double)li.reserved_bytes);
+    return o;
+}
+
+/* lruState(lru, age:Uint32Array, avail:Uint32Array, item:Int32Array, slot:Int32Array) */
+static napi_value lru_state(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(5);
+    EXT(0, bcmc_lru, lru);
+    void *p[4];
+    size_t n;
+    for (int i = 0; i < 4; ++i)
+        if (bytes_of(env, argv[1 + i], &p[i], &n)) {
+            napi_throw_type_error(env, NULL, "expected typed arrays");
+            return NULL;
+        }
+    BCMC_CALL(env, bcmc_lru_read_state(lru, p[0], p[1], p[2], p[3]));
+    return NULL;
+}
+
+static napi_value lru_cache_buffer(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_lru, lru);
+    return mk_ext(env, bcmc_lru_cache_buffer(lru));
+}
+
+/* ---- CompressedMarchingCubes ---------------------------------------------- */
+static napi_value mc_create(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_ctx, ctx);
+    bcmc_mc *m = NULL;
+    BCMC_CALL(env, bcmc_mc_create(ctx, &m));
+    return mk_ext(env, m);
+}
+
+static napi_value mc_destroy(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_mc, mc);
+    BCMC_CALL(env, bcmc_mc_destroy(mc));
+    return NULL;
+}
+
+static int three(napi_env env, napi_value arr, double out[3])
+{
+    for (uint32_t i = 0; i < 3; ++i) {
+        napi_value e;
+        if (napi_get_element(env, arr, i, &e) != napi_ok)
+            return -1;
+        out[i] = num(env, e);
+    }
+    return 0;
+}
+
+/* mcSetVolume(mc, bytes, rate, dims[3], scale[3], initialCacheSlots) */
+static napi_value mc_set_volume(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(6);
+    EXT(0, bcmc_mc, mc);
+    void *d;
+    size_t n;
+    double dd[3], sc[3];
+    if (bytes_of(env, argv[1], &d, &n) || three(env, argv[3], dd) || three(env, argv[4], sc)) {
+        napi_throw_type_error(env, NULL, "expected (mc, Uint8Array, rate, dims[3], scale[3], n)");
+        return NULL;
+    }
+    const uint32_t dims[3] = {(uint32_t)dd[0], (uint32_t)dd[1], (uint32_t)dd[2]};
+    const float scale[3] = {(float)sc[0], (float)sc[1], (float)sc[2]};
+    BCMC_CALL(env, bcmc_mc_set_compressed_volume(mc, d, n, num(env, argv[2]), dims, scale,
+                                                 (uint32_t)num(env, argv[5])));
+    return NULL;
+}
+
+static napi_value mc_set_format(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(2);
+    EXT(0, bcmc_mc, mc);
+    BCMC_CALL(env, bcmc_mc_set_vertex_format(mc, (bcmc_vertex_format)(int)num(env, argv[1])));
+    return NULL;
+}
+
+static napi_value mc_compute_surface(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(2);
+    EXT(0, bcmc_mc, mc);
+    bcmc_perf p;
+    uint64_t nv = 0;
+    BCMC_CALL(env, bcmc_mc_compute_surface(mc, (float)num(env, argv[1]), &p, &nv));
+    napi_value o;
+    NAPI_CALL(env, napi_create_object(env, &o));
+    set_num(env, o, "numVertices", (double)nv);
+    set_num(env, o, "computeActiveBlocks", p.computeActiveBlocks);
+    set_num(env, o, "numActiveBlocks", p.numActiveBlocks);
+    set_num(env, o, "cacheUpdate", p.cacheUpdate);
+    set_num(env, o, "numBlocksDecompressed", p.numBlocksDecompressed);
+    set_num(env, o, "decompression", p.decompression);
+    set_num(env, o, "compactActiveIDs", p.compactActiveIDs);
+    set_num(env, o, "computeBlockHasVertices", p.computeBlockHasVertices);
+    set_num(env, o, "numBlocksWithVertices", p.numBlocksWithVertices);
+    set_num(env, o, "compactBlocksWithVertices", p.compactBlocksWithVertices);
+    set_num(env, o, "computeVertices", p.computeVertices);
+    set_num(env, o, "cacheSize", p.cacheSize);
+    set_num(env, o, "totalMs", p.total_ms);
+    return o;
+}
+
+static napi_value mc_info(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_mc, mc);
+    bcmc_mc_info mi;
+    BCMC_CALL(env, bcmc_mc_get_info(mc, &mi));
+    napi_value o;
+    NAPI_CALL(env, napi_create_object(env, &o));
+    set_num(env, o, "totalBlocks", mi.total_blocks);
+    set_num(env, o, "numActiveBlocks", mi.num_active_blocks);
+    set_num(env, o, "numBlocksWithVertices", mi.num_blocks_with_vertices);
+    set_num(env, o, "newDecompressed", mi.new_decompressed);
+    set_num(env, o, "numVertices", (double)mi.num_vertices);
+    set_num(env, o, "vertexCapacity", (double)mi.vertex_capacity);
+    set_num(env, o, "maxBits", mi.max_bits);
+    set_num(env, o, "format", mi.format);
+    return o;
+}
+
+static napi_value mc_vertex_buffer(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_mc, mc);
+    return mk_ext(env, bcmc_mc_vertex_buffer(mc));
+}
+
+static napi_value mc_lru(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_mc, mc);
+    return mk_ext(env, bcmc_mc_lru(mc));
+}
+
+static napi_value mc_array(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(2);
+    EXT(0, bcmc_mc, mc);
+    return mk_ext(env, bcmc_mc_array_buffer(mc, (bcmc_mc_array)(int)num(env, argv[1])));
+}
+
+static napi_value mc_memory_use(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_mc, mc);
+    uint64_t t = 0, c = 0;
+    BCMC_CALL(env, bcmc_mc_memory_use(mc, &t, &c));
+    napi_value o;
+    NAPI_CALL(env, napi_create_object(env, &o));
+    set_num(env, o, "total", (double)t);
+    set_num(env, o, "cache", (double)c);
+    return o;
+}
+
+static napi_value version(napi_env env, napi_callback_info info)
+{
+    napi_value s;
+    NAPI_CALL(env, napi_create_string_utf8(env, bcmc_version(), NAPI_AUTO_LENGTH, &s));
+    return s;
+}
+
+#define FN(name, fn) {name, NULL, fn, NULL, NULL, NULL, napi_default, NULL}
+
+static napi_value init(napi_env env, napi_value exports)
+{
+    napi_property_descriptor d[] = {
+        FN("version", version),
+        FN("ctxCreate", ctx_create), FN("ctxDestroy", ctx_destroy), FN("ctxSync", ctx_sync),
+        FN("ctxInfo", ctx_info),
+        FN("bufAlloc", buf_alloc), FN("bufFree", buf_free), FN("bufSize", buf_size),
+        FN("bufUpload", buf_upload), FN("bufDownload", buf_download),
+        FN("scanAlignedSize", scan_aligned_size), FN("scanExclusive", scan_exclusive),
+        FN("compactIds", compact_ids), FN("compactFlags", compact_flags),
+        FN("sortAlignedSize", sort_aligned_size), FN("sortByKey", sort_by_key),
+        FN("lruCreate", lru_create), FN("lruDestroy", lru_destroy), FN("lruUpdate", lru_update),
+        FN("lruReset", lru_reset), FN("lruInfo", lru_info), FN("lruState", lru_state),
+        FN("lruCacheBuffer", lru_cache_buffer),
+        FN("mcCreate", mc_create), FN("mcDestroy", mc_destroy), FN("mcSetVolume", mc_set_volume),
+        FN("mcSetFormat", mc_set_format), FN("mcComputeSurface", mc_compute_surface),
+        FN("mcInfo", mc_info), FN("mcVertexBuffer", mc_vertex_buffer), FN("mcLru", mc_lru),
+        FN("mcArray", mc_array), FN("mcMemoryUse", mc_memory_use),
+    };
+    napi_define_properties(env, exports, sizeof(d) / sizeof(d[0]), d);
+    return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)
