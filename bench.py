@@ -40,11 +40,14 @@ def stage_bytes(s, rate, n_blocks):
     a, d, h, v = s["numActiveBlocks"], s["numBlocksDecompressed"], s["numBlocksWithVertices"], \
         s["numVertices"]
     return {
-        "computeActiveBlocks": 8 * n_blocks + 4 * n_blocks + 4 * a,
+        # ranges 8N (+ the N-byte code array written and read) + flags 4N + ids 4A
+        "computeActiveBlocks": 8 * n_blocks + 2 * n_blocks + 4 * n_blocks + 4 * a,
+        # payload 8r + block id 4 + slot 4 + 64 floats 256, per decoded block
         "decompression": d * (8 * rate + 4 + 4 + 256),
-        "computeBlockHasVertices": a * (256 + 244 + 4 + 4),
-        "compactBlocksWithVertices": 4 * a + 12 * h,
-        "computeVertices": h * (256 + 244 + 8 + 4) + 16 * v,
+        # fused count + offsets + emission: active id 4 + own block 256 + ghost
+        # voxels <= 244 per active block, list entry 4+8 per block with
+        # vertices, 16 B per vertex
+        "computeVertices": a * (4 + 256 + 244) + 12 * h + 16 * v,
     }
 
 
@@ -54,8 +57,10 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="fourier512_r4")
-    ap.add_argument("--cpu-queries", type=int, default=2,
-                    help="queries of the sweep timed on the CPU oracle (0 = skip)")
+    ap.add_argument("--cpu-sweeps", type=int, default=16,
+                    help="sweeps of the 8 isovalues timed on the CPU oracle (0 = skip)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run one device copy kernel of this size (HBM counter calibration)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -122,8 +127,7 @@ def main():
     lat = np.array([r["wall_ms"] for r in records])
 
     # per-stage device time and algorithmic bytes over the timed queries
-    stages = ["computeActiveBlocks", "cacheUpdate", "decompression", "computeBlockHasVertices",
-              "compactBlocksWithVertices", "computeVertices"]
+    stages = ["computeActiveBlocks", "cacheUpdate", "decompression", "computeVertices"]
     st_ms = {k: float(sum(r[k] for r in records)) for k in stages}
     st_bytes = {k: 0 for k in stages}
     for r in records:
@@ -173,7 +177,19 @@ def main():
         "cpu_baseline": None,
     }
 
-    if args.cpu_queries > 0:
+    if args.calibrate_copy_mib > 0:
+        nb = args.calibrate_copy_mib << 20
+        src, dst = dev.create_buffer(nb), dev.create_buffer(nb)
+        src.fill_u32(1, nb // 4)
+        bcmc_amd.lib().bcmc_ctx_enable_timing(dev.h, 1)
+        dst.copy_from(src, nb)
+        cms = bcmc_amd.lib().bcmc_ctx_last_kernel_ms(dev.h)
+        bcmc_amd.lib().bcmc_ctx_enable_timing(dev.h, 0)
+        out["copy_calibration"] = {"bytes": nb, "ms": round(cms, 4),
+                                   "GBps": round(2 * nb / (cms * 1e-3) / 1e9, 1)}
+        del src, dst
+
+    if args.cpu_sweeps > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -181,16 +197,29 @@ def main():
         ref = oracle.MC(stream, 64 * rate, dims, initial_cache=cache0, threads=threads)
         t_load = time.time() - t
         tq = time.time()
-        for iso in isos[:args.cpu_queries]:
-            ref.compute_surface(float(iso))
+        nq = 0
+        for _ in range(args.cpu_sweeps):
+            ref.lru_reset()  # one step = reset + sweep, as on the GPU
+            for iso in isos:
+                ref.compute_surface(float(iso))
+                nq += 1
         t_q = time.time() - tq
+        # single-thread sample: one sweep
+        ref1 = oracle.MC(stream, 64 * rate, dims, initial_cache=cache0, threads=1)
+        t1 = time.time()
+        for iso in isos:
+            ref1.compute_surface(float(iso))
+        t_1 = time.time() - t1
         out["cpu_baseline"] = {
-            "value": round(args.cpu_queries / t_q, 4), "unit": "iso-queries/s", "cores": threads,
+            "value": round(nq / t_q, 4), "unit": "iso-queries/s", "cores": threads,
             "kind": "port",
-            "sample": f"first {args.cpu_queries} isovalues of the same sweep after a cold LRU "
-                      f"(same stream); C oracle, OpenMP {threads} threads; load-time range "
-                      f"pass {t_load:.2f}s not included"}
-        log(f"bench: cpu oracle {args.cpu_queries} queries in {t_q:.2f}s with {threads} threads")
+            "sample": f"{args.cpu_sweeps} sweeps x 8 isovalues ({nq} queries, {t_q:.1f}s) of the same "
+                      f"workload on the same stream; C oracle with OpenMP over blocks, {threads} "
+                      f"threads; load-time range pass ({t_load:.2f}s) excluded",
+            "single_thread": {"value": round(len(isos) / t_1, 4), "cores": 1,
+                              "sample": f"1 sweep x 8 isovalues, {t_1:.1f}s"}}
+        log(f"bench: cpu oracle {nq} queries in {t_q:.2f}s with {threads} threads; "
+            f"1 thread: 8 queries in {t_1:.2f}s")
     print(json.dumps(out), flush=True)
 
 

@@ -64,6 +64,11 @@ BCMC_API int bcmc_buf_free(bcmc_buf *buf);
 BCMC_API int bcmc_buf_upload(bcmc_buf *buf, uint64_t offset, const void *src, uint64_t bytes);
 BCMC_API int bcmc_buf_download(bcmc_buf *buf, uint64_t offset, void *dst, uint64_t bytes);
 BCMC_API int bcmc_buf_fill_u32(bcmc_buf *buf, uint64_t offset, uint32_t value, uint64_t count);
+/* commandEncoder.copyBufferToBuffer (used throughout compressed_marching_cubes.js
+ * and lru_cache.js): device copy by a kernel on the context stream; offsets
+ * 16-byte aligned.  Also the calibration kernel for HBM byte counters. */
+BCMC_API int bcmc_buf_copy(bcmc_buf *dst, uint64_t dst_offset, bcmc_buf *src, uint64_t src_offset,
+                           uint64_t bytes);
 BCMC_API uint64_t bcmc_buf_size(const bcmc_buf *buf);
 BCMC_API void *bcmc_buf_device_ptr(const bcmc_buf *buf);
 

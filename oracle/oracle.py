@@ -251,6 +251,9 @@ class MC:
     def ranges(self):
         return _arr(lib().orc_mc_ranges(self.h), 2 * self.n, np.float32).reshape(-1, 2)
 
+    def lru_reset(self):
+        lib().orc_lru_reset(lib().orc_mc_lru(self.h))
+
     def lru_state(self):
         h = lib().orc_mc_lru(self.h)
         c = lib().orc_lru_size(h)

@@ -51,5 +51,5 @@ def test_invalid_arguments_return_codes():
 def test_tri_table_packed_in_library_source_matches_oracle_header():
     # Both copies of the table are generated from the same data; the hash is
     # checked in test_oracle.py.  Here: the kernel header is what mc.hip uses.
-    src = open(os.path.join(ROOT, "webgpu-bcmc_amd", "csrc", "mc.hip")).read()
+    src = open(os.path.join(ROOT, "webgpu-bcmc_amd", "csrc", "mc_device.h")).read()
     assert "BCMC_TRI_EDGES_INIT" in src and "BCMC_TRI_COUNT_INIT" in src

@@ -132,6 +132,17 @@ __global__ void compact_ids_kernel(uint32_t n, const uint32_t *flags, const uint
             out[offsets[i]] = i;
 }
 
+// 16-byte vector copy (n16 = bytes / 16), then a byte tail
+__global__ void copy_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ src, uint64_t n16,
+                            uint8_t *__restrict__ dtail, const uint8_t *__restrict__ stail,
+                            uint32_t ntail) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n16; i += stride)
+        dst[i] = src[i];
+    if (blockIdx.x == 0 && threadIdx.x < ntail)
+        dtail[threadIdx.x] = stail[threadIdx.x];
+}
+
 __global__ void fill_u32_kernel(uint32_t *p, uint32_t v, uint64_t n) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
@@ -328,6 +339,38 @@ BCMC_API int bcmc_buf_fill_u32(bcmc_buf *buf, uint64_t offset, uint32_t value, u
     fill_u32_kernel<<<grid, 256, 0, buf->ctx->stream>>>((uint32_t *)((char *)buf->ptr + offset),
                                                          value, count);
     BCMC_HIP_CHECK(hipGetLastError());
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_buf_copy(bcmc_buf *dst, uint64_t dst_offset, bcmc_buf *src, uint64_t src_offset,
+                           uint64_t bytes) {
+    BCMC_CHECK_ARG(dst && src, "null buffer");
+    BCMC_CHECK_ARG(dst_offset <= dst->bytes && bytes <= dst->bytes - dst_offset &&
+                       src_offset <= src->bytes && bytes <= src->bytes - src_offset,
+                   "copy out of range");
+    BCMC_CHECK_ARG(dst_offset % 16 == 0 && src_offset % 16 == 0, "offsets must be 16-byte aligned");
+    if (!bytes)
+        return BCMC_OK;
+    bcmc_ctx *ctx = dst->ctx;
+    const uint64_t n16 = bytes / 16;
+    const uint32_t ntail = (uint32_t)(bytes % 16);
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(div_up(n16, 256),
+                                                                             persistent_grid(ctx, 16)));
+    char *d = (char *)dst->ptr + dst_offset;
+    const char *sp = (const char *)src->ptr + src_offset;
+    if (ctx->timing)
+        BCMC_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
+    copy_kernel<<<grid, 256, 0, ctx->stream>>>((uint4 *)d, (const uint4 *)sp, n16,
+                                               (uint8_t *)d + n16 * 16,
+                                               (const uint8_t *)sp + n16 * 16, ntail);
+    BCMC_HIP_CHECK(hipGetLastError());
+    if (ctx->timing) {
+        BCMC_HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));
+        BCMC_HIP_CHECK(hipEventSynchronize(ctx->ev1));
+        float ms = 0.0f;
+        BCMC_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->last_kernel_ms = ms;
+    }
     return BCMC_OK;
 }
 

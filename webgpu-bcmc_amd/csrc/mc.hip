@@ -2,23 +2,20 @@
 //
 // One query (computeSurface, :580-809) is a fixed chain of kernels on the
 // context's stream with a single host read-back at the end:
-//   active     block active test (compute_block_active.comp) fused with a
-//              single-pass scan + compaction of the flags -> active ids;
+//   active     block active test (compute_block_active.comp) as a separable
+//              3x3x3 dilation, fused with a single-pass scan + compaction of
+//              the flags -> active ids;
 //   lru        LRUCache.update over the active list (lru.hip);
 //   decode     zfp decode of the misses into their cache slots (zfp.hip);
-//   count      per active block: 5^3 LDS tile with the ghost layer
-//              (compute_vertex_values.comp:96-252) and the per-cell triangle
-//              count (compute_block_has_vertices.comp +
-//              compute_block_voxel_num_verts.comp), one wave per block;
-//   offsets    two-channel single-pass scan of the counts: the compacted list
-//              of blocks with vertices (indices into the active list) and
-//              their 64-bit vertex offsets;
-//   emit       per block with vertices: the same tile, a wave prefix sum of
-//              the per-cell counts, interpolation and the vertex writes
+//   mc         one pass over the active list: per block a 5^3 LDS tile with
+//              the ghost layer (compute_vertex_values.comp:96-252), the
+//              per-cell triangle counts (compute_block_has_vertices.comp,
+//              compute_block_voxel_num_verts.comp), a two-channel look-back
+//              giving the list of blocks with vertices and their 64-bit
+//              vertex offsets, and the vertex writes
 //              (compute_block_vertices.comp:76-174).
 // The vertex buffer grows like the reference (x1.5, :774-785); when a query
-// needs more room the emit kernel writes nothing and is re-run once the
-// buffer has grown.
+// needs more room than it has, the pass is run again once it has grown.
 #include "common.h"
 #include "kernels.h"
 #include "lru.h"
@@ -32,8 +29,16 @@
 namespace bcmc {
 
 // ---- active blocks ---------------------------------------------------------
+// Pass 1: per-block code (own test + x-direction OR), pass 2: y/z OR fused
+// with the single-pass scan and compaction of the flags.
+__global__ void active_code_kernel(const float2 *ranges, uint8_t *code, float iso, uint32_t nbx,
+                                   uint32_t n) {
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < n; b += gridDim.x * blockDim.x)
+        code[b] = (uint8_t)active_code(ranges, b, b % nbx, nbx, iso);
+}
+
 struct ActiveOp {
-    const float2 *ranges;
+    const uint8_t *code;
     uint32_t *flags, *active_ids;
     QState *q;
     float iso;
@@ -46,30 +51,7 @@ struct ActiveOp {
         q->num_verts = 0;
         q->overflow = 0;
     }
-    // compute_block_active.comp:127-164
-    BCMC_DEV void value(uint32_t b, uint32_t (&v)[1]) const {
-        const float2 r = ranges[b];
-        uint32_t act = 0;
-        if (iso >= r.x && iso <= r.y) {
-            act = 1;
-        } else {
-            const int bx = (int)(b % nb[0]), by = (int)((b / nb[0]) % nb[1]),
-                      bz = (int)(b / (nb[0] * nb[1]));
-            for (int k = -1; k < 2 && !act; ++k)
-                for (int j = -1; j < 2 && !act; ++j)
-                    for (int i = -1; i < 2 && !act; ++i) {
-                        const int cx = bx + i, cy = by + j, cz = bz + k;
-                        if ((i | j | k) == 0 || cx < 0 || cy < 0 || cz < 0 || cx >= (int)nb[0] ||
-                            cy >= (int)nb[1] || cz >= (int)nb[2])
-                            continue;
-                        const float2 o = ranges[cx + nb[0] * (cy + nb[1] * cz)];
-                        const float lo = fminf(r.x, o.x), hi = fmaxf(r.y, o.y);
-                        if (iso >= lo && iso <= hi)
-                            act = 1;
-                    }
-        }
-        v[0] = act;
-    }
+    BCMC_DEV void value(uint32_t b, uint32_t (&v)[1]) const { v[0] = active_from_codes(code, b, nb); }
     BCMC_DEV void emit(uint32_t b, const uint32_t (&v)[1], const unsigned long long (&e)[1]) const {
         flags[b] = v[0];
         if (v[0])
@@ -78,75 +60,104 @@ struct ActiveOp {
     BCMC_DEV void total(const unsigned long long (&t)[1]) const { q->num_active = (uint32_t)t[0]; }
 };
 
-// ---- per-block vertex counts -------------------------------------------------
-__global__ __launch_bounds__(kMcThreads) void mc_count_kernel(McView m, const uint32_t *active_ids,
-                                                              const QState *q, uint32_t *counts) {
-    __shared__ float tiles[kMcThreads / 64][kTileStride];
-    __shared__ int32_t slots[kMcThreads / 64][8];
-    const uint32_t wave = threadIdx.x / 64, lane = lane_id();
-    const uint32_t n = q->num_active;
-    const float iso = q->iso;
-    for (uint32_t k = blockIdx.x * (kMcThreads / 64) + wave; k < n; k += gridDim.x * (kMcThreads / 64)) {
-        uint32_t d[3];
-        load_tile(m, active_ids[k], tiles[wave], slots[wave], d);
-        const uint32_t total = block_vertex_count(tiles[wave], d, iso);
-        if (lane == 0)
-            counts[k] = total;
-        __builtin_amdgcn_wave_barrier();
-    }
-}
+// ---- count + offsets + emission in one pass --------------------------------
+// Workgroups take tiles of 16 active blocks in list order (4 waves x 4 blocks).
+// Each wave loads its 4 tiles with shared load rounds, counts the vertices of
+// each block, and the tile's totals (blocks with vertices, vertices) chain
+// through two look-back channels; each block then has its exact 64-bit offset
+// and writes its vertices straight away.  Blocks whose vertices would end past
+// the buffer capacity write nothing; the host grows the buffer and runs the
+// kernel again when the total exceeds the capacity.
+constexpr int kFuseBlocks = 4;
+constexpr uint32_t kFuseTile = (kMcThreads / 64) * kFuseBlocks;
 
-// ---- blocks with vertices + vertex offsets ---------------------------------
-struct CountScanOp {
-    const uint32_t *counts;
+struct FusedArgs {
+    McView m;
+    const uint32_t *active_ids;
+    QState *q;
     uint32_t *with_verts;
     unsigned long long *vert_offsets;
-    QState *q;
-    BCMC_DEV uint32_t size() const { return q->num_active; }
-    BCMC_DEV void begin() const {}
-    BCMC_DEV void value(uint32_t k, uint32_t (&v)[2]) const {
-        const uint32_t c = counts[k];
-        v[0] = c ? 1u : 0u;
-        v[1] = c;
-    }
-    BCMC_DEV void emit(uint32_t k, const uint32_t (&v)[2], const unsigned long long (&e)[2]) const {
-        if (v[0]) {
-            with_verts[e[0]] = k;
-            vert_offsets[e[0]] = e[1];
-        }
-    }
-    BCMC_DEV void total(const unsigned long long (&t)[2]) const {
-        q->num_with = (uint32_t)t[0];
-        q->num_verts = t[1];
-    }
-};
-
-// ---- vertex emission -------------------------------------------------------
-struct EmitArgs {
-    McView m;
-    const uint32_t *active_ids, *with_verts;
-    const unsigned long long *vert_offsets;
-    const QState *q;
     void *verts;
     unsigned long long capacity;
     int fmt;
 };
 
-__global__ __launch_bounds__(kMcThreads) void mc_emit_kernel(EmitArgs a) {
-    __shared__ float tiles[kMcThreads / 64][kTileStride];
-    __shared__ int32_t slots[kMcThreads / 64][8];
-    if (a.q->num_verts > a.capacity)
-        return;  // the host grows the buffer and runs this kernel again
-    const uint32_t wave = threadIdx.x / 64;
-    const uint32_t n = a.q->num_with;
+__global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanSite s1, FusedArgs a) {
+    constexpr int NW = kMcThreads / 64;
+    __shared__ float tiles[NW][kFuseBlocks * kTileStride];
+    __shared__ int32_t slots[NW][kFuseBlocks * 8];
+    __shared__ uint32_t lds_w[2][NW + 1];
+    __shared__ unsigned long long lds_prefix[2];
+    __shared__ uint32_t lds_tile;
+    const uint32_t n = a.q->num_active;
     const float iso = a.q->iso;
-    for (uint32_t h = blockIdx.x * (kMcThreads / 64) + wave; h < n; h += gridDim.x * (kMcThreads / 64)) {
-        const uint32_t bid = a.active_ids[a.with_verts[h]];
-        uint32_t d[3];
-        load_tile(a.m, bid, tiles[wave], slots[wave], d);
-        emit_block_vertices(tiles[wave], d, iso, bid, a.m.nb, a.vert_offsets[h], a.verts, a.fmt);
+    const uint32_t ntiles = div_up(n, kFuseTile);
+    const uint32_t e0 = scan_site_epoch(s0), e1 = scan_site_epoch(s1);
+    const uint32_t wave = threadIdx.x / 64, lane = lane_id();
+    if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+        a.q->num_with = 0;
+        a.q->num_verts = 0;
+    }
+    for (;;) {
+        const uint32_t tile = next_tile(s0, &lds_tile);
+        if (tile >= ntiles)
+            break;
+        const uint32_t k0 = tile * kFuseTile + wave * kFuseBlocks;
+        uint32_t bid[kFuseBlocks], cnt[kFuseBlocks], d[kFuseBlocks][3];
+        bool live[kFuseBlocks];
+#pragma unroll
+        for (int j = 0; j < kFuseBlocks; ++j) {
+            live[j] = k0 + j < n;
+            bid[j] = live[j] ? a.active_ids[k0 + j] : 0u;
+        }
+        load_tiles<kFuseBlocks>(a.m, bid, live, tiles[wave], slots[wave], d);
+        uint32_t hs = 0, vs = 0;
+#pragma unroll
+        for (int j = 0; j < kFuseBlocks; ++j) {
+            cnt[j] = live[j] ? block_vertex_count(tiles[wave] + kTileStride * j, d[j], iso) : 0u;
+            hs += cnt[j] ? 1u : 0u;
+            vs += cnt[j];
+        }
+        if (lane == 0) {
+            lds_w[0][wave] = hs;
+            lds_w[1][wave] = vs;
+        }
+        __syncthreads();
+        uint32_t before_h = 0, before_v = 0, agg_h = 0, agg_v = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t th = lds_w[0][w], tv = lds_w[1][w];
+            before_h += w < (int)wave ? th : 0u;
+            before_v += w < (int)wave ? tv : 0u;
+            agg_h += th;
+            agg_v += tv;
+        }
+        __syncthreads();
+        const unsigned long long ph = tile_lookback<kMcThreads>(s0, tile, e0, agg_h, &lds_prefix[0]);
+        const unsigned long long pv = tile_lookback<kMcThreads>(s1, tile, e1, agg_v, &lds_prefix[1]);
+        unsigned long long h = ph + before_h, v = pv + before_v;
+#pragma unroll
+        for (int j = 0; j < kFuseBlocks; ++j) {
+            if (cnt[j]) {
+                if (lane == 0) {
+                    a.with_verts[h] = k0 + j;
+                    a.vert_offsets[h] = v;
+                }
+                if (v + cnt[j] <= a.capacity)
+                    emit_block_vertices(tiles[wave] + kTileStride * j, d[j], iso, bid[j], a.m.nb, v,
+                                        a.verts, a.fmt);
+                ++h;
+                v += cnt[j];
+            }
+        }
+        if (tile == ntiles - 1 && threadIdx.x == 0) {
+            a.q->num_with = (uint32_t)(ph + agg_h);
+            a.q->num_verts = pv + agg_v;
+        }
         __builtin_amdgcn_wave_barrier();
     }
+    scan_site_finish(s0, e0);
+    scan_site_finish(s1, e1);
 }
 
 static uint32_t vertex_bytes(int fmt) {
@@ -169,7 +180,8 @@ struct bcmc_mc {
     uint64_t *stream = nullptr;
     uint64_t stream_bytes = 0, stream_words = 0;
     float2 *ranges = nullptr;
-    uint32_t *flags = nullptr, *active_ids = nullptr, *counts = nullptr, *with_verts = nullptr;
+    uint32_t *flags = nullptr, *active_ids = nullptr, *with_verts = nullptr;
+    uint8_t *active_code = nullptr;
     unsigned long long *vert_offsets = nullptr;
     bcmc_lru lru;  // non-owning handle over the cache state held here
     QState *q = nullptr, *qh = nullptr;
@@ -190,7 +202,7 @@ static void mc_free_volume(bcmc_mc *m) {
     dev_free(m->ranges);
     dev_free(m->flags);
     dev_free(m->active_ids);
-    dev_free(m->counts);
+    dev_free(m->active_code);
     dev_free(m->with_verts);
     dev_free(m->vert_offsets);
     dev_free(m->verts);
@@ -200,7 +212,8 @@ static void mc_free_volume(bcmc_mc *m) {
     lru_state_free(&m->lru.st);
     m->stream = nullptr;
     m->ranges = nullptr;
-    m->flags = m->active_ids = m->counts = m->with_verts = nullptr;
+    m->flags = m->active_ids = m->with_verts = nullptr;
+    m->active_code = nullptr;
     m->vert_offsets = nullptr;
     m->verts = nullptr;
     m->vert_cap = 0;
@@ -227,20 +240,20 @@ static void mc_refresh_views(bcmc_mc *m) {
     m->lru.cache_view.owned = false;
 }
 
-static int mc_launch_emit(bcmc_mc *m) {
-    EmitArgs a;
+static int mc_launch_fused(bcmc_mc *m) {
+    FusedArgs a;
     a.m = McView{(const float *)m->lru.st.data, m->lru.st.v.item_slot, m->flags,
                  {m->nb[0], m->nb[1], m->nb[2]}, m->lru.st.capacity};
     a.active_ids = m->active_ids;
+    a.q = m->q;
     a.with_verts = m->with_verts;
     a.vert_offsets = m->vert_offsets;
-    a.q = m->q;
     a.verts = m->verts;
     a.capacity = m->vert_cap;
     a.fmt = m->fmt;
-    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(m->total, kMcThreads / 64),
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(m->total, kFuseTile),
                                                                    persistent_grid(m->ctx, 8)));
-    mc_emit_kernel<<<grid, kMcThreads, 0, m->ctx->stream>>>(a);
+    mc_fused_kernel<<<grid, kMcThreads, 0, m->ctx->stream>>>(m->site_c0, m->site_c1, a);
     BCMC_HIP_CHECK(hipGetLastError());
     return BCMC_OK;
 }
@@ -316,12 +329,12 @@ BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *m, const void *stream, uint6
     BCMC_TRY(dev_alloc((void **)&m->ranges, n * 8));
     BCMC_TRY(dev_alloc((void **)&m->flags, n * 4));
     BCMC_TRY(dev_alloc((void **)&m->active_ids, n * 4));
-    BCMC_TRY(dev_alloc((void **)&m->counts, n * 4));
+    BCMC_TRY(dev_alloc((void **)&m->active_code, n));
     BCMC_TRY(dev_alloc((void **)&m->with_verts, n * 4));
     BCMC_TRY(dev_alloc((void **)&m->vert_offsets, n * 8));
     BCMC_TRY(scan_site_alloc(&m->site_active, n, kScanTile));
-    BCMC_TRY(scan_site_alloc(&m->site_c0, n, kScanTile));
-    BCMC_TRY(scan_site_alloc(&m->site_c1, n, kScanTile));
+    BCMC_TRY(scan_site_alloc(&m->site_c0, n, kFuseTile));
+    BCMC_TRY(scan_site_alloc(&m->site_c1, n, kFuseTile));
     const uint32_t init = initial_cache_slots ? initial_cache_slots
                                               : (uint32_t)std::ceil((double)total * 0.05);
     BCMC_TRY(lru_state_alloc(ctx, &m->lru.st, init, 256, m->total));
@@ -370,7 +383,10 @@ BCMC_API int bcmc_mc_compute_surface(bcmc_mc *m, float isovalue, bcmc_perf *perf
     if (timed)
         BCMC_HIP_CHECK(hipEventRecord(m->ev[0], st));
     // active blocks + compaction
-    ActiveOp aop{m->ranges, m->flags, m->active_ids, m->q, isovalue,
+    active_code_kernel<<<std::max<uint32_t>(1, std::min<uint32_t>(div_up(m->total, 256),
+                                                                  persistent_grid(ctx, 16))),
+                         256, 0, st>>>(m->ranges, m->active_code, isovalue, m->nb[0], m->total);
+    ActiveOp aop{m->active_code, m->flags, m->active_ids, m->q, isovalue,
                  {m->nb[0], m->nb[1], m->nb[2]}, m->total};
     launch_scan<1>(ctx, m->site_active, m->site_active, aop, m->total);
     if (timed)
@@ -385,29 +401,16 @@ BCMC_API int bcmc_mc_compute_surface(bcmc_mc *m, float isovalue, bcmc_perf *perf
                            (float *)m->lru.st.data, m->lru.st.capacity);
     if (timed)
         BCMC_HIP_CHECK(hipEventRecord(m->ev[3], st));
-    // per-block counts
-    McView mv{(const float *)m->lru.st.data, m->lru.st.v.item_slot, m->flags,
-              {m->nb[0], m->nb[1], m->nb[2]}, m->lru.st.capacity};
-    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(m->total, kMcThreads / 64),
-                                                                   persistent_grid(ctx, 8)));
-    mc_count_kernel<<<grid, kMcThreads, 0, st>>>(mv, m->active_ids, m->q, m->counts);
+    // counts, offsets and vertices in one pass
+    BCMC_TRY(mc_launch_fused(m));
     if (timed)
         BCMC_HIP_CHECK(hipEventRecord(m->ev[4], st));
-    // blocks with vertices + offsets
-    CountScanOp cop{m->counts, m->with_verts, m->vert_offsets, m->q};
-    launch_scan<2>(ctx, m->site_c0, m->site_c1, cop, m->total);
-    if (timed)
-        BCMC_HIP_CHECK(hipEventRecord(m->ev[5], st));
-    BCMC_TRY(mc_launch_emit(m));
-    if (timed)
-        BCMC_HIP_CHECK(hipEventRecord(m->ev[6], st));
     BCMC_HIP_CHECK(hipGetLastError());
     BCMC_HIP_CHECK(hipMemcpyAsync(m->qh, m->q, sizeof(QState), hipMemcpyDeviceToHost, st));
     BCMC_HIP_CHECK(hipStreamSynchronize(st));
-    const QState qs = *m->qh;
-    double emit_extra_ms = 0.0;
+    QState qs = *m->qh;
     if (qs.num_verts > m->vert_cap) {
-        // grow like compressed_marching_cubes.js:774-785, then emit again
+        // grow like compressed_marching_cubes.js:774-785, then run the pass again
         const uint64_t want = std::max<uint64_t>(qs.num_verts, (uint64_t)std::floor(m->vert_cap * 1.5));
         dev_free(m->verts);
         m->verts = nullptr;
@@ -415,12 +418,13 @@ BCMC_API int bcmc_mc_compute_surface(bcmc_mc *m, float isovalue, bcmc_perf *perf
         BCMC_TRY(dev_alloc(&m->verts, want * vertex_bytes(m->fmt)));
         m->vert_cap = want;
         if (timed)
-            BCMC_HIP_CHECK(hipEventRecord(m->ev[5], st));
-        BCMC_TRY(mc_launch_emit(m));
+            BCMC_HIP_CHECK(hipEventRecord(m->ev[3], st));
+        BCMC_TRY(mc_launch_fused(m));
         if (timed)
-            BCMC_HIP_CHECK(hipEventRecord(m->ev[6], st));
+            BCMC_HIP_CHECK(hipEventRecord(m->ev[4], st));
+        BCMC_HIP_CHECK(hipMemcpyAsync(m->qh, m->q, sizeof(QState), hipMemcpyDeviceToHost, st));
         BCMC_HIP_CHECK(hipStreamSynchronize(st));
-        (void)emit_extra_ms;
+        qs = *m->qh;
     }
     m->lru.st.cache_size = qs.cache_size;
     m->last_active = qs.num_active;
@@ -431,16 +435,16 @@ BCMC_API int bcmc_mc_compute_surface(bcmc_mc *m, float isovalue, bcmc_perf *perf
     if (num_vertices)
         *num_vertices = qs.num_verts;
     if (perf) {
-        float ms[6];
-        for (int i = 0; i < 6; ++i)
+        float ms[4];
+        for (int i = 0; i < 4; ++i)
             BCMC_HIP_CHECK(hipEventElapsedTime(&ms[i], m->ev[i], m->ev[i + 1]));
         perf->computeActiveBlocks = ms[0];
         perf->cacheUpdate = ms[1];
         perf->decompression = ms[2];
-        perf->compactActiveIDs = 0.0;  // fused into computeActiveBlocks
-        perf->computeBlockHasVertices = ms[3];
-        perf->compactBlocksWithVertices = ms[4];
-        perf->computeVertices = ms[5];
+        perf->compactActiveIDs = 0.0;            // fused into computeActiveBlocks
+        perf->computeBlockHasVertices = 0.0;     // fused into computeVertices
+        perf->compactBlocksWithVertices = 0.0;   // fused into computeVertices
+        perf->computeVertices = ms[3];
         perf->numActiveBlocks = qs.num_active;
         perf->numBlocksDecompressed = qs.num_new;
         perf->numBlocksWithVertices = qs.num_with;

@@ -152,6 +152,148 @@ BCMC_DEV void emit_block_vertices(const float *tile, const uint32_t (&d)[3], flo
     }
 }
 
+// ---------------------------------------------------------------------------
+// Active test (compute_block_active.comp:127-164) restated as a dilation.
+// The reference marks a block active when iso lies in its own range, or in
+// the union of its range with any in-bounds neighbour's.  With above = iso < lo
+// and below = iso > hi, the union with neighbour n misses iso exactly when
+// (above && above_n) || (below && below_n), so
+//   active = contains || (above && OR_27(!above_n)) || (below && OR_27(!below_n))
+// where OR_27 runs over the in-bounds 3x3x3 neighbourhood (the block's own
+// bit never contributes: !above is false when above).  The OR is separable:
+// the code byte holds the x-direction OR; the y/z part reads 9 code bytes.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kActContains = 1, kActAbove = 2, kActBelow = 4, kActXNotAbove = 8,
+                   kActXNotBelow = 16;
+
+BCMC_DEV uint32_t active_code(const float2 *ranges, uint32_t b, uint32_t x, uint32_t nbx, float iso) {
+    const float2 r = ranges[b];
+    const bool above = iso < r.x, below = iso > r.y;
+    bool xna = !above, xnb = !below;
+    if (x > 0) {
+        const float2 l = ranges[b - 1];
+        xna |= !(iso < l.x);
+        xnb |= !(iso > l.y);
+    }
+    if (x + 1 < nbx) {
+        const float2 h = ranges[b + 1];
+        xna |= !(iso < h.x);
+        xnb |= !(iso > h.y);
+    }
+    const bool contains = iso >= r.x && iso <= r.y;
+    return (contains ? kActContains : 0u) | (above ? kActAbove : 0u) | (below ? kActBelow : 0u) |
+           (xna ? kActXNotAbove : 0u) | (xnb ? kActXNotBelow : 0u);
+}
+
+BCMC_DEV uint32_t active_from_codes(const uint8_t *code, uint32_t b, const uint32_t (&nb)[3]) {
+    const uint32_t c = code[b];
+    if (c & kActContains)
+        return 1u;
+    const uint32_t need = (c & kActAbove) ? kActXNotAbove : (c & kActBelow) ? kActXNotBelow : 0u;
+    if (!need)
+        return 0u;
+    const uint32_t plane = nb[0] * nb[1];
+    const uint32_t y = (b / nb[0]) % nb[1], z = b / plane;
+    uint32_t acc = 0;
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz) {
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int yy = (int)y + dy, zz = (int)z + dz;
+            if (yy < 0 || zz < 0 || yy >= (int)nb[1] || zz >= (int)nb[2])
+                continue;
+            acc |= code[(int64_t)b + (int64_t)dy * nb[0] + (int64_t)dz * plane];
+        }
+    }
+    return (acc & need) ? 1u : 0u;
+}
+
+// ---------------------------------------------------------------------------
+// NB tiles per wave with the three dependent load rounds shared by all NB
+// blocks: (1) neighbour activity flags and all 8 candidate cache slots,
+// (2) the tile voxels.  bids[] and live[] are wave-uniform.
+// ---------------------------------------------------------------------------
+template <int NB>
+BCMC_DEV void load_tiles(const McView &m, const uint32_t (&bid)[NB], const bool (&live)[NB],
+                         float *tiles /* NB*128 */, int32_t *slots /* NB*8 */, uint32_t (&d)[NB][3]) {
+    const uint32_t lane = lane_id();
+    uint32_t bx[NB], by[NB], bz[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        bx[j] = bid[j] % m.nb[0];
+        by[j] = (bid[j] / m.nb[0]) % m.nb[1];
+        bz[j] = bid[j] / (m.nb[0] * m.nb[1]);
+    }
+    // lane j*8 + s: flag (s < 7: +x,+y,+z,+xy,+xz,+yz,+xyz) and slot of sub-block s
+    bool act = false;
+    int32_t slot = -1;
+    if (lane < 8u * NB) {
+        const uint32_t j = lane / 8, s = lane % 8;
+        uint32_t jx = bx[0], jy = by[0], jz = bz[0];
+        bool jl = live[0];
+#pragma unroll
+        for (int q = 1; q < NB; ++q)
+            if (j == (uint32_t)q) {
+                jx = bx[q];
+                jy = by[q];
+                jz = bz[q];
+                jl = live[q];
+            }
+        // flag codes in the reference's check order; slot sub-block s = sx + 2sy + 4sz
+        const uint32_t fcode = s < 3 ? (1u << s) : (s == 3 ? 3u : s == 4 ? 5u : s == 5 ? 6u : 7u);
+        const uint32_t fx = jx + (fcode & 1u), fy = jy + ((fcode >> 1) & 1u), fz = jz + (fcode >> 2);
+        if (jl && s < 7 && fx < m.nb[0] && fy < m.nb[1] && fz < m.nb[2])
+            act = m.flags[fx + m.nb[0] * (fy + m.nb[1] * fz)] != 0;
+        const uint32_t sx = jx + (s & 1u), sy = jy + ((s >> 1) & 1u), sz = jz + (s >> 2);
+        if (jl && sx < m.nb[0] && sy < m.nb[1] && sz < m.nb[2])
+            slot = m.item_slot[sx + m.nb[0] * (sy + m.nb[1] * sz)];
+        slots[lane] = slot;
+    }
+    const uint64_t ab = __ballot(act);
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const uint32_t a = (uint32_t)(ab >> (8 * j)) & 0x7Fu;
+        uint32_t dx = bx[j] + 1 < m.nb[0] ? 5 : 4, dy = by[j] + 1 < m.nb[1] ? 5 : 4,
+                 dz = bz[j] + 1 < m.nb[2] ? 5 : 4;
+        if (dx == 5 && dy == 5 && dz == 5 && !(a & 64))
+            dx = dy = dz = 4;
+        if (dx == 5 && dy == 5 && !(a & 8))
+            dx = dy = 4;
+        if (dx == 5 && dz == 5 && !(a & 16))
+            dx = dz = 4;
+        if (dy == 5 && dz == 5 && !(a & 32))
+            dy = dz = 4;
+        if (dx == 5 && !(a & 1))
+            dx = 4;
+        if (dy == 5 && !(a & 2))
+            dy = 4;
+        if (dz == 5 && !(a & 4))
+            dz = 4;
+        d[j][0] = dx;
+        d[j][1] = dy;
+        d[j][2] = dz;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        for (uint32_t t = lane; t < 125; t += 64) {
+            const uint32_t x = t % 5, y = (t / 5) % 5, z = t / 25;
+            float v = 0.0f;
+            if (live[j] && x < d[j][0] && y < d[j][1] && z < d[j][2]) {
+                const int32_t s = slots[8 * j + (x >> 2) + 2 * (y >> 2) + 4 * (z >> 2)];
+                if (s >= 0 && (uint32_t)s < m.capacity)
+                    v = m.cache[(uint64_t)(uint32_t)s * 64 + (x & 3) + 4 * (y & 3) + 16 * (z & 3)];
+            }
+            tiles[128 * j + t] = v;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 BCMC_DEV uint32_t block_vertex_count(const float *tile, const uint32_t (&d)[3], float iso) {
     uint32_t base;
     const int c = cell_case(tile, d, iso, base);

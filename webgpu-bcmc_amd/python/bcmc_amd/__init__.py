@@ -96,6 +96,7 @@ def lib():
         "bcmc_buf_upload": (i32, [vp, u64, vp, u64]),
         "bcmc_buf_download": (i32, [vp, u64, vp, u64]),
         "bcmc_buf_fill_u32": (i32, [vp, u64, u32, u64]),
+        "bcmc_buf_copy": (i32, [vp, u64, vp, u64, u64]),
         "bcmc_buf_size": (u64, [vp]),
         "bcmc_buf_device_ptr": (vp, [vp]),
         "bcmc_scan_aligned_size": (u32, [u32]),
@@ -203,6 +204,10 @@ class Buffer:
         out = np.empty(int(count), dt)
         _check(lib().bcmc_buf_download(self.h, offset, out.ctypes.data, out.nbytes))
         return out
+
+    def copy_from(self, src, nbytes, offset=0, src_offset=0):
+        """copyBufferToBuffer(src, src_offset, self, offset, nbytes)."""
+        _check(lib().bcmc_buf_copy(self.h, offset, src.h, src_offset, nbytes))
 
     def fill_u32(self, value, count, offset=0):
         _check(lib().bcmc_buf_fill_u32(self.h, offset, value, count))
