@@ -203,6 +203,38 @@ BCMC_API bcmc_buf *bcmc_mc_array_buffer(bcmc_mc *mc, bcmc_mc_array which);
  * device memory held by the object (out), with the cache share (cache_out). */
 BCMC_API int bcmc_mc_memory_use(bcmc_mc *mc, uint64_t *total_out, uint64_t *cache_out);
 
+/* ---- Batched and served forms (no reference counterpart) ------------------- */
+/* An engine keeps compressed volumes and a decoded-block pool shared by all of
+ * them resident on the device; one bcmc_engine_run call answers many
+ * (volume, isovalue) queries with one launch per stage for all of them.  Each
+ * query's vertices equal those of a single computeSurface on that volume
+ * (same blocks, order and FP32 positions); pool slots are the engine's own
+ * (the least recently used blocks are evicted when the pool is full). */
+typedef struct bcmc_engine bcmc_engine;
+typedef struct {
+    uint64_t num_vertices;        /* this query's vertices */
+    uint64_t vertex_offset;       /* its first vertex in the step's vertex buffer */
+    uint32_t num_active_blocks;   /* this query's active blocks */
+    uint32_t step_decoded_blocks; /* blocks the whole step decoded */
+} bcmc_query_result;
+typedef struct {
+    uint32_t num_volumes, pool_slots, pool_used, epoch;
+    uint64_t pool_bytes, device_bytes;
+} bcmc_engine_info;
+/* pool_bytes: HBM for decoded blocks (256 B per block). */
+BCMC_API int bcmc_engine_create(bcmc_ctx *ctx, uint64_t pool_bytes, bcmc_engine **out);
+BCMC_API int bcmc_engine_destroy(bcmc_engine *e);
+BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t bytes, double rate,
+                                    const uint32_t dims[3], uint32_t *volume_id);
+BCMC_API int bcmc_engine_set_vertex_format(bcmc_engine *e, bcmc_vertex_format fmt);
+/* One step over nq queries; results[i] for query i (may be NULL).  Fails with
+ * BCMC_ERR_NOMEM when the step's blocks do not fit the pool. */
+BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const float *isovalues,
+                             uint32_t nq, bcmc_query_result *results);
+/* The last step's vertices (results[i].vertex_offset .. + num_vertices). */
+BCMC_API bcmc_buf *bcmc_engine_vertex_buffer(bcmc_engine *e);
+BCMC_API int bcmc_engine_stats(bcmc_engine *e, bcmc_engine_info *info);
+
 /* ---- ZFP decode (shaders/zfp_decompress.comp, zfp_decompress_block.comp) -- */
 /* Decode blocks ids[0..n) of a fixed-rate stream into out[i*64 .. i*64+64)
  * (float32, x fastest).  ids may be NULL for blocks 0..n-1. */

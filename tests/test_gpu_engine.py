@@ -1,0 +1,63 @@
+"""Batched form on the GPU: every query of a multi-volume step equals the
+oracle's single-query computeSurface (counts and FP32 vertices bit-exact),
+independent of the other queries in the step and of the pool state."""
+import numpy as np
+import pytest
+
+import bcmc_amd
+import oracle
+from bcmc_amd import synth
+from gpu_common import dev, say  # noqa: F401
+
+pytestmark = pytest.mark.gpu
+
+VOLS = [((32, 32, 32), 8, 3), ((48, 40, 36), 4, 4), ((24, 28, 20), 2, 5), ((13, 10, 7), 4, 6)]
+
+
+def _streams():
+    out = []
+    for dims, rate, seed in VOLS:
+        f = synth.field(synth.BLOBS, seed, dims, 6, (3, 10, 0.5, 1))
+        out.append((synth.encode(f, 64 * rate), rate, dims))
+    return out
+
+
+def _oracle(s, rate, dims, iso):
+    ref = oracle.MC(s, 64 * rate, dims)
+    nv = ref.compute_surface(iso)
+    return nv, ref.results()["verts_f32"]
+
+
+@pytest.mark.parametrize("pool_blocks", [100000, 3000])
+def test_batched_queries_match_single_queries(dev, pool_blocks):
+    say("engine pool", pool_blocks)
+    streams = _streams()
+    eng = bcmc_amd.Engine(dev, pool_blocks * 256)
+    ids = [eng.add_volume(s, r, d) for s, r, d in streams]
+    rng = np.random.default_rng(pool_blocks)
+    cache = {}
+    for step in range(4):
+        nq = int(rng.integers(1, 12))
+        qv = rng.integers(0, len(ids), nq)
+        qi = rng.uniform(0.15, 0.85, nq).astype(np.float32)
+        res = eng.run(qv, qi)
+        for k in range(nq):
+            s, r, d = streams[qv[k]]
+            key = (int(qv[k]), float(qi[k]))
+            if key not in cache:
+                cache[key] = _oracle(s, r, d, float(qi[k]))
+            nv, verts = cache[key]
+            assert res[k]["num_vertices"] == nv, (step, k)
+            got = eng.vertices(res[k])
+            assert np.array_equal(got.view(np.uint32), verts.view(np.uint32)), (step, k)
+    st = eng.stats()
+    assert st["pool_used"] <= st["pool_slots"]
+
+
+def test_pool_too_small_is_an_error(dev):
+    s, r, d = _streams()[1]
+    eng = bcmc_amd.Engine(dev, 64 * 256)
+    vid = eng.add_volume(s, r, d)
+    with pytest.raises(bcmc_amd.BcmcError) as e:
+        eng.run([vid], [0.5])
+    assert e.value.code == -3

@@ -178,35 +178,63 @@ BCMC_DEV uint32_t block_excl_scan(uint32_t v, uint32_t *lds_waves /* NT/64 + 1 *
     return before + incl - v;
 }
 
-// Decoupled look-back across tiles.  Called by all threads of the workgroup
-// after the tile aggregate is known; returns the exclusive prefix of the tile
-// (64-bit; status values are exact below 2^40).  The last tile to finish its
-// lookback publishes an inclusive status that later tiles consume.
+BCMC_DEV unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Decoupled look-back across tiles, done by wave 0 of the workgroup.  Each
+// round reads the statuses of the 64 nearest unvisited predecessors at once
+// (one agent-scope load per lane): if every predecessor up to and including
+// the closest inclusive one is ready, their values are summed and the walk
+// ends at that inclusive one; with no inclusive status in the window (all
+// aggregates) the window moves back 64 tiles.  Returns the exclusive prefix
+// of the tile (status values are exact below 2^40) and publishes the tile's
+// inclusive value.
 template <int NT>
 BCMC_DEV unsigned long long tile_lookback(const ScanSite &site, uint32_t tile, uint32_t epoch,
                                           uint32_t aggregate, unsigned long long *lds_prefix) {
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
+        const uint32_t lane = lane_id();
+        const uint32_t ep = epoch & 0x3FFFFF;
         unsigned long long prefix = 0;
         if (tile == 0) {
-            st_release_u64(&site.status[0], pack_status(epoch, kFlagInclusive, aggregate));
+            if (lane == 0)
+                st_release_u64(&site.status[0], pack_status(ep, kFlagInclusive, aggregate));
         } else {
-            st_release_u64(&site.status[tile], pack_status(epoch, kFlagAggregate, aggregate));
-            int32_t t = (int32_t)tile - 1;
-            while (t >= 0) {
-                unsigned long long s = ld_acquire_u64(&site.status[t]);
-                const uint32_t ep = (uint32_t)(s >> 42), flag = (uint32_t)(s >> 40) & 3u;
-                if (ep != (epoch & 0x3FFFFF) || flag == 0) {
+            if (lane == 0)
+                st_release_u64(&site.status[tile], pack_status(ep, kFlagAggregate, aggregate));
+            int64_t base = (int64_t)tile - 1;
+            for (;;) {
+                const int64_t t = base - (int64_t)lane;
+                unsigned long long s = 0;
+                uint32_t flag = kFlagInclusive;  // before tile 0: an inclusive zero
+                if (t >= 0) {
+                    s = ld_acquire_u64(&site.status[t]);
+                    flag = ((uint32_t)(s >> 42) == ep) ? (uint32_t)(s >> 40) & 3u : 0u;
+                }
+                const uint64_t incl = __ballot(flag == kFlagInclusive);
+                const uint64_t wait = __ballot(flag == 0);
+                const uint32_t first = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
+                const uint64_t need = first == 64 ? ~0ull : ((2ull << first) - 1ull);
+                if (wait & need) {
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                prefix += s & kValueMask;
-                if (flag == kFlagInclusive)
+                const unsigned long long v = (lane <= first && t >= 0) ? (s & kValueMask) : 0ull;
+                prefix += wave_sum_u64(v);
+                if (first < 64)
                     break;
-                --t;
+                base -= 64;
             }
-            st_release_u64(&site.status[tile], pack_status(epoch, kFlagInclusive, prefix + aggregate));
+            if (lane == 0)
+                st_release_u64(&site.status[tile],
+                               pack_status(ep, kFlagInclusive, prefix + aggregate));
         }
-        *lds_prefix = prefix;
+        if (lane == 0)
+            *lds_prefix = prefix;
     }
     __syncthreads();
     const unsigned long long p = *lds_prefix;

@@ -1,0 +1,716 @@
+// Batched and served forms of the marching-cubes path (no reference
+// counterpart: the reference answers one query per computeSurface call,
+// js/compressed_marching_cubes.js:580-809).
+//
+// An engine holds device-resident compressed volumes and one decoded-block
+// pool shared by all of them.  A step answers many (volume, isovalue) queries
+// at once; every stage is one launch whose persistent workgroups pull work
+// from lists that concatenate all queries of the step:
+//   code     per block of every query: the active-test code byte
+//   active   single-pass scan + compaction over all queries' blocks ->
+//            (query, block) pairs in query order, ascending blocks
+//   mark     pairs whose block is pooled stamp its slot with the step epoch;
+//            missing blocks are claimed once per volume (CAS) and listed
+//   alloc    misses take free pool slots, then the least recently stamped
+//            slots not used by this or the previous step (radix-sorted by
+//            stamp), evicting their blocks
+//   decode   one launch per volume of the step (its zfp rate selects the
+//            LDS-staged decoder variant)
+//   mc       the chunked count / look-back / emit pass over all pairs; each
+//            query's vertices come out contiguous, in the same order as a
+//            single-query computeSurface
+// Query results do not depend on which other queries share the step or on
+// the pool state: the decoded values of a block are a pure function of the
+// stream.
+#include "common.h"
+#include "kernels.h"
+#include "mc_device.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace bcmc {
+
+struct EngVol {                 // device descriptor of a volume
+    uint32_t nb[3];
+    uint32_t n;
+    const float2 *ranges;
+    int32_t *slot_map;          // block -> pool slot; -1 none, -2 claimed this step
+    uint32_t *miss_list;        // this step's missing blocks of the volume (decoder input)
+};
+
+struct EngQuery {
+    uint32_t vol;
+    float iso;
+    uint32_t flag_base;         // first index of this query in the concatenated block arrays
+    uint32_t n;
+};
+
+struct EngCounters {
+    uint32_t epoch;
+    uint32_t num_active;
+    uint32_t num_with;
+    uint32_t num_miss;
+    uint32_t used;              // pool slots ever handed out (high-water mark)
+    uint32_t num_cand;          // eviction candidates
+    uint32_t short_slots;       // misses that found no slot (admission failed)
+    uint32_t pad;
+    unsigned long long num_verts;
+    uint32_t vol_miss[64];      // misses per volume
+};
+
+constexpr uint32_t kMaxVols = 64;
+
+// query of a concatenated index (flag_base ascending)
+BCMC_DEV uint32_t query_of(const EngQuery *qs, uint32_t nq, uint32_t g) {
+    uint32_t lo = 0, hi = nq;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (qs[mid].flag_base <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ void eng_zero_u64_kernel(unsigned long long *p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+__global__ void eng_begin_kernel(EngCounters *c) {
+    c->epoch += 1;
+    c->num_active = 0;
+    c->num_with = 0;
+    c->num_miss = 0;
+    c->num_cand = 0;
+    c->short_slots = 0;
+    c->num_verts = 0;
+    for (uint32_t v = 0; v < kMaxVols; ++v)
+        c->vol_miss[v] = 0;
+}
+
+__global__ void eng_code_kernel(const EngQuery *qs, uint32_t nq, const EngVol *vols, uint32_t total,
+                                uint8_t *code) {
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+        const EngQuery q = qs[query_of(qs, nq, g)];
+        const EngVol &v = vols[q.vol];
+        const uint32_t b = g - q.flag_base;
+        code[g] = (uint8_t)active_code(v.ranges, b, b % v.nb[0], v.nb[0], q.iso);
+    }
+}
+
+struct EngActiveOp {
+    const EngQuery *qs;
+    uint32_t nq;
+    const EngVol *vols;
+    const uint8_t *code;
+    uint32_t *flags, *act_q, *act_b, *q_act_start;
+    EngCounters *c;
+    uint32_t nblocks;
+    BCMC_DEV uint32_t size() const { return nblocks; }
+    BCMC_DEV void begin() const {}
+    BCMC_DEV void value(uint32_t g, uint32_t (&v)[1]) const {
+        const uint32_t qi = query_of(qs, nq, g);
+        const EngQuery q = qs[qi];
+        const EngVol &vol = vols[q.vol];
+        v[0] = active_from_codes(code + q.flag_base, g - q.flag_base, vol.nb);
+    }
+    BCMC_DEV void emit(uint32_t g, const uint32_t (&v)[1], const unsigned long long (&e)[1]) const {
+        const uint32_t qi = query_of(qs, nq, g);
+        const uint32_t b = g - qs[qi].flag_base;
+        flags[g] = v[0];
+        if (b == 0)
+            q_act_start[qi] = (uint32_t)e[0];
+        if (v[0]) {
+            act_q[e[0]] = qi;
+            act_b[e[0]] = b;
+        }
+    }
+    BCMC_DEV void total(const unsigned long long (&t)[1]) const { c->num_active = (uint32_t)t[0]; }
+};
+
+// stamp pooled blocks, claim missing ones
+__global__ void eng_mark_kernel(const EngQuery *qs, const EngVol *vols, const uint32_t *act_q,
+                                const uint32_t *act_b, EngCounters *c, uint32_t *stamp,
+                                unsigned long long *miss, uint32_t miss_cap) {
+    const uint32_t n = c->num_active, ep = c->epoch;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const uint32_t vi = qs[act_q[e]].vol, b = act_b[e];
+        int32_t *sm = vols[vi].slot_map;
+        const int32_t s = __hip_atomic_load(&sm[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (s >= 0) {
+            stamp[s] = ep;
+        } else if (s == -1) {
+            int32_t expect = -1;
+            if (__hip_atomic_compare_exchange_strong(&sm[b], &expect, -2, __ATOMIC_RELAXED,
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                const uint32_t w = atomic_add_u32(&c->num_miss, 1u);
+                const uint32_t wv = atomic_add_u32(&c->vol_miss[vi], 1u);
+                if (w < miss_cap)
+                    miss[w] = ((unsigned long long)vi << 32) | b;
+                if (wv < vols[vi].n)
+                    vols[vi].miss_list[wv] = b;
+            }
+        }
+    }
+}
+
+// eviction candidates: pooled slots not stamped by this or the previous step
+struct CandOp {
+    const uint32_t *stamp;
+    uint32_t *cand;
+    EngCounters *c;
+    uint32_t capacity;
+    // only when the misses exceed the free room; only handed-out slots
+    BCMC_DEV uint32_t size() const { return c->num_miss > capacity - c->used ? c->used : 0u; }
+    BCMC_DEV void begin() const {}
+    BCMC_DEV void value(uint32_t s, uint32_t (&v)[1]) const {
+        v[0] = stamp[s] + 1 < c->epoch ? 1u : 0u;
+    }
+    BCMC_DEV void emit(uint32_t s, const uint32_t (&v)[1], const unsigned long long (&e)[1]) const {
+        if (v[0])
+            cand[e[0]] = s;
+    }
+    BCMC_DEV void total(const unsigned long long (&t)[1]) const { c->num_cand = (uint32_t)t[0]; }
+};
+
+// keys for the eviction order: stamp (ascending = least recently used first)
+__global__ void eng_cand_keys_kernel(const uint32_t *stamp, const uint32_t *cand,
+                                     const EngCounters *c, uint32_t *keys) {
+    const uint32_t n = c->num_cand;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        keys[i] = stamp[cand[i]];
+}
+
+// miss w takes free slot used+w while the pool has room, then the w'-th
+// eviction candidate in LRU order; its old block leaves the pool
+__global__ void eng_assign_kernel(const EngVol *vols, EngCounters *c, const unsigned long long *miss,
+                                  const uint32_t *order, uint32_t capacity, uint32_t *stamp,
+                                  unsigned long long *owner) {
+    const uint32_t m = c->num_miss, used = c->used, ep = c->epoch;
+    const uint32_t room = capacity - used;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < m; w += gridDim.x * blockDim.x) {
+        const unsigned long long pr = miss[w];
+        const uint32_t vi = (uint32_t)(pr >> 32), b = (uint32_t)pr;
+        uint32_t s;
+        if (w < room) {
+            s = used + w;
+        } else {
+            const uint32_t k = w - room;
+            if (k >= c->num_cand) {
+                atomic_add_u32(&c->short_slots, 1u);
+                vols[vi].slot_map[b] = -1;
+                continue;
+            }
+            s = order[k];
+            const unsigned long long old = owner[s];
+            if (old != ~0ull)
+                vols[(uint32_t)(old >> 32)].slot_map[(uint32_t)old] = -1;
+        }
+        owner[s] = pr;
+        stamp[s] = ep;
+        vols[vi].slot_map[b] = (int32_t)s;
+    }
+}
+
+__global__ void eng_commit_kernel(EngCounters *c, uint32_t capacity) {
+    const uint32_t room = capacity - c->used;
+    c->used += min(room, c->num_miss);
+}
+
+struct EngFusedArgs {
+    const EngQuery *qs;
+    const EngVol *vols;
+    const uint32_t *act_q, *act_b;
+    const uint32_t *flags;
+    const float *pool;
+    uint32_t capacity;
+    EngCounters *c;
+    unsigned long long *q_verts;  // per-query vertex counts
+    unsigned long long *vert_offsets;
+    void *verts;
+    unsigned long long vert_cap;
+    int fmt;
+};
+
+BCMC_DEV McView eng_view(const EngFusedArgs &a, uint32_t qi) {
+    const EngQuery &q = a.qs[qi];
+    const EngVol &v = a.vols[q.vol];
+    McView m;
+    m.cache = a.pool;
+    m.item_slot = v.slot_map;
+    m.flags = a.flags + q.flag_base;
+    m.nb[0] = v.nb[0];
+    m.nb[1] = v.nb[1];
+    m.nb[2] = v.nb[2];
+    m.capacity = a.capacity;
+    return m;
+}
+
+constexpr int kEngThreads = 256;
+constexpr int kEngBlocks = 4;
+constexpr uint32_t kEngTile = kEngThreads;
+
+// The chunked count / look-back / emit pass of mc.hip over (query, block)
+// pairs of many queries; the vertex channel's offsets are global, and each
+// query's vertices are contiguous because pairs are in query order.
+__global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, EngFusedArgs a) {
+    constexpr int NW = kEngThreads / 64;
+    constexpr uint32_t kRound = NW * kEngBlocks;
+    __shared__ float tiles[NW][kEngBlocks * 128];
+    __shared__ int32_t slots[NW][kEngBlocks * 8];
+    __shared__ uint32_t cnt_s[kEngTile], list_s[kEngTile], off_s[kEngTile];
+    __shared__ uint32_t lds_w[NW + 1];
+    __shared__ unsigned long long lds_prefix;
+    __shared__ uint32_t lds_tile;
+    const uint32_t n = a.c->num_active;
+    const uint32_t ntiles = div_up(n, kEngTile);
+    const uint32_t e0 = scan_site_epoch(s0);
+    const uint32_t wave = threadIdx.x / 64, lane = lane_id();
+    for (;;) {
+        const uint32_t tile = next_tile(s0, &lds_tile);
+        if (tile >= ntiles)
+            break;
+        const uint32_t k0 = tile * kEngTile;
+        for (uint32_t r = 0; r < kEngTile; r += kRound) {
+            const uint32_t kk = r + wave * kEngBlocks;
+            uint32_t bid[kEngBlocks], d[kEngBlocks][3], qi[kEngBlocks];
+            bool live[kEngBlocks];
+            PerBlockViews<kEngBlocks> mv;
+#pragma unroll
+            for (int j = 0; j < kEngBlocks; ++j) {
+                live[j] = k0 + kk + j < n;
+                qi[j] = live[j] ? a.act_q[k0 + kk + j] : 0u;
+                bid[j] = live[j] ? a.act_b[k0 + kk + j] : 0u;
+                mv.m[j] = eng_view(a, qi[j]);
+            }
+            load_tiles<kEngBlocks>(mv, bid, live, tiles[wave], slots[wave], d);
+#pragma unroll
+            for (int j = 0; j < kEngBlocks; ++j) {
+                const uint32_t c = live[j] ? block_vertex_count(tiles[wave] + 128 * j, d[j], a.qs[qi[j]].iso)
+                                           : 0u;
+                if (lane == 0) {
+                    cnt_s[kk + j] = c;
+                    if (c)
+                        atomicAdd(&a.q_verts[qi[j]], (unsigned long long)c);
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t c = cnt_s[threadIdx.x];
+        uint32_t agg_h, agg_v;
+        const uint32_t ex_h = block_excl_scan<kEngThreads>(c ? 1u : 0u, lds_w, &agg_h);
+        const uint32_t ex_v = block_excl_scan<kEngThreads>(c, lds_w, &agg_v);
+        const unsigned long long pv = tile_lookback<kEngThreads>(s0, tile, e0, agg_v, &lds_prefix);
+        if (c) {
+            list_s[ex_h] = threadIdx.x;
+            off_s[ex_h] = ex_v;
+        }
+        if (tile == ntiles - 1 && threadIdx.x == 0)
+            a.c->num_verts = pv + agg_v;
+        __syncthreads();
+        for (uint32_t r = 0; r < agg_h; r += kRound) {
+            const uint32_t hh = r + wave * kEngBlocks;
+            uint32_t bid[kEngBlocks], d[kEngBlocks][3], loc[kEngBlocks], qi[kEngBlocks];
+            bool live[kEngBlocks];
+            PerBlockViews<kEngBlocks> mv;
+#pragma unroll
+            for (int j = 0; j < kEngBlocks; ++j) {
+                live[j] = hh + j < agg_h;
+                loc[j] = live[j] ? list_s[hh + j] : 0u;
+                qi[j] = live[j] ? a.act_q[k0 + loc[j]] : 0u;
+                bid[j] = live[j] ? a.act_b[k0 + loc[j]] : 0u;
+                mv.m[j] = eng_view(a, qi[j]);
+            }
+            load_tiles<kEngBlocks>(mv, bid, live, tiles[wave], slots[wave], d);
+#pragma unroll
+            for (int j = 0; j < kEngBlocks; ++j) {
+                if (!live[j])
+                    continue;
+                const unsigned long long off = pv + off_s[hh + j];
+                if (off + cnt_s[loc[j]] <= a.vert_cap)
+                    emit_block_vertices(tiles[wave] + 128 * j, d[j], a.qs[qi[j]].iso, bid[j], mv.m[j].nb,
+                                        off, a.verts, a.fmt);
+            }
+        }
+        __syncthreads();
+    }
+    scan_site_finish(s0, e0);
+}
+
+static uint32_t vbytes(int fmt) {
+    return fmt == BCMC_VERTS_LOCAL_F32 ? 16u : fmt == BCMC_VERTS_GLOBAL_F32 ? 12u : 8u;
+}
+
+}  // namespace bcmc
+
+using namespace bcmc;
+
+// ---------------------------------------------------------------------------
+// Host objects
+// ---------------------------------------------------------------------------
+struct EngVolume {
+    uint32_t dims[3], nb[3], n, maxbits;
+    uint64_t *stream = nullptr;
+    uint64_t nwords = 0;
+    float2 *ranges = nullptr;
+    int32_t *slot_map = nullptr;
+    uint32_t *miss_list = nullptr;  // per-step block ids to decode (EngVol::miss_list)
+};
+
+struct bcmc_engine {
+    bcmc_ctx *ctx = nullptr;
+    std::vector<EngVolume> vols;
+    EngVol *d_vols = nullptr;
+    uint32_t pool_cap = 0;
+    float *pool = nullptr;
+    uint32_t *stamp = nullptr, *cand = nullptr, *cand_keys = nullptr, *order = nullptr;
+    unsigned long long *owner = nullptr, *miss = nullptr;
+    uint32_t miss_cap = 0;
+    SortScratch sort;
+    EngCounters *c = nullptr, *ch = nullptr;
+    // per-step arrays (grow)
+    uint64_t blocks_cap = 0;
+    uint32_t *flags = nullptr, *act_q = nullptr, *act_b = nullptr;
+    uint8_t *code = nullptr;
+    unsigned long long *vert_offsets = nullptr;
+    uint32_t q_cap = 0;
+    EngQuery *d_q = nullptr;
+    uint32_t *q_act_start = nullptr;
+    unsigned long long *q_verts = nullptr;
+    ScanSite site_active, site_cand, site_mc;
+    void *verts = nullptr;
+    uint64_t vert_cap = 0;
+    int fmt = BCMC_VERTS_LOCAL_F32;
+    bcmc_buf vert_view;
+    uint64_t budget = 0;
+    // last step
+    std::vector<bcmc_query_result> last;
+};
+
+static int eng_ensure_blocks(bcmc_engine *e, uint64_t total) {
+    if (total <= e->blocks_cap)
+        return BCMC_OK;
+    BCMC_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
+    dev_free(e->flags);
+    dev_free(e->act_q);
+    dev_free(e->act_b);
+    dev_free(e->code);
+    dev_free(e->vert_offsets);
+    scan_site_free(&e->site_active);
+    scan_site_free(&e->site_mc);
+    const uint64_t cap = std::max<uint64_t>(total, e->blocks_cap * 3 / 2);
+    BCMC_TRY(dev_alloc((void **)&e->flags, cap * 4));
+    BCMC_TRY(dev_alloc((void **)&e->act_q, cap * 4));
+    BCMC_TRY(dev_alloc((void **)&e->act_b, cap * 4));
+    BCMC_TRY(dev_alloc((void **)&e->code, cap));
+    BCMC_TRY(dev_alloc((void **)&e->vert_offsets, 8));
+    BCMC_TRY(scan_site_alloc(&e->site_active, cap, kScanTile));
+    BCMC_TRY(scan_site_alloc(&e->site_mc, cap, kEngTile));
+    e->blocks_cap = cap;
+    return BCMC_OK;
+}
+
+static int eng_ensure_queries(bcmc_engine *e, uint32_t nq) {
+    if (nq <= e->q_cap)
+        return BCMC_OK;
+    BCMC_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
+    dev_free(e->d_q);
+    dev_free(e->q_act_start);
+    dev_free(e->q_verts);
+    const uint32_t cap = std::max<uint32_t>(nq, 64);
+    BCMC_TRY(dev_alloc((void **)&e->d_q, (size_t)cap * sizeof(EngQuery)));
+    BCMC_TRY(dev_alloc((void **)&e->q_act_start, (size_t)cap * 4));
+    BCMC_TRY(dev_alloc((void **)&e->q_verts, (size_t)cap * 8));
+    e->q_cap = cap;
+    return BCMC_OK;
+}
+
+static uint32_t eng_grid(const bcmc_ctx *ctx, uint64_t n, uint32_t per_cu = 8) {
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(div_up(n, 256), persistent_grid(ctx, per_cu)));
+}
+
+static int eng_launch_fused(bcmc_engine *e, uint32_t nq, uint64_t total) {
+    EngFusedArgs a;
+    a.qs = e->d_q;
+    a.vols = e->d_vols;
+    a.act_q = e->act_q;
+    a.act_b = e->act_b;
+    a.flags = e->flags;
+    a.pool = e->pool;
+    a.capacity = e->pool_cap;
+    a.c = e->c;
+    a.q_verts = e->q_verts;
+    a.vert_offsets = e->vert_offsets;
+    a.verts = e->verts;
+    a.vert_cap = e->vert_cap;
+    a.fmt = e->fmt;
+    eng_zero_u64_kernel<<<1, 256, 0, e->ctx->stream>>>(e->q_verts, nq);
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(total, kEngTile),
+                                                                   persistent_grid(e->ctx, 8)));
+    eng_fused_kernel<<<grid, kEngThreads, 0, e->ctx->stream>>>(e->site_mc, a);
+    BCMC_HIP_CHECK(hipGetLastError());
+    return BCMC_OK;
+}
+
+extern "C" {
+
+BCMC_API int bcmc_engine_create(bcmc_ctx *ctx, uint64_t pool_bytes, bcmc_engine **out) {
+    BCMC_CHECK_ARG(ctx && out, "null argument");
+    BCMC_CHECK_ARG(pool_bytes >= 256 * 64, "decoded-block pool must hold at least 64 blocks");
+    *out = nullptr;
+    bcmc_engine *e = new bcmc_engine;
+    e->ctx = ctx;
+    e->budget = pool_bytes;
+    e->pool_cap = (uint32_t)std::min<uint64_t>(pool_bytes / 256, 0x7FFFFFC0ull);
+    int rc = dev_alloc((void **)&e->pool, (size_t)e->pool_cap * 256);
+    if (rc == BCMC_OK)
+        rc = dev_alloc((void **)&e->stamp, (size_t)e->pool_cap * 4);
+    if (rc == BCMC_OK)
+        rc = dev_alloc((void **)&e->owner, (size_t)e->pool_cap * 8);
+    if (rc == BCMC_OK)
+        rc = dev_alloc((void **)&e->cand, (size_t)e->pool_cap * 4);
+    if (rc == BCMC_OK)
+        rc = dev_alloc((void **)&e->cand_keys, (size_t)e->pool_cap * 4);
+    if (rc == BCMC_OK)
+        rc = dev_alloc((void **)&e->order, (size_t)e->pool_cap * 4);
+    if (rc == BCMC_OK)
+        rc = dev_alloc((void **)&e->d_vols, kMaxVols * sizeof(EngVol));
+    if (rc == BCMC_OK)
+        rc = dev_alloc((void **)&e->c, sizeof(EngCounters));
+    if (rc == BCMC_OK)
+        rc = sort_scratch_alloc(&e->sort, e->pool_cap);
+    if (rc == BCMC_OK)
+        rc = scan_site_alloc(&e->site_cand, e->pool_cap, kScanTile);
+    if (rc == BCMC_OK && hipHostMalloc(&e->ch, sizeof(EngCounters), hipHostMallocDefault) != hipSuccess)
+        rc = BCMC_ERR_NOMEM;
+    if (rc != BCMC_OK) {
+        bcmc_engine_destroy(e);
+        return rc;
+    }
+    BCMC_HIP_CHECK(hipMemsetAsync(e->c, 0, sizeof(EngCounters), ctx->stream));
+    BCMC_HIP_CHECK(hipMemsetAsync(e->stamp, 0, (size_t)e->pool_cap * 4, ctx->stream));
+    BCMC_HIP_CHECK(hipMemsetAsync(e->owner, 0xFF, (size_t)e->pool_cap * 8, ctx->stream));
+    BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    *out = e;
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_engine_destroy(bcmc_engine *e) {
+    if (!e)
+        return BCMC_OK;
+    (void)hipStreamSynchronize(e->ctx->stream);
+    for (auto &v : e->vols) {
+        dev_free(v.stream);
+        dev_free(v.ranges);
+        dev_free(v.slot_map);
+        dev_free(v.miss_list);
+    }
+    dev_free(e->d_vols);
+    dev_free(e->pool);
+    dev_free(e->stamp);
+    dev_free(e->owner);
+    dev_free(e->cand);
+    dev_free(e->cand_keys);
+    dev_free(e->order);
+    dev_free(e->miss);
+    sort_scratch_free(&e->sort);
+    dev_free(e->c);
+    dev_free(e->flags);
+    dev_free(e->act_q);
+    dev_free(e->act_b);
+    dev_free(e->code);
+    dev_free(e->vert_offsets);
+    dev_free(e->d_q);
+    dev_free(e->q_act_start);
+    dev_free(e->q_verts);
+    dev_free(e->verts);
+    scan_site_free(&e->site_active);
+    scan_site_free(&e->site_cand);
+    scan_site_free(&e->site_mc);
+    if (e->ch)
+        (void)hipHostFree(e->ch);
+    delete e;
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t bytes, double rate,
+                                    const uint32_t dims[3], uint32_t *volume_id) {
+    BCMC_CHECK_ARG(e && stream && dims && volume_id, "null argument");
+    BCMC_CHECK_ARG(e->vols.size() < kMaxVols, "too many volumes (max 64)");
+    const double mb = 64.0 * rate;
+    BCMC_CHECK_ARG(mb >= 9.0 && mb <= 4096.0 && mb == std::floor(mb),
+                   "64*compressionRate must be an integer in [9, 4096]");
+    EngVolume v;
+    for (int i = 0; i < 3; ++i) {
+        BCMC_CHECK_ARG(dims[i] > 0, "volume dims must be positive");
+        v.dims[i] = dims[i];
+        v.nb[i] = (dims[i] + 3) / 4;
+    }
+    const uint64_t n = (uint64_t)v.nb[0] * v.nb[1] * v.nb[2];
+    BCMC_CHECK_ARG(n < (1ull << 31), "volume has too many blocks");
+    v.n = (uint32_t)n;
+    v.maxbits = (uint32_t)mb;
+    BCMC_CHECK_ARG(bytes >= (n * v.maxbits + 7) / 8, "compressed stream shorter than the volume");
+    bcmc_ctx *ctx = e->ctx;
+    v.nwords = (bytes + 7) / 8 + 2;
+    BCMC_TRY(dev_alloc((void **)&v.stream, v.nwords * 8));
+    BCMC_HIP_CHECK(hipMemsetAsync(v.stream, 0, v.nwords * 8, ctx->stream));
+    BCMC_HIP_CHECK(hipMemcpyAsync(v.stream, stream, bytes, hipMemcpyHostToDevice, ctx->stream));
+    BCMC_TRY(dev_alloc((void **)&v.ranges, n * 8));
+    BCMC_TRY(dev_alloc((void **)&v.slot_map, n * 4));
+    BCMC_TRY(dev_alloc((void **)&v.miss_list, n * 4));
+    BCMC_HIP_CHECK(hipMemsetAsync(v.slot_map, 0xFF, n * 4, ctx->stream));
+    launch_block_ranges(ctx, v.stream, v.nwords, v.maxbits, v.dims, v.ranges);
+    BCMC_HIP_CHECK(hipGetLastError());
+    EngVol dv;
+    for (int i = 0; i < 3; ++i)
+        dv.nb[i] = v.nb[i];
+    dv.n = v.n;
+    dv.ranges = v.ranges;
+    dv.slot_map = v.slot_map;
+    dv.miss_list = v.miss_list;
+    const uint32_t id = (uint32_t)e->vols.size();
+    BCMC_HIP_CHECK(hipMemcpyAsync(e->d_vols + id, &dv, sizeof(dv), hipMemcpyHostToDevice, ctx->stream));
+    BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    e->vols.push_back(v);
+    // miss list capacity: every block of every volume
+    uint64_t all = 0;
+    for (auto &w : e->vols)
+        all += w.n;
+    dev_free(e->miss);
+    e->miss = nullptr;
+    BCMC_TRY(dev_alloc((void **)&e->miss, all * 8));
+    e->miss_cap = (uint32_t)std::min<uint64_t>(all, 0xFFFFFFFFu);
+    *volume_id = id;
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_engine_set_vertex_format(bcmc_engine *e, bcmc_vertex_format fmt) {
+    BCMC_CHECK_ARG(e, "null argument");
+    BCMC_CHECK_ARG(fmt >= BCMC_VERTS_LOCAL_F32 && fmt <= BCMC_VERTS_PACKED_U32X2, "bad format");
+    if (fmt != e->fmt) {
+        (void)hipStreamSynchronize(e->ctx->stream);
+        dev_free(e->verts);
+        e->verts = nullptr;
+        e->vert_cap = 0;
+        e->fmt = fmt;
+    }
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const float *isovalues,
+                             uint32_t nq, bcmc_query_result *results) {
+    BCMC_CHECK_ARG(e && (nq == 0 || (volume_ids && isovalues)), "null argument");
+    if (nq == 0)
+        return BCMC_OK;
+    bcmc_ctx *ctx = e->ctx;
+    hipStream_t st = ctx->stream;
+    std::vector<EngQuery> qs(nq);
+    std::vector<bool> used(e->vols.size(), false);
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < nq; ++i) {
+        BCMC_CHECK_ARG(volume_ids[i] < e->vols.size(), "unknown volume id");
+        const EngVolume &v = e->vols[volume_ids[i]];
+        qs[i] = EngQuery{volume_ids[i], isovalues[i], (uint32_t)total, v.n};
+        used[volume_ids[i]] = true;
+        total += v.n;
+    }
+    BCMC_CHECK_ARG(total < (1ull << 32) - kScanTile, "too many blocks in one step");
+    BCMC_TRY(eng_ensure_blocks(e, total));
+    BCMC_TRY(eng_ensure_queries(e, nq));
+    BCMC_HIP_CHECK(hipMemcpyAsync(e->d_q, qs.data(), nq * sizeof(EngQuery), hipMemcpyHostToDevice, st));
+    eng_begin_kernel<<<1, 1, 0, st>>>(e->c);
+    eng_code_kernel<<<eng_grid(ctx, total, 16), 256, 0, st>>>(e->d_q, nq, e->d_vols, (uint32_t)total,
+                                                                e->code);
+    EngActiveOp aop{e->d_q, nq, e->d_vols, e->code, e->flags, e->act_q, e->act_b, e->q_act_start, e->c,
+                    (uint32_t)total};
+    launch_scan<1>(ctx, e->site_active, e->site_active, aop, total);
+    eng_mark_kernel<<<eng_grid(ctx, total), 256, 0, st>>>(e->d_q, e->d_vols, e->act_q, e->act_b, e->c,
+                                                          e->stamp, e->miss, e->miss_cap);
+    // eviction order (only does work when the misses exceed the free room)
+    launch_scan<1>(ctx, e->site_cand, e->site_cand, CandOp{e->stamp, e->cand, e->c, e->pool_cap},
+                   e->pool_cap);
+    eng_cand_keys_kernel<<<eng_grid(ctx, e->pool_cap, 2), 256, 0, st>>>(e->stamp, e->cand, e->c,
+                                                                         e->cand_keys);
+    launch_sort_pairs(ctx, &e->sort, e->cand_keys, e->cand, &e->c->num_cand, e->pool_cap, e->order);
+    eng_assign_kernel<<<eng_grid(ctx, total, 4), 256, 0, st>>>(e->d_vols, e->c, e->miss, e->order,
+                                                                e->pool_cap, e->stamp, e->owner);
+    eng_commit_kernel<<<1, 1, 0, st>>>(e->c, e->pool_cap);
+    // decode the misses of each volume into their slots
+    for (uint32_t vi = 0; vi < e->vols.size(); ++vi) {
+        if (!used[vi])
+            continue;
+        EngVolume &v = e->vols[vi];
+        launch_decode_to_cache(ctx, v.stream, v.nwords, v.maxbits, v.miss_list, &e->c->vol_miss[vi],
+                               v.n, v.slot_map, e->pool, e->pool_cap);
+    }
+    BCMC_TRY(eng_launch_fused(e, nq, total));
+    BCMC_HIP_CHECK(hipMemcpyAsync(e->ch, e->c, sizeof(EngCounters), hipMemcpyDeviceToHost, st));
+    std::vector<unsigned long long> qv(nq);
+    std::vector<uint32_t> qa(nq);
+    BCMC_HIP_CHECK(hipMemcpyAsync(qv.data(), e->q_verts, nq * 8, hipMemcpyDeviceToHost, st));
+    BCMC_HIP_CHECK(hipMemcpyAsync(qa.data(), e->q_act_start, nq * 4, hipMemcpyDeviceToHost, st));
+    BCMC_HIP_CHECK(hipStreamSynchronize(st));
+    if (e->ch->short_slots) {
+        set_error("decoded-block pool too small for this step: %u blocks found no slot",
+                  e->ch->short_slots);
+        return BCMC_ERR_NOMEM;
+    }
+    const uint64_t nv = e->ch->num_verts;
+    if (nv > e->vert_cap) {
+        const uint64_t want = std::max<uint64_t>(nv, e->vert_cap * 3 / 2);
+        dev_free(e->verts);
+        e->verts = nullptr;
+        e->vert_cap = 0;
+        BCMC_TRY(dev_alloc(&e->verts, want * vbytes(e->fmt)));
+        e->vert_cap = want;
+        BCMC_TRY(eng_launch_fused(e, nq, total));
+        BCMC_HIP_CHECK(hipMemcpyAsync(qv.data(), e->q_verts, nq * 8, hipMemcpyDeviceToHost, st));
+        BCMC_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    e->last.resize(nq);
+    uint64_t base = 0;
+    const uint32_t na = e->ch->num_active;
+    for (uint32_t i = 0; i < nq; ++i) {
+        bcmc_query_result r;
+        r.num_vertices = qv[i];
+        r.vertex_offset = base;
+        r.num_active_blocks = (i + 1 < nq ? qa[i + 1] : na) - qa[i];
+        r.step_decoded_blocks = e->ch->num_miss;
+        base += qv[i];
+        e->last[i] = r;
+        if (results)
+            results[i] = r;
+    }
+    e->vert_view.ctx = ctx;
+    e->vert_view.ptr = e->verts;
+    e->vert_view.bytes = nv * vbytes(e->fmt);
+    e->vert_view.owned = false;
+    return BCMC_OK;
+}
+
+BCMC_API bcmc_buf *bcmc_engine_vertex_buffer(bcmc_engine *e) { return e ? &e->vert_view : nullptr; }
+
+BCMC_API int bcmc_engine_stats(bcmc_engine *e, bcmc_engine_info *info) {
+    BCMC_CHECK_ARG(e && info, "null argument");
+    BCMC_HIP_CHECK(hipMemcpyAsync(e->ch, e->c, sizeof(EngCounters), hipMemcpyDeviceToHost, e->ctx->stream));
+    BCMC_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
+    info->num_volumes = (uint32_t)e->vols.size();
+    info->pool_slots = e->pool_cap;
+    info->pool_used = e->ch->used;
+    info->epoch = e->ch->epoch;
+    info->pool_bytes = (uint64_t)e->pool_cap * 256;
+    uint64_t dev = info->pool_bytes + (uint64_t)e->pool_cap * 20 + e->blocks_cap * 13 + e->vert_cap * vbytes(e->fmt);
+    for (auto &v : e->vols)
+        dev += v.nwords * 8 + (uint64_t)v.n * 16;
+    info->device_bytes = dev;
+    return BCMC_OK;
+}
+
+}  // extern "C"

@@ -69,24 +69,27 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanSite s0, ScanSit
             break;
         const uint32_t wbase = tile * kScanTile + wave * (64 * kScanItems);
         uint32_t vals[kScanItems][NCH], excl[kScanItems][NCH], run[NCH];
+        // all loads of the tile first (independent), then the scans
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            const uint32_t i = wbase + j * 64 + lane;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c)
+                vals[j][c] = 0;
+            if (i < n)
+                op.value(i, vals[j]);
+        }
 #pragma unroll
         for (int c = 0; c < NCH; ++c)
             run[c] = 0;
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
-            const uint32_t i = wbase + j * 64 + lane;
-            uint32_t v[NCH];
-#pragma unroll
-            for (int c = 0; c < NCH; ++c)
-                v[c] = 0;
-            if (i < n)
-                op.value(i, v);
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
-                const uint32_t inc = wave_incl_scan(v[c]);
-                excl[j][c] = run[c] + inc - v[c];
+                const uint32_t v = vals[j][c];
+                const uint32_t inc = wave_incl_scan(v);
+                excl[j][c] = run[c] + inc - v;
                 run[c] += wave_read_lane(inc, 63);
-                vals[j][c] = v[c];
             }
         }
         unsigned long long prefix[NCH];
@@ -173,6 +176,10 @@ void sort_scratch_free(SortScratch *s);
 void launch_lru_order(bcmc_ctx *ctx, SortScratch *s, const uint32_t *avail_ids,
                       const uint32_t *age, const uint32_t *count_dev, uint32_t count_max,
                       uint32_t *out);
+// Stable ascending sort of `count` (key, value) pairs (count read from device
+// memory); sorted values to out[0..count).
+void launch_sort_pairs(bcmc_ctx *ctx, SortScratch *s, const uint32_t *keys, const uint32_t *vals,
+                       const uint32_t *count_dev, uint32_t count_max, uint32_t *out);
 // RadixSorter.sort semantics for host-known n over an aligned buffer.
 void launch_sort_by_key(bcmc_ctx *ctx, SortScratch *s, uint32_t *keys, uint32_t *vals, uint32_t n,
                         int reverse);

@@ -61,15 +61,17 @@ struct ActiveOp {
 };
 
 // ---- count + offsets + emission in one pass --------------------------------
-// Workgroups take tiles of 16 active blocks in list order (4 waves x 4 blocks).
-// Each wave loads its 4 tiles with shared load rounds, counts the vertices of
-// each block, and the tile's totals (blocks with vertices, vertices) chain
-// through two look-back channels; each block then has its exact 64-bit offset
-// and writes its vertices straight away.  Blocks whose vertices would end past
-// the buffer capacity write nothing; the host grows the buffer and runs the
-// kernel again when the total exceeds the capacity.
+// Workgroups take chunks of 256 consecutive active blocks.  Sweep 1: each wave
+// loads 4 tiles at a time (shared load rounds) and counts their vertices into
+// LDS.  The chunk's two totals (blocks with vertices, vertices) chain through
+// two look-back channels, which gives every block its exact 64-bit vertex
+// offset and its index in the list of blocks with vertices.  Sweep 2: the
+// blocks with vertices reload their tiles (mostly from L2) and write their
+// vertices.  Blocks whose vertices would end past the buffer capacity write
+// nothing; the host grows the buffer and runs the pass again when the total
+// exceeds the capacity.
 constexpr int kFuseBlocks = 4;
-constexpr uint32_t kFuseTile = (kMcThreads / 64) * kFuseBlocks;
+constexpr uint32_t kFuseTile = kMcThreads;  // one block per thread in the chunk scan
 
 struct FusedArgs {
     McView m;
@@ -84,9 +86,11 @@ struct FusedArgs {
 
 __global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanSite s1, FusedArgs a) {
     constexpr int NW = kMcThreads / 64;
+    constexpr uint32_t kRound = NW * kFuseBlocks;
     __shared__ float tiles[NW][kFuseBlocks * kTileStride];
     __shared__ int32_t slots[NW][kFuseBlocks * 8];
-    __shared__ uint32_t lds_w[2][NW + 1];
+    __shared__ uint32_t cnt_s[kFuseTile], list_s[kFuseTile], off_s[kFuseTile];
+    __shared__ uint32_t lds_w[NW + 1];
     __shared__ unsigned long long lds_prefix[2];
     __shared__ uint32_t lds_tile;
     const uint32_t n = a.q->num_active;
@@ -94,6 +98,7 @@ __global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanS
     const uint32_t ntiles = div_up(n, kFuseTile);
     const uint32_t e0 = scan_site_epoch(s0), e1 = scan_site_epoch(s1);
     const uint32_t wave = threadIdx.x / 64, lane = lane_id();
+    const SameView<kFuseBlocks> mv{a.m};
     if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
         a.q->num_with = 0;
         a.q->num_verts = 0;
@@ -102,59 +107,67 @@ __global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanS
         const uint32_t tile = next_tile(s0, &lds_tile);
         if (tile >= ntiles)
             break;
-        const uint32_t k0 = tile * kFuseTile + wave * kFuseBlocks;
-        uint32_t bid[kFuseBlocks], cnt[kFuseBlocks], d[kFuseBlocks][3];
-        bool live[kFuseBlocks];
+        const uint32_t k0 = tile * kFuseTile;
+        // sweep 1: per-block vertex counts
+        for (uint32_t r = 0; r < kFuseTile; r += kRound) {
+            const uint32_t kk = r + wave * kFuseBlocks;
+            uint32_t bid[kFuseBlocks], d[kFuseBlocks][3];
+            bool live[kFuseBlocks];
 #pragma unroll
-        for (int j = 0; j < kFuseBlocks; ++j) {
-            live[j] = k0 + j < n;
-            bid[j] = live[j] ? a.active_ids[k0 + j] : 0u;
-        }
-        load_tiles<kFuseBlocks>(a.m, bid, live, tiles[wave], slots[wave], d);
-        uint32_t hs = 0, vs = 0;
+            for (int j = 0; j < kFuseBlocks; ++j) {
+                live[j] = k0 + kk + j < n;
+                bid[j] = live[j] ? a.active_ids[k0 + kk + j] : 0u;
+            }
+            load_tiles<kFuseBlocks>(mv, bid, live, tiles[wave], slots[wave], d);
 #pragma unroll
-        for (int j = 0; j < kFuseBlocks; ++j) {
-            cnt[j] = live[j] ? block_vertex_count(tiles[wave] + kTileStride * j, d[j], iso) : 0u;
-            hs += cnt[j] ? 1u : 0u;
-            vs += cnt[j];
-        }
-        if (lane == 0) {
-            lds_w[0][wave] = hs;
-            lds_w[1][wave] = vs;
+            for (int j = 0; j < kFuseBlocks; ++j) {
+                const uint32_t c = live[j] ? block_vertex_count(tiles[wave] + kTileStride * j, d[j], iso) : 0u;
+                if (lane == 0)
+                    cnt_s[kk + j] = c;
+            }
         }
         __syncthreads();
-        uint32_t before_h = 0, before_v = 0, agg_h = 0, agg_v = 0;
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            const uint32_t th = lds_w[0][w], tv = lds_w[1][w];
-            before_h += w < (int)wave ? th : 0u;
-            before_v += w < (int)wave ? tv : 0u;
-            agg_h += th;
-            agg_v += tv;
-        }
-        __syncthreads();
+        // chunk scan of (has vertices, vertices), then the two look-backs
+        const uint32_t c = cnt_s[threadIdx.x];
+        uint32_t agg_h, agg_v;
+        const uint32_t ex_h = block_excl_scan<kMcThreads>(c ? 1u : 0u, lds_w, &agg_h);
+        const uint32_t ex_v = block_excl_scan<kMcThreads>(c, lds_w, &agg_v);
         const unsigned long long ph = tile_lookback<kMcThreads>(s0, tile, e0, agg_h, &lds_prefix[0]);
         const unsigned long long pv = tile_lookback<kMcThreads>(s1, tile, e1, agg_v, &lds_prefix[1]);
-        unsigned long long h = ph + before_h, v = pv + before_v;
-#pragma unroll
-        for (int j = 0; j < kFuseBlocks; ++j) {
-            if (cnt[j]) {
-                if (lane == 0) {
-                    a.with_verts[h] = k0 + j;
-                    a.vert_offsets[h] = v;
-                }
-                if (v + cnt[j] <= a.capacity)
-                    emit_block_vertices(tiles[wave] + kTileStride * j, d[j], iso, bid[j], a.m.nb, v,
-                                        a.verts, a.fmt);
-                ++h;
-                v += cnt[j];
-            }
+        if (c) {
+            list_s[ex_h] = threadIdx.x;
+            off_s[ex_h] = ex_v;
+            a.with_verts[ph + ex_h] = k0 + threadIdx.x;
+            a.vert_offsets[ph + ex_h] = pv + ex_v;
         }
         if (tile == ntiles - 1 && threadIdx.x == 0) {
             a.q->num_with = (uint32_t)(ph + agg_h);
             a.q->num_verts = pv + agg_v;
         }
-        __builtin_amdgcn_wave_barrier();
+        __syncthreads();
+        // sweep 2: vertices of the blocks that have some
+        for (uint32_t r = 0; r < agg_h; r += kRound) {
+            const uint32_t hh = r + wave * kFuseBlocks;
+            uint32_t bid[kFuseBlocks], d[kFuseBlocks][3], loc[kFuseBlocks];
+            bool live[kFuseBlocks];
+#pragma unroll
+            for (int j = 0; j < kFuseBlocks; ++j) {
+                live[j] = hh + j < agg_h;
+                loc[j] = live[j] ? list_s[hh + j] : 0u;
+                bid[j] = live[j] ? a.active_ids[k0 + loc[j]] : 0u;
+            }
+            load_tiles<kFuseBlocks>(mv, bid, live, tiles[wave], slots[wave], d);
+#pragma unroll
+            for (int j = 0; j < kFuseBlocks; ++j) {
+                if (!live[j])
+                    continue;
+                const unsigned long long off = pv + off_s[hh + j];
+                if (off + cnt_s[loc[j]] <= a.capacity)
+                    emit_block_vertices(tiles[wave] + kTileStride * j, d[j], iso, bid[j], a.m.nb, off,
+                                        a.verts, a.fmt);
+            }
+        }
+        __syncthreads();
     }
     scan_site_finish(s0, e0);
     scan_site_finish(s1, e1);

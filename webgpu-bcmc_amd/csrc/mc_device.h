@@ -5,7 +5,18 @@
 #include "common.h"
 #include "tri_table.h"
 
+#include <type_traits>
+
 namespace bcmc {
+
+// compile-time unrolled loop: f(std::integral_constant<int, j>) for j < N
+template <int N, class F, int J = 0>
+BCMC_DEV void unroll_for(F &&f) {
+    if constexpr (J < N) {
+        f(std::integral_constant<int, J>{});
+        unroll_for<N, F, J + 1>(static_cast<F &&>(f));
+    }
+}
 
 static __constant__ unsigned long long kTriEdges[256] = BCMC_TRI_EDGES_INIT;
 static __constant__ uint8_t kTriCount[256] = BCMC_TRI_COUNT_INIT;
@@ -209,52 +220,86 @@ BCMC_DEV uint32_t active_from_codes(const uint8_t *code, uint32_t b, const uint3
 }
 
 // ---------------------------------------------------------------------------
-// NB tiles per wave with the three dependent load rounds shared by all NB
-// blocks: (1) neighbour activity flags and all 8 candidate cache slots,
-// (2) the tile voxels.  bids[] and live[] are wave-uniform.
+// NB tiles per wave with the dependent load rounds shared by all NB blocks:
+// (1) neighbour activity flags and all 8 candidate cache slots, (2) the tile
+// voxels.  bid[], live[] and the views are wave-uniform; block j may belong to
+// a different volume/query than block j' (the batched engine).
 // ---------------------------------------------------------------------------
+// Views: `Views::get<j>()` returns the McView of block j (compile-time j).
 template <int NB>
-BCMC_DEV void load_tiles(const McView &m, const uint32_t (&bid)[NB], const bool (&live)[NB],
+struct SameView {
+    McView m;
+    template <int J>
+    BCMC_DEV const McView &get() const { return m; }
+};
+template <int NB>
+struct PerBlockViews {
+    McView m[NB];
+    template <int J>
+    BCMC_DEV const McView &get() const { return m[J]; }
+};
+
+template <int NB, class Views>
+BCMC_DEV void load_tiles(const Views &mv, const uint32_t (&bid)[NB], const bool (&live)[NB],
                          float *tiles /* NB*128 */, int32_t *slots /* NB*8 */, uint32_t (&d)[NB][3]) {
     const uint32_t lane = lane_id();
-    uint32_t bx[NB], by[NB], bz[NB];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        bx[j] = bid[j] % m.nb[0];
-        by[j] = (bid[j] / m.nb[0]) % m.nb[1];
-        bz[j] = bid[j] / (m.nb[0] * m.nb[1]);
-    }
+    uint32_t bx[NB], by[NB], bz[NB], nbx[NB], nby[NB], nbz[NB];
+    const uint32_t *flg[NB];
+    const int32_t *ism[NB];
+    const float *cache[NB];
+    uint32_t cap[NB];
+    unroll_for<NB>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        const McView &m = mv.template get<j>();
+        nbx[j] = m.nb[0];
+        nby[j] = m.nb[1];
+        nbz[j] = m.nb[2];
+        flg[j] = m.flags;
+        ism[j] = m.item_slot;
+        cache[j] = m.cache;
+        cap[j] = m.capacity;
+        bx[j] = bid[j] % nbx[j];
+        by[j] = (bid[j] / nbx[j]) % nby[j];
+        bz[j] = bid[j] / (nbx[j] * nby[j]);
+    });
     // lane j*8 + s: flag (s < 7: +x,+y,+z,+xy,+xz,+yz,+xyz) and slot of sub-block s
     bool act = false;
-    int32_t slot = -1;
     if (lane < 8u * NB) {
         const uint32_t j = lane / 8, s = lane % 8;
-        uint32_t jx = bx[0], jy = by[0], jz = bz[0];
+        uint32_t jx = bx[0], jy = by[0], jz = bz[0], nx = nbx[0], ny = nby[0], nz = nbz[0];
         bool jl = live[0];
+        const uint32_t *f = flg[0];
+        const int32_t *sm = ism[0];
 #pragma unroll
-        for (int q = 1; q < NB; ++q)
-            if (j == (uint32_t)q) {
-                jx = bx[q];
-                jy = by[q];
-                jz = bz[q];
-                jl = live[q];
-            }
+        for (int q = 1; q < NB; ++q) {
+            const bool pick = j == (uint32_t)q;
+            jx = pick ? bx[q] : jx;
+            jy = pick ? by[q] : jy;
+            jz = pick ? bz[q] : jz;
+            nx = pick ? nbx[q] : nx;
+            ny = pick ? nby[q] : ny;
+            nz = pick ? nbz[q] : nz;
+            jl = pick ? live[q] : jl;
+            f = pick ? flg[q] : f;
+            sm = pick ? ism[q] : sm;
+        }
         // flag codes in the reference's check order; slot sub-block s = sx + 2sy + 4sz
         const uint32_t fcode = s < 3 ? (1u << s) : (s == 3 ? 3u : s == 4 ? 5u : s == 5 ? 6u : 7u);
         const uint32_t fx = jx + (fcode & 1u), fy = jy + ((fcode >> 1) & 1u), fz = jz + (fcode >> 2);
-        if (jl && s < 7 && fx < m.nb[0] && fy < m.nb[1] && fz < m.nb[2])
-            act = m.flags[fx + m.nb[0] * (fy + m.nb[1] * fz)] != 0;
+        if (jl && s < 7 && fx < nx && fy < ny && fz < nz)
+            act = f[fx + nx * (fy + ny * fz)] != 0;
         const uint32_t sx = jx + (s & 1u), sy = jy + ((s >> 1) & 1u), sz = jz + (s >> 2);
-        if (jl && sx < m.nb[0] && sy < m.nb[1] && sz < m.nb[2])
-            slot = m.item_slot[sx + m.nb[0] * (sy + m.nb[1] * sz)];
+        int32_t slot = -1;
+        if (jl && sx < nx && sy < ny && sz < nz)
+            slot = sm[sx + nx * (sy + ny * sz)];
         slots[lane] = slot;
     }
     const uint64_t ab = __ballot(act);
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
         const uint32_t a = (uint32_t)(ab >> (8 * j)) & 0x7Fu;
-        uint32_t dx = bx[j] + 1 < m.nb[0] ? 5 : 4, dy = by[j] + 1 < m.nb[1] ? 5 : 4,
-                 dz = bz[j] + 1 < m.nb[2] ? 5 : 4;
+        uint32_t dx = bx[j] + 1 < nbx[j] ? 5 : 4, dy = by[j] + 1 < nby[j] ? 5 : 4,
+                 dz = bz[j] + 1 < nbz[j] ? 5 : 4;
         if (dx == 5 && dy == 5 && dz == 5 && !(a & 64))
             dx = dy = dz = 4;
         if (dx == 5 && dy == 5 && !(a & 8))
@@ -276,18 +321,28 @@ BCMC_DEV void load_tiles(const McView &m, const uint32_t (&bid)[NB], const bool 
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // voxel loads of all NB tiles issued together (2 per lane per tile)
+    float vv[NB][2];
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
-        for (uint32_t t = lane; t < 125; t += 64) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t t = lane + 64u * h;
             const uint32_t x = t % 5, y = (t / 5) % 5, z = t / 25;
             float v = 0.0f;
-            if (live[j] && x < d[j][0] && y < d[j][1] && z < d[j][2]) {
+            if (t < 125 && live[j] && x < d[j][0] && y < d[j][1] && z < d[j][2]) {
                 const int32_t s = slots[8 * j + (x >> 2) + 2 * (y >> 2) + 4 * (z >> 2)];
-                if (s >= 0 && (uint32_t)s < m.capacity)
-                    v = m.cache[(uint64_t)(uint32_t)s * 64 + (x & 3) + 4 * (y & 3) + 16 * (z & 3)];
+                if (s >= 0 && (uint32_t)s < cap[j])
+                    v = cache[j][(uint64_t)(uint32_t)s * 64 + (x & 3) + 4 * (y & 3) + 16 * (z & 3)];
             }
-            tiles[128 * j + t] = v;
+            vv[j][h] = v;
         }
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        tiles[128 * j + lane] = vv[j][0];
+        if (lane + 64 < 125)
+            tiles[128 * j + lane + 64] = vv[j][1];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();

@@ -217,6 +217,12 @@ void launch_lru_order(bcmc_ctx *ctx, SortScratch *s, const uint32_t *avail_ids,
     sort_passes(ctx, s, count_max, count_dev, 0, src, s->keys[1], out, 0);
 }
 
+void launch_sort_pairs(bcmc_ctx *ctx, SortScratch *s, const uint32_t *keys, const uint32_t *vals,
+                       const uint32_t *count_dev, uint32_t count_max, uint32_t *out) {
+    BufSource src{keys, vals, count_dev, 0};
+    sort_passes(ctx, s, count_max, count_dev, 0, src, s->keys[1], out, 0);
+}
+
 void launch_sort_by_key(bcmc_ctx *ctx, SortScratch *s, uint32_t *keys, uint32_t *vals, uint32_t n,
                         int reverse) {
     uint32_t aligned = 64;  // nextPow2(ceil(n/64)) * 64

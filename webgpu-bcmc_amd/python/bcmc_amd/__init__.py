@@ -62,6 +62,17 @@ class _McInfo(ctypes.Structure):
                 ("format", ctypes.c_int)]
 
 
+class _QueryResult(ctypes.Structure):
+    _fields_ = [("num_vertices", ctypes.c_uint64), ("vertex_offset", ctypes.c_uint64),
+                ("num_active_blocks", ctypes.c_uint32), ("step_decoded_blocks", ctypes.c_uint32)]
+
+
+class _EngineInfo(ctypes.Structure):
+    _fields_ = [("num_volumes", ctypes.c_uint32), ("pool_slots", ctypes.c_uint32),
+                ("pool_used", ctypes.c_uint32), ("epoch", ctypes.c_uint32),
+                ("pool_bytes", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64)]
+
+
 _lib = None
 
 
@@ -122,6 +133,13 @@ def lib():
         "bcmc_mc_lru": (vp, [vp]),
         "bcmc_mc_array_buffer": (vp, [vp, i32]),
         "bcmc_mc_memory_use": (i32, [vp, P(u64), P(u64)]),
+        "bcmc_engine_create": (i32, [vp, u64, P(vp)]),
+        "bcmc_engine_destroy": (i32, [vp]),
+        "bcmc_engine_add_volume": (i32, [vp, vp, u64, ctypes.c_double, P(u32), P(u32)]),
+        "bcmc_engine_set_vertex_format": (i32, [vp, i32]),
+        "bcmc_engine_run": (i32, [vp, vp, vp, u32, vp]),
+        "bcmc_engine_vertex_buffer": (vp, [vp]),
+        "bcmc_engine_stats": (i32, [vp, P(_EngineInfo)]),
         "bcmc_zfp_decode_blocks": (i32, [vp, vp, u32, vp, u32, vp]),
         "bcmc_zfp_block_ranges": (i32, [vp, vp, u32, P(u32), vp]),
     }
@@ -455,6 +473,55 @@ class CompressedMarchingCubes:
         t, c = ctypes.c_uint64(), ctypes.c_uint64()
         _check(lib().bcmc_mc_memory_use(self.h, ctypes.byref(t), ctypes.byref(c)))
         return {"total": t.value, "cache": c.value}
+
+
+class Engine:
+    """Batched / served form: resident volumes + a shared decoded-block pool;
+    run() answers many (volume, isovalue) queries in one step."""
+
+    def __init__(self, device, pool_bytes):
+        self.device = device
+        h = ctypes.c_void_p()
+        _check(lib().bcmc_engine_create(device.h, int(pool_bytes), ctypes.byref(h)))
+        self.h = h
+        self.fmt = VERTS_LOCAL_F32
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().bcmc_engine_destroy(self.h)
+            self.h = None
+
+    def add_volume(self, stream, rate, dims):
+        v = np.ascontiguousarray(np.frombuffer(memoryview(stream), np.uint8))
+        vid = ctypes.c_uint32()
+        _check(lib().bcmc_engine_add_volume(self.h, v.ctypes.data, v.nbytes, float(rate),
+                                            _u32arr(dims), ctypes.byref(vid)))
+        return vid.value
+
+    def set_vertex_format(self, fmt):
+        _check(lib().bcmc_engine_set_vertex_format(self.h, fmt))
+        self.fmt = fmt
+
+    def run(self, volume_ids, isovalues):
+        ids = np.ascontiguousarray(volume_ids, np.uint32)
+        isos = np.ascontiguousarray(isovalues, np.float32)
+        res = (_QueryResult * max(len(ids), 1))()
+        _check(lib().bcmc_engine_run(self.h, ids.ctypes.data, isos.ctypes.data, len(ids), res))
+        return [{f: getattr(r, f) for f, _ in _QueryResult._fields_} for r in res[:len(ids)]]
+
+    def vertices(self, result):
+        b = _view(self.device, lib().bcmc_engine_vertex_buffer(self.h))
+        n, off = result["num_vertices"], result["vertex_offset"]
+        if self.fmt == VERTS_LOCAL_F32:
+            return b.download(np.float32, 4 * n, 16 * off).reshape(-1, 4)
+        if self.fmt == VERTS_GLOBAL_F32:
+            return b.download(np.float32, 3 * n, 12 * off).reshape(-1, 3)
+        return b.download(np.uint32, 2 * n, 8 * off).reshape(-1, 2)
+
+    def stats(self):
+        i = _EngineInfo()
+        _check(lib().bcmc_engine_stats(self.h, ctypes.byref(i)))
+        return {f: getattr(i, f) for f, _ in _EngineInfo._fields_}
 
 
 # ---------------------------------------------------------------------------
