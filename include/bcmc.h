@@ -235,6 +235,24 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
 BCMC_API bcmc_buf *bcmc_engine_vertex_buffer(bcmc_engine *e);
 BCMC_API int bcmc_engine_stats(bcmc_engine *e, bcmc_engine_info *info);
 
+/* Served form: replay timestamped requests in real time through one engine
+ * with continuous batching (each step takes every request that has arrived,
+ * FIFO, up to max_batch and an output-vertex budget).  out[i] receives request
+ * i's arrival, step start and finish times (seconds from the call) and its
+ * vertex count; the vertices of the last step stay in the engine buffer. */
+typedef struct {
+    uint32_t max_batch;           /* 0: 256 */
+    uint64_t vertex_budget_bytes; /* HBM for one step's vertices; 0: 64 GiB */
+} bcmc_serve_opts;
+typedef struct {
+    double arrival_s, start_s, finish_s;
+    uint64_t num_vertices;
+    uint32_t batch_size, step;
+} bcmc_serve_result;
+BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s, const uint32_t *volume_ids,
+                              const float *isovalues, uint32_t n, const bcmc_serve_opts *opts,
+                              bcmc_serve_result *out);
+
 /* ---- ZFP decode (shaders/zfp_decompress.comp, zfp_decompress_block.comp) -- */
 /* Decode blocks ids[0..n) of a fixed-rate stream into out[i*64 .. i*64+64)
  * (float32, x fastest).  ids may be NULL for blocks 0..n-1. */

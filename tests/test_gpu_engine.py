@@ -61,3 +61,30 @@ def test_pool_too_small_is_an_error(dev):
     with pytest.raises(bcmc_amd.BcmcError) as e:
         eng.run([vid], [0.5])
     assert e.value.code == -3
+
+
+def test_served_trace_results_match_single_queries(dev):
+    say("serve trace")
+    streams = _streams()
+    eng = bcmc_amd.Engine(dev, 200000 * 256)
+    ids = [eng.add_volume(s, r, d) for s, r, d in streams]
+    rng = np.random.default_rng(11)
+    n = 60
+    arrival = np.cumsum(rng.exponential(1.0 / 4000.0, n))  # 4000 req/s: requests pile up
+    vol = rng.integers(0, len(ids), n)
+    iso = rng.uniform(0.15, 0.85, n).astype(np.float32)
+    rep = rng.uniform(0, 1, n) < 0.5
+    for i in range(1, n):  # half the requests repeat a recent isovalue of their volume
+        prev = [j for j in range(max(0, i - 16), i) if vol[j] == vol[i]]
+        if rep[i] and prev:
+            iso[i] = iso[prev[-1]]
+    out = eng.serve(arrival, vol, iso, max_batch=16)
+    assert np.all(out["finish_s"] >= out["arrival_s"]) and np.all(out["start_s"] >= out["arrival_s"])
+    assert out["batch_size"].max() > 1  # continuous batching actually batched
+    cache = {}
+    for i in range(n):
+        key = (int(vol[i]), float(iso[i]))
+        if key not in cache:
+            s, r, d = streams[vol[i]]
+            cache[key] = _oracle(s, r, d, float(iso[i]))[0]
+        assert out["num_vertices"][i] == cache[key], i

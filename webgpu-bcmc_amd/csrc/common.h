@@ -131,21 +131,26 @@ BCMC_DEV uint32_t wave_read_lane(uint32_t v, int lane) {
 
 BCMC_DEV uint32_t wave_sum(uint32_t v) { return wave_read_lane(wave_incl_scan(v), 63); }
 
-// Agent-scope atomics (coherent across the XCDs' L2 caches).
-BCMC_DEV unsigned long long ld_acquire_u64(const unsigned long long *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Agent-scope relaxed atomics: coherent across the XCDs' L2 caches without
+// the cache write-back / invalidate that acquire/release at agent scope
+// implies.  Every word accessed this way is self-contained (a counter, or a
+// status word carrying its own value and epoch), so no ordering with other
+// memory is needed; data written before a kernel ends is visible to the next
+// kernel on the stream.
+BCMC_DEV unsigned long long ld_agent_u64(const unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-BCMC_DEV void st_release_u64(unsigned long long *p, unsigned long long v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+BCMC_DEV void st_agent_u64(unsigned long long *p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-BCMC_DEV uint32_t ld_acquire_u32(const uint32_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+BCMC_DEV uint32_t ld_agent_u32(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-BCMC_DEV void st_release_u32(uint32_t *p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+BCMC_DEV void st_agent_u32(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 BCMC_DEV uint32_t atomic_add_u32(uint32_t *p, uint32_t v) {
-    return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 constexpr unsigned long long kValueMask = (1ull << 40) - 1;
@@ -202,17 +207,17 @@ BCMC_DEV unsigned long long tile_lookback(const ScanSite &site, uint32_t tile, u
         unsigned long long prefix = 0;
         if (tile == 0) {
             if (lane == 0)
-                st_release_u64(&site.status[0], pack_status(ep, kFlagInclusive, aggregate));
+                st_agent_u64(&site.status[0], pack_status(ep, kFlagInclusive, aggregate));
         } else {
             if (lane == 0)
-                st_release_u64(&site.status[tile], pack_status(ep, kFlagAggregate, aggregate));
+                st_agent_u64(&site.status[tile], pack_status(ep, kFlagAggregate, aggregate));
             int64_t base = (int64_t)tile - 1;
             for (;;) {
                 const int64_t t = base - (int64_t)lane;
                 unsigned long long s = 0;
                 uint32_t flag = kFlagInclusive;  // before tile 0: an inclusive zero
                 if (t >= 0) {
-                    s = ld_acquire_u64(&site.status[t]);
+                    s = ld_agent_u64(&site.status[t]);
                     flag = ((uint32_t)(s >> 42) == ep) ? (uint32_t)(s >> 40) & 3u : 0u;
                 }
                 const uint64_t incl = __ballot(flag == kFlagInclusive);
@@ -230,7 +235,7 @@ BCMC_DEV unsigned long long tile_lookback(const ScanSite &site, uint32_t tile, u
                 base -= 64;
             }
             if (lane == 0)
-                st_release_u64(&site.status[tile],
+                st_agent_u64(&site.status[tile],
                                pack_status(ep, kFlagInclusive, prefix + aggregate));
         }
         if (lane == 0)
@@ -260,19 +265,19 @@ BCMC_DEV void scan_site_finish(const ScanSite &site, uint32_t epoch) {
     if (threadIdx.x == 0) {
         const uint32_t done = atomic_add_u32(&site.ctrl[1], 1u);
         if (done == gridDim.x - 1) {
-            st_release_u32(&site.ctrl[0], 0u);
-            st_release_u32(&site.ctrl[1], 0u);
+            st_agent_u32(&site.ctrl[0], 0u);
+            st_agent_u32(&site.ctrl[1], 0u);
             uint32_t next = (epoch + 1) & 0x3FFFFF;
             if (next == 0) {
                 for (uint32_t t = 0; t < site.max_tiles; ++t)
-                    st_release_u64(&site.status[t], 0ull);
+                    st_agent_u64(&site.status[t], 0ull);
                 next = 1;
             }
-            st_release_u32(&site.ctrl[2], next);
+            st_agent_u32(&site.ctrl[2], next);
         }
     }
 }
 
-BCMC_DEV uint32_t scan_site_epoch(const ScanSite &site) { return ld_acquire_u32(&site.ctrl[2]); }
+BCMC_DEV uint32_t scan_site_epoch(const ScanSite &site) { return ld_agent_u32(&site.ctrl[2]); }
 
 }  // namespace bcmc

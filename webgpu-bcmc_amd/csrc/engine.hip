@@ -30,6 +30,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace bcmc {
@@ -714,3 +715,81 @@ BCMC_API int bcmc_engine_stats(bcmc_engine *e, bcmc_engine_info *info) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Served form: a request loop over an engine (continuous batching)
+// ---------------------------------------------------------------------------
+// Requests arrive at given times (seconds from the start of the call).  The
+// loop keeps one step in flight: every request that has arrived by the time a
+// step starts joins it (FIFO, up to max_batch and the output-vertex budget);
+// requests arriving while a step runs wait for the next one.  Latency is
+// completion time - arrival time.  The decoded-block pool (the engine's LRU)
+// and the compressed volumes stay resident across requests.
+extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s,
+                                         const uint32_t *volume_ids, const float *isovalues,
+                                         uint32_t n, const bcmc_serve_opts *opts,
+                                         bcmc_serve_result *out) {
+    BCMC_CHECK_ARG(e && arrival_s && volume_ids && isovalues && out, "null argument");
+    for (uint32_t i = 1; i < n; ++i)
+        BCMC_CHECK_ARG(arrival_s[i] >= arrival_s[i - 1], "arrivals must be sorted");
+    const uint32_t max_batch = opts && opts->max_batch ? opts->max_batch : 256;
+    const uint64_t vert_budget = opts && opts->vertex_budget_bytes ? opts->vertex_budget_bytes
+                                                                    : (uint64_t)64 << 30;
+    const uint32_t vb = vbytes(e->fmt);
+    // per-volume running estimate of vertices per query (largest seen)
+    std::vector<uint64_t> est(e->vols.size(), 0);
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    auto now_s = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
+    uint32_t next = 0, done = 0, steps = 0;
+    std::vector<uint32_t> pending;
+    size_t head = 0;
+    std::vector<uint32_t> ids;
+    std::vector<float> isos;
+    std::vector<bcmc_query_result> res;
+    while (done < n) {
+        double t = now_s();
+        while (next < n && arrival_s[next] <= t)
+            pending.push_back(next++);
+        if (head == pending.size()) {
+            // idle: wait for the next arrival
+            const double wait = arrival_s[next] - t;
+            if (wait > 0)
+                std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+            continue;
+        }
+        // admission: FIFO up to max_batch and the vertex budget (estimate)
+        ids.clear();
+        isos.clear();
+        std::vector<uint32_t> batch;
+        uint64_t bytes = 0;
+        while (head < pending.size() && batch.size() < max_batch) {
+            const uint32_t r = pending[head];
+            const uint64_t need = est[volume_ids[r]] * vb;
+            if (!batch.empty() && bytes + need > vert_budget)
+                break;
+            bytes += need;
+            batch.push_back(r);
+            ids.push_back(volume_ids[r]);
+            isos.push_back(isovalues[r]);
+            ++head;
+        }
+        res.resize(batch.size());
+        const double start = now_s();
+        BCMC_TRY(bcmc_engine_run(e, ids.data(), isos.data(), (uint32_t)batch.size(), res.data()));
+        const double end = now_s();
+        for (size_t k = 0; k < batch.size(); ++k) {
+            const uint32_t r = batch[k];
+            out[r].arrival_s = arrival_s[r];
+            out[r].start_s = start;
+            out[r].finish_s = end;
+            out[r].num_vertices = res[k].num_vertices;
+            out[r].batch_size = (uint32_t)batch.size();
+            out[r].step = steps;
+            est[ids[k]] = std::max<uint64_t>(est[ids[k]], res[k].num_vertices);
+        }
+        done += (uint32_t)batch.size();
+        ++steps;
+    }
+    return BCMC_OK;
+}

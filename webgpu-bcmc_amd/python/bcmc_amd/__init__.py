@@ -73,6 +73,16 @@ class _EngineInfo(ctypes.Structure):
                 ("pool_bytes", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64)]
 
 
+class _ServeOpts(ctypes.Structure):
+    _fields_ = [("max_batch", ctypes.c_uint32), ("vertex_budget_bytes", ctypes.c_uint64)]
+
+
+class _ServeResult(ctypes.Structure):
+    _fields_ = [("arrival_s", ctypes.c_double), ("start_s", ctypes.c_double),
+                ("finish_s", ctypes.c_double), ("num_vertices", ctypes.c_uint64),
+                ("batch_size", ctypes.c_uint32), ("step", ctypes.c_uint32)]
+
+
 _lib = None
 
 
@@ -140,6 +150,7 @@ def lib():
         "bcmc_engine_run": (i32, [vp, vp, vp, u32, vp]),
         "bcmc_engine_vertex_buffer": (vp, [vp]),
         "bcmc_engine_stats": (i32, [vp, P(_EngineInfo)]),
+        "bcmc_serve_trace": (i32, [vp, vp, vp, vp, u32, P(_ServeOpts), vp]),
         "bcmc_zfp_decode_blocks": (i32, [vp, vp, u32, vp, u32, vp]),
         "bcmc_zfp_block_ranges": (i32, [vp, vp, u32, P(u32), vp]),
     }
@@ -517,6 +528,19 @@ class Engine:
         if self.fmt == VERTS_GLOBAL_F32:
             return b.download(np.float32, 3 * n, 12 * off).reshape(-1, 3)
         return b.download(np.uint32, 2 * n, 8 * off).reshape(-1, 2)
+
+    def serve(self, arrival_s, volume_ids, isovalues, max_batch=0, vertex_budget_bytes=0):
+        """Replay a request trace in real time (continuous batching); returns a
+        structured array with arrival/start/finish times, vertices, batch size."""
+        arr = np.ascontiguousarray(arrival_s, np.float64)
+        ids = np.ascontiguousarray(volume_ids, np.uint32)
+        isos = np.ascontiguousarray(isovalues, np.float32)
+        n = len(arr)
+        out = (_ServeResult * max(n, 1))()
+        opts = _ServeOpts(max_batch, vertex_budget_bytes)
+        _check(lib().bcmc_serve_trace(self.h, arr.ctypes.data, ids.ctypes.data, isos.ctypes.data, n,
+                                      ctypes.byref(opts), out))
+        return {f: np.array([getattr(r, f) for r in out[:n]]) for f, _ in _ServeResult._fields_}
 
     def stats(self):
         i = _EngineInfo()
