@@ -269,6 +269,9 @@ __global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, Eng
     __shared__ uint32_t lds_w[NW + 1];
     __shared__ unsigned long long lds_prefix;
     __shared__ uint32_t lds_tile;
+    __shared__ McTables tab;
+    __shared__ EmitScratch es[NW];
+    load_mc_tables(&tab);
     const uint32_t n = a.c->num_active;
     const uint32_t ntiles = div_up(n, kEngTile);
     const uint32_t e0 = scan_site_epoch(s0);
@@ -293,7 +296,7 @@ __global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, Eng
             load_tiles<kEngBlocks>(mv, bid, live, tiles[wave], slots[wave], d);
 #pragma unroll
             for (int j = 0; j < kEngBlocks; ++j) {
-                const uint32_t c = live[j] ? block_vertex_count(tiles[wave] + 128 * j, d[j], a.qs[qi[j]].iso)
+                const uint32_t c = live[j] ? block_vertex_count(tiles[wave] + 128 * j, d[j], a.qs[qi[j]].iso, tab)
                                            : 0u;
                 if (lane == 0) {
                     cnt_s[kk + j] = c;
@@ -333,10 +336,8 @@ __global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, Eng
             for (int j = 0; j < kEngBlocks; ++j) {
                 if (!live[j])
                     continue;
-                const unsigned long long off = pv + off_s[hh + j];
-                if (off + cnt_s[loc[j]] <= a.vert_cap)
-                    emit_block_vertices(tiles[wave] + 128 * j, d[j], a.qs[qi[j]].iso, bid[j], mv.m[j].nb,
-                                        off, a.verts, a.fmt);
+                emit_block_vertices(tiles[wave] + 128 * j, d[j], a.qs[qi[j]].iso, bid[j], mv.m[j].nb,
+                                    pv + off_s[hh + j], a.vert_cap, a.verts, a.fmt, tab, es[wave]);
             }
         }
         __syncthreads();

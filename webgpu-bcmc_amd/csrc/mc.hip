@@ -93,6 +93,9 @@ __global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanS
     __shared__ uint32_t lds_w[NW + 1];
     __shared__ unsigned long long lds_prefix[2];
     __shared__ uint32_t lds_tile;
+    __shared__ McTables tab;
+    __shared__ EmitScratch es[NW];
+    load_mc_tables(&tab);
     const uint32_t n = a.q->num_active;
     const float iso = a.q->iso;
     const uint32_t ntiles = div_up(n, kFuseTile);
@@ -121,7 +124,7 @@ __global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanS
             load_tiles<kFuseBlocks>(mv, bid, live, tiles[wave], slots[wave], d);
 #pragma unroll
             for (int j = 0; j < kFuseBlocks; ++j) {
-                const uint32_t c = live[j] ? block_vertex_count(tiles[wave] + kTileStride * j, d[j], iso) : 0u;
+                const uint32_t c = live[j] ? block_vertex_count(tiles[wave] + kTileStride * j, d[j], iso, tab) : 0u;
                 if (lane == 0)
                     cnt_s[kk + j] = c;
             }
@@ -161,10 +164,8 @@ __global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanS
             for (int j = 0; j < kFuseBlocks; ++j) {
                 if (!live[j])
                     continue;
-                const unsigned long long off = pv + off_s[hh + j];
-                if (off + cnt_s[loc[j]] <= a.capacity)
-                    emit_block_vertices(tiles[wave] + kTileStride * j, d[j], iso, bid[j], a.m.nb, off,
-                                        a.verts, a.fmt);
+                emit_block_vertices(tiles[wave] + kTileStride * j, d[j], iso, bid[j], a.m.nb,
+                                    pv + off_s[hh + j], a.capacity, a.verts, a.fmt, tab, es[wave]);
             }
         }
         __syncthreads();

@@ -21,14 +21,14 @@ BCMC_DEV void unroll_for(F &&f) {
 static __constant__ unsigned long long kTriEdges[256] = BCMC_TRI_EDGES_INIT;
 static __constant__ uint8_t kTriCount[256] = BCMC_TRI_COUNT_INIT;
 // edge_vertices (compute_block_vertices.comp:10-23) as two nibbles per edge
-static __constant__ uint8_t kEdgeVerts[12] = {0x10, 0x21, 0x32, 0x03, 0x54, 0x56,
-                                       0x76, 0x47, 0x40, 0x51, 0x62, 0x73};
+constexpr uint8_t kEdgeVertsHost[12] = {0x10, 0x21, 0x32, 0x03, 0x54, 0x56,
+                                        0x76, 0x47, 0x40, 0x51, 0x62, 0x73};
 
 // index_to_vertex (compute_vertex_values.comp:48-57): corner i at
 // (x, y, z) = (((i+1)>>1)&1, (i>>1)&1, i>>2)
-BCMC_DEV uint32_t corner_x(uint32_t i) { return ((i + 1) >> 1) & 1u; }
-BCMC_DEV uint32_t corner_y(uint32_t i) { return (i >> 1) & 1u; }
-BCMC_DEV uint32_t corner_z(uint32_t i) { return i >> 2; }
+__host__ __device__ constexpr uint32_t corner_x(uint32_t i) { return ((i + 1) >> 1) & 1u; }
+__host__ __device__ constexpr uint32_t corner_y(uint32_t i) { return (i >> 1) & 1u; }
+__host__ __device__ constexpr uint32_t corner_z(uint32_t i) { return i >> 2; }
 
 struct McView {
     const float *cache;
@@ -41,57 +41,41 @@ struct McView {
 constexpr int kMcThreads = 256;  // 4 waves, one block each
 constexpr int kTileStride = 128;
 
-// compute_block_dims_with_ghost + load_block (compute_vertex_values.comp:96-252).
-// Tile layout x + 5y + 25z.  Returns the dims through d[3] (wave uniform).
-BCMC_DEV void load_tile(const McView &m, uint32_t bid, float *tile, int32_t *slots, uint32_t (&d)[3]) {
-    const uint32_t lane = lane_id();
-    const uint32_t bx = bid % m.nb[0], by = (bid / m.nb[0]) % m.nb[1], bz = bid / (m.nb[0] * m.nb[1]);
-    // neighbour activity: +x, +y, +z, +xy, +xz, +yz, +xyz
-    bool act = false;
-    if (lane < 7) {
-        const uint32_t code = lane < 3 ? (1u << lane) : (lane == 3 ? 3u : lane == 4 ? 5u : lane == 5 ? 6u : 7u);
-        const uint32_t nx = bx + (code & 1u), ny = by + ((code >> 1) & 1u), nz = bz + (code >> 2);
-        if (nx < m.nb[0] && ny < m.nb[1] && nz < m.nb[2])
-            act = m.flags[nx + m.nb[0] * (ny + m.nb[1] * nz)] != 0;
+// Triangle tables staged in LDS once per workgroup (the per-cell lookups are
+// divergent, so they would otherwise be vector loads through the L1).
+struct McTables {
+    unsigned long long edges[256];
+    uint8_t count[256];
+};
+
+BCMC_DEV void load_mc_tables(McTables *t) {
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) {
+        t->edges[i] = kTriEdges[i];
+        t->count[i] = kTriCount[i];
     }
-    const uint64_t a = __ballot(act);
-    uint32_t dx = bx + 1 < m.nb[0] ? 5 : 4, dy = by + 1 < m.nb[1] ? 5 : 4, dz = bz + 1 < m.nb[2] ? 5 : 4;
-    if (dx == 5 && dy == 5 && dz == 5 && !(a & 64))
-        dx = dy = dz = 4;
-    if (dx == 5 && dy == 5 && !(a & 8))
-        dx = dy = 4;
-    if (dx == 5 && dz == 5 && !(a & 16))
-        dx = dz = 4;
-    if (dy == 5 && dz == 5 && !(a & 32))
-        dy = dz = 4;
-    if (dx == 5 && !(a & 1))
-        dx = 4;
-    if (dy == 5 && !(a & 2))
-        dy = 4;
-    if (dz == 5 && !(a & 4))
-        dz = 4;
-    d[0] = dx;
-    d[1] = dy;
-    d[2] = dz;
-    // cache slots of the (up to 8) blocks the tile touches
-    if (lane < 8) {
-        const uint32_t sx = lane & 1u, sy = (lane >> 1) & 1u, sz = lane >> 2;
-        const bool need = (!sx || dx == 5) && (!sy || dy == 5) && (!sz || dz == 5);
-        slots[lane] = need ? m.item_slot[(bx + sx) + m.nb[0] * ((by + sy) + m.nb[1] * (bz + sz))] : -1;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (uint32_t t = lane; t < 125; t += 64) {
-        const uint32_t x = t % 5, y = (t / 5) % 5, z = t / 25;
-        float v = 0.0f;
-        if (x < dx && y < dy && z < dz) {
-            const int32_t s = slots[(x >> 2) + 2 * (y >> 2) + 4 * (z >> 2)];
-            if (s >= 0 && (uint32_t)s < m.capacity)
-                v = m.cache[(uint64_t)(uint32_t)s * 64 + (x & 3) + 4 * (y & 3) + 16 * (z & 3)];
-        }
-        tile[t] = v;
-    }
+    __syncthreads();
+}
+
+// Per-wave LDS scratch of the vertex-parallel emission: the case and first
+// vertex of each of the 64 cells, and the owning cell of each vertex.
+struct EmitScratch {
+    uint8_t cell_case[64];
+    uint16_t cell_first[64];
+    uint8_t vert_cell[64 * 15];
+};
+
+// Corners a and b of edge e as 4-bit fields (edge_vertices,
+// compute_block_vertices.comp:10-23), and the tile offset of corner i.
+constexpr unsigned long long edge_corner_bits(int which) {
+    unsigned long long r = 0;
+    for (int e = 0; e < 12; ++e)
+        r |= (unsigned long long)((kEdgeVertsHost[e] >> (4 * which)) & 15u) << (4 * e);
+    return r;
+}
+constexpr unsigned long long kEdgeCornerA = edge_corner_bits(0), kEdgeCornerB = edge_corner_bits(1);
+BCMC_DEV uint32_t corner_offset(uint32_t i) { return corner_x(i) + 5 * corner_y(i) + 25 * corner_z(i); }
+
+BCMC_DEV void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -99,52 +83,63 @@ BCMC_DEV void load_tile(const McView &m, uint32_t bid, float *tile, int32_t *slo
 
 // Cell of this lane: case index (bit i iff corner i <= iso) or -1 if the cell
 // is outside the tile (compute_block_has_vertices.comp:28-37).
-BCMC_DEV int cell_case(const float *tile, const uint32_t (&d)[3], float iso, uint32_t &base) {
+BCMC_DEV int cell_case(const float *tile, const uint32_t (&d)[3], float iso) {
     const uint32_t lane = lane_id();
     const uint32_t x = lane & 3u, y = (lane >> 2) & 3u, z = lane >> 4;
-    base = x + 5 * y + 25 * z;
+    const uint32_t base = x + 5 * y + 25 * z;
     if (!(x + 1 < d[0] && y + 1 < d[1] && z + 1 < d[2]))
         return -1;
     int c = 0;
 #pragma unroll
     for (uint32_t i = 0; i < 8; ++i) {
-        const float v = tile[base + corner_x(i) + 5 * corner_y(i) + 25 * corner_z(i)];
+        const float v = tile[base + corner_offset(i)];
         c |= (v <= iso) ? (1 << i) : 0;
     }
     return c;
 }
 
-
-// Vertices of one block (compute_block_vertices.comp:76-174): a wave prefix
-// sum of the per-cell counts, then each lane interpolates and writes the
-// vertices of its cell starting at out0 + its prefix.
+// Vertices of one block (compute_block_vertices.comp:76-174), vertex-parallel:
+// the cells' counts are prefix-summed across the wave, each cell marks its
+// vertices' owner in LDS, then lane v of each round interpolates vertex v, so
+// the 64 lanes of a round write one contiguous run of the vertex buffer.
+// Vertex order is the reference's: cells in x-fastest order, within a cell the
+// order of its triangle-table row.  Vertices at or past `cap` are not written.
 BCMC_DEV void emit_block_vertices(const float *tile, const uint32_t (&d)[3], float iso, uint32_t bid,
-                                  const uint32_t (&nb)[3], unsigned long long base_offset,
-                                  void *verts, int fmt) {
-    uint32_t base;
-    const int c = cell_case(tile, d, iso, base);
-    const uint32_t nv = c >= 0 ? kTriCount[c] : 0u;
-    const uint32_t excl = wave_incl_scan(nv) - nv;
-    const unsigned long long out0 = base_offset + excl;
+                                  const uint32_t (&nb)[3], unsigned long long out0,
+                                  unsigned long long cap, void *verts, int fmt, const McTables &tab,
+                                  EmitScratch &es) {
     const uint32_t lane = lane_id();
-    const float vx = (float)(lane & 3u), vy = (float)((lane >> 2) & 3u), vz = (float)(lane >> 4);
-    const unsigned long long edges = c >= 0 ? kTriEdges[c] : 0ull;
-    const uint32_t bx = bid % nb[0], by = (bid / nb[0]) % nb[1],
-                   bz = bid / (nb[0] * nb[1]);
-    for (uint32_t j = 0; j < nv; ++j) {
-        const uint32_t e = (uint32_t)(edges >> (4 * j)) & 15u;
-        const uint32_t ev = kEdgeVerts[e], ca = ev & 15u, cb = ev >> 4;
-        const float fa = tile[base + corner_x(ca) + 5 * corner_y(ca) + 25 * corner_z(ca)];
-        const float fb = tile[base + corner_x(cb) + 5 * corner_y(cb) + 25 * corner_z(cb)];
+    const int c = cell_case(tile, d, iso);
+    const uint32_t nv = c >= 0 ? tab.count[c] : 0u;
+    const uint32_t incl = wave_incl_scan(nv);
+    const uint32_t first = incl - nv, total = wave_read_lane(incl, 63);
+    es.cell_case[lane] = (uint8_t)c;
+    es.cell_first[lane] = (uint16_t)first;
+    for (uint32_t k = 0; k < nv; ++k)
+        es.vert_cell[first + k] = (uint8_t)lane;
+    wave_lds_sync();
+    const uint32_t bx = bid % nb[0], by = (bid / nb[0]) % nb[1], bz = bid / (nb[0] * nb[1]);
+    for (uint32_t v = lane; v < total; v += 64) {
+        const uint32_t cell = es.vert_cell[v];
+        const uint32_t k = v - es.cell_first[cell];
+        const uint32_t e = (uint32_t)(tab.edges[es.cell_case[cell]] >> (4 * k)) & 15u;
+        const uint32_t ca = (uint32_t)(kEdgeCornerA >> (4 * e)) & 15u,
+                       cb = (uint32_t)(kEdgeCornerB >> (4 * e)) & 15u;
+        const uint32_t cx = cell & 3u, cy = (cell >> 2) & 3u, cz = cell >> 4;
+        const uint32_t cbase = cx + 5 * cy + 25 * cz;
+        const float fa = tile[cbase + corner_offset(ca)];
+        const float fb = tile[cbase + corner_offset(cb)];
         // lerp_verts (compute_block_vertices.comp:62-74), then + voxel + 0.5
         float t = 0.0f;
         if (!(fabsf(fa - fb) < 0.001f))
             t = (iso - fa) / (fb - fa);
         const float ax = (float)corner_x(ca), ay = (float)corner_y(ca), az = (float)corner_z(ca);
-        const float px = (ax + t * ((float)corner_x(cb) - ax) + vx) + 0.5f;
-        const float py = (ay + t * ((float)corner_y(cb) - ay) + vy) + 0.5f;
-        const float pz = (az + t * ((float)corner_z(cb) - az) + vz) + 0.5f;
-        const unsigned long long o = out0 + j;
+        const float px = (ax + t * ((float)corner_x(cb) - ax) + (float)cx) + 0.5f;
+        const float py = (ay + t * ((float)corner_y(cb) - ay) + (float)cy) + 0.5f;
+        const float pz = (az + t * ((float)corner_z(cb) - az) + (float)cz) + 0.5f;
+        const unsigned long long o = out0 + v;
+        if (o >= cap)
+            break;
         if (fmt == BCMC_VERTS_LOCAL_F32) {
             reinterpret_cast<float4 *>(verts)[o] = make_float4(px, py, pz, __uint_as_float(bid));
         } else if (fmt == BCMC_VERTS_GLOBAL_F32) {
@@ -161,6 +156,7 @@ BCMC_DEV void emit_block_vertices(const float *tile, const uint32_t (&d)[3], flo
             reinterpret_cast<uint2 *>(verts)[o] = make_uint2(packed, bid);
         }
     }
+    wave_lds_sync();  // the scratch is reused by the next block
 }
 
 // ---------------------------------------------------------------------------
@@ -220,7 +216,10 @@ BCMC_DEV uint32_t active_from_codes(const uint8_t *code, uint32_t b, const uint3
 }
 
 // ---------------------------------------------------------------------------
-// NB tiles per wave with the dependent load rounds shared by all NB blocks:
+// Block tiles with their ghost layer (compute_block_dims_with_ghost and
+// load_block, compute_vertex_values.comp:96-252): 5^3 voxels at x + 5y + 25z,
+// the ghost planes taken from the +x/+y/+z neighbours only when those are
+// active.  NB tiles per wave with the dependent load rounds shared by all NB blocks:
 // (1) neighbour activity flags and all 8 candidate cache slots, (2) the tile
 // voxels.  bid[], live[] and the views are wave-uniform; block j may belong to
 // a different volume/query than block j' (the batched engine).
@@ -349,10 +348,10 @@ BCMC_DEV void load_tiles(const Views &mv, const uint32_t (&bid)[NB], const bool 
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-BCMC_DEV uint32_t block_vertex_count(const float *tile, const uint32_t (&d)[3], float iso) {
-    uint32_t base;
-    const int c = cell_case(tile, d, iso, base);
-    return wave_sum(c >= 0 ? kTriCount[c] : 0u);
+BCMC_DEV uint32_t block_vertex_count(const float *tile, const uint32_t (&d)[3], float iso,
+                                    const McTables &tab) {
+    const int c = cell_case(tile, d, iso);
+    return wave_sum(c >= 0 ? tab.count[c] : 0u);
 }
 
 }  // namespace bcmc
