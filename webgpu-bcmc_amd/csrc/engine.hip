@@ -239,22 +239,24 @@ struct EngFusedArgs {
     int fmt;
 };
 
-BCMC_DEV McView eng_view(const EngFusedArgs &a, uint32_t qi) {
+// The row's block of pair k: its query's volume, flags and isovalue.
+BCMC_DEV void eng_row_block(const EngFusedArgs &a, uint32_t k, bool live, RowBlock &b) {
+    const uint32_t qi = live ? a.act_q[k] : 0u;
     const EngQuery &q = a.qs[qi];
     const EngVol &v = a.vols[q.vol];
-    McView m;
-    m.cache = a.pool;
-    m.item_slot = v.slot_map;
-    m.flags = a.flags + q.flag_base;
-    m.nb[0] = v.nb[0];
-    m.nb[1] = v.nb[1];
-    m.nb[2] = v.nb[2];
-    m.capacity = a.capacity;
-    return m;
+    b.cache = a.pool;
+    b.item_slot = v.slot_map;
+    b.flags = a.flags + q.flag_base;
+    b.nb[0] = v.nb[0];
+    b.nb[1] = v.nb[1];
+    b.nb[2] = v.nb[2];
+    b.cap = a.capacity;
+    b.iso = q.iso;
+    b.live = live;
+    b.bid = live ? a.act_b[k] : 0u;
 }
 
 constexpr int kEngThreads = 256;
-constexpr int kEngBlocks = 4;
 constexpr uint32_t kEngTile = kEngThreads;
 
 // The chunked count / look-back / emit pass of mc.hip over (query, block)
@@ -262,9 +264,7 @@ constexpr uint32_t kEngTile = kEngThreads;
 // query's vertices are contiguous because pairs are in query order.
 __global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, EngFusedArgs a) {
     constexpr int NW = kEngThreads / 64;
-    constexpr uint32_t kRound = NW * kEngBlocks;
-    __shared__ float tiles[NW][kEngBlocks * 128];
-    __shared__ int32_t slots[NW][kEngBlocks * 8];
+    constexpr uint32_t kRound = NW * kRowBlocks;
     __shared__ uint32_t cnt_s[kEngTile], list_s[kEngTile], off_s[kEngTile];
     __shared__ uint32_t lds_w[NW + 1];
     __shared__ unsigned long long lds_prefix;
@@ -275,34 +275,23 @@ __global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, Eng
     const uint32_t n = a.c->num_active;
     const uint32_t ntiles = div_up(n, kEngTile);
     const uint32_t e0 = scan_site_epoch(s0);
-    const uint32_t wave = threadIdx.x / 64, lane = lane_id();
+    const uint32_t wave = threadIdx.x / 64, lane = lane_id(), row = lane >> 4;
     for (;;) {
         const uint32_t tile = next_tile(s0, &lds_tile);
         if (tile >= ntiles)
             break;
         const uint32_t k0 = tile * kEngTile;
         for (uint32_t r = 0; r < kEngTile; r += kRound) {
-            const uint32_t kk = r + wave * kEngBlocks;
-            uint32_t bid[kEngBlocks], d[kEngBlocks][3], qi[kEngBlocks];
-            bool live[kEngBlocks];
-            PerBlockViews<kEngBlocks> mv;
-#pragma unroll
-            for (int j = 0; j < kEngBlocks; ++j) {
-                live[j] = k0 + kk + j < n;
-                qi[j] = live[j] ? a.act_q[k0 + kk + j] : 0u;
-                bid[j] = live[j] ? a.act_b[k0 + kk + j] : 0u;
-                mv.m[j] = eng_view(a, qi[j]);
-            }
-            load_tiles<kEngBlocks>(mv, bid, live, tiles[wave], slots[wave], d);
-#pragma unroll
-            for (int j = 0; j < kEngBlocks; ++j) {
-                const uint32_t c = live[j] ? block_vertex_count(tiles[wave] + 128 * j, d[j], a.qs[qi[j]].iso, tab)
-                                           : 0u;
-                if (lane == 0) {
-                    cnt_s[kk + j] = c;
-                    if (c)
-                        atomicAdd(&a.q_verts[qi[j]], (unsigned long long)c);
-                }
+            const uint32_t kk = r + wave * kRowBlocks + row;
+            RowBlock b;
+            eng_row_block(a, k0 + kk, k0 + kk < n, b);
+            RowTile t;
+            row_load(b, t);
+            const uint32_t c = row_block_count(t, b.iso, tab);
+            if ((lane & 15u) == 15u) {
+                cnt_s[kk] = c;
+                if (c)
+                    atomicAdd(&a.q_verts[a.act_q[k0 + kk]], (unsigned long long)c);
             }
         }
         __syncthreads();
@@ -319,26 +308,16 @@ __global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, Eng
             a.c->num_verts = pv + agg_v;
         __syncthreads();
         for (uint32_t r = 0; r < agg_h; r += kRound) {
-            const uint32_t hh = r + wave * kEngBlocks;
-            uint32_t bid[kEngBlocks], d[kEngBlocks][3], loc[kEngBlocks], qi[kEngBlocks];
-            bool live[kEngBlocks];
-            PerBlockViews<kEngBlocks> mv;
-#pragma unroll
-            for (int j = 0; j < kEngBlocks; ++j) {
-                live[j] = hh + j < agg_h;
-                loc[j] = live[j] ? list_s[hh + j] : 0u;
-                qi[j] = live[j] ? a.act_q[k0 + loc[j]] : 0u;
-                bid[j] = live[j] ? a.act_b[k0 + loc[j]] : 0u;
-                mv.m[j] = eng_view(a, qi[j]);
-            }
-            load_tiles<kEngBlocks>(mv, bid, live, tiles[wave], slots[wave], d);
-#pragma unroll
-            for (int j = 0; j < kEngBlocks; ++j) {
-                if (!live[j])
-                    continue;
-                emit_block_vertices(tiles[wave] + 128 * j, d[j], a.qs[qi[j]].iso, bid[j], mv.m[j].nb,
-                                    pv + off_s[hh + j], a.vert_cap, a.verts, a.fmt, tab, es[wave]);
-            }
+            const uint32_t hh = r + wave * kRowBlocks;
+            if (hh >= agg_h)
+                continue;
+            const uint32_t h = hh + row;
+            const bool live = h < agg_h;
+            RowBlock b;
+            eng_row_block(a, k0 + (live ? list_s[h] : 0u), live, b);
+            RowTile t;
+            row_load(b, t);
+            emit_row_blocks(b, t, tab, es[wave], pv + off_s[hh], a.vert_cap, a.verts, a.fmt);
         }
         __syncthreads();
     }
@@ -454,7 +433,7 @@ static int eng_launch_fused(bcmc_engine *e, uint32_t nq, uint64_t total) {
     a.fmt = e->fmt;
     eng_zero_u64_kernel<<<1, 256, 0, e->ctx->stream>>>(e->q_verts, nq);
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(total, kEngTile),
-                                                                   persistent_grid(e->ctx, 8)));
+                                                                   persistent_grid(e->ctx, 4)));
     eng_fused_kernel<<<grid, kEngThreads, 0, e->ctx->stream>>>(e->site_mc, a);
     BCMC_HIP_CHECK(hipGetLastError());
     return BCMC_OK;

@@ -62,19 +62,22 @@ struct ActiveOp {
 
 // ---- count + offsets + emission in one pass --------------------------------
 // Workgroups take chunks of 256 consecutive active blocks.  Sweep 1: each wave
-// loads 4 tiles at a time (shared load rounds) and counts their vertices into
-// LDS.  The chunk's two totals (blocks with vertices, vertices) chain through
-// two look-back channels, which gives every block its exact 64-bit vertex
-// offset and its index in the list of blocks with vertices.  Sweep 2: the
-// blocks with vertices reload their tiles (mostly from L2) and write their
-// vertices.  Blocks whose vertices would end past the buffer capacity write
-// nothing; the host grows the buffer and runs the pass again when the total
-// exceeds the capacity.
-constexpr int kFuseBlocks = 4;
+// counts the vertices of 4 blocks at a time (one per 16-lane row) into LDS.
+// The chunk's two totals (blocks with vertices, vertices) chain through two
+// look-back channels, which gives every block its exact 64-bit vertex offset
+// and its index in the list of blocks with vertices.  Sweep 2: each wave takes
+// 4 consecutive blocks of that list, reloads them (mostly from L2) and writes
+// their vertices as one contiguous run.  Vertices past the buffer capacity are
+// not written; the host grows the buffer and runs the pass again when the
+// total exceeds the capacity.
 constexpr uint32_t kFuseTile = kMcThreads;  // one block per thread in the chunk scan
 
 struct FusedArgs {
-    McView m;
+    const float *cache;
+    const int32_t *item_slot;
+    const uint32_t *flags;
+    uint32_t nb[3];
+    uint32_t cache_cap;
     const uint32_t *active_ids;
     QState *q;
     uint32_t *with_verts;
@@ -86,9 +89,7 @@ struct FusedArgs {
 
 __global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanSite s1, FusedArgs a) {
     constexpr int NW = kMcThreads / 64;
-    constexpr uint32_t kRound = NW * kFuseBlocks;
-    __shared__ float tiles[NW][kFuseBlocks * kTileStride];
-    __shared__ int32_t slots[NW][kFuseBlocks * 8];
+    constexpr uint32_t kRound = NW * kRowBlocks;
     __shared__ uint32_t cnt_s[kFuseTile], list_s[kFuseTile], off_s[kFuseTile];
     __shared__ uint32_t lds_w[NW + 1];
     __shared__ unsigned long long lds_prefix[2];
@@ -100,8 +101,16 @@ __global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanS
     const float iso = a.q->iso;
     const uint32_t ntiles = div_up(n, kFuseTile);
     const uint32_t e0 = scan_site_epoch(s0), e1 = scan_site_epoch(s1);
-    const uint32_t wave = threadIdx.x / 64, lane = lane_id();
-    const SameView<kFuseBlocks> mv{a.m};
+    const uint32_t wave = threadIdx.x / 64, lane = lane_id(), row = lane >> 4;
+    RowBlock b;
+    b.cache = a.cache;
+    b.item_slot = a.item_slot;
+    b.flags = a.flags;
+    b.nb[0] = a.nb[0];
+    b.nb[1] = a.nb[1];
+    b.nb[2] = a.nb[2];
+    b.cap = a.cache_cap;
+    b.iso = iso;
     if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
         a.q->num_with = 0;
         a.q->num_verts = 0;
@@ -113,21 +122,14 @@ __global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanS
         const uint32_t k0 = tile * kFuseTile;
         // sweep 1: per-block vertex counts
         for (uint32_t r = 0; r < kFuseTile; r += kRound) {
-            const uint32_t kk = r + wave * kFuseBlocks;
-            uint32_t bid[kFuseBlocks], d[kFuseBlocks][3];
-            bool live[kFuseBlocks];
-#pragma unroll
-            for (int j = 0; j < kFuseBlocks; ++j) {
-                live[j] = k0 + kk + j < n;
-                bid[j] = live[j] ? a.active_ids[k0 + kk + j] : 0u;
-            }
-            load_tiles<kFuseBlocks>(mv, bid, live, tiles[wave], slots[wave], d);
-#pragma unroll
-            for (int j = 0; j < kFuseBlocks; ++j) {
-                const uint32_t c = live[j] ? block_vertex_count(tiles[wave] + kTileStride * j, d[j], iso, tab) : 0u;
-                if (lane == 0)
-                    cnt_s[kk + j] = c;
-            }
+            const uint32_t kk = r + wave * kRowBlocks + row;
+            b.live = k0 + kk < n;
+            b.bid = b.live ? a.active_ids[k0 + kk] : 0u;
+            RowTile t;
+            row_load(b, t);
+            const uint32_t c = row_block_count(t, iso, tab);
+            if ((lane & 15u) == 15u)
+                cnt_s[kk] = c;
         }
         __syncthreads();
         // chunk scan of (has vertices, vertices), then the two look-backs
@@ -148,25 +150,17 @@ __global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanS
             a.q->num_verts = pv + agg_v;
         }
         __syncthreads();
-        // sweep 2: vertices of the blocks that have some
+        // sweep 2: vertices of 4 consecutive blocks with vertices per wave
         for (uint32_t r = 0; r < agg_h; r += kRound) {
-            const uint32_t hh = r + wave * kFuseBlocks;
-            uint32_t bid[kFuseBlocks], d[kFuseBlocks][3], loc[kFuseBlocks];
-            bool live[kFuseBlocks];
-#pragma unroll
-            for (int j = 0; j < kFuseBlocks; ++j) {
-                live[j] = hh + j < agg_h;
-                loc[j] = live[j] ? list_s[hh + j] : 0u;
-                bid[j] = live[j] ? a.active_ids[k0 + loc[j]] : 0u;
-            }
-            load_tiles<kFuseBlocks>(mv, bid, live, tiles[wave], slots[wave], d);
-#pragma unroll
-            for (int j = 0; j < kFuseBlocks; ++j) {
-                if (!live[j])
-                    continue;
-                emit_block_vertices(tiles[wave] + kTileStride * j, d[j], iso, bid[j], a.m.nb,
-                                    pv + off_s[hh + j], a.capacity, a.verts, a.fmt, tab, es[wave]);
-            }
+            const uint32_t hh = r + wave * kRowBlocks;
+            if (hh >= agg_h)
+                continue;
+            const uint32_t h = hh + row;
+            b.live = h < agg_h;
+            b.bid = b.live ? a.active_ids[k0 + list_s[h]] : 0u;
+            RowTile t;
+            row_load(b, t);
+            emit_row_blocks(b, t, tab, es[wave], pv + off_s[hh], a.capacity, a.verts, a.fmt);
         }
         __syncthreads();
     }
@@ -256,8 +250,13 @@ static void mc_refresh_views(bcmc_mc *m) {
 
 static int mc_launch_fused(bcmc_mc *m) {
     FusedArgs a;
-    a.m = McView{(const float *)m->lru.st.data, m->lru.st.v.item_slot, m->flags,
-                 {m->nb[0], m->nb[1], m->nb[2]}, m->lru.st.capacity};
+    a.cache = (const float *)m->lru.st.data;
+    a.item_slot = m->lru.st.v.item_slot;
+    a.flags = m->flags;
+    a.nb[0] = m->nb[0];
+    a.nb[1] = m->nb[1];
+    a.nb[2] = m->nb[2];
+    a.cache_cap = m->lru.st.capacity;
     a.active_ids = m->active_ids;
     a.q = m->q;
     a.with_verts = m->with_verts;
@@ -266,7 +265,7 @@ static int mc_launch_fused(bcmc_mc *m) {
     a.capacity = m->vert_cap;
     a.fmt = m->fmt;
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(m->total, kFuseTile),
-                                                                   persistent_grid(m->ctx, 8)));
+                                                                   persistent_grid(m->ctx, 4)));
     mc_fused_kernel<<<grid, kMcThreads, 0, m->ctx->stream>>>(m->site_c0, m->site_c1, a);
     BCMC_HIP_CHECK(hipGetLastError());
     return BCMC_OK;
