@@ -38,6 +38,7 @@ namespace bcmc {
 struct EngVol {                 // device descriptor of a volume
     uint32_t nb[3];
     uint32_t n;
+    FastDiv div_x, div_y;       // by nb[0], nb[1]
     const float2 *ranges;
     int32_t *slot_map;          // block -> pool slot; -1 none, -2 claimed this step
     uint32_t *miss_list;        // this step's missing blocks of the volume (decoder input)
@@ -239,9 +240,8 @@ struct EngFusedArgs {
     int fmt;
 };
 
-// The row's block of pair k: its query's volume, flags and isovalue.
-BCMC_DEV void eng_row_block(const EngFusedArgs &a, uint32_t k, bool live, RowBlock &b) {
-    const uint32_t qi = live ? a.act_q[k] : 0u;
+// The RowBlock of a (query, block) pair: its query's volume, flags, isovalue.
+BCMC_DEV void eng_row_block(const EngFusedArgs &a, uint32_t qi, uint32_t bid, bool live, RowBlock &b) {
     const EngQuery &q = a.qs[qi];
     const EngVol &v = a.vols[q.vol];
     b.cache = a.pool;
@@ -250,22 +250,22 @@ BCMC_DEV void eng_row_block(const EngFusedArgs &a, uint32_t k, bool live, RowBlo
     b.nb[0] = v.nb[0];
     b.nb[1] = v.nb[1];
     b.nb[2] = v.nb[2];
+    b.div_x = v.div_x;
+    b.div_y = v.div_y;
     b.cap = a.capacity;
     b.iso = q.iso;
     b.live = live;
-    b.bid = live ? a.act_b[k] : 0u;
+    b.bid = bid;
 }
 
-constexpr int kEngThreads = 256;
-constexpr uint32_t kEngTile = kEngThreads;
+constexpr int kEngThreads = kMcThreads;
 
 // The chunked count / look-back / emit pass of mc.hip over (query, block)
 // pairs of many queries; the vertex channel's offsets are global, and each
 // query's vertices are contiguous because pairs are in query order.
 __global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, EngFusedArgs a) {
     constexpr int NW = kEngThreads / 64;
-    constexpr uint32_t kRound = NW * kRowBlocks;
-    __shared__ uint32_t cnt_s[kEngTile], list_s[kEngTile], off_s[kEngTile];
+    __shared__ ChunkShared cs;
     __shared__ uint32_t lds_w[NW + 1];
     __shared__ unsigned long long lds_prefix;
     __shared__ uint32_t lds_tile;
@@ -273,52 +273,36 @@ __global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, Eng
     __shared__ EmitScratch es[NW];
     load_mc_tables(&tab);
     const uint32_t n = a.c->num_active;
-    const uint32_t ntiles = div_up(n, kEngTile);
+    const uint32_t ntiles = div_up(n, kChunk);
     const uint32_t e0 = scan_site_epoch(s0);
-    const uint32_t wave = threadIdx.x / 64, lane = lane_id(), row = lane >> 4;
+    const uint32_t wave = threadIdx.x / 64;
     for (;;) {
         const uint32_t tile = next_tile(s0, &lds_tile);
         if (tile >= ntiles)
             break;
-        const uint32_t k0 = tile * kEngTile;
-        for (uint32_t r = 0; r < kEngTile; r += kRound) {
-            const uint32_t kk = r + wave * kRowBlocks + row;
-            RowBlock b;
-            eng_row_block(a, k0 + kk, k0 + kk < n, b);
-            RowTile t;
-            row_load(b, t);
-            const uint32_t c = row_block_count(t, b.iso, tab);
-            if ((lane & 15u) == 15u) {
-                cnt_s[kk] = c;
-                if (c)
-                    atomicAdd(&a.q_verts[a.act_q[k0 + kk]], (unsigned long long)c);
-            }
-        }
+        const uint32_t k0 = tile * kChunk;
+        const bool in = k0 + threadIdx.x < n;
+        cs.bid[threadIdx.x] = in ? a.act_b[k0 + threadIdx.x] : 0u;
+        cs.aux[threadIdx.x] = in ? a.act_q[k0 + threadIdx.x] : 0u;
         __syncthreads();
-        const uint32_t c = cnt_s[threadIdx.x];
+        auto setup = [&](uint32_t e, RowBlock &b) { eng_row_block(a, cs.aux[e], cs.bid[e], k0 + e < n, b); };
+        count_sweep(cs, wave, tab, setup);
+        __syncthreads();
+        const uint32_t c = cs.cnt[threadIdx.x];
+        if (c)
+            atomicAdd(&a.q_verts[cs.aux[threadIdx.x]], (unsigned long long)c);
         uint32_t agg_h, agg_v;
         const uint32_t ex_h = block_excl_scan<kEngThreads>(c ? 1u : 0u, lds_w, &agg_h);
         const uint32_t ex_v = block_excl_scan<kEngThreads>(c, lds_w, &agg_v);
         const unsigned long long pv = tile_lookback<kEngThreads>(s0, tile, e0, agg_v, &lds_prefix);
         if (c) {
-            list_s[ex_h] = threadIdx.x;
-            off_s[ex_h] = ex_v;
+            cs.list[ex_h] = threadIdx.x;
+            cs.off[ex_h] = ex_v;
         }
         if (tile == ntiles - 1 && threadIdx.x == 0)
             a.c->num_verts = pv + agg_v;
         __syncthreads();
-        for (uint32_t r = 0; r < agg_h; r += kRound) {
-            const uint32_t hh = r + wave * kRowBlocks;
-            if (hh >= agg_h)
-                continue;
-            const uint32_t h = hh + row;
-            const bool live = h < agg_h;
-            RowBlock b;
-            eng_row_block(a, k0 + (live ? list_s[h] : 0u), live, b);
-            RowTile t;
-            row_load(b, t);
-            emit_row_blocks(b, t, tab, es[wave], pv + off_s[hh], a.vert_cap, a.verts, a.fmt);
-        }
+        emit_sweep(cs, wave, agg_h, pv, tab, es[wave], a.vert_cap, a.verts, a.fmt, setup);
         __syncthreads();
     }
     scan_site_finish(s0, e0);
@@ -392,7 +376,7 @@ static int eng_ensure_blocks(bcmc_engine *e, uint64_t total) {
     BCMC_TRY(dev_alloc((void **)&e->code, cap));
     BCMC_TRY(dev_alloc((void **)&e->vert_offsets, 8));
     BCMC_TRY(scan_site_alloc(&e->site_active, cap, kScanTile));
-    BCMC_TRY(scan_site_alloc(&e->site_mc, cap, kEngTile));
+    BCMC_TRY(scan_site_alloc(&e->site_mc, cap, kChunk));
     e->blocks_cap = cap;
     return BCMC_OK;
 }
@@ -432,7 +416,7 @@ static int eng_launch_fused(bcmc_engine *e, uint32_t nq, uint64_t total) {
     a.vert_cap = e->vert_cap;
     a.fmt = e->fmt;
     eng_zero_u64_kernel<<<1, 256, 0, e->ctx->stream>>>(e->q_verts, nq);
-    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(total, kEngTile),
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(total, kChunk),
                                                                    persistent_grid(e->ctx, 4)));
     eng_fused_kernel<<<grid, kEngThreads, 0, e->ctx->stream>>>(e->site_mc, a);
     BCMC_HIP_CHECK(hipGetLastError());
@@ -449,7 +433,10 @@ BCMC_API int bcmc_engine_create(bcmc_ctx *ctx, uint64_t pool_bytes, bcmc_engine 
     e->ctx = ctx;
     e->budget = pool_bytes;
     e->pool_cap = (uint32_t)std::min<uint64_t>(pool_bytes / 256, 0x7FFFFFC0ull);
-    int rc = dev_alloc((void **)&e->pool, (size_t)e->pool_cap * 256);
+    // one extra zeroed slot past the end: absent blocks read it
+    int rc = dev_alloc((void **)&e->pool, (size_t)e->pool_cap * 256 + 256);
+    if (rc == BCMC_OK && hipMemsetAsync((char *)e->pool + (size_t)e->pool_cap * 256, 0, 256, ctx->stream) != hipSuccess)
+        rc = BCMC_ERR_HIP;
     if (rc == BCMC_OK)
         rc = dev_alloc((void **)&e->stamp, (size_t)e->pool_cap * 4);
     if (rc == BCMC_OK)
@@ -553,6 +540,8 @@ BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t
     for (int i = 0; i < 3; ++i)
         dv.nb[i] = v.nb[i];
     dv.n = v.n;
+    dv.div_x = make_fast_div(v.nb[0]);
+    dv.div_y = make_fast_div(v.nb[1]);
     dv.ranges = v.ranges;
     dv.slot_map = v.slot_map;
     dv.miss_list = v.miss_list;

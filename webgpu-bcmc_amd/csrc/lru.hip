@@ -186,7 +186,9 @@ int lru_state_alloc(bcmc_ctx *ctx, LruState *L, uint32_t initial, uint32_t elem_
     BCMC_TRY(dev_alloc((void **)&L->v.avail_ids, cap * 4));
     BCMC_TRY(dev_alloc((void **)&L->v.order, cap * 4));
     BCMC_TRY(dev_alloc((void **)&L->v.item_slot, (size_t)L->aligned_total * 4 + 4));
-    BCMC_TRY(dev_alloc((void **)&L->data, cap * elem_bytes));
+    // one extra zeroed 256-byte slot past the end: absent blocks read it
+    BCMC_TRY(dev_alloc((void **)&L->data, cap * elem_bytes + 256));
+    BCMC_HIP_CHECK(hipMemsetAsync((char *)L->data + cap * elem_bytes, 0, 256, ctx->stream));
     BCMC_TRY(dev_alloc((void **)&L->new_items, (size_t)L->aligned_total * 4 + 4));
     BCMC_TRY(scan_site_alloc(&L->site_miss, L->aligned_total, kScanTile));
     BCMC_TRY(scan_site_alloc(&L->site_avail, cap, kScanTile));

@@ -70,13 +70,12 @@ struct ActiveOp {
 // their vertices as one contiguous run.  Vertices past the buffer capacity are
 // not written; the host grows the buffer and runs the pass again when the
 // total exceeds the capacity.
-constexpr uint32_t kFuseTile = kMcThreads;  // one block per thread in the chunk scan
-
 struct FusedArgs {
     const float *cache;
     const int32_t *item_slot;
     const uint32_t *flags;
     uint32_t nb[3];
+    FastDiv div_x, div_y;
     uint32_t cache_cap;
     const uint32_t *active_ids;
     QState *q;
@@ -89,8 +88,7 @@ struct FusedArgs {
 
 __global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanSite s1, FusedArgs a) {
     constexpr int NW = kMcThreads / 64;
-    constexpr uint32_t kRound = NW * kRowBlocks;
-    __shared__ uint32_t cnt_s[kFuseTile], list_s[kFuseTile], off_s[kFuseTile];
+    __shared__ ChunkShared cs;
     __shared__ uint32_t lds_w[NW + 1];
     __shared__ unsigned long long lds_prefix[2];
     __shared__ uint32_t lds_tile;
@@ -98,19 +96,20 @@ __global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanS
     __shared__ EmitScratch es[NW];
     load_mc_tables(&tab);
     const uint32_t n = a.q->num_active;
-    const float iso = a.q->iso;
-    const uint32_t ntiles = div_up(n, kFuseTile);
+    const uint32_t ntiles = div_up(n, kChunk);
     const uint32_t e0 = scan_site_epoch(s0), e1 = scan_site_epoch(s1);
-    const uint32_t wave = threadIdx.x / 64, lane = lane_id(), row = lane >> 4;
-    RowBlock b;
-    b.cache = a.cache;
-    b.item_slot = a.item_slot;
-    b.flags = a.flags;
-    b.nb[0] = a.nb[0];
-    b.nb[1] = a.nb[1];
-    b.nb[2] = a.nb[2];
-    b.cap = a.cache_cap;
-    b.iso = iso;
+    const uint32_t wave = threadIdx.x / 64;
+    RowBlock base;
+    base.cache = a.cache;
+    base.item_slot = a.item_slot;
+    base.flags = a.flags;
+    base.nb[0] = a.nb[0];
+    base.nb[1] = a.nb[1];
+    base.nb[2] = a.nb[2];
+    base.div_x = a.div_x;
+    base.div_y = a.div_y;
+    base.cap = a.cache_cap;
+    base.iso = a.q->iso;
     if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
         a.q->num_with = 0;
         a.q->num_verts = 0;
@@ -119,29 +118,27 @@ __global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanS
         const uint32_t tile = next_tile(s0, &lds_tile);
         if (tile >= ntiles)
             break;
-        const uint32_t k0 = tile * kFuseTile;
+        const uint32_t k0 = tile * kChunk;
+        cs.bid[threadIdx.x] = k0 + threadIdx.x < n ? a.active_ids[k0 + threadIdx.x] : 0u;
+        __syncthreads();
+        auto setup = [&](uint32_t e, RowBlock &b) {
+            b = base;
+            b.live = k0 + e < n;
+            b.bid = cs.bid[e];
+        };
         // sweep 1: per-block vertex counts
-        for (uint32_t r = 0; r < kFuseTile; r += kRound) {
-            const uint32_t kk = r + wave * kRowBlocks + row;
-            b.live = k0 + kk < n;
-            b.bid = b.live ? a.active_ids[k0 + kk] : 0u;
-            RowTile t;
-            row_load(b, t);
-            const uint32_t c = row_block_count(t, iso, tab);
-            if ((lane & 15u) == 15u)
-                cnt_s[kk] = c;
-        }
+        count_sweep(cs, wave, tab, setup);
         __syncthreads();
         // chunk scan of (has vertices, vertices), then the two look-backs
-        const uint32_t c = cnt_s[threadIdx.x];
+        const uint32_t c = cs.cnt[threadIdx.x];
         uint32_t agg_h, agg_v;
         const uint32_t ex_h = block_excl_scan<kMcThreads>(c ? 1u : 0u, lds_w, &agg_h);
         const uint32_t ex_v = block_excl_scan<kMcThreads>(c, lds_w, &agg_v);
         const unsigned long long ph = tile_lookback<kMcThreads>(s0, tile, e0, agg_h, &lds_prefix[0]);
         const unsigned long long pv = tile_lookback<kMcThreads>(s1, tile, e1, agg_v, &lds_prefix[1]);
         if (c) {
-            list_s[ex_h] = threadIdx.x;
-            off_s[ex_h] = ex_v;
+            cs.list[ex_h] = threadIdx.x;
+            cs.off[ex_h] = ex_v;
             a.with_verts[ph + ex_h] = k0 + threadIdx.x;
             a.vert_offsets[ph + ex_h] = pv + ex_v;
         }
@@ -151,17 +148,7 @@ __global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanS
         }
         __syncthreads();
         // sweep 2: vertices of 4 consecutive blocks with vertices per wave
-        for (uint32_t r = 0; r < agg_h; r += kRound) {
-            const uint32_t hh = r + wave * kRowBlocks;
-            if (hh >= agg_h)
-                continue;
-            const uint32_t h = hh + row;
-            b.live = h < agg_h;
-            b.bid = b.live ? a.active_ids[k0 + list_s[h]] : 0u;
-            RowTile t;
-            row_load(b, t);
-            emit_row_blocks(b, t, tab, es[wave], pv + off_s[hh], a.capacity, a.verts, a.fmt);
-        }
+        emit_sweep(cs, wave, agg_h, pv, tab, es[wave], a.capacity, a.verts, a.fmt, setup);
         __syncthreads();
     }
     scan_site_finish(s0, e0);
@@ -256,6 +243,8 @@ static int mc_launch_fused(bcmc_mc *m) {
     a.nb[0] = m->nb[0];
     a.nb[1] = m->nb[1];
     a.nb[2] = m->nb[2];
+    a.div_x = make_fast_div(m->nb[0]);
+    a.div_y = make_fast_div(m->nb[1]);
     a.cache_cap = m->lru.st.capacity;
     a.active_ids = m->active_ids;
     a.q = m->q;
@@ -264,7 +253,7 @@ static int mc_launch_fused(bcmc_mc *m) {
     a.verts = m->verts;
     a.capacity = m->vert_cap;
     a.fmt = m->fmt;
-    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(m->total, kFuseTile),
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(m->total, kChunk),
                                                                    persistent_grid(m->ctx, 4)));
     mc_fused_kernel<<<grid, kMcThreads, 0, m->ctx->stream>>>(m->site_c0, m->site_c1, a);
     BCMC_HIP_CHECK(hipGetLastError());
@@ -346,8 +335,8 @@ BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *m, const void *stream, uint6
     BCMC_TRY(dev_alloc((void **)&m->with_verts, n * 4));
     BCMC_TRY(dev_alloc((void **)&m->vert_offsets, n * 8));
     BCMC_TRY(scan_site_alloc(&m->site_active, n, kScanTile));
-    BCMC_TRY(scan_site_alloc(&m->site_c0, n, kFuseTile));
-    BCMC_TRY(scan_site_alloc(&m->site_c1, n, kFuseTile));
+    BCMC_TRY(scan_site_alloc(&m->site_c0, n, kChunk));
+    BCMC_TRY(scan_site_alloc(&m->site_c1, n, kChunk));
     const uint32_t init = initial_cache_slots ? initial_cache_slots
                                               : (uint32_t)std::ceil((double)total * 0.05);
     BCMC_TRY(lru_state_alloc(ctx, &m->lru.st, init, 256, m->total));

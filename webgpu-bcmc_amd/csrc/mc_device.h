@@ -6,8 +6,14 @@
 // holds the voxel row (x = 0..4, y, z) in registers; the rows at y + 1 and
 // z + 1 come from lanes l + 1, l + 4 and l + 5 through DPP row shifts, and the
 // lanes on the block's upper faces also load the ghost rows from the +y/+z
-// neighbours.  Corner tests become 5-bit row masks, so a cell's case is a
-// fixed gather of 8 bits from 4 masks.
+// neighbours.  The corner tests (v <= iso) of the 4 rows a lane's cells touch
+// are interleaved into one word W with row r's bit at voxel x in bit 4x + r,
+// so the 8 corners of cell x are the byte (W >> 4x) & 0xFF; the triangle
+// tables are re-indexed by that byte once per workgroup.
+//
+// Chunks of kChunk active blocks go to a workgroup; a count sweep stores each
+// block's cache slots and tile dims in LDS, so the emit sweep only reloads
+// voxels.
 #pragma once
 
 #include "common.h"
@@ -17,9 +23,10 @@ namespace bcmc {
 
 static __constant__ unsigned long long kTriEdges[256] = BCMC_TRI_EDGES_INIT;
 static __constant__ uint8_t kTriCount[256] = BCMC_TRI_COUNT_INIT;
-// edge_vertices (compute_block_vertices.comp:10-23) as two nibbles per edge
-constexpr uint8_t kEdgeVertsHost[12] = {0x10, 0x21, 0x32, 0x03, 0x54, 0x56,
-                                        0x76, 0x47, 0x40, 0x51, 0x62, 0x73};
+// edge_vertices (compute_block_vertices.comp:10-23): edge e joins corners
+// a = byte e & 15 and b = byte e >> 4 (bytes from the low end; 12 edges)
+constexpr unsigned long long kEdgePairsLo = 0x4776565403322110ull;  // edges 0..7
+constexpr unsigned long long kEdgePairsHi = 0x73625140ull;          // edges 8..11
 
 // index_to_vertex (compute_vertex_values.comp:48-57): corner i at
 // (x, y, z) = (((i+1)>>1)&1, (i>>1)&1, i>>2)
@@ -27,22 +34,89 @@ __host__ __device__ constexpr uint32_t corner_x(uint32_t i) { return ((i + 1) >>
 __host__ __device__ constexpr uint32_t corner_y(uint32_t i) { return (i >> 1) & 1u; }
 __host__ __device__ constexpr uint32_t corner_z(uint32_t i) { return i >> 2; }
 
-constexpr int kMcThreads = 256;   // 4 waves per workgroup
-constexpr int kRowBlocks = 4;     // blocks per wave (one per 16-lane row)
+constexpr int kMcThreads = 256;          // 4 waves per workgroup
+constexpr int kRowBlocks = 4;            // blocks per wave (one per 16-lane row)
+constexpr uint32_t kChunk = kMcThreads;  // active blocks per chunk (one per thread in its scan)
 // Block tile in LDS: 5^3 voxels at x + 8y + 40z (rows 32-byte aligned).
 constexpr uint32_t kTileY = 8, kTileZ = 40, kTileFloats = 200;
 
-// Triangle tables staged in LDS once per workgroup (the per-cell lookups are
-// divergent, so they would otherwise be vector loads through the L1).
-struct McTables {
-    unsigned long long edges[256];
-    uint8_t count[256];
+// Corner byte B of a cell (bit 4 dx + r, r = dy + 2 dz over the rows
+// (y, z), (y+1, z), (y, z+1), (y+1, z+1)) -> reference case index (bit i iff
+// corner i <= iso).  Corner i sits at B bit kCornerBit[i].
+__host__ __device__ constexpr uint32_t corner_byte_bit(uint32_t i) {
+    return 4 * corner_x(i) + corner_y(i) + 2 * corner_z(i);
+}
+__host__ __device__ constexpr uint32_t case_of_corner_byte(uint32_t byte) {
+    uint32_t c = 0;
+    for (uint32_t i = 0; i < 8; ++i)
+        c |= ((byte >> corner_byte_bit(i)) & 1u) << i;
+    return c;
+}
+
+// Interpolation data of edge e: corner a's position, b - a, and both corners'
+// offsets in the tile.
+struct EdgeInfo {
+    float a[3];
+    float off_a;
+    float d[3];
+    float off_b;
 };
+
+// Tables staged in LDS once per workgroup (the lookups are divergent, so
+// they would otherwise be vector loads through the L1).
+struct McTables {
+    unsigned long long edges[256];  // by corner byte
+    uint8_t count[256];             // by corner byte: vertices of the cell
+    uint8_t dims[1024];             // tile-dims rule, see tile_dims_rule
+    EdgeInfo edge[12];
+};
+
+// compute_block_dims_with_ghost (compute_vertex_values.comp:96-170): the
+// ghost planes are part of the tile only where the +x/+y/+z neighbours (and,
+// for the shared edges and corner, the diagonal neighbours) are active.
+// idx bits 0..6: neighbour +x, +y, +z, +xy, +xz, +yz, +xyz active; bits 7..9:
+// the block is not last in x, y, z.  Returns dx5 | dy5 << 1 | dz5 << 2.
+BCMC_DEV uint32_t tile_dims_rule(uint32_t idx) {
+    const uint32_t a = idx & 0x7Fu;
+    bool dx = (idx >> 7) & 1u, dy = (idx >> 8) & 1u, dz = (idx >> 9) & 1u;
+    if (dx && dy && dz && !(a & 64))
+        dx = dy = dz = false;
+    if (dx && dy && !(a & 8))
+        dx = dy = false;
+    if (dx && dz && !(a & 16))
+        dx = dz = false;
+    if (dy && dz && !(a & 32))
+        dy = dz = false;
+    if (dx && !(a & 1))
+        dx = false;
+    if (dy && !(a & 2))
+        dy = false;
+    if (dz && !(a & 4))
+        dz = false;
+    return (dx ? 1u : 0u) | (dy ? 2u : 0u) | (dz ? 4u : 0u);
+}
 
 BCMC_DEV void load_mc_tables(McTables *t) {
     for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) {
-        t->edges[i] = kTriEdges[i];
-        t->count[i] = kTriCount[i];
+        const uint32_t c = case_of_corner_byte(i);
+        t->edges[i] = kTriEdges[c];
+        t->count[i] = kTriCount[c];
+    }
+    for (uint32_t i = threadIdx.x; i < 1024; i += blockDim.x)
+        t->dims[i] = (uint8_t)tile_dims_rule(i);
+    if (threadIdx.x < 12) {
+        const uint32_t e = threadIdx.x;
+        const uint32_t pair = (uint32_t)((e < 8 ? kEdgePairsLo >> (8 * e) : kEdgePairsHi >> (8 * (e - 8))) & 0xFFu);
+        const uint32_t ca = pair & 15u, cb = pair >> 4;
+        EdgeInfo &g = t->edge[e];
+        g.a[0] = (float)corner_x(ca);
+        g.a[1] = (float)corner_y(ca);
+        g.a[2] = (float)corner_z(ca);
+        g.d[0] = (float)corner_x(cb) - g.a[0];
+        g.d[1] = (float)corner_y(cb) - g.a[1];
+        g.d[2] = (float)corner_z(cb) - g.a[2];
+        g.off_a = __uint_as_float(corner_x(ca) + kTileY * corner_y(ca) + kTileZ * corner_z(ca));
+        g.off_b = __uint_as_float(corner_x(cb) + kTileY * corner_y(cb) + kTileZ * corner_z(cb));
     }
     __syncthreads();
 }
@@ -68,126 +142,168 @@ BCMC_DEV uint32_t row_incl_scan(uint32_t v) {
     return v;
 }
 
+// Inclusive running maximum across the wave (same DPP pattern as the sum).
+BCMC_DEV uint32_t wave_incl_max(uint32_t v) {
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false));
+    return v;
+}
+
+// n / d for n < 2^31 as a multiply-high: with c = ceil(log2 d), k = 31 + c and
+// m = ceil(2^k / d), floor(n m / 2^k) = floor(n / d) because the rounding error
+// n (m d - 2^k) / (d 2^k) stays below 2^(31-k) <= 1/d.
+struct FastDiv {
+    uint32_t m, k;
+};
+
+inline FastDiv make_fast_div(uint32_t d) {
+    uint32_t c = 0;
+    while ((1ull << c) < d)
+        ++c;
+    const uint32_t k = 31 + c;
+    return FastDiv{(uint32_t)(((1ull << k) + d - 1) / d), k};
+}
+
+BCMC_DEV uint32_t fast_div(uint32_t n, FastDiv f) { return (uint32_t)(((uint64_t)n * f.m) >> f.k); }
+
 // The block a row works on, as seen by each of its lanes.  In the per-query
 // pipeline every field but bid/live is uniform; in the batched engine rows
 // may belong to different volumes and isovalues.
 struct RowBlock {
-    const float *cache;        // decoded blocks, 64 floats per slot
+    const float *cache;        // decoded blocks, 64 floats per slot; slot `cap` reads as zeros
     const int32_t *item_slot;  // block -> cache slot (< 0: not cached)
     const uint32_t *flags;     // active flags of the query
     uint32_t nb[3];
-    uint32_t cap;              // cache slots (reads outside are skipped)
-    uint32_t bid;
+    FastDiv div_x, div_y;      // by nb[0], nb[1]
+    uint32_t cap;              // cache slots
     float iso;
+    uint32_t bid;
     bool live;
 };
 
-// Voxels held by a lane: its row and, on the upper faces, the ghost rows
-// (y+1 | z+1 | y+1,z+1) from the neighbouring blocks; tile dims (4 or 5 each).
-struct RowTile {
-    float r[5], gy[5], gz[5], gyz[5];
-    uint32_t dx, dy, dz;
-    uint32_t bx, by, bz;
-    bool live;
-};
-
-BCMC_DEV int32_t row_slot(const RowBlock &b, uint32_t bx, uint32_t by, uint32_t bz, uint32_t s) {
-    const uint32_t x = bx + (s & 1u), y = by + ((s >> 1) & 1u), z = bz + (s >> 2);
-    if (!b.live || x >= b.nb[0] || y >= b.nb[1] || z >= b.nb[2])
-        return -1;
-    return b.item_slot[x + b.nb[0] * (y + b.nb[1] * z)];
+BCMC_DEV void block_coords(const RowBlock &b, uint32_t &bx, uint32_t &by, uint32_t &bz) {
+    const uint32_t q1 = fast_div(b.bid, b.div_x), q2 = fast_div(q1, b.div_y);
+    bx = b.bid - q1 * b.nb[0];
+    by = q1 - q2 * b.nb[1];
+    bz = q2;
 }
 
+// Per-block words kept in LDS between the count and emit sweeps of a chunk:
+// the cache slots of the 8 sub-blocks (s = sx + 2sy + 4sz, -1 when absent)
+// and the tile dims (dx5 | dy5 << 1 | dz5 << 2).
+constexpr uint32_t kMetaWords = 9;
+
+// One load per lane: lanes 0..7 of the row fetch the cache slot of sub-block
+// l, lanes 8..14 the active flag of neighbour +x, +y, +z, +xy, +xz, +yz,
+// +xyz (the reference's check order).
+struct RowMeta {
+    int32_t v;
+    uint32_t grid;  // block not last in x | y << 1 | z << 2
+};
+
+BCMC_DEV RowMeta row_meta_load(const RowBlock &b) {
+    const uint32_t l = lane_id() & 15u;
+    uint32_t bx, by, bz;
+    block_coords(b, bx, by, bz);
+    RowMeta m;
+    m.grid = (bx + 1 < b.nb[0] ? 1u : 0u) | (by + 1 < b.nb[1] ? 2u : 0u) | (bz + 1 < b.nb[2] ? 4u : 0u);
+    const uint32_t f = l - 8;
+    const uint32_t code = l < 8 ? l : (f < 3 ? (1u << f) : (f == 3 ? 3u : f == 4 ? 5u : f == 5 ? 6u : 7u));
+    const uint32_t x = bx + (code & 1u), y = by + ((code >> 1) & 1u), z = bz + (code >> 2);
+    m.v = l < 8 ? -1 : 0;
+    if (b.live && l < 15 && x < b.nb[0] && y < b.nb[1] && z < b.nb[2]) {
+        const uint32_t idx = x + b.nb[0] * (y + b.nb[1] * z);
+        // raw words: testing the flag here would wait for every load in flight
+        m.v = l < 8 ? b.item_slot[idx] : (int32_t)b.flags[idx];
+    }
+    return m;
+}
+
+// Stores the row's slots and tile dims into its block's meta words.  Must be
+// called by all 64 lanes.
+BCMC_DEV void row_meta_store(const RowMeta &m, const McTables &tab, uint32_t *meta) {
+    const uint32_t lane = lane_id(), l = lane & 15u;
+    const uint64_t ab = __ballot(l >= 8 && l < 15 && m.v != 0);
+    const uint32_t a = (uint32_t)(ab >> ((lane & 48u) + 8u)) & 0x7Fu;
+    if (l < 8)
+        meta[l] = (uint32_t)m.v;
+    if (l == 8)
+        meta[8] = tab.dims[a | (m.grid << 7)];
+}
+
+// Voxels held by a lane: its row and, on the upper faces, the ghost rows
+// (y+1 | z+1 | y+1,z+1) from the neighbouring blocks.
+struct RowTile {
+    float r[5], gy[5], gz[5], gyz[5];
+    uint32_t dims;  // dx5 | dy5 << 1 | dz5 << 2
+    bool live;
+};
+
 BCMC_DEV void row_fetch(const RowBlock &b, int32_t s, int32_t sx, uint32_t off, float (&r)[5]) {
-    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float w = 0.0f;
-    if (s >= 0 && (uint32_t)s < b.cap)
-        v = *reinterpret_cast<const float4 *>(b.cache + (uint64_t)(uint32_t)s * 64 + off);
-    if (sx >= 0 && (uint32_t)sx < b.cap)
-        w = b.cache[(uint64_t)(uint32_t)sx * 64 + off];
+    const uint32_t si = (s >= 0 && (uint32_t)s < b.cap) ? (uint32_t)s : b.cap;
+    const uint32_t sj = (sx >= 0 && (uint32_t)sx < b.cap) ? (uint32_t)sx : b.cap;
+    const float4 v = *reinterpret_cast<const float4 *>(b.cache + (uint64_t)si * 64 + off);
     r[0] = v.x;
     r[1] = v.y;
     r[2] = v.z;
     r[3] = v.w;
-    r[4] = w;
+    r[4] = b.cache[(uint64_t)sj * 64 + off];
 }
 
-// compute_block_dims_with_ghost + load_block (compute_vertex_values.comp:96-252):
-// the ghost planes are part of the tile only where the +x/+y/+z neighbours
-// (and, for the shared edges and corner, the diagonal neighbours) are active.
-// Must be called by all 64 lanes.
-BCMC_DEV void row_load(const RowBlock &b, RowTile &t) {
-    const uint32_t lane = lane_id(), l = lane & 15u, y = l & 3u, z = l >> 2;
+// load_block (compute_vertex_values.comp:171-252) for this lane's rows, from
+// the block's meta words: own row (x = 0..3 of the block, x = 4 from +x), then
+// the ghost rows of the upper faces.  Absent blocks read the zero slot.
+BCMC_DEV void row_voxels(const RowBlock &b, const uint32_t *meta, RowTile &t) {
+    const uint32_t l = lane_id() & 15u, y = l & 3u, z = l >> 2;
+    t.dims = meta[8];
     t.live = b.live;
-    t.bx = b.bid % b.nb[0];
-    t.by = (b.bid / b.nb[0]) % b.nb[1];
-    t.bz = b.bid / (b.nb[0] * b.nb[1]);
-    // neighbour activity, in the reference's check order: lanes 0..6 of the
-    // row test +x, +y, +z, +xy, +xz, +yz, +xyz
-    bool act = false;
-    if (b.live && l < 7) {
-        const uint32_t code = l < 3 ? (1u << l) : (l == 3 ? 3u : l == 4 ? 5u : l == 5 ? 6u : 7u);
-        const uint32_t nx = t.bx + (code & 1u), ny = t.by + ((code >> 1) & 1u), nz = t.bz + (code >> 2);
-        if (nx < b.nb[0] && ny < b.nb[1] && nz < b.nb[2])
-            act = b.flags[nx + b.nb[0] * (ny + b.nb[1] * nz)] != 0;
+    int32_t s0 = -1, s1 = -1, s2 = -1, s3 = -1, s4 = -1, s5 = -1, s6 = -1, s7 = -1;
+    if (b.live) {
+        s0 = (int32_t)meta[0];
+        s1 = (int32_t)meta[1];
+        if (y == 3) {
+            s2 = (int32_t)meta[2];
+            s3 = (int32_t)meta[3];
+        }
+        if (z == 3) {
+            s4 = (int32_t)meta[4];
+            s5 = (int32_t)meta[5];
+        }
+        if (y == 3 && z == 3) {
+            s6 = (int32_t)meta[6];
+            s7 = (int32_t)meta[7];
+        }
     }
-    // cache slots of the sub-blocks this lane reads: s = sx + 2sy + 4sz
-    const int32_t s0 = row_slot(b, t.bx, t.by, t.bz, 0), s1 = row_slot(b, t.bx, t.by, t.bz, 1);
-    int32_t s2 = -1, s3 = -1, s4 = -1, s5 = -1, s6 = -1, s7 = -1;
-    if (y == 3) {
-        s2 = row_slot(b, t.bx, t.by, t.bz, 2);
-        s3 = row_slot(b, t.bx, t.by, t.bz, 3);
-    }
-    if (z == 3) {
-        s4 = row_slot(b, t.bx, t.by, t.bz, 4);
-        s5 = row_slot(b, t.bx, t.by, t.bz, 5);
-    }
-    if (y == 3 && z == 3) {
-        s6 = row_slot(b, t.bx, t.by, t.bz, 6);
-        s7 = row_slot(b, t.bx, t.by, t.bz, 7);
-    }
-    const uint64_t ab = __ballot(act);
-    const uint32_t a = (uint32_t)(ab >> (lane & 48u)) & 0x7Fu;
-    uint32_t dx = t.bx + 1 < b.nb[0] ? 5 : 4, dy = t.by + 1 < b.nb[1] ? 5 : 4,
-             dz = t.bz + 1 < b.nb[2] ? 5 : 4;
-    if (dx == 5 && dy == 5 && dz == 5 && !(a & 64))
-        dx = dy = dz = 4;
-    if (dx == 5 && dy == 5 && !(a & 8))
-        dx = dy = 4;
-    if (dx == 5 && dz == 5 && !(a & 16))
-        dx = dz = 4;
-    if (dy == 5 && dz == 5 && !(a & 32))
-        dy = dz = 4;
-    if (dx == 5 && !(a & 1))
-        dx = 4;
-    if (dy == 5 && !(a & 2))
-        dy = 4;
-    if (dz == 5 && !(a & 4))
-        dz = 4;
-    t.dx = dx;
-    t.dy = dy;
-    t.dz = dz;
-    // voxel rows: own (x = 0..3 of this block, x = 4 from +x), then the ghosts
     row_fetch(b, s0, s1, 4 * y + 16 * z, t.r);
     row_fetch(b, s2, s3, 16 * z, t.gy);
     row_fetch(b, s4, s5, 4 * y, t.gz);
     row_fetch(b, s6, s7, 0, t.gyz);
 }
 
+// Corner tests of a row: bit 4x iff r[x] <= iso.
 BCMC_DEV uint32_t row_mask(const float (&r)[5], float iso) {
     uint32_t m = 0;
 #pragma unroll
     for (int x = 0; x < 5; ++x)
-        m |= (r[x] <= iso) ? (1u << x) : 0u;
+        m |= (r[x] <= iso) ? (1u << (4 * x)) : 0u;
     return m;
 }
 
-// Cases of the lane's 4 cells (bit i iff corner i <= iso,
-// compute_block_has_vertices.comp:28-37), packed 8 bits per cell, and the
-// per-cell vertex counts (0 for cells outside the tile and for rows without
-// a block), packed 8 bits per cell.  Must be called by all 64 lanes.
-BCMC_DEV void row_cases(const RowTile &t, float iso, const McTables &tab, uint32_t &cases,
-                        uint32_t &counts) {
+// Corner word of the lane's 4 cells (compute_block_has_vertices.comp:28-37):
+// cell x's corner byte is (w >> 4x) & 0xFF.  Cells outside the tile (and all
+// cells of rows without a block) get byte 0, whose cell has no vertices: rows
+// outside give w = 0, and x3_ok is false when the cell at x = 3 is outside.
+// Must be called by all 64 lanes.
+struct CornerWord {
+    uint32_t w;
+    bool x3_ok;
+};
+
+BCMC_DEV CornerWord row_corner_word(const RowTile &t, float iso) {
     const uint32_t l = lane_id() & 15u, y = l & 3u, z = l >> 2;
     const uint32_t m00 = row_mask(t.r, iso), my = row_mask(t.gy, iso), mz = row_mask(t.gz, iso),
                    myz = row_mask(t.gyz, iso);
@@ -196,36 +312,35 @@ BCMC_DEV void row_cases(const RowTile &t, float iso, const McTables &tab, uint32
     const uint32_t m10 = y < 3 ? n1 : my;
     const uint32_t m01 = z < 3 ? n4 : mz;
     const uint32_t m11 = y < 3 ? (z < 3 ? n5 : z1) : (z < 3 ? y4 : myz);
-    const uint32_t w = m00 | (m10 << 5) | (m01 << 10) | (m11 << 15);
-    const bool row_ok = t.live && y + 1 < t.dy && z + 1 < t.dz;
-    cases = 0;
-    counts = 0;
-#pragma unroll
-    for (uint32_t x = 0; x < 4; ++x) {
-        // corners 0..7 sit at bits x + {0, 1, 6, 5, 10, 11, 16, 15} of w
-        const uint32_t s = w >> x;
-        const uint32_t c = (s & 3u) | ((s >> 4) & 4u) | ((s >> 2) & 8u) | ((s >> 6) & 0x30u) |
-                           ((s >> 10) & 0x40u) | ((s >> 8) & 0x80u);
-        const uint32_t nv = (row_ok && x + 1 < t.dx) ? tab.count[c] : 0u;
-        cases |= c << (8 * x);
-        counts |= nv << (8 * x);
-    }
+    // cells need y + 1 < dy and z + 1 < dz; the cell at x = 3 needs dx = 5
+    const bool row_ok = t.live && (y < 3 || (t.dims & 2u)) && (z < 3 || (t.dims & 4u));
+    CornerWord cw;
+    cw.w = row_ok ? (m00 | (m10 << 1) | (m01 << 2) | (m11 << 3)) : 0u;
+    cw.x3_ok = (t.dims & 1u) != 0;
+    return cw;
 }
 
-BCMC_DEV uint32_t sum_bytes(uint32_t v) { return (v & 0xFF) + ((v >> 8) & 0xFF) + ((v >> 16) & 0xFF) + (v >> 24); }
+template <uint32_t X>
+BCMC_DEV uint32_t cell_byte(const CornerWord &cw) {
+    const uint32_t b = (cw.w >> (4 * X)) & 0xFFu;
+    return (X == 3 && !cw.x3_ok) ? 0u : b;
+}
 
 // Vertex count of the row's block, valid in lane 15 of the row.
 BCMC_DEV uint32_t row_block_count(const RowTile &t, float iso, const McTables &tab) {
-    uint32_t cases, counts;
-    row_cases(t, iso, tab, cases, counts);
-    return row_incl_scan(sum_bytes(counts));
+    const CornerWord cw = row_corner_word(t, iso);
+    const uint32_t n = (uint32_t)tab.count[cell_byte<0>(cw)] + tab.count[cell_byte<1>(cw)] +
+                       tab.count[cell_byte<2>(cw)] + tab.count[cell_byte<3>(cw)];
+    return row_incl_scan(n);
 }
 
 // Per-wave LDS scratch of the emission of 4 consecutive blocks.
-struct EmitScratch {
+constexpr uint32_t kMaxGroupTris = kRowBlocks * 64 * 5;
+struct alignas(16) EmitScratch {
     float tile[kRowBlocks][kTileFloats];
-    uint16_t first[kRowBlocks][64];  // first vertex of each cell, from the group start
-    uint8_t cell_case[kRowBlocks][64];
+    uint16_t first[kRowBlocks * 64];  // first vertex of each cell, from the group start
+    uint8_t corner[kRowBlocks * 64];  // corner byte of each cell
+    alignas(16) uint8_t start[kMaxGroupTris];  // at a cell's first triangle: the cell (0 elsewhere)
     uint32_t bid[kRowBlocks], bpos[kRowBlocks][3];
     float iso[kRowBlocks];
 };
@@ -235,29 +350,22 @@ BCMC_DEV void row_store(float *dst, const float (&r)[5]) {
     dst[4] = r[4];
 }
 
-// Corners a and b of edge e as 4-bit fields, and the tile offset of corner i.
-constexpr unsigned long long edge_corner_bits(int which) {
-    unsigned long long r = 0;
-    for (int e = 0; e < 12; ++e)
-        r |= (unsigned long long)((kEdgeVertsHost[e] >> (4 * which)) & 15u) << (4 * e);
-    return r;
-}
-constexpr unsigned long long kEdgeCornerA = edge_corner_bits(0), kEdgeCornerB = edge_corner_bits(1);
-BCMC_DEV uint32_t corner_offset(uint32_t i) { return corner_x(i) + kTileY * corner_y(i) + kTileZ * corner_z(i); }
-
 // Vertices of 4 blocks that are consecutive in the output (rows 0..3 of the
 // wave; rows with live = false have none), written from out0 on
 // (compute_block_vertices.comp:76-174).  Vertex order is the reference's:
 // blocks in row order, cells in x-fastest order, within a cell its
-// triangle-table row.  Vertex-parallel: lane v of each round finds its cell by
-// a binary search of the cells' first-vertex offsets and interpolates, so a
-// round writes one contiguous run.  Vertices at or past `cap` are not written.
-// Must be called by all 64 lanes.
+// triangle-table row.  Vertex-parallel: every cell with vertices marks its
+// first triangle in LDS with its index g = 64 row + cell (cells ascend with
+// their vertices), so the owner of vertex v is the running maximum of the
+// marks up to triangle v / 3; lane v of each round interpolates vertex v, and a round
+// writes one contiguous run.  Vertices at or past `cap` are not written.  Must
+// be called by all 64 lanes.
 BCMC_DEV void emit_row_blocks(const RowBlock &b, const RowTile &t, const McTables &tab, EmitScratch &es,
                               unsigned long long out0, unsigned long long cap, void *verts, int fmt) {
     const uint32_t lane = lane_id(), row = lane >> 4, l = lane & 15u, y = l & 3u, z = l >> 2;
-    uint32_t cases, counts;
-    row_cases(t, b.iso, tab, cases, counts);
+    const CornerWord cw = row_corner_word(t, b.iso);
+    const uint32_t cb[4] = {cell_byte<0>(cw), cell_byte<1>(cw), cell_byte<2>(cw), cell_byte<3>(cw)};
+    const uint32_t nv[4] = {tab.count[cb[0]], tab.count[cb[1]], tab.count[cb[2]], tab.count[cb[3]]};
     // the tile, for the interpolation
     float *tile = es.tile[row];
     row_store(tile + kTileY * y + kTileZ * z, t.r);
@@ -268,54 +376,64 @@ BCMC_DEV void emit_row_blocks(const RowBlock &b, const RowTile &t, const McTable
     if (y == 3 && z == 3)
         row_store(tile + kTileY * 4 + kTileZ * 4, t.gyz);
     // first vertex of each cell: group prefix of the block totals + row scan
-    const uint32_t lane_total = sum_bytes(counts);
+    const uint32_t lane_total = nv[0] + nv[1] + nv[2] + nv[3];
     const uint32_t incl = row_incl_scan(lane_total);
     const uint32_t t0 = wave_read_lane(incl, 15), t1 = wave_read_lane(incl, 31),
                    t2 = wave_read_lane(incl, 47), t3 = wave_read_lane(incl, 63);
-    const uint32_t s1 = t0, s2 = t0 + t1, s3 = t0 + t1 + t2, total = s3 + t3;
-    uint32_t first = (row == 0 ? 0u : row == 1 ? s1 : row == 2 ? s2 : s3) + incl - lane_total;
-#pragma unroll
-    for (uint32_t x = 0; x < 4; ++x) {
-        es.first[row][4 * l + x] = (uint16_t)first;
-        es.cell_case[row][4 * l + x] = (uint8_t)(cases >> (8 * x));
-        first += (counts >> (8 * x)) & 0xFFu;
-    }
+    const uint32_t total = t0 + t1 + t2 + t3;
+    const uint32_t base = row == 0 ? 0u : row == 1 ? t0 : row == 2 ? t0 + t1 : t0 + t1 + t2;
+    // vertices this call writes: those below the buffer capacity
+    const uint32_t limit = out0 >= cap ? 0u : (uint32_t)min((unsigned long long)total, cap - out0);
+    // clear the start marks of this group's triangles (vertex counts are
+    // multiples of 3)
+    for (uint32_t i = 16 * lane; i < total / 3; i += 1024)
+        *reinterpret_cast<uint4 *>(&es.start[i]) = make_uint4(0, 0, 0, 0);
     if (l == 0) {
+        uint32_t bx, by, bz;
+        block_coords(b, bx, by, bz);
         es.bid[row] = b.bid;
         es.iso[row] = b.iso;
-        es.bpos[row][0] = t.bx;
-        es.bpos[row][1] = t.by;
-        es.bpos[row][2] = t.bz;
+        es.bpos[row][0] = bx;
+        es.bpos[row][1] = by;
+        es.bpos[row][2] = bz;
     }
     wave_lds_sync();
-    for (uint32_t v = lane; v < total; v += 64) {
-        const uint32_t j = (v >= s1 ? 1u : 0u) + (v >= s2 ? 1u : 0u) + (v >= s3 ? 1u : 0u);
-        // last cell whose first vertex is <= v (cells after the owner start past v)
-        const uint16_t *fj = es.first[j];
-        uint32_t g = 0;
+    uint32_t first = base + incl - lane_total;
 #pragma unroll
-        for (uint32_t step = 32; step > 0; step >>= 1)
-            g = (fj[g + step] <= v) ? g + step : g;
-        const uint32_t k = v - fj[g];
-        const uint32_t e = (uint32_t)(tab.edges[es.cell_case[j][g]] >> (4 * k)) & 15u;
-        const uint32_t ca = (uint32_t)(kEdgeCornerA >> (4 * e)) & 15u,
-                       cb = (uint32_t)(kEdgeCornerB >> (4 * e)) & 15u;
-        const uint32_t cx = g & 3u, cy = (g >> 2) & 3u, cz = g >> 4;
-        const float *tj = es.tile[j] + cx + kTileY * cy + kTileZ * cz;
-        const float fa = tj[corner_offset(ca)];
-        const float fb = tj[corner_offset(cb)];
+    for (uint32_t x = 0; x < 4; ++x) {
+        const uint32_t g = 64 * row + 4 * l + x;
+        es.first[g] = (uint16_t)first;
+        es.corner[g] = (uint8_t)cb[x];
+        if (nv[x])
+            es.start[first / 3] = (uint8_t)g;
+        first += nv[x];
+    }
+    wave_lds_sync();
+    uint32_t carry = 0;
+    for (uint32_t v0 = 0; v0 < limit; v0 += 64) {
+        const uint32_t v = v0 + lane;
+        const uint32_t tri = (v * 0xAAABu) >> 17;  // v / 3 for v < 2^16
+        const uint32_t g = max(carry, wave_incl_max(v < limit ? (uint32_t)es.start[tri] : 0u));
+        carry = wave_read_lane(g, 63);
+        if (v >= limit)
+            break;
+        const uint32_t j = g >> 6, cell = g & 63u;
+        const uint32_t k = v - es.first[g];
+        const uint32_t e = (uint32_t)(tab.edges[es.corner[g]] >> (4 * k)) & 15u;
+        const EdgeInfo &ei = tab.edge[e];
+        const uint32_t cx = cell & 3u, cy = (cell >> 2) & 3u, cz = cell >> 4;
+        const float *tc = es.tile[j] + cx + kTileY * cy + kTileZ * cz;
+        const float fa = tc[__float_as_uint(ei.off_a)];
+        const float fb = tc[__float_as_uint(ei.off_b)];
         const float iso = es.iso[j];
         // lerp_verts (compute_block_vertices.comp:62-74), then + voxel + 0.5
         float tt = 0.0f;
         if (!(fabsf(fa - fb) < 0.001f))
             tt = (iso - fa) / (fb - fa);
-        const float ax = (float)corner_x(ca), ay = (float)corner_y(ca), az = (float)corner_z(ca);
-        const float px = (ax + tt * ((float)corner_x(cb) - ax) + (float)cx) + 0.5f;
-        const float py = (ay + tt * ((float)corner_y(cb) - ay) + (float)cy) + 0.5f;
-        const float pz = (az + tt * ((float)corner_z(cb) - az) + (float)cz) + 0.5f;
+        const float px = (ei.a[0] + tt * ei.d[0] + (float)cx) + 0.5f;
+        const float py = (ei.a[1] + tt * ei.d[1] + (float)cy) + 0.5f;
+        const float pz = (ei.a[2] + tt * ei.d[2] + (float)cz) + 0.5f;
         const unsigned long long o = out0 + v;
-        if (o >= cap)
-            break;
         const uint32_t bid = es.bid[j];
         if (fmt == BCMC_VERTS_LOCAL_F32) {
             reinterpret_cast<float4 *>(verts)[o] = make_float4(px, py, pz, __uint_as_float(bid));
@@ -334,6 +452,65 @@ BCMC_DEV void emit_row_blocks(const RowBlock &b, const RowTile &t, const McTable
         }
     }
     wave_lds_sync();  // the scratch is reused by the next group
+}
+
+// LDS of a workgroup's chunk: block ids (and a caller word, e.g. the query)
+// per entry, the per-block meta words, the per-block vertex counts, and the
+// list of entries with vertices with their vertex offsets in the chunk.
+struct ChunkShared {
+    uint32_t bid[kChunk];
+    uint32_t aux[kChunk];
+    uint32_t meta[kChunk][kMetaWords];
+    uint32_t cnt[kChunk], list[kChunk], off[kChunk];
+};
+
+// Count sweep of wave w over chunk entries [64w, 64w + 64), 4 per round; the
+// next round's slot/flag loads are issued before this round's voxels are
+// used.  setup(e, b) fills the RowBlock of entry e.
+template <class Setup>
+BCMC_DEV void count_sweep(ChunkShared &cs, uint32_t wave, const McTables &tab, Setup &&setup) {
+    constexpr uint32_t kRounds = 64 / kRowBlocks;
+    const uint32_t lane = lane_id(), row = lane >> 4;
+    RowBlock b;
+    setup(64 * wave + row, b);
+    RowMeta m = row_meta_load(b);
+    for (uint32_t q = 0; q < kRounds; ++q) {
+        const uint32_t e = 64 * wave + kRowBlocks * q + row;
+        const RowBlock bq = b;
+        row_meta_store(m, tab, cs.meta[e]);
+        wave_lds_sync();
+        RowTile t;
+        row_voxels(bq, cs.meta[e], t);
+        if (q + 1 < kRounds) {
+            setup(e + kRowBlocks, b);
+            m = row_meta_load(b);
+        }
+        const uint32_t c = row_block_count(t, bq.iso, tab);
+        if ((lane & 15u) == 15u)
+            cs.cnt[e] = c;
+    }
+}
+
+// Emit sweep: wave w takes groups of 4 consecutive entries of the list of
+// blocks with vertices (whose outputs are contiguous) and writes their
+// vertices from out0 + the group's offset in the chunk.
+template <class Setup>
+BCMC_DEV void emit_sweep(ChunkShared &cs, uint32_t wave, uint32_t num_with, unsigned long long out0,
+                         const McTables &tab, EmitScratch &es, unsigned long long cap, void *verts,
+                         int fmt, Setup &&setup) {
+    constexpr uint32_t kRound = (kMcThreads / 64) * kRowBlocks;
+    const uint32_t row = lane_id() >> 4;
+    for (uint32_t hh = wave * kRowBlocks; hh < num_with; hh += kRound) {
+        const uint32_t h = hh + row;
+        const bool live = h < num_with;
+        const uint32_t e = live ? cs.list[h] : 0u;
+        RowBlock b;
+        setup(e, b);
+        b.live = live;
+        RowTile t;
+        row_voxels(b, cs.meta[e], t);
+        emit_row_blocks(b, t, tab, es, out0 + cs.off[hh], cap, verts, fmt);
+    }
 }
 
 // ---------------------------------------------------------------------------
