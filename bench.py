@@ -51,6 +51,143 @@ def stage_bytes(s, rate, n_blocks):
     }
 
 
+SERVED_SIZES = [(128, 128, 128), (192, 192, 192), (256, 256, 256), (320, 320, 320),
+                (384, 384, 384), (512, 512, 512), (768, 768, 768), (1024, 1024, 1080)]
+SERVED_RATES = [2, 4, 8, 2, 4, 8, 2, 2]   # cycling 2/4/8; the 1024x1024x1080 volume at 2
+BATCH_SIZES = [(128, 128, 128), (256, 256, 256), (384, 384, 384), (512, 512, 512)]
+BATCH_RATES = [8, 4, 4, 2]
+
+
+def make_trace(rng, n, nvol, repeat=0.5, window=16):
+    """Volume ids uniform over the volumes; isovalues U[0.1, 0.9], of which a
+    fraction `repeat` re-use one of the last `window` isovalues of the same volume."""
+    vols = rng.integers(0, nvol, n)
+    hist = [[] for _ in range(nvol)]
+    isos = np.empty(n, np.float32)
+    for i, v in enumerate(vols):
+        h = hist[v]
+        if h and rng.random() < repeat:
+            iso = h[int(rng.integers(0, len(h)))]
+        else:
+            iso = np.float32(rng.uniform(0.1, 0.9))
+        isos[i] = iso
+        h.append(iso)
+        del h[:-window]
+    return vols.astype(np.uint32), isos
+
+
+def plain_replay(bcmc_amd, dev, vols, ids, isos, cache_frac=0.25):
+    """The plain per-query pipeline on a request sequence: one
+    CompressedMarchingCubes per volume (resident, its own LRU cache), queries
+    answered one at a time.  Returns per-query service times (s) and vertex counts."""
+    mcs = []
+    for stream, rate, dims in vols:
+        nb = int(np.prod([(d + 3) // 4 for d in dims]))
+        mc = bcmc_amd.CompressedMarchingCubes(dev)
+        mc.setCompressedVolume(stream, rate, dims, (1, 1, 1), max(32, int(cache_frac * nb)))
+        mcs.append(mc)
+    # warm: one query per volume (allocations, first growth)
+    for mc in mcs:
+        mc.computeSurface(0.5)
+    dev.sync()
+    t_q, nv = np.empty(len(ids)), np.empty(len(ids), np.uint64)
+    for k, (v, iso) in enumerate(zip(ids, isos)):
+        t = time.perf_counter()
+        nv[k] = mcs[v].computeSurface(float(iso))
+        t_q[k] = time.perf_counter() - t
+    del mcs
+    return t_q, nv
+
+
+def served_bench(bcmc_amd, synth, dev, args):
+    t0 = time.time()
+    vols = []
+    for i, (dims, rate) in enumerate(zip(SERVED_SIZES, SERVED_RATES)):
+        s, sha = synth.make_stream(synth.blob_workload(dims, 7 + i, rate))
+        vols.append((s, rate, dims))
+    log(f"bench: served volumes generated in {time.time() - t0:.1f}s")
+    rng = np.random.default_rng(20240607)
+    ids, isos = make_trace(rng, args.served_queries, len(vols))
+    t_plain, nv_plain = plain_replay(bcmc_amd, dev, vols, ids, isos)
+    plain_qps = len(ids) / t_plain.sum()
+    log(f"bench: plain replay {len(ids)} queries: {plain_qps:.1f} q/s")
+    decoded = sum(int(np.prod([(d + 3) // 4 for d in v[2]])) for v in vols) * 256
+    eng = bcmc_amd.Engine(dev, min(args.served_pool_gib << 30, 2 * decoded))
+    for s, rate, dims in vols:
+        eng.add_volume(s, rate, dims)
+    budget = 200 << 30
+    eng.serve(np.zeros(len(vols)), np.arange(len(vols), dtype=np.uint32),
+              np.full(len(vols), 0.5, np.float32), vertex_budget_bytes=budget)  # warm
+    dev.sync()
+    sat = eng.serve(np.zeros(len(ids)), ids, isos, max_batch=args.served_batch,
+                    vertex_budget_bytes=budget)
+    makespan = float(sat["finish_s"].max())
+    served_qps = len(ids) / makespan
+    exact = bool(np.array_equal(sat["num_vertices"].astype(np.uint64), nv_plain))
+    lat = sat["finish_s"] - sat["arrival_s"]
+    log(f"bench: served saturation {served_qps:.1f} q/s in {int(sat['step'].max()) + 1} steps; "
+        f"counts equal to plain: {exact}")
+    out = {"workload": "8 resident blob volumes 128^3..1024x1024x1080 (seeds 7-14, zfp rates "
+                       "2/4/8/2/4/8/2/2), isovalues U[0.1,0.9] with 50% repeats of the last 16 of "
+                       "the same volume",
+           "queries": len(ids), "plain_qps": round(plain_qps, 2),
+           "plain_latency_ms": {"p50": round(float(np.percentile(t_plain, 50)) * 1e3, 3),
+                                "p99": round(float(np.percentile(t_plain, 99)) * 1e3, 3)},
+           "served_qps": round(served_qps, 2), "served_vs_plain": round(served_qps / plain_qps, 3),
+           "saturation_latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                                     "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
+           "counts_equal_plain": exact, "steps": int(sat["step"].max()) + 1,
+           "mean_batch": round(float(sat["batch_size"].mean()), 1),
+           "hbm_budget_bytes": budget, "pool_bytes": eng.stats()["pool_bytes"], "poisson": []}
+    for rate in args.served_rates:
+        n = min(len(ids), max(64, int(rate * args.served_seconds)))
+        arr = np.cumsum(rng.exponential(1.0 / rate, n))
+        r = eng.serve(arr, ids[:n], isos[:n], max_batch=args.served_batch, vertex_budget_bytes=budget)
+        lat = r["finish_s"] - r["arrival_s"]
+        span = float(r["finish_s"].max() - arr[0])
+        out["poisson"].append({"rate_rps": rate, "queries": n, "qps": round(n / span, 2),
+                               "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                               "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                               "mean_batch": round(float(r["batch_size"].mean()), 2)})
+        log(f"bench: served poisson {rate} req/s: p50 {out['poisson'][-1]['p50_ms']} ms "
+            f"p99 {out['poisson'][-1]['p99_ms']} ms")
+    del eng
+    return out
+
+
+def batched_bench(bcmc_amd, synth, dev, args):
+    vols = []
+    for i, (dims, rate) in enumerate(zip(BATCH_SIZES, BATCH_RATES)):
+        s, _ = synth.make_stream(synth.blob_workload(dims, 3 + i, rate))
+        vols.append((s, rate, dims))
+    rng = np.random.default_rng(3)
+    ids = rng.integers(0, len(vols), 64).astype(np.uint32)
+    isos = rng.uniform(0.1, 0.9, 64).astype(np.float32)
+    t_plain, nv_plain = plain_replay(bcmc_amd, dev, vols, ids, isos)
+    decoded = sum(int(np.prod([(d + 3) // 4 for d in v[2]])) for v in vols) * 256
+    eng = bcmc_amd.Engine(dev, 2 * decoded)
+    for s, rate, dims in vols:
+        eng.add_volume(s, rate, dims)
+    eng.run(ids, isos)  # warm (pool fill, buffer sizes)
+    dev.sync()
+    reps = []
+    for _ in range(args.batched_reps):
+        t = time.perf_counter()
+        res = eng.run(ids, isos)
+        reps.append(time.perf_counter() - t)
+    nv = np.array([r["num_vertices"] for r in res], np.uint64)
+    step = float(np.median(reps))
+    out = {"workload": "64 queries over blob volumes 128^3 r8, 256^3 r4, 384^3 r4, 512^3 r2 "
+                       "(seeds 3-6), isovalues U[0.1,0.9], one engine step",
+           "step_ms": round(step * 1e3, 3), "qps": round(64 / step, 2),
+           "plain_qps": round(64 / t_plain.sum(), 2),
+           "batched_vs_plain": round(t_plain.sum() / step, 3),
+           "counts_equal_plain": bool(np.array_equal(nv, nv_plain))}
+    log(f"bench: batched step {out['step_ms']} ms ({out['qps']} q/s; plain {out['plain_qps']} q/s)")
+    del eng
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +199,14 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy kernel of this size (HBM counter calibration)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--served-queries", type=int, default=512)
+    ap.add_argument("--served-batch", type=int, default=64)
+    ap.add_argument("--served-pool-gib", type=int, default=32)
+    ap.add_argument("--served-rates", type=float, nargs="*", default=[100, 300, 1000, 2000])
+    ap.add_argument("--served-seconds", type=float, default=1.0)
+    ap.add_argument("--batched-reps", type=int, default=5)
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -176,6 +321,11 @@ def main():
                               "numVertices")},
         "cpu_baseline": None,
     }
+
+    if not args.no_batched:
+        out["batched"] = batched_bench(bcmc_amd, synth, dev, args)
+    if not args.no_served:
+        out["served"] = served_bench(bcmc_amd, synth, dev, args)
 
     if args.calibrate_copy_mib > 0:
         nb = args.calibrate_copy_mib << 20

@@ -79,10 +79,6 @@ BCMC_DEV uint32_t query_of(const EngQuery *qs, uint32_t nq, uint32_t g) {
     return lo;
 }
 
-__global__ void eng_zero_u64_kernel(unsigned long long *p, uint32_t n) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        p[i] = 0;
-}
 
 __global__ void eng_begin_kernel(EngCounters *c) {
     c->epoch += 1;
@@ -144,7 +140,9 @@ __global__ void eng_mark_kernel(const EngQuery *qs, const EngVol *vols, const ui
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
         const uint32_t vi = qs[act_q[e]].vol, b = act_b[e];
         int32_t *sm = vols[vi].slot_map;
-        const int32_t s = __hip_atomic_load(&sm[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // plain load: slots of pooled blocks do not change during the kernel;
+        // the compare-and-swap below settles races between claimers
+        const int32_t s = sm[b];
         if (s >= 0) {
             stamp[s] = ep;
         } else if (s == -1) {
@@ -233,7 +231,7 @@ struct EngFusedArgs {
     const float *pool;
     uint32_t capacity;
     EngCounters *c;
-    unsigned long long *q_verts;  // per-query vertex counts
+    unsigned long long *q_vfirst;  // per query: vertex offset of its first active pair
     unsigned long long *vert_offsets;
     void *verts;
     unsigned long long vert_cap;
@@ -259,6 +257,8 @@ BCMC_DEV void eng_row_block(const EngFusedArgs &a, uint32_t qi, uint32_t bid, bo
 }
 
 constexpr int kEngThreads = kMcThreads;
+// Blocks of all queries of one step (u32 indices into the concatenated arrays).
+constexpr uint64_t kMaxStepBlocks = 1ull << 31;
 
 // The chunked count / look-back / emit pass of mc.hip over (query, block)
 // pairs of many queries; the vertex channel's offsets are global, and each
@@ -289,8 +289,6 @@ __global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, Eng
         count_sweep(cs, wave, tab, setup);
         __syncthreads();
         const uint32_t c = cs.cnt[threadIdx.x];
-        if (c)
-            atomicAdd(&a.q_verts[cs.aux[threadIdx.x]], (unsigned long long)c);
         uint32_t agg_h, agg_v;
         const uint32_t ex_h = block_excl_scan<kEngThreads>(c ? 1u : 0u, lds_w, &agg_h);
         const uint32_t ex_v = block_excl_scan<kEngThreads>(c, lds_w, &agg_v);
@@ -298,6 +296,14 @@ __global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, Eng
         if (c) {
             cs.list[ex_h] = threadIdx.x;
             cs.off[ex_h] = ex_v;
+        }
+        // the first pair of each query records where the query's vertices start
+        const uint32_t k = k0 + threadIdx.x;
+        if (k < n) {
+            const uint32_t qi = cs.aux[threadIdx.x];
+            const uint32_t prev = threadIdx.x ? cs.aux[threadIdx.x - 1] : (k ? a.act_q[k - 1] : ~0u);
+            if (qi != prev)
+                a.q_vfirst[qi] = pv + ex_v;
         }
         if (tile == ntiles - 1 && threadIdx.x == 0)
             a.c->num_verts = pv + agg_v;
@@ -347,7 +353,7 @@ struct bcmc_engine {
     uint32_t q_cap = 0;
     EngQuery *d_q = nullptr;
     uint32_t *q_act_start = nullptr;
-    unsigned long long *q_verts = nullptr;
+    unsigned long long *q_vfirst = nullptr;
     ScanSite site_active, site_cand, site_mc;
     void *verts = nullptr;
     uint64_t vert_cap = 0;
@@ -387,11 +393,11 @@ static int eng_ensure_queries(bcmc_engine *e, uint32_t nq) {
     BCMC_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
     dev_free(e->d_q);
     dev_free(e->q_act_start);
-    dev_free(e->q_verts);
+    dev_free(e->q_vfirst);
     const uint32_t cap = std::max<uint32_t>(nq, 64);
     BCMC_TRY(dev_alloc((void **)&e->d_q, (size_t)cap * sizeof(EngQuery)));
     BCMC_TRY(dev_alloc((void **)&e->q_act_start, (size_t)cap * 4));
-    BCMC_TRY(dev_alloc((void **)&e->q_verts, (size_t)cap * 8));
+    BCMC_TRY(dev_alloc((void **)&e->q_vfirst, (size_t)cap * 8));
     e->q_cap = cap;
     return BCMC_OK;
 }
@@ -410,12 +416,11 @@ static int eng_launch_fused(bcmc_engine *e, uint32_t nq, uint64_t total) {
     a.pool = e->pool;
     a.capacity = e->pool_cap;
     a.c = e->c;
-    a.q_verts = e->q_verts;
+    a.q_vfirst = e->q_vfirst;
     a.vert_offsets = e->vert_offsets;
     a.verts = e->verts;
     a.vert_cap = e->vert_cap;
     a.fmt = e->fmt;
-    eng_zero_u64_kernel<<<1, 256, 0, e->ctx->stream>>>(e->q_verts, nq);
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(total, kChunk),
                                                                    persistent_grid(e->ctx, 4)));
     eng_fused_kernel<<<grid, kEngThreads, 0, e->ctx->stream>>>(e->site_mc, a);
@@ -461,6 +466,7 @@ BCMC_API int bcmc_engine_create(bcmc_ctx *ctx, uint64_t pool_bytes, bcmc_engine 
         bcmc_engine_destroy(e);
         return rc;
     }
+    *e->ch = EngCounters{};
     BCMC_HIP_CHECK(hipMemsetAsync(e->c, 0, sizeof(EngCounters), ctx->stream));
     BCMC_HIP_CHECK(hipMemsetAsync(e->stamp, 0, (size_t)e->pool_cap * 4, ctx->stream));
     BCMC_HIP_CHECK(hipMemsetAsync(e->owner, 0xFF, (size_t)e->pool_cap * 8, ctx->stream));
@@ -496,7 +502,7 @@ BCMC_API int bcmc_engine_destroy(bcmc_engine *e) {
     dev_free(e->vert_offsets);
     dev_free(e->d_q);
     dev_free(e->q_act_start);
-    dev_free(e->q_verts);
+    dev_free(e->q_vfirst);
     dev_free(e->verts);
     scan_site_free(&e->site_active);
     scan_site_free(&e->site_cand);
@@ -591,7 +597,7 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
         used[volume_ids[i]] = true;
         total += v.n;
     }
-    BCMC_CHECK_ARG(total < (1ull << 32) - kScanTile, "too many blocks in one step");
+    BCMC_CHECK_ARG(total <= kMaxStepBlocks, "too many blocks in one step");
     BCMC_TRY(eng_ensure_blocks(e, total));
     BCMC_TRY(eng_ensure_queries(e, nq));
     BCMC_HIP_CHECK(hipMemcpyAsync(e->d_q, qs.data(), nq * sizeof(EngQuery), hipMemcpyHostToDevice, st));
@@ -603,12 +609,21 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     launch_scan<1>(ctx, e->site_active, e->site_active, aop, total);
     eng_mark_kernel<<<eng_grid(ctx, total), 256, 0, st>>>(e->d_q, e->d_vols, e->act_q, e->act_b, e->c,
                                                           e->stamp, e->miss, e->miss_cap);
-    // eviction order (only does work when the misses exceed the free room)
-    launch_scan<1>(ctx, e->site_cand, e->site_cand, CandOp{e->stamp, e->cand, e->c, e->pool_cap},
-                   e->pool_cap);
-    eng_cand_keys_kernel<<<eng_grid(ctx, e->pool_cap, 2), 256, 0, st>>>(e->stamp, e->cand, e->c,
-                                                                         e->cand_keys);
-    launch_sort_pairs(ctx, &e->sort, e->cand_keys, e->cand, &e->c->num_cand, e->pool_cap, e->order);
+    // eviction order (only does work when the misses exceed the free room).
+    // A step misses at most every block of each volume it touches; when the
+    // free room after the last step covers that, eviction cannot happen and
+    // the chain is not launched (assign then never reads the order).
+    uint64_t most_misses = 0;
+    for (uint32_t vi = 0; vi < e->vols.size(); ++vi)
+        most_misses += used[vi] ? e->vols[vi].n : 0u;
+    if ((uint64_t)e->pool_cap - e->ch->used < most_misses) {
+        launch_scan<1>(ctx, e->site_cand, e->site_cand, CandOp{e->stamp, e->cand, e->c, e->pool_cap},
+                       e->pool_cap);
+        eng_cand_keys_kernel<<<eng_grid(ctx, e->pool_cap, 2), 256, 0, st>>>(e->stamp, e->cand, e->c,
+                                                                             e->cand_keys);
+        launch_sort_pairs(ctx, &e->sort, e->cand_keys, e->cand, &e->c->num_cand, e->pool_cap,
+                          e->order);
+    }
     eng_assign_kernel<<<eng_grid(ctx, total, 4), 256, 0, st>>>(e->d_vols, e->c, e->miss, e->order,
                                                                 e->pool_cap, e->stamp, e->owner);
     eng_commit_kernel<<<1, 1, 0, st>>>(e->c, e->pool_cap);
@@ -624,7 +639,7 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     BCMC_HIP_CHECK(hipMemcpyAsync(e->ch, e->c, sizeof(EngCounters), hipMemcpyDeviceToHost, st));
     std::vector<unsigned long long> qv(nq);
     std::vector<uint32_t> qa(nq);
-    BCMC_HIP_CHECK(hipMemcpyAsync(qv.data(), e->q_verts, nq * 8, hipMemcpyDeviceToHost, st));
+    BCMC_HIP_CHECK(hipMemcpyAsync(qv.data(), e->q_vfirst, nq * 8, hipMemcpyDeviceToHost, st));
     BCMC_HIP_CHECK(hipMemcpyAsync(qa.data(), e->q_act_start, nq * 4, hipMemcpyDeviceToHost, st));
     BCMC_HIP_CHECK(hipStreamSynchronize(st));
     if (e->ch->short_slots) {
@@ -641,19 +656,20 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
         BCMC_TRY(dev_alloc(&e->verts, want * vbytes(e->fmt)));
         e->vert_cap = want;
         BCMC_TRY(eng_launch_fused(e, nq, total));
-        BCMC_HIP_CHECK(hipMemcpyAsync(qv.data(), e->q_verts, nq * 8, hipMemcpyDeviceToHost, st));
         BCMC_HIP_CHECK(hipStreamSynchronize(st));
     }
+    // a query's vertices run from its first active pair's offset to the next
+    // non-empty query's (or the step total)
     e->last.resize(nq);
-    uint64_t base = 0;
     const uint32_t na = e->ch->num_active;
-    for (uint32_t i = 0; i < nq; ++i) {
+    uint64_t next = nv;
+    for (uint32_t i = nq; i-- > 0;) {
         bcmc_query_result r;
-        r.num_vertices = qv[i];
-        r.vertex_offset = base;
         r.num_active_blocks = (i + 1 < nq ? qa[i + 1] : na) - qa[i];
+        r.vertex_offset = r.num_active_blocks ? qv[i] : next;
+        r.num_vertices = next - r.vertex_offset;
         r.step_decoded_blocks = e->ch->num_miss;
-        base += qv[i];
+        next = r.vertex_offset;
         e->last[i] = r;
         if (results)
             results[i] = r;
@@ -727,17 +743,20 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
                 std::this_thread::sleep_for(std::chrono::duration<double>(wait));
             continue;
         }
-        // admission: FIFO up to max_batch and the vertex budget (estimate)
+        // admission: FIFO up to max_batch, the vertex budget (estimate) and the
+        // per-step block limit of the concatenated block arrays
         ids.clear();
         isos.clear();
         std::vector<uint32_t> batch;
-        uint64_t bytes = 0;
+        uint64_t bytes = 0, blocks = 0;
         while (head < pending.size() && batch.size() < max_batch) {
             const uint32_t r = pending[head];
-            const uint64_t need = est[volume_ids[r]] * vb;
-            if (!batch.empty() && bytes + need > vert_budget)
+            BCMC_CHECK_ARG(volume_ids[r] < e->vols.size(), "unknown volume id");
+            const uint64_t need = est[volume_ids[r]] * vb, nblk = e->vols[volume_ids[r]].n;
+            if (!batch.empty() && (bytes + need > vert_budget || blocks + nblk > kMaxStepBlocks))
                 break;
             bytes += need;
+            blocks += nblk;
             batch.push_back(r);
             ids.push_back(volume_ids[r]);
             isos.push_back(isovalues[r]);

@@ -17,15 +17,16 @@ struct QState {
     float iso;
     uint32_t num_active;       // A
     uint32_t num_new;          // D: LRU misses
-    uint32_t num_avail;        // S: available slots (after growth)
+    uint32_t num_avail;        // S: available slots before growth
     uint32_t cache_size;       // C: logical LRU size before this update
     uint32_t cache_size_next;  // C after this update (growth)
     uint32_t num_with;         // H: active blocks with vertices
     uint32_t overflow;         // emit skipped: V > vertex capacity
     unsigned long long num_verts;     // V
     unsigned long long vert_capacity; // vertices the output buffer holds
-    uint32_t avail_display;    // S - D (displayNumSlotsAvailable)
-    uint32_t pad[5];
+    uint32_t avail_display;    // S' - D (displayNumSlotsAvailable)
+    uint32_t num_avail_total;  // S': available slots after growth
+    uint32_t pad[4];
 };
 
 // ---------------------------------------------------------------------------
