@@ -38,8 +38,9 @@ namespace bcmc {
 struct EngVol {                 // device descriptor of a volume
     uint32_t nb[3];
     uint32_t n;
+    const float2 *dilated;      // per block: range over its 3x3x3 neighbourhood (block_active)
+    const float2 *seg_range;    // per row segment: union of its blocks' dilated ranges
     FastDiv div_x, div_y;       // by nb[0], nb[1]
-    const float2 *ranges;
     int32_t *slot_map;          // block -> pool slot; -1 none, -2 claimed this step
     uint32_t *miss_list;        // this step's missing blocks of the volume (decoder input)
 };
@@ -49,6 +50,8 @@ struct EngQuery {
     float iso;
     uint32_t flag_base;         // first index of this query in the concatenated block arrays
     uint32_t n;
+    uint32_t row_base;          // first index of this query in the concatenated row segments
+    uint32_t pad[3];
 };
 
 struct EngCounters {
@@ -59,19 +62,19 @@ struct EngCounters {
     uint32_t used;              // pool slots ever handed out (high-water mark)
     uint32_t num_cand;          // eviction candidates
     uint32_t short_slots;       // misses that found no slot (admission failed)
-    uint32_t pad;
+    uint32_t num_rows;          // candidate row segments of all queries
     unsigned long long num_verts;
     uint32_t vol_miss[64];      // misses per volume
 };
 
 constexpr uint32_t kMaxVols = 64;
 
-// query of a concatenated index (flag_base ascending)
-BCMC_DEV uint32_t query_of(const EngQuery *qs, uint32_t nq, uint32_t g) {
+// query of a concatenated block row (row_base ascending)
+BCMC_DEV uint32_t query_of_row(const EngQuery *qs, uint32_t nq, uint32_t r) {
     uint32_t lo = 0, hi = nq;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) / 2;
-        if (qs[mid].flag_base <= g)
+        if (qs[mid].row_base <= r)
             lo = mid;
         else
             hi = mid;
@@ -79,83 +82,174 @@ BCMC_DEV uint32_t query_of(const EngQuery *qs, uint32_t nq, uint32_t g) {
     return lo;
 }
 
+constexpr uint32_t kNoPairs = 0xFFFFFFFFu;
 
-__global__ void eng_begin_kernel(EngCounters *c) {
-    c->epoch += 1;
-    c->num_active = 0;
-    c->num_with = 0;
-    c->num_miss = 0;
-    c->num_cand = 0;
-    c->short_slots = 0;
-    c->num_verts = 0;
-    for (uint32_t v = 0; v < kMaxVols; ++v)
+__global__ void eng_begin_kernel(EngCounters *c, uint32_t *q_act_start, uint32_t nq) {
+    if (threadIdx.x == 0) {
+        c->epoch += 1;
+        c->num_active = 0;
+        c->num_with = 0;
+        c->num_miss = 0;
+        c->num_cand = 0;
+        c->short_slots = 0;
+        c->num_rows = 0;
+        c->num_verts = 0;
+    }
+    for (uint32_t v = threadIdx.x; v < kMaxVols; v += blockDim.x)
         c->vol_miss[v] = 0;
+    for (uint32_t q = threadIdx.x; q < nq; q += blockDim.x)
+        q_act_start[q] = kNoPairs;
 }
 
-__global__ void eng_code_kernel(const EngQuery *qs, uint32_t nq, const EngVol *vols, uint32_t total,
-                                uint8_t *code) {
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
-        const EngQuery q = qs[query_of(qs, nq, g)];
-        const EngVol &v = vols[q.vol];
-        const uint32_t b = g - q.flag_base;
-        code[g] = (uint8_t)active_code(v.ranges, b, b % v.nb[0], v.nb[0], q.iso);
+// Block-row segments of kSegBlocks blocks along x.  Per volume, once at load:
+// the union of the dilated block ranges (block_active) of each segment.  A
+// block is active only if the isovalue lies in its dilated range, hence in its
+// segment's.
+constexpr uint32_t kSegBlocks = 16;
+
+__global__ void eng_seg_range_kernel(const float2 *dilated, uint32_t nbx, uint32_t nsegx, uint32_t nsegs,
+                                     float2 *out) {
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nsegs; s += gridDim.x * blockDim.x) {
+        const uint32_t sx = s % nsegx, row = s / nsegx;
+        const uint32_t x0 = sx * kSegBlocks, x1 = min(nbx, x0 + kSegBlocks);
+        float lo = 1e30f, hi = -1e30f;
+        const float2 *r = dilated + (uint64_t)row * nbx;
+        for (uint32_t x = x0; x < x1; ++x) {
+            const float2 v = r[x];
+            lo = fminf(lo, v.x);
+            hi = fmaxf(hi, v.y);
+        }
+        out[s] = make_float2(lo, hi);
     }
 }
 
-struct EngActiveOp {
+// Candidate segments of every query: iso within the segment's dilated range.
+struct CandSegOp {
     const EngQuery *qs;
     uint32_t nq;
     const EngVol *vols;
-    const uint8_t *code;
+    uint32_t *cand_q, *cand_seg;
+    EngCounters *c;
+    uint32_t nsegs;
+    BCMC_DEV uint32_t size() const { return nsegs; }
+    BCMC_DEV void begin() const {}
+    BCMC_DEV void value(uint32_t r, uint32_t (&v)[1]) const {
+        const uint32_t qi = query_of_row(qs, nq, r);
+        const EngQuery q = qs[qi];
+        const float2 rr = vols[q.vol].seg_range[r - q.row_base];
+        v[0] = (q.iso >= rr.x && q.iso <= rr.y) ? 1u : 0u;
+    }
+    BCMC_DEV void emit(uint32_t r, const uint32_t (&v)[1], const unsigned long long (&e)[1]) const {
+        if (v[0]) {
+            const uint32_t qi = query_of_row(qs, nq, r);
+            cand_q[e[0]] = qi;
+            cand_seg[e[0]] = r - qs[qi].row_base;
+        }
+    }
+    BCMC_DEV void total(const unsigned long long (&t)[1]) const { c->num_rows = (uint32_t)t[0]; }
+};
+
+// Active test (compute_block_active.comp:30-60, as block_active) over the
+// blocks of the candidate segments (kSegBlocks elements each).  Output: the active
+// (query, block) pairs in query order, ascending blocks; flag words stamped
+// with the step epoch; the first pair of each query.
+struct SegActiveOp {
+    const EngQuery *qs;
+    const EngVol *vols;
+    const uint32_t *cand_q, *cand_seg;
     uint32_t *flags, *act_q, *act_b, *q_act_start;
     EngCounters *c;
-    uint32_t nblocks;
-    BCMC_DEV uint32_t size() const { return nblocks; }
+    BCMC_DEV uint32_t size() const { return c->num_rows * kSegBlocks; }
     BCMC_DEV void begin() const {}
-    BCMC_DEV void value(uint32_t g, uint32_t (&v)[1]) const {
-        const uint32_t qi = query_of(qs, nq, g);
-        const EngQuery q = qs[qi];
-        const EngVol &vol = vols[q.vol];
-        v[0] = active_from_codes(code + q.flag_base, g - q.flag_base, vol.nb);
+    // block of element i (>= nb[0] in x: padding)
+    BCMC_DEV bool block_of(uint32_t i, const EngVol &vol, uint32_t &x, uint32_t &row) const {
+        const uint32_t k = i / kSegBlocks, seg = cand_seg[k];
+        const uint32_t nsegx = div_up(vol.nb[0], kSegBlocks);
+        x = (seg % nsegx) * kSegBlocks + i % kSegBlocks;
+        row = seg / nsegx;
+        return x < vol.nb[0];
     }
-    BCMC_DEV void emit(uint32_t g, const uint32_t (&v)[1], const unsigned long long (&e)[1]) const {
-        const uint32_t qi = query_of(qs, nq, g);
-        const uint32_t b = g - qs[qi].flag_base;
-        flags[g] = v[0];
-        if (b == 0)
+    BCMC_DEV void value(uint32_t i, uint32_t (&v)[1]) const {
+        const EngQuery q = qs[cand_q[i / kSegBlocks]];
+        const EngVol &vol = vols[q.vol];
+        uint32_t x, row;
+        v[0] = (block_of(i, vol, x, row) && block_active(vol.dilated[x + vol.nb[0] * row], q.iso)) ? 1u : 0u;
+    }
+    BCMC_DEV void emit(uint32_t i, const uint32_t (&v)[1], const unsigned long long (&e)[1]) const {
+        const uint32_t k = i / kSegBlocks;
+        const uint32_t qi = cand_q[k];
+        if (i % kSegBlocks == 0 && (k == 0 || cand_q[k - 1] != qi))
             q_act_start[qi] = (uint32_t)e[0];
         if (v[0]) {
+            const EngQuery q = qs[qi];
+            const EngVol &vol = vols[q.vol];
+            uint32_t x, row;
+            block_of(i, vol, x, row);
+            const uint32_t b = x + vol.nb[0] * row;
             act_q[e[0]] = qi;
             act_b[e[0]] = b;
+            flags[q.flag_base + b] = c->epoch;
         }
     }
     BCMC_DEV void total(const unsigned long long (&t)[1]) const { c->num_active = (uint32_t)t[0]; }
 };
 
-// stamp pooled blocks, claim missing ones
+// stamp pooled blocks, claim missing ones (once per block: the claim is a
+// compare-and-swap -1 -> -2).  Whole waves walk the pairs together so that
+// the claims of a wave reserve their list entries with one atomic per list.
 __global__ void eng_mark_kernel(const EngQuery *qs, const EngVol *vols, const uint32_t *act_q,
                                 const uint32_t *act_b, EngCounters *c, uint32_t *stamp,
                                 unsigned long long *miss, uint32_t miss_cap) {
     const uint32_t n = c->num_active, ep = c->epoch;
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        const uint32_t vi = qs[act_q[e]].vol, b = act_b[e];
-        int32_t *sm = vols[vi].slot_map;
-        // plain load: slots of pooled blocks do not change during the kernel;
-        // the compare-and-swap below settles races between claimers
-        const int32_t s = sm[b];
-        if (s >= 0) {
-            stamp[s] = ep;
-        } else if (s == -1) {
-            int32_t expect = -1;
-            if (__hip_atomic_compare_exchange_strong(&sm[b], &expect, -2, __ATOMIC_RELAXED,
-                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                const uint32_t w = atomic_add_u32(&c->num_miss, 1u);
-                const uint32_t wv = atomic_add_u32(&c->vol_miss[vi], 1u);
-                if (w < miss_cap)
-                    miss[w] = ((unsigned long long)vi << 32) | b;
-                if (wv < vols[vi].n)
-                    vols[vi].miss_list[wv] = b;
+    const uint32_t lane = lane_id();
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint32_t nw = (gridDim.x * blockDim.x) / 64;
+    for (uint32_t base = wid * 64; base < n; base += nw * 64) {
+        const uint32_t e = base + lane;
+        bool claimed = false;
+        uint32_t vi = 0, b = 0;
+        if (e < n) {
+            vi = qs[act_q[e]].vol;
+            b = act_b[e];
+            int32_t *sm = vols[vi].slot_map;
+            // plain load: slots of pooled blocks do not change during the
+            // kernel; the compare-and-swap settles races between claimers
+            const int32_t s = sm[b];
+            if (s >= 0) {
+                stamp[s] = ep;
+            } else if (s == -1) {
+                int32_t expect = -1;
+                claimed = __hip_atomic_compare_exchange_strong(&sm[b], &expect, -2, __ATOMIC_RELAXED,
+                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
+        }
+        const uint64_t m = __ballot(claimed);
+        if (m == 0)
+            continue;
+        uint32_t w0 = 0;
+        if (lane == 0)
+            w0 = atomic_add_u32(&c->num_miss, (uint32_t)__popcll(m));
+        w0 = wave_read_lane(w0, 0);
+        const uint32_t w = w0 + (uint32_t)__popcll(m & lt);
+        if (claimed && w < miss_cap)
+            miss[w] = ((unsigned long long)vi << 32) | b;
+        // per-volume decode lists: one reservation per volume present in the wave
+        uint64_t rem = m;
+        while (rem) {
+            const uint32_t v0 = wave_read_lane(vi, (int)__builtin_ctzll(rem));
+            const bool mine = claimed && vi == v0;
+            const uint64_t same = __ballot(mine);
+            uint32_t p0 = 0;
+            if (lane == 0)
+                p0 = atomic_add_u32(&c->vol_miss[v0], (uint32_t)__popcll(same));
+            p0 = wave_read_lane(p0, 0);
+            if (mine) {
+                const uint32_t wv = p0 + (uint32_t)__popcll(same & lt);
+                if (wv < vols[v0].n)
+                    vols[v0].miss_list[wv] = b;
+            }
+            rem &= ~same;
         }
     }
 }
@@ -239,7 +333,8 @@ struct EngFusedArgs {
 };
 
 // The RowBlock of a (query, block) pair: its query's volume, flags, isovalue.
-BCMC_DEV void eng_row_block(const EngFusedArgs &a, uint32_t qi, uint32_t bid, bool live, RowBlock &b) {
+BCMC_DEV void eng_row_block(const EngFusedArgs &a, uint32_t qi, uint32_t bid, bool live, uint32_t epoch,
+                            RowBlock &b) {
     const EngQuery &q = a.qs[qi];
     const EngVol &v = a.vols[q.vol];
     b.cache = a.pool;
@@ -251,6 +346,7 @@ BCMC_DEV void eng_row_block(const EngFusedArgs &a, uint32_t qi, uint32_t bid, bo
     b.div_x = v.div_x;
     b.div_y = v.div_y;
     b.cap = a.capacity;
+    b.tag = epoch;  // flag words hold the epoch of the step that found the block active
     b.iso = q.iso;
     b.live = live;
     b.bid = bid;
@@ -272,7 +368,7 @@ __global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, Eng
     __shared__ McTables tab;
     __shared__ EmitScratch es[NW];
     load_mc_tables(&tab);
-    const uint32_t n = a.c->num_active;
+    const uint32_t n = a.c->num_active, epoch = a.c->epoch;
     const uint32_t ntiles = div_up(n, kChunk);
     const uint32_t e0 = scan_site_epoch(s0);
     const uint32_t wave = threadIdx.x / 64;
@@ -285,7 +381,9 @@ __global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, Eng
         cs.bid[threadIdx.x] = in ? a.act_b[k0 + threadIdx.x] : 0u;
         cs.aux[threadIdx.x] = in ? a.act_q[k0 + threadIdx.x] : 0u;
         __syncthreads();
-        auto setup = [&](uint32_t e, RowBlock &b) { eng_row_block(a, cs.aux[e], cs.bid[e], k0 + e < n, b); };
+        auto setup = [&](uint32_t e, RowBlock &b) {
+            eng_row_block(a, cs.aux[e], cs.bid[e], k0 + e < n, epoch, b);
+        };
         count_sweep(cs, wave, tab, setup);
         __syncthreads();
         const uint32_t c = cs.cnt[threadIdx.x];
@@ -329,7 +427,7 @@ struct EngVolume {
     uint32_t dims[3], nb[3], n, maxbits;
     uint64_t *stream = nullptr;
     uint64_t nwords = 0;
-    float2 *ranges = nullptr;
+    float2 *ranges = nullptr, *dilated = nullptr, *seg_range = nullptr;
     int32_t *slot_map = nullptr;
     uint32_t *miss_list = nullptr;  // per-step block ids to decode (EngVol::miss_list)
 };
@@ -348,8 +446,10 @@ struct bcmc_engine {
     // per-step arrays (grow)
     uint64_t blocks_cap = 0;
     uint32_t *flags = nullptr, *act_q = nullptr, *act_b = nullptr;
-    uint8_t *code = nullptr;
     unsigned long long *vert_offsets = nullptr;
+    uint64_t rows_cap = 0, row_elems_cap = 0;
+    uint32_t *cand_q = nullptr, *cand_seg = nullptr;
+    ScanSite site_rows;
     uint32_t q_cap = 0;
     EngQuery *d_q = nullptr;
     uint32_t *q_act_start = nullptr;
@@ -371,19 +471,41 @@ static int eng_ensure_blocks(bcmc_engine *e, uint64_t total) {
     dev_free(e->flags);
     dev_free(e->act_q);
     dev_free(e->act_b);
-    dev_free(e->code);
     dev_free(e->vert_offsets);
-    scan_site_free(&e->site_active);
     scan_site_free(&e->site_mc);
     const uint64_t cap = std::max<uint64_t>(total, e->blocks_cap * 3 / 2);
     BCMC_TRY(dev_alloc((void **)&e->flags, cap * 4));
+    // flag words hold the epoch of the step that found the block active;
+    // zero is no step's epoch
+    BCMC_HIP_CHECK(hipMemsetAsync(e->flags, 0, cap * 4, e->ctx->stream));
     BCMC_TRY(dev_alloc((void **)&e->act_q, cap * 4));
     BCMC_TRY(dev_alloc((void **)&e->act_b, cap * 4));
-    BCMC_TRY(dev_alloc((void **)&e->code, cap));
     BCMC_TRY(dev_alloc((void **)&e->vert_offsets, 8));
-    BCMC_TRY(scan_site_alloc(&e->site_active, cap, kScanTile));
     BCMC_TRY(scan_site_alloc(&e->site_mc, cap, kChunk));
     e->blocks_cap = cap;
+    return BCMC_OK;
+}
+
+// candidate-row lists and the padded row-block element space of a step
+static int eng_ensure_rows(bcmc_engine *e, uint64_t rows, uint64_t row_elems) {
+    if (rows > e->rows_cap) {
+        BCMC_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
+        dev_free(e->cand_q);
+        dev_free(e->cand_seg);
+        scan_site_free(&e->site_rows);
+        const uint64_t cap = std::max<uint64_t>(rows, e->rows_cap * 3 / 2);
+        BCMC_TRY(dev_alloc((void **)&e->cand_q, cap * 4));
+        BCMC_TRY(dev_alloc((void **)&e->cand_seg, cap * 4));
+        BCMC_TRY(scan_site_alloc(&e->site_rows, cap, kScanTile));
+        e->rows_cap = cap;
+    }
+    if (row_elems > e->row_elems_cap) {
+        BCMC_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
+        scan_site_free(&e->site_active);
+        const uint64_t cap = std::max<uint64_t>(row_elems, e->row_elems_cap * 3 / 2);
+        BCMC_TRY(scan_site_alloc(&e->site_active, cap, kScanTile));
+        e->row_elems_cap = cap;
+    }
     return BCMC_OK;
 }
 
@@ -482,6 +604,8 @@ BCMC_API int bcmc_engine_destroy(bcmc_engine *e) {
     for (auto &v : e->vols) {
         dev_free(v.stream);
         dev_free(v.ranges);
+        dev_free(v.seg_range);
+        dev_free(v.dilated);
         dev_free(v.slot_map);
         dev_free(v.miss_list);
     }
@@ -498,7 +622,9 @@ BCMC_API int bcmc_engine_destroy(bcmc_engine *e) {
     dev_free(e->flags);
     dev_free(e->act_q);
     dev_free(e->act_b);
-    dev_free(e->code);
+    dev_free(e->cand_q);
+    dev_free(e->cand_seg);
+    scan_site_free(&e->site_rows);
     dev_free(e->vert_offsets);
     dev_free(e->d_q);
     dev_free(e->q_act_start);
@@ -541,14 +667,24 @@ BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t
     BCMC_TRY(dev_alloc((void **)&v.miss_list, n * 4));
     BCMC_HIP_CHECK(hipMemsetAsync(v.slot_map, 0xFF, n * 4, ctx->stream));
     launch_block_ranges(ctx, v.stream, v.nwords, v.maxbits, v.dims, v.ranges);
+    const uint32_t nsegx = div_up(v.nb[0], kSegBlocks), nsegs = nsegx * v.nb[1] * v.nb[2];
+    BCMC_TRY(dev_alloc((void **)&v.dilated, n * 8));
+    BCMC_TRY(dev_alloc((void **)&v.seg_range, (size_t)nsegs * 8));
+    launch_dilate_ranges(ctx, v.ranges, v.nb, v.dilated);
+    eng_seg_range_kernel<<<eng_grid(ctx, nsegs), 256, 0, ctx->stream>>>(v.dilated, v.nb[0], nsegx, nsegs,
+                                                                         v.seg_range);
     BCMC_HIP_CHECK(hipGetLastError());
+    BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    dev_free(v.ranges);  // only the dilated ranges are used from here on
+    v.ranges = nullptr;
     EngVol dv;
     for (int i = 0; i < 3; ++i)
         dv.nb[i] = v.nb[i];
     dv.n = v.n;
     dv.div_x = make_fast_div(v.nb[0]);
     dv.div_y = make_fast_div(v.nb[1]);
-    dv.ranges = v.ranges;
+    dv.dilated = v.dilated;
+    dv.seg_range = v.seg_range;
     dv.slot_map = v.slot_map;
     dv.miss_list = v.miss_list;
     const uint32_t id = (uint32_t)e->vols.size();
@@ -589,24 +725,29 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     hipStream_t st = ctx->stream;
     std::vector<EngQuery> qs(nq);
     std::vector<bool> used(e->vols.size(), false);
-    uint64_t total = 0;
+    uint64_t total = 0, rows = 0;
     for (uint32_t i = 0; i < nq; ++i) {
         BCMC_CHECK_ARG(volume_ids[i] < e->vols.size(), "unknown volume id");
         const EngVolume &v = e->vols[volume_ids[i]];
-        qs[i] = EngQuery{volume_ids[i], isovalues[i], (uint32_t)total, v.n};
+        qs[i] = EngQuery{volume_ids[i], isovalues[i], (uint32_t)total, v.n, (uint32_t)rows, {0, 0, 0}};
         used[volume_ids[i]] = true;
         total += v.n;
+        rows += (uint64_t)div_up(v.nb[0], kSegBlocks) * v.nb[1] * v.nb[2];
     }
     BCMC_CHECK_ARG(total <= kMaxStepBlocks, "too many blocks in one step");
+    const uint64_t row_elems = rows * kSegBlocks;
+    BCMC_CHECK_ARG(row_elems < (1ull << 32) - kScanTile, "too many row segments in one step");
     BCMC_TRY(eng_ensure_blocks(e, total));
+    BCMC_TRY(eng_ensure_rows(e, rows, row_elems));
     BCMC_TRY(eng_ensure_queries(e, nq));
     BCMC_HIP_CHECK(hipMemcpyAsync(e->d_q, qs.data(), nq * sizeof(EngQuery), hipMemcpyHostToDevice, st));
-    eng_begin_kernel<<<1, 1, 0, st>>>(e->c);
-    eng_code_kernel<<<eng_grid(ctx, total, 16), 256, 0, st>>>(e->d_q, nq, e->d_vols, (uint32_t)total,
-                                                                e->code);
-    EngActiveOp aop{e->d_q, nq, e->d_vols, e->code, e->flags, e->act_q, e->act_b, e->q_act_start, e->c,
-                    (uint32_t)total};
-    launch_scan<1>(ctx, e->site_active, e->site_active, aop, total);
+    eng_begin_kernel<<<1, 256, 0, st>>>(e->c, e->q_act_start, nq);
+    // active pairs: candidate row segments first, then their blocks
+    launch_scan<1>(ctx, e->site_rows, e->site_rows,
+                   CandSegOp{e->d_q, nq, e->d_vols, e->cand_q, e->cand_seg, e->c, (uint32_t)rows}, rows);
+    SegActiveOp aop{e->d_q, e->d_vols, e->cand_q, e->cand_seg, e->flags, e->act_q, e->act_b,
+                    e->q_act_start, e->c};
+    launch_scan<1>(ctx, e->site_active, e->site_active, aop, row_elems);
     eng_mark_kernel<<<eng_grid(ctx, total), 256, 0, st>>>(e->d_q, e->d_vols, e->act_q, e->act_b, e->c,
                                                           e->stamp, e->miss, e->miss_cap);
     // eviction order (only does work when the misses exceed the free room).
@@ -621,8 +762,12 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
                        e->pool_cap);
         eng_cand_keys_kernel<<<eng_grid(ctx, e->pool_cap, 2), 256, 0, st>>>(e->stamp, e->cand, e->c,
                                                                              e->cand_keys);
+        // keys are stamps, at most this step's epoch
+        uint32_t bits = 0;
+        while (bits < 32 && ((uint64_t)(e->ch->epoch + 1) >> bits) != 0)
+            ++bits;
         launch_sort_pairs(ctx, &e->sort, e->cand_keys, e->cand, &e->c->num_cand, e->pool_cap,
-                          e->order);
+                          e->order, std::max<uint32_t>(1, (bits + 7) / 8));
     }
     eng_assign_kernel<<<eng_grid(ctx, total, 4), 256, 0, st>>>(e->d_vols, e->c, e->miss, e->order,
                                                                 e->pool_cap, e->stamp, e->owner);
@@ -661,12 +806,19 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     // a query's vertices run from its first active pair's offset to the next
     // non-empty query's (or the step total)
     e->last.resize(nq);
-    const uint32_t na = e->ch->num_active;
+    // q_act_start[i]: index of the query's first active pair (kNoPairs: none)
+    uint32_t next_a = e->ch->num_active;
     uint64_t next = nv;
     for (uint32_t i = nq; i-- > 0;) {
         bcmc_query_result r;
-        r.num_active_blocks = (i + 1 < nq ? qa[i + 1] : na) - qa[i];
-        r.vertex_offset = r.num_active_blocks ? qv[i] : next;
+        if (qa[i] == kNoPairs) {
+            r.num_active_blocks = 0;
+            r.vertex_offset = next;
+        } else {
+            r.num_active_blocks = next_a - qa[i];
+            r.vertex_offset = qv[i];
+            next_a = qa[i];
+        }
         r.num_vertices = next - r.vertex_offset;
         r.step_decoded_blocks = e->ch->num_miss;
         next = r.vertex_offset;

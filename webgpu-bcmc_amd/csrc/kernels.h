@@ -157,6 +157,17 @@ void launch_decode_to_cache(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwor
                             uint32_t maxbits, const uint32_t *ids, const uint32_t *n_dev,
                             uint32_t n_max, const int32_t *item_slot, float *cache,
                             uint32_t capacity);
+// LRU assignment fused with the decode: new item w (w < D, 0 when no slot is
+// available) evicts the previous item of slot order[w], takes the slot
+// {age 0, unavailable} and is decoded into it; commits the cache size.
+void launch_decode_assign(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords, uint32_t maxbits,
+                          const uint32_t *new_items, uint32_t n_max, const uint32_t *order,
+                          uint32_t *slot_age, uint32_t *slot_avail, int32_t *slot_item,
+                          int32_t *item_slot, uint32_t total, float *cache, uint32_t capacity,
+                          QState *q);
+// Per block the range [min lo, max hi] over its in-bounds 3x3x3 neighbourhood
+// (the active test's interval, see block_active).
+void launch_dilate_ranges(bcmc_ctx *ctx, const float2 *ranges, const uint32_t nb[3], float2 *out);
 void launch_decode_dense(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords, uint32_t maxbits,
                          const uint32_t *ids, uint32_t n, float *out);
 
@@ -173,14 +184,16 @@ void sort_scratch_free(SortScratch *s);
 
 // Orders the `count` available cache slots for LRU assignment exactly as the
 // reference's RadixSorter.sort(ages, slot ids, S, reverse=true) does, writing
-// slot ids to out[0..count).  count is read from device memory.
+// slot ids to out[0..count).  count is read from device memory.  npasses: the
+// 8-bit digits of ~age that vary (those above every age are all 0xFF).
 void launch_lru_order(bcmc_ctx *ctx, SortScratch *s, const uint32_t *avail_ids,
                       const uint32_t *age, const uint32_t *count_dev, uint32_t count_max,
-                      uint32_t *out);
+                      uint32_t *out, uint32_t npasses);
 // Stable ascending sort of `count` (key, value) pairs (count read from device
-// memory); sorted values to out[0..count).
+// memory) on the low 8 * npasses key bits; sorted values to out[0..count).
 void launch_sort_pairs(bcmc_ctx *ctx, SortScratch *s, const uint32_t *keys, const uint32_t *vals,
-                       const uint32_t *count_dev, uint32_t count_max, uint32_t *out);
+                       const uint32_t *count_dev, uint32_t count_max, uint32_t *out,
+                       uint32_t npasses);
 // RadixSorter.sort semantics for host-known n over an aligned buffer.
 void launch_sort_by_key(bcmc_ctx *ctx, SortScratch *s, uint32_t *keys, uint32_t *vals, uint32_t n,
                         int reverse);

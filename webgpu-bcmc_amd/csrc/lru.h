@@ -23,6 +23,7 @@ struct LruState {
     uint32_t *new_items = nullptr;// [aligned_total] ids to decode, ascending
     uint32_t total = 0, aligned_total = 0, elem_bytes = 0, capacity = 0;
     uint32_t cache_size = 0;      // host mirror of the logical size
+    uint64_t age_bound = 0;       // every slot age is below this (host-side bound)
     ScanSite site_miss, site_avail;
     SortScratch sort;
 };
@@ -30,11 +31,12 @@ struct LruState {
 int lru_state_alloc(bcmc_ctx *ctx, LruState *L, uint32_t initial, uint32_t elem_bytes,
                     uint32_t total);
 void lru_state_free(LruState *L);
-// LRUCache.update for a marching-cubes query: `flags` are the N active flags,
-// `active_ids` their ascending compaction with q->num_active entries; does
-// nothing when q->num_active == 0 (compressed_marching_cubes.js:630-641).
-void lru_enqueue_update_active(bcmc_ctx *ctx, LruState *L, const uint32_t *flags,
-                               const uint32_t *active_ids, QState *q);
+// LRUCache.update for a marching-cubes query whose misses the active pass has
+// already listed (new_items, q->num_new): `flags` are the N active flags; does
+// nothing when q->num_active == 0 (compressed_marching_cubes.js:630-641).  Ends
+// with the eviction order in v.order; the slot assignment is done by the
+// decoder (launch_decode_assign), which decodes each new item into its slot.
+void lru_enqueue_update_mc(bcmc_ctx *ctx, LruState *L, const uint32_t *flags, QState *q);
 // LRUCache.update for arbitrary 0/1 `needed` flags over all items.
 void lru_enqueue_update_all(bcmc_ctx *ctx, LruState *L, const uint32_t *needed, QState *q);
 void lru_enqueue_reset(bcmc_ctx *ctx, LruState *L);

@@ -43,28 +43,8 @@ __global__ void lru_init_kernel(LruView v, uint32_t first, uint32_t end, uint32_
             v.item_slot[i] = -1;
 }
 
-// age + mark (lru_cache_age_slots.comp:45-52, lru_cache_mark_new_items.comp:80-98)
-__global__ void lru_age_mark_kernel(LruView v, const uint32_t *needed, const QState *q,
-                                    const uint32_t *gate) {
-    if (gate && *gate == 0)
-        return;
-    const uint32_t c = q->cache_size;
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < c; s += gridDim.x * blockDim.x) {
-        uint32_t a = v.age[s] + 1;
-        const int32_t p = v.item[s];
-        if (p >= 0 && (uint32_t)p < v.total) {
-            if (needed[p] == 1) {
-                a = 0;
-                v.avail[s] = 0;
-            } else {
-                v.avail[s] = 1;
-            }
-        }
-        v.age[s] = a;
-    }
-}
-
-// misses over all items: needed (0/1) and not cached
+// misses over all items: needed (0/1) and not cached (lru_cache_mark_new_items.comp:28-43,
+// lru_cache.js:463-492)
 struct MissAllOp {
     LruView v;
     const uint32_t *needed;
@@ -82,73 +62,74 @@ struct MissAllOp {
     BCMC_DEV void total(const unsigned long long (&t)[1]) const { q->num_new = (uint32_t)t[0]; }
 };
 
-// misses over the active list of a marching-cubes query (only active blocks
-// can be needed); the list is ascending, so the result is too
-struct MissActiveOp {
+// Age + mark over the C slots (lru_cache_age_slots.comp:45-52,
+// lru_cache_mark_new_items.comp:80-98) fused with the extraction of the
+// available slots (lru_cache.js:501, :684-710): a slot ages by one; a slot
+// whose item is needed becomes {age 0, unavailable}, a slot whose item is
+// cached but not needed becomes available (the item->slot map is a bijection
+// on cached items, so visiting slots visits the cached items).  Available
+// slots are listed only when there is something to insert (q->num_new, known
+// before this pass), as the reference returns before extracting them
+// otherwise.  `gate`: skip the whole update when *gate == 0.
+struct AgeAvailOp {
     LruView v;
-    const uint32_t *active_ids;
-    uint32_t *new_items;
+    const uint32_t *needed;
     QState *q;
-    BCMC_DEV uint32_t size() const { return q->num_active; }
+    const uint32_t *gate;
+    BCMC_DEV uint32_t size() const { return (gate && *gate == 0) ? 0u : q->cache_size; }
     BCMC_DEV void begin() const {}
-    BCMC_DEV void value(uint32_t k, uint32_t (&x)[1]) const {
-        x[0] = v.item_slot[active_ids[k]] < 0 ? 1u : 0u;
+    BCMC_DEV void value(uint32_t s, uint32_t (&x)[1]) const {
+        uint32_t a = v.age[s] + 1, av = v.avail[s];
+        const int32_t p = v.item[s];
+        if (p >= 0 && (uint32_t)p < v.total) {
+            av = needed[p] == 1 ? 0u : 1u;
+            if (!av)
+                a = 0;
+            v.avail[s] = av;
+        }
+        v.age[s] = a;
+        x[0] = (av != 0 && q->num_new != 0) ? 1u : 0u;
     }
-    BCMC_DEV void emit(uint32_t k, const uint32_t (&x)[1], const unsigned long long (&e)[1]) const {
-        if (x[0])
-            new_items[e[0]] = active_ids[k];
-    }
-    BCMC_DEV void total(const unsigned long long (&t)[1]) const { q->num_new = (uint32_t)t[0]; }
-};
-
-// available slots (skipped when there is nothing to insert)
-struct AvailOp {
-    LruView v;
-    QState *q;
-    BCMC_DEV uint32_t size() const { return q->num_new ? q->cache_size : 0u; }
-    BCMC_DEV void begin() const {}
-    BCMC_DEV void value(uint32_t s, uint32_t (&x)[1]) const { x[0] = v.avail[s] != 0 ? 1u : 0u; }
     BCMC_DEV void emit(uint32_t s, const uint32_t (&x)[1], const unsigned long long (&e)[1]) const {
         if (x[0])
             v.avail_ids[e[0]] = s;
     }
-    BCMC_DEV void total(const unsigned long long (&t)[1]) const { q->num_avail = (uint32_t)t[0]; }
+    BCMC_DEV void total(const unsigned long long (&t)[1]) const {
+        q->num_avail = q->num_new ? (uint32_t)t[0] : 0u;
+    }
 };
 
-// lru_cache.js:510-522 on one thread: next size and total available slots.
-__global__ void lru_plan_kernel(LruView v, QState *q, const uint32_t *gate) {
+// Growth (lru_cache.js:510-672) for the available count S of this update:
+// when S < D, C' = alignTo(min(C + ceil(1.5 (D - S)), N), 32) (storage for
+// alignTo(N, 32) slots is reserved up front); slots [C, C') become
+// {100000, 1, -1} and are appended to the available list.  Every thread
+// evaluates the rule; thread 0 of workgroup 0 records C' and S'.
+__global__ void lru_grow_kernel(LruView v, QState *q, const uint32_t *gate) {
     const bool run = !(gate && *gate == 0) && q->num_new != 0;
-    const uint32_t c = q->cache_size;
-    uint32_t next = c, avail = run ? q->num_avail : 0u;
+    const uint32_t c = q->cache_size, avail = run ? q->num_avail : 0u;
+    uint32_t next = c;
     if (run && avail < q->num_new) {
         const uint64_t want = (uint64_t)c + ((uint64_t)(q->num_new - avail) * 3 + 1) / 2;
         uint64_t sz = std::min<uint64_t>(want, v.total);
         sz = (sz + 31) / 32 * 32;
         next = (uint32_t)std::min<uint64_t>(sz, v.capacity);
-        avail += next - c;
     }
-    q->cache_size_next = next;
-    q->num_avail = avail;
-}
-
-// new slots [C, C'): {100000, 1, -1}, appended to the available list
-__global__ void lru_grow_kernel(LruView v, const QState *q) {
-    const uint32_t c = q->cache_size, c2 = q->cache_size_next;
-    if (c2 <= c)
-        return;
-    const uint32_t base = q->num_avail - (c2 - c);
-    for (uint32_t s = c + blockIdx.x * blockDim.x + threadIdx.x; s < c2;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        q->cache_size_next = next;
+        q->num_avail_total = avail + (next - c);
+    }
+    for (uint32_t s = c + blockIdx.x * blockDim.x + threadIdx.x; s < next;
          s += gridDim.x * blockDim.x) {
         v.age[s] = 100000;
         v.avail[s] = 1;
         v.item[s] = -1;
-        v.avail_ids[base + (s - c)] = s;
+        v.avail_ids[avail + (s - c)] = s;
     }
 }
 
 // lru_cache_update.comp:173-191; thread 0 also commits the new size.
 __global__ void lru_assign_kernel(LruView v, const uint32_t *new_items, QState *q) {
-    const uint32_t n = q->num_avail ? q->num_new : 0u;
+    const uint32_t n = q->num_avail_total ? q->num_new : 0u;
     for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) {
         const uint32_t it = new_items[w], s = v.order[w];
         if (s >= v.capacity || it >= v.total)
@@ -162,7 +143,7 @@ __global__ void lru_assign_kernel(LruView v, const uint32_t *new_items, QState *
         v.item_slot[it] = (int32_t)s;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        q->avail_display = q->num_avail - n;
+        q->avail_display = q->num_avail_total - n;
         q->cache_size = q->cache_size_next;
     }
 }
@@ -195,6 +176,7 @@ int lru_state_alloc(bcmc_ctx *ctx, LruState *L, uint32_t initial, uint32_t elem_
     BCMC_TRY(sort_scratch_alloc(&L->sort, L->capacity));
     L->v.total = total;
     L->v.capacity = L->capacity;
+    L->age_bound = std::max<uint64_t>((uint64_t)L->cache_size + 1, 100000);
     lru_init_kernel<<<grid_for(ctx, cap), 256, 0, ctx->stream>>>(L->v, 0, L->cache_size, 0, 1);
     BCMC_HIP_CHECK(hipGetLastError());
     return BCMC_OK;
@@ -215,32 +197,38 @@ void lru_state_free(LruState *L) {
     *L = LruState{};
 }
 
-// Shared tail of an update once q->num_new is on the device.
-static void enqueue_tail(bcmc_ctx *ctx, LruState *L, QState *q, const uint32_t *gate) {
-    hipStream_t st = ctx->stream;
-    launch_scan<1>(ctx, L->site_avail, L->site_avail, AvailOp{L->v, q}, L->capacity);
-    lru_plan_kernel<<<1, 1, 0, st>>>(L->v, q, gate);
-    lru_grow_kernel<<<grid_for(ctx, L->capacity, 2), 256, 0, st>>>(L->v, q);
-    launch_lru_order(ctx, &L->sort, L->v.avail_ids, L->v.age, &q->num_avail, L->capacity,
-                     L->v.order);
-    lru_assign_kernel<<<grid_for(ctx, L->capacity, 2), 256, 0, st>>>(L->v, L->new_items, q);
+// Radix-sort passes for the eviction order: keys are ~age, and every age is
+// below the tracked bound (initial ages 2+s, grown and reset slots 100000, +1
+// per update), so the 8-bit digits above the bound's are all 0xFF and their
+// passes would keep the order.
+static uint32_t lru_sort_passes(const LruState *L) {
+    uint32_t bits = 0;
+    while (bits < 32 && (L->age_bound >> bits) != 0)
+        ++bits;
+    return std::max<uint32_t>(1, (bits + 7) / 8);
 }
 
-void lru_enqueue_update_active(bcmc_ctx *ctx, LruState *L, const uint32_t *flags,
-                               const uint32_t *active_ids, QState *q) {
-    lru_age_mark_kernel<<<grid_for(ctx, L->capacity), 256, 0, ctx->stream>>>(L->v, flags, q,
-                                                                              &q->num_active);
-    launch_scan<1>(ctx, L->site_miss, L->site_miss, MissActiveOp{L->v, active_ids, L->new_items, q},
-                   L->aligned_total);
-    enqueue_tail(ctx, L, q, &q->num_active);
+// Age/mark + available list, growth and eviction order, once q->num_new and
+// the new items are on the device.
+static void enqueue_tail(bcmc_ctx *ctx, LruState *L, const uint32_t *needed, QState *q,
+                         const uint32_t *gate) {
+    hipStream_t st = ctx->stream;
+    launch_scan<1>(ctx, L->site_avail, L->site_avail, AgeAvailOp{L->v, needed, q, gate}, L->capacity);
+    lru_grow_kernel<<<grid_for(ctx, L->capacity, 2), 256, 0, st>>>(L->v, q, gate);
+    launch_lru_order(ctx, &L->sort, L->v.avail_ids, L->v.age, &q->num_avail_total, L->capacity,
+                     L->v.order, lru_sort_passes(L));
+    L->age_bound += 1;
+}
+
+void lru_enqueue_update_mc(bcmc_ctx *ctx, LruState *L, const uint32_t *flags, QState *q) {
+    enqueue_tail(ctx, L, flags, q, &q->num_active);
 }
 
 void lru_enqueue_update_all(bcmc_ctx *ctx, LruState *L, const uint32_t *needed, QState *q) {
-    lru_age_mark_kernel<<<grid_for(ctx, L->capacity), 256, 0, ctx->stream>>>(L->v, needed, q,
-                                                                              nullptr);
     launch_scan<1>(ctx, L->site_miss, L->site_miss, MissAllOp{L->v, needed, L->new_items, q},
                    L->aligned_total);
-    enqueue_tail(ctx, L, q, nullptr);
+    enqueue_tail(ctx, L, needed, q, nullptr);
+    lru_assign_kernel<<<grid_for(ctx, L->capacity, 2), 256, 0, ctx->stream>>>(L->v, L->new_items, q);
 }
 
 void lru_enqueue_reset(bcmc_ctx *ctx, LruState *L) {

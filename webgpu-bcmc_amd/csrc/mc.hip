@@ -29,35 +29,40 @@
 namespace bcmc {
 
 // ---- active blocks ---------------------------------------------------------
-// Pass 1: per-block code (own test + x-direction OR), pass 2: y/z OR fused
-// with the single-pass scan and compaction of the flags.
-__global__ void active_code_kernel(const float2 *ranges, uint8_t *code, float iso, uint32_t nbx,
-                                   uint32_t n) {
-    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < n; b += gridDim.x * blockDim.x)
-        code[b] = (uint8_t)active_code(ranges, b, b % nbx, nbx, iso);
-}
-
+// One single-pass scan over the N blocks with two channels: the active flags
+// (one comparison against the dilated range, see block_active) compacted to
+// the ascending active ids, and the LRU misses among them (active and not
+// cached, lru_cache_mark_new_items.comp:28-43) compacted to the ascending new
+// item ids (lru_cache.js:463-492).
 struct ActiveOp {
-    const uint8_t *code;
-    uint32_t *flags, *active_ids;
+    const float2 *dilated;
+    const int32_t *item_slot;
+    uint32_t *flags, *active_ids, *new_items;
     QState *q;
     float iso;
-    uint32_t nb[3], n;
+    uint32_t n;
     BCMC_DEV uint32_t size() const { return n; }
     BCMC_DEV void begin() const {
         q->iso = iso;
-        q->num_new = 0;
         q->num_with = 0;
         q->num_verts = 0;
         q->overflow = 0;
     }
-    BCMC_DEV void value(uint32_t b, uint32_t (&v)[1]) const { v[0] = active_from_codes(code, b, nb); }
-    BCMC_DEV void emit(uint32_t b, const uint32_t (&v)[1], const unsigned long long (&e)[1]) const {
+    BCMC_DEV void value(uint32_t b, uint32_t (&v)[2]) const {
+        v[0] = block_active(dilated[b], iso) ? 1u : 0u;
+        v[1] = (v[0] && item_slot[b] < 0) ? 1u : 0u;
+    }
+    BCMC_DEV void emit(uint32_t b, const uint32_t (&v)[2], const unsigned long long (&e)[2]) const {
         flags[b] = v[0];
         if (v[0])
             active_ids[e[0]] = b;
+        if (v[1])
+            new_items[e[1]] = b;
     }
-    BCMC_DEV void total(const unsigned long long (&t)[1]) const { q->num_active = (uint32_t)t[0]; }
+    BCMC_DEV void total(const unsigned long long (&t)[2]) const {
+        q->num_active = (uint32_t)t[0];
+        q->num_new = (uint32_t)t[1];
+    }
 };
 
 // ---- count + offsets + emission in one pass --------------------------------
@@ -109,6 +114,7 @@ __global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanS
     base.div_x = a.div_x;
     base.div_y = a.div_y;
     base.cap = a.cache_cap;
+    base.tag = 1;
     base.iso = a.q->iso;
     if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
         a.q->num_with = 0;
@@ -176,7 +182,7 @@ struct bcmc_mc {
     uint64_t stream_bytes = 0, stream_words = 0;
     float2 *ranges = nullptr;
     uint32_t *flags = nullptr, *active_ids = nullptr, *with_verts = nullptr;
-    uint8_t *active_code = nullptr;
+    float2 *dilated = nullptr;  // per block: range over its 3x3x3 neighbourhood
     unsigned long long *vert_offsets = nullptr;
     bcmc_lru lru;  // non-owning handle over the cache state held here
     QState *q = nullptr, *qh = nullptr;
@@ -197,7 +203,7 @@ static void mc_free_volume(bcmc_mc *m) {
     dev_free(m->ranges);
     dev_free(m->flags);
     dev_free(m->active_ids);
-    dev_free(m->active_code);
+    dev_free(m->dilated);
     dev_free(m->with_verts);
     dev_free(m->vert_offsets);
     dev_free(m->verts);
@@ -208,7 +214,7 @@ static void mc_free_volume(bcmc_mc *m) {
     m->stream = nullptr;
     m->ranges = nullptr;
     m->flags = m->active_ids = m->with_verts = nullptr;
-    m->active_code = nullptr;
+    m->dilated = nullptr;
     m->vert_offsets = nullptr;
     m->verts = nullptr;
     m->vert_cap = 0;
@@ -331,7 +337,7 @@ BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *m, const void *stream, uint6
     BCMC_TRY(dev_alloc((void **)&m->ranges, n * 8));
     BCMC_TRY(dev_alloc((void **)&m->flags, n * 4));
     BCMC_TRY(dev_alloc((void **)&m->active_ids, n * 4));
-    BCMC_TRY(dev_alloc((void **)&m->active_code, n));
+    BCMC_TRY(dev_alloc((void **)&m->dilated, n * 8));
     BCMC_TRY(dev_alloc((void **)&m->with_verts, n * 4));
     BCMC_TRY(dev_alloc((void **)&m->vert_offsets, n * 8));
     BCMC_TRY(scan_site_alloc(&m->site_active, n, kScanTile));
@@ -347,6 +353,7 @@ BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *m, const void *stream, uint6
     BCMC_HIP_CHECK(hipMemcpyAsync(m->q, m->qh, sizeof(QState), hipMemcpyHostToDevice, ctx->stream));
     // computeBlockRanges (:522-578): full-volume decode, min/max per block
     launch_block_ranges(ctx, m->stream, m->stream_words, maxbits, m->dims, m->ranges);
+    launch_dilate_ranges(ctx, m->ranges, m->nb, m->dilated);
     BCMC_HIP_CHECK(hipGetLastError());
     BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     m->has_volume = true;
@@ -384,23 +391,21 @@ BCMC_API int bcmc_mc_compute_surface(bcmc_mc *m, float isovalue, bcmc_perf *perf
     const bool timed = perf != nullptr;
     if (timed)
         BCMC_HIP_CHECK(hipEventRecord(m->ev[0], st));
-    // active blocks + compaction
-    active_code_kernel<<<std::max<uint32_t>(1, std::min<uint32_t>(div_up(m->total, 256),
-                                                                  persistent_grid(ctx, 16))),
-                         256, 0, st>>>(m->ranges, m->active_code, isovalue, m->nb[0], m->total);
-    ActiveOp aop{m->active_code, m->flags, m->active_ids, m->q, isovalue,
-                 {m->nb[0], m->nb[1], m->nb[2]}, m->total};
-    launch_scan<1>(ctx, m->site_active, m->site_active, aop, m->total);
+    // active blocks and the LRU misses among them, one scan
+    ActiveOp aop{m->dilated, m->lru.st.v.item_slot, m->flags, m->active_ids, m->lru.st.new_items,
+                 m->q, isovalue, m->total};
+    launch_scan<2>(ctx, m->site_active, m->lru.st.site_miss, aop, m->total);
     if (timed)
         BCMC_HIP_CHECK(hipEventRecord(m->ev[1], st));
-    // cache update
-    lru_enqueue_update_active(ctx, &m->lru.st, m->flags, m->active_ids, m->q);
+    // cache update: age/mark + available slots, growth, eviction order
+    lru_enqueue_update_mc(ctx, &m->lru.st, m->flags, m->q);
     if (timed)
         BCMC_HIP_CHECK(hipEventRecord(m->ev[2], st));
-    // decode misses into their slots
-    launch_decode_to_cache(ctx, m->stream, m->stream_words, m->maxbits, m->lru.st.new_items,
-                           &m->q->num_new, m->total, m->lru.st.v.item_slot,
-                           (float *)m->lru.st.data, m->lru.st.capacity);
+    // slot assignment and decode of the misses into their slots
+    LruState &L = m->lru.st;
+    launch_decode_assign(ctx, m->stream, m->stream_words, m->maxbits, L.new_items, m->total, L.v.order,
+                         L.v.age, L.v.avail, L.v.item, L.v.item_slot, L.total, (float *)L.data,
+                         L.capacity, m->q);
     if (timed)
         BCMC_HIP_CHECK(hipEventRecord(m->ev[3], st));
     // counts, offsets and vertices in one pass

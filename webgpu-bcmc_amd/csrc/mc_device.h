@@ -180,6 +180,7 @@ struct RowBlock {
     uint32_t nb[3];
     FastDiv div_x, div_y;      // by nb[0], nb[1]
     uint32_t cap;              // cache slots
+    uint32_t tag;              // a neighbour is active iff its flag word equals tag
     float iso;
     uint32_t bid;
     bool live;
@@ -225,9 +226,9 @@ BCMC_DEV RowMeta row_meta_load(const RowBlock &b) {
 
 // Stores the row's slots and tile dims into its block's meta words.  Must be
 // called by all 64 lanes.
-BCMC_DEV void row_meta_store(const RowMeta &m, const McTables &tab, uint32_t *meta) {
+BCMC_DEV void row_meta_store(const RowMeta &m, uint32_t tag, const McTables &tab, uint32_t *meta) {
     const uint32_t lane = lane_id(), l = lane & 15u;
-    const uint64_t ab = __ballot(l >= 8 && l < 15 && m.v != 0);
+    const uint64_t ab = __ballot(l >= 8 && l < 15 && (uint32_t)m.v == tag);
     const uint32_t a = (uint32_t)(ab >> ((lane & 48u) + 8u)) & 0x7Fu;
     if (l < 8)
         meta[l] = (uint32_t)m.v;
@@ -477,7 +478,7 @@ BCMC_DEV void count_sweep(ChunkShared &cs, uint32_t wave, const McTables &tab, S
     for (uint32_t q = 0; q < kRounds; ++q) {
         const uint32_t e = 64 * wave + kRowBlocks * q + row;
         const RowBlock bq = b;
-        row_meta_store(m, tab, cs.meta[e]);
+        row_meta_store(m, bq.tag, tab, cs.meta[e]);
         wave_lds_sync();
         RowTile t;
         row_voxels(bq, cs.meta[e], t);
@@ -514,59 +515,16 @@ BCMC_DEV void emit_sweep(ChunkShared &cs, uint32_t wave, uint32_t num_with, unsi
 }
 
 // ---------------------------------------------------------------------------
-// Active test (compute_block_active.comp:127-164) restated as a dilation.
-// The reference marks a block active when iso lies in its own range, or in
-// the union of its range with any in-bounds neighbour's.  With above = iso < lo
-// and below = iso > hi, the union with neighbour n misses iso exactly when
-// (above && above_n) || (below && below_n), so
-//   active = contains || (above && OR_27(!above_n)) || (below && OR_27(!below_n))
-// where OR_27 runs over the in-bounds 3x3x3 neighbourhood (the block's own
-// bit never contributes: !above is false when above).  The OR is separable:
-// the code byte holds the x-direction OR; the y/z part reads 9 code bytes.
+// Active test (compute_block_active.comp:30-60).  The reference marks block b
+// active when iso lies in [lo_b, hi_b], or in [min(lo_b, lo_n), max(hi_b, hi_n)]
+// for some in-bounds neighbour n of its 3x3x3 neighbourhood.  Each of these
+// intervals contains [lo_b, hi_b], so their union is the single interval
+// [min lo, max hi] over the neighbourhood (b included), and the test is one
+// pair of comparisons against that dilated range, computed once per volume
+// (launch_dilate_ranges).
+// min and max select one of their operands, so the comparisons see the same
+// float values as the reference's.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kActContains = 1, kActAbove = 2, kActBelow = 4, kActXNotAbove = 8,
-                   kActXNotBelow = 16;
-
-BCMC_DEV uint32_t active_code(const float2 *ranges, uint32_t b, uint32_t x, uint32_t nbx, float iso) {
-    const float2 r = ranges[b];
-    const bool above = iso < r.x, below = iso > r.y;
-    bool xna = !above, xnb = !below;
-    if (x > 0) {
-        const float2 l = ranges[b - 1];
-        xna |= !(iso < l.x);
-        xnb |= !(iso > l.y);
-    }
-    if (x + 1 < nbx) {
-        const float2 h = ranges[b + 1];
-        xna |= !(iso < h.x);
-        xnb |= !(iso > h.y);
-    }
-    const bool contains = iso >= r.x && iso <= r.y;
-    return (contains ? kActContains : 0u) | (above ? kActAbove : 0u) | (below ? kActBelow : 0u) |
-           (xna ? kActXNotAbove : 0u) | (xnb ? kActXNotBelow : 0u);
-}
-
-BCMC_DEV uint32_t active_from_codes(const uint8_t *code, uint32_t b, const uint32_t (&nb)[3]) {
-    const uint32_t c = code[b];
-    if (c & kActContains)
-        return 1u;
-    const uint32_t need = (c & kActAbove) ? kActXNotAbove : (c & kActBelow) ? kActXNotBelow : 0u;
-    if (!need)
-        return 0u;
-    const uint32_t plane = nb[0] * nb[1];
-    const uint32_t y = (b / nb[0]) % nb[1], z = b / plane;
-    uint32_t acc = 0;
-#pragma unroll
-    for (int dz = -1; dz <= 1; ++dz) {
-#pragma unroll
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int yy = (int)y + dy, zz = (int)z + dz;
-            if (yy < 0 || zz < 0 || yy >= (int)nb[1] || zz >= (int)nb[2])
-                continue;
-            acc |= code[(int64_t)b + (int64_t)dy * nb[0] + (int64_t)dz * plane];
-        }
-    }
-    return (acc & need) ? 1u : 0u;
-}
+BCMC_DEV bool block_active(float2 dilated, float iso) { return iso >= dilated.x && iso <= dilated.y; }
 
 }  // namespace bcmc

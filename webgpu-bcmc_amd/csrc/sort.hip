@@ -146,7 +146,8 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(Src src, uin
                     if (leader)
                         cnt[d] += (uint32_t)__popcll(peers);
                     if (dst >= out_skip && dst - out_skip < out_cap) {
-                        okeys[dst - out_skip] = k;
+                        if (okeys)
+                            okeys[dst - out_skip] = k;
                         ovals[dst - out_skip] = v;
                     }
                 }
@@ -156,29 +157,30 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(Src src, uin
     }
 }
 
+// `npasses` 8-bit digit passes, LSB first; the final pass writes only the
+// values (to final_vals, from position out_skip on).  Pass p reads the source
+// (p = 0) or buffer (p - 1) & 1, and writes buffer p & 1 or the final output.
 template <class FirstSrc>
 static void sort_passes(bcmc_ctx *ctx, SortScratch *s, uint32_t n_max, const uint32_t *n_dev,
-                        uint32_t n_host, const FirstSrc &first_src, uint32_t *final_keys,
+                        uint32_t n_host, const FirstSrc &first_src, uint32_t npasses,
                         uint32_t *final_vals, uint32_t out_skip) {
     const uint32_t tiles = std::max<uint32_t>(1, div_up(n_max, kSortTile));
     const uint32_t grid = std::min<uint32_t>(tiles, persistent_grid(ctx, 4));
     hipStream_t st = ctx->stream;
     HistScanOp hop{s->hist, n_dev, n_host};
-    for (int pass = 0; pass < 4; ++pass) {
+    for (uint32_t pass = 0; pass < npasses; ++pass) {
         const uint32_t shift = 8 * pass;
-        // pass 0: source -> buf0, 1: buf0 -> buf1, 2: buf1 -> buf0, 3: buf0 -> final
-        const int src_buf = (pass + 1) & 1;
-        const int dst_buf = pass & 1;
-        uint32_t *ok = pass == 3 ? final_keys : s->keys[dst_buf];
-        uint32_t *ov = pass == 3 ? final_vals : s->vals[dst_buf];
-        const uint32_t skip = pass == 3 ? out_skip : 0;
+        const bool last = pass + 1 == npasses;
+        uint32_t *ok = last ? nullptr : s->keys[pass & 1];
+        uint32_t *ov = last ? final_vals : s->vals[pass & 1];
+        const uint32_t skip = last ? out_skip : 0;
         if (pass == 0) {
             sort_hist_kernel<<<grid, kSortThreads, 0, st>>>(first_src, shift, s->hist);
             launch_scan<1>(ctx, s->site, s->site, hop, 256ull * tiles);
             sort_scatter_kernel<<<grid, kSortThreads, 0, st>>>(first_src, shift, s->hist, ok, ov, skip,
                                                                s->max_elems);
         } else {
-            BufSource b{s->keys[src_buf], s->vals[src_buf], n_dev, n_host};
+            BufSource b{s->keys[(pass - 1) & 1], s->vals[(pass - 1) & 1], n_dev, n_host};
             sort_hist_kernel<<<grid, kSortThreads, 0, st>>>(b, shift, s->hist);
             launch_scan<1>(ctx, s->site, s->site, hop, 256ull * tiles);
             sort_scatter_kernel<<<grid, kSortThreads, 0, st>>>(b, shift, s->hist, ok, ov, skip,
@@ -212,15 +214,16 @@ void sort_scratch_free(SortScratch *s) {
 
 void launch_lru_order(bcmc_ctx *ctx, SortScratch *s, const uint32_t *avail_ids,
                       const uint32_t *age, const uint32_t *count_dev, uint32_t count_max,
-                      uint32_t *out) {
+                      uint32_t *out, uint32_t npasses) {
     LruSource src{avail_ids, age, count_dev, count_max};
-    sort_passes(ctx, s, count_max, count_dev, 0, src, s->keys[1], out, 0);
+    sort_passes(ctx, s, count_max, count_dev, 0, src, npasses, out, 0);
 }
 
 void launch_sort_pairs(bcmc_ctx *ctx, SortScratch *s, const uint32_t *keys, const uint32_t *vals,
-                       const uint32_t *count_dev, uint32_t count_max, uint32_t *out) {
+                       const uint32_t *count_dev, uint32_t count_max, uint32_t *out,
+                       uint32_t npasses) {
     BufSource src{keys, vals, count_dev, 0};
-    sort_passes(ctx, s, count_max, count_dev, 0, src, s->keys[1], out, 0);
+    sort_passes(ctx, s, count_max, count_dev, 0, src, npasses, out, 0);
 }
 
 void launch_sort_by_key(bcmc_ctx *ctx, SortScratch *s, uint32_t *keys, uint32_t *vals, uint32_t n,
@@ -230,8 +233,7 @@ void launch_sort_by_key(bcmc_ctx *ctx, SortScratch *s, uint32_t *keys, uint32_t 
         aligned *= 2;
     GenericSource src{keys, vals, n, aligned, (uint32_t)(reverse != 0)};
     // reversed: the reference keeps the last n of the descending order
-    sort_passes(ctx, s, aligned, nullptr, aligned, src, s->keys[1], s->vals[1],
-                reverse ? aligned - n : 0);
+    sort_passes(ctx, s, aligned, nullptr, aligned, src, 4, s->vals[1], reverse ? aligned - n : 0);
 }
 
 }  // namespace bcmc

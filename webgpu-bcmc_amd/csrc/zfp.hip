@@ -200,7 +200,7 @@ BCMC_DEV void decode_block(BitReader &rd, uint32_t maxbits, float (&v)[64]) {
 // ---------------------------------------------------------------------------
 constexpr int kZfpThreads = 256;  // 4 waves = 256 blocks per workgroup pass
 
-enum ZfpMode { kModeRanges = 0, kModeCache = 1, kModeDense = 2 };
+enum ZfpMode { kModeRanges = 0, kModeCache = 1, kModeDense = 2, kModeLruAssign = 3 };
 
 struct ZfpArgs {
     const uint64_t *stream;
@@ -214,6 +214,12 @@ struct ZfpArgs {
     float *out;                // cache / dense output
     float2 *ranges;            // ranges mode
     uint32_t dims[3], nb[3];
+    // LRU-assign mode: new item w takes slot order[w] (lru_cache_update.comp)
+    const uint32_t *order;
+    uint32_t *slot_age, *slot_avail;
+    int32_t *slot_item, *item_slot_w;
+    uint32_t total;            // totalElements
+    QState *q;
 };
 
 // W = u64 words per block when 64 | maxbits (LDS-staged), 0 = generic.
@@ -222,7 +228,15 @@ __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
     constexpr int kStage = W > 0 ? 64 * W + 1 : 1;
     __shared__ uint64_t stage[kZfpThreads / 64][kStage];
     const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
-    const uint32_t n = a.n_dev ? *a.n_dev : a.n_host;
+    uint32_t n = a.n_dev ? *a.n_dev : a.n_host;
+    if constexpr (MODE == kModeLruAssign) {
+        // no assignment without available slots (lru_cache.js:717-769)
+        n = a.q->num_avail_total ? a.q->num_new : 0u;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            a.q->avail_display = a.q->num_avail_total - n;
+            a.q->cache_size = a.q->cache_size_next;
+        }
+    }
     const uint32_t waves_total = gridDim.x * (kZfpThreads / 64);
     for (uint32_t chunk = blockIdx.x * (kZfpThreads / 64) + wave; (uint64_t)chunk * 64 < n;
          chunk += waves_total) {
@@ -285,6 +299,22 @@ __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
                     continue;
                 dst = (uint64_t)(uint32_t)slot * 64;
             }
+            if constexpr (MODE == kModeLruAssign) {
+                // lru_cache_update.comp:24-36: evict the slot's previous item,
+                // install this one (new items are never cached ones, so no two
+                // threads write the same item entry)
+                const uint32_t s = a.order[i];
+                if (s >= a.capacity || blk >= a.total)
+                    continue;  // cannot happen for a consistent state
+                const int32_t prev = a.slot_item[s];
+                if (prev >= 0 && (uint32_t)prev < a.total)
+                    a.item_slot_w[prev] = -1;
+                a.slot_age[s] = 0;
+                a.slot_item[s] = (int32_t)blk;
+                a.slot_avail[s] = 0;
+                a.item_slot_w[blk] = (int32_t)s;
+                dst = (uint64_t)s * 64;
+            }
             float4 *o = reinterpret_cast<float4 *>(a.out + dst);
 #pragma unroll
             for (int j = 0; j < 16; ++j)
@@ -328,6 +358,39 @@ void launch_block_ranges(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords,
         launch_mode<kModeRanges>(a, grid, ctx->stream);
 }
 
+// Per block: [min lo, max hi] over its in-bounds 3x3x3 neighbourhood.
+__global__ void dilate_ranges_kernel(const float2 *ranges, uint32_t nbx, uint32_t nby, uint32_t nbz,
+                                     float2 *out) {
+    const uint64_t n = (uint64_t)nbx * nby * nbz;
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n;
+         b += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = (uint32_t)(b % nbx), y = (uint32_t)((b / nbx) % nby), z = (uint32_t)(b / ((uint64_t)nbx * nby));
+        float lo = ranges[b].x, hi = ranges[b].y;
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int yy = (int)y + dy, zz = (int)z + dz;
+                if (yy < 0 || zz < 0 || yy >= (int)nby || zz >= (int)nbz)
+                    continue;
+                const float2 *row = ranges + (uint64_t)nbx * ((uint32_t)yy + (uint64_t)nby * (uint32_t)zz);
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int xx = (int)x + dx;
+                    if (xx < 0 || xx >= (int)nbx)
+                        continue;
+                    const float2 v = row[xx];
+                    lo = fminf(lo, v.x);
+                    hi = fmaxf(hi, v.y);
+                }
+            }
+        out[b] = make_float2(lo, hi);
+    }
+}
+
+void launch_dilate_ranges(bcmc_ctx *ctx, const float2 *ranges, const uint32_t nb[3], float2 *out) {
+    const uint64_t n = (uint64_t)nb[0] * nb[1] * nb[2];
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(n, 256), persistent_grid(ctx, 8)));
+    dilate_ranges_kernel<<<grid, 256, 0, ctx->stream>>>(ranges, nb[0], nb[1], nb[2], out);
+}
+
 void launch_decode_to_cache(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords,
                             uint32_t maxbits, const uint32_t *ids, const uint32_t *n_dev,
                             uint32_t n_max, const int32_t *item_slot, float *cache,
@@ -342,6 +405,27 @@ void launch_decode_to_cache(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwor
     const uint32_t grid = std::min(div_up(n_max, kZfpThreads), persistent_grid(ctx, 8));
     if (grid)
         launch_mode<kModeCache>(a, grid, ctx->stream);
+}
+
+void launch_decode_assign(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords, uint32_t maxbits,
+                          const uint32_t *new_items, uint32_t n_max, const uint32_t *order,
+                          uint32_t *slot_age, uint32_t *slot_avail, int32_t *slot_item,
+                          int32_t *item_slot, uint32_t total, float *cache, uint32_t capacity,
+                          QState *q) {
+    ZfpArgs a = base_args(stream, nwords, maxbits);
+    a.ids = new_items;
+    a.n_host = n_max;
+    a.order = order;
+    a.slot_age = slot_age;
+    a.slot_avail = slot_avail;
+    a.slot_item = slot_item;
+    a.item_slot_w = item_slot;
+    a.total = total;
+    a.capacity = capacity;
+    a.out = cache;
+    a.q = q;
+    const uint32_t grid = std::max(1u, std::min(div_up(n_max, kZfpThreads), persistent_grid(ctx, 8)));
+    launch_mode<kModeLruAssign>(a, grid, ctx->stream);
 }
 
 void launch_decode_dense(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords, uint32_t maxbits,
