@@ -199,7 +199,7 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy kernel of this size (HBM counter calibration)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--served-queries", type=int, default=512)
+    ap.add_argument("--served-queries", type=int, default=2048)
     ap.add_argument("--served-batch", type=int, default=64)
     ap.add_argument("--served-pool-gib", type=int, default=32)
     ap.add_argument("--served-rates", type=float, nargs="*", default=[100, 300, 1000, 2000])

@@ -6,9 +6,13 @@
 // pool shared by all of them.  A step answers many (volume, isovalue) queries
 // at once; every stage is one launch whose persistent workgroups pull work
 // from lists that concatenate all queries of the step:
-//   code     per block of every query: the active-test code byte
-//   active   single-pass scan + compaction over all queries' blocks ->
-//            (query, block) pairs in query order, ascending blocks
+//   rows     per query and row segment of kSegBlocks blocks: does the
+//            isovalue lie in the segment's union of dilated block ranges?
+//            single-pass scan + compaction -> candidate segments
+//   active   over the blocks of the candidate segments: the active test
+//            (one comparison against the block's dilated range), scan +
+//            compaction -> (query, block) pairs in query order, ascending
+//            blocks
 //   mark     pairs whose block is pooled stamp its slot with the step epoch;
 //            missing blocks are claimed once per volume (CAS) and listed
 //   alloc    misses take free pool slots, then the least recently stamped
