@@ -385,10 +385,25 @@ __global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, Eng
         cs.bid[threadIdx.x] = in ? a.act_b[k0 + threadIdx.x] : 0u;
         cs.aux[threadIdx.x] = in ? a.act_q[k0 + threadIdx.x] : 0u;
         __syncthreads();
+        // a chunk inside one query (the common case) uses workgroup-uniform
+        // volume views; chunks across query boundaries use per-lane views
+        const uint32_t q_first = (uint32_t)__builtin_amdgcn_readfirstlane((int)cs.aux[0]);
+        const uint32_t q_last = (uint32_t)__builtin_amdgcn_readfirstlane((int)cs.aux[min(n - k0, kChunk) - 1]);
+        RowBlock ub;
+        eng_row_block(a, q_first, 0, true, epoch, ub);
+        auto setup_one = [&](uint32_t e, RowBlock &b) {
+            b = ub;
+            b.live = k0 + e < n;
+            b.bid = cs.bid[e];
+        };
         auto setup = [&](uint32_t e, RowBlock &b) {
             eng_row_block(a, cs.aux[e], cs.bid[e], k0 + e < n, epoch, b);
         };
-        count_sweep(cs, wave, tab, setup);
+        const bool one_query = q_first == q_last;
+        if (one_query)
+            count_sweep(cs, wave, tab, setup_one);
+        else
+            count_sweep(cs, wave, tab, setup);
         __syncthreads();
         const uint32_t c = cs.cnt[threadIdx.x];
         uint32_t agg_h, agg_v;
@@ -410,7 +425,10 @@ __global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, Eng
         if (tile == ntiles - 1 && threadIdx.x == 0)
             a.c->num_verts = pv + agg_v;
         __syncthreads();
-        emit_sweep(cs, wave, agg_h, pv, tab, es[wave], a.vert_cap, a.verts, a.fmt, setup);
+        if (one_query)
+            emit_sweep(cs, wave, agg_h, pv, tab, es[wave], a.vert_cap, a.verts, a.fmt, setup_one);
+        else
+            emit_sweep(cs, wave, agg_h, pv, tab, es[wave], a.vert_cap, a.verts, a.fmt, setup);
         __syncthreads();
     }
     scan_site_finish(s0, e0);
