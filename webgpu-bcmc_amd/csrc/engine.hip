@@ -44,6 +44,7 @@ struct EngVol {                 // device descriptor of a volume
     uint32_t n;
     const float2 *dilated;      // per block: range over its 3x3x3 neighbourhood (block_active)
     const float2 *seg_range;    // per row segment: union of its blocks' dilated ranges
+    const float2 *fwd;          // per block: forward range (launch_forward_ranges)
     FastDiv div_x, div_y;       // by nb[0], nb[1]
     int32_t *slot_map;          // block -> pool slot; -1 none, -2 claimed this step
     uint32_t *miss_list;        // this step's missing blocks of the volume (decoder input)
@@ -382,9 +383,15 @@ __global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, Eng
             break;
         const uint32_t k0 = tile * kChunk;
         const bool in = k0 + threadIdx.x < n;
-        cs.bid[threadIdx.x] = in ? a.act_b[k0 + threadIdx.x] : 0u;
-        cs.aux[threadIdx.x] = in ? a.act_q[k0 + threadIdx.x] : 0u;
-        __syncthreads();
+        const uint32_t bid = in ? a.act_b[k0 + threadIdx.x] : 0u, qi = in ? a.act_q[k0 + threadIdx.x] : 0u;
+        float2 fwd = make_float2(1.0f, 0.0f);
+        float iso = 0.0f;
+        if (in) {
+            const EngQuery &q = a.qs[qi];
+            iso = q.iso;
+            fwd = a.vols[q.vol].fwd[bid];
+        }
+        const uint32_t ncand = chunk_prologue(cs, in, bid, qi, fwd, iso, lds_w);
         // a chunk inside one query (the common case) uses workgroup-uniform
         // volume views; chunks across query boundaries use per-lane views
         const uint32_t q_first = (uint32_t)__builtin_amdgcn_readfirstlane((int)cs.aux[0]);
@@ -401,9 +408,9 @@ __global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, Eng
         };
         const bool one_query = q_first == q_last;
         if (one_query)
-            count_sweep(cs, wave, tab, setup_one);
+            count_sweep(cs, ncand, wave, tab, setup_one);
         else
-            count_sweep(cs, wave, tab, setup);
+            count_sweep(cs, ncand, wave, tab, setup);
         __syncthreads();
         const uint32_t c = cs.cnt[threadIdx.x];
         uint32_t agg_h, agg_v;
@@ -449,7 +456,7 @@ struct EngVolume {
     uint32_t dims[3], nb[3], n, maxbits;
     uint64_t *stream = nullptr;
     uint64_t nwords = 0;
-    float2 *ranges = nullptr, *dilated = nullptr, *seg_range = nullptr;
+    float2 *ranges = nullptr, *dilated = nullptr, *seg_range = nullptr, *fwd = nullptr;
     int32_t *slot_map = nullptr;
     uint32_t *miss_list = nullptr;  // per-step block ids to decode (EngVol::miss_list)
 };
@@ -628,6 +635,7 @@ BCMC_API int bcmc_engine_destroy(bcmc_engine *e) {
         dev_free(v.ranges);
         dev_free(v.seg_range);
         dev_free(v.dilated);
+        dev_free(v.fwd);
         dev_free(v.slot_map);
         dev_free(v.miss_list);
     }
@@ -688,7 +696,11 @@ BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t
     BCMC_TRY(dev_alloc((void **)&v.slot_map, n * 4));
     BCMC_TRY(dev_alloc((void **)&v.miss_list, n * 4));
     BCMC_HIP_CHECK(hipMemsetAsync(v.slot_map, 0xFF, n * 4, ctx->stream));
-    launch_block_ranges(ctx, v.stream, v.nwords, v.maxbits, v.dims, v.ranges);
+    float2 *full = nullptr;
+    BCMC_TRY(dev_alloc((void **)&full, n * 8));
+    BCMC_TRY(dev_alloc((void **)&v.fwd, n * 8));
+    launch_block_ranges(ctx, v.stream, v.nwords, v.maxbits, v.dims, v.ranges, full);
+    launch_forward_ranges(ctx, full, v.nb, v.fwd);
     const uint32_t nsegx = div_up(v.nb[0], kSegBlocks), nsegs = nsegx * v.nb[1] * v.nb[2];
     BCMC_TRY(dev_alloc((void **)&v.dilated, n * 8));
     BCMC_TRY(dev_alloc((void **)&v.seg_range, (size_t)nsegs * 8));
@@ -697,7 +709,8 @@ BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t
                                                                          v.seg_range);
     BCMC_HIP_CHECK(hipGetLastError());
     BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    dev_free(v.ranges);  // only the dilated ranges are used from here on
+    dev_free(v.ranges);  // only the dilated and forward ranges are used from here on
+    dev_free(full);
     v.ranges = nullptr;
     EngVol dv;
     for (int i = 0; i < 3; ++i)
@@ -707,6 +720,7 @@ BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t
     dv.div_y = make_fast_div(v.nb[1]);
     dv.dilated = v.dilated;
     dv.seg_range = v.seg_range;
+    dv.fwd = v.fwd;
     dv.slot_map = v.slot_map;
     dv.miss_list = v.miss_list;
     const uint32_t id = (uint32_t)e->vols.size();

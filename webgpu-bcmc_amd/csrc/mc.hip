@@ -76,6 +76,7 @@ struct ActiveOp {
 // not written; the host grows the buffer and runs the pass again when the
 // total exceeds the capacity.
 struct FusedArgs {
+    const float2 *fwd;  // per block: forward range (launch_forward_ranges)
     const float *cache;
     const int32_t *item_slot;
     const uint32_t *flags;
@@ -125,15 +126,17 @@ __global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanS
         if (tile >= ntiles)
             break;
         const uint32_t k0 = tile * kChunk;
-        cs.bid[threadIdx.x] = k0 + threadIdx.x < n ? a.active_ids[k0 + threadIdx.x] : 0u;
-        __syncthreads();
+        const bool in = k0 + threadIdx.x < n;
+        const uint32_t bid = in ? a.active_ids[k0 + threadIdx.x] : 0u;
+        const uint32_t ncand = chunk_prologue(cs, in, bid, 0u, in ? a.fwd[bid] : make_float2(1.0f, 0.0f),
+                                              base.iso, lds_w);
         auto setup = [&](uint32_t e, RowBlock &b) {
             b = base;
             b.live = k0 + e < n;
             b.bid = cs.bid[e];
         };
         // sweep 1: per-block vertex counts
-        count_sweep(cs, wave, tab, setup);
+        count_sweep(cs, ncand, wave, tab, setup);
         __syncthreads();
         // chunk scan of (has vertices, vertices), then the two look-backs
         const uint32_t c = cs.cnt[threadIdx.x];
@@ -183,6 +186,7 @@ struct bcmc_mc {
     float2 *ranges = nullptr;
     uint32_t *flags = nullptr, *active_ids = nullptr, *with_verts = nullptr;
     float2 *dilated = nullptr;  // per block: range over its 3x3x3 neighbourhood
+    float2 *fwd = nullptr;      // per block: range of every voxel its cells can read
     unsigned long long *vert_offsets = nullptr;
     bcmc_lru lru;  // non-owning handle over the cache state held here
     QState *q = nullptr, *qh = nullptr;
@@ -204,6 +208,7 @@ static void mc_free_volume(bcmc_mc *m) {
     dev_free(m->flags);
     dev_free(m->active_ids);
     dev_free(m->dilated);
+    dev_free(m->fwd);
     dev_free(m->with_verts);
     dev_free(m->vert_offsets);
     dev_free(m->verts);
@@ -215,6 +220,7 @@ static void mc_free_volume(bcmc_mc *m) {
     m->ranges = nullptr;
     m->flags = m->active_ids = m->with_verts = nullptr;
     m->dilated = nullptr;
+    m->fwd = nullptr;
     m->vert_offsets = nullptr;
     m->verts = nullptr;
     m->vert_cap = 0;
@@ -243,6 +249,7 @@ static void mc_refresh_views(bcmc_mc *m) {
 
 static int mc_launch_fused(bcmc_mc *m) {
     FusedArgs a;
+    a.fwd = m->fwd;
     a.cache = (const float *)m->lru.st.data;
     a.item_slot = m->lru.st.v.item_slot;
     a.flags = m->flags;
@@ -338,6 +345,7 @@ BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *m, const void *stream, uint6
     BCMC_TRY(dev_alloc((void **)&m->flags, n * 4));
     BCMC_TRY(dev_alloc((void **)&m->active_ids, n * 4));
     BCMC_TRY(dev_alloc((void **)&m->dilated, n * 8));
+    BCMC_TRY(dev_alloc((void **)&m->fwd, n * 8));
     BCMC_TRY(dev_alloc((void **)&m->with_verts, n * 4));
     BCMC_TRY(dev_alloc((void **)&m->vert_offsets, n * 8));
     BCMC_TRY(scan_site_alloc(&m->site_active, n, kScanTile));
@@ -352,10 +360,14 @@ BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *m, const void *stream, uint6
     *m->qh = qi;
     BCMC_HIP_CHECK(hipMemcpyAsync(m->q, m->qh, sizeof(QState), hipMemcpyHostToDevice, ctx->stream));
     // computeBlockRanges (:522-578): full-volume decode, min/max per block
-    launch_block_ranges(ctx, m->stream, m->stream_words, maxbits, m->dims, m->ranges);
+    float2 *full = nullptr;
+    BCMC_TRY(dev_alloc((void **)&full, n * 8));
+    launch_block_ranges(ctx, m->stream, m->stream_words, maxbits, m->dims, m->ranges, full);
     launch_dilate_ranges(ctx, m->ranges, m->nb, m->dilated);
+    launch_forward_ranges(ctx, full, m->nb, m->fwd);
     BCMC_HIP_CHECK(hipGetLastError());
     BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    dev_free(full);
     m->has_volume = true;
     m->last_active = m->last_with = m->last_new = 0;
     m->last_verts = 0;

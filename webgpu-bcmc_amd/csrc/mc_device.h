@@ -456,39 +456,77 @@ BCMC_DEV void emit_row_blocks(const RowBlock &b, const RowTile &t, const McTable
 }
 
 // LDS of a workgroup's chunk: block ids (and a caller word, e.g. the query)
-// per entry, the per-block meta words, the per-block vertex counts, and the
-// list of entries with vertices with their vertex offsets in the chunk.
+// per entry, the per-block meta words, the per-block vertex counts, the
+// candidate entries of the count sweep, and the list of entries with vertices
+// with their vertex offsets in the chunk.  Entry kChunk is a scratch entry for
+// rows without a block.
 struct ChunkShared {
-    uint32_t bid[kChunk];
-    uint32_t aux[kChunk];
-    uint32_t meta[kChunk][kMetaWords];
-    uint32_t cnt[kChunk], list[kChunk], off[kChunk];
+    uint32_t bid[kChunk + 1];
+    uint32_t aux[kChunk + 1];
+    uint32_t meta[kChunk + 1][kMetaWords];
+    uint32_t cnt[kChunk], off[kChunk];
+    uint32_t list[kChunk];  // count sweep: the candidate entries; afterwards: entries with vertices
 };
 
-// Count sweep of wave w over chunk entries [64w, 64w + 64), 4 per round; the
-// next round's slot/flag loads are issued before this round's voxels are
-// used.  setup(e, b) fills the RowBlock of entry e.
+// Chunk prologue (all threads): entry t of the chunk gets its block id and
+// caller word; it is a count candidate when the isovalue lies in
+// [lo, hi) of its forward range (launch_forward_ranges): with iso < lo every
+// corner test fails and with iso >= hi every one passes, so the block's cells
+// have case 0 or 255 and no vertices.  Returns the number of candidates
+// (listed in cs.list, ascending).
+BCMC_DEV uint32_t chunk_prologue(ChunkShared &cs, bool in, uint32_t bid, uint32_t aux, float2 fwd, float iso,
+                                 uint32_t *lds_waves) {
+    const uint32_t t = threadIdx.x;
+    cs.bid[t] = bid;
+    cs.aux[t] = aux;
+    cs.cnt[t] = 0;
+    if (t == 0) {
+        cs.bid[kChunk] = 0;
+        cs.aux[kChunk] = 0;
+    }
+    const bool cand = in && iso >= fwd.x && iso < fwd.y;
+    uint32_t ncand;
+    const uint32_t pos = block_excl_scan<kMcThreads>(cand ? 1u : 0u, lds_waves, &ncand);
+    if (cand)
+        cs.list[pos] = t;
+    __syncthreads();
+    return ncand;
+}
+
+// Count sweep over the candidate entries: wave w takes candidates
+// 4w + 16q + row (q = 0, 1, ...); the next round's slot/flag loads are issued
+// before this round's voxels are used.  setup(e, b) fills the RowBlock of
+// entry e (e = kChunk: no block).
 template <class Setup>
-BCMC_DEV void count_sweep(ChunkShared &cs, uint32_t wave, const McTables &tab, Setup &&setup) {
-    constexpr uint32_t kRounds = 64 / kRowBlocks;
+BCMC_DEV void count_sweep(ChunkShared &cs, uint32_t ncand, uint32_t wave, const McTables &tab, Setup &&setup) {
+    constexpr uint32_t kRound = (kMcThreads / 64) * kRowBlocks;
     const uint32_t lane = lane_id(), row = lane >> 4;
+    const uint32_t first = wave * kRowBlocks;
+    if (first >= ncand)
+        return;
+    auto setup_c = [&](uint32_t c, uint32_t &e, RowBlock &b) {
+        e = c < ncand ? cs.list[c] : kChunk;
+        setup(e, b);
+        b.live = b.live && e < kChunk;
+    };
     RowBlock b;
-    setup(64 * wave + row, b);
+    uint32_t e;
+    setup_c(first + row, e, b);
     RowMeta m = row_meta_load(b);
-    for (uint32_t q = 0; q < kRounds; ++q) {
-        const uint32_t e = 64 * wave + kRowBlocks * q + row;
+    for (uint32_t base = first; base < ncand; base += kRound) {
         const RowBlock bq = b;
-        row_meta_store(m, bq.tag, tab, cs.meta[e]);
+        const uint32_t eq = e;
+        row_meta_store(m, bq.tag, tab, cs.meta[eq]);
         wave_lds_sync();
         RowTile t;
-        row_voxels(bq, cs.meta[e], t);
-        if (q + 1 < kRounds) {
-            setup(e + kRowBlocks, b);
+        row_voxels(bq, cs.meta[eq], t);
+        if (base + kRound < ncand) {
+            setup_c(base + kRound + row, e, b);
             m = row_meta_load(b);
         }
         const uint32_t c = row_block_count(t, bq.iso, tab);
-        if ((lane & 15u) == 15u)
-            cs.cnt[e] = c;
+        if ((lane & 15u) == 15u && eq < kChunk)
+            cs.cnt[eq] = c;
     }
 }
 

@@ -212,7 +212,8 @@ struct ZfpArgs {
     const int32_t *item_slot;  // cache mode: destination slot per block id
     uint32_t capacity;         // cache mode: slots in `out` (others are skipped)
     float *out;                // cache / dense output
-    float2 *ranges;            // ranges mode
+    float2 *ranges;            // ranges mode: range over the in-volume voxels
+    float2 *full_ranges;       // ranges mode (nullable): range over all 64 decoded voxels
     uint32_t dims[3], nb[3];
     // LRU-assign mode: new item w takes slot order[w] (lru_cache_update.comp)
     const uint32_t *order;
@@ -281,7 +282,7 @@ __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
             const uint32_t sx = bx + 4 > a.dims[0] ? a.dims[0] - bx : 4;
             const uint32_t sy = by + 4 > a.dims[1] ? a.dims[1] - by : 4;
             const uint32_t sz = bz + 4 > a.dims[2] ? a.dims[2] - bz : 4;
-            float lo = 1e20f, hi = -1e20f;
+            float lo = 1e20f, hi = -1e20f, alo = v[0], ahi = v[0];
 #pragma unroll
             for (int j = 0; j < 64; ++j) {
                 const bool in = (uint32_t)(j & 3) < sx && (uint32_t)((j >> 2) & 3) < sy &&
@@ -289,8 +290,12 @@ __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
                 const float x = v[j];
                 lo = (in && x < lo) ? x : lo;
                 hi = (in && x > hi) ? x : hi;
+                alo = fminf(alo, x);
+                ahi = fmaxf(ahi, x);
             }
             a.ranges[blk] = make_float2(lo, hi);
+            if (a.full_ranges)
+                a.full_ranges[blk] = make_float2(alo, ahi);
         } else {
             uint64_t dst = (uint64_t)i * 64;
             if constexpr (MODE == kModeCache) {
@@ -345,7 +350,7 @@ static ZfpArgs base_args(const uint64_t *stream, uint64_t nwords, uint32_t maxbi
 }
 
 void launch_block_ranges(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords, uint32_t maxbits,
-                         const uint32_t dims[3], float2 *ranges) {
+                         const uint32_t dims[3], float2 *ranges, float2 *full_ranges) {
     ZfpArgs a = base_args(stream, nwords, maxbits);
     for (int i = 0; i < 3; ++i) {
         a.dims[i] = dims[i];
@@ -353,6 +358,7 @@ void launch_block_ranges(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords,
     }
     a.n_host = a.nb[0] * a.nb[1] * a.nb[2];
     a.ranges = ranges;
+    a.full_ranges = full_ranges;
     const uint32_t grid = div_up(a.n_host, kZfpThreads);
     if (grid)
         launch_mode<kModeRanges>(a, grid, ctx->stream);
@@ -383,6 +389,35 @@ __global__ void dilate_ranges_kernel(const float2 *ranges, uint32_t nbx, uint32_
             }
         out[b] = make_float2(lo, hi);
     }
+}
+
+// Per block: range over all decoded voxels (padding included) of the block and
+// of its in-grid +x/+y/+z, edge and corner neighbours: every voxel a cell of
+// the block can read (compute_vertex_values.comp:96-252).
+__global__ void forward_ranges_kernel(const float2 *full, uint32_t nbx, uint32_t nby, uint32_t nbz,
+                                      float2 *out) {
+    const uint64_t n = (uint64_t)nbx * nby * nbz;
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n;
+         b += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = (uint32_t)(b % nbx), y = (uint32_t)((b / nbx) % nby),
+                       z = (uint32_t)(b / ((uint64_t)nbx * nby));
+        float lo = full[b].x, hi = full[b].y;
+        for (uint32_t s = 1; s < 8; ++s) {
+            const uint32_t xx = x + (s & 1u), yy = y + ((s >> 1) & 1u), zz = z + (s >> 2);
+            if (xx >= nbx || yy >= nby || zz >= nbz)
+                continue;
+            const float2 v = full[xx + (uint64_t)nbx * (yy + (uint64_t)nby * zz)];
+            lo = fminf(lo, v.x);
+            hi = fmaxf(hi, v.y);
+        }
+        out[b] = make_float2(lo, hi);
+    }
+}
+
+void launch_forward_ranges(bcmc_ctx *ctx, const float2 *full_ranges, const uint32_t nb[3], float2 *out) {
+    const uint64_t n = (uint64_t)nb[0] * nb[1] * nb[2];
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(n, 256), persistent_grid(ctx, 8)));
+    forward_ranges_kernel<<<grid, 256, 0, ctx->stream>>>(full_ranges, nb[0], nb[1], nb[2], out);
 }
 
 void launch_dilate_ranges(bcmc_ctx *ctx, const float2 *ranges, const uint32_t nb[3], float2 *out) {
