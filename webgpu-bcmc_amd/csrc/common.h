@@ -198,53 +198,87 @@ BCMC_DEV unsigned long long wave_sum_u64(unsigned long long v) {
 // aggregates) the window moves back 64 tiles.  Returns the exclusive prefix
 // of the tile (status values are exact below 2^40) and publishes the tile's
 // inclusive value.
+// The look-back walk of one channel, run by one whole wave: returns the
+// exclusive prefix of `tile` and publishes its inclusive value.
+BCMC_DEV unsigned long long lookback_wave(const ScanSite &site, uint32_t tile, uint32_t epoch,
+                                          uint32_t aggregate) {
+    const uint32_t lane = lane_id();
+    const uint32_t ep = epoch & 0x3FFFFF;
+    unsigned long long prefix = 0;
+    if (tile == 0) {
+        if (lane == 0)
+            st_agent_u64(&site.status[0], pack_status(ep, kFlagInclusive, aggregate));
+        return 0;
+    }
+    if (lane == 0)
+        st_agent_u64(&site.status[tile], pack_status(ep, kFlagAggregate, aggregate));
+    int64_t base = (int64_t)tile - 1;
+    for (;;) {
+        const int64_t t = base - (int64_t)lane;
+        unsigned long long s = 0;
+        uint32_t flag = kFlagInclusive;  // before tile 0: an inclusive zero
+        if (t >= 0) {
+            s = ld_agent_u64(&site.status[t]);
+            flag = ((uint32_t)(s >> 42) == ep) ? (uint32_t)(s >> 40) & 3u : 0u;
+        }
+        const uint64_t incl = __ballot(flag == kFlagInclusive);
+        const uint64_t wait = __ballot(flag == 0);
+        const uint32_t first = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
+        const uint64_t need = first == 64 ? ~0ull : ((2ull << first) - 1ull);
+        if (wait & need) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        const unsigned long long v = (lane <= first && t >= 0) ? (s & kValueMask) : 0ull;
+        prefix += wave_sum_u64(v);
+        if (first < 64)
+            break;
+        base -= 64;
+    }
+    if (lane == 0)
+        st_agent_u64(&site.status[tile], pack_status(ep, kFlagInclusive, prefix + aggregate));
+    return prefix;
+}
+
+// Decoupled look-back across tiles, done by wave 0 of the workgroup.  Each
+// round reads the statuses of the 64 nearest unvisited predecessors at once
+// (one agent-scope load per lane): if every predecessor up to and including
+// the closest inclusive one is ready, their values are summed and the walk
+// ends at that inclusive one; with no inclusive status in the window (all
+// aggregates) the window moves back 64 tiles.  Returns the exclusive prefix
+// of the tile (status values are exact below 2^40) and publishes the tile's
+// inclusive value.
 template <int NT>
 BCMC_DEV unsigned long long tile_lookback(const ScanSite &site, uint32_t tile, uint32_t epoch,
                                           uint32_t aggregate, unsigned long long *lds_prefix) {
     if (threadIdx.x < 64) {
-        const uint32_t lane = lane_id();
-        const uint32_t ep = epoch & 0x3FFFFF;
-        unsigned long long prefix = 0;
-        if (tile == 0) {
-            if (lane == 0)
-                st_agent_u64(&site.status[0], pack_status(ep, kFlagInclusive, aggregate));
-        } else {
-            if (lane == 0)
-                st_agent_u64(&site.status[tile], pack_status(ep, kFlagAggregate, aggregate));
-            int64_t base = (int64_t)tile - 1;
-            for (;;) {
-                const int64_t t = base - (int64_t)lane;
-                unsigned long long s = 0;
-                uint32_t flag = kFlagInclusive;  // before tile 0: an inclusive zero
-                if (t >= 0) {
-                    s = ld_agent_u64(&site.status[t]);
-                    flag = ((uint32_t)(s >> 42) == ep) ? (uint32_t)(s >> 40) & 3u : 0u;
-                }
-                const uint64_t incl = __ballot(flag == kFlagInclusive);
-                const uint64_t wait = __ballot(flag == 0);
-                const uint32_t first = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
-                const uint64_t need = first == 64 ? ~0ull : ((2ull << first) - 1ull);
-                if (wait & need) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                const unsigned long long v = (lane <= first && t >= 0) ? (s & kValueMask) : 0ull;
-                prefix += wave_sum_u64(v);
-                if (first < 64)
-                    break;
-                base -= 64;
-            }
-            if (lane == 0)
-                st_agent_u64(&site.status[tile],
-                               pack_status(ep, kFlagInclusive, prefix + aggregate));
-        }
-        if (lane == 0)
-            *lds_prefix = prefix;
+        const unsigned long long p = lookback_wave(site, tile, epoch, aggregate);
+        if (threadIdx.x == 0)
+            *lds_prefix = p;
     }
     __syncthreads();
     const unsigned long long p = *lds_prefix;
     __syncthreads();
     return p;
+}
+
+// Two channels at once: wave 0 walks channel 0, wave 1 channel 1.
+template <int NT>
+BCMC_DEV void tile_lookback2(const ScanSite &s0, const ScanSite &s1, uint32_t tile, uint32_t e0, uint32_t e1,
+                             uint32_t agg0, uint32_t agg1, unsigned long long *lds_prefix /* [2] */,
+                             unsigned long long &p0, unsigned long long &p1) {
+    static_assert(NT >= 128, "two look-back waves");
+    const uint32_t wave = threadIdx.x / 64;
+    if (wave < 2) {
+        const unsigned long long p = wave == 0 ? lookback_wave(s0, tile, e0, agg0)
+                                               : lookback_wave(s1, tile, e1, agg1);
+        if ((threadIdx.x & 63u) == 0)
+            lds_prefix[wave] = p;
+    }
+    __syncthreads();
+    p0 = lds_prefix[0];
+    p1 = lds_prefix[1];
+    __syncthreads();
 }
 
 // Dynamic tile index for persistent workgroups (broadcast through LDS).

@@ -113,9 +113,12 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanSite s0, ScanSit
             }
         }
         __syncthreads();
-#pragma unroll
-        for (int c = 0; c < NCH; ++c)
-            prefix[c] = tile_lookback<kScanThreads>(*sites[c], tile, epoch[c], agg[c], &lds_prefix[c]);
+        if constexpr (NCH == 2) {
+            tile_lookback2<kScanThreads>(s0, s1, tile, epoch[0], epoch[1], agg[0], agg[1], lds_prefix,
+                                         prefix[0], prefix[1]);
+        } else {
+            prefix[0] = tile_lookback<kScanThreads>(s0, tile, epoch[0], agg[0], &lds_prefix[0]);
+        }
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const uint32_t i = wbase + j * 64 + lane;

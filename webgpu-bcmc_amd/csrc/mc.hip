@@ -143,8 +143,8 @@ __global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanS
         uint32_t agg_h, agg_v;
         const uint32_t ex_h = block_excl_scan<kMcThreads>(c ? 1u : 0u, lds_w, &agg_h);
         const uint32_t ex_v = block_excl_scan<kMcThreads>(c, lds_w, &agg_v);
-        const unsigned long long ph = tile_lookback<kMcThreads>(s0, tile, e0, agg_h, &lds_prefix[0]);
-        const unsigned long long pv = tile_lookback<kMcThreads>(s1, tile, e1, agg_v, &lds_prefix[1]);
+        unsigned long long ph, pv;
+        tile_lookback2<kMcThreads>(s0, s1, tile, e0, e1, agg_h, agg_v, lds_prefix, ph, pv);
         if (c) {
             cs.list[ex_h] = threadIdx.x;
             cs.off[ex_h] = ex_v;
