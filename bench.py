@@ -243,11 +243,11 @@ def main():
     mc = bcmc_amd.CompressedMarchingCubes(dev)
     mc.setCompressedVolume(stream, rate, dims, (1, 1, 1), cache0)
 
-    def step(records):
+    def step(records, timed=False):
         mc.lruCache.reset()
         for iso in isos:
             t = time.perf_counter()
-            nv = mc.computeSurface(float(iso))
+            nv = mc.computeSurface(float(iso), timed=timed)
             dt = time.perf_counter() - t
             if records is not None:
                 rec = dict(mc.lastPerf)
@@ -270,6 +270,12 @@ def main():
     queries = args.steps * len(isos)
     qps = queries / elapsed
     lat = np.array([r["wall_ms"] for r in records])
+    # per-stage device times: the same steps again, timed with events (direct
+    # launches; the timed region above replays the captured query graph)
+    records = []
+    for i in range(args.steps):
+        step(records, timed=True)
+    dev.sync()
 
     # per-stage device time and algorithmic bytes over the timed queries
     stages = ["computeActiveBlocks", "cacheUpdate", "decompression", "computeVertices"]

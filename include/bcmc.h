@@ -171,7 +171,10 @@ BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *mc, const void *stream, uint
                                            const float scale[3], uint32_t initial_cache_slots);
 BCMC_API int bcmc_mc_set_vertex_format(bcmc_mc *mc, bcmc_vertex_format fmt);
 /* computeSurface(isovalue, perfTracker) (compressed_marching_cubes.js:580-809).
- * The isovalue is used as FP32 (:613-619).  perf may be NULL. */
+ * The isovalue is used as FP32 (:613-619).  perf may be NULL: the query chain
+ * is then replayed from a HIP graph captured on first use (and captured again
+ * when its buffers change); with perf the stages are launched directly and
+ * timed with events.  Results are identical either way. */
 BCMC_API int bcmc_mc_compute_surface(bcmc_mc *mc, float isovalue, bcmc_perf *perf,
                                      uint64_t *num_vertices);
 /* Public fields of the reference object. */

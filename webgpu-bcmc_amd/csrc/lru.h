@@ -36,7 +36,12 @@ void lru_state_free(LruState *L);
 // nothing when q->num_active == 0 (compressed_marching_cubes.js:630-641).  Ends
 // with the eviction order in v.order; the slot assignment is done by the
 // decoder (launch_decode_assign), which decodes each new item into its slot.
-void lru_enqueue_update_mc(bcmc_ctx *ctx, LruState *L, const uint32_t *flags, QState *q);
+// npasses = lru_sort_passes(L) at enqueue time; the caller adds 1 to
+// L->age_bound per update.
+void lru_enqueue_update_mc(bcmc_ctx *ctx, LruState *L, const uint32_t *flags, QState *q,
+                           uint32_t npasses);
+// Radix passes for the eviction order given the current age bound.
+uint32_t lru_sort_passes(const LruState *L);
 // LRUCache.update for arbitrary 0/1 `needed` flags over all items.
 void lru_enqueue_update_all(bcmc_ctx *ctx, LruState *L, const uint32_t *needed, QState *q);
 void lru_enqueue_reset(bcmc_ctx *ctx, LruState *L);

@@ -201,7 +201,7 @@ void lru_state_free(LruState *L) {
 // below the tracked bound (initial ages 2+s, grown and reset slots 100000, +1
 // per update), so the 8-bit digits above the bound's are all 0xFF and their
 // passes would keep the order.
-static uint32_t lru_sort_passes(const LruState *L) {
+uint32_t lru_sort_passes(const LruState *L) {
     uint32_t bits = 0;
     while (bits < 32 && (L->age_bound >> bits) != 0)
         ++bits;
@@ -211,24 +211,25 @@ static uint32_t lru_sort_passes(const LruState *L) {
 // Age/mark + available list, growth and eviction order, once q->num_new and
 // the new items are on the device.
 static void enqueue_tail(bcmc_ctx *ctx, LruState *L, const uint32_t *needed, QState *q,
-                         const uint32_t *gate) {
+                         const uint32_t *gate, uint32_t npasses) {
     hipStream_t st = ctx->stream;
     launch_scan<1>(ctx, L->site_avail, L->site_avail, AgeAvailOp{L->v, needed, q, gate}, L->capacity);
     lru_grow_kernel<<<grid_for(ctx, L->capacity, 2), 256, 0, st>>>(L->v, q, gate);
     launch_lru_order(ctx, &L->sort, L->v.avail_ids, L->v.age, &q->num_avail_total, L->capacity,
-                     L->v.order, lru_sort_passes(L));
-    L->age_bound += 1;
+                     L->v.order, npasses);
 }
 
-void lru_enqueue_update_mc(bcmc_ctx *ctx, LruState *L, const uint32_t *flags, QState *q) {
-    enqueue_tail(ctx, L, flags, q, &q->num_active);
+void lru_enqueue_update_mc(bcmc_ctx *ctx, LruState *L, const uint32_t *flags, QState *q,
+                           uint32_t npasses) {
+    enqueue_tail(ctx, L, flags, q, &q->num_active, npasses);
 }
 
 void lru_enqueue_update_all(bcmc_ctx *ctx, LruState *L, const uint32_t *needed, QState *q) {
     launch_scan<1>(ctx, L->site_miss, L->site_miss, MissAllOp{L->v, needed, L->new_items, q},
                    L->aligned_total);
-    enqueue_tail(ctx, L, needed, q, nullptr);
+    enqueue_tail(ctx, L, needed, q, nullptr, lru_sort_passes(L));
     lru_assign_kernel<<<grid_for(ctx, L->capacity, 2), 256, 0, ctx->stream>>>(L->v, L->new_items, q);
+    L->age_bound += 1;
 }
 
 void lru_enqueue_reset(bcmc_ctx *ctx, LruState *L) {

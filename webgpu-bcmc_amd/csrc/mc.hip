@@ -38,18 +38,16 @@ struct ActiveOp {
     const float2 *dilated;
     const int32_t *item_slot;
     uint32_t *flags, *active_ids, *new_items;
-    QState *q;
-    float iso;
+    QState *q;  // q->iso holds the query's isovalue (copied in before the scan)
     uint32_t n;
     BCMC_DEV uint32_t size() const { return n; }
     BCMC_DEV void begin() const {
-        q->iso = iso;
         q->num_with = 0;
         q->num_verts = 0;
         q->overflow = 0;
     }
     BCMC_DEV void value(uint32_t b, uint32_t (&v)[2]) const {
-        v[0] = block_active(dilated[b], iso) ? 1u : 0u;
+        v[0] = block_active(dilated[b], q->iso) ? 1u : 0u;
         v[1] = (v[0] && item_slot[b] < 0) ? 1u : 0u;
     }
     BCMC_DEV void emit(uint32_t b, const uint32_t (&v)[2], const unsigned long long (&e)[2]) const {
@@ -198,11 +196,30 @@ struct bcmc_mc {
     hipEvent_t ev[7] = {};
     uint32_t last_active = 0, last_with = 0, last_new = 0;
     uint64_t last_verts = 0;
+    // the query chain is captured once into a HIP graph and replayed; it is
+    // captured again when anything it bakes in (buffers, format, sort passes)
+    // changes.  The isovalue reaches the device through a captured copy from
+    // pinned memory.
+    float *iso_h = nullptr;
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    uint64_t graph_key[4] = {0, 0, 0, 0};
 };
+
+static void mc_drop_graph(bcmc_mc *m) {
+    if (m->gexec)
+        (void)hipGraphExecDestroy(m->gexec);
+    if (m->graph)
+        (void)hipGraphDestroy(m->graph);
+    m->gexec = nullptr;
+    m->graph = nullptr;
+}
 
 static void mc_free_volume(bcmc_mc *m) {
     if (m->ctx)
         (void)hipStreamSynchronize(m->ctx->stream);
+    mc_drop_graph(m);
     dev_free(m->stream);
     dev_free(m->ranges);
     dev_free(m->flags);
@@ -273,6 +290,42 @@ static int mc_launch_fused(bcmc_mc *m) {
     return BCMC_OK;
 }
 
+// One query as a kernel chain on the context's stream (capturable when not
+// timed): the isovalue copy, active scan, LRU update, assignment + decode, MC
+// pass, and the read-back of the counters; `timed` records the perfTracker
+// events between the stages (direct launches only: the elapsed time of events
+// recorded by a graph is not available).
+static int mc_enqueue_query(bcmc_mc *m, uint32_t npasses, bool timed) {
+    bcmc_ctx *ctx = m->ctx;
+    hipStream_t st = ctx->stream;
+    LruState &L = m->lru.st;
+    BCMC_HIP_CHECK(hipMemcpyAsync(&m->q->iso, m->iso_h, sizeof(float), hipMemcpyHostToDevice, st));
+    if (timed)
+        BCMC_HIP_CHECK(hipEventRecord(m->ev[0], st));
+    // active blocks and the LRU misses among them, one scan
+    ActiveOp aop{m->dilated, L.v.item_slot, m->flags, m->active_ids, L.new_items, m->q, m->total};
+    launch_scan<2>(ctx, m->site_active, L.site_miss, aop, m->total);
+    if (timed)
+        BCMC_HIP_CHECK(hipEventRecord(m->ev[1], st));
+    // cache update: age/mark + available slots, growth, eviction order
+    lru_enqueue_update_mc(ctx, &L, m->flags, m->q, npasses);
+    if (timed)
+        BCMC_HIP_CHECK(hipEventRecord(m->ev[2], st));
+    // slot assignment and decode of the misses into their slots
+    launch_decode_assign(ctx, m->stream, m->stream_words, m->maxbits, L.new_items, m->total, L.v.order,
+                         L.v.age, L.v.avail, L.v.item, L.v.item_slot, L.total, (float *)L.data,
+                         L.capacity, m->q);
+    if (timed)
+        BCMC_HIP_CHECK(hipEventRecord(m->ev[3], st));
+    // counts, offsets and vertices in one pass
+    BCMC_TRY(mc_launch_fused(m));
+    if (timed)
+        BCMC_HIP_CHECK(hipEventRecord(m->ev[4], st));
+    BCMC_HIP_CHECK(hipGetLastError());
+    BCMC_HIP_CHECK(hipMemcpyAsync(m->qh, m->q, sizeof(QState), hipMemcpyDeviceToHost, st));
+    return BCMC_OK;
+}
+
 extern "C" {
 
 BCMC_API int bcmc_mc_create(bcmc_ctx *ctx, bcmc_mc **out) {
@@ -280,8 +333,11 @@ BCMC_API int bcmc_mc_create(bcmc_ctx *ctx, bcmc_mc **out) {
     bcmc_mc *m = new bcmc_mc;
     m->ctx = ctx;
     if (dev_alloc((void **)&m->q, sizeof(QState)) != BCMC_OK ||
-        hipHostMalloc(&m->qh, sizeof(QState), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc(&m->qh, sizeof(QState), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&m->iso_h, sizeof(float), hipHostMallocDefault) != hipSuccess) {
         dev_free(m->q);
+        if (m->qh)
+            (void)hipHostFree(m->qh);
         delete m;
         set_error("allocation of the query counters failed");
         return BCMC_ERR_NOMEM;
@@ -292,6 +348,8 @@ BCMC_API int bcmc_mc_create(bcmc_ctx *ctx, bcmc_mc **out) {
     m->lru.q = m->q;
     m->lru.qh = m->qh;
     m->lru.owns = false;
+    const char *ng = std::getenv("BCMC_NO_GRAPH");
+    m->use_graph = !(ng && ng[0] == '1');
     *out = m;
     return BCMC_OK;
 }
@@ -302,6 +360,7 @@ BCMC_API int bcmc_mc_destroy(bcmc_mc *m) {
     mc_free_volume(m);
     dev_free(m->q);
     (void)hipHostFree(m->qh);
+    (void)hipHostFree(m->iso_h);
     for (auto &e : m->ev)
         (void)hipEventDestroy(e);
     delete m;
@@ -401,32 +460,38 @@ BCMC_API int bcmc_mc_compute_surface(bcmc_mc *m, float isovalue, bcmc_perf *perf
     bcmc_ctx *ctx = m->ctx;
     hipStream_t st = ctx->stream;
     const bool timed = perf != nullptr;
-    if (timed)
-        BCMC_HIP_CHECK(hipEventRecord(m->ev[0], st));
-    // active blocks and the LRU misses among them, one scan
-    ActiveOp aop{m->dilated, m->lru.st.v.item_slot, m->flags, m->active_ids, m->lru.st.new_items,
-                 m->q, isovalue, m->total};
-    launch_scan<2>(ctx, m->site_active, m->lru.st.site_miss, aop, m->total);
-    if (timed)
-        BCMC_HIP_CHECK(hipEventRecord(m->ev[1], st));
-    // cache update: age/mark + available slots, growth, eviction order
-    lru_enqueue_update_mc(ctx, &m->lru.st, m->flags, m->q);
-    if (timed)
-        BCMC_HIP_CHECK(hipEventRecord(m->ev[2], st));
-    // slot assignment and decode of the misses into their slots
-    LruState &L = m->lru.st;
-    launch_decode_assign(ctx, m->stream, m->stream_words, m->maxbits, L.new_items, m->total, L.v.order,
-                         L.v.age, L.v.avail, L.v.item, L.v.item_slot, L.total, (float *)L.data,
-                         L.capacity, m->q);
-    if (timed)
-        BCMC_HIP_CHECK(hipEventRecord(m->ev[3], st));
-    // counts, offsets and vertices in one pass
-    BCMC_TRY(mc_launch_fused(m));
-    if (timed)
-        BCMC_HIP_CHECK(hipEventRecord(m->ev[4], st));
-    BCMC_HIP_CHECK(hipGetLastError());
-    BCMC_HIP_CHECK(hipMemcpyAsync(m->qh, m->q, sizeof(QState), hipMemcpyDeviceToHost, st));
+    *m->iso_h = isovalue;
+    const uint32_t npasses = lru_sort_passes(&m->lru.st);
+    const uint64_t key[4] = {(uint64_t)(uintptr_t)m->verts, m->vert_cap, (uint64_t)m->fmt, npasses};
+    bool launched = false;
+    if (m->use_graph && !timed) {
+        if (!m->gexec || std::memcmp(key, m->graph_key, sizeof(key)) != 0) {
+            mc_drop_graph(m);
+            BCMC_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            const int rc = mc_enqueue_query(m, npasses, false);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(st, &g);
+            if (rc == BCMC_OK && ec == hipSuccess && g &&
+                hipGraphInstantiate(&m->gexec, g, nullptr, nullptr, 0) == hipSuccess) {
+                m->graph = g;
+                std::memcpy(m->graph_key, key, sizeof(key));
+            } else {
+                if (g)
+                    (void)hipGraphDestroy(g);
+                m->gexec = nullptr;
+                m->use_graph = false;  // replay is an optimisation: fall back to direct launches
+                (void)hipGetLastError();
+            }
+        }
+        if (m->gexec) {
+            BCMC_HIP_CHECK(hipGraphLaunch(m->gexec, st));
+            launched = true;
+        }
+    }
+    if (!launched)
+        BCMC_TRY(mc_enqueue_query(m, npasses, timed));
     BCMC_HIP_CHECK(hipStreamSynchronize(st));
+    m->lru.st.age_bound += 1;
     QState qs = *m->qh;
     if (qs.num_verts > m->vert_cap) {
         // grow like compressed_marching_cubes.js:774-785, then run the pass again

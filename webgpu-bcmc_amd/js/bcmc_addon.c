@@ -411,13 +411,28 @@ static napi_value mc_set_format(napi_env env, napi_callback_info info)
     return NULL;
 }
 
+/* mcComputeSurface(mc, iso, timed): with timed the per-stage device times are
+ * recorded (direct launches); otherwise the captured query graph is replayed
+ * and only the counts are returned. */
 static napi_value mc_compute_surface(napi_env env, napi_callback_info info)
 {
-    GET_ARGS(2);
+    GET_ARGS(3);
     EXT(0, bcmc_mc, mc);
+    bool timed = true;
+    if (argc > 2)
+        napi_get_value_bool(env, argv[2], &timed);
     bcmc_perf p;
+    memset(&p, 0, sizeof(p));
     uint64_t nv = 0;
-    BCMC_CALL(env, bcmc_mc_compute_surface(mc, (float)num(env, argv[1]), &p, &nv));
+    BCMC_CALL(env, bcmc_mc_compute_surface(mc, (float)num(env, argv[1]), timed ? &p : NULL, &nv));
+    if (!timed) {
+        bcmc_mc_info mi;
+        BCMC_CALL(env, bcmc_mc_get_info(mc, &mi));
+        p.numActiveBlocks = mi.num_active_blocks;
+        p.numBlocksDecompressed = mi.new_decompressed;
+        p.numBlocksWithVertices = mi.num_blocks_with_vertices;
+        p.numVertices = mi.num_vertices;
+    }
     napi_value o;
     NAPI_CALL(env, napi_create_object(env, &o));
     set_num(env, o, "numVertices", (double)nv);

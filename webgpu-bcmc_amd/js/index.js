@@ -235,7 +235,8 @@ class CompressedMarchingCubes {
         native.mcSetFormat(this.handle, fmt);
     }
     async computeSurface(isovalue, perfTracker) {
-        const r = native.mcComputeSurface(this.handle, isovalue);
+        // without a perfTracker the captured query graph is replayed (no stage times)
+        const r = native.mcComputeSurface(this.handle, isovalue, !!perfTracker);
         if (perfTracker) {
             if (perfTracker.computeActiveBlocks === undefined) {
                 perfTracker.totalBlocks = this.totalBlocks;

@@ -413,13 +413,26 @@ class CompressedMarchingCubes:
     def setVertexFormat(self, fmt):
         _check(lib().bcmc_mc_set_vertex_format(self.h, fmt))
 
-    def computeSurface(self, isovalue, perf_tracker=None):
-        """Returns the vertex count; fills perf_tracker (dict of lists) like the reference."""
+    def computeSurface(self, isovalue, perf_tracker=None, timed=None):
+        """Returns the vertex count; fills perf_tracker (dict of lists) like the reference.
+
+        timed (default: whether perf_tracker is given) records the per-stage
+        device times in lastPerf; untimed queries replay the captured graph of
+        the query chain and lastPerf holds the counts only."""
+        if timed is None:
+            timed = perf_tracker is not None
         perf = _Perf()
         nv = ctypes.c_uint64()
-        _check(lib().bcmc_mc_compute_surface(self.h, float(isovalue), ctypes.byref(perf),
-                                             ctypes.byref(nv)))
-        self.lastPerf = {f: getattr(perf, f) for f, _ in _Perf._fields_}
+        _check(lib().bcmc_mc_compute_surface(self.h, float(isovalue),
+                                             ctypes.byref(perf) if timed else None, ctypes.byref(nv)))
+        if timed:
+            self.lastPerf = {f: getattr(perf, f) for f, _ in _Perf._fields_}
+        else:
+            i = self.info()
+            self.lastPerf = {"numActiveBlocks": i["num_active_blocks"],
+                             "numBlocksDecompressed": i["new_decompressed"],
+                             "numBlocksWithVertices": i["num_blocks_with_vertices"],
+                             "numVertices": i["num_vertices"]}
         if perf_tracker is not None:
             if "computeActiveBlocks" not in perf_tracker:
                 perf_tracker["totalBlocks"] = self.info()["total_blocks"]
