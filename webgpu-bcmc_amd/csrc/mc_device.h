@@ -355,12 +355,11 @@ BCMC_DEV void row_store(float *dst, const float (&r)[5]) {
 // wave; rows with live = false have none), written from out0 on
 // (compute_block_vertices.comp:76-174).  Vertex order is the reference's:
 // blocks in row order, cells in x-fastest order, within a cell its
-// triangle-table row.  Vertex-parallel: every cell with vertices marks its
+// triangle-table row.  Triangle-parallel: every cell with vertices marks its
 // first triangle in LDS with its index g = 64 row + cell (cells ascend with
-// their vertices), so the owner of vertex v is the running maximum of the
-// marks up to triangle v / 3; lane v of each round interpolates vertex v, and a round
-// writes one contiguous run.  Vertices at or past `cap` are not written.  Must
-// be called by all 64 lanes.
+// their triangles), so the owner of triangle t is the running maximum of the
+// marks up to t.  Vertices at or past `cap` are not written.  Must be called
+// by all 64 lanes.
 BCMC_DEV void emit_row_blocks(const RowBlock &b, const RowTile &t, const McTables &tab, EmitScratch &es,
                               unsigned long long out0, unsigned long long cap, void *verts, int fmt) {
     const uint32_t lane = lane_id(), row = lane >> 4, l = lane & 15u, y = l & 3u, z = l >> 2;
@@ -410,46 +409,61 @@ BCMC_DEV void emit_row_blocks(const RowBlock &b, const RowTile &t, const McTable
         first += nv[x];
     }
     wave_lds_sync();
+    // Rounds of 64 triangles: lane t looks up the cell of triangle t0 + t and
+    // its 3 edges; then 3 sub-rounds of 64 vertices each, where lane l handles
+    // vertex 3 t0 + 64 r + l and fetches its triangle's lookup from lane
+    // (64 r + l) / 3 (ds_bpermute), so the stores stay one contiguous run.
+    const uint32_t ntri = (limit + 2) / 3;
     uint32_t carry = 0;
-    for (uint32_t v0 = 0; v0 < limit; v0 += 64) {
-        const uint32_t v = v0 + lane;
-        const uint32_t tri = (v * 0xAAABu) >> 17;  // v / 3 for v < 2^16
-        const uint32_t g = max(carry, wave_incl_max(v < limit ? (uint32_t)es.start[tri] : 0u));
+    for (uint32_t t0 = 0; t0 < ntri; t0 += 64) {
+        const uint32_t t = t0 + lane;
+        const uint32_t g = max(carry, wave_incl_max(t < ntri ? (uint32_t)es.start[t] : 0u));
         carry = wave_read_lane(g, 63);
-        if (v >= limit)
-            break;
-        const uint32_t j = g >> 6, cell = g & 63u;
-        const uint32_t k = v - es.first[g];
-        const uint32_t e = (uint32_t)(tab.edges[es.corner[g]] >> (4 * k)) & 15u;
-        const EdgeInfo &ei = tab.edge[e];
-        const uint32_t cx = cell & 3u, cy = (cell >> 2) & 3u, cz = cell >> 4;
-        const float *tc = es.tile[j] + cx + kTileY * cy + kTileZ * cz;
-        const float fa = tc[__float_as_uint(ei.off_a)];
-        const float fb = tc[__float_as_uint(ei.off_b)];
-        const float iso = es.iso[j];
-        // lerp_verts (compute_block_vertices.comp:62-74), then + voxel + 0.5
-        float tt = 0.0f;
-        if (!(fabsf(fa - fb) < 0.001f))
-            tt = (iso - fa) / (fb - fa);
-        const float px = (ei.a[0] + tt * ei.d[0] + (float)cx) + 0.5f;
-        const float py = (ei.a[1] + tt * ei.d[1] + (float)cy) + 0.5f;
-        const float pz = (ei.a[2] + tt * ei.d[2] + (float)cz) + 0.5f;
-        const unsigned long long o = out0 + v;
-        const uint32_t bid = es.bid[j];
-        if (fmt == BCMC_VERTS_LOCAL_F32) {
-            reinterpret_cast<float4 *>(verts)[o] = make_float4(px, py, pz, __uint_as_float(bid));
-        } else if (fmt == BCMC_VERTS_GLOBAL_F32) {
-            float *gp = reinterpret_cast<float *>(verts) + 3 * o;
-            gp[0] = (float)(4 * es.bpos[j][0]) + px;
-            gp[1] = (float)(4 * es.bpos[j][1]) + py;
-            gp[2] = (float)(4 * es.bpos[j][2]) + pz;
-        } else {
-            // compress_position (block_vertex_position.comp:9-17)
-            const uint32_t qx = (uint32_t)(((px - 0.5f) * 0.25f) * 1023.0f);
-            const uint32_t qy = (uint32_t)(((py - 0.5f) * 0.25f) * 1023.0f);
-            const uint32_t qz = (uint32_t)(((pz - 0.5f) * 0.25f) * 1023.0f);
-            const uint32_t packed = ((qx & 0x3ffu) << 20) | ((qy & 0x3ffu) << 10) | (qz & 0x3ffu);
-            reinterpret_cast<uint2 *>(verts)[o] = make_uint2(packed, bid);
+        uint32_t info = 0;  // edges (12 bits) | g << 12
+        if (t < ntri) {
+            const uint32_t k = 3 * t - es.first[g];  // first vertex of the triangle in its cell
+            info = ((uint32_t)(tab.edges[es.corner[g]] >> (4 * k)) & 0xFFFu) | (g << 12);
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < 3; ++r) {
+            const uint32_t lv = 64 * r + lane;          // vertex of this round
+            const uint32_t src = (lv * 0xAAABu) >> 17;  // lv / 3
+            const uint32_t inf = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)info);
+            const uint32_t v = 3 * t0 + lv;
+            if (v >= limit)
+                continue;
+            const uint32_t gg = inf >> 12, j = gg >> 6, cell = gg & 63u;
+            const uint32_t m = lv - 3 * src;
+            const EdgeInfo &ei = tab.edge[(inf >> (4 * m)) & 15u];
+            const uint32_t cx = cell & 3u, cy = (cell >> 2) & 3u, cz = cell >> 4;
+            const float *tc = es.tile[j] + cx + kTileY * cy + kTileZ * cz;
+            const float fa = tc[__float_as_uint(ei.off_a)];
+            const float fb = tc[__float_as_uint(ei.off_b)];
+            const float iso = es.iso[j];
+            // lerp_verts (compute_block_vertices.comp:62-74), then + voxel + 0.5
+            float tt = 0.0f;
+            if (!(fabsf(fa - fb) < 0.001f))
+                tt = (iso - fa) / (fb - fa);
+            const float px = (ei.a[0] + tt * ei.d[0] + (float)cx) + 0.5f;
+            const float py = (ei.a[1] + tt * ei.d[1] + (float)cy) + 0.5f;
+            const float pz = (ei.a[2] + tt * ei.d[2] + (float)cz) + 0.5f;
+            const unsigned long long o = out0 + v;
+            const uint32_t bid = es.bid[j];
+            if (fmt == BCMC_VERTS_LOCAL_F32) {
+                reinterpret_cast<float4 *>(verts)[o] = make_float4(px, py, pz, __uint_as_float(bid));
+            } else if (fmt == BCMC_VERTS_GLOBAL_F32) {
+                float *gp = reinterpret_cast<float *>(verts) + 3 * o;
+                gp[0] = (float)(4 * es.bpos[j][0]) + px;
+                gp[1] = (float)(4 * es.bpos[j][1]) + py;
+                gp[2] = (float)(4 * es.bpos[j][2]) + pz;
+            } else {
+                // compress_position (block_vertex_position.comp:9-17)
+                const uint32_t qx = (uint32_t)(((px - 0.5f) * 0.25f) * 1023.0f);
+                const uint32_t qy = (uint32_t)(((py - 0.5f) * 0.25f) * 1023.0f);
+                const uint32_t qz = (uint32_t)(((pz - 0.5f) * 0.25f) * 1023.0f);
+                const uint32_t packed = ((qx & 0x3ffu) << 20) | ((qy & 0x3ffu) << 10) | (qz & 0x3ffu);
+                reinterpret_cast<uint2 *>(verts)[o] = make_uint2(packed, bid);
+            }
         }
     }
     wave_lds_sync();  // the scratch is reused by the next group
