@@ -207,6 +207,8 @@ def main():
     ap.add_argument("--batched-reps", type=int, default=5)
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of timing candidates first")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -226,6 +228,8 @@ def main():
 
     dev = bcmc_amd.Device(0)
     info = dev.info()
+    tuning = dev.tuning() if args.no_autotune else dev.autotune()
+    log(f"bench: launch shapes (workgroups per CU) {tuning}")
     cache0 = n_blocks // 4
     isos = np.linspace(0.2, 0.8, 8).astype(np.float32)
 
@@ -326,6 +330,7 @@ def main():
                              ("numActiveBlocks", "numBlocksDecompressed", "numBlocksWithVertices",
                               "numVertices")},
         "cpu_baseline": None,
+        "tuning": tuning,
     }
 
     if not args.no_batched:

@@ -271,6 +271,33 @@ BCMC_API int bcmc_zfp_block_ranges(bcmc_ctx *ctx, bcmc_buf *stream, uint32_t max
 BCMC_API int bcmc_ctx_enable_timing(bcmc_ctx *ctx, int enable);
 BCMC_API double bcmc_ctx_last_kernel_ms(bcmc_ctx *ctx);
 
+/* Launch shapes of the context's persistent kernels: workgroups per compute
+ * unit for the single-pass scans, the radix-sort passes, the zfp decoders and
+ * the fused marching-cubes pass.  No reference counterpart (WebGPU dispatch
+ * sizes are fixed in compressed_marching_cubes.js / exclusive_scan.js /
+ * radix_sort.js); results are identical for every shape, only speed changes.
+ * Defaults: scan 4, sort 4, decode 8, mc 4.  The *_ms fields are filled by
+ * bcmc_ctx_autotune (device time of the chosen shape on its probe workload)
+ * and ignored by bcmc_ctx_set_tuning. */
+typedef struct bcmc_tuning {
+    uint32_t scan_wg_per_cu;
+    uint32_t sort_wg_per_cu;
+    uint32_t decode_wg_per_cu;
+    uint32_t mc_wg_per_cu;
+    float scan_ms;   /* 16M-element exclusive scan */
+    float sort_ms;   /* 2M pairs, 24-bit keys (3 digit passes) */
+    float decode_ms; /* 256K blocks of a synthetic rate-4 stream into a block cache */
+    float mc_ms;     /* not measured: the fused pass runs at its occupancy limit */
+} bcmc_tuning;
+BCMC_API int bcmc_ctx_get_tuning(bcmc_ctx *ctx, bcmc_tuning *out);
+/* Each *_wg_per_cu must be in 1..64; pipelines re-capture their graphs. */
+BCMC_API int bcmc_ctx_set_tuning(bcmc_ctx *ctx, const bcmc_tuning *tuning);
+/* Times the scan, sort and decode candidates on synthetic device data
+ * (about 200 MiB of temporary device memory, well under a second) and keeps
+ * the fastest shape of each family in the context; out (nullable) receives
+ * the result.  Call it before the first query: it synchronises the stream. */
+BCMC_API int bcmc_ctx_autotune(bcmc_ctx *ctx, bcmc_tuning *out);
+
 #ifdef __cplusplus
 }
 #endif

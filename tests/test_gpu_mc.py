@@ -133,3 +133,30 @@ def test_perf_tracker_keys(dev):
     assert perf["totalBlocks"] == 512
     mem = mc.reportMemoryUse()
     assert mem["total"] > mem["cache"] > 0
+
+
+def test_launch_shapes_leave_results_unchanged(dev):
+    """Autotuned and extreme launch shapes give the oracle's bit-exact results."""
+    d = bcmc_amd.Device(0)
+    t = d.autotune()
+    say("mc autotune", t)
+    assert all(1 <= t[k] <= 64 for k in ("scan", "sort", "decode", "mc"))
+    assert all(v > 0 for v in t["ms"].values())
+    with pytest.raises(bcmc_amd.BcmcError):
+        d.set_tuning(scan=0)
+    dims, rate = (96, 80, 64), 4
+    f = synth.field(synth.FOURIER, 2, dims, 128, (1, 24, 0, 0))
+    s = synth.encode(f, 64 * rate)
+    n = oracle.nblocks(dims)
+    for shape in (dict(scan=1, sort=1, decode=1, mc=1), dict(scan=16, sort=8, decode=32, mc=8)):
+        say("mc launch shape", shape)
+        d.set_tuning(**shape)
+        assert {k: d.tuning()[k] for k in shape} == shape
+        mc = bcmc_amd.CompressedMarchingCubes(d)
+        mc.setCompressedVolume(s, rate, dims, (1, 1, 1), n // 4)
+        ref = oracle.MC(s, 64 * rate, dims, initial_cache=n // 4)
+        for iso in (0.3, 0.5, 0.7, 0.5):
+            _check_query(mc, ref, iso)
+            v = mc.vertices()
+            assert np.array_equal(v.view(np.uint32), ref.results()["verts_f32"].view(np.uint32))
+    d.set_tuning(**{k: t[k] for k in ("scan", "sort", "decode", "mc")})

@@ -58,6 +58,13 @@ struct bcmc_ctx {
     // small pinned staging area for counters read back after a query
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
+    // launch shapes: persistent workgroups per CU of each kernel family
+    // (bcmc_ctx_autotune measures and sets them; results never depend on them)
+    struct {
+        uint32_t scan = 4, sort = 4, decode = 8, mc = 4;
+    } tune;
+    // bumped on every change of `tune`: captured graphs keyed on it re-capture
+    uint64_t tune_gen = 0;
 };
 
 struct bcmc_buf {

@@ -573,7 +573,7 @@ static int eng_launch_fused(bcmc_engine *e, uint32_t nq, uint64_t total) {
     a.vert_cap = e->vert_cap;
     a.fmt = e->fmt;
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(total, kChunk),
-                                                                   persistent_grid(e->ctx, 4)));
+                                                                   persistent_grid(e->ctx, e->ctx->tune.mc)));
     eng_fused_kernel<<<grid, kEngThreads, 0, e->ctx->stream>>>(e->site_mc, a);
     BCMC_HIP_CHECK(hipGetLastError());
     return BCMC_OK;

@@ -147,7 +147,7 @@ template <int NCH, class Op>
 inline void launch_scan(bcmc_ctx *ctx, const ScanSite &s0, const ScanSite &s1, const Op &op,
                         uint64_t n_max) {
     uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(n_max, kScanTile),
-                                                             persistent_grid(ctx, 4)));
+                                                             persistent_grid(ctx, ctx->tune.scan)));
     scan_kernel<NCH, Op><<<grid, kScanThreads, 0, ctx->stream>>>(s0, s1, op);
 }
 

@@ -284,7 +284,7 @@ static int mc_launch_fused(bcmc_mc *m) {
     a.capacity = m->vert_cap;
     a.fmt = m->fmt;
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(m->total, kChunk),
-                                                                   persistent_grid(m->ctx, 4)));
+                                                                   persistent_grid(m->ctx, m->ctx->tune.mc)));
     mc_fused_kernel<<<grid, kMcThreads, 0, m->ctx->stream>>>(m->site_c0, m->site_c1, a);
     BCMC_HIP_CHECK(hipGetLastError());
     return BCMC_OK;
@@ -462,7 +462,8 @@ BCMC_API int bcmc_mc_compute_surface(bcmc_mc *m, float isovalue, bcmc_perf *perf
     const bool timed = perf != nullptr;
     *m->iso_h = isovalue;
     const uint32_t npasses = lru_sort_passes(&m->lru.st);
-    const uint64_t key[4] = {(uint64_t)(uintptr_t)m->verts, m->vert_cap, (uint64_t)m->fmt, npasses};
+    const uint64_t key[4] = {(uint64_t)(uintptr_t)m->verts, m->vert_cap,
+                             (uint64_t)m->fmt | ctx->tune_gen << 8, npasses};
     bool launched = false;
     if (m->use_graph && !timed) {
         if (!m->gexec || std::memcmp(key, m->graph_key, sizeof(key)) != 0) {

@@ -165,7 +165,7 @@ static void sort_passes(bcmc_ctx *ctx, SortScratch *s, uint32_t n_max, const uin
                         uint32_t n_host, const FirstSrc &first_src, uint32_t npasses,
                         uint32_t *final_vals, uint32_t out_skip) {
     const uint32_t tiles = std::max<uint32_t>(1, div_up(n_max, kSortTile));
-    const uint32_t grid = std::min<uint32_t>(tiles, persistent_grid(ctx, 4));
+    const uint32_t grid = std::min<uint32_t>(tiles, persistent_grid(ctx, ctx->tune.sort));
     hipStream_t st = ctx->stream;
     HistScanOp hop{s->hist, n_dev, n_host};
     for (uint32_t pass = 0; pass < npasses; ++pass) {

@@ -1,0 +1,221 @@
+// Launch-shape autotuner: times the persistent-grid candidates of the scan,
+// radix-sort and zfp-decode kernel families on synthetic device data and keeps
+// the fastest in the context.  Results of every kernel are independent of the
+// grid (tiles are fixed-size and claimed in order, per-tile histograms and
+// look-back status words are indexed by tile), so tuning changes speed only.
+#include "common.h"
+#include "kernels.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace bcmc {
+namespace {
+
+// 32-bit mix of (index, seed): deterministic synthetic payloads.
+__device__ inline uint32_t mix32(uint64_t i, uint32_t seed) {
+    uint64_t z = i * 0x9E3779B97F4A7C15ull + seed;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return (uint32_t)(z ^ (z >> 31));
+}
+
+__global__ void tune_fill_kernel(uint32_t *p, uint64_t n, uint32_t seed, uint32_t mask,
+                                 int identity) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = identity ? (uint32_t)i : (mix32(i, seed) & mask);
+}
+
+struct TuneScanOp {
+    uint32_t *data;
+    uint32_t n;
+    unsigned long long *total_out;
+    BCMC_DEV uint32_t size() const { return n; }
+    BCMC_DEV void begin() const {}
+    BCMC_DEV void value(uint32_t i, uint32_t (&v)[1]) const { v[0] = data[i] & 0xFF; }
+    BCMC_DEV void emit(uint32_t i, const uint32_t (&)[1], const unsigned long long (&e)[1]) const {
+        data[i] = (uint32_t)e[0];
+    }
+    BCMC_DEV void total(const unsigned long long (&t)[1]) const { *total_out = t[0]; }
+};
+
+// Device buffers of one tuning run, freed on every exit path.
+struct TuneBufs {
+    std::vector<void *> ptrs;
+    ~TuneBufs() {
+        for (void *p : ptrs)
+            dev_free(p);
+    }
+    template <class T>
+    int alloc(T **p, size_t bytes) {
+        BCMC_TRY(dev_alloc((void **)p, bytes));
+        ptrs.push_back(*p);
+        return BCMC_OK;
+    }
+};
+
+struct Events {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Events() {
+        if (a)
+            (void)hipEventDestroy(a);
+        if (b)
+            (void)hipEventDestroy(b);
+    }
+};
+
+void fill(bcmc_ctx *ctx, uint32_t *p, uint64_t n, uint32_t seed, uint32_t mask, int identity) {
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(div_up(n, 256), persistent_grid(ctx, 8));
+    tune_fill_kernel<<<std::max(grid, 1u), 256, 0, ctx->stream>>>(p, n, seed, mask, identity);
+}
+
+// Median device time (ms) of `reps` runs of f after one warm-up run.
+template <class F>
+int time_median(bcmc_ctx *ctx, Events &ev, int reps, F &&f, float *ms) {
+    f();
+    BCMC_HIP_CHECK(hipGetLastError());
+    std::vector<float> t;
+    for (int r = 0; r < reps; ++r) {
+        BCMC_HIP_CHECK(hipEventRecord(ev.a, ctx->stream));
+        f();
+        BCMC_HIP_CHECK(hipEventRecord(ev.b, ctx->stream));
+        BCMC_HIP_CHECK(hipEventSynchronize(ev.b));
+        float x = 0.f;
+        BCMC_HIP_CHECK(hipEventElapsedTime(&x, ev.a, ev.b));
+        t.push_back(x);
+    }
+    std::sort(t.begin(), t.end());
+    *ms = t[t.size() / 2];
+    return BCMC_OK;
+}
+
+// Tries each candidate for `slot`, leaves the fastest in it; best time in *best_ms.
+template <class F>
+int pick(bcmc_ctx *ctx, Events &ev, uint32_t &slot, const std::vector<uint32_t> &cands, F &&f,
+         float *best_ms) {
+    uint32_t best = slot;
+    float best_t = 0.f;
+    for (size_t i = 0; i < cands.size(); ++i) {
+        slot = cands[i];
+        float t = 0.f;
+        BCMC_TRY(time_median(ctx, ev, 5, f, &t));
+        if (i == 0 || t < best_t) {
+            best_t = t;
+            best = cands[i];
+        }
+    }
+    slot = best;
+    *best_ms = best_t;
+    return BCMC_OK;
+}
+
+}  // namespace
+}  // namespace bcmc
+
+using namespace bcmc;
+
+static void to_public(const bcmc_ctx *ctx, bcmc_tuning *t) {
+    t->scan_wg_per_cu = ctx->tune.scan;
+    t->sort_wg_per_cu = ctx->tune.sort;
+    t->decode_wg_per_cu = ctx->tune.decode;
+    t->mc_wg_per_cu = ctx->tune.mc;
+}
+
+extern "C" {
+
+BCMC_API int bcmc_ctx_get_tuning(bcmc_ctx *ctx, bcmc_tuning *out) {
+    BCMC_CHECK_ARG(ctx && out, "null argument");
+    *out = bcmc_tuning{};
+    to_public(ctx, out);
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_ctx_set_tuning(bcmc_ctx *ctx, const bcmc_tuning *t) {
+    BCMC_CHECK_ARG(ctx && t, "null argument");
+    const uint32_t v[4] = {t->scan_wg_per_cu, t->sort_wg_per_cu, t->decode_wg_per_cu,
+                           t->mc_wg_per_cu};
+    for (uint32_t x : v)
+        BCMC_CHECK_ARG(x >= 1 && x <= 64, "workgroups per CU must be in 1..64");
+    ctx->tune.scan = v[0];
+    ctx->tune.sort = v[1];
+    ctx->tune.decode = v[2];
+    ctx->tune.mc = v[3];
+    ++ctx->tune_gen;
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_ctx_autotune(bcmc_ctx *ctx, bcmc_tuning *out) {
+    BCMC_CHECK_ARG(ctx, "ctx is null");
+    BCMC_HIP_CHECK(hipSetDevice(ctx->device));
+    BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    Events ev;
+    BCMC_HIP_CHECK(hipEventCreate(&ev.a));
+    BCMC_HIP_CHECK(hipEventCreate(&ev.b));
+    TuneBufs B;
+    bcmc_tuning res{};
+
+    // scan: 16M elements through the single-pass scan
+    constexpr uint32_t kScanN = 1u << 24;
+    uint32_t *scan_data = nullptr;
+    unsigned long long *total = nullptr;
+    BCMC_TRY(B.alloc(&scan_data, (size_t)kScanN * 4));
+    BCMC_TRY(B.alloc(&total, 8));
+    ScanSite site;
+    BCMC_TRY(scan_site_alloc(&site, kScanN, kScanTile));
+    struct SiteGuard {
+        ScanSite *s;
+        ~SiteGuard() { scan_site_free(s); }
+    } sg{&site};
+    fill(ctx, scan_data, kScanN, 1, 0xFF, 0);
+    const TuneScanOp sop{scan_data, kScanN, total};
+    BCMC_TRY(pick(ctx, ev, ctx->tune.scan, {2, 4, 8, 16},
+                  [&] { launch_scan<1>(ctx, site, site, sop, kScanN); }, &res.scan_ms));
+
+    // radix sort: 2M pairs with 24-bit keys (3 digit passes, as an LRU order)
+    constexpr uint32_t kSortN = 1u << 21;
+    uint32_t *keys = nullptr, *vals = nullptr, *sorted = nullptr, *count = nullptr;
+    BCMC_TRY(B.alloc(&keys, (size_t)kSortN * 4));
+    BCMC_TRY(B.alloc(&vals, (size_t)kSortN * 4));
+    BCMC_TRY(B.alloc(&sorted, (size_t)kSortN * 4));
+    BCMC_TRY(B.alloc(&count, 4));
+    fill(ctx, keys, kSortN, 2, 0xFFFFFF, 0);
+    fill(ctx, vals, kSortN, 0, 0, 1);
+    BCMC_HIP_CHECK(hipMemcpyAsync(count, &kSortN, 4, hipMemcpyHostToDevice, ctx->stream));
+    SortScratch ss;
+    struct SortGuard {
+        SortScratch *s;
+        ~SortGuard() { sort_scratch_free(s); }
+    } sog{&ss};
+    BCMC_TRY(sort_scratch_alloc(&ss, kSortN));
+    BCMC_TRY(pick(ctx, ev, ctx->tune.sort, {1, 2, 4, 8},
+                  [&] { launch_sort_pairs(ctx, &ss, keys, vals, count, kSortN, sorted, 3); },
+                  &res.sort_ms));
+
+    // zfp decode: 256K blocks of a synthetic rate-4 stream into a block cache
+    constexpr uint32_t kDecN = 1u << 18, kMaxBits = 256;
+    const uint64_t nwords = (uint64_t)kDecN * kMaxBits / 64;
+    uint32_t *stream = nullptr, *slots = nullptr;
+    float *cache = nullptr;
+    BCMC_TRY(B.alloc(&stream, nwords * 8));
+    BCMC_TRY(B.alloc(&slots, (size_t)kDecN * 4));
+    BCMC_TRY(B.alloc(&cache, (size_t)kDecN * 64 * 4));
+    fill(ctx, stream, nwords * 2, 3, 0xFFFFFFFFu, 0);
+    fill(ctx, slots, kDecN, 0, 0, 1);
+    BCMC_TRY(pick(ctx, ev, ctx->tune.decode, {2, 4, 8, 16},
+                  [&] {
+                      launch_decode_to_cache(ctx, (const uint64_t *)stream, nwords, kMaxBits,
+                                             nullptr, nullptr, kDecN, (const int32_t *)slots,
+                                             cache, kDecN);
+                  },
+                  &res.decode_ms));
+    BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ++ctx->tune_gen;
+    if (out) {
+        to_public(ctx, &res);
+        *out = res;
+    }
+    return BCMC_OK;
+}
+
+}  // extern "C"

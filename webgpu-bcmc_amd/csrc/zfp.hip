@@ -437,7 +437,7 @@ void launch_decode_to_cache(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwor
     a.item_slot = item_slot;
     a.capacity = capacity;
     a.out = cache;
-    const uint32_t grid = std::min(div_up(n_max, kZfpThreads), persistent_grid(ctx, 8));
+    const uint32_t grid = std::min(div_up(n_max, kZfpThreads), persistent_grid(ctx, ctx->tune.decode));
     if (grid)
         launch_mode<kModeCache>(a, grid, ctx->stream);
 }
@@ -459,7 +459,7 @@ void launch_decode_assign(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords
     a.capacity = capacity;
     a.out = cache;
     a.q = q;
-    const uint32_t grid = std::max(1u, std::min(div_up(n_max, kZfpThreads), persistent_grid(ctx, 8)));
+    const uint32_t grid = std::max(1u, std::min(div_up(n_max, kZfpThreads), persistent_grid(ctx, ctx->tune.decode)));
     launch_mode<kModeLruAssign>(a, grid, ctx->stream);
 }
 

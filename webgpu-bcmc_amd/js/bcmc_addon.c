@@ -163,6 +163,25 @@ static napi_value ctx_info(napi_env env, napi_callback_info info)
     return o;
 }
 
+/* ctxAutotune(ctx) -> {scan, sort, decode, mc, scanMs, sortMs, decodeMs} */
+static napi_value ctx_autotune(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_ctx, ctx);
+    bcmc_tuning t;
+    BCMC_CALL(env, bcmc_ctx_autotune(ctx, &t));
+    napi_value o;
+    NAPI_CALL(env, napi_create_object(env, &o));
+    set_num(env, o, "scan", t.scan_wg_per_cu);
+    set_num(env, o, "sort", t.sort_wg_per_cu);
+    set_num(env, o, "decode", t.decode_wg_per_cu);
+    set_num(env, o, "mc", t.mc_wg_per_cu);
+    set_num(env, o, "scanMs", t.scan_ms);
+    set_num(env, o, "sortMs", t.sort_ms);
+    set_num(env, o, "decodeMs", t.decode_ms);
+    return o;
+}
+
 static napi_value buf_alloc(napi_env env, napi_callback_info info)
 {
     GET_ARGS(2);
@@ -518,7 +537,7 @@ static napi_value init(napi_env env, napi_value exports)
     napi_property_descriptor d[] = {
         FN("version", version),
         FN("ctxCreate", ctx_create), FN("ctxDestroy", ctx_destroy), FN("ctxSync", ctx_sync),
-        FN("ctxInfo", ctx_info),
+        FN("ctxInfo", ctx_info), FN("ctxAutotune", ctx_autotune),
         FN("bufAlloc", buf_alloc), FN("bufFree", buf_free), FN("bufSize", buf_size),
         FN("bufUpload", buf_upload), FN("bufDownload", buf_download),
         FN("scanAlignedSize", scan_aligned_size), FN("scanExclusive", scan_exclusive),

@@ -61,6 +61,11 @@ class Device {
     async onSubmittedWorkDone() {
         native.ctxSync(this.handle);
     }
+    // Times launch-shape candidates of the scan/sort/decode kernels and keeps
+    // the fastest (results never depend on the shape).
+    autotune() {
+        return native.ctxAutotune(this.handle);
+    }
     destroy() {
         if (this.handle) native.ctxDestroy(this.handle);
         this.handle = null;

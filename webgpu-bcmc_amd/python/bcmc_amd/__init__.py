@@ -73,6 +73,13 @@ class _EngineInfo(ctypes.Structure):
                 ("pool_bytes", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64)]
 
 
+class _Tuning(ctypes.Structure):
+    _fields_ = [("scan_wg_per_cu", ctypes.c_uint32), ("sort_wg_per_cu", ctypes.c_uint32),
+                ("decode_wg_per_cu", ctypes.c_uint32), ("mc_wg_per_cu", ctypes.c_uint32),
+                ("scan_ms", ctypes.c_float), ("sort_ms", ctypes.c_float),
+                ("decode_ms", ctypes.c_float), ("mc_ms", ctypes.c_float)]
+
+
 class _ServeOpts(ctypes.Structure):
     _fields_ = [("max_batch", ctypes.c_uint32), ("vertex_budget_bytes", ctypes.c_uint64)]
 
@@ -111,6 +118,9 @@ def lib():
         "bcmc_ctx_info": (i32, [vp, P(i32), ctypes.c_char_p, ctypes.c_size_t]),
         "bcmc_ctx_enable_timing": (i32, [vp, i32]),
         "bcmc_ctx_last_kernel_ms": (ctypes.c_double, [vp]),
+        "bcmc_ctx_get_tuning": (i32, [vp, P(_Tuning)]),
+        "bcmc_ctx_set_tuning": (i32, [vp, P(_Tuning)]),
+        "bcmc_ctx_autotune": (i32, [vp, P(_Tuning)]),
         "bcmc_buf_alloc": (i32, [vp, u64, P(vp)]),
         "bcmc_buf_wrap": (i32, [vp, vp, u64, P(vp)]),
         "bcmc_buf_free": (i32, [vp]),
@@ -195,6 +205,36 @@ class Device:
         name = ctypes.create_string_buffer(128)
         _check(lib().bcmc_ctx_info(self.h, ctypes.byref(n), name, 128))
         return {"num_cus": n.value, "name": name.value.decode()}
+
+    _TUNE_KEYS = ("scan", "sort", "decode", "mc")
+
+    @staticmethod
+    def _tuning_dict(t):
+        d = {k: int(getattr(t, f"{k}_wg_per_cu")) for k in Device._TUNE_KEYS}
+        d["ms"] = {k: float(getattr(t, f"{k}_ms")) for k in Device._TUNE_KEYS[:3]}
+        return d
+
+    def tuning(self):
+        """Workgroups per CU of the scan / sort / decode / mc kernel families."""
+        t = _Tuning()
+        _check(lib().bcmc_ctx_get_tuning(self.h, ctypes.byref(t)))
+        return self._tuning_dict(t)
+
+    def set_tuning(self, **wg_per_cu):
+        t = _Tuning()
+        _check(lib().bcmc_ctx_get_tuning(self.h, ctypes.byref(t)))
+        for k, v in wg_per_cu.items():
+            if k not in self._TUNE_KEYS:
+                raise ValueError(f"unknown kernel family {k!r}")
+            setattr(t, f"{k}_wg_per_cu", int(v))
+        _check(lib().bcmc_ctx_set_tuning(self.h, ctypes.byref(t)))
+
+    def autotune(self):
+        """Times the launch-shape candidates on synthetic data and keeps the
+        fastest per kernel family; returns the chosen shapes and their times."""
+        t = _Tuning()
+        _check(lib().bcmc_ctx_autotune(self.h, ctypes.byref(t)))
+        return self._tuning_dict(t)
 
     def create_buffer(self, nbytes):
         return Buffer.alloc(self, nbytes)
