@@ -34,6 +34,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+STAGE_KERNEL = {"computeVertices": "mc_fused_kernel", "decompression": "zfp_kernel<4,3>",
+                "computeActiveBlocks": "scan_kernel<2,ActiveOp>"}
+
+
+def profiled_traffic(workload, stage):
+    """HBM bytes per launch of the stage's kernel on this workload, from the
+    committed FETCH_SIZE/WRITE_SIZE counter passes (copy-calibrated), or None."""
+    import re
+    kernel = STAGE_KERNEL.get(stage)
+    path = os.path.join(ROOT, "profiles", f"{kernel.split('<')[0]}_{workload}_hbm_traffic.md") \
+        if kernel else None
+    if not path or not os.path.exists(path):
+        return None, None
+    m = re.search(rf"^traffic_bytes_per_launch {re.escape(kernel)}: (\d+)$", open(path).read(), re.M)
+    return (int(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
+
+
 def stage_bytes(s, rate, n_blocks):
     """Algorithmic HBM bytes per stage of one query (per-stage model of the
     reference's kernels)."""
@@ -194,7 +211,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="fourier512_r4")
-    ap.add_argument("--cpu-sweeps", type=int, default=16,
+    ap.add_argument("--cpu-sweeps", type=int, default=8,
                     help="sweeps of the 8 isovalues timed on the CPU oracle (0 = skip)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy kernel of this size (HBM counter calibration)")
@@ -292,6 +309,7 @@ def main():
     gbps = {k: (st_bytes[k] / (st_ms[k] * 1e-3) / 1e9 if st_ms[k] > 0 else 0.0)
             for k in stages if k != "cacheUpdate"}
     decode_bytes_range = n_blocks * (8 * rate + 8)
+    traffic, traffic_src = profiled_traffic(args.workload, dom)
 
     out = {
         "metric": "iso-queries/s (plain computeSurface, 1 GPU)",
@@ -325,7 +343,9 @@ def main():
         },
         "roofline": {"bound": "hbm", "kernel": dom,
                      "achieved": round(gbps[dom], 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(gbps[dom] / HBM_PEAK_GBPS, 4), "traffic": None},
+                     "frac": round(gbps[dom] / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "algorithmic_bytes_per_launch": round(st_bytes[dom] / queries)},
         "counts_per_query": {k: round(float(np.mean([r[k] for r in records])), 1) for k in
                              ("numActiveBlocks", "numBlocksDecompressed", "numBlocksWithVertices",
                               "numVertices")},
