@@ -219,7 +219,7 @@ def main():
     ap.add_argument("--served-queries", type=int, default=2048)
     ap.add_argument("--served-batch", type=int, default=64)
     ap.add_argument("--served-pool-gib", type=int, default=32)
-    ap.add_argument("--served-rates", type=float, nargs="*", default=[100, 300, 1000, 2000])
+    ap.add_argument("--served-rates", type=float, nargs="*", default=[10, 100, 300, 1000, 2000])
     ap.add_argument("--served-seconds", type=float, default=1.0)
     ap.add_argument("--batched-reps", type=int, default=5)
     ap.add_argument("--no-served", action="store_true")

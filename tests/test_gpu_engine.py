@@ -88,3 +88,26 @@ def test_served_trace_results_match_single_queries(dev):
             s, r, d = streams[vol[i]]
             cache[key] = _oracle(s, r, d, float(iso[i]))[0]
         assert out["num_vertices"][i] == cache[key], i
+
+
+def test_result_is_the_same_alone_and_in_the_mix(dev):
+    """A query's vertices are bit-identical whether it runs alone or among
+    other queries of other volumes (and after the pool has been churned)."""
+    streams = _streams()
+    eng = bcmc_amd.Engine(dev, 100000 * 256)
+    ids = [eng.add_volume(s, r, d) for s, r, d in streams]
+    probe_v, probe_iso = ids[1], 0.47
+    alone = eng.run([probe_v], [probe_iso])[0]
+    want = eng.vertices(alone).copy()
+    rng = np.random.default_rng(5)
+    for trial in range(3):
+        nq = 9
+        qv = list(rng.integers(0, len(ids), nq))
+        qi = list(rng.uniform(0.15, 0.85, nq).astype(np.float32))
+        pos = int(rng.integers(0, nq + 1))
+        qv.insert(pos, probe_v)
+        qi.insert(pos, np.float32(probe_iso))
+        res = eng.run(qv, qi)
+        got = eng.vertices(res[pos])
+        assert res[pos]["num_vertices"] == alone["num_vertices"], trial
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), trial

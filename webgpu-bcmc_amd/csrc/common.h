@@ -61,7 +61,7 @@ struct bcmc_ctx {
     // launch shapes: persistent workgroups per CU of each kernel family
     // (bcmc_ctx_autotune measures and sets them; results never depend on them)
     struct {
-        uint32_t scan = 4, sort = 4, decode = 8, mc = 4;
+        uint32_t scan = 4, sort = 4, decode = 8, mc = 5, count = 8;
     } tune;
     // bumped on every change of `tune`: captured graphs keyed on it re-capture
     uint64_t tune_gen = 0;
@@ -197,95 +197,126 @@ BCMC_DEV unsigned long long wave_sum_u64(unsigned long long v) {
     return v;
 }
 
-// Decoupled look-back across tiles, done by wave 0 of the workgroup.  Each
-// round reads the statuses of the 64 nearest unvisited predecessors at once
-// (one agent-scope load per lane): if every predecessor up to and including
-// the closest inclusive one is ready, their values are summed and the walk
-// ends at that inclusive one; with no inclusive status in the window (all
-// aggregates) the window moves back 64 tiles.  Returns the exclusive prefix
-// of the tile (status values are exact below 2^40) and publishes the tile's
-// inclusive value.
-// The look-back walk of one channel, run by one whole wave: returns the
-// exclusive prefix of `tile` and publishes its inclusive value.
-BCMC_DEV unsigned long long lookback_wave(const ScanSite &site, uint32_t tile, uint32_t epoch,
-                                          uint32_t aggregate) {
-    const uint32_t lane = lane_id();
-    const uint32_t ep = epoch & 0x3FFFFF;
-    unsigned long long prefix = 0;
+// Decoupled look-back across tiles by the whole workgroup, for one or two
+// channels.  Each step reads the statuses of the NT nearest unvisited
+// predecessors at once (one agent-scope load per thread and channel): if
+// every predecessor up to and including the closest inclusive one is ready,
+// their values are summed and the walk ends at that inclusive one; with no
+// inclusive status in the window (all aggregates) the window moves back NT
+// tiles; when a needed predecessor has not published yet the step is read
+// again.  Positions before tile 0 count as an inclusive zero.  Returns the
+// exclusive prefixes of the tile (status values are exact below 2^40) and
+// publishes the tile's inclusive values.  Called by every thread of the
+// workgroup; all decisions are taken from LDS, so control flow is uniform.
+template <int NT>
+struct LookbackLds {
+    unsigned long long incl[2][NT / 64], wait[2][NT / 64], part[2][NT / 64];
+};
+
+template <int NT, int NCH>
+BCMC_DEV void block_lookback(const ScanSite &s0, const ScanSite &s1, uint32_t tile, uint32_t e0,
+                             uint32_t e1, uint32_t agg0, uint32_t agg1, LookbackLds<NT> &sh,
+                             unsigned long long &p0, unsigned long long &p1) {
+    static_assert(NCH == 1 || NCH == 2, "one or two channels");
+    constexpr int NW = NT / 64;
+    constexpr uint32_t kWindow = NT;
+    const ScanSite *site[2] = {&s0, &s1};
+    const uint32_t ep[2] = {e0 & 0x3FFFFF, e1 & 0x3FFFFF};
+    const uint32_t agg[2] = {agg0, agg1};
+    const uint32_t lane = lane_id(), wave = threadIdx.x / 64, pos = threadIdx.x;
+    unsigned long long prefix[2] = {0, 0};
     if (tile == 0) {
-        if (lane == 0)
-            st_agent_u64(&site.status[0], pack_status(ep, kFlagInclusive, aggregate));
-        return 0;
-    }
-    if (lane == 0)
-        st_agent_u64(&site.status[tile], pack_status(ep, kFlagAggregate, aggregate));
-    int64_t base = (int64_t)tile - 1;
-    for (;;) {
-        const int64_t t = base - (int64_t)lane;
-        unsigned long long s = 0;
-        uint32_t flag = kFlagInclusive;  // before tile 0: an inclusive zero
-        if (t >= 0) {
-            s = ld_agent_u64(&site.status[t]);
-            flag = ((uint32_t)(s >> 42) == ep) ? (uint32_t)(s >> 40) & 3u : 0u;
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int c = 0; c < NCH; ++c)
+                st_agent_u64(&site[c]->status[0], pack_status(ep[c], kFlagInclusive, agg[c]));
         }
-        const uint64_t incl = __ballot(flag == kFlagInclusive);
-        const uint64_t wait = __ballot(flag == 0);
-        const uint32_t first = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
-        const uint64_t need = first == 64 ? ~0ull : ((2ull << first) - 1ull);
-        if (wait & need) {
+        p0 = p1 = 0;
+        return;
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            st_agent_u64(&site[c]->status[tile], pack_status(ep[c], kFlagAggregate, agg[c]));
+    }
+    bool done[2] = {false, NCH < 2};
+    int64_t base = (int64_t)tile - 1;  // the tile at window position 0
+    while (!(done[0] && done[1])) {
+        const int64_t t = base - (int64_t)pos;
+        unsigned long long s[2] = {0, 0};
+        uint32_t flag[2] = {kFlagInclusive, kFlagInclusive};
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            if (!done[c] && t >= 0) {
+                s[c] = ld_agent_u64(&site[c]->status[t]);
+                flag[c] = ((uint32_t)(s[c] >> 42) == ep[c]) ? (uint32_t)(s[c] >> 40) & 3u : 0u;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const uint64_t incl = __ballot(flag[c] == kFlagInclusive);
+            const uint64_t wait = __ballot(flag[c] == 0);
+            if (lane == 0) {
+                sh.incl[c][wave] = incl;
+                sh.wait[c][wave] = wait;
+            }
+        }
+        __syncthreads();
+        uint32_t first[2] = {kWindow, kWindow};
+        bool retry = false;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            if (done[c])
+                continue;
+#pragma unroll
+            for (int w = NW - 1; w >= 0; --w) {
+                const uint64_t m = sh.incl[c][w];
+                if (m)
+                    first[c] = 64u * w + (uint32_t)__builtin_ctzll(m);
+            }
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                const uint32_t lo = 64u * w;
+                const uint64_t need = first[c] >= lo + 64 ? ~0ull
+                                      : first[c] < lo     ? 0ull
+                                                          : ((2ull << (first[c] - lo)) - 1ull);
+                retry |= (sh.wait[c][w] & need) != 0;
+            }
+        }
+        if (retry) {
+            __syncthreads();
             __builtin_amdgcn_s_sleep(1);
             continue;
         }
-        const unsigned long long v = (lane <= first && t >= 0) ? (s & kValueMask) : 0ull;
-        prefix += wave_sum_u64(v);
-        if (first < 64)
-            break;
-        base -= 64;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            if (done[c])
+                continue;
+            const unsigned long long v = (pos <= first[c] && t >= 0) ? (s[c] & kValueMask) : 0ull;
+            const unsigned long long ws = wave_sum_u64(v);
+            if (lane == 0)
+                sh.part[c][wave] = ws;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            if (done[c])
+                continue;
+#pragma unroll
+            for (int w = 0; w < NW; ++w)
+                prefix[c] += sh.part[c][w];
+            done[c] = first[c] < kWindow;
+        }
+        base -= kWindow;
+        __syncthreads();
     }
-    if (lane == 0)
-        st_agent_u64(&site.status[tile], pack_status(ep, kFlagInclusive, prefix + aggregate));
-    return prefix;
-}
-
-// Decoupled look-back across tiles, done by wave 0 of the workgroup.  Each
-// round reads the statuses of the 64 nearest unvisited predecessors at once
-// (one agent-scope load per lane): if every predecessor up to and including
-// the closest inclusive one is ready, their values are summed and the walk
-// ends at that inclusive one; with no inclusive status in the window (all
-// aggregates) the window moves back 64 tiles.  Returns the exclusive prefix
-// of the tile (status values are exact below 2^40) and publishes the tile's
-// inclusive value.
-template <int NT>
-BCMC_DEV unsigned long long tile_lookback(const ScanSite &site, uint32_t tile, uint32_t epoch,
-                                          uint32_t aggregate, unsigned long long *lds_prefix) {
-    if (threadIdx.x < 64) {
-        const unsigned long long p = lookback_wave(site, tile, epoch, aggregate);
-        if (threadIdx.x == 0)
-            *lds_prefix = p;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            st_agent_u64(&site[c]->status[tile], pack_status(ep[c], kFlagInclusive, prefix[c] + agg[c]));
     }
-    __syncthreads();
-    const unsigned long long p = *lds_prefix;
-    __syncthreads();
-    return p;
-}
-
-// Two channels at once: wave 0 walks channel 0, wave 1 channel 1.
-template <int NT>
-BCMC_DEV void tile_lookback2(const ScanSite &s0, const ScanSite &s1, uint32_t tile, uint32_t e0, uint32_t e1,
-                             uint32_t agg0, uint32_t agg1, unsigned long long *lds_prefix /* [2] */,
-                             unsigned long long &p0, unsigned long long &p1) {
-    static_assert(NT >= 128, "two look-back waves");
-    const uint32_t wave = threadIdx.x / 64;
-    if (wave < 2) {
-        const unsigned long long p = wave == 0 ? lookback_wave(s0, tile, e0, agg0)
-                                               : lookback_wave(s1, tile, e1, agg1);
-        if ((threadIdx.x & 63u) == 0)
-            lds_prefix[wave] = p;
-    }
-    __syncthreads();
-    p0 = lds_prefix[0];
-    p1 = lds_prefix[1];
-    __syncthreads();
+    p0 = prefix[0];
+    p1 = prefix[1];
 }
 
 // Dynamic tile index for persistent workgroups (broadcast through LDS).

@@ -322,7 +322,7 @@ __global__ void eng_commit_kernel(EngCounters *c, uint32_t capacity) {
     c->used += min(room, c->num_miss);
 }
 
-struct EngFusedArgs {
+struct EngMcArgs {
     const EngQuery *qs;
     const EngVol *vols;
     const uint32_t *act_q, *act_b;
@@ -330,15 +330,17 @@ struct EngFusedArgs {
     const float *pool;
     uint32_t capacity;
     EngCounters *c;
-    unsigned long long *q_vfirst;  // per query: vertex offset of its first active pair
-    unsigned long long *vert_offsets;
+    uint32_t *counts;                    // per pair: vertices
+    uint32_t *chunk_verts;               // per chunk: vertices
+    unsigned long long *chunk_verts_pre; // exclusive prefixes of chunk_verts
+    unsigned long long *q_vfirst;        // per query: vertex offset of its first active pair
     void *verts;
     unsigned long long vert_cap;
     int fmt;
 };
 
 // The RowBlock of a (query, block) pair: its query's volume, flags, isovalue.
-BCMC_DEV void eng_row_block(const EngFusedArgs &a, uint32_t qi, uint32_t bid, bool live, uint32_t epoch,
+BCMC_DEV void eng_row_block(const EngMcArgs &a, uint32_t qi, uint32_t bid, bool live, uint32_t epoch,
                             RowBlock &b) {
     const EngQuery &q = a.qs[qi];
     const EngVol &v = a.vols[q.vol];
@@ -361,27 +363,22 @@ constexpr int kEngThreads = kMcThreads;
 // Blocks of all queries of one step (u32 indices into the concatenated arrays).
 constexpr uint64_t kMaxStepBlocks = 1ull << 31;
 
-// The chunked count / look-back / emit pass of mc.hip over (query, block)
-// pairs of many queries; the vertex channel's offsets are global, and each
-// query's vertices are contiguous because pairs are in query order.
-__global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, EngFusedArgs a) {
+// The count / prefix / emit launches of mc.hip over (query, block) pairs of
+// many queries; vertex offsets are global, and each query's vertices are
+// contiguous because pairs are in query order.  A chunk inside one query (the
+// common case) uses workgroup-uniform volume views; chunks across query
+// boundaries use per-lane views.
+__global__ __launch_bounds__(kEngThreads) void eng_count_kernel(EngMcArgs a) {
     constexpr int NW = kEngThreads / 64;
-    __shared__ ChunkShared cs;
+    __shared__ CountShared cs;
     __shared__ uint32_t lds_w[NW + 1];
-    __shared__ unsigned long long lds_prefix;
-    __shared__ uint32_t lds_tile;
-    __shared__ McTables tab;
-    __shared__ EmitScratch es[NW];
-    load_mc_tables(&tab);
+    __shared__ CountTables tab;
+    load_count_tables(&tab);
     const uint32_t n = a.c->num_active, epoch = a.c->epoch;
-    const uint32_t ntiles = div_up(n, kChunk);
-    const uint32_t e0 = scan_site_epoch(s0);
+    const uint32_t nchunks = div_up(n, kChunk);
     const uint32_t wave = threadIdx.x / 64;
-    for (;;) {
-        const uint32_t tile = next_tile(s0, &lds_tile);
-        if (tile >= ntiles)
-            break;
-        const uint32_t k0 = tile * kChunk;
+    for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+        const uint32_t k0 = chunk * kChunk;
         const bool in = k0 + threadIdx.x < n;
         const uint32_t bid = in ? a.act_b[k0 + threadIdx.x] : 0u, qi = in ? a.act_q[k0 + threadIdx.x] : 0u;
         float2 fwd = make_float2(1.0f, 0.0f);
@@ -392,53 +389,91 @@ __global__ __launch_bounds__(kEngThreads) void eng_fused_kernel(ScanSite s0, Eng
             fwd = a.vols[q.vol].fwd[bid];
         }
         const uint32_t ncand = chunk_prologue(cs, in, bid, qi, fwd, iso, lds_w);
-        // a chunk inside one query (the common case) uses workgroup-uniform
-        // volume views; chunks across query boundaries use per-lane views
         const uint32_t q_first = (uint32_t)__builtin_amdgcn_readfirstlane((int)cs.aux[0]);
         const uint32_t q_last = (uint32_t)__builtin_amdgcn_readfirstlane((int)cs.aux[min(n - k0, kChunk) - 1]);
-        RowBlock ub;
-        eng_row_block(a, q_first, 0, true, epoch, ub);
-        auto setup_one = [&](uint32_t e, RowBlock &b) {
-            b = ub;
-            b.live = k0 + e < n;
-            b.bid = cs.bid[e];
-        };
-        auto setup = [&](uint32_t e, RowBlock &b) {
-            eng_row_block(a, cs.aux[e], cs.bid[e], k0 + e < n, epoch, b);
-        };
-        const bool one_query = q_first == q_last;
-        if (one_query)
-            count_sweep(cs, ncand, wave, tab, setup_one);
-        else
-            count_sweep(cs, ncand, wave, tab, setup);
+        if (q_first == q_last) {
+            RowBlock ub;
+            eng_row_block(a, q_first, 0, true, epoch, ub);
+            count_sweep(cs, ncand, wave, tab, [&](uint32_t e, RowBlock &b) {
+                b = ub;
+                b.live = k0 + e < n;
+                b.bid = cs.bid[e];
+            });
+        } else {
+            count_sweep(cs, ncand, wave, tab, [&](uint32_t e, RowBlock &b) {
+                eng_row_block(a, cs.aux[e], cs.bid[e], k0 + e < n, epoch, b);
+            });
+        }
         __syncthreads();
         const uint32_t c = cs.cnt[threadIdx.x];
-        uint32_t agg_h, agg_v;
-        const uint32_t ex_h = block_excl_scan<kEngThreads>(c ? 1u : 0u, lds_w, &agg_h);
-        const uint32_t ex_v = block_excl_scan<kEngThreads>(c, lds_w, &agg_v);
-        const unsigned long long pv = tile_lookback<kEngThreads>(s0, tile, e0, agg_v, &lds_prefix);
-        if (c) {
-            cs.list[ex_h] = threadIdx.x;
-            cs.off[ex_h] = ex_v;
-        }
-        // the first pair of each query records where the query's vertices start
+        if (in)
+            a.counts[k0 + threadIdx.x] = c;
+        uint32_t agg_v;
+        block_excl_scan<kEngThreads>(c, lds_w, &agg_v);
+        if (threadIdx.x == 0)
+            a.chunk_verts[chunk] = agg_v;
+    }
+}
+
+struct EngChunkPrefixOp {
+    const uint32_t *verts;
+    unsigned long long *verts_pre;
+    EngCounters *c;
+    BCMC_DEV uint32_t size() const { return div_up(c->num_active, kChunk); }
+    BCMC_DEV void begin() const {}
+    BCMC_DEV void value(uint32_t i, uint32_t (&v)[1]) const { v[0] = verts[i]; }
+    BCMC_DEV void emit(uint32_t i, const uint32_t (&)[1], const unsigned long long (&e)[1]) const {
+        verts_pre[i] = e[0];
+    }
+    BCMC_DEV void total(const unsigned long long (&t)[1]) const { c->num_verts = t[0]; }
+};
+
+__global__ __launch_bounds__(kEngThreads) __attribute__((amdgpu_waves_per_eu(5))) void eng_emit_kernel(EngMcArgs a) {
+    constexpr int NW = kEngThreads / 64;
+    __shared__ EmitShared cs;
+    __shared__ uint32_t lds_w[NW + 1];
+    __shared__ McTables tab;
+    __shared__ EmitScratch es[NW];
+    load_mc_tables(&tab);
+    const uint32_t n = a.c->num_active, epoch = a.c->epoch;
+    const uint32_t nchunks = div_up(n, kChunk);
+    const uint32_t wave = threadIdx.x / 64;
+    for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+        const uint32_t k0 = chunk * kChunk;
         const uint32_t k = k0 + threadIdx.x;
-        if (k < n) {
-            const uint32_t qi = cs.aux[threadIdx.x];
+        const bool in = k < n;
+        const uint32_t c = in ? a.counts[k] : 0u;
+        const uint32_t bid = in ? a.act_b[k] : 0u, qi = in ? a.act_q[k] : 0u;
+        uint32_t ex_h, ex_v, agg_v;
+        const uint32_t num_with = emit_prologue(cs, c, bid, qi, lds_w, &ex_h, &ex_v, &agg_v);
+        const unsigned long long pv = a.chunk_verts_pre[chunk];
+        // the first pair of each query records where the query's vertices start
+        if (in) {
             const uint32_t prev = threadIdx.x ? cs.aux[threadIdx.x - 1] : (k ? a.act_q[k - 1] : ~0u);
             if (qi != prev)
                 a.q_vfirst[qi] = pv + ex_v;
         }
-        if (tile == ntiles - 1 && threadIdx.x == 0)
-            a.c->num_verts = pv + agg_v;
-        __syncthreads();
-        if (one_query)
-            emit_sweep(cs, wave, agg_h, pv, tab, es[wave], a.vert_cap, a.verts, a.fmt, setup_one);
-        else
-            emit_sweep(cs, wave, agg_h, pv, tab, es[wave], a.vert_cap, a.verts, a.fmt, setup);
+        if (num_with == 0)
+            continue;
+        const uint32_t q_first = (uint32_t)__builtin_amdgcn_readfirstlane((int)cs.aux[0]);
+        const uint32_t q_last = (uint32_t)__builtin_amdgcn_readfirstlane((int)cs.aux[min(n - k0, kChunk) - 1]);
+        if (q_first == q_last) {
+            RowBlock ub;
+            eng_row_block(a, q_first, 0, true, epoch, ub);
+            auto setup = [&](uint32_t e, RowBlock &b) {
+                b = ub;
+                b.live = k0 + e < n;
+                b.bid = cs.bid[e];
+            };
+            emit_sweep(cs, wave, num_with, pv, tab, es[wave], a.vert_cap, a.verts, a.fmt, setup);
+        } else {
+            auto setup = [&](uint32_t e, RowBlock &b) {
+                eng_row_block(a, cs.aux[e], cs.bid[e], k0 + e < n, epoch, b);
+            };
+            emit_sweep(cs, wave, num_with, pv, tab, es[wave], a.vert_cap, a.verts, a.fmt, setup);
+        }
         __syncthreads();
     }
-    scan_site_finish(s0, e0);
 }
 
 static uint32_t vbytes(int fmt) {
@@ -475,7 +510,8 @@ struct bcmc_engine {
     // per-step arrays (grow)
     uint64_t blocks_cap = 0;
     uint32_t *flags = nullptr, *act_q = nullptr, *act_b = nullptr;
-    unsigned long long *vert_offsets = nullptr;
+    uint32_t *counts = nullptr, *chunk_verts = nullptr;
+    unsigned long long *chunk_verts_pre = nullptr;
     uint64_t rows_cap = 0, row_elems_cap = 0;
     uint32_t *cand_q = nullptr, *cand_seg = nullptr;
     ScanSite site_rows;
@@ -483,7 +519,7 @@ struct bcmc_engine {
     EngQuery *d_q = nullptr;
     uint32_t *q_act_start = nullptr;
     unsigned long long *q_vfirst = nullptr;
-    ScanSite site_active, site_cand, site_mc;
+    ScanSite site_active, site_cand, site_chunks;
     void *verts = nullptr;
     uint64_t vert_cap = 0;
     int fmt = BCMC_VERTS_LOCAL_F32;
@@ -500,8 +536,10 @@ static int eng_ensure_blocks(bcmc_engine *e, uint64_t total) {
     dev_free(e->flags);
     dev_free(e->act_q);
     dev_free(e->act_b);
-    dev_free(e->vert_offsets);
-    scan_site_free(&e->site_mc);
+    dev_free(e->counts);
+    dev_free(e->chunk_verts);
+    dev_free(e->chunk_verts_pre);
+    scan_site_free(&e->site_chunks);
     const uint64_t cap = std::max<uint64_t>(total, e->blocks_cap * 3 / 2);
     BCMC_TRY(dev_alloc((void **)&e->flags, cap * 4));
     // flag words hold the epoch of the step that found the block active;
@@ -509,8 +547,11 @@ static int eng_ensure_blocks(bcmc_engine *e, uint64_t total) {
     BCMC_HIP_CHECK(hipMemsetAsync(e->flags, 0, cap * 4, e->ctx->stream));
     BCMC_TRY(dev_alloc((void **)&e->act_q, cap * 4));
     BCMC_TRY(dev_alloc((void **)&e->act_b, cap * 4));
-    BCMC_TRY(dev_alloc((void **)&e->vert_offsets, 8));
-    BCMC_TRY(scan_site_alloc(&e->site_mc, cap, kChunk));
+    BCMC_TRY(dev_alloc((void **)&e->counts, cap * 4));
+    const uint64_t nch = div_up(cap, kChunk);
+    BCMC_TRY(dev_alloc((void **)&e->chunk_verts, nch * 4));
+    BCMC_TRY(dev_alloc((void **)&e->chunk_verts_pre, nch * 8));
+    BCMC_TRY(scan_site_alloc(&e->site_chunks, nch, kScanTile));
     e->blocks_cap = cap;
     return BCMC_OK;
 }
@@ -557,8 +598,8 @@ static uint32_t eng_grid(const bcmc_ctx *ctx, uint64_t n, uint32_t per_cu = 8) {
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(div_up(n, 256), persistent_grid(ctx, per_cu)));
 }
 
-static int eng_launch_fused(bcmc_engine *e, uint32_t nq, uint64_t total) {
-    EngFusedArgs a;
+static EngMcArgs eng_mc_args(bcmc_engine *e) {
+    EngMcArgs a;
     a.qs = e->d_q;
     a.vols = e->d_vols;
     a.act_q = e->act_q;
@@ -567,16 +608,34 @@ static int eng_launch_fused(bcmc_engine *e, uint32_t nq, uint64_t total) {
     a.pool = e->pool;
     a.capacity = e->pool_cap;
     a.c = e->c;
+    a.counts = e->counts;
+    a.chunk_verts = e->chunk_verts;
+    a.chunk_verts_pre = e->chunk_verts_pre;
     a.q_vfirst = e->q_vfirst;
-    a.vert_offsets = e->vert_offsets;
     a.verts = e->verts;
     a.vert_cap = e->vert_cap;
     a.fmt = e->fmt;
+    return a;
+}
+
+static int eng_launch_emit(bcmc_engine *e, uint64_t total) {
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(total, kChunk),
                                                                    persistent_grid(e->ctx, e->ctx->tune.mc)));
-    eng_fused_kernel<<<grid, kEngThreads, 0, e->ctx->stream>>>(e->site_mc, a);
+    eng_emit_kernel<<<grid, kEngThreads, 0, e->ctx->stream>>>(eng_mc_args(e));
     BCMC_HIP_CHECK(hipGetLastError());
     return BCMC_OK;
+}
+
+// count -> chunk prefixes -> emit over the step's active pairs
+static int eng_launch_mc(bcmc_engine *e, uint64_t total) {
+    const EngMcArgs a = eng_mc_args(e);
+    const uint32_t max_chunks = (uint32_t)div_up(total, kChunk);
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(max_chunks,
+                                                                   persistent_grid(e->ctx, e->ctx->tune.count)));
+    eng_count_kernel<<<grid, kEngThreads, 0, e->ctx->stream>>>(a);
+    launch_scan<1>(e->ctx, e->site_chunks, e->site_chunks,
+                   EngChunkPrefixOp{a.chunk_verts, a.chunk_verts_pre, a.c}, max_chunks);
+    return eng_launch_emit(e, total);
 }
 
 extern "C" {
@@ -655,14 +714,16 @@ BCMC_API int bcmc_engine_destroy(bcmc_engine *e) {
     dev_free(e->cand_q);
     dev_free(e->cand_seg);
     scan_site_free(&e->site_rows);
-    dev_free(e->vert_offsets);
+    dev_free(e->counts);
+    dev_free(e->chunk_verts);
+    dev_free(e->chunk_verts_pre);
     dev_free(e->d_q);
     dev_free(e->q_act_start);
     dev_free(e->q_vfirst);
     dev_free(e->verts);
     scan_site_free(&e->site_active);
     scan_site_free(&e->site_cand);
-    scan_site_free(&e->site_mc);
+    scan_site_free(&e->site_chunks);
     if (e->ch)
         (void)hipHostFree(e->ch);
     delete e;
@@ -816,7 +877,7 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
         launch_decode_to_cache(ctx, v.stream, v.nwords, v.maxbits, v.miss_list, &e->c->vol_miss[vi],
                                v.n, v.slot_map, e->pool, e->pool_cap);
     }
-    BCMC_TRY(eng_launch_fused(e, nq, total));
+    BCMC_TRY(eng_launch_mc(e, total));
     BCMC_HIP_CHECK(hipMemcpyAsync(e->ch, e->c, sizeof(EngCounters), hipMemcpyDeviceToHost, st));
     std::vector<unsigned long long> qv(nq);
     std::vector<uint32_t> qa(nq);
@@ -836,7 +897,7 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
         e->vert_cap = 0;
         BCMC_TRY(dev_alloc(&e->verts, want * vbytes(e->fmt)));
         e->vert_cap = want;
-        BCMC_TRY(eng_launch_fused(e, nq, total));
+        BCMC_TRY(eng_launch_emit(e, total));
         BCMC_HIP_CHECK(hipStreamSynchronize(st));
     }
     // a query's vertices run from its first active pair's offset to the next

@@ -45,7 +45,7 @@ template <int NCH, class Op>
 __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanSite s0, ScanSite s1, Op op) {
     constexpr int NW = kScanThreads / 64;
     __shared__ uint32_t lds_w[NCH][NW + 1];
-    __shared__ unsigned long long lds_prefix[NCH];
+    __shared__ LookbackLds<kScanThreads> lds_lb;
     __shared__ uint32_t lds_tile;
     const ScanSite *sites[2] = {&s0, &s1};
     const uint32_t n = op.size();
@@ -113,11 +113,13 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanSite s0, ScanSit
             }
         }
         __syncthreads();
-        if constexpr (NCH == 2) {
-            tile_lookback2<kScanThreads>(s0, s1, tile, epoch[0], epoch[1], agg[0], agg[1], lds_prefix,
-                                         prefix[0], prefix[1]);
-        } else {
-            prefix[0] = tile_lookback<kScanThreads>(s0, tile, epoch[0], agg[0], &lds_prefix[0]);
+        {
+            unsigned long long q[2];
+            block_lookback<kScanThreads, NCH>(s0, NCH == 2 ? s1 : s0, tile, epoch[0], epoch[NCH - 1], agg[0],
+                                              agg[NCH - 1], lds_lb, q[0], q[1]);
+#pragma unroll
+            for (int c = 0; c < NCH; ++c)
+                prefix[c] = q[c];
         }
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {

@@ -63,17 +63,18 @@ struct ActiveOp {
     }
 };
 
-// ---- count + offsets + emission in one pass --------------------------------
-// Workgroups take chunks of 256 consecutive active blocks.  Sweep 1: each wave
-// counts the vertices of 4 blocks at a time (one per 16-lane row) into LDS.
-// The chunk's two totals (blocks with vertices, vertices) chain through two
-// look-back channels, which gives every block its exact 64-bit vertex offset
-// and its index in the list of blocks with vertices.  Sweep 2: each wave takes
-// 4 consecutive blocks of that list, reloads them (mostly from L2) and writes
+// ---- counts, offsets and vertices -------------------------------------------
+// Three launches over chunks of 256 consecutive active blocks.  Count: each
+// wave counts the vertices of 4 blocks at a time (one per 16-lane row); every
+// block's count and every chunk's two totals (blocks with vertices, vertices)
+// go to memory.  Prefix: a two-channel scan of the chunk totals.  Emit: each
+// chunk's list of blocks with vertices (its slice of the reference's
+// compacted list) and their 64-bit vertex offsets, then each wave takes 4
+// consecutive blocks of that list, reloads them (mostly from L2) and writes
 // their vertices as one contiguous run.  Vertices past the buffer capacity are
-// not written; the host grows the buffer and runs the pass again when the
-// total exceeds the capacity.
-struct FusedArgs {
+// not written; the host grows the buffer and runs the emit launch again when
+// the total exceeds the capacity.  No chunk waits for another.
+struct McArgs {
     const float2 *fwd;  // per block: forward range (launch_forward_ranges)
     const float *cache;
     const int32_t *item_slot;
@@ -83,6 +84,11 @@ struct FusedArgs {
     uint32_t cache_cap;
     const uint32_t *active_ids;
     QState *q;
+    uint32_t *counts;         // per active block: vertices
+    uint32_t *chunk_with;     // per chunk: blocks with vertices
+    uint32_t *chunk_verts;    // per chunk: vertices
+    uint32_t *chunk_with_pre;           // exclusive prefixes of the chunk totals
+    unsigned long long *chunk_verts_pre;
     uint32_t *with_verts;
     unsigned long long *vert_offsets;
     void *verts;
@@ -90,76 +96,116 @@ struct FusedArgs {
     int fmt;
 };
 
-__global__ __launch_bounds__(kMcThreads) void mc_fused_kernel(ScanSite s0, ScanSite s1, FusedArgs a) {
+BCMC_DEV RowBlock mc_row_base(const McArgs &a) {
+    RowBlock b;
+    b.cache = a.cache;
+    b.item_slot = a.item_slot;
+    b.flags = a.flags;
+    b.nb[0] = a.nb[0];
+    b.nb[1] = a.nb[1];
+    b.nb[2] = a.nb[2];
+    b.div_x = a.div_x;
+    b.div_y = a.div_y;
+    b.cap = a.cache_cap;
+    b.tag = 1;
+    b.iso = a.q->iso;
+    b.bid = 0;
+    b.live = false;
+    return b;
+}
+
+__global__ __launch_bounds__(kMcThreads) void mc_count_kernel(McArgs a) {
     constexpr int NW = kMcThreads / 64;
-    __shared__ ChunkShared cs;
+    __shared__ CountShared cs;
     __shared__ uint32_t lds_w[NW + 1];
-    __shared__ unsigned long long lds_prefix[2];
-    __shared__ uint32_t lds_tile;
-    __shared__ McTables tab;
-    __shared__ EmitScratch es[NW];
-    load_mc_tables(&tab);
+    __shared__ CountTables tab;
+    load_count_tables(&tab);
     const uint32_t n = a.q->num_active;
-    const uint32_t ntiles = div_up(n, kChunk);
-    const uint32_t e0 = scan_site_epoch(s0), e1 = scan_site_epoch(s1);
+    const uint32_t nchunks = div_up(n, kChunk);
     const uint32_t wave = threadIdx.x / 64;
-    RowBlock base;
-    base.cache = a.cache;
-    base.item_slot = a.item_slot;
-    base.flags = a.flags;
-    base.nb[0] = a.nb[0];
-    base.nb[1] = a.nb[1];
-    base.nb[2] = a.nb[2];
-    base.div_x = a.div_x;
-    base.div_y = a.div_y;
-    base.cap = a.cache_cap;
-    base.tag = 1;
-    base.iso = a.q->iso;
-    if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
-        a.q->num_with = 0;
-        a.q->num_verts = 0;
-    }
-    for (;;) {
-        const uint32_t tile = next_tile(s0, &lds_tile);
-        if (tile >= ntiles)
-            break;
-        const uint32_t k0 = tile * kChunk;
+    const RowBlock base = mc_row_base(a);
+    for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+        const uint32_t k0 = chunk * kChunk;
         const bool in = k0 + threadIdx.x < n;
         const uint32_t bid = in ? a.active_ids[k0 + threadIdx.x] : 0u;
         const uint32_t ncand = chunk_prologue(cs, in, bid, 0u, in ? a.fwd[bid] : make_float2(1.0f, 0.0f),
                                               base.iso, lds_w);
+        count_sweep(cs, ncand, wave, tab, [&](uint32_t e, RowBlock &b) {
+            b = base;
+            b.live = k0 + e < n;
+            b.bid = cs.bid[e];
+        });
+        __syncthreads();
+        const uint32_t c = cs.cnt[threadIdx.x];
+        if (in)
+            a.counts[k0 + threadIdx.x] = c;
+        uint32_t agg_h, agg_v;
+        block_excl_scan<kMcThreads>(c ? 1u : 0u, lds_w, &agg_h);
+        block_excl_scan<kMcThreads>(c, lds_w, &agg_v);
+        if (threadIdx.x == 0) {
+            a.chunk_with[chunk] = agg_h;
+            a.chunk_verts[chunk] = agg_v;
+        }
+    }
+}
+
+// Exclusive prefixes of the chunk totals; the grand totals are the query's
+// numBlocksWithVertices and numVertices.
+struct ChunkPrefixOp {
+    const uint32_t *with, *verts;
+    uint32_t *with_pre;
+    unsigned long long *verts_pre;
+    QState *q;
+    BCMC_DEV uint32_t size() const { return div_up(q->num_active, kChunk); }
+    BCMC_DEV void begin() const {}
+    BCMC_DEV void value(uint32_t i, uint32_t (&v)[2]) const {
+        v[0] = with[i];
+        v[1] = verts[i];
+    }
+    BCMC_DEV void emit(uint32_t i, const uint32_t (&)[2], const unsigned long long (&e)[2]) const {
+        with_pre[i] = (uint32_t)e[0];
+        verts_pre[i] = e[1];
+    }
+    BCMC_DEV void total(const unsigned long long (&t)[2]) const {
+        q->num_with = (uint32_t)t[0];
+        q->num_verts = t[1];
+    }
+};
+
+__global__ __launch_bounds__(kMcThreads) __attribute__((amdgpu_waves_per_eu(5))) void mc_emit_kernel(McArgs a) {
+    constexpr int NW = kMcThreads / 64;
+    __shared__ EmitShared cs;
+    __shared__ uint32_t lds_w[NW + 1];
+    __shared__ McTables tab;
+    __shared__ EmitScratch es[NW];
+    load_mc_tables(&tab);
+    const uint32_t n = a.q->num_active;
+    const uint32_t nchunks = div_up(n, kChunk);
+    const uint32_t wave = threadIdx.x / 64;
+    const RowBlock base = mc_row_base(a);
+    for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+        const uint32_t k0 = chunk * kChunk;
+        const bool in = k0 + threadIdx.x < n;
+        const uint32_t c = in ? a.counts[k0 + threadIdx.x] : 0u;
+        const uint32_t bid = in ? a.active_ids[k0 + threadIdx.x] : 0u;
+        uint32_t ex_h, ex_v, agg_v;
+        const uint32_t num_with = emit_prologue(cs, c, bid, 0u, lds_w, &ex_h, &ex_v, &agg_v);
+        if (num_with == 0)
+            continue;
+        const uint32_t ph = a.chunk_with_pre[chunk];
+        const unsigned long long pv = a.chunk_verts_pre[chunk];
+        if (c) {
+            a.with_verts[ph + ex_h] = k0 + threadIdx.x;
+            a.vert_offsets[ph + ex_h] = pv + ex_v;
+        }
         auto setup = [&](uint32_t e, RowBlock &b) {
             b = base;
             b.live = k0 + e < n;
             b.bid = cs.bid[e];
         };
-        // sweep 1: per-block vertex counts
-        count_sweep(cs, ncand, wave, tab, setup);
-        __syncthreads();
-        // chunk scan of (has vertices, vertices), then the two look-backs
-        const uint32_t c = cs.cnt[threadIdx.x];
-        uint32_t agg_h, agg_v;
-        const uint32_t ex_h = block_excl_scan<kMcThreads>(c ? 1u : 0u, lds_w, &agg_h);
-        const uint32_t ex_v = block_excl_scan<kMcThreads>(c, lds_w, &agg_v);
-        unsigned long long ph, pv;
-        tile_lookback2<kMcThreads>(s0, s1, tile, e0, e1, agg_h, agg_v, lds_prefix, ph, pv);
-        if (c) {
-            cs.list[ex_h] = threadIdx.x;
-            cs.off[ex_h] = ex_v;
-            a.with_verts[ph + ex_h] = k0 + threadIdx.x;
-            a.vert_offsets[ph + ex_h] = pv + ex_v;
-        }
-        if (tile == ntiles - 1 && threadIdx.x == 0) {
-            a.q->num_with = (uint32_t)(ph + agg_h);
-            a.q->num_verts = pv + agg_v;
-        }
-        __syncthreads();
-        // sweep 2: vertices of 4 consecutive blocks with vertices per wave
-        emit_sweep(cs, wave, agg_h, pv, tab, es[wave], a.capacity, a.verts, a.fmt, setup);
+        emit_sweep(cs, wave, num_with, pv, tab, es[wave], a.capacity, a.verts, a.fmt, setup);
         __syncthreads();
     }
-    scan_site_finish(s0, e0);
-    scan_site_finish(s1, e1);
 }
 
 static uint32_t vertex_bytes(int fmt) {
@@ -186,6 +232,10 @@ struct bcmc_mc {
     float2 *dilated = nullptr;  // per block: range over its 3x3x3 neighbourhood
     float2 *fwd = nullptr;      // per block: range of every voxel its cells can read
     unsigned long long *vert_offsets = nullptr;
+    // marching-cubes pass: per active block vertex counts, per chunk totals
+    // and their exclusive prefixes
+    uint32_t *counts = nullptr, *chunk_with = nullptr, *chunk_verts = nullptr, *chunk_with_pre = nullptr;
+    unsigned long long *chunk_verts_pre = nullptr;
     bcmc_lru lru;  // non-owning handle over the cache state held here
     QState *q = nullptr, *qh = nullptr;
     ScanSite site_active, site_c0, site_c1;
@@ -228,6 +278,11 @@ static void mc_free_volume(bcmc_mc *m) {
     dev_free(m->fwd);
     dev_free(m->with_verts);
     dev_free(m->vert_offsets);
+    dev_free(m->counts);
+    dev_free(m->chunk_with);
+    dev_free(m->chunk_verts);
+    dev_free(m->chunk_with_pre);
+    dev_free(m->chunk_verts_pre);
     dev_free(m->verts);
     scan_site_free(&m->site_active);
     scan_site_free(&m->site_c0);
@@ -239,6 +294,8 @@ static void mc_free_volume(bcmc_mc *m) {
     m->dilated = nullptr;
     m->fwd = nullptr;
     m->vert_offsets = nullptr;
+    m->counts = m->chunk_with = m->chunk_verts = m->chunk_with_pre = nullptr;
+    m->chunk_verts_pre = nullptr;
     m->verts = nullptr;
     m->vert_cap = 0;
     m->has_volume = false;
@@ -264,8 +321,8 @@ static void mc_refresh_views(bcmc_mc *m) {
     m->lru.cache_view.owned = false;
 }
 
-static int mc_launch_fused(bcmc_mc *m) {
-    FusedArgs a;
+static McArgs mc_args(bcmc_mc *m) {
+    McArgs a;
     a.fwd = m->fwd;
     a.cache = (const float *)m->lru.st.data;
     a.item_slot = m->lru.st.v.item_slot;
@@ -278,16 +335,38 @@ static int mc_launch_fused(bcmc_mc *m) {
     a.cache_cap = m->lru.st.capacity;
     a.active_ids = m->active_ids;
     a.q = m->q;
+    a.counts = m->counts;
+    a.chunk_with = m->chunk_with;
+    a.chunk_verts = m->chunk_verts;
+    a.chunk_with_pre = m->chunk_with_pre;
+    a.chunk_verts_pre = m->chunk_verts_pre;
     a.with_verts = m->with_verts;
     a.vert_offsets = m->vert_offsets;
     a.verts = m->verts;
     a.capacity = m->vert_cap;
     a.fmt = m->fmt;
+    return a;
+}
+
+static int mc_launch_emit(bcmc_mc *m) {
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(m->total, kChunk),
                                                                    persistent_grid(m->ctx, m->ctx->tune.mc)));
-    mc_fused_kernel<<<grid, kMcThreads, 0, m->ctx->stream>>>(m->site_c0, m->site_c1, a);
+    mc_emit_kernel<<<grid, kMcThreads, 0, m->ctx->stream>>>(mc_args(m));
     BCMC_HIP_CHECK(hipGetLastError());
     return BCMC_OK;
+}
+
+// count -> chunk prefixes -> emit
+static int mc_launch_pass(bcmc_mc *m) {
+    const McArgs a = mc_args(m);
+    const uint32_t max_chunks = div_up(m->total, kChunk);
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(max_chunks,
+                                                                   persistent_grid(m->ctx, m->ctx->tune.count)));
+    mc_count_kernel<<<grid, kMcThreads, 0, m->ctx->stream>>>(a);
+    launch_scan<2>(m->ctx, m->site_c0, m->site_c1,
+                   ChunkPrefixOp{a.chunk_with, a.chunk_verts, a.chunk_with_pre, a.chunk_verts_pre, a.q},
+                   max_chunks);
+    return mc_launch_emit(m);
 }
 
 // One query as a kernel chain on the context's stream (capturable when not
@@ -317,8 +396,8 @@ static int mc_enqueue_query(bcmc_mc *m, uint32_t npasses, bool timed) {
                          L.capacity, m->q);
     if (timed)
         BCMC_HIP_CHECK(hipEventRecord(m->ev[3], st));
-    // counts, offsets and vertices in one pass
-    BCMC_TRY(mc_launch_fused(m));
+    // counts, offsets and vertices
+    BCMC_TRY(mc_launch_pass(m));
     if (timed)
         BCMC_HIP_CHECK(hipEventRecord(m->ev[4], st));
     BCMC_HIP_CHECK(hipGetLastError());
@@ -408,8 +487,14 @@ BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *m, const void *stream, uint6
     BCMC_TRY(dev_alloc((void **)&m->with_verts, n * 4));
     BCMC_TRY(dev_alloc((void **)&m->vert_offsets, n * 8));
     BCMC_TRY(scan_site_alloc(&m->site_active, n, kScanTile));
-    BCMC_TRY(scan_site_alloc(&m->site_c0, n, kChunk));
-    BCMC_TRY(scan_site_alloc(&m->site_c1, n, kChunk));
+    const size_t nch = div_up((uint64_t)n, kChunk);
+    BCMC_TRY(dev_alloc((void **)&m->counts, n * 4));
+    BCMC_TRY(dev_alloc((void **)&m->chunk_with, nch * 4));
+    BCMC_TRY(dev_alloc((void **)&m->chunk_verts, nch * 4));
+    BCMC_TRY(dev_alloc((void **)&m->chunk_with_pre, nch * 4));
+    BCMC_TRY(dev_alloc((void **)&m->chunk_verts_pre, nch * 8));
+    BCMC_TRY(scan_site_alloc(&m->site_c0, nch, kScanTile));
+    BCMC_TRY(scan_site_alloc(&m->site_c1, nch, kScanTile));
     const uint32_t init = initial_cache_slots ? initial_cache_slots
                                               : (uint32_t)std::ceil((double)total * 0.05);
     BCMC_TRY(lru_state_alloc(ctx, &m->lru.st, init, 256, m->total));
@@ -504,7 +589,7 @@ BCMC_API int bcmc_mc_compute_surface(bcmc_mc *m, float isovalue, bcmc_perf *perf
         m->vert_cap = want;
         if (timed)
             BCMC_HIP_CHECK(hipEventRecord(m->ev[3], st));
-        BCMC_TRY(mc_launch_fused(m));
+        BCMC_TRY(mc_launch_emit(m));
         if (timed)
             BCMC_HIP_CHECK(hipEventRecord(m->ev[4], st));
         BCMC_HIP_CHECK(hipMemcpyAsync(m->qh, m->q, sizeof(QState), hipMemcpyDeviceToHost, st));

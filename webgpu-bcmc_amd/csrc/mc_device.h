@@ -63,7 +63,13 @@ struct EdgeInfo {
 };
 
 // Tables staged in LDS once per workgroup (the lookups are divergent, so
-// they would otherwise be vector loads through the L1).
+// they would otherwise be vector loads through the L1).  The count pass needs
+// only the vertex counts and the tile-dims rule.
+struct CountTables {
+    uint8_t count[256];  // by corner byte: vertices of the cell
+    uint8_t dims[1024];  // tile-dims rule, see tile_dims_rule
+};
+
 struct McTables {
     unsigned long long edges[256];  // by corner byte
     uint8_t count[256];             // by corner byte: vertices of the cell
@@ -94,6 +100,14 @@ BCMC_DEV uint32_t tile_dims_rule(uint32_t idx) {
     if (dz && !(a & 4))
         dz = false;
     return (dx ? 1u : 0u) | (dy ? 2u : 0u) | (dz ? 4u : 0u);
+}
+
+BCMC_DEV void load_count_tables(CountTables *t) {
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x)
+        t->count[i] = kTriCount[case_of_corner_byte(i)];
+    for (uint32_t i = threadIdx.x; i < 1024; i += blockDim.x)
+        t->dims[i] = (uint8_t)tile_dims_rule(i);
+    __syncthreads();
 }
 
 BCMC_DEV void load_mc_tables(McTables *t) {
@@ -226,14 +240,14 @@ BCMC_DEV RowMeta row_meta_load(const RowBlock &b) {
 
 // Stores the row's slots and tile dims into its block's meta words.  Must be
 // called by all 64 lanes.
-BCMC_DEV void row_meta_store(const RowMeta &m, uint32_t tag, const McTables &tab, uint32_t *meta) {
+BCMC_DEV void row_meta_store(const RowMeta &m, uint32_t tag, const uint8_t *dims, uint32_t *meta) {
     const uint32_t lane = lane_id(), l = lane & 15u;
     const uint64_t ab = __ballot(l >= 8 && l < 15 && (uint32_t)m.v == tag);
     const uint32_t a = (uint32_t)(ab >> ((lane & 48u) + 8u)) & 0x7Fu;
     if (l < 8)
         meta[l] = (uint32_t)m.v;
     if (l == 8)
-        meta[8] = tab.dims[a | (m.grid << 7)];
+        meta[8] = dims[a | (m.grid << 7)];
 }
 
 // Voxels held by a lane: its row and, on the upper faces, the ghost rows
@@ -328,10 +342,10 @@ BCMC_DEV uint32_t cell_byte(const CornerWord &cw) {
 }
 
 // Vertex count of the row's block, valid in lane 15 of the row.
-BCMC_DEV uint32_t row_block_count(const RowTile &t, float iso, const McTables &tab) {
+BCMC_DEV uint32_t row_block_count(const RowTile &t, float iso, const uint8_t *count) {
     const CornerWord cw = row_corner_word(t, iso);
-    const uint32_t n = (uint32_t)tab.count[cell_byte<0>(cw)] + tab.count[cell_byte<1>(cw)] +
-                       tab.count[cell_byte<2>(cw)] + tab.count[cell_byte<3>(cw)];
+    const uint32_t n = (uint32_t)count[cell_byte<0>(cw)] + count[cell_byte<1>(cw)] +
+                       count[cell_byte<2>(cw)] + count[cell_byte<3>(cw)];
     return row_incl_scan(n);
 }
 
@@ -344,6 +358,7 @@ struct alignas(16) EmitScratch {
     alignas(16) uint8_t start[kMaxGroupTris];  // at a cell's first triangle: the cell (0 elsewhere)
     uint32_t bid[kRowBlocks], bpos[kRowBlocks][3];
     float iso[kRowBlocks];
+    uint32_t meta[kRowBlocks][kMetaWords];  // meta words of the group's blocks
 };
 
 BCMC_DEV void row_store(float *dst, const float (&r)[5]) {
@@ -469,26 +484,39 @@ BCMC_DEV void emit_row_blocks(const RowBlock &b, const RowTile &t, const McTable
     wave_lds_sync();  // the scratch is reused by the next group
 }
 
-// LDS of a workgroup's chunk: block ids (and a caller word, e.g. the query)
-// per entry, the per-block meta words, the per-block vertex counts, the
-// candidate entries of the count sweep, and the list of entries with vertices
-// with their vertex offsets in the chunk.  Entry kChunk is a scratch entry for
-// rows without a block.
-struct ChunkShared {
+// The marching-cubes pass runs as three launches over chunks of kChunk
+// consecutive active blocks: a count pass (per-block vertex counts and
+// per-chunk totals), a scan of the chunk totals, and an emit pass that builds
+// each chunk's list of blocks with vertices and writes their vertices.
+
+// LDS of the count pass over one chunk: block ids and a caller word per
+// entry, per-block vertex counts, the candidate entries, and per wave the
+// meta words of the blocks its rows hold.  Entry kChunk is a scratch entry
+// for rows without a block.
+struct CountShared {
     uint32_t bid[kChunk + 1];
     uint32_t aux[kChunk + 1];
-    uint32_t meta[kChunk + 1][kMetaWords];
-    uint32_t cnt[kChunk], off[kChunk];
-    uint32_t list[kChunk];  // count sweep: the candidate entries; afterwards: entries with vertices
+    uint32_t cnt[kChunk];
+    uint32_t list[kChunk];
+    uint32_t stage[kMcThreads / 64][kRowBlocks][kMetaWords];
 };
 
-// Chunk prologue (all threads): entry t of the chunk gets its block id and
-// caller word; it is a count candidate when the isovalue lies in
-// [lo, hi) of its forward range (launch_forward_ranges): with iso < lo every
-// corner test fails and with iso >= hi every one passes, so the block's cells
-// have case 0 or 255 and no vertices.  Returns the number of candidates
+// LDS of the emit pass over one chunk: block ids and caller words, and the
+// list of entries with vertices with their vertex offsets in the chunk.
+struct EmitShared {
+    uint32_t bid[kChunk + 1];
+    uint32_t aux[kChunk + 1];
+    uint32_t off[kChunk];
+    uint32_t list[kChunk];
+};
+
+// Chunk prologue of the count pass (all threads): entry t of the chunk gets
+// its block id and caller word; it is a count candidate when the isovalue
+// lies in [lo, hi) of its forward range (launch_forward_ranges): with iso < lo
+// every corner test fails and with iso >= hi every one passes, so the block's
+// cells have case 0 or 255 and no vertices.  Returns the number of candidates
 // (listed in cs.list, ascending).
-BCMC_DEV uint32_t chunk_prologue(ChunkShared &cs, bool in, uint32_t bid, uint32_t aux, float2 fwd, float iso,
+BCMC_DEV uint32_t chunk_prologue(CountShared &cs, bool in, uint32_t bid, uint32_t aux, float2 fwd, float iso,
                                  uint32_t *lds_waves) {
     const uint32_t t = threadIdx.x;
     cs.bid[t] = bid;
@@ -512,12 +540,14 @@ BCMC_DEV uint32_t chunk_prologue(ChunkShared &cs, bool in, uint32_t bid, uint32_
 // before this round's voxels are used.  setup(e, b) fills the RowBlock of
 // entry e (e = kChunk: no block).
 template <class Setup>
-BCMC_DEV void count_sweep(ChunkShared &cs, uint32_t ncand, uint32_t wave, const McTables &tab, Setup &&setup) {
+BCMC_DEV void count_sweep(CountShared &cs, uint32_t ncand, uint32_t wave, const CountTables &tab,
+                          Setup &&setup) {
     constexpr uint32_t kRound = (kMcThreads / 64) * kRowBlocks;
     const uint32_t lane = lane_id(), row = lane >> 4;
     const uint32_t first = wave * kRowBlocks;
     if (first >= ncand)
         return;
+    uint32_t *stage = cs.stage[wave][row];
     auto setup_c = [&](uint32_t c, uint32_t &e, RowBlock &b) {
         e = c < ncand ? cs.list[c] : kChunk;
         setup(e, b);
@@ -530,15 +560,16 @@ BCMC_DEV void count_sweep(ChunkShared &cs, uint32_t ncand, uint32_t wave, const 
     for (uint32_t base = first; base < ncand; base += kRound) {
         const RowBlock bq = b;
         const uint32_t eq = e;
-        row_meta_store(m, bq.tag, tab, cs.meta[eq]);
+        wave_lds_sync();  // the previous round's reads of the stage are done
+        row_meta_store(m, bq.tag, tab.dims, stage);
         wave_lds_sync();
         RowTile t;
-        row_voxels(bq, cs.meta[eq], t);
+        row_voxels(bq, stage, t);
         if (base + kRound < ncand) {
             setup_c(base + kRound + row, e, b);
             m = row_meta_load(b);
         }
-        const uint32_t c = row_block_count(t, bq.iso, tab);
+        const uint32_t c = row_block_count(t, bq.iso, tab.count);
         if ((lane & 15u) == 15u && eq < kChunk)
             cs.cnt[eq] = c;
     }
@@ -546,24 +577,62 @@ BCMC_DEV void count_sweep(ChunkShared &cs, uint32_t ncand, uint32_t wave, const 
 
 // Emit sweep: wave w takes groups of 4 consecutive entries of the list of
 // blocks with vertices (whose outputs are contiguous) and writes their
-// vertices from out0 + the group's offset in the chunk.
+// vertices from out0 + the group's offset in the chunk.  The next group's
+// slot/flag loads are issued before this group's vertices are written.
 template <class Setup>
-BCMC_DEV void emit_sweep(ChunkShared &cs, uint32_t wave, uint32_t num_with, unsigned long long out0,
+BCMC_DEV void emit_sweep(EmitShared &cs, uint32_t wave, uint32_t num_with, unsigned long long out0,
                          const McTables &tab, EmitScratch &es, unsigned long long cap, void *verts,
                          int fmt, Setup &&setup) {
     constexpr uint32_t kRound = (kMcThreads / 64) * kRowBlocks;
     const uint32_t row = lane_id() >> 4;
-    for (uint32_t hh = wave * kRowBlocks; hh < num_with; hh += kRound) {
-        const uint32_t h = hh + row;
+    uint32_t hh = wave * kRowBlocks;
+    if (hh >= num_with)
+        return;
+    auto block_of = [&](uint32_t h, RowBlock &b) {
         const bool live = h < num_with;
-        const uint32_t e = live ? cs.list[h] : 0u;
-        RowBlock b;
-        setup(e, b);
+        setup(live ? cs.list[h] : 0u, b);
         b.live = live;
+    };
+    RowBlock b;
+    block_of(hh + row, b);
+    RowMeta m = row_meta_load(b);
+    for (; hh < num_with; hh += kRound) {
+        const RowBlock bq = b;
+        wave_lds_sync();  // the previous group's reads of the meta words are done
+        row_meta_store(m, bq.tag, tab.dims, es.meta[row]);
+        wave_lds_sync();
         RowTile t;
-        row_voxels(b, cs.meta[e], t);
-        emit_row_blocks(b, t, tab, es, out0 + cs.off[hh], cap, verts, fmt);
+        row_voxels(bq, es.meta[row], t);
+        if (hh + kRound < num_with) {
+            block_of(hh + kRound + row, b);
+            m = row_meta_load(b);
+        }
+        emit_row_blocks(bq, t, tab, es, out0 + cs.off[hh], cap, verts, fmt);
     }
+}
+
+// Emit-pass chunk prologue (all threads): the chunk's list of entries with
+// vertices (cnt[t] > 0) with their vertex offsets in the chunk.  Returns the
+// number of entries with vertices; *ex_h / *ex_v receive this thread's
+// exclusive positions and *agg_v the chunk's vertex total.
+BCMC_DEV uint32_t emit_prologue(EmitShared &cs, uint32_t c, uint32_t bid, uint32_t aux, uint32_t *lds_waves,
+                                uint32_t *ex_h, uint32_t *ex_v, uint32_t *agg_v) {
+    const uint32_t t = threadIdx.x;
+    cs.bid[t] = bid;
+    cs.aux[t] = aux;
+    if (t == 0) {
+        cs.bid[kChunk] = 0;
+        cs.aux[kChunk] = 0;
+    }
+    uint32_t agg_h;
+    *ex_h = block_excl_scan<kMcThreads>(c ? 1u : 0u, lds_waves, &agg_h);
+    *ex_v = block_excl_scan<kMcThreads>(c, lds_waves, agg_v);
+    if (c) {
+        cs.list[*ex_h] = t;
+        cs.off[*ex_h] = *ex_v;
+    }
+    __syncthreads();
+    return agg_h;
 }
 
 // ---------------------------------------------------------------------------
