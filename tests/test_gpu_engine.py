@@ -111,3 +111,31 @@ def test_result_is_the_same_alone_and_in_the_mix(dev):
         got = eng.vertices(res[pos])
         assert res[pos]["num_vertices"] == alone["num_vertices"], trial
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), trial
+
+
+def test_many_queries_in_one_step(dev):
+    """More queries than one workgroup of the chunk table, with empty queries
+    (isovalues outside the field) between them; counts equal the oracle's."""
+    streams = _streams()
+    eng = bcmc_amd.Engine(dev, 100000 * 256)
+    ids = [eng.add_volume(s, r, d) for s, r, d in streams]
+    rng = np.random.default_rng(21)
+    nq = 300
+    qv = rng.integers(0, len(ids), nq)
+    qi = rng.uniform(0.15, 0.85, nq).astype(np.float32)
+    qi[rng.uniform(0, 1, nq) < 0.2] = 5.0  # no active blocks
+    res = eng.run(qv, qi)
+    cache = {}
+    off = 0
+    for k in range(nq):
+        key = (int(qv[k]), float(qi[k]))
+        if key not in cache:
+            s, r, d = streams[qv[k]]
+            cache[key] = _oracle(s, r, d, float(qi[k]))
+        nv, verts = cache[key]
+        assert res[k]["num_vertices"] == nv, k
+        assert res[k]["vertex_offset"] == off, k
+        off += nv
+        if k % 37 == 0:
+            got = eng.vertices(res[k])
+            assert np.array_equal(got.view(np.uint32), verts.view(np.uint32)), k

@@ -68,6 +68,8 @@ struct EngCounters {
     uint32_t num_cand;          // eviction candidates
     uint32_t short_slots;       // misses that found no slot (admission failed)
     uint32_t num_rows;          // candidate row segments of all queries
+    uint32_t num_chunks;        // marching-cubes chunks of all queries
+    uint32_t pad0;
     unsigned long long num_verts;
     uint32_t vol_miss[64];      // misses per volume
 };
@@ -98,6 +100,7 @@ __global__ void eng_begin_kernel(EngCounters *c, uint32_t *q_act_start, uint32_t
         c->num_cand = 0;
         c->short_slots = 0;
         c->num_rows = 0;
+        c->num_chunks = 0;
         c->num_verts = 0;
     }
     for (uint32_t v = threadIdx.x; v < kMaxVols; v += blockDim.x)
@@ -330,6 +333,10 @@ struct EngMcArgs {
     const float *pool;
     uint32_t capacity;
     EngCounters *c;
+    uint32_t nq;
+    const uint32_t *q_act_start;         // per query: first pair (kNoPairs: none)
+    const uint32_t *q_pair_end;          // per query: one past its last pair
+    const uint32_t *q_chunk_base;        // per query: its first chunk
     uint32_t *counts;                    // per pair: vertices
     uint32_t *chunk_verts;               // per chunk: vertices
     unsigned long long *chunk_verts_pre; // exclusive prefixes of chunk_verts
@@ -363,47 +370,117 @@ constexpr int kEngThreads = kMcThreads;
 // Blocks of all queries of one step (u32 indices into the concatenated arrays).
 constexpr uint64_t kMaxStepBlocks = 1ull << 31;
 
+// Chunks of the marching-cubes pass never cross a query: query q's pairs
+// [start, end) form ceil((end - start) / kChunk) chunks from q_chunk_base[q]
+// on.  One workgroup: per query the end of its pairs (the next query's first
+// pair, or the step's total) and the exclusive scan of the chunk counts.
+__global__ void eng_chunk_table_kernel(EngCounters *c, const uint32_t *q_act_start, uint32_t nq,
+                                       uint32_t *q_pair_end, uint32_t *q_chunk_base) {
+    constexpr int NT = 256;
+    __shared__ uint32_t lds_w[NT / 64 + 1];
+    __shared__ uint32_t lds_next;
+    const uint32_t t = threadIdx.x, lane = lane_id(), wave = t / 64;
+    const uint32_t n = c->num_active;
+    if (t == 0)
+        lds_next = n;
+    __syncthreads();
+    // backwards in blocks of NT queries: end = min start of the later queries
+    for (uint32_t hi = nq; hi > 0;) {
+        const uint32_t lo = hi > NT ? hi - NT : 0u;
+        const uint32_t q = lo + t;
+        const uint32_t st = q < hi ? q_act_start[q] : kNoPairs;
+        // suffix minimum over the block (exclusive), carried from later blocks
+        uint32_t v = st;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = (uint32_t)__shfl_down((int)v, o, 64);
+            if (lane + o < 64)
+                v = min(v, u);
+        }
+        if (lane == 0)
+            lds_w[wave] = v;  // minimum of the wave
+        __syncthreads();
+        uint32_t later = lds_next;
+        for (uint32_t w = wave + 1; w < NT / 64; ++w)
+            later = min(later, lds_w[w]);
+        const uint32_t after = (uint32_t)__shfl_down((int)v, 1, 64);
+        const uint32_t excl = min(later, lane + 1 < 64 ? after : 0xFFFFFFFFu);
+        if (q < hi)
+            q_pair_end[q] = st == kNoPairs ? st : excl;
+        __syncthreads();
+        if (t == 0)
+            lds_next = min(lds_next, min(min(lds_w[0], lds_w[1]), min(lds_w[2], lds_w[3])));
+        __syncthreads();
+        hi = lo;
+    }
+    __syncthreads();
+    // forward: chunk bases
+    uint32_t carry = 0;
+    for (uint32_t lo = 0; lo < nq; lo += NT) {
+        const uint32_t q = lo + t;
+        uint32_t k = 0;
+        if (q < nq) {
+            const uint32_t st = q_act_start[q];
+            k = st == kNoPairs ? 0u : div_up(q_pair_end[q] - st, kChunk);
+        }
+        uint32_t total;
+        const uint32_t ex = block_excl_scan<NT>(k, lds_w, &total);
+        if (q < nq)
+            q_chunk_base[q] = carry + ex;
+        carry += total;
+    }
+    if (t == 0)
+        c->num_chunks = carry;
+}
+
+// The query of chunk ch: the last query whose first chunk is <= ch
+// (workgroup-uniform binary search).
+BCMC_DEV uint32_t eng_query_of_chunk(const EngMcArgs &a, uint32_t ch) {
+    uint32_t lo = 0, hi = a.nq;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (a.q_chunk_base[mid] <= ch)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Pairs [k0, k1) of chunk ch of query q.
+BCMC_DEV void eng_chunk_range(const EngMcArgs &a, uint32_t ch, uint32_t q, uint32_t &k0, uint32_t &k1) {
+    k0 = a.q_act_start[q] + (ch - a.q_chunk_base[q]) * kChunk;
+    k1 = min(a.q_pair_end[q], k0 + kChunk);
+}
+
 // The count / prefix / emit launches of mc.hip over (query, block) pairs of
-// many queries; vertex offsets are global, and each query's vertices are
-// contiguous because pairs are in query order.  A chunk inside one query (the
-// common case) uses workgroup-uniform volume views; chunks across query
-// boundaries use per-lane views.
+// many queries, with workgroup-uniform volume views (every chunk lies inside
+// one query); vertex offsets are global, and each query's vertices are
+// contiguous because pairs and chunks are in query order.
 __global__ __launch_bounds__(kEngThreads) void eng_count_kernel(EngMcArgs a) {
     constexpr int NW = kEngThreads / 64;
     __shared__ CountShared cs;
     __shared__ uint32_t lds_w[NW + 1];
     __shared__ CountTables tab;
     load_count_tables(&tab);
-    const uint32_t n = a.c->num_active, epoch = a.c->epoch;
-    const uint32_t nchunks = div_up(n, kChunk);
+    const uint32_t nchunks = a.c->num_chunks, epoch = a.c->epoch;
     const uint32_t wave = threadIdx.x / 64;
     for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
-        const uint32_t k0 = chunk * kChunk;
-        const bool in = k0 + threadIdx.x < n;
-        const uint32_t bid = in ? a.act_b[k0 + threadIdx.x] : 0u, qi = in ? a.act_q[k0 + threadIdx.x] : 0u;
-        float2 fwd = make_float2(1.0f, 0.0f);
-        float iso = 0.0f;
-        if (in) {
-            const EngQuery &q = a.qs[qi];
-            iso = q.iso;
-            fwd = a.vols[q.vol].fwd[bid];
-        }
-        const uint32_t ncand = chunk_prologue(cs, in, bid, qi, fwd, iso, lds_w);
-        const uint32_t q_first = (uint32_t)__builtin_amdgcn_readfirstlane((int)cs.aux[0]);
-        const uint32_t q_last = (uint32_t)__builtin_amdgcn_readfirstlane((int)cs.aux[min(n - k0, kChunk) - 1]);
-        if (q_first == q_last) {
-            RowBlock ub;
-            eng_row_block(a, q_first, 0, true, epoch, ub);
-            count_sweep(cs, ncand, wave, tab, [&](uint32_t e, RowBlock &b) {
-                b = ub;
-                b.live = k0 + e < n;
-                b.bid = cs.bid[e];
-            });
-        } else {
-            count_sweep(cs, ncand, wave, tab, [&](uint32_t e, RowBlock &b) {
-                eng_row_block(a, cs.aux[e], cs.bid[e], k0 + e < n, epoch, b);
-            });
-        }
+        const uint32_t q = eng_query_of_chunk(a, chunk);
+        uint32_t k0, k1;
+        eng_chunk_range(a, chunk, q, k0, k1);
+        RowBlock ub;
+        eng_row_block(a, q, 0, true, epoch, ub);
+        const float2 *fwd = a.vols[a.qs[q].vol].fwd;
+        const bool in = k0 + threadIdx.x < k1;
+        const uint32_t bid = in ? a.act_b[k0 + threadIdx.x] : 0u;
+        const uint32_t ncand = chunk_prologue(cs, in, bid, q, in ? fwd[bid] : make_float2(1.0f, 0.0f), ub.iso,
+                                              lds_w);
+        count_sweep(cs, ncand, wave, tab, [&](uint32_t e, RowBlock &b) {
+            b = ub;
+            b.live = k0 + e < k1;
+            b.bid = cs.bid[e];
+        });
         __syncthreads();
         const uint32_t c = cs.cnt[threadIdx.x];
         if (in)
@@ -419,7 +496,7 @@ struct EngChunkPrefixOp {
     const uint32_t *verts;
     unsigned long long *verts_pre;
     EngCounters *c;
-    BCMC_DEV uint32_t size() const { return div_up(c->num_active, kChunk); }
+    BCMC_DEV uint32_t size() const { return c->num_chunks; }
     BCMC_DEV void begin() const {}
     BCMC_DEV void value(uint32_t i, uint32_t (&v)[1]) const { v[0] = verts[i]; }
     BCMC_DEV void emit(uint32_t i, const uint32_t (&)[1], const unsigned long long (&e)[1]) const {
@@ -435,43 +512,32 @@ __global__ __launch_bounds__(kEngThreads) __attribute__((amdgpu_waves_per_eu(5))
     __shared__ McTables tab;
     __shared__ EmitScratch es[NW];
     load_mc_tables(&tab);
-    const uint32_t n = a.c->num_active, epoch = a.c->epoch;
-    const uint32_t nchunks = div_up(n, kChunk);
+    const uint32_t nchunks = a.c->num_chunks, epoch = a.c->epoch;
     const uint32_t wave = threadIdx.x / 64;
     for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
-        const uint32_t k0 = chunk * kChunk;
+        const uint32_t q = eng_query_of_chunk(a, chunk);
+        uint32_t k0, k1;
+        eng_chunk_range(a, chunk, q, k0, k1);
         const uint32_t k = k0 + threadIdx.x;
-        const bool in = k < n;
+        const bool in = k < k1;
         const uint32_t c = in ? a.counts[k] : 0u;
-        const uint32_t bid = in ? a.act_b[k] : 0u, qi = in ? a.act_q[k] : 0u;
+        const uint32_t bid = in ? a.act_b[k] : 0u;
         uint32_t ex_h, ex_v, agg_v;
-        const uint32_t num_with = emit_prologue(cs, c, bid, qi, lds_w, &ex_h, &ex_v, &agg_v);
+        const uint32_t num_with = emit_prologue(cs, c, bid, q, lds_w, &ex_h, &ex_v, &agg_v);
         const unsigned long long pv = a.chunk_verts_pre[chunk];
-        // the first pair of each query records where the query's vertices start
-        if (in) {
-            const uint32_t prev = threadIdx.x ? cs.aux[threadIdx.x - 1] : (k ? a.act_q[k - 1] : ~0u);
-            if (qi != prev)
-                a.q_vfirst[qi] = pv + ex_v;
-        }
+        // where the query's vertices start
+        if (threadIdx.x == 0 && chunk == a.q_chunk_base[q])
+            a.q_vfirst[q] = pv;
         if (num_with == 0)
             continue;
-        const uint32_t q_first = (uint32_t)__builtin_amdgcn_readfirstlane((int)cs.aux[0]);
-        const uint32_t q_last = (uint32_t)__builtin_amdgcn_readfirstlane((int)cs.aux[min(n - k0, kChunk) - 1]);
-        if (q_first == q_last) {
-            RowBlock ub;
-            eng_row_block(a, q_first, 0, true, epoch, ub);
-            auto setup = [&](uint32_t e, RowBlock &b) {
-                b = ub;
-                b.live = k0 + e < n;
-                b.bid = cs.bid[e];
-            };
-            emit_sweep(cs, wave, num_with, pv, tab, es[wave], a.vert_cap, a.verts, a.fmt, setup);
-        } else {
-            auto setup = [&](uint32_t e, RowBlock &b) {
-                eng_row_block(a, cs.aux[e], cs.bid[e], k0 + e < n, epoch, b);
-            };
-            emit_sweep(cs, wave, num_with, pv, tab, es[wave], a.vert_cap, a.verts, a.fmt, setup);
-        }
+        RowBlock ub;
+        eng_row_block(a, q, 0, true, epoch, ub);
+        emit_sweep(cs, wave, num_with, pv, tab, es[wave], a.vert_cap, a.verts, a.fmt,
+                   [&](uint32_t e, RowBlock &b) {
+                       b = ub;
+                       b.live = k0 + e < k1;
+                       b.bid = cs.bid[e];
+                   });
         __syncthreads();
     }
 }
@@ -512,14 +578,16 @@ struct bcmc_engine {
     uint32_t *flags = nullptr, *act_q = nullptr, *act_b = nullptr;
     uint32_t *counts = nullptr, *chunk_verts = nullptr;
     unsigned long long *chunk_verts_pre = nullptr;
+    uint64_t chunks_cap = 0;
+    ScanSite site_chunks;
     uint64_t rows_cap = 0, row_elems_cap = 0;
     uint32_t *cand_q = nullptr, *cand_seg = nullptr;
     ScanSite site_rows;
     uint32_t q_cap = 0;
     EngQuery *d_q = nullptr;
-    uint32_t *q_act_start = nullptr;
+    uint32_t *q_act_start = nullptr, *q_pair_end = nullptr, *q_chunk_base = nullptr;
     unsigned long long *q_vfirst = nullptr;
-    ScanSite site_active, site_cand, site_chunks;
+    ScanSite site_active, site_cand;
     void *verts = nullptr;
     uint64_t vert_cap = 0;
     int fmt = BCMC_VERTS_LOCAL_F32;
@@ -537,9 +605,6 @@ static int eng_ensure_blocks(bcmc_engine *e, uint64_t total) {
     dev_free(e->act_q);
     dev_free(e->act_b);
     dev_free(e->counts);
-    dev_free(e->chunk_verts);
-    dev_free(e->chunk_verts_pre);
-    scan_site_free(&e->site_chunks);
     const uint64_t cap = std::max<uint64_t>(total, e->blocks_cap * 3 / 2);
     BCMC_TRY(dev_alloc((void **)&e->flags, cap * 4));
     // flag words hold the epoch of the step that found the block active;
@@ -548,10 +613,6 @@ static int eng_ensure_blocks(bcmc_engine *e, uint64_t total) {
     BCMC_TRY(dev_alloc((void **)&e->act_q, cap * 4));
     BCMC_TRY(dev_alloc((void **)&e->act_b, cap * 4));
     BCMC_TRY(dev_alloc((void **)&e->counts, cap * 4));
-    const uint64_t nch = div_up(cap, kChunk);
-    BCMC_TRY(dev_alloc((void **)&e->chunk_verts, nch * 4));
-    BCMC_TRY(dev_alloc((void **)&e->chunk_verts_pre, nch * 8));
-    BCMC_TRY(scan_site_alloc(&e->site_chunks, nch, kScanTile));
     e->blocks_cap = cap;
     return BCMC_OK;
 }
@@ -585,10 +646,14 @@ static int eng_ensure_queries(bcmc_engine *e, uint32_t nq) {
     BCMC_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
     dev_free(e->d_q);
     dev_free(e->q_act_start);
+    dev_free(e->q_pair_end);
+    dev_free(e->q_chunk_base);
     dev_free(e->q_vfirst);
     const uint32_t cap = std::max<uint32_t>(nq, 64);
     BCMC_TRY(dev_alloc((void **)&e->d_q, (size_t)cap * sizeof(EngQuery)));
     BCMC_TRY(dev_alloc((void **)&e->q_act_start, (size_t)cap * 4));
+    BCMC_TRY(dev_alloc((void **)&e->q_pair_end, (size_t)cap * 4));
+    BCMC_TRY(dev_alloc((void **)&e->q_chunk_base, (size_t)cap * 4));
     BCMC_TRY(dev_alloc((void **)&e->q_vfirst, (size_t)cap * 8));
     e->q_cap = cap;
     return BCMC_OK;
@@ -598,8 +663,12 @@ static uint32_t eng_grid(const bcmc_ctx *ctx, uint64_t n, uint32_t per_cu = 8) {
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(div_up(n, 256), persistent_grid(ctx, per_cu)));
 }
 
-static EngMcArgs eng_mc_args(bcmc_engine *e) {
+static EngMcArgs eng_mc_args(bcmc_engine *e, uint32_t nq) {
     EngMcArgs a;
+    a.nq = nq;
+    a.q_act_start = e->q_act_start;
+    a.q_pair_end = e->q_pair_end;
+    a.q_chunk_base = e->q_chunk_base;
     a.qs = e->d_q;
     a.vols = e->d_vols;
     a.act_q = e->act_q;
@@ -618,24 +687,40 @@ static EngMcArgs eng_mc_args(bcmc_engine *e) {
     return a;
 }
 
-static int eng_launch_emit(bcmc_engine *e, uint64_t total) {
-    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(total, kChunk),
-                                                                   persistent_grid(e->ctx, e->ctx->tune.mc)));
-    eng_emit_kernel<<<grid, kEngThreads, 0, e->ctx->stream>>>(eng_mc_args(e));
+// chunks of a step: at most one partial chunk per query
+static uint64_t eng_max_chunks(uint64_t total, uint32_t nq) { return div_up(total, kChunk) + nq; }
+
+static int eng_launch_emit(bcmc_engine *e, uint32_t nq, uint64_t total) {
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>(eng_max_chunks(total, nq), persistent_grid(e->ctx, e->ctx->tune.mc)));
+    eng_emit_kernel<<<grid, kEngThreads, 0, e->ctx->stream>>>(eng_mc_args(e, nq));
     BCMC_HIP_CHECK(hipGetLastError());
     return BCMC_OK;
 }
 
-// count -> chunk prefixes -> emit over the step's active pairs
-static int eng_launch_mc(bcmc_engine *e, uint64_t total) {
-    const EngMcArgs a = eng_mc_args(e);
-    const uint32_t max_chunks = (uint32_t)div_up(total, kChunk);
-    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(max_chunks,
-                                                                   persistent_grid(e->ctx, e->ctx->tune.count)));
+// chunk table -> count -> chunk prefixes -> emit over the step's active pairs
+static int eng_launch_mc(bcmc_engine *e, uint32_t nq, uint64_t total) {
+    const uint64_t max_chunks = eng_max_chunks(total, nq);
+    if (max_chunks > e->chunks_cap) {
+        BCMC_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
+        dev_free(e->chunk_verts);
+        dev_free(e->chunk_verts_pre);
+        scan_site_free(&e->site_chunks);
+        const uint64_t cap = std::max<uint64_t>(max_chunks, e->chunks_cap * 3 / 2);
+        BCMC_TRY(dev_alloc((void **)&e->chunk_verts, cap * 4));
+        BCMC_TRY(dev_alloc((void **)&e->chunk_verts_pre, cap * 8));
+        BCMC_TRY(scan_site_alloc(&e->site_chunks, cap, kScanTile));
+        e->chunks_cap = cap;
+    }
+    const EngMcArgs a = eng_mc_args(e, nq);
+    eng_chunk_table_kernel<<<1, 256, 0, e->ctx->stream>>>(e->c, e->q_act_start, nq, e->q_pair_end,
+                                                          e->q_chunk_base);
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>(max_chunks, persistent_grid(e->ctx, e->ctx->tune.count)));
     eng_count_kernel<<<grid, kEngThreads, 0, e->ctx->stream>>>(a);
     launch_scan<1>(e->ctx, e->site_chunks, e->site_chunks,
                    EngChunkPrefixOp{a.chunk_verts, a.chunk_verts_pre, a.c}, max_chunks);
-    return eng_launch_emit(e, total);
+    return eng_launch_emit(e, nq, total);
 }
 
 extern "C" {
@@ -719,6 +804,8 @@ BCMC_API int bcmc_engine_destroy(bcmc_engine *e) {
     dev_free(e->chunk_verts_pre);
     dev_free(e->d_q);
     dev_free(e->q_act_start);
+    dev_free(e->q_pair_end);
+    dev_free(e->q_chunk_base);
     dev_free(e->q_vfirst);
     dev_free(e->verts);
     scan_site_free(&e->site_active);
@@ -877,7 +964,7 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
         launch_decode_to_cache(ctx, v.stream, v.nwords, v.maxbits, v.miss_list, &e->c->vol_miss[vi],
                                v.n, v.slot_map, e->pool, e->pool_cap);
     }
-    BCMC_TRY(eng_launch_mc(e, total));
+    BCMC_TRY(eng_launch_mc(e, nq, total));
     BCMC_HIP_CHECK(hipMemcpyAsync(e->ch, e->c, sizeof(EngCounters), hipMemcpyDeviceToHost, st));
     std::vector<unsigned long long> qv(nq);
     std::vector<uint32_t> qa(nq);
@@ -897,7 +984,7 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
         e->vert_cap = 0;
         BCMC_TRY(dev_alloc(&e->verts, want * vbytes(e->fmt)));
         e->vert_cap = want;
-        BCMC_TRY(eng_launch_emit(e, total));
+        BCMC_TRY(eng_launch_emit(e, nq, total));
         BCMC_HIP_CHECK(hipStreamSynchronize(st));
     }
     // a query's vertices run from its first active pair's offset to the next
@@ -908,7 +995,7 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     uint64_t next = nv;
     for (uint32_t i = nq; i-- > 0;) {
         bcmc_query_result r;
-        if (qa[i] == kNoPairs) {
+        if (qa[i] == kNoPairs || qa[i] == next_a) {
             r.num_active_blocks = 0;
             r.vertex_offset = next;
         } else {
