@@ -56,7 +56,13 @@ struct EngQuery {
     uint32_t flag_base;         // first index of this query in the concatenated block arrays
     uint32_t n;
     uint32_t row_base;          // first index of this query in the concatenated row segments
-    uint32_t pad[3];
+    uint32_t nb0, nsegx;        // the volume's blocks in x and row segments in x
+    uint32_t pad;
+    // the volume's arrays, copied here so that per-element work reads one descriptor
+    const float2 *dilated;
+    const float2 *seg_range;
+    int32_t *slot_map;
+    uint64_t pad2;
 };
 
 struct EngCounters {
@@ -143,8 +149,8 @@ struct CandSegOp {
     BCMC_DEV void begin() const {}
     BCMC_DEV void value(uint32_t r, uint32_t (&v)[1]) const {
         const uint32_t qi = query_of_row(qs, nq, r);
-        const EngQuery q = qs[qi];
-        const float2 rr = vols[q.vol].seg_range[r - q.row_base];
+        const EngQuery &q = qs[qi];
+        const float2 rr = q.seg_range[r - q.row_base];
         v[0] = (q.iso >= rr.x && q.iso <= rr.y) ? 1u : 0u;
     }
     BCMC_DEV void emit(uint32_t r, const uint32_t (&v)[1], const unsigned long long (&e)[1]) const {
@@ -170,18 +176,16 @@ struct SegActiveOp {
     BCMC_DEV uint32_t size() const { return c->num_rows * kSegBlocks; }
     BCMC_DEV void begin() const {}
     // block of element i (>= nb[0] in x: padding)
-    BCMC_DEV bool block_of(uint32_t i, const EngVol &vol, uint32_t &x, uint32_t &row) const {
-        const uint32_t k = i / kSegBlocks, seg = cand_seg[k];
-        const uint32_t nsegx = div_up(vol.nb[0], kSegBlocks);
-        x = (seg % nsegx) * kSegBlocks + i % kSegBlocks;
-        row = seg / nsegx;
-        return x < vol.nb[0];
+    BCMC_DEV bool block_of(uint32_t i, const EngQuery &q, uint32_t &x, uint32_t &row) const {
+        const uint32_t seg = cand_seg[i / kSegBlocks];
+        x = (seg % q.nsegx) * kSegBlocks + i % kSegBlocks;
+        row = seg / q.nsegx;
+        return x < q.nb0;
     }
     BCMC_DEV void value(uint32_t i, uint32_t (&v)[1]) const {
-        const EngQuery q = qs[cand_q[i / kSegBlocks]];
-        const EngVol &vol = vols[q.vol];
+        const EngQuery &q = qs[cand_q[i / kSegBlocks]];
         uint32_t x, row;
-        v[0] = (block_of(i, vol, x, row) && block_active(vol.dilated[x + vol.nb[0] * row], q.iso)) ? 1u : 0u;
+        v[0] = (block_of(i, q, x, row) && block_active(q.dilated[x + q.nb0 * row], q.iso)) ? 1u : 0u;
     }
     BCMC_DEV void emit(uint32_t i, const uint32_t (&v)[1], const unsigned long long (&e)[1]) const {
         const uint32_t k = i / kSegBlocks;
@@ -189,11 +193,10 @@ struct SegActiveOp {
         if (i % kSegBlocks == 0 && (k == 0 || cand_q[k - 1] != qi))
             q_act_start[qi] = (uint32_t)e[0];
         if (v[0]) {
-            const EngQuery q = qs[qi];
-            const EngVol &vol = vols[q.vol];
+            const EngQuery &q = qs[qi];
             uint32_t x, row;
-            block_of(i, vol, x, row);
-            const uint32_t b = x + vol.nb[0] * row;
+            block_of(i, q, x, row);
+            const uint32_t b = x + q.nb0 * row;
             act_q[e[0]] = qi;
             act_b[e[0]] = b;
             flags[q.flag_base + b] = c->epoch;
@@ -201,66 +204,6 @@ struct SegActiveOp {
     }
     BCMC_DEV void total(const unsigned long long (&t)[1]) const { c->num_active = (uint32_t)t[0]; }
 };
-
-// stamp pooled blocks, claim missing ones (once per block: the claim is a
-// compare-and-swap -1 -> -2).  Whole waves walk the pairs together so that
-// the claims of a wave reserve their list entries with one atomic per list.
-__global__ void eng_mark_kernel(const EngQuery *qs, const EngVol *vols, const uint32_t *act_q,
-                                const uint32_t *act_b, EngCounters *c, uint32_t *stamp,
-                                unsigned long long *miss, uint32_t miss_cap) {
-    const uint32_t n = c->num_active, ep = c->epoch;
-    const uint32_t lane = lane_id();
-    const uint64_t lt = (1ull << lane) - 1ull;
-    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
-    const uint32_t nw = (gridDim.x * blockDim.x) / 64;
-    for (uint32_t base = wid * 64; base < n; base += nw * 64) {
-        const uint32_t e = base + lane;
-        bool claimed = false;
-        uint32_t vi = 0, b = 0;
-        if (e < n) {
-            vi = qs[act_q[e]].vol;
-            b = act_b[e];
-            int32_t *sm = vols[vi].slot_map;
-            // plain load: slots of pooled blocks do not change during the
-            // kernel; the compare-and-swap settles races between claimers
-            const int32_t s = sm[b];
-            if (s >= 0) {
-                stamp[s] = ep;
-            } else if (s == -1) {
-                int32_t expect = -1;
-                claimed = __hip_atomic_compare_exchange_strong(&sm[b], &expect, -2, __ATOMIC_RELAXED,
-                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-        const uint64_t m = __ballot(claimed);
-        if (m == 0)
-            continue;
-        uint32_t w0 = 0;
-        if (lane == 0)
-            w0 = atomic_add_u32(&c->num_miss, (uint32_t)__popcll(m));
-        w0 = wave_read_lane(w0, 0);
-        const uint32_t w = w0 + (uint32_t)__popcll(m & lt);
-        if (claimed && w < miss_cap)
-            miss[w] = ((unsigned long long)vi << 32) | b;
-        // per-volume decode lists: one reservation per volume present in the wave
-        uint64_t rem = m;
-        while (rem) {
-            const uint32_t v0 = wave_read_lane(vi, (int)__builtin_ctzll(rem));
-            const bool mine = claimed && vi == v0;
-            const uint64_t same = __ballot(mine);
-            uint32_t p0 = 0;
-            if (lane == 0)
-                p0 = atomic_add_u32(&c->vol_miss[v0], (uint32_t)__popcll(same));
-            p0 = wave_read_lane(p0, 0);
-            if (mine) {
-                const uint32_t wv = p0 + (uint32_t)__popcll(same & lt);
-                if (wv < vols[v0].n)
-                    vols[v0].miss_list[wv] = b;
-            }
-            rem &= ~same;
-        }
-    }
-}
 
 // eviction candidates: pooled slots not stamped by this or the previous step
 struct CandOp {
@@ -451,6 +394,57 @@ BCMC_DEV uint32_t eng_query_of_chunk(const EngMcArgs &a, uint32_t ch) {
 BCMC_DEV void eng_chunk_range(const EngMcArgs &a, uint32_t ch, uint32_t q, uint32_t &k0, uint32_t &k1) {
     k0 = a.q_act_start[q] + (ch - a.q_chunk_base[q]) * kChunk;
     k1 = min(a.q_pair_end[q], k0 + kChunk);
+}
+
+// Stamp pooled blocks, claim missing ones (once per block: the claim is a
+// compare-and-swap -1 -> -2), one chunk of one query per workgroup and
+// iteration; the claims of a wave reserve their list entries with one atomic
+// per list.
+__global__ __launch_bounds__(256) void eng_mark_kernel(EngMcArgs a, uint32_t *stamp, unsigned long long *miss,
+                                                       uint32_t miss_cap) {
+    const uint32_t nchunks = a.c->num_chunks, ep = a.c->epoch;
+    const uint32_t lane = lane_id();
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+        const uint32_t q = eng_query_of_chunk(a, chunk);
+        uint32_t k0, k1;
+        eng_chunk_range(a, chunk, q, k0, k1);
+        const uint32_t vi = a.qs[q].vol;
+        int32_t *sm = a.qs[q].slot_map;
+        const uint32_t k = k0 + threadIdx.x;
+        bool claimed = false;
+        uint32_t b = 0;
+        if (k < k1) {
+            b = a.act_b[k];
+            // plain load: slots of pooled blocks do not change during the
+            // kernel; the compare-and-swap settles races between claimers
+            const int32_t sl = sm[b];
+            if (sl >= 0) {
+                stamp[sl] = ep;
+            } else if (sl == -1) {
+                int32_t expect = -1;
+                claimed = __hip_atomic_compare_exchange_strong(&sm[b], &expect, -2, __ATOMIC_RELAXED,
+                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        const uint64_t m = __ballot(claimed);
+        if (m == 0)
+            continue;
+        uint32_t w0 = 0, p0 = 0;
+        if (lane == 0) {
+            w0 = atomic_add_u32(&a.c->num_miss, (uint32_t)__popcll(m));
+            p0 = atomic_add_u32(&a.c->vol_miss[vi], (uint32_t)__popcll(m));
+        }
+        w0 = wave_read_lane(w0, 0);
+        p0 = wave_read_lane(p0, 0);
+        if (claimed) {
+            const uint32_t r = (uint32_t)__popcll(m & lt);
+            if (w0 + r < miss_cap)
+                miss[w0 + r] = ((unsigned long long)vi << 32) | b;
+            if (p0 + r < a.vols[vi].n)
+                a.vols[vi].miss_list[p0 + r] = b;
+        }
+    }
 }
 
 // The count / prefix / emit launches of mc.hip over (query, block) pairs of
@@ -698,9 +692,7 @@ static int eng_launch_emit(bcmc_engine *e, uint32_t nq, uint64_t total) {
     return BCMC_OK;
 }
 
-// chunk table -> count -> chunk prefixes -> emit over the step's active pairs
-static int eng_launch_mc(bcmc_engine *e, uint32_t nq, uint64_t total) {
-    const uint64_t max_chunks = eng_max_chunks(total, nq);
+static int eng_ensure_chunks(bcmc_engine *e, uint64_t max_chunks) {
     if (max_chunks > e->chunks_cap) {
         BCMC_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
         dev_free(e->chunk_verts);
@@ -712,9 +704,14 @@ static int eng_launch_mc(bcmc_engine *e, uint32_t nq, uint64_t total) {
         BCMC_TRY(scan_site_alloc(&e->site_chunks, cap, kScanTile));
         e->chunks_cap = cap;
     }
+    return BCMC_OK;
+}
+
+// count -> chunk prefixes -> emit over the step's active pairs (chunk table
+// built before the mark)
+static int eng_launch_mc(bcmc_engine *e, uint32_t nq, uint64_t total) {
+    const uint64_t max_chunks = eng_max_chunks(total, nq);
     const EngMcArgs a = eng_mc_args(e, nq);
-    eng_chunk_table_kernel<<<1, 256, 0, e->ctx->stream>>>(e->c, e->q_act_start, nq, e->q_pair_end,
-                                                          e->q_chunk_base);
     const uint32_t grid = (uint32_t)std::max<uint64_t>(
         1, std::min<uint64_t>(max_chunks, persistent_grid(e->ctx, e->ctx->tune.count)));
     eng_count_kernel<<<grid, kEngThreads, 0, e->ctx->stream>>>(a);
@@ -913,7 +910,18 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     for (uint32_t i = 0; i < nq; ++i) {
         BCMC_CHECK_ARG(volume_ids[i] < e->vols.size(), "unknown volume id");
         const EngVolume &v = e->vols[volume_ids[i]];
-        qs[i] = EngQuery{volume_ids[i], isovalues[i], (uint32_t)total, v.n, (uint32_t)rows, {0, 0, 0}};
+        EngQuery &q = qs[i];
+        q = EngQuery{};
+        q.vol = volume_ids[i];
+        q.iso = isovalues[i];
+        q.flag_base = (uint32_t)total;
+        q.n = v.n;
+        q.row_base = (uint32_t)rows;
+        q.nb0 = v.nb[0];
+        q.nsegx = div_up(v.nb[0], kSegBlocks);
+        q.dilated = v.dilated;
+        q.seg_range = v.seg_range;
+        q.slot_map = v.slot_map;
         used[volume_ids[i]] = true;
         total += v.n;
         rows += (uint64_t)div_up(v.nb[0], kSegBlocks) * v.nb[1] * v.nb[2];
@@ -932,8 +940,12 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     SegActiveOp aop{e->d_q, e->d_vols, e->cand_q, e->cand_seg, e->flags, e->act_q, e->act_b,
                     e->q_act_start, e->c};
     launch_scan<1>(ctx, e->site_active, e->site_active, aop, row_elems);
-    eng_mark_kernel<<<eng_grid(ctx, total), 256, 0, st>>>(e->d_q, e->d_vols, e->act_q, e->act_b, e->c,
-                                                          e->stamp, e->miss, e->miss_cap);
+    // marching-cubes chunks of every query (also the work units of the mark)
+    const uint64_t max_chunks = eng_max_chunks(total, nq);
+    BCMC_TRY(eng_ensure_chunks(e, max_chunks));
+    eng_chunk_table_kernel<<<1, 256, 0, st>>>(e->c, e->q_act_start, nq, e->q_pair_end, e->q_chunk_base);
+    eng_mark_kernel<<<(uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(max_chunks, persistent_grid(ctx, 8))),
+                      256, 0, st>>>(eng_mc_args(e, nq), e->stamp, e->miss, e->miss_cap);
     // eviction order (only does work when the misses exceed the free room).
     // A step misses at most every block of each volume it touches; when the
     // free room after the last step covers that, eviction cannot happen and
