@@ -46,7 +46,7 @@ struct EngVol {                 // device descriptor of a volume
     const float2 *seg_range;    // per row segment: union of its blocks' dilated ranges
     const float2 *fwd;          // per block: forward range (launch_forward_ranges)
     FastDiv div_x, div_y;       // by nb[0], nb[1]
-    int32_t *slot_map;          // block -> pool slot; -1 none, -2 claimed this step
+    int32_t *slot_map;          // block -> pool slot; -1 none
     uint32_t *miss_list;        // this step's missing blocks of the volume (decoder input)
 };
 
@@ -62,7 +62,9 @@ struct EngQuery {
     const float2 *dilated;
     const float2 *seg_range;
     int32_t *slot_map;
-    uint64_t pad2;
+    // flag bases of the earlier queries of this step on the same volume:
+    // same_list[same_off .. same_off + same_cnt)
+    uint32_t same_off, same_cnt;
 };
 
 struct EngCounters {
@@ -277,6 +279,7 @@ struct EngMcArgs {
     uint32_t capacity;
     EngCounters *c;
     uint32_t nq;
+    const uint32_t *same_list;           // see EngQuery::same_off
     const uint32_t *q_act_start;         // per query: first pair (kNoPairs: none)
     const uint32_t *q_pair_end;          // per query: one past its last pair
     const uint32_t *q_chunk_base;        // per query: its first chunk
@@ -396,54 +399,89 @@ BCMC_DEV void eng_chunk_range(const EngMcArgs &a, uint32_t ch, uint32_t q, uint3
     k1 = min(a.q_pair_end[q], k0 + kChunk);
 }
 
-// Stamp pooled blocks, claim missing ones (once per block: the claim is a
-// compare-and-swap -1 -> -2), one chunk of one query per workgroup and
-// iteration; the claims of a wave reserve their list entries with one atomic
-// per list.
+// Stamp pooled blocks and list the missing ones, each once: a missing block
+// is claimed by the first query of the step (in query order) that has it
+// active, i.e. by a pair whose earlier same-volume queries all have the
+// block's flag word below this step's epoch.  Each thread takes kMarkPairs
+// pairs whose loads are all issued before any is used.  List entries are
+// reserved per workgroup and iteration: ranks from LDS counters per volume,
+// then one global atomic per volume present (and one for the step list), so
+// that cold steps with millions of misses do not serialise on two counters.
+constexpr uint32_t kMarkPairs = 4;
+
 __global__ __launch_bounds__(256) void eng_mark_kernel(EngMcArgs a, uint32_t *stamp, unsigned long long *miss,
                                                        uint32_t miss_cap) {
-    const uint32_t nchunks = a.c->num_chunks, ep = a.c->epoch;
-    const uint32_t lane = lane_id();
-    const uint64_t lt = (1ull << lane) - 1ull;
-    for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
-        const uint32_t q = eng_query_of_chunk(a, chunk);
-        uint32_t k0, k1;
-        eng_chunk_range(a, chunk, q, k0, k1);
-        const uint32_t vi = a.qs[q].vol;
-        int32_t *sm = a.qs[q].slot_map;
-        const uint32_t k = k0 + threadIdx.x;
-        bool claimed = false;
-        uint32_t b = 0;
-        if (k < k1) {
-            b = a.act_b[k];
-            // plain load: slots of pooled blocks do not change during the
-            // kernel; the compare-and-swap settles races between claimers
-            const int32_t sl = sm[b];
-            if (sl >= 0) {
-                stamp[sl] = ep;
-            } else if (sl == -1) {
-                int32_t expect = -1;
-                claimed = __hip_atomic_compare_exchange_strong(&sm[b], &expect, -2, __ATOMIC_RELAXED,
-                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __shared__ uint32_t hist[kMaxVols], vbase[kMaxVols], lbase[kMaxVols];
+    const uint32_t n = a.c->num_active, ep = a.c->epoch;
+    const uint32_t span = blockDim.x * kMarkPairs;
+    for (uint32_t base = blockIdx.x * span; base < n; base += gridDim.x * span) {
+        if (threadIdx.x < kMaxVols)
+            hist[threadIdx.x] = 0;
+        uint32_t qi[kMarkPairs], b[kMarkPairs], vi[kMarkPairs], rank[kMarkPairs];
+        int32_t *sm[kMarkPairs];
+        int32_t sl[kMarkPairs];
+        uint32_t so[kMarkPairs], sc[kMarkPairs];
+        bool claimed[kMarkPairs];
+#pragma unroll
+        for (uint32_t j = 0; j < kMarkPairs; ++j) {
+            const uint32_t k = base + j * blockDim.x + threadIdx.x;
+            qi[j] = k < n ? a.act_q[k] : 0u;
+            b[j] = k < n ? a.act_b[k] : 0u;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kMarkPairs; ++j) {
+            const EngQuery &q = a.qs[qi[j]];
+            sm[j] = q.slot_map;
+            so[j] = q.same_off;
+            sc[j] = q.same_cnt;
+            vi[j] = q.vol;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kMarkPairs; ++j) {
+            const uint32_t k = base + j * blockDim.x + threadIdx.x;
+            // plain load: slots of pooled blocks do not change during the kernel
+            sl[j] = k < n ? sm[j][b[j]] : -3;
+        }
+        __syncthreads();  // hist cleared
+#pragma unroll
+        for (uint32_t j = 0; j < kMarkPairs; ++j) {
+            claimed[j] = false;
+            if (sl[j] >= 0) {
+                stamp[sl[j]] = ep;
+            } else if (sl[j] == -1) {
+                claimed[j] = true;
+                for (uint32_t t = 0; t < sc[j] && claimed[j]; ++t)
+                    claimed[j] = a.flags[a.same_list[so[j] + t] + b[j]] != ep;
             }
+            if (claimed[j])
+                rank[j] = atomicAdd(&hist[vi[j]], 1u);
         }
-        const uint64_t m = __ballot(claimed);
-        if (m == 0)
-            continue;
-        uint32_t w0 = 0, p0 = 0;
-        if (lane == 0) {
-            w0 = atomic_add_u32(&a.c->num_miss, (uint32_t)__popcll(m));
-            p0 = atomic_add_u32(&a.c->vol_miss[vi], (uint32_t)__popcll(m));
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = 0;
+            for (uint32_t v = 0; v < kMaxVols; ++v) {
+                lbase[v] = tot;
+                tot += hist[v];
+            }
+            const uint32_t g = tot ? atomic_add_u32(&a.c->num_miss, tot) : 0u;
+            for (uint32_t v = 0; v < kMaxVols; ++v)
+                lbase[v] += g;
         }
-        w0 = wave_read_lane(w0, 0);
-        p0 = wave_read_lane(p0, 0);
-        if (claimed) {
-            const uint32_t r = (uint32_t)__popcll(m & lt);
-            if (w0 + r < miss_cap)
-                miss[w0 + r] = ((unsigned long long)vi << 32) | b;
-            if (p0 + r < a.vols[vi].n)
-                a.vols[vi].miss_list[p0 + r] = b;
+        if (threadIdx.x < kMaxVols && hist[threadIdx.x])
+            vbase[threadIdx.x] = atomic_add_u32(&a.c->vol_miss[threadIdx.x], hist[threadIdx.x]);
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < kMarkPairs; ++j) {
+            if (!claimed[j])
+                continue;
+            const uint32_t w = lbase[vi[j]] + rank[j];
+            if (w < miss_cap)
+                miss[w] = ((unsigned long long)vi[j] << 32) | b[j];
+            const uint32_t wv = vbase[vi[j]] + rank[j];
+            if (wv < a.vols[vi[j]].n)
+                a.vols[vi[j]].miss_list[wv] = b[j];
         }
+        __syncthreads();  // LDS reused by the next iteration
     }
 }
 
@@ -580,6 +618,8 @@ struct bcmc_engine {
     uint32_t q_cap = 0;
     EngQuery *d_q = nullptr;
     uint32_t *q_act_start = nullptr, *q_pair_end = nullptr, *q_chunk_base = nullptr;
+    uint32_t *same_list = nullptr;
+    uint64_t same_cap = 0;
     unsigned long long *q_vfirst = nullptr;
     ScanSite site_active, site_cand;
     void *verts = nullptr;
@@ -660,6 +700,7 @@ static uint32_t eng_grid(const bcmc_ctx *ctx, uint64_t n, uint32_t per_cu = 8) {
 static EngMcArgs eng_mc_args(bcmc_engine *e, uint32_t nq) {
     EngMcArgs a;
     a.nq = nq;
+    a.same_list = e->same_list;
     a.q_act_start = e->q_act_start;
     a.q_pair_end = e->q_pair_end;
     a.q_chunk_base = e->q_chunk_base;
@@ -804,6 +845,7 @@ BCMC_API int bcmc_engine_destroy(bcmc_engine *e) {
     dev_free(e->q_pair_end);
     dev_free(e->q_chunk_base);
     dev_free(e->q_vfirst);
+    dev_free(e->same_list);
     dev_free(e->verts);
     scan_site_free(&e->site_active);
     scan_site_free(&e->site_cand);
@@ -906,6 +948,8 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     hipStream_t st = ctx->stream;
     std::vector<EngQuery> qs(nq);
     std::vector<bool> used(e->vols.size(), false);
+    std::vector<std::vector<uint32_t>> earlier(e->vols.size());
+    std::vector<uint32_t> same;
     uint64_t total = 0, rows = 0;
     for (uint32_t i = 0; i < nq; ++i) {
         BCMC_CHECK_ARG(volume_ids[i] < e->vols.size(), "unknown volume id");
@@ -922,6 +966,10 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
         q.dilated = v.dilated;
         q.seg_range = v.seg_range;
         q.slot_map = v.slot_map;
+        q.same_off = (uint32_t)same.size();
+        q.same_cnt = (uint32_t)earlier[q.vol].size();
+        same.insert(same.end(), earlier[q.vol].begin(), earlier[q.vol].end());
+        earlier[q.vol].push_back(q.flag_base);
         used[volume_ids[i]] = true;
         total += v.n;
         rows += (uint64_t)div_up(v.nb[0], kSegBlocks) * v.nb[1] * v.nb[2];
@@ -932,7 +980,17 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     BCMC_TRY(eng_ensure_blocks(e, total));
     BCMC_TRY(eng_ensure_rows(e, rows, row_elems));
     BCMC_TRY(eng_ensure_queries(e, nq));
+    if (same.size() > e->same_cap || !e->same_list) {
+        BCMC_HIP_CHECK(hipStreamSynchronize(st));
+        dev_free(e->same_list);
+        e->same_list = nullptr;
+        const uint64_t cap = std::max<uint64_t>(std::max<uint64_t>(same.size(), 1024), e->same_cap * 3 / 2);
+        BCMC_TRY(dev_alloc((void **)&e->same_list, cap * 4));
+        e->same_cap = cap;
+    }
     BCMC_HIP_CHECK(hipMemcpyAsync(e->d_q, qs.data(), nq * sizeof(EngQuery), hipMemcpyHostToDevice, st));
+    if (!same.empty())
+        BCMC_HIP_CHECK(hipMemcpyAsync(e->same_list, same.data(), same.size() * 4, hipMemcpyHostToDevice, st));
     eng_begin_kernel<<<1, 256, 0, st>>>(e->c, e->q_act_start, nq);
     // active pairs: candidate row segments first, then their blocks
     launch_scan<1>(ctx, e->site_rows, e->site_rows,
@@ -944,7 +1002,8 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     const uint64_t max_chunks = eng_max_chunks(total, nq);
     BCMC_TRY(eng_ensure_chunks(e, max_chunks));
     eng_chunk_table_kernel<<<1, 256, 0, st>>>(e->c, e->q_act_start, nq, e->q_pair_end, e->q_chunk_base);
-    eng_mark_kernel<<<(uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(max_chunks, persistent_grid(ctx, 8))),
+    eng_mark_kernel<<<(uint32_t)std::max<uint64_t>(
+                          1, std::min<uint64_t>(div_up(total, 256 * kMarkPairs), persistent_grid(ctx, 8))),
                       256, 0, st>>>(eng_mc_args(e, nq), e->stamp, e->miss, e->miss_cap);
     // eviction order (only does work when the misses exceed the free room).
     // A step misses at most every block of each volume it touches; when the
