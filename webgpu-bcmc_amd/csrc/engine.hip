@@ -620,6 +620,10 @@ struct bcmc_engine {
     uint32_t *q_act_start = nullptr, *q_pair_end = nullptr, *q_chunk_base = nullptr;
     uint32_t *same_list = nullptr;
     uint64_t same_cap = 0;
+    // pinned host staging of the per-step query tables and results
+    EngQuery *h_q = nullptr;
+    uint32_t *h_same = nullptr, *h_act = nullptr;
+    unsigned long long *h_vfirst = nullptr;
     unsigned long long *q_vfirst = nullptr;
     ScanSite site_active, site_cand;
     void *verts = nullptr;
@@ -683,7 +687,19 @@ static int eng_ensure_queries(bcmc_engine *e, uint32_t nq) {
     dev_free(e->q_pair_end);
     dev_free(e->q_chunk_base);
     dev_free(e->q_vfirst);
+    if (e->h_q)
+        (void)hipHostFree(e->h_q);
+    if (e->h_act)
+        (void)hipHostFree(e->h_act);
+    if (e->h_vfirst)
+        (void)hipHostFree(e->h_vfirst);
+    e->h_q = nullptr;
+    e->h_act = nullptr;
+    e->h_vfirst = nullptr;
     const uint32_t cap = std::max<uint32_t>(nq, 64);
+    BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_q, (size_t)cap * sizeof(EngQuery), hipHostMallocDefault));
+    BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_act, (size_t)cap * 4, hipHostMallocDefault));
+    BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_vfirst, (size_t)cap * 8, hipHostMallocDefault));
     BCMC_TRY(dev_alloc((void **)&e->d_q, (size_t)cap * sizeof(EngQuery)));
     BCMC_TRY(dev_alloc((void **)&e->q_act_start, (size_t)cap * 4));
     BCMC_TRY(dev_alloc((void **)&e->q_pair_end, (size_t)cap * 4));
@@ -846,6 +862,9 @@ BCMC_API int bcmc_engine_destroy(bcmc_engine *e) {
     dev_free(e->q_chunk_base);
     dev_free(e->q_vfirst);
     dev_free(e->same_list);
+    for (void *h : {(void *)e->h_q, (void *)e->h_same, (void *)e->h_act, (void *)e->h_vfirst})
+        if (h)
+            (void)hipHostFree(h);
     dev_free(e->verts);
     scan_site_free(&e->site_active);
     scan_site_free(&e->site_cand);
@@ -984,13 +1003,22 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
         BCMC_HIP_CHECK(hipStreamSynchronize(st));
         dev_free(e->same_list);
         e->same_list = nullptr;
+        if (e->h_same)
+            (void)hipHostFree(e->h_same);
+        e->h_same = nullptr;
         const uint64_t cap = std::max<uint64_t>(std::max<uint64_t>(same.size(), 1024), e->same_cap * 3 / 2);
         BCMC_TRY(dev_alloc((void **)&e->same_list, cap * 4));
+        BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_same, cap * 4, hipHostMallocDefault));
         e->same_cap = cap;
     }
-    BCMC_HIP_CHECK(hipMemcpyAsync(e->d_q, qs.data(), nq * sizeof(EngQuery), hipMemcpyHostToDevice, st));
-    if (!same.empty())
-        BCMC_HIP_CHECK(hipMemcpyAsync(e->same_list, same.data(), same.size() * 4, hipMemcpyHostToDevice, st));
+    // pinned staging: the copies run asynchronously (the step ends with a
+    // stream synchronisation before the staging is written again)
+    std::memcpy(e->h_q, qs.data(), nq * sizeof(EngQuery));
+    BCMC_HIP_CHECK(hipMemcpyAsync(e->d_q, e->h_q, nq * sizeof(EngQuery), hipMemcpyHostToDevice, st));
+    if (!same.empty()) {
+        std::memcpy(e->h_same, same.data(), same.size() * 4);
+        BCMC_HIP_CHECK(hipMemcpyAsync(e->same_list, e->h_same, same.size() * 4, hipMemcpyHostToDevice, st));
+    }
     eng_begin_kernel<<<1, 256, 0, st>>>(e->c, e->q_act_start, nq);
     // active pairs: candidate row segments first, then their blocks
     launch_scan<1>(ctx, e->site_rows, e->site_rows,
@@ -1037,10 +1065,10 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     }
     BCMC_TRY(eng_launch_mc(e, nq, total));
     BCMC_HIP_CHECK(hipMemcpyAsync(e->ch, e->c, sizeof(EngCounters), hipMemcpyDeviceToHost, st));
-    std::vector<unsigned long long> qv(nq);
-    std::vector<uint32_t> qa(nq);
-    BCMC_HIP_CHECK(hipMemcpyAsync(qv.data(), e->q_vfirst, nq * 8, hipMemcpyDeviceToHost, st));
-    BCMC_HIP_CHECK(hipMemcpyAsync(qa.data(), e->q_act_start, nq * 4, hipMemcpyDeviceToHost, st));
+    const unsigned long long *qv = e->h_vfirst;
+    const uint32_t *qa = e->h_act;
+    BCMC_HIP_CHECK(hipMemcpyAsync(e->h_vfirst, e->q_vfirst, nq * 8, hipMemcpyDeviceToHost, st));
+    BCMC_HIP_CHECK(hipMemcpyAsync(e->h_act, e->q_act_start, nq * 4, hipMemcpyDeviceToHost, st));
     BCMC_HIP_CHECK(hipStreamSynchronize(st));
     if (e->ch->short_slots) {
         set_error("decoded-block pool too small for this step: %u blocks found no slot",
