@@ -1077,7 +1077,7 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     }
     const uint64_t nv = e->ch->num_verts;
     if (nv > e->vert_cap) {
-        const uint64_t want = std::max<uint64_t>(nv, e->vert_cap * 3 / 2);
+        const uint64_t want = std::max<uint64_t>(nv, e->vert_cap * 2);
         dev_free(e->verts);
         e->verts = nullptr;
         e->vert_cap = 0;
@@ -1156,6 +1156,33 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
     const uint64_t vert_budget = opts && opts->vertex_budget_bytes ? opts->vertex_budget_bytes
                                                                     : (uint64_t)64 << 30;
     const uint32_t vb = vbytes(e->fmt);
+    // capacity planning before the clock starts: a step takes consecutive
+    // requests (FIFO), at most max_batch of them, so the largest window of
+    // max_batch consecutive requests bounds every step's block and row counts
+    {
+        uint64_t blocks = 0, rows = 0, best_b = 0, best_r = 0;
+        auto vol_rows = [&](uint32_t r) -> uint64_t {
+            const EngVolume &v = e->vols[volume_ids[r]];
+            return (uint64_t)div_up(v.nb[0], kSegBlocks) * v.nb[1] * v.nb[2];
+        };
+        for (uint32_t i = 0; i < n; ++i) {
+            BCMC_CHECK_ARG(volume_ids[i] < e->vols.size(), "unknown volume id");
+            blocks += e->vols[volume_ids[i]].n;
+            rows += vol_rows(i);
+            if (i >= max_batch) {
+                blocks -= e->vols[volume_ids[i - max_batch]].n;
+                rows -= vol_rows(i - max_batch);
+            }
+            best_b = std::max(best_b, std::min<uint64_t>(blocks, kMaxStepBlocks));
+            best_r = std::max(best_r, rows);
+        }
+        const uint32_t nq = std::min<uint32_t>(n, max_batch);
+        BCMC_TRY(eng_ensure_blocks(e, best_b));
+        if (best_r * kSegBlocks < (1ull << 32) - kScanTile)
+            BCMC_TRY(eng_ensure_rows(e, best_r, best_r * kSegBlocks));
+        BCMC_TRY(eng_ensure_queries(e, nq));
+        BCMC_TRY(eng_ensure_chunks(e, eng_max_chunks(best_b, nq)));
+    }
     // per-volume running estimate of vertices per query (largest seen)
     std::vector<uint64_t> est(e->vols.size(), 0);
     using clk = std::chrono::steady_clock;
