@@ -34,8 +34,8 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-STAGE_KERNEL = {"computeVertices": "mc_fused_kernel", "decompression": "zfp_kernel<4,3>",
-                "computeActiveBlocks": "scan_kernel<2,ActiveOp>"}
+STAGE_KERNEL = {"computeVertices": "mc_emit_kernel", "computeBlockHasVertices": "mc_count_kernel",
+                "decompression": "zfp_kernel<4,3>", "computeActiveBlocks": "scan_kernel<2,ActiveOp>"}
 
 
 def profiled_traffic(workload, stage):
@@ -57,14 +57,21 @@ def stage_bytes(s, rate, n_blocks):
     a, d, h, v = s["numActiveBlocks"], s["numBlocksDecompressed"], s["numBlocksWithVertices"], \
         s["numVertices"]
     return {
-        # ranges 8N (+ the N-byte code array written and read) + flags 4N + ids 4A
-        "computeActiveBlocks": 8 * n_blocks + 2 * n_blocks + 4 * n_blocks + 4 * a,
+        # dilated ranges 8N + item slots 4N + flags 4N + active ids 4A + miss ids 4D
+        "computeActiveBlocks": 16 * n_blocks + 4 * a + 4 * d,
         # payload 8r + block id 4 + slot 4 + 64 floats 256, per decoded block
         "decompression": d * (8 * rate + 4 + 4 + 256),
-        # fused count + offsets + emission: active id 4 + own block 256 + ghost
-        # voxels <= 244 per active block, list entry 4+8 per block with
-        # vertices, 16 B per vertex
-        "computeVertices": a * (4 + 256 + 244) + 12 * h + 16 * v,
+        # count: active id 4 + forward range 8 + count 4 per active block; per
+        # candidate its slots/flags 60, own block 256 and ghost voxels <= 244
+        # (the candidates, which the pass does not report, include the H blocks
+        # with vertices: H is used, a lower bound)
+        "computeBlockHasVertices": 16 * a + 560 * h,
+        # chunk totals read and prefixes written
+        "compactBlocksWithVertices": 20 * ((a + 255) // 256),
+        # emit: count + id 8 per active block; per block with vertices its
+        # slots/flags 60, own block 256, ghost voxels <= 244, list entry 4 + 8;
+        # 16 B per vertex
+        "computeVertices": 8 * a + 572 * h + 16 * v,
     }
 
 
@@ -299,7 +306,8 @@ def main():
     dev.sync()
 
     # per-stage device time and algorithmic bytes over the timed queries
-    stages = ["computeActiveBlocks", "cacheUpdate", "decompression", "computeVertices"]
+    stages = ["computeActiveBlocks", "cacheUpdate", "decompression", "computeBlockHasVertices",
+              "compactBlocksWithVertices", "computeVertices"]
     st_ms = {k: float(sum(r[k] for r in records)) for k in stages}
     st_bytes = {k: 0 for k in stages}
     for r in records:

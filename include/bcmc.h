@@ -149,8 +149,10 @@ typedef enum {
 
 typedef struct {
     /* perfTracker keys of compressed_marching_cubes.js:593-610, milliseconds of
-     * device time per stage (stages the reference runs separately but this
-     * library fuses report 0 in the fused-away key). */
+     * device time per stage: computeBlockHasVertices is the count launch,
+     * compactBlocksWithVertices the scan of the chunk totals, computeVertices
+     * the emit launch; compactActiveIDs is 0 (it runs inside
+     * computeActiveBlocks). */
     double computeActiveBlocks, cacheUpdate, decompression, compactActiveIDs,
         computeBlockHasVertices, compactBlocksWithVertices, computeVertices;
     double total_ms; /* wall time of the call */
@@ -273,10 +275,10 @@ BCMC_API double bcmc_ctx_last_kernel_ms(bcmc_ctx *ctx);
 
 /* Launch shapes of the context's persistent kernels: workgroups per compute
  * unit for the single-pass scans, the radix-sort passes, the zfp decoders and
- * the fused marching-cubes pass.  No reference counterpart (WebGPU dispatch
+ * the marching-cubes emit and count launches.  No reference counterpart (WebGPU dispatch
  * sizes are fixed in compressed_marching_cubes.js / exclusive_scan.js /
  * radix_sort.js); results are identical for every shape, only speed changes.
- * Defaults: scan 4, sort 4, decode 8, mc 4.  The *_ms fields are filled by
+ * Defaults: scan 4, sort 4, decode 8, mc (emit) 5, count 8.  The *_ms fields are filled by
  * bcmc_ctx_autotune (device time of the chosen shape on its probe workload)
  * and ignored by bcmc_ctx_set_tuning. */
 typedef struct bcmc_tuning {
@@ -284,10 +286,11 @@ typedef struct bcmc_tuning {
     uint32_t sort_wg_per_cu;
     uint32_t decode_wg_per_cu;
     uint32_t mc_wg_per_cu;
+    uint32_t count_wg_per_cu;
     float scan_ms;   /* 16M-element exclusive scan */
     float sort_ms;   /* 2M pairs, 24-bit keys (3 digit passes) */
     float decode_ms; /* 256K blocks of a synthetic rate-4 stream into a block cache */
-    float mc_ms;     /* not measured: the fused pass runs at its occupancy limit */
+    float mc_ms;     /* not measured: the MC launches keep their occupancy-derived shapes */
 } bcmc_tuning;
 BCMC_API int bcmc_ctx_get_tuning(bcmc_ctx *ctx, bcmc_tuning *out);
 /* Each *_wg_per_cu must be in 1..64; pipelines re-capture their graphs. */

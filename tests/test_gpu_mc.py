@@ -140,7 +140,7 @@ def test_launch_shapes_leave_results_unchanged(dev):
     d = bcmc_amd.Device(0)
     t = d.autotune()
     say("mc autotune", t)
-    assert all(1 <= t[k] <= 64 for k in ("scan", "sort", "decode", "mc"))
+    assert all(1 <= t[k] <= 64 for k in ("scan", "sort", "decode", "mc", "count"))
     assert all(v > 0 for v in t["ms"].values())
     with pytest.raises(bcmc_amd.BcmcError):
         d.set_tuning(scan=0)
@@ -148,7 +148,8 @@ def test_launch_shapes_leave_results_unchanged(dev):
     f = synth.field(synth.FOURIER, 2, dims, 128, (1, 24, 0, 0))
     s = synth.encode(f, 64 * rate)
     n = oracle.nblocks(dims)
-    for shape in (dict(scan=1, sort=1, decode=1, mc=1), dict(scan=16, sort=8, decode=32, mc=8)):
+    for shape in (dict(scan=1, sort=1, decode=1, mc=1, count=1),
+                  dict(scan=16, sort=8, decode=32, mc=8, count=16)):
         say("mc launch shape", shape)
         d.set_tuning(**shape)
         assert {k: d.tuning()[k] for k in shape} == shape
@@ -159,4 +160,4 @@ def test_launch_shapes_leave_results_unchanged(dev):
             _check_query(mc, ref, iso)
             v = mc.vertices()
             assert np.array_equal(v.view(np.uint32), ref.results()["verts_f32"].view(np.uint32))
-    d.set_tuning(**{k: t[k] for k in ("scan", "sort", "decode", "mc")})
+    d.set_tuning(**{k: t[k] for k in ("scan", "sort", "decode", "mc", "count")})

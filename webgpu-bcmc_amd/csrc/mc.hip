@@ -2,20 +2,20 @@
 //
 // One query (computeSurface, :580-809) is a fixed chain of kernels on the
 // context's stream with a single host read-back at the end:
-//   active     block active test (compute_block_active.comp) as a separable
-//              3x3x3 dilation, fused with a single-pass scan + compaction of
-//              the flags -> active ids;
+//   active     block active test (compute_block_active.comp) against a
+//              per-volume dilated range, in a single-pass scan + compaction
+//              of the flags -> active ids and LRU misses;
 //   lru        LRUCache.update over the active list (lru.hip);
 //   decode     zfp decode of the misses into their cache slots (zfp.hip);
-//   mc         one pass over the active list: per block a 5^3 LDS tile with
-//              the ghost layer (compute_vertex_values.comp:96-252), the
-//              per-cell triangle counts (compute_block_has_vertices.comp,
-//              compute_block_voxel_num_verts.comp), a two-channel look-back
-//              giving the list of blocks with vertices and their 64-bit
-//              vertex offsets, and the vertex writes
+//   mc         count: per block a 5^3 tile with the ghost layer
+//              (compute_vertex_values.comp:96-252) and the per-cell triangle
+//              counts (compute_block_has_vertices.comp,
+//              compute_block_voxel_num_verts.comp); prefix: a two-channel scan
+//              of the chunk totals giving the list of blocks with vertices and
+//              their 64-bit vertex offsets; emit: the vertex writes
 //              (compute_block_vertices.comp:76-174).
 // The vertex buffer grows like the reference (x1.5, :774-785); when a query
-// needs more room than it has, the pass is run again once it has grown.
+// needs more room than it has, the emit launch runs again once it has grown.
 #include "common.h"
 #include "kernels.h"
 #include "lru.h"
@@ -356,16 +356,21 @@ static int mc_launch_emit(bcmc_mc *m) {
     return BCMC_OK;
 }
 
-// count -> chunk prefixes -> emit
-static int mc_launch_pass(bcmc_mc *m) {
+// count -> chunk prefixes -> emit; `ev` (nullable) receives an event after
+// the count and after the prefix scan
+static int mc_launch_pass(bcmc_mc *m, hipEvent_t *ev) {
     const McArgs a = mc_args(m);
     const uint32_t max_chunks = div_up(m->total, kChunk);
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(max_chunks,
                                                                    persistent_grid(m->ctx, m->ctx->tune.count)));
     mc_count_kernel<<<grid, kMcThreads, 0, m->ctx->stream>>>(a);
+    if (ev)
+        BCMC_HIP_CHECK(hipEventRecord(ev[0], m->ctx->stream));
     launch_scan<2>(m->ctx, m->site_c0, m->site_c1,
                    ChunkPrefixOp{a.chunk_with, a.chunk_verts, a.chunk_with_pre, a.chunk_verts_pre, a.q},
                    max_chunks);
+    if (ev)
+        BCMC_HIP_CHECK(hipEventRecord(ev[1], m->ctx->stream));
     return mc_launch_emit(m);
 }
 
@@ -397,9 +402,9 @@ static int mc_enqueue_query(bcmc_mc *m, uint32_t npasses, bool timed) {
     if (timed)
         BCMC_HIP_CHECK(hipEventRecord(m->ev[3], st));
     // counts, offsets and vertices
-    BCMC_TRY(mc_launch_pass(m));
+    BCMC_TRY(mc_launch_pass(m, timed ? &m->ev[4] : nullptr));
     if (timed)
-        BCMC_HIP_CHECK(hipEventRecord(m->ev[4], st));
+        BCMC_HIP_CHECK(hipEventRecord(m->ev[6], st));
     BCMC_HIP_CHECK(hipGetLastError());
     BCMC_HIP_CHECK(hipMemcpyAsync(m->qh, m->q, sizeof(QState), hipMemcpyDeviceToHost, st));
     return BCMC_OK;
@@ -588,10 +593,10 @@ BCMC_API int bcmc_mc_compute_surface(bcmc_mc *m, float isovalue, bcmc_perf *perf
         BCMC_TRY(dev_alloc(&m->verts, want * vertex_bytes(m->fmt)));
         m->vert_cap = want;
         if (timed)
-            BCMC_HIP_CHECK(hipEventRecord(m->ev[3], st));
+            BCMC_HIP_CHECK(hipEventRecord(m->ev[5], st));
         BCMC_TRY(mc_launch_emit(m));
         if (timed)
-            BCMC_HIP_CHECK(hipEventRecord(m->ev[4], st));
+            BCMC_HIP_CHECK(hipEventRecord(m->ev[6], st));
         BCMC_HIP_CHECK(hipMemcpyAsync(m->qh, m->q, sizeof(QState), hipMemcpyDeviceToHost, st));
         BCMC_HIP_CHECK(hipStreamSynchronize(st));
         qs = *m->qh;
@@ -605,16 +610,16 @@ BCMC_API int bcmc_mc_compute_surface(bcmc_mc *m, float isovalue, bcmc_perf *perf
     if (num_vertices)
         *num_vertices = qs.num_verts;
     if (perf) {
-        float ms[4];
-        for (int i = 0; i < 4; ++i)
+        float ms[6];
+        for (int i = 0; i < 6; ++i)
             BCMC_HIP_CHECK(hipEventElapsedTime(&ms[i], m->ev[i], m->ev[i + 1]));
         perf->computeActiveBlocks = ms[0];
         perf->cacheUpdate = ms[1];
         perf->decompression = ms[2];
         perf->compactActiveIDs = 0.0;            // fused into computeActiveBlocks
-        perf->computeBlockHasVertices = 0.0;     // fused into computeVertices
-        perf->compactBlocksWithVertices = 0.0;   // fused into computeVertices
-        perf->computeVertices = ms[3];
+        perf->computeBlockHasVertices = ms[3];   // count launch
+        perf->compactBlocksWithVertices = ms[4]; // prefix scan of the chunk totals
+        perf->computeVertices = ms[5];           // emit launch (after a growth: its rerun)
         perf->numActiveBlocks = qs.num_active;
         perf->numBlocksDecompressed = qs.num_new;
         perf->numBlocksWithVertices = qs.num_with;

@@ -120,6 +120,7 @@ static void to_public(const bcmc_ctx *ctx, bcmc_tuning *t) {
     t->sort_wg_per_cu = ctx->tune.sort;
     t->decode_wg_per_cu = ctx->tune.decode;
     t->mc_wg_per_cu = ctx->tune.mc;
+    t->count_wg_per_cu = ctx->tune.count;
 }
 
 extern "C" {
@@ -133,14 +134,15 @@ BCMC_API int bcmc_ctx_get_tuning(bcmc_ctx *ctx, bcmc_tuning *out) {
 
 BCMC_API int bcmc_ctx_set_tuning(bcmc_ctx *ctx, const bcmc_tuning *t) {
     BCMC_CHECK_ARG(ctx && t, "null argument");
-    const uint32_t v[4] = {t->scan_wg_per_cu, t->sort_wg_per_cu, t->decode_wg_per_cu,
-                           t->mc_wg_per_cu};
+    const uint32_t v[5] = {t->scan_wg_per_cu, t->sort_wg_per_cu, t->decode_wg_per_cu,
+                           t->mc_wg_per_cu, t->count_wg_per_cu};
     for (uint32_t x : v)
         BCMC_CHECK_ARG(x >= 1 && x <= 64, "workgroups per CU must be in 1..64");
     ctx->tune.scan = v[0];
     ctx->tune.sort = v[1];
     ctx->tune.decode = v[2];
     ctx->tune.mc = v[3];
+    ctx->tune.count = v[4];
     ++ctx->tune_gen;
     return BCMC_OK;
 }

@@ -176,6 +176,7 @@ static napi_value ctx_autotune(napi_env env, napi_callback_info info)
     set_num(env, o, "sort", t.sort_wg_per_cu);
     set_num(env, o, "decode", t.decode_wg_per_cu);
     set_num(env, o, "mc", t.mc_wg_per_cu);
+    set_num(env, o, "count", t.count_wg_per_cu);
     set_num(env, o, "scanMs", t.scan_ms);
     set_num(env, o, "sortMs", t.sort_ms);
     set_num(env, o, "decodeMs", t.decode_ms);

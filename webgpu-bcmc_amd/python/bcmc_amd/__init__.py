@@ -76,6 +76,7 @@ class _EngineInfo(ctypes.Structure):
 class _Tuning(ctypes.Structure):
     _fields_ = [("scan_wg_per_cu", ctypes.c_uint32), ("sort_wg_per_cu", ctypes.c_uint32),
                 ("decode_wg_per_cu", ctypes.c_uint32), ("mc_wg_per_cu", ctypes.c_uint32),
+                ("count_wg_per_cu", ctypes.c_uint32),
                 ("scan_ms", ctypes.c_float), ("sort_ms", ctypes.c_float),
                 ("decode_ms", ctypes.c_float), ("mc_ms", ctypes.c_float)]
 
@@ -206,7 +207,7 @@ class Device:
         _check(lib().bcmc_ctx_info(self.h, ctypes.byref(n), name, 128))
         return {"num_cus": n.value, "name": name.value.decode()}
 
-    _TUNE_KEYS = ("scan", "sort", "decode", "mc")
+    _TUNE_KEYS = ("scan", "sort", "decode", "mc", "count")
 
     @staticmethod
     def _tuning_dict(t):
