@@ -43,7 +43,7 @@ def profiled_traffic(workload, stage):
     committed FETCH_SIZE/WRITE_SIZE counter passes (copy-calibrated), or None."""
     import re
     kernel = STAGE_KERNEL.get(stage)
-    path = os.path.join(ROOT, "profiles", f"{kernel.split('<')[0]}_{workload}_hbm_traffic.md") \
+    path = os.path.join(ROOT, "profiles", f"{kernel.split('<')[0]}_{workload}_hbm_traffic.txt") \
         if kernel else None
     if not path or not os.path.exists(path):
         return None, None
