@@ -85,3 +85,18 @@ def test_sort(dev, n, reverse, keyset):
     sorter.sort(kb, vb, n, reverse)
     want, _ = oracle.sort_by_key(keys, vals, reverse)
     assert np.array_equal(vb.download(np.uint32, n), want)
+
+
+def test_scan_after_scratch_growth(dev):
+    """Each larger scan reallocates the context's look-back state; the first
+    scan on the new state must not see the previous state's status words."""
+    d = bcmc_amd.Device(0)
+    rng = np.random.default_rng(12)
+    for n in (3000, 70000, 300000, 1200000, 2500000):
+        data = rng.integers(0, 1000, n).astype(np.uint32)
+        pipe = bcmc_amd.ExclusiveScanPipeline(d)
+        sc = pipe.prepareInput(data)
+        total = sc.scan(n)
+        want, wt = oracle.exclusive_scan(data)
+        assert total == wt, n
+        assert np.array_equal(sc.inputBuf.download(np.uint32, n), want), n

@@ -80,6 +80,9 @@ int scan_site_alloc(ScanSite *s, uint64_t max_elems, uint32_t tile) {
     BCMC_TRY(dev_alloc((void **)&s->status, (size_t)s->max_tiles * 8));
     BCMC_HIP_CHECK(hipMemset(s->ctrl, 0, 16));
     BCMC_HIP_CHECK(hipMemset(s->status, 0, (size_t)s->max_tiles * 8));
+    // the memsets run on the null stream, which the library's non-blocking
+    // streams do not wait for: finish them before any kernel reads the words
+    BCMC_HIP_CHECK(hipDeviceSynchronize());
     return BCMC_OK;
 }
 

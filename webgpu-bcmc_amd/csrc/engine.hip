@@ -84,9 +84,12 @@ struct EngCounters {
 
 constexpr uint32_t kMaxVols = 64;
 
-// query of a concatenated block row (row_base ascending)
-BCMC_DEV uint32_t query_of_row(const EngQuery *qs, uint32_t nq, uint32_t r) {
-    uint32_t lo = 0, hi = nq;
+// query of a concatenated row segment (row_base ascending): a binary search
+// between the queries of the scan tile's first and last rows (tile_q[t] is the
+// query of row t * kScanTile)
+BCMC_DEV uint32_t query_of_row(const EngQuery *qs, const uint32_t *tile_q, uint32_t r) {
+    const uint32_t t = r / kScanTile;
+    uint32_t lo = tile_q[t], hi = tile_q[t + 1] + 1;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) / 2;
         if (qs[mid].row_base <= r)
@@ -142,7 +145,7 @@ __global__ void eng_seg_range_kernel(const float2 *dilated, uint32_t nbx, uint32
 // Candidate segments of every query: iso within the segment's dilated range.
 struct CandSegOp {
     const EngQuery *qs;
-    uint32_t nq;
+    const uint32_t *tile_q;
     const EngVol *vols;
     uint32_t *cand_q, *cand_seg;
     EngCounters *c;
@@ -150,14 +153,14 @@ struct CandSegOp {
     BCMC_DEV uint32_t size() const { return nsegs; }
     BCMC_DEV void begin() const {}
     BCMC_DEV void value(uint32_t r, uint32_t (&v)[1]) const {
-        const uint32_t qi = query_of_row(qs, nq, r);
+        const uint32_t qi = query_of_row(qs, tile_q, r);
         const EngQuery &q = qs[qi];
         const float2 rr = q.seg_range[r - q.row_base];
         v[0] = (q.iso >= rr.x && q.iso <= rr.y) ? 1u : 0u;
     }
     BCMC_DEV void emit(uint32_t r, const uint32_t (&v)[1], const unsigned long long (&e)[1]) const {
         if (v[0]) {
-            const uint32_t qi = query_of_row(qs, nq, r);
+            const uint32_t qi = query_of_row(qs, tile_q, r);
             cand_q[e[0]] = qi;
             cand_seg[e[0]] = r - qs[qi].row_base;
         }
@@ -620,6 +623,8 @@ struct bcmc_engine {
     uint32_t *q_act_start = nullptr, *q_pair_end = nullptr, *q_chunk_base = nullptr;
     uint32_t *same_list = nullptr;
     uint64_t same_cap = 0;
+    uint32_t *tile_q = nullptr, *h_tile_q = nullptr;  // query of each scan tile's first row segment
+    uint64_t tile_q_cap = 0;
     // pinned host staging of the per-step query tables and results
     EngQuery *h_q = nullptr;
     uint32_t *h_same = nullptr, *h_act = nullptr;
@@ -862,7 +867,9 @@ BCMC_API int bcmc_engine_destroy(bcmc_engine *e) {
     dev_free(e->q_chunk_base);
     dev_free(e->q_vfirst);
     dev_free(e->same_list);
-    for (void *h : {(void *)e->h_q, (void *)e->h_same, (void *)e->h_act, (void *)e->h_vfirst})
+    dev_free(e->tile_q);
+    for (void *h : {(void *)e->h_q, (void *)e->h_same, (void *)e->h_act, (void *)e->h_vfirst,
+                    (void *)e->h_tile_q})
         if (h)
             (void)hipHostFree(h);
     dev_free(e->verts);
@@ -1011,8 +1018,29 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
         BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_same, cap * 4, hipHostMallocDefault));
         e->same_cap = cap;
     }
+    // query of the first row segment of every scan tile (+ a last entry)
+    const uint64_t ntiles = div_up(rows, kScanTile) + 1;
+    if (ntiles > e->tile_q_cap) {
+        BCMC_HIP_CHECK(hipStreamSynchronize(st));
+        dev_free(e->tile_q);
+        e->tile_q = nullptr;
+        if (e->h_tile_q)
+            (void)hipHostFree(e->h_tile_q);
+        e->h_tile_q = nullptr;
+        const uint64_t cap = std::max<uint64_t>(ntiles, e->tile_q_cap * 2);
+        BCMC_TRY(dev_alloc((void **)&e->tile_q, cap * 4));
+        BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_tile_q, cap * 4, hipHostMallocDefault));
+        e->tile_q_cap = cap;
+    }
+    for (uint64_t t = 0, q = 0; t < ntiles; ++t) {
+        const uint64_t r = std::min<uint64_t>(t * kScanTile, rows ? rows - 1 : 0);
+        while (q + 1 < nq && qs[q + 1].row_base <= r)
+            ++q;
+        e->h_tile_q[t] = (uint32_t)q;
+    }
     // pinned staging: the copies run asynchronously (the step ends with a
     // stream synchronisation before the staging is written again)
+    BCMC_HIP_CHECK(hipMemcpyAsync(e->tile_q, e->h_tile_q, ntiles * 4, hipMemcpyHostToDevice, st));
     std::memcpy(e->h_q, qs.data(), nq * sizeof(EngQuery));
     BCMC_HIP_CHECK(hipMemcpyAsync(e->d_q, e->h_q, nq * sizeof(EngQuery), hipMemcpyHostToDevice, st));
     if (!same.empty()) {
@@ -1022,7 +1050,7 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     eng_begin_kernel<<<1, 256, 0, st>>>(e->c, e->q_act_start, nq);
     // active pairs: candidate row segments first, then their blocks
     launch_scan<1>(ctx, e->site_rows, e->site_rows,
-                   CandSegOp{e->d_q, nq, e->d_vols, e->cand_q, e->cand_seg, e->c, (uint32_t)rows}, rows);
+                   CandSegOp{e->d_q, e->tile_q, e->d_vols, e->cand_q, e->cand_seg, e->c, (uint32_t)rows}, rows);
     SegActiveOp aop{e->d_q, e->d_vols, e->cand_q, e->cand_seg, e->flags, e->act_q, e->act_b,
                     e->q_act_start, e->c};
     launch_scan<1>(ctx, e->site_active, e->site_active, aop, row_elems);
