@@ -29,97 +29,75 @@ __host__ __device__ constexpr int perm3d(int i) {
            16 * (kPermXYZ[3 * i + 2] - '0');
 }
 
-// LSB-first reader over u64 words (LDS or global).  buf holds nb unread bits;
-// the next word is loaded only when more than nb bits are needed, so a block
-// never reads past its own last bit.
-struct BitReader {
+// LSB-first window over u64 words (LDS or global): the 64 bits from bit
+// position p of the block, read from words i and i + 1 of its first word
+// (bits past the block are junk and are masked off by the caller).  No
+// reader state: every read is one pair of word loads and a funnel shift, so
+// the decoder loops carry only the bit position.
+// kGuard: words at or past nw read as 0 (a block read straight from a stream
+// buffer of nw words; LDS-staged words carry their own zero padding).
+template <bool kGuard>
+struct BitWindow {
     const uint64_t *w;
-    uint32_t idx;
-    uint32_t nb;
-    uint64_t buf;
-
-    BCMC_DEV void init(const uint64_t *words, uint32_t first_bit) {
-        w = words;
-        idx = 1;
-        const uint64_t w0 = words[0];
-        buf = first_bit ? (w0 >> first_bit) : w0;
-        nb = 64 - first_bit;
-    }
-    // next n bits (1 <= n <= 64) without consuming them; bits past n are junk
-    BCMC_DEV uint64_t peek(uint32_t n) const {
-        if (n <= nb)
-            return buf;
-        const uint64_t nx = w[idx];
-        return nb ? (buf | (nx << nb)) : nx;
-    }
-    BCMC_DEV void skip(uint32_t n) {  // n <= 64
-        if (n <= nb) {
-            buf = n == 64 ? 0ull : (buf >> n);
-            nb -= n;
+    uint32_t p0;  // the block's first bit within w[0]
+    uint64_t nw;  // words readable from w (kGuard)
+    BCMC_DEV uint64_t at(uint32_t p) const {
+        const uint32_t a = p0 + p, i = a >> 6, o = a & 63u;
+        uint64_t lo, hi;
+        if constexpr (kGuard) {
+            lo = i < nw ? w[i] : 0ull;
+            hi = i + 1 < nw ? w[i + 1] : 0ull;
         } else {
-            const uint64_t nx = w[idx++];
-            const uint32_t r = n - nb;
-            buf = r == 64 ? 0ull : (nx >> r);
-            nb = 64 - r;
+            lo = w[i];
+            hi = w[i + 1];
         }
-    }
-    BCMC_DEV uint64_t read(uint32_t n) {  // n <= 64
-        if (n == 0)
-            return 0;
-        const uint64_t v = peek(n);
-        skip(n);
-        return n == 64 ? v : (v & ((1ull << n) - 1ull));
-    }
-    BCMC_DEV uint32_t bit() {
-        if (nb == 0) {
-            buf = w[idx++];
-            nb = 64;
-        }
-        const uint32_t b = (uint32_t)buf & 1u;
-        buf >>= 1;
-        --nb;
-        return b;
+        return o ? ((lo >> o) | (hi << (64u - o))) : lo;
     }
 };
 
 // Decode one block into v[64] (raster order); an all-zero block (flag bit 0)
 // decodes to +0.
-BCMC_DEV void decode_block(BitReader &rd, uint32_t maxbits, float (&v)[64]) {
-    if (rd.bit() == 0) {
+template <bool kGuard>
+BCMC_DEV void decode_block(const BitWindow<kGuard> &bw, uint32_t maxbits, float (&v)[64]) {
+    const uint64_t head = bw.at(0);
+    if ((head & 1u) == 0) {
 #pragma unroll
         for (int i = 0; i < 64; ++i)
             v[i] = 0.0f;
         return;
     }
-    const int emax = (int)rd.read(8) - 127;
+    const int emax = (int)((head >> 1) & 0xFFu) - 127;
     uint32_t c[64];
 #pragma unroll
     for (int i = 0; i < 64; ++i)
         c[i] = 0;
-    uint32_t bits = maxbits - 9, n = 0;
+    uint32_t p = 9, bits = maxbits - 9, n = 0;
     for (uint32_t k = 32; bits != 0 && k-- > 0;) {
+        // the n coefficients already significant: one bit each, verbatim
         const uint32_t m = n < bits ? n : bits;
         bits -= m;
-        uint64_t x = rd.read(m);
-        // group tests + unary runs (zfp_decompress.comp:282-287)
+        uint64_t x = 0;
+        if (m) {
+            x = bw.at(p) & (m == 64 ? ~0ull : ((1ull << m) - 1ull));
+            p += m;
+        }
+        // group tests + unary runs (zfp_decompress.comp:282-287): a group
+        // bit 1 is followed by zeros up to the next significant coefficient
+        // (one count-trailing-zeros over the window), or by 63 - n zeros when
+        // the run reaches the last coefficient, which is then implied
         while (n < 64 && bits != 0) {
+            const uint64_t win = bw.at(p);
             --bits;
-            if (rd.bit() == 0)
+            ++p;
+            if ((win & 1u) == 0)
                 break;
             const uint32_t lim = min(63u - n, bits);
-            if (lim) {
-                const uint64_t pk = rd.peek(lim) & ((1ull << lim) - 1ull);
-                if (pk) {
-                    const uint32_t p = (uint32_t)__builtin_ctzll(pk);
-                    rd.skip(p + 1);
-                    bits -= p + 1;
-                    n += p;
-                } else {
-                    rd.skip(lim);
-                    bits -= lim;
-                    n += lim;
-                }
-            }
+            const uint64_t pk = (win >> 1) & ((1ull << lim) - 1ull);
+            const uint32_t q = pk ? (uint32_t)__builtin_ctzll(pk) : lim;
+            const uint32_t adv = pk ? q + 1 : lim;
+            p += adv;
+            bits -= adv;
+            n += q;
             x |= 1ull << n;
             ++n;
         }
@@ -226,7 +204,7 @@ struct ZfpArgs {
 // W = u64 words per block when 64 | maxbits (LDS-staged), 0 = generic.
 template <int W, int MODE>
 __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
-    constexpr int kStage = W > 0 ? 64 * W + 1 : 1;
+    constexpr int kStage = W > 0 ? 64 * W + 2 : 1;
     __shared__ uint64_t stage[kZfpThreads / 64][kStage];
     const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
     uint32_t n = a.n_dev ? *a.n_dev : a.n_host;
@@ -244,7 +222,7 @@ __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
         const uint32_t i = chunk * 64 + lane;
         const bool live = i < n;
         const uint32_t blk = live ? (a.ids ? a.ids[i] : i) : 0u;
-        BitReader rd;
+        float v[64];
         if constexpr (W > 0) {
             uint64_t *sw = stage[wave];
             // coalesced copy of the wave's fixed-rate words into LDS
@@ -258,21 +236,20 @@ __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
                 }
                 sw[j] = word;
             }
-            if (lane == 0)
-                sw[64 * W] = 0;
+            if (lane < 2)
+                sw[64 * W + lane] = 0;
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            rd.init(sw + lane * W, 0);
-        } else {
-            const uint64_t bit0 = (uint64_t)blk * a.maxbits;
-            rd.init(a.stream + (bit0 >> 6), (uint32_t)(bit0 & 63));
-        }
-        float v[64];
-        if (live)
-            decode_block(rd, a.maxbits, v);
-        if constexpr (W > 0)
+            if (live)
+                decode_block(BitWindow<false>{sw + lane * W, 0u, 0ull}, a.maxbits, v);
             __builtin_amdgcn_wave_barrier();
+        } else {
+            const uint64_t bit0 = (uint64_t)blk * a.maxbits, w0 = bit0 >> 6;
+            if (live)
+                decode_block(BitWindow<true>{a.stream + w0, (uint32_t)(bit0 & 63), a.nwords > w0 ? a.nwords - w0 : 0ull},
+                             a.maxbits, v);
+        }
         if (!live)
             continue;
         if constexpr (MODE == kModeRanges) {
