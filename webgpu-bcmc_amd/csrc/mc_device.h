@@ -361,6 +361,9 @@ struct alignas(16) EmitScratch {
     uint32_t meta[kRowBlocks][kMetaWords];  // meta words of the group's blocks
 };
 
+typedef float V4f __attribute__((ext_vector_type(4)));
+typedef uint32_t V2u __attribute__((ext_vector_type(2)));
+
 BCMC_DEV void row_store(float *dst, const float (&r)[5]) {
     *reinterpret_cast<float4 *>(dst) = make_float4(r[0], r[1], r[2], r[3]);
     dst[4] = r[4];
@@ -464,20 +467,25 @@ BCMC_DEV void emit_row_blocks(const RowBlock &b, const RowTile &t, const McTable
             const float pz = (ei.a[2] + tt * ei.d[2] + (float)cz) + 0.5f;
             const unsigned long long o = out0 + v;
             const uint32_t bid = es.bid[j];
+            // vertices are written once and not read back by the pass:
+            // non-temporal stores keep them from displacing the block data
+            // in the caches
             if (fmt == BCMC_VERTS_LOCAL_F32) {
-                reinterpret_cast<float4 *>(verts)[o] = make_float4(px, py, pz, __uint_as_float(bid));
+                const V4f vv = {px, py, pz, __uint_as_float(bid)};
+                __builtin_nontemporal_store(vv, reinterpret_cast<V4f *>(verts) + o);
             } else if (fmt == BCMC_VERTS_GLOBAL_F32) {
                 float *gp = reinterpret_cast<float *>(verts) + 3 * o;
-                gp[0] = (float)(4 * es.bpos[j][0]) + px;
-                gp[1] = (float)(4 * es.bpos[j][1]) + py;
-                gp[2] = (float)(4 * es.bpos[j][2]) + pz;
+                __builtin_nontemporal_store((float)(4 * es.bpos[j][0]) + px, gp);
+                __builtin_nontemporal_store((float)(4 * es.bpos[j][1]) + py, gp + 1);
+                __builtin_nontemporal_store((float)(4 * es.bpos[j][2]) + pz, gp + 2);
             } else {
                 // compress_position (block_vertex_position.comp:9-17)
                 const uint32_t qx = (uint32_t)(((px - 0.5f) * 0.25f) * 1023.0f);
                 const uint32_t qy = (uint32_t)(((py - 0.5f) * 0.25f) * 1023.0f);
                 const uint32_t qz = (uint32_t)(((pz - 0.5f) * 0.25f) * 1023.0f);
                 const uint32_t packed = ((qx & 0x3ffu) << 20) | ((qy & 0x3ffu) << 10) | (qz & 0x3ffu);
-                reinterpret_cast<uint2 *>(verts)[o] = make_uint2(packed, bid);
+                const V2u vv = {packed, bid};
+                __builtin_nontemporal_store(vv, reinterpret_cast<V2u *>(verts) + o);
             }
         }
     }
