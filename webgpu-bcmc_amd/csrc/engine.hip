@@ -142,12 +142,23 @@ __global__ void eng_seg_range_kernel(const float2 *dilated, uint32_t nbx, uint32
     }
 }
 
+// A candidate row segment with everything its active test and output need,
+// so that the per-block work reads one 32-byte descriptor and one range.
+struct CandRow {
+    const float2 *dilated;  // range of the segment's first block
+    uint32_t b0;            // the segment's first block
+    uint32_t qi;            // query
+    float iso;
+    uint32_t nx;            // blocks of the segment inside the volume (<= kSegBlocks)
+    uint32_t flag_base;     // the query's first index in the flag words
+    uint32_t pad;
+};
+
 // Candidate segments of every query: iso within the segment's dilated range.
 struct CandSegOp {
     const EngQuery *qs;
     const uint32_t *tile_q;
-    const EngVol *vols;
-    uint32_t *cand_q, *cand_seg;
+    CandRow *cand;
     EngCounters *c;
     uint32_t nsegs;
     BCMC_DEV uint32_t size() const { return nsegs; }
@@ -161,8 +172,18 @@ struct CandSegOp {
     BCMC_DEV void emit(uint32_t r, const uint32_t (&v)[1], const unsigned long long (&e)[1]) const {
         if (v[0]) {
             const uint32_t qi = query_of_row(qs, tile_q, r);
-            cand_q[e[0]] = qi;
-            cand_seg[e[0]] = r - qs[qi].row_base;
+            const EngQuery &q = qs[qi];
+            const uint32_t seg = r - q.row_base;
+            const uint32_t x0 = (seg % q.nsegx) * kSegBlocks, row = seg / q.nsegx;
+            CandRow cr;
+            cr.b0 = x0 + q.nb0 * row;
+            cr.dilated = q.dilated + cr.b0;
+            cr.qi = qi;
+            cr.iso = q.iso;
+            cr.nx = min(kSegBlocks, q.nb0 - x0);
+            cr.flag_base = q.flag_base;
+            cr.pad = 0;
+            cand[e[0]] = cr;
         }
     }
     BCMC_DEV void total(const unsigned long long (&t)[1]) const { c->num_rows = (uint32_t)t[0]; }
@@ -173,38 +194,26 @@ struct CandSegOp {
 // (query, block) pairs in query order, ascending blocks; flag words stamped
 // with the step epoch; the first pair of each query.
 struct SegActiveOp {
-    const EngQuery *qs;
-    const EngVol *vols;
-    const uint32_t *cand_q, *cand_seg;
+    const CandRow *cand;
     uint32_t *flags, *act_q, *act_b, *q_act_start;
     EngCounters *c;
     BCMC_DEV uint32_t size() const { return c->num_rows * kSegBlocks; }
     BCMC_DEV void begin() const {}
-    // block of element i (>= nb[0] in x: padding)
-    BCMC_DEV bool block_of(uint32_t i, const EngQuery &q, uint32_t &x, uint32_t &row) const {
-        const uint32_t seg = cand_seg[i / kSegBlocks];
-        x = (seg % q.nsegx) * kSegBlocks + i % kSegBlocks;
-        row = seg / q.nsegx;
-        return x < q.nb0;
-    }
     BCMC_DEV void value(uint32_t i, uint32_t (&v)[1]) const {
-        const EngQuery &q = qs[cand_q[i / kSegBlocks]];
-        uint32_t x, row;
-        v[0] = (block_of(i, q, x, row) && block_active(q.dilated[x + q.nb0 * row], q.iso)) ? 1u : 0u;
+        const CandRow &cr = cand[i / kSegBlocks];
+        const uint32_t j = i % kSegBlocks;
+        v[0] = (j < cr.nx && block_active(cr.dilated[j], cr.iso)) ? 1u : 0u;
     }
     BCMC_DEV void emit(uint32_t i, const uint32_t (&v)[1], const unsigned long long (&e)[1]) const {
-        const uint32_t k = i / kSegBlocks;
-        const uint32_t qi = cand_q[k];
-        if (i % kSegBlocks == 0 && (k == 0 || cand_q[k - 1] != qi))
-            q_act_start[qi] = (uint32_t)e[0];
+        const uint32_t k = i / kSegBlocks, j = i % kSegBlocks;
+        const CandRow &cr = cand[k];
+        if (j == 0 && (k == 0 || cand[k - 1].qi != cr.qi))
+            q_act_start[cr.qi] = (uint32_t)e[0];
         if (v[0]) {
-            const EngQuery &q = qs[qi];
-            uint32_t x, row;
-            block_of(i, q, x, row);
-            const uint32_t b = x + q.nb0 * row;
-            act_q[e[0]] = qi;
+            const uint32_t b = cr.b0 + j;
+            act_q[e[0]] = cr.qi;
             act_b[e[0]] = b;
-            flags[q.flag_base + b] = c->epoch;
+            flags[cr.flag_base + b] = c->epoch;
         }
     }
     BCMC_DEV void total(const unsigned long long (&t)[1]) const { c->num_active = (uint32_t)t[0]; }
@@ -616,7 +625,7 @@ struct bcmc_engine {
     uint64_t chunks_cap = 0;
     ScanSite site_chunks;
     uint64_t rows_cap = 0, row_elems_cap = 0;
-    uint32_t *cand_q = nullptr, *cand_seg = nullptr;
+    CandRow *cand_rows = nullptr;
     ScanSite site_rows;
     uint32_t q_cap = 0;
     EngQuery *d_q = nullptr;
@@ -664,12 +673,10 @@ static int eng_ensure_blocks(bcmc_engine *e, uint64_t total) {
 static int eng_ensure_rows(bcmc_engine *e, uint64_t rows, uint64_t row_elems) {
     if (rows > e->rows_cap) {
         BCMC_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
-        dev_free(e->cand_q);
-        dev_free(e->cand_seg);
+        dev_free(e->cand_rows);
         scan_site_free(&e->site_rows);
         const uint64_t cap = std::max<uint64_t>(rows, e->rows_cap * 3 / 2);
-        BCMC_TRY(dev_alloc((void **)&e->cand_q, cap * 4));
-        BCMC_TRY(dev_alloc((void **)&e->cand_seg, cap * 4));
+        BCMC_TRY(dev_alloc((void **)&e->cand_rows, cap * sizeof(CandRow)));
         BCMC_TRY(scan_site_alloc(&e->site_rows, cap, kScanTile));
         e->rows_cap = cap;
     }
@@ -855,8 +862,7 @@ BCMC_API int bcmc_engine_destroy(bcmc_engine *e) {
     dev_free(e->flags);
     dev_free(e->act_q);
     dev_free(e->act_b);
-    dev_free(e->cand_q);
-    dev_free(e->cand_seg);
+    dev_free(e->cand_rows);
     scan_site_free(&e->site_rows);
     dev_free(e->counts);
     dev_free(e->chunk_verts);
@@ -1050,9 +1056,8 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     eng_begin_kernel<<<1, 256, 0, st>>>(e->c, e->q_act_start, nq);
     // active pairs: candidate row segments first, then their blocks
     launch_scan<1>(ctx, e->site_rows, e->site_rows,
-                   CandSegOp{e->d_q, e->tile_q, e->d_vols, e->cand_q, e->cand_seg, e->c, (uint32_t)rows}, rows);
-    SegActiveOp aop{e->d_q, e->d_vols, e->cand_q, e->cand_seg, e->flags, e->act_q, e->act_b,
-                    e->q_act_start, e->c};
+                   CandSegOp{e->d_q, e->tile_q, e->cand_rows, e->c, (uint32_t)rows}, rows);
+    SegActiveOp aop{e->cand_rows, e->flags, e->act_q, e->act_b, e->q_act_start, e->c};
     launch_scan<1>(ctx, e->site_active, e->site_active, aop, row_elems);
     // marching-cubes chunks of every query (also the work units of the mark)
     const uint64_t max_chunks = eng_max_chunks(total, nq);
