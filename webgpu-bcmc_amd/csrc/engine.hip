@@ -34,6 +34,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <initializer_list>
 #include <thread>
 #include <vector>
 
@@ -44,7 +45,7 @@ struct EngVol {                 // device descriptor of a volume
     uint32_t n;
     const float2 *dilated;      // per block: range over its 3x3x3 neighbourhood (block_active)
     const float2 *seg_range;    // per row segment: union of its blocks' dilated ranges
-    const float2 *fwd;          // per block: forward range (launch_forward_ranges)
+    const float2 *tile;         // per block: tile range (launch_tile_ranges)
     FastDiv div_x, div_y;       // by nb[0], nb[1]
     int32_t *slot_map;          // block -> pool slot; -1 none
     uint32_t *miss_list;        // this step's missing blocks of the volume (decoder input)
@@ -515,10 +516,10 @@ __global__ __launch_bounds__(kEngThreads) void eng_count_kernel(EngMcArgs a) {
         eng_chunk_range(a, chunk, q, k0, k1);
         RowBlock ub;
         eng_row_block(a, q, 0, true, epoch, ub);
-        const float2 *fwd = a.vols[a.qs[q].vol].fwd;
+        const float2 *tile = a.vols[a.qs[q].vol].tile;
         const bool in = k0 + threadIdx.x < k1;
         const uint32_t bid = in ? a.act_b[k0 + threadIdx.x] : 0u;
-        const uint32_t ncand = chunk_prologue(cs, in, bid, q, in ? fwd[bid] : make_float2(1.0f, 0.0f), ub.iso,
+        const uint32_t ncand = chunk_prologue(cs, in, bid, q, in ? tile[bid] : make_float2(1.0f, 0.0f), ub.iso,
                                               lds_w);
         count_sweep(cs, ncand, wave, tab, [&](uint32_t e, RowBlock &b) {
             b = ub;
@@ -601,7 +602,7 @@ struct EngVolume {
     uint32_t dims[3], nb[3], n, maxbits;
     uint64_t *stream = nullptr;
     uint64_t nwords = 0;
-    float2 *ranges = nullptr, *dilated = nullptr, *seg_range = nullptr, *fwd = nullptr;
+    float2 *ranges = nullptr, *dilated = nullptr, *seg_range = nullptr, *tile = nullptr;
     int32_t *slot_map = nullptr;
     uint32_t *miss_list = nullptr;  // per-step block ids to decode (EngVol::miss_list)
 };
@@ -845,7 +846,7 @@ BCMC_API int bcmc_engine_destroy(bcmc_engine *e) {
         dev_free(v.ranges);
         dev_free(v.seg_range);
         dev_free(v.dilated);
-        dev_free(v.fwd);
+        dev_free(v.tile);
         dev_free(v.slot_map);
         dev_free(v.miss_list);
     }
@@ -907,6 +908,23 @@ BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t
     v.maxbits = (uint32_t)mb;
     BCMC_CHECK_ARG(bytes >= (n * v.maxbits + 7) / 8, "compressed stream shorter than the volume");
     bcmc_ctx *ctx = e->ctx;
+    float2 *parts = nullptr;  // 8 part ranges per block, transient
+    // frees the volume's arrays on an error return; disarmed once it is registered
+    struct VolGuard {
+        EngVolume *v;
+        float2 **parts;
+        bcmc_ctx *ctx;
+        bool armed = true;
+        ~VolGuard() {
+            if (!armed)
+                return;
+            (void)hipStreamSynchronize(ctx->stream);
+            for (void *p : {(void *)v->stream, (void *)v->ranges, (void *)v->seg_range,
+                            (void *)v->dilated, (void *)v->tile, (void *)v->slot_map,
+                            (void *)v->miss_list, (void *)*parts})
+                dev_free(p);
+        }
+    } guard{&v, &parts, ctx};
     v.nwords = (bytes + 7) / 8 + 2;
     BCMC_TRY(dev_alloc((void **)&v.stream, v.nwords * 8));
     BCMC_HIP_CHECK(hipMemsetAsync(v.stream, 0, v.nwords * 8, ctx->stream));
@@ -915,11 +933,10 @@ BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t
     BCMC_TRY(dev_alloc((void **)&v.slot_map, n * 4));
     BCMC_TRY(dev_alloc((void **)&v.miss_list, n * 4));
     BCMC_HIP_CHECK(hipMemsetAsync(v.slot_map, 0xFF, n * 4, ctx->stream));
-    float2 *full = nullptr;
-    BCMC_TRY(dev_alloc((void **)&full, n * 8));
-    BCMC_TRY(dev_alloc((void **)&v.fwd, n * 8));
-    launch_block_ranges(ctx, v.stream, v.nwords, v.maxbits, v.dims, v.ranges, full);
-    launch_forward_ranges(ctx, full, v.nb, v.fwd);
+    BCMC_TRY(dev_alloc((void **)&parts, n * 64));
+    BCMC_TRY(dev_alloc((void **)&v.tile, n * 8));
+    launch_block_ranges(ctx, v.stream, v.nwords, v.maxbits, v.dims, v.ranges, parts);
+    launch_tile_ranges(ctx, parts, v.nb, v.tile);
     const uint32_t nsegx = div_up(v.nb[0], kSegBlocks), nsegs = nsegx * v.nb[1] * v.nb[2];
     BCMC_TRY(dev_alloc((void **)&v.dilated, n * 8));
     BCMC_TRY(dev_alloc((void **)&v.seg_range, (size_t)nsegs * 8));
@@ -928,8 +945,9 @@ BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t
                                                                          v.seg_range);
     BCMC_HIP_CHECK(hipGetLastError());
     BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    dev_free(v.ranges);  // only the dilated and forward ranges are used from here on
-    dev_free(full);
+    dev_free(v.ranges);  // only the dilated and tile ranges are used from here on
+    dev_free(parts);
+    parts = nullptr;
     v.ranges = nullptr;
     EngVol dv;
     for (int i = 0; i < 3; ++i)
@@ -939,13 +957,14 @@ BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t
     dv.div_y = make_fast_div(v.nb[1]);
     dv.dilated = v.dilated;
     dv.seg_range = v.seg_range;
-    dv.fwd = v.fwd;
+    dv.tile = v.tile;
     dv.slot_map = v.slot_map;
     dv.miss_list = v.miss_list;
     const uint32_t id = (uint32_t)e->vols.size();
     BCMC_HIP_CHECK(hipMemcpyAsync(e->d_vols + id, &dv, sizeof(dv), hipMemcpyHostToDevice, ctx->stream));
     BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     e->vols.push_back(v);
+    guard.armed = false;
     // miss list capacity: every block of every volume
     uint64_t all = 0;
     for (auto &w : e->vols)

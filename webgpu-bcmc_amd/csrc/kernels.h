@@ -157,7 +157,7 @@ inline void launch_scan(bcmc_ctx *ctx, const ScanSite &s0, const ScanSite &s1, c
 // Launchers implemented in the .hip files
 // ---------------------------------------------------------------------------
 void launch_block_ranges(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords, uint32_t maxbits,
-                         const uint32_t dims[3], float2 *ranges, float2 *full_ranges = nullptr);
+                         const uint32_t dims[3], float2 *ranges, float2 *part_ranges = nullptr);
 void launch_decode_to_cache(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords,
                             uint32_t maxbits, const uint32_t *ids, const uint32_t *n_dev,
                             uint32_t n_max, const int32_t *item_slot, float *cache,
@@ -172,11 +172,12 @@ void launch_decode_assign(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords
                           QState *q);
 // Per block the range [min lo, max hi] over its in-bounds 3x3x3 neighbourhood
 // (the active test's interval, see block_active).
-// Per block the range of every decoded voxel its cells can read (its own
-// block and its in-grid forward neighbours, padding voxels included), from the
-// full-block ranges of the range pass.  A block has vertices only if
-// lo <= iso < hi: otherwise all its cells have case 0 or 255.
-void launch_forward_ranges(bcmc_ctx *ctx, const float2 *full_ranges, const uint32_t nb[3], float2 *out);
+// Per block its tile range: [min, max] of every decoded voxel its cells can
+// read (its own 64, padding included, and the ghost faces, edges and corner of
+// its in-grid +x/+y/+z neighbours), from the range pass's 8 part ranges per
+// block.  A block has vertices only if lo <= iso < hi: otherwise all its cells
+// have case 0 or 255.
+void launch_tile_ranges(bcmc_ctx *ctx, const float2 *part_ranges, const uint32_t nb[3], float2 *out);
 void launch_dilate_ranges(bcmc_ctx *ctx, const float2 *ranges, const uint32_t nb[3], float2 *out);
 void launch_decode_dense(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords, uint32_t maxbits,
                          const uint32_t *ids, uint32_t n, float *out);

@@ -75,7 +75,7 @@ struct ActiveOp {
 // not written; the host grows the buffer and runs the emit launch again when
 // the total exceeds the capacity.  No chunk waits for another.
 struct McArgs {
-    const float2 *fwd;  // per block: forward range (launch_forward_ranges)
+    const float2 *tile;  // per block: tile range (launch_tile_ranges)
     const float *cache;
     const int32_t *item_slot;
     const uint32_t *flags;
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(kMcThreads) void mc_count_kernel(McArgs a) {
         const uint32_t k0 = chunk * kChunk;
         const bool in = k0 + threadIdx.x < n;
         const uint32_t bid = in ? a.active_ids[k0 + threadIdx.x] : 0u;
-        const uint32_t ncand = chunk_prologue(cs, in, bid, 0u, in ? a.fwd[bid] : make_float2(1.0f, 0.0f),
+        const uint32_t ncand = chunk_prologue(cs, in, bid, 0u, in ? a.tile[bid] : make_float2(1.0f, 0.0f),
                                               base.iso, lds_w);
         count_sweep(cs, ncand, wave, tab, [&](uint32_t e, RowBlock &b) {
             b = base;
@@ -230,7 +230,7 @@ struct bcmc_mc {
     float2 *ranges = nullptr;
     uint32_t *flags = nullptr, *active_ids = nullptr, *with_verts = nullptr;
     float2 *dilated = nullptr;  // per block: range over its 3x3x3 neighbourhood
-    float2 *fwd = nullptr;      // per block: range of every voxel its cells can read
+    float2 *tile = nullptr;     // per block: range of every voxel its cells can read
     unsigned long long *vert_offsets = nullptr;
     // marching-cubes pass: per active block vertex counts, per chunk totals
     // and their exclusive prefixes
@@ -275,7 +275,7 @@ static void mc_free_volume(bcmc_mc *m) {
     dev_free(m->flags);
     dev_free(m->active_ids);
     dev_free(m->dilated);
-    dev_free(m->fwd);
+    dev_free(m->tile);
     dev_free(m->with_verts);
     dev_free(m->vert_offsets);
     dev_free(m->counts);
@@ -292,7 +292,7 @@ static void mc_free_volume(bcmc_mc *m) {
     m->ranges = nullptr;
     m->flags = m->active_ids = m->with_verts = nullptr;
     m->dilated = nullptr;
-    m->fwd = nullptr;
+    m->tile = nullptr;
     m->vert_offsets = nullptr;
     m->counts = m->chunk_with = m->chunk_verts = m->chunk_with_pre = nullptr;
     m->chunk_verts_pre = nullptr;
@@ -323,7 +323,7 @@ static void mc_refresh_views(bcmc_mc *m) {
 
 static McArgs mc_args(bcmc_mc *m) {
     McArgs a;
-    a.fwd = m->fwd;
+    a.tile = m->tile;
     a.cache = (const float *)m->lru.st.data;
     a.item_slot = m->lru.st.v.item_slot;
     a.flags = m->flags;
@@ -488,7 +488,7 @@ BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *m, const void *stream, uint6
     BCMC_TRY(dev_alloc((void **)&m->flags, n * 4));
     BCMC_TRY(dev_alloc((void **)&m->active_ids, n * 4));
     BCMC_TRY(dev_alloc((void **)&m->dilated, n * 8));
-    BCMC_TRY(dev_alloc((void **)&m->fwd, n * 8));
+    BCMC_TRY(dev_alloc((void **)&m->tile, n * 8));
     BCMC_TRY(dev_alloc((void **)&m->with_verts, n * 4));
     BCMC_TRY(dev_alloc((void **)&m->vert_offsets, n * 8));
     BCMC_TRY(scan_site_alloc(&m->site_active, n, kScanTile));
@@ -509,14 +509,14 @@ BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *m, const void *stream, uint6
     *m->qh = qi;
     BCMC_HIP_CHECK(hipMemcpyAsync(m->q, m->qh, sizeof(QState), hipMemcpyHostToDevice, ctx->stream));
     // computeBlockRanges (:522-578): full-volume decode, min/max per block
-    float2 *full = nullptr;
-    BCMC_TRY(dev_alloc((void **)&full, n * 8));
-    launch_block_ranges(ctx, m->stream, m->stream_words, maxbits, m->dims, m->ranges, full);
+    float2 *parts = nullptr;  // 8 part ranges per block, transient
+    BCMC_TRY(dev_alloc((void **)&parts, n * 64));
+    launch_block_ranges(ctx, m->stream, m->stream_words, maxbits, m->dims, m->ranges, parts);
     launch_dilate_ranges(ctx, m->ranges, m->nb, m->dilated);
-    launch_forward_ranges(ctx, full, m->nb, m->fwd);
+    launch_tile_ranges(ctx, parts, m->nb, m->tile);
     BCMC_HIP_CHECK(hipGetLastError());
     BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    dev_free(full);
+    dev_free(parts);
     m->has_volume = true;
     m->last_active = m->last_with = m->last_new = 0;
     m->last_verts = 0;

@@ -520,11 +520,11 @@ struct EmitShared {
 
 // Chunk prologue of the count pass (all threads): entry t of the chunk gets
 // its block id and caller word; it is a count candidate when the isovalue
-// lies in [lo, hi) of its forward range (launch_forward_ranges): with iso < lo
+// lies in [lo, hi) of its tile range (launch_tile_ranges): with iso < lo
 // every corner test fails and with iso >= hi every one passes, so the block's
 // cells have case 0 or 255 and no vertices.  Returns the number of candidates
 // (listed in cs.list, ascending).
-BCMC_DEV uint32_t chunk_prologue(CountShared &cs, bool in, uint32_t bid, uint32_t aux, float2 fwd, float iso,
+BCMC_DEV uint32_t chunk_prologue(CountShared &cs, bool in, uint32_t bid, uint32_t aux, float2 tile, float iso,
                                  uint32_t *lds_waves) {
     const uint32_t t = threadIdx.x;
     cs.bid[t] = bid;
@@ -534,7 +534,7 @@ BCMC_DEV uint32_t chunk_prologue(CountShared &cs, bool in, uint32_t bid, uint32_
         cs.bid[kChunk] = 0;
         cs.aux[kChunk] = 0;
     }
-    const bool cand = in && iso >= fwd.x && iso < fwd.y;
+    const bool cand = in && iso >= tile.x && iso < tile.y;
     uint32_t ncand;
     const uint32_t pos = block_excl_scan<kMcThreads>(cand ? 1u : 0u, lds_waves, &ncand);
     if (cand)

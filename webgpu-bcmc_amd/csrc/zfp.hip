@@ -191,7 +191,7 @@ struct ZfpArgs {
     uint32_t capacity;         // cache mode: slots in `out` (others are skipped)
     float *out;                // cache / dense output
     float2 *ranges;            // ranges mode: range over the in-volume voxels
-    float2 *full_ranges;       // ranges mode (nullable): range over all 64 decoded voxels
+    float2 *part_ranges;       // ranges mode (nullable): 8 part ranges per block (all 64 voxels, faces, edges, voxel 0)
     uint32_t dims[3], nb[3];
     // LRU-assign mode: new item w takes slot order[w] (lru_cache_update.comp)
     const uint32_t *order;
@@ -259,7 +259,17 @@ __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
             const uint32_t sx = bx + 4 > a.dims[0] ? a.dims[0] - bx : 4;
             const uint32_t sy = by + 4 > a.dims[1] ? a.dims[1] - by : 4;
             const uint32_t sz = bz + 4 > a.dims[2] ? a.dims[2] - bz : 4;
-            float lo = 1e20f, hi = -1e20f, alo = v[0], ahi = v[0];
+            float lo = 1e20f, hi = -1e20f;
+            // part ranges over all 64 decoded voxels (padding included): the
+            // block, its x = 0 / y = 0 / z = 0 faces, the x = y = 0, x = z = 0
+            // and y = z = 0 edges and voxel 0 -- the parts a block's -x / -y /
+            // -z / ... neighbours read as ghost voxels (launch_tile_ranges)
+            float plo[8], phi[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                plo[k] = v[0];
+                phi[k] = v[0];
+            }
 #pragma unroll
             for (int j = 0; j < 64; ++j) {
                 const bool in = (uint32_t)(j & 3) < sx && (uint32_t)((j >> 2) & 3) < sy &&
@@ -267,12 +277,22 @@ __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
                 const float x = v[j];
                 lo = (in && x < lo) ? x : lo;
                 hi = (in && x > hi) ? x : hi;
-                alo = fminf(alo, x);
-                ahi = fmaxf(ahi, x);
+                const bool x0 = (j & 3) == 0, y0 = ((j >> 2) & 3) == 0, z0 = (j >> 4) == 0;
+                const bool part[8] = {true, x0, y0, z0, x0 && y0, x0 && z0, y0 && z0, j == 0};
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (part[k]) {
+                        plo[k] = fminf(plo[k], x);
+                        phi[k] = fmaxf(phi[k], x);
+                    }
+                }
             }
             a.ranges[blk] = make_float2(lo, hi);
-            if (a.full_ranges)
-                a.full_ranges[blk] = make_float2(alo, ahi);
+            if (a.part_ranges) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    a.part_ranges[(uint64_t)blk * 8 + k] = make_float2(plo[k], phi[k]);
+            }
         } else {
             uint64_t dst = (uint64_t)i * 64;
             if constexpr (MODE == kModeCache) {
@@ -327,7 +347,7 @@ static ZfpArgs base_args(const uint64_t *stream, uint64_t nwords, uint32_t maxbi
 }
 
 void launch_block_ranges(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords, uint32_t maxbits,
-                         const uint32_t dims[3], float2 *ranges, float2 *full_ranges) {
+                         const uint32_t dims[3], float2 *ranges, float2 *part_ranges) {
     ZfpArgs a = base_args(stream, nwords, maxbits);
     for (int i = 0; i < 3; ++i) {
         a.dims[i] = dims[i];
@@ -335,7 +355,7 @@ void launch_block_ranges(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords,
     }
     a.n_host = a.nb[0] * a.nb[1] * a.nb[2];
     a.ranges = ranges;
-    a.full_ranges = full_ranges;
+    a.part_ranges = part_ranges;
     const uint32_t grid = div_up(a.n_host, kZfpThreads);
     if (grid)
         launch_mode<kModeRanges>(a, grid, ctx->stream);
@@ -371,19 +391,26 @@ __global__ void dilate_ranges_kernel(const float2 *ranges, uint32_t nbx, uint32_
 // Per block: range over all decoded voxels (padding included) of the block and
 // of its in-grid +x/+y/+z, edge and corner neighbours: every voxel a cell of
 // the block can read (compute_vertex_values.comp:96-252).
-__global__ void forward_ranges_kernel(const float2 *full, uint32_t nbx, uint32_t nby, uint32_t nbz,
-                                      float2 *out) {
+// Per block: [min, max] over its whole 5^3 tile, i.e. every decoded voxel its
+// cells can read: its own 64 (padding included) and the ghost voxels of its
+// in-grid +x / +y / +z / +xy / +xz / +yz / +xyz neighbours, which are their
+// x = 0 face, y = 0 face, z = 0 face, x = y = 0 edge, x = z = 0 edge,
+// y = z = 0 edge and voxel 0 (part ranges 1..7 of the range pass).
+__global__ void tile_ranges_kernel(const float2 *parts, uint32_t nbx, uint32_t nby, uint32_t nbz,
+                                   float2 *out) {
     const uint64_t n = (uint64_t)nbx * nby * nbz;
     for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n;
          b += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t x = (uint32_t)(b % nbx), y = (uint32_t)((b / nbx) % nby),
                        z = (uint32_t)(b / ((uint64_t)nbx * nby));
-        float lo = full[b].x, hi = full[b].y;
+        float lo = parts[b * 8].x, hi = parts[b * 8].y;
         for (uint32_t s = 1; s < 8; ++s) {
             const uint32_t xx = x + (s & 1u), yy = y + ((s >> 1) & 1u), zz = z + (s >> 2);
             if (xx >= nbx || yy >= nby || zz >= nbz)
                 continue;
-            const float2 v = full[xx + (uint64_t)nbx * (yy + (uint64_t)nby * zz)];
+            // neighbour s: its part k reads with x0 for +x, y0 for +y, z0 for +z
+            const uint32_t k = s == 1 ? 1u : s == 2 ? 2u : s == 4 ? 3u : s == 3 ? 4u : s == 5 ? 5u : s == 6 ? 6u : 7u;
+            const float2 v = parts[(xx + (uint64_t)nbx * (yy + (uint64_t)nby * zz)) * 8 + k];
             lo = fminf(lo, v.x);
             hi = fmaxf(hi, v.y);
         }
@@ -391,10 +418,10 @@ __global__ void forward_ranges_kernel(const float2 *full, uint32_t nbx, uint32_t
     }
 }
 
-void launch_forward_ranges(bcmc_ctx *ctx, const float2 *full_ranges, const uint32_t nb[3], float2 *out) {
+void launch_tile_ranges(bcmc_ctx *ctx, const float2 *part_ranges, const uint32_t nb[3], float2 *out) {
     const uint64_t n = (uint64_t)nb[0] * nb[1] * nb[2];
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(n, 256), persistent_grid(ctx, 8)));
-    forward_ranges_kernel<<<grid, 256, 0, ctx->stream>>>(full_ranges, nb[0], nb[1], nb[2], out);
+    tile_ranges_kernel<<<grid, 256, 0, ctx->stream>>>(part_ranges, nb[0], nb[1], nb[2], out);
 }
 
 void launch_dilate_ranges(bcmc_ctx *ctx, const float2 *ranges, const uint32_t nb[3], float2 *out) {
