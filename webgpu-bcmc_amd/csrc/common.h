@@ -324,7 +324,7 @@ BCMC_DEV uint32_t next_tile(const ScanSite &site, uint32_t *lds_tile) {
     if (threadIdx.x == 0)
         *lds_tile = atomic_add_u32(&site.ctrl[0], 1u);
     __syncthreads();
-    const uint32_t t = *lds_tile;
+    const uint32_t t = __builtin_amdgcn_readfirstlane(*lds_tile);
     __syncthreads();
     return t;
 }

@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <initializer_list>
 #include <thread>
@@ -57,8 +58,9 @@ struct EngQuery {
     uint32_t flag_base;         // first index of this query in the concatenated block arrays
     uint32_t n;
     uint32_t row_base;          // first index of this query in the concatenated row segments
+                                // (a multiple of the scan tile: every tile has one query)
     uint32_t nb0, nsegx;        // the volume's blocks in x and row segments in x
-    uint32_t pad;
+    uint32_t nrows;             // the volume's row segments
     // the volume's arrays, copied here so that per-element work reads one descriptor
     const float2 *dilated;
     const float2 *seg_range;
@@ -85,20 +87,10 @@ struct EngCounters {
 
 constexpr uint32_t kMaxVols = 64;
 
-// query of a concatenated row segment (row_base ascending): a binary search
-// between the queries of the scan tile's first and last rows (tile_q[t] is the
-// query of row t * kScanTile)
-BCMC_DEV uint32_t query_of_row(const EngQuery *qs, const uint32_t *tile_q, uint32_t r) {
-    const uint32_t t = r / kScanTile;
-    uint32_t lo = tile_q[t], hi = tile_q[t + 1] + 1;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (qs[mid].row_base <= r)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
+// Word N of the lane's 16-lane row (DPP row_newbcast:N).
+template <int N>
+BCMC_DEV uint32_t row_share(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x150 + N, 0xF, 0xF, false);
 }
 
 constexpr uint32_t kNoPairs = 0xFFFFFFFFu;
@@ -145,6 +137,7 @@ __global__ void eng_seg_range_kernel(const float2 *dilated, uint32_t nbx, uint32
 
 // A candidate row segment with everything its active test and output need,
 // so that the per-block work reads one 32-byte descriptor and one range.
+// SegActiveOp reads it as 8 words: dilated (0, 1), b0, qi, iso, nx, flag_base.
 struct CandRow {
     const float2 *dilated;  // range of the segment's first block
     uint32_t b0;            // the segment's first block
@@ -154,8 +147,13 @@ struct CandRow {
     uint32_t flag_base;     // the query's first index in the flag words
     uint32_t pad;
 };
+static_assert(sizeof(CandRow) == 32 && offsetof(CandRow, b0) == 8 && offsetof(CandRow, qi) == 12 &&
+                  offsetof(CandRow, iso) == 16 && offsetof(CandRow, nx) == 20 && offsetof(CandRow, flag_base) == 24,
+              "SegActiveOp reads CandRow by word");
 
 // Candidate segments of every query: iso within the segment's dilated range.
+// Each query's segments start on a scan tile (the rows past its nrows are
+// padding), so a tile's query is tile_q[tile].
 struct CandSegOp {
     const EngQuery *qs;
     const uint32_t *tile_q;
@@ -164,25 +162,35 @@ struct CandSegOp {
     uint32_t nsegs;
     BCMC_DEV uint32_t size() const { return nsegs; }
     BCMC_DEV void begin() const {}
-    BCMC_DEV void value(uint32_t r, uint32_t (&v)[1]) const {
-        const uint32_t qi = query_of_row(qs, tile_q, r);
+    struct Tile {
+        uint32_t qi, row_base, nrows, nb0, nsegx, flag_base;
+        float iso;
+        const float2 *dilated, *seg_range;
+    };
+    using Item = NoItem;
+    BCMC_DEV Tile tile(uint32_t t) const {
+        const uint32_t qi = tile_q[t];
         const EngQuery &q = qs[qi];
-        const float2 rr = q.seg_range[r - q.row_base];
-        v[0] = (q.iso >= rr.x && q.iso <= rr.y) ? 1u : 0u;
+        return {qi, q.row_base, q.nrows, q.nb0, q.nsegx, q.flag_base, q.iso, q.dilated, q.seg_range};
     }
-    BCMC_DEV void emit(uint32_t r, const uint32_t (&v)[1], const unsigned long long (&e)[1]) const {
+    BCMC_DEV void value(const Tile &tc, uint32_t r, uint32_t (&v)[1], Item &) const {
+        const uint32_t seg = r - tc.row_base;
+        const bool in = seg < tc.nrows;
+        const float2 rr = tc.seg_range[in ? seg : 0u];
+        v[0] = (uint32_t)in & (uint32_t)(tc.iso >= rr.x) & (uint32_t)(tc.iso <= rr.y);
+    }
+    BCMC_DEV void emit(const Tile &tc, uint32_t r, const uint32_t (&v)[1], const Item &,
+                       const unsigned long long (&e)[1]) const {
         if (v[0]) {
-            const uint32_t qi = query_of_row(qs, tile_q, r);
-            const EngQuery &q = qs[qi];
-            const uint32_t seg = r - q.row_base;
-            const uint32_t x0 = (seg % q.nsegx) * kSegBlocks, row = seg / q.nsegx;
+            const uint32_t seg = r - tc.row_base;
+            const uint32_t x0 = (seg % tc.nsegx) * kSegBlocks, row = seg / tc.nsegx;
             CandRow cr;
-            cr.b0 = x0 + q.nb0 * row;
-            cr.dilated = q.dilated + cr.b0;
-            cr.qi = qi;
-            cr.iso = q.iso;
-            cr.nx = min(kSegBlocks, q.nb0 - x0);
-            cr.flag_base = q.flag_base;
+            cr.b0 = x0 + tc.nb0 * row;
+            cr.dilated = tc.dilated + cr.b0;
+            cr.qi = tc.qi;
+            cr.iso = tc.iso;
+            cr.nx = min(kSegBlocks, tc.nb0 - x0);
+            cr.flag_base = tc.flag_base;
             cr.pad = 0;
             cand[e[0]] = cr;
         }
@@ -200,21 +208,47 @@ struct SegActiveOp {
     EngCounters *c;
     BCMC_DEV uint32_t size() const { return c->num_rows * kSegBlocks; }
     BCMC_DEV void begin() const {}
-    BCMC_DEV void value(uint32_t i, uint32_t (&v)[1]) const {
-        const CandRow &cr = cand[i / kSegBlocks];
-        const uint32_t j = i % kSegBlocks;
-        v[0] = (j < cr.nx && block_active(cr.dilated[j], cr.iso)) ? 1u : 0u;
+    struct Tile {
+        uint32_t epoch;
+    };
+    struct Item {
+        uint32_t qf;  // query | 1 << 31 at the first block of the query's first segment
+        uint32_t b;   // block
+        uint32_t f;   // its flag word
+    };
+    BCMC_DEV Tile tile(uint32_t) const { return {c->epoch}; }
+    // The 16 lanes of a row hold the 16 blocks of one candidate row k (n and
+    // the scan's 64-element rounds are multiples of 16): lanes 0..7 of the row
+    // load the 8 words of cand[k], lanes 8..15 those of cand[k - 1], and the
+    // fields reach every lane of the row by DPP.  Positions past the end all
+    // read the last row (words chosen by lane, so the addresses stay inside).
+    BCMC_DEV void value(const Tile &, uint32_t i, uint32_t (&v)[1], Item &it) const {
+        const uint32_t k = i / kSegBlocks, j = i % kSegBlocks, l = lane_id() & 15u;
+        const uint32_t kk = l < 8 ? k : (k ? k - 1 : 0u);
+        const uint32_t w = reinterpret_cast<const uint32_t *>(cand + kk)[l & 7u];
+        const uint32_t d_lo = row_share<0>(w), d_hi = row_share<1>(w), b0 = row_share<2>(w),
+                       qi = row_share<3>(w), nx = row_share<5>(w), fb = row_share<6>(w),
+                       prev_q = row_share<11>(w);
+        const float iso = __uint_as_float(row_share<4>(w));
+        const float2 *dil = reinterpret_cast<const float2 *>(((uint64_t)d_hi << 32) | d_lo);
+        const bool in = j < nx;
+        const float2 d = dil[in ? j : 0u];
+        // bitwise: a short-circuit would let the load sink under a branch
+        v[0] = (uint32_t)in & (uint32_t)block_active(d, iso);
+        const bool first = (j == 0) & ((k == 0) | (prev_q != qi));
+        it.qf = qi | (first ? 0x80000000u : 0u);
+        it.b = b0 + j;
+        it.f = fb + b0 + j;
     }
-    BCMC_DEV void emit(uint32_t i, const uint32_t (&v)[1], const unsigned long long (&e)[1]) const {
-        const uint32_t k = i / kSegBlocks, j = i % kSegBlocks;
-        const CandRow &cr = cand[k];
-        if (j == 0 && (k == 0 || cand[k - 1].qi != cr.qi))
-            q_act_start[cr.qi] = (uint32_t)e[0];
+    BCMC_DEV void emit(const Tile &tc, uint32_t, const uint32_t (&v)[1], const Item &it,
+                       const unsigned long long (&e)[1]) const {
+        const uint32_t qi = it.qf & 0x7FFFFFFFu;
+        if (it.qf >> 31)
+            q_act_start[qi] = (uint32_t)e[0];
         if (v[0]) {
-            const uint32_t b = cr.b0 + j;
-            act_q[e[0]] = cr.qi;
-            act_b[e[0]] = b;
-            flags[cr.flag_base + b] = c->epoch;
+            act_q[e[0]] = qi;
+            act_b[e[0]] = it.b;
+            flags[it.f] = tc.epoch;
         }
     }
     BCMC_DEV void total(const unsigned long long (&t)[1]) const { c->num_active = (uint32_t)t[0]; }
@@ -229,10 +263,16 @@ struct CandOp {
     // only when the misses exceed the free room; only handed-out slots
     BCMC_DEV uint32_t size() const { return c->num_miss > capacity - c->used ? c->used : 0u; }
     BCMC_DEV void begin() const {}
-    BCMC_DEV void value(uint32_t s, uint32_t (&v)[1]) const {
-        v[0] = stamp[s] + 1 < c->epoch ? 1u : 0u;
+    struct Tile {
+        uint32_t epoch;
+    };
+    using Item = NoItem;
+    BCMC_DEV Tile tile(uint32_t) const { return {c->epoch}; }
+    BCMC_DEV void value(const Tile &tc, uint32_t s, uint32_t (&v)[1], Item &) const {
+        v[0] = stamp[s] + 1 < tc.epoch ? 1u : 0u;
     }
-    BCMC_DEV void emit(uint32_t s, const uint32_t (&v)[1], const unsigned long long (&e)[1]) const {
+    BCMC_DEV void emit(const Tile &, uint32_t s, const uint32_t (&v)[1], const Item &,
+                       const unsigned long long (&e)[1]) const {
         if (v[0])
             cand[e[0]] = s;
     }
@@ -543,8 +583,12 @@ struct EngChunkPrefixOp {
     EngCounters *c;
     BCMC_DEV uint32_t size() const { return c->num_chunks; }
     BCMC_DEV void begin() const {}
-    BCMC_DEV void value(uint32_t i, uint32_t (&v)[1]) const { v[0] = verts[i]; }
-    BCMC_DEV void emit(uint32_t i, const uint32_t (&)[1], const unsigned long long (&e)[1]) const {
+    using Tile = NoTile;
+    using Item = NoItem;
+    BCMC_DEV Tile tile(uint32_t) const { return {}; }
+    BCMC_DEV void value(const Tile &, uint32_t i, uint32_t (&v)[1], Item &) const { v[0] = verts[i]; }
+    BCMC_DEV void emit(const Tile &, uint32_t i, const uint32_t (&)[1], const Item &,
+                       const unsigned long long (&e)[1]) const {
         verts_pre[i] = e[0];
     }
     BCMC_DEV void total(const unsigned long long (&t)[1]) const { c->num_verts = t[0]; }
@@ -1014,6 +1058,7 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
         q.row_base = (uint32_t)rows;
         q.nb0 = v.nb[0];
         q.nsegx = div_up(v.nb[0], kSegBlocks);
+        q.nrows = q.nsegx * v.nb[1] * v.nb[2];
         q.dilated = v.dilated;
         q.seg_range = v.seg_range;
         q.slot_map = v.slot_map;
@@ -1023,7 +1068,7 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
         earlier[q.vol].push_back(q.flag_base);
         used[volume_ids[i]] = true;
         total += v.n;
-        rows += (uint64_t)div_up(v.nb[0], kSegBlocks) * v.nb[1] * v.nb[2];
+        rows += (uint64_t)div_up(q.nrows, kScanTile) * kScanTile;  // tile-aligned
     }
     BCMC_CHECK_ARG(total <= kMaxStepBlocks, "too many blocks in one step");
     const uint64_t row_elems = rows * kSegBlocks;
@@ -1043,7 +1088,7 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
         BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_same, cap * 4, hipHostMallocDefault));
         e->same_cap = cap;
     }
-    // query of the first row segment of every scan tile (+ a last entry)
+    // query of every scan tile of the row segments (+ a last entry)
     const uint64_t ntiles = div_up(rows, kScanTile) + 1;
     if (ntiles > e->tile_q_cap) {
         BCMC_HIP_CHECK(hipStreamSynchronize(st));
@@ -1215,7 +1260,8 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
         uint64_t blocks = 0, rows = 0, best_b = 0, best_r = 0;
         auto vol_rows = [&](uint32_t r) -> uint64_t {
             const EngVolume &v = e->vols[volume_ids[r]];
-            return (uint64_t)div_up(v.nb[0], kSegBlocks) * v.nb[1] * v.nb[2];
+            const uint64_t nrows = (uint64_t)div_up(v.nb[0], kSegBlocks) * v.nb[1] * v.nb[2];
+            return (uint64_t)div_up(nrows, kScanTile) * kScanTile;  // tile-aligned, as in a step
         };
         for (uint32_t i = 0; i < n; ++i) {
             BCMC_CHECK_ARG(volume_ids[i] < e->vols.size(), "unknown volume id");

@@ -32,15 +32,25 @@ struct QState {
 // ---------------------------------------------------------------------------
 // Single-pass (decoupled look-back) scan over a device-sized range with one or
 // two u32 channels.  Op provides:
+//   struct Tile, struct Item              per-tile context, per-element payload
 //   uint32_t size() const                 number of elements (may read memory)
 //   void begin() const                    once, by thread 0 of workgroup 0
-//   void value(uint32_t i, uint32_t v[NCH]) const
-//   void emit(uint32_t i, const uint32_t v[NCH], const unsigned long long excl[NCH]) const
+//   Tile tile(uint32_t t) const           context of tile t (every thread)
+//   void value(const Tile &, uint32_t i, uint32_t v[NCH], Item &) const
+//   void emit(const Tile &, uint32_t i, const uint32_t v[NCH], const Item &,
+//             const unsigned long long excl[NCH]) const
 //   void total(const unsigned long long t[NCH]) const  (one thread, once)
+// value() only reads memory (no stores) and takes no branch on what it reads,
+// so the loads of all 16 elements of a thread are in flight together; it is
+// also evaluated for the positions past the end (at element n - 1, with the
+// values then zeroed).  Stores belong in emit(), which the payload feeds.
 // Each wave scans a contiguous 1024-element run of its tile in 16 coalesced
 // rounds with DPP; wave totals combine through LDS; tiles chain through the
 // look-back status words of site[c].
 // ---------------------------------------------------------------------------
+struct NoTile {};
+struct NoItem {};
+
 template <int NCH, class Op>
 __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanSite s0, ScanSite s1, Op op) {
     constexpr int NW = kScanThreads / 64;
@@ -68,17 +78,22 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanSite s0, ScanSit
         const uint32_t tile = next_tile(s0, &lds_tile);
         if (tile >= ntiles)
             break;
+        const typename Op::Tile tc = op.tile(tile);
         const uint32_t wbase = tile * kScanTile + wave * (64 * kScanItems);
         uint32_t vals[kScanItems][NCH], excl[kScanItems][NCH], run[NCH];
+        typename Op::Item items[kScanItems];
         // all loads of the tile first (independent), then the scans
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const uint32_t i = wbase + j * 64 + lane;
+            op.value(tc, min(i, n - 1), vals[j], items[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            const bool in = wbase + j * 64 + lane < n;
 #pragma unroll
             for (int c = 0; c < NCH; ++c)
-                vals[j][c] = 0;
-            if (i < n)
-                op.value(i, vals[j]);
+                vals[j][c] = in ? vals[j][c] : 0u;
         }
 #pragma unroll
         for (int c = 0; c < NCH; ++c)
@@ -129,7 +144,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanSite s0, ScanSit
 #pragma unroll
                 for (int c = 0; c < NCH; ++c)
                     e[c] = prefix[c] + before[c] + excl[j][c];
-                op.emit(i, vals[j], e);
+                op.emit(tc, i, vals[j], items[j], e);
             }
         }
         if (tile == ntiles - 1 && threadIdx.x == 0) {
@@ -139,6 +154,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanSite s0, ScanSit
                 t[c] = prefix[c] + agg[c];
             op.total(t);
         }
+        // the grid's first claims are tiles 0 .. gridDim.x - 1: with no more
+        // tiles than that, no workgroup claims a second one
+        if (ntiles <= gridDim.x)
+            break;
     }
 #pragma unroll
     for (int c = 0; c < NCH; ++c)

@@ -52,10 +52,16 @@ struct MissAllOp {
     QState *q;
     BCMC_DEV uint32_t size() const { return v.total; }
     BCMC_DEV void begin() const {}
-    BCMC_DEV void value(uint32_t i, uint32_t (&x)[1]) const {
-        x[0] = (v.item_slot[i] < 0 && needed[i] != 0) ? 1u : 0u;
+    using Tile = NoTile;
+    using Item = NoItem;
+    BCMC_DEV Tile tile(uint32_t) const { return {}; }
+    BCMC_DEV void value(const Tile &, uint32_t i, uint32_t (&x)[1], Item &) const {
+        const int32_t s = v.item_slot[i];
+        const uint32_t nd = needed[i];
+        x[0] = (uint32_t)(s < 0) & (uint32_t)(nd != 0);
     }
-    BCMC_DEV void emit(uint32_t i, const uint32_t (&x)[1], const unsigned long long (&e)[1]) const {
+    BCMC_DEV void emit(const Tile &, uint32_t i, const uint32_t (&x)[1], const Item &,
+                       const unsigned long long (&e)[1]) const {
         if (x[0])
             new_items[e[0]] = i;
     }
@@ -78,19 +84,30 @@ struct AgeAvailOp {
     const uint32_t *gate;
     BCMC_DEV uint32_t size() const { return (gate && *gate == 0) ? 0u : q->cache_size; }
     BCMC_DEV void begin() const {}
-    BCMC_DEV void value(uint32_t s, uint32_t (&x)[1]) const {
-        uint32_t a = v.age[s] + 1, av = v.avail[s];
+    struct Tile {
+        bool inserting;  // q->num_new != 0
+    };
+    struct Item {
+        uint32_t age;
+        uint32_t avail;  // bit 0: available; bit 1: the slot holds an item (avail is rewritten)
+    };
+    BCMC_DEV Tile tile(uint32_t) const { return {q->num_new != 0}; }
+    BCMC_DEV void value(const Tile &tc, uint32_t s, uint32_t (&x)[1], Item &it) const {
+        const uint32_t age = v.age[s], avail = v.avail[s];
         const int32_t p = v.item[s];
-        if (p >= 0 && (uint32_t)p < v.total) {
-            av = needed[p] == 1 ? 0u : 1u;
-            if (!av)
-                a = 0;
-            v.avail[s] = av;
-        }
-        v.age[s] = a;
-        x[0] = (av != 0 && q->num_new != 0) ? 1u : 0u;
+        const bool cached = p >= 0 && (uint32_t)p < v.total;
+        const uint32_t nd = needed[cached ? (uint32_t)p : 0u];
+        const bool keep = cached & (nd == 1);  // needed: {age 0, unavailable}
+        const uint32_t av = cached ? (keep ? 0u : 1u) : avail;
+        it.avail = av | (cached ? 2u : 0u);
+        it.age = keep ? 0u : age + 1;
+        x[0] = (uint32_t)(av != 0) & (uint32_t)tc.inserting;
     }
-    BCMC_DEV void emit(uint32_t s, const uint32_t (&x)[1], const unsigned long long (&e)[1]) const {
+    BCMC_DEV void emit(const Tile &, uint32_t s, const uint32_t (&x)[1], const Item &it,
+                       const unsigned long long (&e)[1]) const {
+        v.age[s] = it.age;
+        if (it.avail & 2u)
+            v.avail[s] = it.avail & 1u;
         if (x[0])
             v.avail_ids[e[0]] = s;
     }

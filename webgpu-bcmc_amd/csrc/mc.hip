@@ -46,11 +46,20 @@ struct ActiveOp {
         q->num_verts = 0;
         q->overflow = 0;
     }
-    BCMC_DEV void value(uint32_t b, uint32_t (&v)[2]) const {
-        v[0] = block_active(dilated[b], q->iso) ? 1u : 0u;
-        v[1] = (v[0] && item_slot[b] < 0) ? 1u : 0u;
+    struct Tile {
+        float iso;
+    };
+    using Item = NoItem;
+    BCMC_DEV Tile tile(uint32_t) const { return {q->iso}; }
+    BCMC_DEV void value(const Tile &tc, uint32_t b, uint32_t (&v)[2], Item &) const {
+        const float2 d = dilated[b];
+        const int32_t s = item_slot[b];
+        const bool act = block_active(d, tc.iso);
+        v[0] = act ? 1u : 0u;
+        v[1] = (uint32_t)act & (uint32_t)(s < 0);
     }
-    BCMC_DEV void emit(uint32_t b, const uint32_t (&v)[2], const unsigned long long (&e)[2]) const {
+    BCMC_DEV void emit(const Tile &, uint32_t b, const uint32_t (&v)[2], const Item &,
+                       const unsigned long long (&e)[2]) const {
         flags[b] = v[0];
         if (v[0])
             active_ids[e[0]] = b;
@@ -158,11 +167,15 @@ struct ChunkPrefixOp {
     QState *q;
     BCMC_DEV uint32_t size() const { return div_up(q->num_active, kChunk); }
     BCMC_DEV void begin() const {}
-    BCMC_DEV void value(uint32_t i, uint32_t (&v)[2]) const {
+    using Tile = NoTile;
+    using Item = NoItem;
+    BCMC_DEV Tile tile(uint32_t) const { return {}; }
+    BCMC_DEV void value(const Tile &, uint32_t i, uint32_t (&v)[2], Item &) const {
         v[0] = with[i];
         v[1] = verts[i];
     }
-    BCMC_DEV void emit(uint32_t i, const uint32_t (&)[2], const unsigned long long (&e)[2]) const {
+    BCMC_DEV void emit(const Tile &, uint32_t i, const uint32_t (&)[2], const Item &,
+                       const unsigned long long (&e)[2]) const {
         with_pre[i] = (uint32_t)e[0];
         verts_pre[i] = e[1];
     }

@@ -33,8 +33,12 @@ struct TuneScanOp {
     unsigned long long *total_out;
     BCMC_DEV uint32_t size() const { return n; }
     BCMC_DEV void begin() const {}
-    BCMC_DEV void value(uint32_t i, uint32_t (&v)[1]) const { v[0] = data[i] & 0xFF; }
-    BCMC_DEV void emit(uint32_t i, const uint32_t (&)[1], const unsigned long long (&e)[1]) const {
+    using Tile = NoTile;
+    using Item = NoItem;
+    BCMC_DEV Tile tile(uint32_t) const { return {}; }
+    BCMC_DEV void value(const Tile &, uint32_t i, uint32_t (&v)[1], Item &) const { v[0] = data[i] & 0xFF; }
+    BCMC_DEV void emit(const Tile &, uint32_t i, const uint32_t (&)[1], const Item &,
+                       const unsigned long long (&e)[1]) const {
         data[i] = (uint32_t)e[0];
     }
     BCMC_DEV void total(const unsigned long long (&t)[1]) const { *total_out = t[0]; }
