@@ -23,8 +23,26 @@
 namespace bcmc {
 
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 4;
-constexpr uint32_t kSortTile = kSortThreads * kSortItems;  // 1024
+constexpr int kSortItems = 16;
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr uint32_t kSortTile = kSortThreads * kSortItems;  // 4096
+
+// Position of item j of a lane in tile t: each wave owns a contiguous run of
+// 64 * kSortItems positions, read in kSortItems coalesced rounds.
+BCMC_DEV uint32_t sort_pos(uint32_t tile, uint32_t wave, int j, uint32_t lane) {
+    return tile * kSortTile + wave * (64u * kSortItems) + (uint32_t)j * 64u + lane;
+}
+
+// Lanes of the wave holding the same digit as this lane (among `valid` ones).
+BCMC_DEV uint64_t digit_peers(uint32_t d, bool valid) {
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const uint64_t m = __ballot((d >> b) & 1u);
+        peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    return peers;
+}
 
 // ---- element sources -------------------------------------------------------
 struct BufSource {  // passes >= 1
@@ -51,7 +69,9 @@ struct LruSource {
         const uint32_t m = min(64u, n - c0);
         const uint32_t j = c0 + (m - 1 - (p - c0));
         v = avail_ids[j];
-        k = v < capacity ? ~age[v] : 0u;
+        const bool ok = v < capacity;
+        const uint32_t a = age[ok ? v : 0u];
+        k = ok ? ~a : 0u;
     }
 };
 
@@ -70,22 +90,32 @@ struct GenericSource {
 };
 
 // ---- kernels ---------------------------------------------------------------
+// Elements are read for every position of a tile (positions past n read
+// element n - 1 and are ignored), so all loads of a thread are in flight
+// together.
 template <class Src>
 __global__ __launch_bounds__(kSortThreads) void sort_hist_kernel(Src src, uint32_t shift,
                                                                  uint32_t *hist) {
     __shared__ uint32_t h[256];
     const uint32_t n = src.size(), ntiles = div_up(n, kSortTile);
+    const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
+    const uint64_t lt = (1ull << lane) - 1ull;
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint32_t key[kSortItems];
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j) {
+            uint32_t v;
+            src.get(min(sort_pos(tile, wave, j, lane), n - 1), key[j], v);
+        }
         h[threadIdx.x] = 0;
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kSortItems; ++j) {
-            const uint32_t p = tile * kSortTile + j * kSortThreads + threadIdx.x;
-            if (p < n) {
-                uint32_t k, v;
-                src.get(p, k, v);
-                atomicAdd(&h[(k >> shift) & 255u], 1u);
-            }
+            const bool valid = sort_pos(tile, wave, j, lane) < n;
+            const uint32_t d = (key[j] >> shift) & 255u;
+            const uint64_t peers = digit_peers(d, valid);
+            if (valid && (peers & lt) == 0)
+                atomicAdd(&h[d], (uint32_t)__popcll(peers));
         }
         __syncthreads();
         hist[threadIdx.x * ntiles + tile] = h[threadIdx.x];
@@ -113,51 +143,63 @@ struct HistScanOp {
     BCMC_DEV void total(const unsigned long long (&)[1]) const {}
 };
 
-// Stable scatter: rank = elements of the tile with the same digit and a
-// smaller position, found per round with 8 ballots and a per-digit LDS count
-// that the four waves of a round update in order.
+// Stable scatter.  Each wave ranks its run in order: an element's rank is the
+// count of its digit in the wave's earlier rounds (a per-wave LDS counter
+// that the digit's lowest lane advances) plus its lower lanes of the same
+// digit in this round; then every digit's start in the tile (from the scanned
+// histogram table) is split among the four waves by their counts.
 template <class Src>
 __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(Src src, uint32_t shift,
                                                                     const uint32_t *hist,
                                                                     uint32_t *okeys, uint32_t *ovals,
                                                                     uint32_t out_skip, uint32_t out_cap) {
-    __shared__ uint32_t cnt[256];
+    __shared__ uint32_t cnt[kSortWaves][256];
     const uint32_t n = src.size(), ntiles = div_up(n, kSortTile);
     const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
     const uint64_t lt = (1ull << lane) - 1ull;
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        cnt[threadIdx.x] = hist[threadIdx.x * ntiles + tile];
+        uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j)
+            src.get(min(sort_pos(tile, wave, j, lane), n - 1), key[j], val[j]);
+        const uint32_t start = hist[threadIdx.x * ntiles + tile];  // digit threadIdx.x
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w)
+            cnt[w][threadIdx.x] = 0;
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kSortItems; ++j) {
-            const uint32_t p = tile * kSortTile + j * kSortThreads + threadIdx.x;
-            const bool valid = p < n;
-            uint32_t k = 0, v = 0;
-            if (valid)
-                src.get(p, k, v);
-            const uint32_t d = (k >> shift) & 255u;
-            uint64_t peers = __ballot(valid);
+            const bool valid = sort_pos(tile, wave, j, lane) < n;
+            const uint32_t d = (key[j] >> shift) & 255u;
+            const uint64_t peers = digit_peers(d, valid);
+            const uint32_t before = cnt[wave][d];
+            rank[j] = before + (uint32_t)__popcll(peers & lt);
+            if (valid && (peers & lt) == 0)
+                cnt[wave][d] = before + (uint32_t)__popcll(peers);
+        }
+        __syncthreads();
+        {
+            uint32_t run = start;
 #pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                const uint64_t m = __ballot((d >> b) & 1u);
-                peers &= ((d >> b) & 1u) ? m : ~m;
-            }
-            const uint32_t rank = (uint32_t)__popcll(peers & lt);
-            const bool leader = valid && (peers & lt) == 0;
-            for (uint32_t w = 0; w < kSortThreads / 64; ++w) {
-                if (wave == w && valid) {
-                    const uint32_t dst = cnt[d] + rank;
-                    if (leader)
-                        cnt[d] += (uint32_t)__popcll(peers);
-                    if (dst >= out_skip && dst - out_skip < out_cap) {
-                        if (okeys)
-                            okeys[dst - out_skip] = k;
-                        ovals[dst - out_skip] = v;
-                    }
-                }
-                __syncthreads();
+            for (int w = 0; w < kSortWaves; ++w) {
+                const uint32_t c = cnt[w][threadIdx.x];
+                cnt[w][threadIdx.x] = run;
+                run += c;
             }
         }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kSortItems; ++j) {
+            if (sort_pos(tile, wave, j, lane) >= n)
+                continue;
+            const uint32_t dst = cnt[wave][(key[j] >> shift) & 255u] + rank[j];
+            if (dst >= out_skip && dst - out_skip < out_cap) {
+                if (okeys)
+                    okeys[dst - out_skip] = key[j];
+                ovals[dst - out_skip] = val[j];
+            }
+        }
+        __syncthreads();  // the counters are cleared for the next tile
     }
 }
 
