@@ -61,7 +61,7 @@ struct bcmc_ctx {
     // launch shapes: persistent workgroups per CU of each kernel family
     // (bcmc_ctx_autotune measures and sets them; results never depend on them)
     struct {
-        uint32_t scan = 4, sort = 4, decode = 8, mc = 5, count = 8;
+        uint32_t scan = 2, sort = 8, decode = 8, mc = 5, count = 8;
     } tune;
     // bumped on every change of `tune`: captured graphs keyed on it re-capture
     uint64_t tune_gen = 0;
@@ -98,9 +98,9 @@ struct ScanSite {
 int scan_site_alloc(ScanSite *s, uint64_t max_elems, uint32_t tile);
 void scan_site_free(ScanSite *s);
 
-constexpr int kScanThreads = 256;
+constexpr int kScanThreads = 512;
 constexpr int kScanItems = 16;
-constexpr uint32_t kScanTile = kScanThreads * kScanItems;  // 4096 elements
+constexpr uint32_t kScanTile = kScanThreads * kScanItems;  // 8192 elements
 
 __host__ __device__ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 

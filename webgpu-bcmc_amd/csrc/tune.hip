@@ -175,7 +175,7 @@ BCMC_API int bcmc_ctx_autotune(bcmc_ctx *ctx, bcmc_tuning *out) {
     } sg{&site};
     fill(ctx, scan_data, kScanN, 1, 0xFF, 0);
     const TuneScanOp sop{scan_data, kScanN, total};
-    BCMC_TRY(pick(ctx, ev, ctx->tune.scan, {2, 4, 8, 16},
+    BCMC_TRY(pick(ctx, ev, ctx->tune.scan, {1, 2, 4, 8},
                   [&] { launch_scan<1>(ctx, site, site, sop, kScanN); }, &res.scan_ms));
 
     // radix sort: 2M pairs with 24-bit keys (3 digit passes, as an LRU order)
@@ -194,7 +194,7 @@ BCMC_API int bcmc_ctx_autotune(bcmc_ctx *ctx, bcmc_tuning *out) {
         ~SortGuard() { sort_scratch_free(s); }
     } sog{&ss};
     BCMC_TRY(sort_scratch_alloc(&ss, kSortN));
-    BCMC_TRY(pick(ctx, ev, ctx->tune.sort, {1, 2, 4, 8},
+    BCMC_TRY(pick(ctx, ev, ctx->tune.sort, {2, 4, 8, 16},
                   [&] { launch_sort_pairs(ctx, &ss, keys, vals, count, kSortN, sorted, 3); },
                   &res.sort_ms));
 
