@@ -594,6 +594,7 @@ struct EngChunkPrefixOp {
     BCMC_DEV void total(const unsigned long long (&t)[1]) const { c->num_verts = t[0]; }
 };
 
+template <int FMT>
 __global__ __launch_bounds__(kEngThreads) __attribute__((amdgpu_waves_per_eu(5))) void eng_emit_kernel(EngMcArgs a) {
     constexpr int NW = kEngThreads / 64;
     __shared__ EmitShared cs;
@@ -621,12 +622,12 @@ __global__ __launch_bounds__(kEngThreads) __attribute__((amdgpu_waves_per_eu(5))
             continue;
         RowBlock ub;
         eng_row_block(a, q, 0, true, epoch, ub);
-        emit_sweep(cs, wave, num_with, pv, tab, es[wave], a.vert_cap, a.verts, a.fmt,
-                   [&](uint32_t e, RowBlock &b) {
-                       b = ub;
-                       b.live = k0 + e < k1;
-                       b.bid = cs.bid[e];
-                   });
+        emit_sweep<FMT>(cs, wave, num_with, pv, tab, es[wave], a.vert_cap, a.verts,
+                        [&](uint32_t e, RowBlock &b) {
+                            b = ub;
+                            b.live = k0 + e < k1;
+                            b.bid = cs.bid[e];
+                        });
         __syncthreads();
     }
 }
@@ -801,7 +802,13 @@ static uint64_t eng_max_chunks(uint64_t total, uint32_t nq) { return div_up(tota
 static int eng_launch_emit(bcmc_engine *e, uint32_t nq, uint64_t total) {
     const uint32_t grid = (uint32_t)std::max<uint64_t>(
         1, std::min<uint64_t>(eng_max_chunks(total, nq), persistent_grid(e->ctx, e->ctx->tune.mc)));
-    eng_emit_kernel<<<grid, kEngThreads, 0, e->ctx->stream>>>(eng_mc_args(e, nq));
+    hipStream_t st = e->ctx->stream;
+    if (e->fmt == BCMC_VERTS_LOCAL_F32)
+        eng_emit_kernel<BCMC_VERTS_LOCAL_F32><<<grid, kEngThreads, 0, st>>>(eng_mc_args(e, nq));
+    else if (e->fmt == BCMC_VERTS_GLOBAL_F32)
+        eng_emit_kernel<BCMC_VERTS_GLOBAL_F32><<<grid, kEngThreads, 0, st>>>(eng_mc_args(e, nq));
+    else
+        eng_emit_kernel<BCMC_VERTS_PACKED_U32X2><<<grid, kEngThreads, 0, st>>>(eng_mc_args(e, nq));
     BCMC_HIP_CHECK(hipGetLastError());
     return BCMC_OK;
 }

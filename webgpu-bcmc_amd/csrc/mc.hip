@@ -185,6 +185,7 @@ struct ChunkPrefixOp {
     }
 };
 
+template <int FMT>
 __global__ __launch_bounds__(kMcThreads) __attribute__((amdgpu_waves_per_eu(5))) void mc_emit_kernel(McArgs a) {
     constexpr int NW = kMcThreads / 64;
     __shared__ EmitShared cs;
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(kMcThreads) __attribute__((amdgpu_waves_per_eu(5)))
             b.live = k0 + e < n;
             b.bid = cs.bid[e];
         };
-        emit_sweep(cs, wave, num_with, pv, tab, es[wave], a.capacity, a.verts, a.fmt, setup);
+        emit_sweep<FMT>(cs, wave, num_with, pv, tab, es[wave], a.capacity, a.verts, setup);
         __syncthreads();
     }
 }
@@ -364,7 +365,13 @@ static McArgs mc_args(bcmc_mc *m) {
 static int mc_launch_emit(bcmc_mc *m) {
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(m->total, kChunk),
                                                                    persistent_grid(m->ctx, m->ctx->tune.mc)));
-    mc_emit_kernel<<<grid, kMcThreads, 0, m->ctx->stream>>>(mc_args(m));
+    hipStream_t st = m->ctx->stream;
+    if (m->fmt == BCMC_VERTS_LOCAL_F32)
+        mc_emit_kernel<BCMC_VERTS_LOCAL_F32><<<grid, kMcThreads, 0, st>>>(mc_args(m));
+    else if (m->fmt == BCMC_VERTS_GLOBAL_F32)
+        mc_emit_kernel<BCMC_VERTS_GLOBAL_F32><<<grid, kMcThreads, 0, st>>>(mc_args(m));
+    else
+        mc_emit_kernel<BCMC_VERTS_PACKED_U32X2><<<grid, kMcThreads, 0, st>>>(mc_args(m));
     BCMC_HIP_CHECK(hipGetLastError());
     return BCMC_OK;
 }

@@ -55,7 +55,7 @@ __host__ __device__ constexpr uint32_t case_of_corner_byte(uint32_t byte) {
 
 // Interpolation data of edge e: corner a's position, b - a, and both corners'
 // offsets in the tile.
-struct EdgeInfo {
+struct alignas(16) EdgeInfo {
     float a[3];
     float off_a;
     float d[3];
@@ -376,10 +376,11 @@ BCMC_DEV void row_store(float *dst, const float (&r)[5]) {
 // triangle-table row.  Triangle-parallel: every cell with vertices marks its
 // first triangle in LDS with its index g = 64 row + cell (cells ascend with
 // their triangles), so the owner of triangle t is the running maximum of the
-// marks up to t.  Vertices at or past `cap` are not written.  Must be called
-// by all 64 lanes.
+// marks up to t.  Vertices at or past `cap` are not written.  FMT is the
+// bcmc_vertex_format.  Must be called by all 64 lanes.
+template <int FMT>
 BCMC_DEV void emit_row_blocks(const RowBlock &b, const RowTile &t, const McTables &tab, EmitScratch &es,
-                              unsigned long long out0, unsigned long long cap, void *verts, int fmt) {
+                              unsigned long long out0, unsigned long long cap, void *verts) {
     const uint32_t lane = lane_id(), row = lane >> 4, l = lane & 15u, y = l & 3u, z = l >> 2;
     const CornerWord cw = row_corner_word(t, b.iso);
     const uint32_t cb[4] = {cell_byte<0>(cw), cell_byte<1>(cw), cell_byte<2>(cw), cell_byte<3>(cw)};
@@ -448,32 +449,39 @@ BCMC_DEV void emit_row_blocks(const RowBlock &b, const RowTile &t, const McTable
             const uint32_t src = (lv * 0xAAABu) >> 17;  // lv / 3
             const uint32_t inf = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)info);
             const uint32_t v = 3 * t0 + lv;
-            if (v >= limit)
-                continue;
+            // lanes past the limit compute from valid LDS entries (info 0 is
+            // cell 0, edge 0 of row 0) and do not store
             const uint32_t gg = inf >> 12, j = gg >> 6, cell = gg & 63u;
             const uint32_t m = lv - 3 * src;
+            // the edge's two table rows, the block's isovalue and id in one
+            // batch of LDS reads, then the two corner values
             const EdgeInfo &ei = tab.edge[(inf >> (4 * m)) & 15u];
+            const float4 ea = *reinterpret_cast<const float4 *>(ei.a);  // a[0..2], off_a
+            const float4 ed = *reinterpret_cast<const float4 *>(ei.d);  // d[0..2], off_b
+            const float iso = es.iso[j];
+            const uint32_t bid = es.bid[j];
             const uint32_t cx = cell & 3u, cy = (cell >> 2) & 3u, cz = cell >> 4;
             const float *tc = es.tile[j] + cx + kTileY * cy + kTileZ * cz;
-            const float fa = tc[__float_as_uint(ei.off_a)];
-            const float fb = tc[__float_as_uint(ei.off_b)];
-            const float iso = es.iso[j];
-            // lerp_verts (compute_block_vertices.comp:62-74), then + voxel + 0.5
-            float tt = 0.0f;
-            if (!(fabsf(fa - fb) < 0.001f))
-                tt = (iso - fa) / (fb - fa);
-            const float px = (ei.a[0] + tt * ei.d[0] + (float)cx) + 0.5f;
-            const float py = (ei.a[1] + tt * ei.d[1] + (float)cy) + 0.5f;
-            const float pz = (ei.a[2] + tt * ei.d[2] + (float)cz) + 0.5f;
+            const float fa = tc[__float_as_uint(ea.w)];
+            const float fb = tc[__float_as_uint(ed.w)];
+            // lerp_verts (compute_block_vertices.comp:62-74), then + voxel +
+            // 0.5; the quotient is formed unconditionally (no branch) and
+            // replaced by 0 for a flat edge
+            const float q = (iso - fa) / (fb - fa);
+            const float tt = fabsf(fa - fb) < 0.001f ? 0.0f : q;
+            const float px = (ea.x + tt * ed.x + (float)cx) + 0.5f;
+            const float py = (ea.y + tt * ed.y + (float)cy) + 0.5f;
+            const float pz = (ea.z + tt * ed.z + (float)cz) + 0.5f;
             const unsigned long long o = out0 + v;
-            const uint32_t bid = es.bid[j];
+            if (v >= limit)
+                continue;
             // vertices are written once and not read back by the pass:
             // non-temporal stores keep them from displacing the block data
             // in the caches
-            if (fmt == BCMC_VERTS_LOCAL_F32) {
+            if constexpr (FMT == BCMC_VERTS_LOCAL_F32) {
                 const V4f vv = {px, py, pz, __uint_as_float(bid)};
                 __builtin_nontemporal_store(vv, reinterpret_cast<V4f *>(verts) + o);
-            } else if (fmt == BCMC_VERTS_GLOBAL_F32) {
+            } else if constexpr (FMT == BCMC_VERTS_GLOBAL_F32) {
                 float *gp = reinterpret_cast<float *>(verts) + 3 * o;
                 __builtin_nontemporal_store((float)(4 * es.bpos[j][0]) + px, gp);
                 __builtin_nontemporal_store((float)(4 * es.bpos[j][1]) + py, gp + 1);
@@ -587,10 +595,10 @@ BCMC_DEV void count_sweep(CountShared &cs, uint32_t ncand, uint32_t wave, const 
 // blocks with vertices (whose outputs are contiguous) and writes their
 // vertices from out0 + the group's offset in the chunk.  The next group's
 // slot/flag loads are issued before this group's vertices are written.
-template <class Setup>
+template <int FMT, class Setup>
 BCMC_DEV void emit_sweep(EmitShared &cs, uint32_t wave, uint32_t num_with, unsigned long long out0,
                          const McTables &tab, EmitScratch &es, unsigned long long cap, void *verts,
-                         int fmt, Setup &&setup) {
+                         Setup &&setup) {
     constexpr uint32_t kRound = (kMcThreads / 64) * kRowBlocks;
     const uint32_t row = lane_id() >> 4;
     uint32_t hh = wave * kRowBlocks;
@@ -615,7 +623,7 @@ BCMC_DEV void emit_sweep(EmitShared &cs, uint32_t wave, uint32_t num_with, unsig
             block_of(hh + kRound + row, b);
             m = row_meta_load(b);
         }
-        emit_row_blocks(bq, t, tab, es, out0 + cs.off[hh], cap, verts, fmt);
+        emit_row_blocks<FMT>(bq, t, tab, es, out0 + cs.off[hh], cap, verts);
     }
 }
 
