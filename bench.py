@@ -34,7 +34,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-STAGE_KERNEL = {"computeVertices": "mc_emit_kernel", "computeBlockHasVertices": "mc_count_kernel",
+STAGE_KERNEL = {"computeVertices": "mc_emit_kernel<0>", "computeBlockHasVertices": "mc_count_kernel",
                 "decompression": "zfp_kernel<4,3>", "computeActiveBlocks": "scan_kernel<2,ActiveOp>"}
 
 
