@@ -48,3 +48,37 @@ def test_lru_no_new_items_is_age_only(dev):
         assert gpu.update(dev.buffer_from(f))[0] == cpu.update(f)[0]
     for a, b in zip(gpu.state(), cpu.state()):
         assert np.array_equal(a, b)
+
+
+def test_lru_long_run_after_reset_matches_oracle(dev):
+    """150 updates after a reset, with growth and many equal ages: the
+    eviction order sorts compressed ages after a reset (one radix pass for the
+    first 127 updates, two from then on) and must give the oracle's slots at
+    every step, before the reset (constructor ages, raw keys) as well."""
+    total, initial = 3000, 96
+    rng = np.random.default_rng(7)
+    gpu = bcmc_amd.LRUCache(dev, None, None, initial, 256, total)
+    cpu = oracle.LRU(initial, 256, total)
+
+    def step(tag):
+        needed = _needed(total, rng, 0.02 + 0.2 * rng.uniform())
+        n_g, ids_g = gpu.update(dev.buffer_from(needed))
+        n_c, ids_c = cpu.update(needed)
+        assert n_g == n_c, tag
+        if n_c:
+            assert np.array_equal(ids_g.download(np.uint32, n_g), ids_c), tag
+
+    def same_state(tag):
+        for a, b, what in zip(gpu.state(), cpu.state(), ("age", "avail", "item", "item_slot")):
+            assert np.array_equal(a, b), (tag, what)
+
+    for k in range(4):
+        step(("before reset", k))
+    same_state("before reset")
+    gpu.reset()
+    cpu.reset()
+    for k in range(150):
+        step(k)
+        if k % 10 == 0 or 125 <= k <= 130:
+            same_state(k)
+    same_state("end")

@@ -212,13 +212,30 @@ struct SortScratch {
 int sort_scratch_alloc(SortScratch *s, uint32_t max_elems);
 void sort_scratch_free(SortScratch *s);
 
+// Update chains since an LRU reset below which the eviction order sorts
+// compressed ages.  After a reset every slot is fresh (age 100000); in the
+// u-th update chain since then (u counts the current one), a slot used since
+// the reset has an age below u and a fresh slot an age in [100000,
+// 100000 + u].  So age -> age (used) or u + age - 100000 (fresh) keeps the
+// order and every tie, and maps into [0, 2u]: one 8-bit radix pass for the
+// first 127 updates after a reset instead of three.
+constexpr uint32_t kCompressedUpdates = 0x8000;
+
+// Sort key of a slot's age: ~(compressed age) while u < kCompressedUpdates,
+// ~age otherwise (u = 0xFFFFFFFF before the first reset: constructor ages).
+BCMC_DEV inline uint32_t lru_order_key(uint32_t age, uint32_t u) {
+    const uint32_t c = age >= 100000u ? u + (age - 100000u) : age;
+    return ~(u < kCompressedUpdates ? c : age);
+}
+
 // Orders the `count` available cache slots for LRU assignment exactly as the
 // reference's RadixSorter.sort(ages, slot ids, S, reverse=true) does, writing
-// slot ids to out[0..count).  count is read from device memory.  npasses: the
-// 8-bit digits of ~age that vary (those above every age are all 0xFF).
+// slot ids to out[0..count).  count and the update count ctl[0] are read from
+// device memory.  npasses: the 8-bit digits of the keys that vary (those above
+// every key are all 0xFF).
 void launch_lru_order(bcmc_ctx *ctx, SortScratch *s, const uint32_t *avail_ids,
-                      const uint32_t *age, const uint32_t *count_dev, uint32_t count_max,
-                      uint32_t *out, uint32_t npasses);
+                      const uint32_t *age, const uint32_t *ctl, const uint32_t *count_dev,
+                      uint32_t count_max, uint32_t *out, uint32_t npasses);
 // Stable ascending sort of `count` (key, value) pairs (count read from device
 // memory) on the low 8 * npasses key bits; sorted values to out[0..count).
 void launch_sort_pairs(bcmc_ctx *ctx, SortScratch *s, const uint32_t *keys, const uint32_t *vals,

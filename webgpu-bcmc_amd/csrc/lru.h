@@ -24,6 +24,11 @@ struct LruState {
     uint32_t total = 0, aligned_total = 0, elem_bytes = 0, capacity = 0;
     uint32_t cache_size = 0;      // host mirror of the logical size
     uint64_t age_bound = 0;       // every slot age is below this (host-side bound)
+    // Update chains since the last reset (kUpdatesUnknown before the first
+    // one): device word, advanced by each chain's growth kernel, and its host
+    // mirror, advanced per enqueued update.  See lru_order_key.
+    uint32_t *ctl = nullptr;
+    uint32_t updates = 0xFFFFFFFFu;
     ScanSite site_miss, site_avail;
     SortScratch sort;
 };
@@ -36,12 +41,13 @@ void lru_state_free(LruState *L);
 // nothing when q->num_active == 0 (compressed_marching_cubes.js:630-641).  Ends
 // with the eviction order in v.order; the slot assignment is done by the
 // decoder (launch_decode_assign), which decodes each new item into its slot.
-// npasses = lru_sort_passes(L) at enqueue time; the caller adds 1 to
-// L->age_bound per update.
+// npasses = lru_begin_update(L), called once per update before it is enqueued.
 void lru_enqueue_update_mc(bcmc_ctx *ctx, LruState *L, const uint32_t *flags, QState *q,
                            uint32_t npasses);
 // Radix passes for the eviction order given the current age bound.
-uint32_t lru_sort_passes(const LruState *L);
+// Counts one more update (age bound, updates since the reset) and returns the
+// radix passes its eviction order needs.
+uint32_t lru_begin_update(LruState *L);
 // LRUCache.update for arbitrary 0/1 `needed` flags over all items.
 void lru_enqueue_update_all(bcmc_ctx *ctx, LruState *L, const uint32_t *needed, QState *q);
 void lru_enqueue_reset(bcmc_ctx *ctx, LruState *L);

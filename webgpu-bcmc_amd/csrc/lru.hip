@@ -29,8 +29,12 @@
 
 namespace bcmc {
 
+constexpr uint32_t kUpdatesUnknown = 0xFFFFFFFFu;
+
 __global__ void lru_init_kernel(LruView v, uint32_t first, uint32_t end, uint32_t age0,
-                                int init_items) {
+                                int init_items, uint32_t *ctl, uint32_t ctl_value) {
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        ctl[0] = ctl_value;
     for (uint32_t s = first + blockIdx.x * blockDim.x + threadIdx.x; s < end;
          s += gridDim.x * blockDim.x) {
         v.age[s] = age0 == 0 ? 2 + s : age0;  // constructor ages 2+i (lru_cache.js:55-59)
@@ -121,8 +125,12 @@ struct AgeAvailOp {
 // alignTo(N, 32) slots is reserved up front); slots [C, C') become
 // {100000, 1, -1} and are appended to the available list.  Every thread
 // evaluates the rule; thread 0 of workgroup 0 records C' and S'.
-__global__ void lru_grow_kernel(LruView v, QState *q, const uint32_t *gate) {
+__global__ void lru_grow_kernel(LruView v, QState *q, const uint32_t *gate, uint32_t *ctl) {
     const bool run = !(gate && *gate == 0) && q->num_new != 0;
+    // one more update chain since the reset (counted whether or not the
+    // update had work: the count only has to bound the updates from above)
+    if (blockIdx.x == 0 && threadIdx.x == 0 && ctl[0] != kUpdatesUnknown)
+        ctl[0] += 1;
     const uint32_t c = q->cache_size, avail = run ? q->num_avail : 0u;
     uint32_t next = c;
     if (run && avail < q->num_new) {
@@ -188,13 +196,16 @@ int lru_state_alloc(bcmc_ctx *ctx, LruState *L, uint32_t initial, uint32_t elem_
     BCMC_TRY(dev_alloc((void **)&L->data, cap * elem_bytes + 256));
     BCMC_HIP_CHECK(hipMemsetAsync((char *)L->data + cap * elem_bytes, 0, 256, ctx->stream));
     BCMC_TRY(dev_alloc((void **)&L->new_items, (size_t)L->aligned_total * 4 + 4));
+    BCMC_TRY(dev_alloc((void **)&L->ctl, 4));
     BCMC_TRY(scan_site_alloc(&L->site_miss, L->aligned_total, kScanTile));
     BCMC_TRY(scan_site_alloc(&L->site_avail, cap, kScanTile));
     BCMC_TRY(sort_scratch_alloc(&L->sort, L->capacity));
     L->v.total = total;
     L->v.capacity = L->capacity;
     L->age_bound = std::max<uint64_t>((uint64_t)L->cache_size + 1, 100000);
-    lru_init_kernel<<<grid_for(ctx, cap), 256, 0, ctx->stream>>>(L->v, 0, L->cache_size, 0, 1);
+    L->updates = kUpdatesUnknown;  // constructor ages 2+i: no compressed order keys
+    lru_init_kernel<<<grid_for(ctx, cap), 256, 0, ctx->stream>>>(L->v, 0, L->cache_size, 0, 1, L->ctl,
+                                                                  kUpdatesUnknown);
     BCMC_HIP_CHECK(hipGetLastError());
     return BCMC_OK;
 }
@@ -208,19 +219,31 @@ void lru_state_free(LruState *L) {
     dev_free(L->v.item_slot);
     dev_free(L->data);
     dev_free(L->new_items);
+    dev_free(L->ctl);
     scan_site_free(&L->site_miss);
     scan_site_free(&L->site_avail);
     sort_scratch_free(&L->sort);
     *L = LruState{};
 }
 
-// Radix-sort passes for the eviction order: keys are ~age, and every age is
-// below the tracked bound (initial ages 2+s, grown and reset slots 100000, +1
-// per update), so the 8-bit digits above the bound's are all 0xFF and their
-// passes would keep the order.
-uint32_t lru_sort_passes(const LruState *L) {
+// Radix-sort passes for the eviction order.  Keys are ~age (or ~compressed
+// age after a reset, lru_order_key), and every key is below the bound (raw
+// ages: initial 2+s, grown and reset slots 100000, +1 per update), so the
+// 8-bit digits above the bound's are all 0xFF and their passes would keep the
+// order.
+uint32_t lru_begin_update(LruState *L) {
+    L->age_bound += 1;
+    if (L->updates != kUpdatesUnknown)
+        L->updates += 1;
+    // The device counts the same chains (lru_grow_kernel) and sorts
+    // compressed keys (lru_order_key, at most 2u for u chains) while its count
+    // is below kCompressedUpdates; the host counts first and switches to the
+    // raw-age bound 0x1000 updates earlier, so its bound always covers the
+    // device's keys.
+    const uint64_t bound = L->updates < kCompressedUpdates - 0x1000 ? 2 * (uint64_t)L->updates + 1
+                                                                     : L->age_bound;
     uint32_t bits = 0;
-    while (bits < 32 && (L->age_bound >> bits) != 0)
+    while (bits < 32 && (bound >> bits) != 0)
         ++bits;
     return std::max<uint32_t>(1, (bits + 7) / 8);
 }
@@ -231,8 +254,8 @@ static void enqueue_tail(bcmc_ctx *ctx, LruState *L, const uint32_t *needed, QSt
                          const uint32_t *gate, uint32_t npasses) {
     hipStream_t st = ctx->stream;
     launch_scan<1>(ctx, L->site_avail, L->site_avail, AgeAvailOp{L->v, needed, q, gate}, L->capacity);
-    lru_grow_kernel<<<grid_for(ctx, L->capacity, 2), 256, 0, st>>>(L->v, q, gate);
-    launch_lru_order(ctx, &L->sort, L->v.avail_ids, L->v.age, &q->num_avail_total, L->capacity,
+    lru_grow_kernel<<<grid_for(ctx, L->capacity, 2), 256, 0, st>>>(L->v, q, gate, L->ctl);
+    launch_lru_order(ctx, &L->sort, L->v.avail_ids, L->v.age, L->ctl, &q->num_avail_total, L->capacity,
                      L->v.order, npasses);
 }
 
@@ -244,14 +267,14 @@ void lru_enqueue_update_mc(bcmc_ctx *ctx, LruState *L, const uint32_t *flags, QS
 void lru_enqueue_update_all(bcmc_ctx *ctx, LruState *L, const uint32_t *needed, QState *q) {
     launch_scan<1>(ctx, L->site_miss, L->site_miss, MissAllOp{L->v, needed, L->new_items, q},
                    L->aligned_total);
-    enqueue_tail(ctx, L, needed, q, nullptr, lru_sort_passes(L));
+    enqueue_tail(ctx, L, needed, q, nullptr, lru_begin_update(L));
     lru_assign_kernel<<<grid_for(ctx, L->capacity, 2), 256, 0, ctx->stream>>>(L->v, L->new_items, q);
-    L->age_bound += 1;
 }
 
 void lru_enqueue_reset(bcmc_ctx *ctx, LruState *L) {
+    L->updates = 0;
     lru_init_kernel<<<grid_for(ctx, std::max(L->capacity, L->aligned_total)), 256, 0, ctx->stream>>>(
-        L->v, 0, L->cache_size, 100000, 1);
+        L->v, 0, L->cache_size, 100000, 1, L->ctl, 0u);
 }
 
 }  // namespace bcmc

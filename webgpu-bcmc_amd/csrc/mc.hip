@@ -571,7 +571,8 @@ BCMC_API int bcmc_mc_compute_surface(bcmc_mc *m, float isovalue, bcmc_perf *perf
     hipStream_t st = ctx->stream;
     const bool timed = perf != nullptr;
     *m->iso_h = isovalue;
-    const uint32_t npasses = lru_sort_passes(&m->lru.st);
+    // counted before the launch, so the host's update count never lags the device's
+    const uint32_t npasses = lru_begin_update(&m->lru.st);
     const uint64_t key[4] = {(uint64_t)(uintptr_t)m->verts, m->vert_cap,
                              (uint64_t)m->fmt | ctx->tune_gen << 8, npasses};
     bool launched = false;
@@ -602,7 +603,6 @@ BCMC_API int bcmc_mc_compute_surface(bcmc_mc *m, float isovalue, bcmc_perf *perf
     if (!launched)
         BCMC_TRY(mc_enqueue_query(m, npasses, timed));
     BCMC_HIP_CHECK(hipStreamSynchronize(st));
-    m->lru.st.age_bound += 1;
     QState qs = *m->qh;
     if (qs.num_verts > m->vert_cap) {
         // grow like compressed_marching_cubes.js:774-785, then run the pass again

@@ -60,6 +60,7 @@ struct BufSource {  // passes >= 1
 // (64-chunks ascending, descending inside a chunk), key = ~age (descending).
 struct LruSource {
     const uint32_t *avail_ids, *age;
+    const uint32_t *ctl;  // ctl[0]: update chains since the reset (lru_order_key)
     const uint32_t *n_dev;
     uint32_t capacity;
     BCMC_DEV uint32_t size() const { return *n_dev; }
@@ -71,7 +72,7 @@ struct LruSource {
         v = avail_ids[j];
         const bool ok = v < capacity;
         const uint32_t a = age[ok ? v : 0u];
-        k = ok ? ~a : 0u;
+        k = ok ? lru_order_key(a, *ctl) : 0u;
     }
 };
 
@@ -259,9 +260,9 @@ void sort_scratch_free(SortScratch *s) {
 }
 
 void launch_lru_order(bcmc_ctx *ctx, SortScratch *s, const uint32_t *avail_ids,
-                      const uint32_t *age, const uint32_t *count_dev, uint32_t count_max,
-                      uint32_t *out, uint32_t npasses) {
-    LruSource src{avail_ids, age, count_dev, count_max};
+                      const uint32_t *age, const uint32_t *ctl, const uint32_t *count_dev,
+                      uint32_t count_max, uint32_t *out, uint32_t npasses) {
+    LruSource src{avail_ids, age, ctl, count_dev, count_max};
     sort_passes(ctx, s, count_max, count_dev, 0, src, npasses, out, 0);
 }
 
