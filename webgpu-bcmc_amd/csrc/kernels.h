@@ -230,12 +230,12 @@ BCMC_DEV inline uint32_t lru_order_key(uint32_t age, uint32_t u) {
 
 // Orders the `count` available cache slots for LRU assignment exactly as the
 // reference's RadixSorter.sort(ages, slot ids, S, reverse=true) does, writing
-// slot ids to out[0..count).  count and the update count ctl[0] are read from
-// device memory.  npasses: the 8-bit digits of the keys that vary (those above
-// every key are all 0xFF).
+// slot ids to out[0..count).  avail_keys[i] is the order key of slot
+// avail_ids[i] (lru_order_key).  count is read from device memory.  npasses:
+// the 8-bit digits of the keys that vary (those above every key are all 0xFF).
 void launch_lru_order(bcmc_ctx *ctx, SortScratch *s, const uint32_t *avail_ids,
-                      const uint32_t *age, const uint32_t *ctl, const uint32_t *count_dev,
-                      uint32_t count_max, uint32_t *out, uint32_t npasses);
+                      const uint32_t *avail_keys, const uint32_t *count_dev, uint32_t count_max,
+                      uint32_t *out, uint32_t npasses);
 // Stable ascending sort of `count` (key, value) pairs (count read from device
 // memory) on the low 8 * npasses key bits; sorted values to out[0..count).
 void launch_sort_pairs(bcmc_ctx *ctx, SortScratch *s, const uint32_t *keys, const uint32_t *vals,

@@ -12,6 +12,7 @@ struct LruView {
     int32_t *item = nullptr;      // [capacity], -1 = empty
     int32_t *item_slot = nullptr; // [aligned_total], -1 = not cached (cachedItemSlots)
     uint32_t *avail_ids = nullptr;// [capacity] compacted available slots
+    uint32_t *avail_keys = nullptr;// [capacity] their eviction-order keys (lru_order_key)
     uint32_t *order = nullptr;    // [capacity] available slots in eviction order
     uint32_t total = 0;           // totalElements
     uint32_t capacity = 0;        // reserved slots
@@ -24,9 +25,10 @@ struct LruState {
     uint32_t total = 0, aligned_total = 0, elem_bytes = 0, capacity = 0;
     uint32_t cache_size = 0;      // host mirror of the logical size
     uint64_t age_bound = 0;       // every slot age is below this (host-side bound)
-    // Update chains since the last reset (kUpdatesUnknown before the first
-    // one): device word, advanced by each chain's growth kernel, and its host
-    // mirror, advanced per enqueued update.  See lru_order_key.
+    // ctl[0]: update chains since the last reset (kUpdatesUnknown before the
+    // first one), advanced once per chain by the growth kernel; ctl[1]: the
+    // order key of a fresh slot in the current chain.  `updates` is the host
+    // mirror of ctl[0], advanced per enqueued update.  See lru_order_key.
     uint32_t *ctl = nullptr;
     uint32_t updates = 0xFFFFFFFFu;
     ScanSite site_miss, site_avail;

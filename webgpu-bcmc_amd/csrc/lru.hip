@@ -86,16 +86,23 @@ struct AgeAvailOp {
     const uint32_t *needed;
     QState *q;
     const uint32_t *gate;
+    uint32_t *ctl;  // [0] update chains since the reset (read only here), [1] set by total()
     BCMC_DEV uint32_t size() const { return (gate && *gate == 0) ? 0u : q->cache_size; }
     BCMC_DEV void begin() const {}
     struct Tile {
         bool inserting;  // q->num_new != 0
+        uint32_t u;      // this chain's count for the order keys
     };
     struct Item {
         uint32_t age;
         uint32_t avail;  // bit 0: available; bit 1: the slot holds an item (avail is rewritten)
     };
-    BCMC_DEV Tile tile(uint32_t) const { return {q->num_new != 0}; }
+    // this chain's count: the chains counted so far plus this one
+    BCMC_DEV uint32_t chain_count() const {
+        const uint32_t c = ctl[0];
+        return c == kUpdatesUnknown ? c : c + 1;
+    }
+    BCMC_DEV Tile tile(uint32_t) const { return {q->num_new != 0, chain_count()}; }
     BCMC_DEV void value(const Tile &tc, uint32_t s, uint32_t (&x)[1], Item &it) const {
         const uint32_t age = v.age[s], avail = v.avail[s];
         const int32_t p = v.item[s];
@@ -107,16 +114,19 @@ struct AgeAvailOp {
         it.age = keep ? 0u : age + 1;
         x[0] = (uint32_t)(av != 0) & (uint32_t)tc.inserting;
     }
-    BCMC_DEV void emit(const Tile &, uint32_t s, const uint32_t (&x)[1], const Item &it,
+    BCMC_DEV void emit(const Tile &tc, uint32_t s, const uint32_t (&x)[1], const Item &it,
                        const unsigned long long (&e)[1]) const {
         v.age[s] = it.age;
         if (it.avail & 2u)
             v.avail[s] = it.avail & 1u;
-        if (x[0])
+        if (x[0]) {
             v.avail_ids[e[0]] = s;
+            v.avail_keys[e[0]] = lru_order_key(it.age, tc.u);
+        }
     }
     BCMC_DEV void total(const unsigned long long (&t)[1]) const {
         q->num_avail = q->num_new ? (uint32_t)t[0] : 0u;
+        ctl[1] = lru_order_key(100000, chain_count());  // for the slots the growth appends
     }
 };
 
@@ -127,8 +137,10 @@ struct AgeAvailOp {
 // evaluates the rule; thread 0 of workgroup 0 records C' and S'.
 __global__ void lru_grow_kernel(LruView v, QState *q, const uint32_t *gate, uint32_t *ctl) {
     const bool run = !(gate && *gate == 0) && q->num_new != 0;
+    const uint32_t fresh_key = ctl[1];
     // one more update chain since the reset (counted whether or not the
-    // update had work: the count only has to bound the updates from above)
+    // update had work: the count only has to bound the updates from above);
+    // nothing else in this launch reads ctl[0]
     if (blockIdx.x == 0 && threadIdx.x == 0 && ctl[0] != kUpdatesUnknown)
         ctl[0] += 1;
     const uint32_t c = q->cache_size, avail = run ? q->num_avail : 0u;
@@ -149,6 +161,7 @@ __global__ void lru_grow_kernel(LruView v, QState *q, const uint32_t *gate, uint
         v.avail[s] = 1;
         v.item[s] = -1;
         v.avail_ids[avail + (s - c)] = s;
+        v.avail_keys[avail + (s - c)] = fresh_key;
     }
 }
 
@@ -190,13 +203,14 @@ int lru_state_alloc(bcmc_ctx *ctx, LruState *L, uint32_t initial, uint32_t elem_
     BCMC_TRY(dev_alloc((void **)&L->v.avail, cap * 4));
     BCMC_TRY(dev_alloc((void **)&L->v.item, cap * 4));
     BCMC_TRY(dev_alloc((void **)&L->v.avail_ids, cap * 4));
+    BCMC_TRY(dev_alloc((void **)&L->v.avail_keys, cap * 4));
     BCMC_TRY(dev_alloc((void **)&L->v.order, cap * 4));
     BCMC_TRY(dev_alloc((void **)&L->v.item_slot, (size_t)L->aligned_total * 4 + 4));
     // one extra zeroed 256-byte slot past the end: absent blocks read it
     BCMC_TRY(dev_alloc((void **)&L->data, cap * elem_bytes + 256));
     BCMC_HIP_CHECK(hipMemsetAsync((char *)L->data + cap * elem_bytes, 0, 256, ctx->stream));
     BCMC_TRY(dev_alloc((void **)&L->new_items, (size_t)L->aligned_total * 4 + 4));
-    BCMC_TRY(dev_alloc((void **)&L->ctl, 4));
+    BCMC_TRY(dev_alloc((void **)&L->ctl, 8));
     BCMC_TRY(scan_site_alloc(&L->site_miss, L->aligned_total, kScanTile));
     BCMC_TRY(scan_site_alloc(&L->site_avail, cap, kScanTile));
     BCMC_TRY(sort_scratch_alloc(&L->sort, L->capacity));
@@ -215,6 +229,7 @@ void lru_state_free(LruState *L) {
     dev_free(L->v.avail);
     dev_free(L->v.item);
     dev_free(L->v.avail_ids);
+    dev_free(L->v.avail_keys);
     dev_free(L->v.order);
     dev_free(L->v.item_slot);
     dev_free(L->data);
@@ -253,9 +268,9 @@ uint32_t lru_begin_update(LruState *L) {
 static void enqueue_tail(bcmc_ctx *ctx, LruState *L, const uint32_t *needed, QState *q,
                          const uint32_t *gate, uint32_t npasses) {
     hipStream_t st = ctx->stream;
-    launch_scan<1>(ctx, L->site_avail, L->site_avail, AgeAvailOp{L->v, needed, q, gate}, L->capacity);
+    launch_scan<1>(ctx, L->site_avail, L->site_avail, AgeAvailOp{L->v, needed, q, gate, L->ctl}, L->capacity);
     lru_grow_kernel<<<grid_for(ctx, L->capacity, 2), 256, 0, st>>>(L->v, q, gate, L->ctl);
-    launch_lru_order(ctx, &L->sort, L->v.avail_ids, L->v.age, L->ctl, &q->num_avail_total, L->capacity,
+    launch_lru_order(ctx, &L->sort, L->v.avail_ids, L->v.avail_keys, &q->num_avail_total, L->capacity,
                      L->v.order, npasses);
 }
 

@@ -59,8 +59,7 @@ struct BufSource {  // passes >= 1
 // LRU: position p of the available-slot list in the reference's tie order
 // (64-chunks ascending, descending inside a chunk), key = ~age (descending).
 struct LruSource {
-    const uint32_t *avail_ids, *age;
-    const uint32_t *ctl;  // ctl[0]: update chains since the reset (lru_order_key)
+    const uint32_t *avail_ids, *avail_keys;
     const uint32_t *n_dev;
     uint32_t capacity;
     BCMC_DEV uint32_t size() const { return *n_dev; }
@@ -70,9 +69,8 @@ struct LruSource {
         const uint32_t m = min(64u, n - c0);
         const uint32_t j = c0 + (m - 1 - (p - c0));
         v = avail_ids[j];
-        const bool ok = v < capacity;
-        const uint32_t a = age[ok ? v : 0u];
-        k = ok ? lru_order_key(a, *ctl) : 0u;
+        const uint32_t kk = avail_keys[j];
+        k = v < capacity ? kk : 0u;
     }
 };
 
@@ -260,9 +258,9 @@ void sort_scratch_free(SortScratch *s) {
 }
 
 void launch_lru_order(bcmc_ctx *ctx, SortScratch *s, const uint32_t *avail_ids,
-                      const uint32_t *age, const uint32_t *ctl, const uint32_t *count_dev,
-                      uint32_t count_max, uint32_t *out, uint32_t npasses) {
-    LruSource src{avail_ids, age, ctl, count_dev, count_max};
+                      const uint32_t *avail_keys, const uint32_t *count_dev, uint32_t count_max,
+                      uint32_t *out, uint32_t npasses) {
+    LruSource src{avail_ids, avail_keys, count_dev, count_max};
     sort_passes(ctx, s, count_max, count_dev, 0, src, npasses, out, 0);
 }
 
