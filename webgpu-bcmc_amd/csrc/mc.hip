@@ -38,10 +38,12 @@ struct ActiveOp {
     const float2 *dilated;
     const int32_t *item_slot;
     uint32_t *flags, *active_ids, *new_items;
-    QState *q;  // q->iso holds the query's isovalue (copied in before the scan)
+    QState *q;
+    const float *iso_src;  // the query's isovalue, in host-coherent pinned memory
     uint32_t n;
     BCMC_DEV uint32_t size() const { return n; }
     BCMC_DEV void begin() const {
+        q->iso = *iso_src;  // for the later launches of the query
         q->num_with = 0;
         q->num_verts = 0;
         q->overflow = 0;
@@ -50,7 +52,7 @@ struct ActiveOp {
         float iso;
     };
     using Item = NoItem;
-    BCMC_DEV Tile tile(uint32_t) const { return {q->iso}; }
+    BCMC_DEV Tile tile(uint32_t) const { return {*iso_src}; }
     BCMC_DEV void value(const Tile &tc, uint32_t b, uint32_t (&v)[2], Item &) const {
         const float2 d = dilated[b];
         const int32_t s = item_slot[b];
@@ -158,6 +160,13 @@ __global__ __launch_bounds__(kMcThreads) void mc_count_kernel(McArgs a) {
     }
 }
 
+// The query's counters into host-coherent pinned memory.
+__global__ void qstate_out_kernel(const QState *q, QState *out) {
+    static_assert(sizeof(QState) % 4 == 0 && sizeof(QState) <= 64 * 4, "one word per lane");
+    if (threadIdx.x < sizeof(QState) / 4)
+        reinterpret_cast<uint32_t *>(out)[threadIdx.x] = reinterpret_cast<const uint32_t *>(q)[threadIdx.x];
+}
+
 // Exclusive prefixes of the chunk totals; the grand totals are the query's
 // numBlocksWithVertices and numVertices.
 struct ChunkPrefixOp {
@@ -262,9 +271,12 @@ struct bcmc_mc {
     uint64_t last_verts = 0;
     // the query chain is captured once into a HIP graph and replayed; it is
     // captured again when anything it bakes in (buffers, format, sort passes)
-    // changes.  The isovalue reaches the device through a captured copy from
-    // pinned memory.
+    // changes.  The isovalue is read by the active scan straight from
+    // host-coherent pinned memory, and the counters are written back into
+    // such memory by a one-wave kernel (no copy nodes).
     float *iso_h = nullptr;
+    const float *iso_d = nullptr;  // device address of iso_h
+    QState *qh_d = nullptr;        // device address of qh
     bool use_graph = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -403,11 +415,10 @@ static int mc_enqueue_query(bcmc_mc *m, uint32_t npasses, bool timed) {
     bcmc_ctx *ctx = m->ctx;
     hipStream_t st = ctx->stream;
     LruState &L = m->lru.st;
-    BCMC_HIP_CHECK(hipMemcpyAsync(&m->q->iso, m->iso_h, sizeof(float), hipMemcpyHostToDevice, st));
     if (timed)
         BCMC_HIP_CHECK(hipEventRecord(m->ev[0], st));
     // active blocks and the LRU misses among them, one scan
-    ActiveOp aop{m->dilated, L.v.item_slot, m->flags, m->active_ids, L.new_items, m->q, m->total};
+    ActiveOp aop{m->dilated, L.v.item_slot, m->flags, m->active_ids, L.new_items, m->q, m->iso_d, m->total};
     launch_scan<2>(ctx, m->site_active, L.site_miss, aop, m->total);
     if (timed)
         BCMC_HIP_CHECK(hipEventRecord(m->ev[1], st));
@@ -425,8 +436,8 @@ static int mc_enqueue_query(bcmc_mc *m, uint32_t npasses, bool timed) {
     BCMC_TRY(mc_launch_pass(m, timed ? &m->ev[4] : nullptr));
     if (timed)
         BCMC_HIP_CHECK(hipEventRecord(m->ev[6], st));
+    qstate_out_kernel<<<1, 64, 0, st>>>(m->q, m->qh_d);
     BCMC_HIP_CHECK(hipGetLastError());
-    BCMC_HIP_CHECK(hipMemcpyAsync(m->qh, m->q, sizeof(QState), hipMemcpyDeviceToHost, st));
     return BCMC_OK;
 }
 
@@ -437,11 +448,15 @@ BCMC_API int bcmc_mc_create(bcmc_ctx *ctx, bcmc_mc **out) {
     bcmc_mc *m = new bcmc_mc;
     m->ctx = ctx;
     if (dev_alloc((void **)&m->q, sizeof(QState)) != BCMC_OK ||
-        hipHostMalloc(&m->qh, sizeof(QState), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(&m->iso_h, sizeof(float), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc(&m->qh, sizeof(QState), hipHostMallocCoherent) != hipSuccess ||
+        hipHostMalloc(&m->iso_h, sizeof(float), hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&m->qh_d, m->qh, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&m->iso_d, m->iso_h, 0) != hipSuccess) {
         dev_free(m->q);
         if (m->qh)
             (void)hipHostFree(m->qh);
+        if (m->iso_h)
+            (void)hipHostFree(m->iso_h);
         delete m;
         set_error("allocation of the query counters failed");
         return BCMC_ERR_NOMEM;
