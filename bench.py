@@ -161,6 +161,7 @@ def served_bench(bcmc_amd, synth, dev, args):
            "saturation_latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                                      "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
            "counts_equal_plain": exact, "steps": int(sat["step"].max()) + 1,
+           "vertices_total": int(nv_plain.sum()),
            "mean_batch": round(float(sat["batch_size"].mean()), 1),
            "hbm_budget_bytes": budget, "pool_bytes": eng.stats()["pool_bytes"], "poisson": []}
     for rate in args.served_rates:

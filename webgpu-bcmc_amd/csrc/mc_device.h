@@ -226,8 +226,9 @@ BCMC_DEV RowMeta row_meta_load(const RowBlock &b) {
     block_coords(b, bx, by, bz);
     RowMeta m;
     m.grid = (bx + 1 < b.nb[0] ? 1u : 0u) | (by + 1 < b.nb[1] ? 2u : 0u) | (bz + 1 < b.nb[2] ? 4u : 0u);
-    const uint32_t f = l - 8;
-    const uint32_t code = l < 8 ? l : (f < 3 ? (1u << f) : (f == 3 ? 3u : f == 4 ? 5u : f == 5 ? 6u : 7u));
+    // lanes 8..14: neighbour code (x | y << 1 | z << 2) 1, 2, 4, 3, 5, 6, 7,
+    // one nibble each of 0x7653421 (a select, not a branch tree)
+    const uint32_t code = l < 8 ? l : ((0x7653421u >> (4u * (l - 8u))) & 15u);
     const uint32_t x = bx + (code & 1u), y = by + ((code >> 1) & 1u), z = bz + (code >> 2);
     m.v = l < 8 ? -1 : 0;
     if (b.live && l < 15 && x < b.nb[0] && y < b.nb[1] && z < b.nb[2]) {
@@ -259,8 +260,10 @@ struct RowTile {
 };
 
 BCMC_DEV void row_fetch(const RowBlock &b, int32_t s, int32_t sx, uint32_t off, float (&r)[5]) {
-    const uint32_t si = (s >= 0 && (uint32_t)s < b.cap) ? (uint32_t)s : b.cap;
-    const uint32_t sj = (sx >= 0 && (uint32_t)sx < b.cap) ? (uint32_t)sx : b.cap;
+    // a slot that is absent (negative: huge as unsigned) or past the cache
+    // reads the zero slot `cap`
+    const uint32_t si = min((uint32_t)s, b.cap);
+    const uint32_t sj = min((uint32_t)sx, b.cap);
     const float4 v = *reinterpret_cast<const float4 *>(b.cache + (uint64_t)si * 64 + off);
     r[0] = v.x;
     r[1] = v.y;
@@ -276,23 +279,13 @@ BCMC_DEV void row_voxels(const RowBlock &b, const uint32_t *meta, RowTile &t) {
     const uint32_t l = lane_id() & 15u, y = l & 3u, z = l >> 2;
     t.dims = meta[8];
     t.live = b.live;
-    int32_t s0 = -1, s1 = -1, s2 = -1, s3 = -1, s4 = -1, s5 = -1, s6 = -1, s7 = -1;
-    if (b.live) {
-        s0 = (int32_t)meta[0];
-        s1 = (int32_t)meta[1];
-        if (y == 3) {
-            s2 = (int32_t)meta[2];
-            s3 = (int32_t)meta[3];
-        }
-        if (z == 3) {
-            s4 = (int32_t)meta[4];
-            s5 = (int32_t)meta[5];
-        }
-        if (y == 3 && z == 3) {
-            s6 = (int32_t)meta[6];
-            s7 = (int32_t)meta[7];
-        }
-    }
+    // all 8 words are read (LDS; selects, not branches); rows without a
+    // block and lanes off the upper faces take -1 (absent)
+    const bool gy = b.live && y == 3, gz = b.live && z == 3;
+    const int32_t s0 = b.live ? (int32_t)meta[0] : -1, s1 = b.live ? (int32_t)meta[1] : -1;
+    const int32_t s2 = gy ? (int32_t)meta[2] : -1, s3 = gy ? (int32_t)meta[3] : -1;
+    const int32_t s4 = gz ? (int32_t)meta[4] : -1, s5 = gz ? (int32_t)meta[5] : -1;
+    const int32_t s6 = gy && gz ? (int32_t)meta[6] : -1, s7 = gy && gz ? (int32_t)meta[7] : -1;
     row_fetch(b, s0, s1, 4 * y + 16 * z, t.r);
     row_fetch(b, s2, s3, 16 * z, t.gy);
     row_fetch(b, s4, s5, 4 * y, t.gz);
