@@ -251,6 +251,29 @@ def test_lru_invariants(initial):
             assert lru.size == (want + 31) // 32 * 32
 
 
+@pytest.mark.parametrize("initial", [32, 256])
+def test_lru_ages_after_reset_are_bounded(initial):
+    """The invariant behind the GPU's compressed eviction keys: in the u-th
+    update after a reset, every slot used since the reset has an age below u
+    and every other slot an age in [100000, 100000 + u] (ages read after the
+    update; the keys are formed during it, one aging earlier), whatever the
+    needed sets and however the cache grows."""
+    total = 1000
+    rng = np.random.default_rng(initial + 5)
+    lru = oracle.LRU(initial, 256, total)
+    lru.update(_needed_sets(total, 1, rng)[0])
+    lru.reset()
+    age, _, _, _ = lru.state()
+    assert np.all(age == 100000)
+    for u, needed in enumerate(_needed_sets(total, 40, rng), start=1):
+        lru.update(needed)
+        age, _, item, _ = lru.state()
+        fresh = age >= 100000
+        assert np.all(age[~fresh] <= u - 1), u
+        assert np.all(age[fresh] <= 100000 + u), u
+        assert np.all(item[fresh] == -1), u
+
+
 def test_lru_first_misses_take_highest_slots():
     lru = oracle.LRU(64, 256, 100)
     f = np.zeros(100, np.uint32)
