@@ -52,8 +52,9 @@ def profiled_traffic(workload, stage):
 
 
 def stage_bytes(s, rate, n_blocks):
-    """Algorithmic HBM bytes per stage of one query (per-stage model of the
-    reference's kernels)."""
+    """Algorithmic HBM bytes per stage of one query: what each stage must read
+    and write at least once (ids, ranges, slot words, block data, ghost voxels,
+    vertices), from the query's counts."""
     a, d, h, v = s["numActiveBlocks"], s["numBlocksDecompressed"], s["numBlocksWithVertices"], \
         s["numVertices"]
     return {
@@ -237,7 +238,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(the path shards by z-slab or by volume; see DESIGN.md)", file=sys.stderr)
+              "(a multi-GPU run would shard the volume by z-slabs of blocks, or the served "
+              "volumes by GPU; that path is not built)", file=sys.stderr)
         sys.exit(2)
 
     import bcmc_amd
