@@ -62,7 +62,7 @@ def stage_bytes(s, rate, n_blocks):
         "computeActiveBlocks": 16 * n_blocks + 4 * a + 4 * d,
         # payload 8r + block id 4 + slot 4 + 64 floats 256, per decoded block
         "decompression": d * (8 * rate + 4 + 4 + 256),
-        # count: active id 4 + forward range 8 + count 4 per active block; per
+        # count: active id 4 + tile range 8 + count 4 per active block; per
         # candidate its slots/flags 60, own block 256 and ghost voxels <= 244
         # (the candidates, which the pass does not report, include the H blocks
         # with vertices: H is used, a lower bound)
