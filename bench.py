@@ -177,6 +177,8 @@ def served_bench(bcmc_amd, synth, dev, args):
                                "mean_batch": round(float(r["batch_size"].mean()), 2)})
         log(f"bench: served poisson {rate} req/s: p50 {out['poisson'][-1]['p50_ms']} ms "
             f"p99 {out['poisson'][-1]['p99_ms']} ms")
+    st = eng.stats()
+    out["engine_steps"], out["graph_replayed_steps"] = st["steps"], st["graph_steps"]
     del eng
     return out
 
@@ -208,7 +210,8 @@ def batched_bench(bcmc_amd, synth, dev, args):
            "step_ms": round(step * 1e3, 3), "qps": round(64 / step, 2),
            "plain_qps": round(64 / t_plain.sum(), 2),
            "batched_vs_plain": round(t_plain.sum() / step, 3),
-           "counts_equal_plain": bool(np.array_equal(nv, nv_plain))}
+           "counts_equal_plain": bool(np.array_equal(nv, nv_plain)),
+           "graph_replayed_steps": eng.stats()["graph_steps"], "timed_steps": len(reps)}
     log(f"bench: batched step {out['step_ms']} ms ({out['qps']} q/s; plain {out['plain_qps']} q/s)")
     del eng
     return out

@@ -225,6 +225,8 @@ typedef struct {
 typedef struct {
     uint32_t num_volumes, pool_slots, pool_used, epoch;
     uint64_t pool_bytes, device_bytes;
+    uint32_t steps;       /* bcmc_engine_run calls that launched a step */
+    uint32_t graph_steps; /* of which replayed a captured HIP graph */
 } bcmc_engine_info;
 /* pool_bytes: HBM for decoded blocks (256 B per block). */
 BCMC_API int bcmc_engine_create(bcmc_ctx *ctx, uint64_t pool_bytes, bcmc_engine **out);

@@ -139,3 +139,37 @@ def test_many_queries_in_one_step(dev):
         if k % 37 == 0:
             got = eng.vertices(res[k])
             assert np.array_equal(got.view(np.uint32), verts.view(np.uint32)), k
+
+
+@pytest.mark.parametrize("pool_blocks", [100000, 3000])
+def test_repeated_step_shapes_replay_a_graph(dev, pool_blocks):
+    """A step shape (volumes in query order) seen a second time is captured
+    into a HIP graph and replayed whenever it comes again; replayed steps with
+    new isovalues, after a step of another shape and (small pool) with the
+    eviction chain inside the graph all still equal the oracle's single
+    queries."""
+    say("engine graph replay, pool", pool_blocks)
+    streams = _streams()
+    eng = bcmc_amd.Engine(dev, pool_blocks * 256)
+    ids = [eng.add_volume(s, r, d) for s, r, d in streams]
+    shape_a = np.array([ids[0], ids[1], ids[1], ids[2], ids[3], ids[0]], np.uint32)
+    shape_b = shape_a[1:4]
+    rng = np.random.default_rng(pool_blocks + 1)
+    cache = {}
+    plan = [shape_a] * 5 + [shape_b] + [shape_a] * 3
+    for step, qv in enumerate(plan):
+        qi = rng.uniform(0.15, 0.85, len(qv)).astype(np.float32)
+        res = eng.run(qv, qi)
+        for k in range(len(qv)):
+            s, r, d = streams[qv[k]]
+            key = (int(qv[k]), float(qi[k]))
+            if key not in cache:
+                cache[key] = _oracle(s, r, d, float(qi[k]))
+            nv, verts = cache[key]
+            assert res[k]["num_vertices"] == nv, (step, k)
+            got = eng.vertices(res[k])
+            assert np.array_equal(got.view(np.uint32), verts.view(np.uint32)), (step, k)
+    st = eng.stats()
+    assert st["steps"] == len(plan)
+    # vertex-buffer growth re-captures, so not every repeat replays
+    assert st["graph_steps"] >= 2, st

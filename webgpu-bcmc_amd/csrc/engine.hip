@@ -31,9 +31,11 @@
 #include "mc_device.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
 #include <initializer_list>
 #include <thread>
@@ -652,6 +654,15 @@ struct EngVolume {
     uint32_t *miss_list = nullptr;  // per-step block ids to decode (EngVol::miss_list)
 };
 
+using EngKey = std::array<uint64_t, 6>;
+struct EngGraph {
+    EngKey key;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    uint64_t last_use;
+};
+constexpr size_t kEngGraphs = 16, kEngSeen = 64;
+
 struct bcmc_engine {
     bcmc_ctx *ctx = nullptr;
     std::vector<EngVolume> vols;
@@ -691,19 +702,66 @@ struct bcmc_engine {
     int fmt = BCMC_VERTS_LOCAL_F32;
     bcmc_buf vert_view;
     uint64_t budget = 0;
+    // replay of the step's kernel chain from captured HIP graphs: a step
+    // shape (key in bcmc_engine_run) is captured the second time it is seen
+    // among the last kEngSeen steps, and replayed whenever it comes again; the
+    // kEngGraphs most recently used graphs are kept.  gen is bumped whenever a
+    // buffer the chain reads is replaced, which also drops every graph.
+    bool use_graph = true;
+    uint64_t gen = 0;
+    std::vector<EngGraph> graphs;
+    std::vector<EngKey> seen;
+    size_t seen_next = 0;
+    uint64_t clock = 0;
+    uint32_t steps = 0, graph_steps = 0;
     // last step
     std::vector<bcmc_query_result> last;
 };
 
+static void eng_drop_graph(EngGraph &g) {
+    (void)hipGraphExecDestroy(g.exec);
+    (void)hipGraphDestroy(g.graph);
+}
+
+static void eng_drop_graphs(bcmc_engine *e) {
+    for (EngGraph &g : e->graphs)
+        eng_drop_graph(g);
+    e->graphs.clear();
+    e->seen.clear();
+    e->seen_next = 0;
+}
+
+// a buffer the step chain reads was replaced (the caller has synchronised)
+static void eng_changed(bcmc_engine *e) {
+    ++e->gen;
+    eng_drop_graphs(e);
+}
+
+template <class T> static void eng_release(T *&p) {
+    dev_free((void *)p);
+    p = nullptr;
+}
+
+template <class T> static void eng_release_host(T *&p) {
+    if (p)
+        (void)hipHostFree((void *)p);
+    p = nullptr;
+}
+
+// Growing a buffer first synchronises, drops the captured chain and zeroes
+// the capacity, so an allocation failure part-way leaves no dangling pointer
+// and the next step allocates again.
 static int eng_ensure_blocks(bcmc_engine *e, uint64_t total) {
     if (total <= e->blocks_cap)
         return BCMC_OK;
     BCMC_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
-    dev_free(e->flags);
-    dev_free(e->act_q);
-    dev_free(e->act_b);
-    dev_free(e->counts);
+    eng_changed(e);
     const uint64_t cap = std::max<uint64_t>(total, e->blocks_cap * 3 / 2);
+    e->blocks_cap = 0;
+    eng_release(e->flags);
+    eng_release(e->act_q);
+    eng_release(e->act_b);
+    eng_release(e->counts);
     BCMC_TRY(dev_alloc((void **)&e->flags, cap * 4));
     // flag words hold the epoch of the step that found the block active;
     // zero is no step's epoch
@@ -719,17 +777,21 @@ static int eng_ensure_blocks(bcmc_engine *e, uint64_t total) {
 static int eng_ensure_rows(bcmc_engine *e, uint64_t rows, uint64_t row_elems) {
     if (rows > e->rows_cap) {
         BCMC_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
-        dev_free(e->cand_rows);
-        scan_site_free(&e->site_rows);
+        eng_changed(e);
         const uint64_t cap = std::max<uint64_t>(rows, e->rows_cap * 3 / 2);
+        e->rows_cap = 0;
+        eng_release(e->cand_rows);
+        scan_site_free(&e->site_rows);
         BCMC_TRY(dev_alloc((void **)&e->cand_rows, cap * sizeof(CandRow)));
         BCMC_TRY(scan_site_alloc(&e->site_rows, cap, kScanTile));
         e->rows_cap = cap;
     }
     if (row_elems > e->row_elems_cap) {
         BCMC_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
-        scan_site_free(&e->site_active);
+        eng_changed(e);
         const uint64_t cap = std::max<uint64_t>(row_elems, e->row_elems_cap * 3 / 2);
+        e->row_elems_cap = 0;
+        scan_site_free(&e->site_active);
         BCMC_TRY(scan_site_alloc(&e->site_active, cap, kScanTile));
         e->row_elems_cap = cap;
     }
@@ -740,20 +802,16 @@ static int eng_ensure_queries(bcmc_engine *e, uint32_t nq) {
     if (nq <= e->q_cap)
         return BCMC_OK;
     BCMC_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
-    dev_free(e->d_q);
-    dev_free(e->q_act_start);
-    dev_free(e->q_pair_end);
-    dev_free(e->q_chunk_base);
-    dev_free(e->q_vfirst);
-    if (e->h_q)
-        (void)hipHostFree(e->h_q);
-    if (e->h_act)
-        (void)hipHostFree(e->h_act);
-    if (e->h_vfirst)
-        (void)hipHostFree(e->h_vfirst);
-    e->h_q = nullptr;
-    e->h_act = nullptr;
-    e->h_vfirst = nullptr;
+    eng_changed(e);
+    e->q_cap = 0;
+    eng_release(e->d_q);
+    eng_release(e->q_act_start);
+    eng_release(e->q_pair_end);
+    eng_release(e->q_chunk_base);
+    eng_release(e->q_vfirst);
+    eng_release_host(e->h_q);
+    eng_release_host(e->h_act);
+    eng_release_host(e->h_vfirst);
     const uint32_t cap = std::max<uint32_t>(nq, 64);
     BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_q, (size_t)cap * sizeof(EngQuery), hipHostMallocDefault));
     BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_act, (size_t)cap * 4, hipHostMallocDefault));
@@ -816,10 +874,12 @@ static int eng_launch_emit(bcmc_engine *e, uint32_t nq, uint64_t total) {
 static int eng_ensure_chunks(bcmc_engine *e, uint64_t max_chunks) {
     if (max_chunks > e->chunks_cap) {
         BCMC_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
-        dev_free(e->chunk_verts);
-        dev_free(e->chunk_verts_pre);
-        scan_site_free(&e->site_chunks);
+        eng_changed(e);
         const uint64_t cap = std::max<uint64_t>(max_chunks, e->chunks_cap * 3 / 2);
+        e->chunks_cap = 0;
+        eng_release(e->chunk_verts);
+        eng_release(e->chunk_verts_pre);
+        scan_site_free(&e->site_chunks);
         BCMC_TRY(dev_alloc((void **)&e->chunk_verts, cap * 4));
         BCMC_TRY(dev_alloc((void **)&e->chunk_verts_pre, cap * 8));
         BCMC_TRY(scan_site_alloc(&e->site_chunks, cap, kScanTile));
@@ -839,6 +899,55 @@ static int eng_launch_mc(bcmc_engine *e, uint32_t nq, uint64_t total) {
     launch_scan<1>(e->ctx, e->site_chunks, e->site_chunks,
                    EngChunkPrefixOp{a.chunk_verts, a.chunk_verts_pre, a.c}, max_chunks);
     return eng_launch_emit(e, nq, total);
+}
+
+// What a step's kernel chain bakes in besides the engine's buffers.
+struct EngPlan {
+    uint32_t nq;
+    uint64_t rows, row_elems, total;
+    uint64_t used_mask;    // volumes the step touches (bit per volume id)
+    uint32_t sort_passes;  // eviction-order sort passes; 0: no eviction chain
+};
+
+// The step's kernel chain, from the epoch bump to the emit.  It reads the
+// step's queries, tile table and same-volume lists from the device tables
+// uploaded before it, and makes no allocation, copy or memset, so it can be
+// captured into a graph and replayed.
+static int eng_enqueue_step(bcmc_engine *e, const EngPlan &p) {
+    bcmc_ctx *ctx = e->ctx;
+    hipStream_t st = ctx->stream;
+    const uint32_t nq = p.nq;
+    eng_begin_kernel<<<1, 256, 0, st>>>(e->c, e->q_act_start, nq);
+    // active pairs: candidate row segments first, then their blocks
+    launch_scan<1>(ctx, e->site_rows, e->site_rows,
+                   CandSegOp{e->d_q, e->tile_q, e->cand_rows, e->c, (uint32_t)p.rows}, p.rows);
+    SegActiveOp aop{e->cand_rows, e->flags, e->act_q, e->act_b, e->q_act_start, e->c};
+    launch_scan<1>(ctx, e->site_active, e->site_active, aop, p.row_elems);
+    // marching-cubes chunks of every query (also the work units of the mark)
+    eng_chunk_table_kernel<<<1, 256, 0, st>>>(e->c, e->q_act_start, nq, e->q_pair_end, e->q_chunk_base);
+    eng_mark_kernel<<<(uint32_t)std::max<uint64_t>(
+                          1, std::min<uint64_t>(div_up(p.total, 256 * kMarkPairs), persistent_grid(ctx, 8))),
+                      256, 0, st>>>(eng_mc_args(e, nq), e->stamp, e->miss, e->miss_cap);
+    if (p.sort_passes) {
+        launch_scan<1>(ctx, e->site_cand, e->site_cand, CandOp{e->stamp, e->cand, e->c, e->pool_cap},
+                       e->pool_cap);
+        eng_cand_keys_kernel<<<eng_grid(ctx, e->pool_cap, 2), 256, 0, st>>>(e->stamp, e->cand, e->c,
+                                                                             e->cand_keys);
+        launch_sort_pairs(ctx, &e->sort, e->cand_keys, e->cand, &e->c->num_cand, e->pool_cap,
+                          e->order, p.sort_passes);
+    }
+    eng_assign_kernel<<<eng_grid(ctx, p.total, 4), 256, 0, st>>>(e->d_vols, e->c, e->miss, e->order,
+                                                                  e->pool_cap, e->stamp, e->owner);
+    eng_commit_kernel<<<1, 1, 0, st>>>(e->c, e->pool_cap);
+    // decode the misses of each volume into their slots
+    for (uint32_t vi = 0; vi < e->vols.size(); ++vi) {
+        if (!(p.used_mask >> vi & 1))
+            continue;
+        EngVolume &v = e->vols[vi];
+        launch_decode_to_cache(ctx, v.stream, v.nwords, v.maxbits, v.miss_list, &e->c->vol_miss[vi],
+                               v.n, v.slot_map, e->pool, e->pool_cap);
+    }
+    return eng_launch_mc(e, nq, p.total);
 }
 
 extern "C" {
@@ -880,6 +989,8 @@ BCMC_API int bcmc_engine_create(bcmc_ctx *ctx, uint64_t pool_bytes, bcmc_engine 
         return rc;
     }
     *e->ch = EngCounters{};
+    const char *ng = std::getenv("BCMC_NO_GRAPH");
+    e->use_graph = !(ng && ng[0] == '1');
     BCMC_HIP_CHECK(hipMemsetAsync(e->c, 0, sizeof(EngCounters), ctx->stream));
     BCMC_HIP_CHECK(hipMemsetAsync(e->stamp, 0, (size_t)e->pool_cap * 4, ctx->stream));
     BCMC_HIP_CHECK(hipMemsetAsync(e->owner, 0xFF, (size_t)e->pool_cap * 8, ctx->stream));
@@ -892,6 +1003,7 @@ BCMC_API int bcmc_engine_destroy(bcmc_engine *e) {
     if (!e)
         return BCMC_OK;
     (void)hipStreamSynchronize(e->ctx->stream);
+    eng_drop_graphs(e);
     for (auto &v : e->vols) {
         dev_free(v.stream);
         dev_free(v.ranges);
@@ -1020,8 +1132,9 @@ BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t
     uint64_t all = 0;
     for (auto &w : e->vols)
         all += w.n;
-    dev_free(e->miss);
-    e->miss = nullptr;
+    eng_changed(e);
+    e->miss_cap = 0;
+    eng_release(e->miss);
     BCMC_TRY(dev_alloc((void **)&e->miss, all * 8));
     e->miss_cap = (uint32_t)std::min<uint64_t>(all, 0xFFFFFFFFu);
     *volume_id = id;
@@ -1033,8 +1146,8 @@ BCMC_API int bcmc_engine_set_vertex_format(bcmc_engine *e, bcmc_vertex_format fm
     BCMC_CHECK_ARG(fmt >= BCMC_VERTS_LOCAL_F32 && fmt <= BCMC_VERTS_PACKED_U32X2, "bad format");
     if (fmt != e->fmt) {
         (void)hipStreamSynchronize(e->ctx->stream);
-        dev_free(e->verts);
-        e->verts = nullptr;
+        eng_changed(e);
+        eng_release(e->verts);
         e->vert_cap = 0;
         e->fmt = fmt;
     }
@@ -1046,6 +1159,10 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     BCMC_CHECK_ARG(e && (nq == 0 || (volume_ids && isovalues)), "null argument");
     if (nq == 0)
         return BCMC_OK;
+    if (!e->miss) {
+        set_error("engine has no miss list (an earlier add_volume failed to allocate it)");
+        return BCMC_ERR_STATE;
+    }
     bcmc_ctx *ctx = e->ctx;
     hipStream_t st = ctx->stream;
     std::vector<EngQuery> qs(nq);
@@ -1085,12 +1202,11 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     BCMC_TRY(eng_ensure_queries(e, nq));
     if (same.size() > e->same_cap || !e->same_list) {
         BCMC_HIP_CHECK(hipStreamSynchronize(st));
-        dev_free(e->same_list);
-        e->same_list = nullptr;
-        if (e->h_same)
-            (void)hipHostFree(e->h_same);
-        e->h_same = nullptr;
+        eng_changed(e);
         const uint64_t cap = std::max<uint64_t>(std::max<uint64_t>(same.size(), 1024), e->same_cap * 3 / 2);
+        e->same_cap = 0;
+        eng_release(e->same_list);
+        eng_release_host(e->h_same);
         BCMC_TRY(dev_alloc((void **)&e->same_list, cap * 4));
         BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_same, cap * 4, hipHostMallocDefault));
         e->same_cap = cap;
@@ -1099,12 +1215,11 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     const uint64_t ntiles = div_up(rows, kScanTile) + 1;
     if (ntiles > e->tile_q_cap) {
         BCMC_HIP_CHECK(hipStreamSynchronize(st));
-        dev_free(e->tile_q);
-        e->tile_q = nullptr;
-        if (e->h_tile_q)
-            (void)hipHostFree(e->h_tile_q);
-        e->h_tile_q = nullptr;
+        eng_changed(e);
         const uint64_t cap = std::max<uint64_t>(ntiles, e->tile_q_cap * 2);
+        e->tile_q_cap = 0;
+        eng_release(e->tile_q);
+        eng_release_host(e->h_tile_q);
         BCMC_TRY(dev_alloc((void **)&e->tile_q, cap * 4));
         BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_tile_q, cap * 4, hipHostMallocDefault));
         e->tile_q_cap = cap;
@@ -1115,8 +1230,71 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
             ++q;
         e->h_tile_q[t] = (uint32_t)q;
     }
-    // pinned staging: the copies run asynchronously (the step ends with a
-    // stream synchronisation before the staging is written again)
+    // eviction order (only does work when the misses exceed the free room).
+    // A step misses at most every block of each volume it touches; when the
+    // free room after the last step covers that, eviction cannot happen and
+    // the chain is not launched (assign then never reads the order).
+    EngPlan p{nq, rows, row_elems, total, 0, 0};
+    uint64_t most_misses = 0;
+    for (uint32_t vi = 0; vi < e->vols.size(); ++vi) {
+        most_misses += used[vi] ? e->vols[vi].n : 0u;
+        p.used_mask |= used[vi] ? 1ull << vi : 0;
+    }
+    if ((uint64_t)e->pool_cap - e->ch->used < most_misses) {
+        // keys are stamps, at most this step's epoch
+        uint32_t bits = 0;
+        while (bits < 32 && ((uint64_t)(e->ch->epoch + 1) >> bits) != 0)
+            ++bits;
+        p.sort_passes = std::max<uint32_t>(1, (bits + 7) / 8);
+    }
+    BCMC_TRY(eng_ensure_chunks(e, eng_max_chunks(total, nq)));
+    // the chain reads the step through the uploaded tables; everything the
+    // launches bake in (sizes, volumes touched, sort passes, buffers, vertex
+    // format, tuned grids) is in the key
+    const EngKey key = {nq | (uint64_t)p.sort_passes << 32, rows, total, p.used_mask,
+                        (uint64_t)e->fmt | ctx->tune_gen << 8, e->gen};
+    EngGraph *hit = nullptr;
+    if (e->use_graph) {
+        for (EngGraph &g : e->graphs)
+            hit = g.key == key ? &g : hit;
+        const bool seen = std::find(e->seen.begin(), e->seen.end(), key) != e->seen.end();
+        if (!hit && seen) {
+            // the stream is idle here: nothing was reallocated (gen is in the
+            // key) and the previous step ended with a synchronisation
+            BCMC_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            const int rc = eng_enqueue_step(e, p);
+            hipGraph_t g = nullptr;
+            hipGraphExec_t x = nullptr;
+            const hipError_t ec = hipStreamEndCapture(st, &g);
+            if (rc == BCMC_OK && ec == hipSuccess && g && hipGraphInstantiate(&x, g, nullptr, nullptr, 0) == hipSuccess) {
+                if (e->graphs.size() == kEngGraphs) {
+                    auto lru = std::min_element(e->graphs.begin(), e->graphs.end(),
+                                                [](const EngGraph &u, const EngGraph &v) { return u.last_use < v.last_use; });
+                    eng_drop_graph(*lru);
+                    e->graphs.erase(lru);
+                }
+                e->graphs.push_back(EngGraph{key, g, x, 0});
+                hit = &e->graphs.back();
+            } else {
+                if (g)
+                    (void)hipGraphDestroy(g);
+                eng_drop_graphs(e);
+                e->use_graph = false;  // replay is an optimisation: fall back to direct launches
+                (void)hipGetLastError();
+            }
+        } else if (!hit) {
+            if (e->seen.size() < kEngSeen)
+                e->seen.push_back(key);
+            else
+                e->seen[e->seen_next] = key;
+            e->seen_next = (e->seen_next + 1) % kEngSeen;
+        }
+        if (hit)
+            hit->last_use = ++e->clock;
+    }
+    // pinned staging: the copies run asynchronously, ahead of the chain on the
+    // same stream (the step ends with a stream synchronisation before the
+    // staging is written again)
     BCMC_HIP_CHECK(hipMemcpyAsync(e->tile_q, e->h_tile_q, ntiles * 4, hipMemcpyHostToDevice, st));
     std::memcpy(e->h_q, qs.data(), nq * sizeof(EngQuery));
     BCMC_HIP_CHECK(hipMemcpyAsync(e->d_q, e->h_q, nq * sizeof(EngQuery), hipMemcpyHostToDevice, st));
@@ -1124,50 +1302,12 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
         std::memcpy(e->h_same, same.data(), same.size() * 4);
         BCMC_HIP_CHECK(hipMemcpyAsync(e->same_list, e->h_same, same.size() * 4, hipMemcpyHostToDevice, st));
     }
-    eng_begin_kernel<<<1, 256, 0, st>>>(e->c, e->q_act_start, nq);
-    // active pairs: candidate row segments first, then their blocks
-    launch_scan<1>(ctx, e->site_rows, e->site_rows,
-                   CandSegOp{e->d_q, e->tile_q, e->cand_rows, e->c, (uint32_t)rows}, rows);
-    SegActiveOp aop{e->cand_rows, e->flags, e->act_q, e->act_b, e->q_act_start, e->c};
-    launch_scan<1>(ctx, e->site_active, e->site_active, aop, row_elems);
-    // marching-cubes chunks of every query (also the work units of the mark)
-    const uint64_t max_chunks = eng_max_chunks(total, nq);
-    BCMC_TRY(eng_ensure_chunks(e, max_chunks));
-    eng_chunk_table_kernel<<<1, 256, 0, st>>>(e->c, e->q_act_start, nq, e->q_pair_end, e->q_chunk_base);
-    eng_mark_kernel<<<(uint32_t)std::max<uint64_t>(
-                          1, std::min<uint64_t>(div_up(total, 256 * kMarkPairs), persistent_grid(ctx, 8))),
-                      256, 0, st>>>(eng_mc_args(e, nq), e->stamp, e->miss, e->miss_cap);
-    // eviction order (only does work when the misses exceed the free room).
-    // A step misses at most every block of each volume it touches; when the
-    // free room after the last step covers that, eviction cannot happen and
-    // the chain is not launched (assign then never reads the order).
-    uint64_t most_misses = 0;
-    for (uint32_t vi = 0; vi < e->vols.size(); ++vi)
-        most_misses += used[vi] ? e->vols[vi].n : 0u;
-    if ((uint64_t)e->pool_cap - e->ch->used < most_misses) {
-        launch_scan<1>(ctx, e->site_cand, e->site_cand, CandOp{e->stamp, e->cand, e->c, e->pool_cap},
-                       e->pool_cap);
-        eng_cand_keys_kernel<<<eng_grid(ctx, e->pool_cap, 2), 256, 0, st>>>(e->stamp, e->cand, e->c,
-                                                                             e->cand_keys);
-        // keys are stamps, at most this step's epoch
-        uint32_t bits = 0;
-        while (bits < 32 && ((uint64_t)(e->ch->epoch + 1) >> bits) != 0)
-            ++bits;
-        launch_sort_pairs(ctx, &e->sort, e->cand_keys, e->cand, &e->c->num_cand, e->pool_cap,
-                          e->order, std::max<uint32_t>(1, (bits + 7) / 8));
-    }
-    eng_assign_kernel<<<eng_grid(ctx, total, 4), 256, 0, st>>>(e->d_vols, e->c, e->miss, e->order,
-                                                                e->pool_cap, e->stamp, e->owner);
-    eng_commit_kernel<<<1, 1, 0, st>>>(e->c, e->pool_cap);
-    // decode the misses of each volume into their slots
-    for (uint32_t vi = 0; vi < e->vols.size(); ++vi) {
-        if (!used[vi])
-            continue;
-        EngVolume &v = e->vols[vi];
-        launch_decode_to_cache(ctx, v.stream, v.nwords, v.maxbits, v.miss_list, &e->c->vol_miss[vi],
-                               v.n, v.slot_map, e->pool, e->pool_cap);
-    }
-    BCMC_TRY(eng_launch_mc(e, nq, total));
+    if (hit)
+        BCMC_HIP_CHECK(hipGraphLaunch(hit->exec, st));
+    else
+        BCMC_TRY(eng_enqueue_step(e, p));
+    ++e->steps;
+    e->graph_steps += hit ? 1 : 0;
     BCMC_HIP_CHECK(hipMemcpyAsync(e->ch, e->c, sizeof(EngCounters), hipMemcpyDeviceToHost, st));
     const unsigned long long *qv = e->h_vfirst;
     const uint32_t *qa = e->h_act;
@@ -1182,8 +1322,8 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     const uint64_t nv = e->ch->num_verts;
     if (nv > e->vert_cap) {
         const uint64_t want = std::max<uint64_t>(nv, e->vert_cap * 2);
-        dev_free(e->verts);
-        e->verts = nullptr;
+        eng_changed(e);
+        eng_release(e->verts);
         e->vert_cap = 0;
         BCMC_TRY(dev_alloc(&e->verts, want * vbytes(e->fmt)));
         e->vert_cap = want;
@@ -1235,6 +1375,8 @@ BCMC_API int bcmc_engine_stats(bcmc_engine *e, bcmc_engine_info *info) {
     for (auto &v : e->vols)
         dev += v.nwords * 8 + (uint64_t)v.n * 16;
     info->device_bytes = dev;
+    info->steps = e->steps;
+    info->graph_steps = e->graph_steps;
     return BCMC_OK;
 }
 

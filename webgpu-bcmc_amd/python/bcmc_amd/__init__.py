@@ -70,7 +70,8 @@ class _QueryResult(ctypes.Structure):
 class _EngineInfo(ctypes.Structure):
     _fields_ = [("num_volumes", ctypes.c_uint32), ("pool_slots", ctypes.c_uint32),
                 ("pool_used", ctypes.c_uint32), ("epoch", ctypes.c_uint32),
-                ("pool_bytes", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64)]
+                ("pool_bytes", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
+                ("steps", ctypes.c_uint32), ("graph_steps", ctypes.c_uint32)]
 
 
 class _Tuning(ctypes.Structure):
