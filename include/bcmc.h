@@ -268,6 +268,15 @@ BCMC_API int bcmc_zfp_decode_blocks(bcmc_ctx *ctx, bcmc_buf *stream, uint32_t ma
 /* Block value ranges (zfp_compute_block_range.comp): float2 per block. */
 BCMC_API int bcmc_zfp_block_ranges(bcmc_ctx *ctx, bcmc_buf *stream, uint32_t max_bits,
                                    const uint32_t dims[3], bcmc_buf *ranges);
+/* Fixed-rate zfp encoder on the device: the ingest side of the stream format
+ * the decoders read.  No reference counterpart (the reference loads streams
+ * compressed offline with zfp 0.5.5, whose zfp_compress in fixed-rate mode at
+ * rate max_bits/64 this matches byte for byte).  field: finite float32
+ * values, x fastest, dims[0]*dims[1]*dims[2] of them; stream: at least
+ * ceil(nblocks*max_bits/64)*8 bytes, all of it overwritten (zeroed, then the
+ * blocks' bits; leave two extra words for the decoders' read-ahead). */
+BCMC_API int bcmc_zfp_encode(bcmc_ctx *ctx, bcmc_buf *field, const uint32_t dims[3],
+                             uint32_t max_bits, bcmc_buf *stream);
 
 /* With timing enabled, bcmc_zfp_decode_blocks and bcmc_zfp_block_ranges wait
  * for their kernel and record its device time (HIP events on the context

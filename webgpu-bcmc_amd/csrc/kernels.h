@@ -223,7 +223,7 @@ constexpr uint32_t kCompressedUpdates = 0x8000;
 
 // Sort key of a slot's age: ~(compressed age) while u < kCompressedUpdates,
 // ~age otherwise (u = 0xFFFFFFFF before the first reset: constructor ages).
-BCMC_DEV inline uint32_t lru_order_key(uint32_t age, uint32_t u) {
+BCMC_DEV uint32_t lru_order_key(uint32_t age, uint32_t u) {
     const uint32_t c = age >= 100000u ? u + (age - 100000u) : age;
     return ~(u < kCompressedUpdates ? c : age);
 }

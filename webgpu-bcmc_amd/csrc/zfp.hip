@@ -478,6 +478,203 @@ void launch_decode_dense(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords,
         launch_mode<kModeDense>(a, grid, ctx->stream);
 }
 
+// ---------------------------------------------------------------------------
+// Fixed-rate encoder (zfp 0.5.5 compress, float32 3-D, fixed-rate mode): the
+// inverse of the decoder above, byte-identical to libzfp's zfp_compress.  One
+// lane per block: gather with zfp's partial-block padding, block exponent,
+// quantise to int32, forward decorrelating lifts (x, then y, then z),
+// sequency permutation + negabinary, embedded bit-plane coding with group
+// tests, zero padding up to exactly max_bits bits.  The stream words must be
+// zero on entry; a block's first and last words may be shared with its
+// neighbours (max_bits not a multiple of 64) and are merged with atomic ORs,
+// every other word is owned by one block and stored plainly.
+// ---------------------------------------------------------------------------
+constexpr int kEncThreads = 256;
+
+// zfp's pad_block for one line of 4 values with n valid (n <= 4)
+BCMC_DEV void pad_line(float &p0, float &p1, float &p2, float &p3, uint32_t n) {
+    p0 = n == 0 ? 0.0f : p0;
+    p1 = n < 2 ? p0 : p1;
+    p2 = n < 3 ? p1 : p2;
+    p3 = n < 4 ? p0 : p3;
+}
+
+BCMC_DEV uint32_t asr1(uint32_t v) { return (uint32_t)((int32_t)v >> 1); }
+
+// forward lift of 4 values (int32, wrapping; zfp fwd_lift)
+BCMC_DEV void fwd_lift4(uint32_t &x, uint32_t &y, uint32_t &z, uint32_t &w) {
+    x += w; x = asr1(x); w -= x;
+    z += y; z = asr1(z); y -= z;
+    x += z; x = asr1(x); z -= x;
+    w += y; w = asr1(w); y -= w;
+    w += asr1(y); y -= asr1(w);
+}
+
+// float -> int32 as x86's truncating conversion (what libzfp's (int32)(s*f)
+// compiles to): values outside the int32 range, infinities and NaN give
+// INT32_MIN.  In range it is the plain truncation.
+BCMC_DEV uint32_t quantise(float v) {
+    return (v >= -2147483648.0f && v < 2147483648.0f) ? (uint32_t)(int32_t)v : 0x80000000u;
+}
+
+// LSB-first writer of one block's bits into zeroed stream words.
+struct BitSink {
+    uint64_t *out;
+    uint64_t word;               // index of the word being filled
+    uint64_t first, last;        // the block's first and last words
+    bool share_first, share_last;
+    uint64_t acc = 0;
+    uint32_t fill;               // bits of acc in use
+    BCMC_DEV void flush() {
+        const bool shared = (word == first && share_first) || (word == last && share_last);
+        if (shared)
+            atomicOr(reinterpret_cast<unsigned long long *>(out + word), (unsigned long long)acc);
+        else
+            out[word] = acc;
+        ++word;
+    }
+    // the low n bits of v (bits above n are zero), n <= 64
+    BCMC_DEV void put(uint64_t v, uint32_t n) {
+        acc |= fill < 64 ? v << fill : 0ull;
+        if (fill + n >= 64) {
+            const uint32_t used = 64 - fill;  // bits of v that went into acc
+            flush();
+            acc = used < 64 ? v >> used : 0ull;
+            fill = fill + n - 64;
+        } else {
+            fill += n;
+        }
+    }
+    BCMC_DEV void finish() {
+        if (fill)
+            flush();
+    }
+};
+
+BCMC_DEV uint64_t low_bits(uint64_t x, uint32_t m) { return m >= 64 ? x : x & ((1ull << m) - 1ull); }
+
+__global__ __launch_bounds__(kEncThreads) void zfp_encode_kernel(const float *vol, uint32_t d0, uint32_t d1,
+                                                                  uint32_t d2, uint32_t nbx, uint32_t nby,
+                                                                  uint64_t nblocks, uint32_t maxbits,
+                                                                  uint64_t *out) {
+    const uint64_t stride = (uint64_t)gridDim.x * kEncThreads;
+    for (uint64_t b = (uint64_t)blockIdx.x * kEncThreads + threadIdx.x; b < nblocks; b += stride) {
+        const uint32_t bx = (uint32_t)(b % nbx), by = (uint32_t)((b / nbx) % nby),
+                       bz = (uint32_t)(b / ((uint64_t)nbx * nby));
+        const uint32_t x0 = 4 * bx, y0 = 4 * by, z0 = 4 * bz;
+        const uint32_t nx = min(4u, d0 - x0), ny = min(4u, d1 - y0), nz = min(4u, d2 - z0);
+        // gather with clamped coordinates, then pad along x, y, z (the
+        // clamped values are all overwritten by the padding)
+        float f[64];
+#pragma unroll
+        for (int z = 0; z < 4; ++z)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+                const uint64_t row = ((uint64_t)min(z0 + z, d2 - 1) * d1 + min(y0 + y, d1 - 1)) * d0;
+#pragma unroll
+                for (int x = 0; x < 4; ++x)
+                    f[16 * z + 4 * y + x] = vol[row + min(x0 + x, d0 - 1)];
+            }
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            pad_line(f[4 * r], f[4 * r + 1], f[4 * r + 2], f[4 * r + 3], nx);
+#pragma unroll
+        for (int z = 0; z < 4; ++z)
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+                pad_line(f[16 * z + x], f[16 * z + 4 + x], f[16 * z + 8 + x], f[16 * z + 12 + x], ny);
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            pad_line(f[r], f[16 + r], f[32 + r], f[48 + r], nz);
+        float amax = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 64; ++i)
+            amax = fmaxf(amax, fabsf(f[i]));
+        // zfp's exponent: frexp exponent of the largest magnitude, at least
+        // 1 - 127 (subnormals), -127 for an all-zero block
+        const uint32_t eb = (__float_as_uint(amax) >> 23) & 0xFFu;
+        const int emax = amax > 0.0f ? (eb == 0 ? -126 : (int)eb - 126) : -127;
+        const uint32_t e = (uint32_t)(emax + 127);
+        const uint64_t start = b * maxbits;
+        BitSink w;
+        w.out = out;
+        w.first = start >> 6;
+        w.last = (start + maxbits - 1) >> 6;
+        w.word = w.first;
+        w.share_first = (start & 63u) != 0;
+        w.share_last = ((start + maxbits) & 63u) != 0;
+        w.fill = (uint32_t)(start & 63u);
+        if (e == 0)
+            continue;  // flag bit 0 and zero padding: the words are already zero
+        w.put(2ull * e + 1ull, 9);
+        const float s = ldexpf(1.0f, 30 - emax);
+        uint32_t q[64];
+#pragma unroll
+        for (int i = 0; i < 64; ++i)
+            q[i] = quantise(s * f[i]);
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            fwd_lift4(q[4 * r], q[4 * r + 1], q[4 * r + 2], q[4 * r + 3]);
+#pragma unroll
+        for (int z = 0; z < 4; ++z)
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+                fwd_lift4(q[16 * z + x], q[16 * z + 4 + x], q[16 * z + 8 + x], q[16 * z + 12 + x]);
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            fwd_lift4(q[r], q[16 + r], q[32 + r], q[48 + r]);
+        uint32_t u[64];
+#pragma unroll
+        for (int i = 0; i < 64; ++i)
+            u[i] = (q[perm3d(i)] + 0xaaaaaaaau) ^ 0xaaaaaaaau;
+        // embedded coding: per bit plane, the first n coefficients' bits
+        // verbatim, then group tests: a 1 says another coefficient becomes
+        // significant, followed by the run of zeros up to it and its 1 bit
+        uint32_t bits = maxbits - 9, n = 0;
+        for (int k = 31; k >= 0 && bits; --k) {
+            uint64_t x = 0;
+#pragma unroll
+            for (int i = 0; i < 64; ++i)
+                x |= (uint64_t)((u[i] >> k) & 1u) << i;
+            const uint32_t m = min(n, bits);
+            bits -= m;
+            w.put(low_bits(x, m), m);
+            x = m >= 64 ? 0ull : x >> m;
+            while (n < 64 && bits) {
+                --bits;
+                w.put(x != 0, 1);
+                if (!x)
+                    break;
+                // zeros up to the next 1, capped by the coefficient count
+                // and the budget
+                const uint32_t tz = (uint32_t)__builtin_ctzll(x), cap = min(63u - n, bits);
+                if (tz < cap) {
+                    w.put(1ull << tz, tz + 1);
+                    bits -= tz + 1;
+                    x >>= tz;
+                    n += tz;
+                } else {
+                    w.put(0, cap);
+                    bits -= cap;
+                    x = cap >= 64 ? 0ull : x >> cap;
+                    n += cap;
+                }
+                x >>= 1;
+                ++n;
+            }
+        }
+        w.finish();
+    }
+}
+
+void launch_encode(bcmc_ctx *ctx, const float *vol, const uint32_t d[3], uint32_t maxbits, uint64_t *out) {
+    const uint32_t nbx = div_up(d[0], 4), nby = div_up(d[1], 4), nbz = div_up(d[2], 4);
+    const uint64_t n = (uint64_t)nbx * nby * nbz;
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>(div_up(n, kEncThreads), persistent_grid(ctx, 8)));
+    zfp_encode_kernel<<<grid, kEncThreads, 0, ctx->stream>>>(vol, d[0], d[1], d[2], nbx, nby, n, maxbits, out);
+}
+
 }  // namespace bcmc
 
 using namespace bcmc;
@@ -524,6 +721,26 @@ extern "C" BCMC_API int bcmc_zfp_block_ranges(bcmc_ctx *ctx, bcmc_buf *stream, u
         BCMC_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
     launch_block_ranges(ctx, (const uint64_t *)stream->ptr, stream->bytes / 8, max_bits, dims,
                         (float2 *)ranges->ptr);
+    BCMC_HIP_CHECK(hipGetLastError());
+    return finish_timing(ctx);
+}
+
+extern "C" BCMC_API int bcmc_zfp_encode(bcmc_ctx *ctx, bcmc_buf *field, const uint32_t dims[3],
+                                        uint32_t max_bits, bcmc_buf *stream) {
+    BCMC_CHECK_ARG(ctx && field && stream && dims, "null argument");
+    BCMC_CHECK_ARG(max_bits >= 9 && max_bits <= 4096, "max_bits must be in [9, 4096]");
+    BCMC_CHECK_ARG(dims[0] > 0 && dims[1] > 0 && dims[2] > 0, "bad dims");
+    const uint64_t nb = (uint64_t)((dims[0] + 3) / 4) * ((dims[1] + 3) / 4) * ((dims[2] + 3) / 4);
+    const uint64_t voxels = (uint64_t)dims[0] * dims[1] * dims[2];
+    BCMC_CHECK_ARG(field->bytes >= voxels * 4, "field buffer smaller than dims");
+    const uint64_t words = (nb * max_bits + 63) / 64;
+    BCMC_CHECK_ARG(stream->bytes >= words * 8, "stream buffer smaller than the encoded volume");
+    BCMC_CHECK_ARG(nb < (1ull << 40), "volume has too many blocks");
+    if (ctx->timing)
+        BCMC_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
+    // the kernel ORs bits into shared words: the whole stream buffer starts at zero
+    BCMC_HIP_CHECK(hipMemsetAsync(stream->ptr, 0, stream->bytes, ctx->stream));
+    launch_encode(ctx, (const float *)field->ptr, dims, max_bits, (uint64_t *)stream->ptr);
     BCMC_HIP_CHECK(hipGetLastError());
     return finish_timing(ctx);
 }

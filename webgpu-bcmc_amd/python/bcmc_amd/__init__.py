@@ -18,6 +18,7 @@ gfx950 GPU raises BcmcError.
 """
 import ctypes
 import os
+import re
 
 import numpy as np
 
@@ -165,6 +166,7 @@ def lib():
         "bcmc_serve_trace": (i32, [vp, vp, vp, vp, u32, P(_ServeOpts), vp]),
         "bcmc_zfp_decode_blocks": (i32, [vp, vp, u32, vp, u32, vp]),
         "bcmc_zfp_block_ranges": (i32, [vp, vp, u32, P(u32), vp]),
+        "bcmc_zfp_encode": (i32, [vp, vp, P(u32), u32, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -611,6 +613,54 @@ def decode_blocks(device, stream_buf, max_bits, n, ids_buf=None):
     _check(lib().bcmc_zfp_decode_blocks(device.h, stream_buf.h, max_bits,
                                         ids_buf.h if ids_buf is not None else None, n, out.h))
     return out
+
+
+def stream_bytes(dims, max_bits):
+    """Bytes of a fixed-rate stream of a volume: whole 64-bit words."""
+    nb = int(np.prod([(d + 3) // 4 for d in dims]))
+    return (nb * max_bits + 63) // 64 * 8
+
+
+def encode(device, field_buf, dims, max_bits):
+    """Fixed-rate zfp encoding of a float32 device volume (x fastest) on the
+    GPU; returns a device stream buffer of stream_bytes(dims, max_bits) bytes
+    plus two zero words, so the decoders can read it in place."""
+    out = Buffer.alloc(device, stream_bytes(dims, max_bits) + 16)
+    _check(lib().bcmc_zfp_encode(device.h, field_buf.h, _u32arr(dims), max_bits, out.h))
+    return out
+
+
+_CRATE = re.compile(r"(\w+)_(\d+)x(\d+)x(\d+)_(\w+)\.*")
+_CRATE_RATE = re.compile(r"\.crate(\d+)$")
+
+
+def crate_info(filename):
+    """Dims and compression rate of a zfp volume file from its name, e.g.
+    'vertebra_512x512x512_uint16.raw.crate4' -> ((512, 512, 512), 4).  Dims
+    follow the reference's name rule (js/volumes.js:147-152); the rate is the
+    number after '.crate' (None when the name has none)."""
+    base = os.path.basename(filename)
+    m = _CRATE.search(base)
+    if not m:
+        raise ValueError(f"no <name>_<X>x<Y>x<Z>_<type> pattern in {base!r}")
+    r = _CRATE_RATE.search(base)
+    return (int(m.group(2)), int(m.group(3)), int(m.group(4))), (int(r.group(1)) if r else None)
+
+
+def load_crate(filename, rate=None):
+    """Read a headerless zfp fixed-rate stream file; returns (bytes, rate,
+    dims) as setCompressedVolume takes them.  rate overrides the name's."""
+    dims, name_rate = crate_info(filename)
+    rate = rate if rate is not None else name_rate
+    if rate is None:
+        raise ValueError(f"{filename!r}: no '.crate<rate>' suffix and no rate given")
+    with open(filename, "rb") as f:
+        data = np.frombuffer(f.read(), np.uint8)
+    nb = int(np.prod([(d + 3) // 4 for d in dims]))
+    if data.size * 8 < nb * int(round(64 * rate)):
+        raise ValueError(f"{filename!r}: {data.size} bytes are fewer than a {dims} volume "
+                         f"at rate {rate} needs")
+    return data, rate, dims
 
 
 def block_ranges(device, stream_buf, max_bits, dims):
