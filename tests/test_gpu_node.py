@@ -55,3 +55,36 @@ def test_facade_sweep(tmp_path):
         assert np.array_equal(slots, ref.lru_state()[3])
     assert out["perfLen"] == 3 and "computeVertices" in out["perfKeys"]
     assert out["after_reset_age0"] == 100000
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+def test_sweep_benchmark_matches_oracle(tmp_path):
+    """The facade's sweepUp benchmark (100 isovalues after an LRU reset, on a
+    cache too small for the volume) gives the oracle's counts, decoded blocks
+    included, at every step."""
+    assert os.path.exists(ADDON), "N-API addon not built"
+    dims, rate, cache, rng_ = (36, 40, 28), 4, 160, [0.2, 0.8]
+    f = synth.field(synth.BLOBS, 23, dims, 6, (3, 10, 0.5, 1))
+    s = synth.encode(f, 64 * rate)
+    (tmp_path / "params.json").write_text(json.dumps({"dims": dims, "rate": rate, "cache": cache,
+                                                      "range": rng_}))
+    (tmp_path / "stream.bin").write_bytes(s.tobytes())
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "benchmark_protocol.js"), str(tmp_path),
+                        "gpu"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads((tmp_path / "out.json").read_text())
+    perf = out["perf"]
+    assert out["fileName"] == "bcmc-explicit-blob-sweepUp.json"
+    assert len(perf["isovalue"]) == 100 and len(perf["numVertices"]) == 100
+    ref = oracle.MC(s, 64 * rate, dims, initial_cache=cache)
+    ref.compute_surface(0.5)
+    ref.lru_reset()
+    cur = rng_[0]
+    for k in range(100):
+        cur += (rng_[1] - rng_[0]) / 100
+        assert perf["isovalue"][k] == cur
+        nv = ref.compute_surface(cur)
+        assert perf["numVertices"][k] == nv, k
+        assert perf["numActiveBlocks"][k] == ref.stats["num_active"], k
+        assert perf["numBlocksDecompressed"][k] == ref.stats["num_decoded"], k
+        assert perf["numBlocksWithVertices"][k] == ref.stats["num_with_verts"], k

@@ -293,4 +293,5 @@ module.exports = {
     VertexFormat,
     McArray,
     version: native.version,
+    ...require('./benchmark.js'),
 };

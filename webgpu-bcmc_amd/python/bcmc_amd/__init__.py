@@ -19,6 +19,7 @@ gfx950 GPU raises BcmcError.
 import ctypes
 import os
 import re
+import time
 
 import numpy as np
 
@@ -541,6 +542,45 @@ class CompressedMarchingCubes:
         t, c = ctypes.c_uint64(), ctypes.c_uint64()
         _check(lib().bcmc_mc_memory_use(self.h, ctypes.byref(t), ctypes.byref(c)))
         return {"total": t.value, "cache": c.value}
+
+
+def isovalue_benchmark_values(kind, value_range, random=None):
+    """The 100 isovalues of the reference app's benchmarks
+    (js/run_benchmark.js:1-47): 'random' draws U[lo, hi) from random()
+    (default numpy's generator), 'sweepUp' / 'sweepDown' move by
+    (hi - lo) / 100 per step from lo / hi, accumulated."""
+    lo, hi = value_range
+    if kind == "random":
+        draw = random or np.random.default_rng().random
+        return [draw() * (hi - lo) + lo for _ in range(100)]
+    if kind not in ("sweepUp", "sweepDown"):
+        raise ValueError(f"unknown benchmark {kind!r} (random, sweepUp or sweepDown)")
+    out, cur, step = [], lo if kind == "sweepUp" else hi, (hi - lo) / 100
+    for _ in range(100):
+        cur = cur + step if kind == "sweepUp" else cur - step
+        out.append(cur)
+    return out
+
+
+def run_isovalue_benchmark(mc, kind, value_range, dataset_name="volume", random=None, enable_cache=True):
+    """One benchmark on a CompressedMarchingCubes as the reference app drives it
+    (js/bcmc.js:319-370): LRU reset, then a computeSurface with a perfTracker
+    whenever the isovalue changed (every step without the cache: reset
+    first).  Returns (perf results, the file name the app saves them under)."""
+    perf = {"isovalue": [], "totalTime": [], "memUse": []}
+    mc.lruCache.reset()
+    current = None
+    for iso in isovalue_benchmark_values(kind, value_range, random):
+        if not enable_cache:
+            mc.lruCache.reset()
+        if iso != current:
+            current = iso
+            t = time.perf_counter()
+            mc.computeSurface(iso, perf)
+            perf["totalTime"].append((time.perf_counter() - t) * 1e3)
+            perf["isovalue"].append(iso)
+            perf["memUse"].append(mc.reportMemoryUse())
+    return perf, f"bcmc-explicit-{dataset_name}-{kind}.json"
 
 
 class Engine:
