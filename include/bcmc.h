@@ -147,6 +147,14 @@ typedef enum {
     BCMC_VERTS_PACKED_U32X2 = 2
 } bcmc_vertex_format;
 
+/* World-space float xyz of vertices first..first+n of a BCMC_VERTS_LOCAL_F32
+ * buffer, as the reference's renderer places them (shaders/
+ * mc_isosurface.vert:51-55): scale * ((4*block_pos + local) / dims - 0.5),
+ * dims the volume's dims.  out: at least 12*n bytes.  Enqueued on the
+ * context stream. */
+BCMC_API int bcmc_vertices_world(bcmc_ctx *ctx, bcmc_buf *verts, uint64_t first, uint64_t n,
+                                 const uint32_t dims[3], const float scale[3], bcmc_buf *out);
+
 typedef struct {
     /* perfTracker keys of compressed_marching_cubes.js:593-610, milliseconds of
      * device time per stage: computeBlockHasVertices is the count launch,

@@ -161,3 +161,36 @@ def test_launch_shapes_leave_results_unchanged(dev):
             v = mc.vertices()
             assert np.array_equal(v.view(np.uint32), ref.results()["verts_f32"].view(np.uint32))
     d.set_tuning(**{k: t[k] for k in ("scan", "sort", "decode", "mc", "count")})
+
+
+def test_world_positions_follow_the_renderer_formula(dev):
+    """World-space expansion of the block-local vertices: the renderer's
+    placement (mc_isosurface.vert:51-55) evaluated in float32 with numpy,
+    separately rounded steps, bit for bit; and within float rounding of the
+    FP64 positions."""
+    say("world positions")
+    dims, rate, scale = (37, 30, 26), 4, (1.0, 1.0, 26 / 37)
+    f = synth.field(synth.GAUSS, 8, dims, 6, (3, 10, 0.5, 1))
+    s = synth.encode(f, 64 * rate)
+    mc = bcmc_amd.CompressedMarchingCubes(dev)
+    mc.setCompressedVolume(s, rate, dims, (1, 1, 1), 200)
+    nv = mc.computeSurface(0.45)
+    assert nv > 1000
+    local = mc.vertices()
+    w = bcmc_amd.world_positions(dev, mc.vertexBuffer, nv, dims, scale).download(np.float32, 3 * nv)
+    w = w.reshape(-1, 3)
+    bid = local[:, 3].view(np.uint32)
+    nbx, nby = (dims[0] + 3) // 4, (dims[1] + 3) // 4
+    bpos = np.stack([bid % nbx, (bid // nbx) % nby, bid // nbx // nby], 1)
+    g = (4 * bpos).astype(np.float32) + local[:, :3]
+    want = np.float32(scale) * (g / np.float32(dims) - np.float32(0.5))
+    assert want.dtype == np.float32
+    assert np.array_equal(w.view(np.uint32), want.view(np.uint32))
+    exact = np.array(scale) * ((4 * bpos + local[:, :3].astype(np.float64)) / np.array(dims) - 0.5)
+    assert np.abs(w - exact).max() < 1e-6
+    # a sub-range, and the argument checks
+    part = bcmc_amd.world_positions(dev, mc.vertexBuffer, 5, dims, scale, first=nv - 5)
+    assert np.array_equal(part.download(np.float32, 15).reshape(5, 3).view(np.uint32),
+                          want[-5:].view(np.uint32))
+    with pytest.raises(bcmc_amd.BcmcError):
+        bcmc_amd.world_positions(dev, mc.vertexBuffer, 10, dims, scale, first=nv - 5)

@@ -173,6 +173,25 @@ struct CtxScratch {
 
 static CtxScratch *scratch_of(bcmc_ctx *ctx);
 
+// World-space positions of block-local vertices, as the reference's vertex
+// shader places them (mc_isosurface.vert:51-55): scale * ((4 * block_pos +
+// local) / dims - 0.5), each step a separately rounded float operation.
+// The local positions are the exact float ones (the shader reads them back
+// from the 10-bit packed form).
+__global__ void world_positions_kernel(const float4 *__restrict__ v, uint64_t n, uint32_t nbx, uint32_t nby,
+                                       float3 dims, float3 scale, float *__restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float4 p = v[i];
+        const uint32_t bid = __float_as_uint(p.w);
+        const uint32_t bx = bid % nbx, by = (bid / nbx) % nby, bz = bid / nbx / nby;
+        const float gx = (float)(4 * bx) + p.x, gy = (float)(4 * by) + p.y, gz = (float)(4 * bz) + p.z;
+        out[3 * i] = scale.x * (gx / dims.x - 0.5f);
+        out[3 * i + 1] = scale.y * (gy / dims.y - 0.5f);
+        out[3 * i + 2] = scale.z * (gz / dims.z - 0.5f);
+    }
+}
+
 }  // namespace bcmc
 
 using namespace bcmc;
@@ -478,6 +497,25 @@ BCMC_API int bcmc_sort_u32_by_key(bcmc_ctx *ctx, bcmc_buf *keys, bcmc_buf *vals,
                                   hipMemcpyDeviceToDevice, ctx->stream));
     BCMC_HIP_CHECK(hipGetLastError());
     BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_vertices_world(bcmc_ctx *ctx, bcmc_buf *verts, uint64_t first, uint64_t n,
+                                 const uint32_t dims[3], const float scale[3], bcmc_buf *out) {
+    BCMC_CHECK_ARG(ctx && verts && dims && scale && out, "null argument");
+    BCMC_CHECK_ARG(dims[0] > 0 && dims[1] > 0 && dims[2] > 0, "bad dims");
+    BCMC_CHECK_ARG(first <= verts->bytes / 16 && n <= verts->bytes / 16 - first,
+                   "vertex range past the end of the buffer");
+    BCMC_CHECK_ARG(out->bytes / 12 >= n, "output buffer smaller than n float xyz");
+    if (n == 0)
+        return BCMC_OK;
+    const uint32_t nbx = (dims[0] + 3) / 4, nby = (dims[1] + 3) / 4;
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(div_up(n, 256), persistent_grid(ctx, 8)));
+    world_positions_kernel<<<grid, 256, 0, ctx->stream>>>(
+        reinterpret_cast<const float4 *>(verts->ptr) + first, n, nbx, nby,
+        make_float3((float)dims[0], (float)dims[1], (float)dims[2]), make_float3(scale[0], scale[1], scale[2]),
+        (float *)out->ptr);
+    BCMC_HIP_CHECK(hipGetLastError());
     return BCMC_OK;
 }
 

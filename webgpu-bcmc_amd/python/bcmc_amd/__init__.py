@@ -168,6 +168,7 @@ def lib():
         "bcmc_zfp_decode_blocks": (i32, [vp, vp, u32, vp, u32, vp]),
         "bcmc_zfp_block_ranges": (i32, [vp, vp, u32, P(u32), vp]),
         "bcmc_zfp_encode": (i32, [vp, vp, P(u32), u32, vp]),
+        "bcmc_vertices_world": (i32, [vp, vp, u64, u64, P(u32), P(ctypes.c_float), vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -652,6 +653,16 @@ def decode_blocks(device, stream_buf, max_bits, n, ids_buf=None):
     out = Buffer.alloc(device, max(n, 1) * 256)
     _check(lib().bcmc_zfp_decode_blocks(device.h, stream_buf.h, max_bits,
                                         ids_buf.h if ids_buf is not None else None, n, out.h))
+    return out
+
+
+def world_positions(device, verts_buf, n, dims, scale=(1, 1, 1), first=0):
+    """World-space float xyz (n x 3, on the device) of vertices first..first+n
+    of a block-local float vertex buffer, placed as the reference's renderer
+    places them: scale * ((4 * block_pos + local) / dims - 0.5)."""
+    out = Buffer.alloc(device, max(12 * n, 4))
+    sc = (ctypes.c_float * 3)(*[float(v) for v in scale])
+    _check(lib().bcmc_vertices_world(device.h, verts_buf.h, first, n, _u32arr(dims), sc, out.h))
     return out
 
 
