@@ -88,3 +88,41 @@ def test_sweep_benchmark_matches_oracle(tmp_path):
         assert perf["numActiveBlocks"][k] == ref.stats["num_active"], k
         assert perf["numBlocksDecompressed"][k] == ref.stats["num_decoded"], k
         assert perf["numBlocksWithVertices"][k] == ref.stats["num_with_verts"], k
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+def test_engine_from_node_matches_oracle(tmp_path):
+    """The batched and served engine reached from Node: every query's count
+    and block-local vertices equal the oracle's single computeSurface, and
+    the served trace gives the same counts."""
+    assert os.path.exists(ADDON), "N-API addon not built"
+    vols = [((32, 28, 24), 4, 31), ((20, 24, 36), 8, 32), ((13, 10, 7), 2, 33)]
+    streams = []
+    for i, (dims, rate, seed) in enumerate(vols):
+        f = synth.field(synth.BLOBS, seed, dims, 6, (3, 10, 0.5, 1))
+        streams.append(synth.encode(f, 64 * rate))
+        (tmp_path / f"vol_{i}.bin").write_bytes(streams[-1].tobytes())
+    rng = np.random.default_rng(41)
+    qv = rng.integers(0, len(vols), 9).tolist()
+    qi = rng.uniform(0.2, 0.8, 9).astype(np.float32).tolist()
+    arrivals = np.cumsum(rng.exponential(1e-4, 9)).tolist()
+    (tmp_path / "params.json").write_text(json.dumps({
+        "poolBytes": 50000 * 256, "vols": [{"dims": d, "rate": r} for d, r, _ in vols],
+        "qv": qv, "qi": qi, "arrivals": arrivals}))
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "engine_facade.js"), str(tmp_path)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads((tmp_path / "out.json").read_text())
+    verts = np.fromfile(tmp_path / "verts.bin", np.float32).reshape(-1, 4)
+    off = 0
+    for k in range(len(qv)):
+        dims, rate, _ = vols[qv[k]]
+        ref = oracle.MC(streams[qv[k]], 64 * rate, dims)
+        nv = ref.compute_surface(np.float32(qi[k]))
+        got = out["res"][k]
+        assert got["numVertices"] == nv and got["vertexOffset"] == off, k
+        want = ref.results()["verts_f32"]
+        assert np.array_equal(verts[off:off + nv].view(np.uint32), want.view(np.uint32)), k
+        assert out["served"][k]["numVertices"] == nv, k
+        off += nv
+    assert out["stats"]["numVolumes"] == 3 and out["stats"]["steps"] >= 2

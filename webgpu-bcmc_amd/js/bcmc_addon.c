@@ -10,6 +10,7 @@
 #define NAPI_VERSION 6
 #include <node_api.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "bcmc.h"
@@ -524,6 +525,158 @@ static napi_value mc_memory_use(napi_env env, napi_callback_info info)
     return o;
 }
 
+/* ---- batched / served engine ---------------------------------------------- */
+static napi_value eng_create(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(2);
+    EXT(0, bcmc_ctx, ctx);
+    bcmc_engine *e = NULL;
+    BCMC_CALL(env, bcmc_engine_create(ctx, (uint64_t)num(env, argv[1]), &e));
+    return mk_ext(env, e);
+}
+
+static napi_value eng_destroy(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_engine, eng);
+    BCMC_CALL(env, bcmc_engine_destroy(eng));
+    return NULL;
+}
+
+/* engAddVolume(eng, bytes, rate, dims[3]) -> volume id */
+static napi_value eng_add_volume(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(4);
+    EXT(0, bcmc_engine, eng);
+    void *d;
+    size_t n;
+    double dd[3];
+    if (bytes_of(env, argv[1], &d, &n) || three(env, argv[3], dd)) {
+        napi_throw_type_error(env, NULL, "expected (engine, Uint8Array, rate, dims[3])");
+        return NULL;
+    }
+    const uint32_t dims[3] = {(uint32_t)dd[0], (uint32_t)dd[1], (uint32_t)dd[2]};
+    uint32_t id = 0;
+    BCMC_CALL(env, bcmc_engine_add_volume(eng, d, n, num(env, argv[2]), dims, &id));
+    return mk_num(env, id);
+}
+
+/* engRun(eng, Uint32Array volumeIds, Float32Array isovalues) -> [{...}] */
+static napi_value eng_run(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(3);
+    EXT(0, bcmc_engine, eng);
+    void *ids, *isos;
+    size_t ni, nf;
+    if (bytes_of(env, argv[1], &ids, &ni) || bytes_of(env, argv[2], &isos, &nf) || ni / 4 != nf / 4) {
+        napi_throw_type_error(env, NULL, "expected (engine, Uint32Array, Float32Array) of one length");
+        return NULL;
+    }
+    const uint32_t nq = (uint32_t)(ni / 4);
+    bcmc_query_result *r = (bcmc_query_result *)calloc(nq ? nq : 1, sizeof(*r));
+    if (!r) {
+        napi_throw_error(env, NULL, "out of host memory");
+        return NULL;
+    }
+    const int rc = bcmc_engine_run(eng, (const uint32_t *)ids, (const float *)isos, nq, r);
+    if (rc != BCMC_OK) {
+        free(r);
+        BCMC_CALL(env, rc);
+    }
+    napi_value arr;
+    napi_create_array_with_length(env, nq, &arr);
+    for (uint32_t i = 0; i < nq; ++i) {
+        napi_value o;
+        napi_create_object(env, &o);
+        set_num(env, o, "numVertices", (double)r[i].num_vertices);
+        set_num(env, o, "vertexOffset", (double)r[i].vertex_offset);
+        set_num(env, o, "numActiveBlocks", r[i].num_active_blocks);
+        set_num(env, o, "stepDecodedBlocks", r[i].step_decoded_blocks);
+        napi_set_element(env, arr, i, o);
+    }
+    free(r);
+    return arr;
+}
+
+static napi_value eng_vertex_buffer(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_engine, eng);
+    return mk_ext(env, bcmc_engine_vertex_buffer(eng));
+}
+
+static napi_value eng_set_format(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(2);
+    EXT(0, bcmc_engine, eng);
+    BCMC_CALL(env, bcmc_engine_set_vertex_format(eng, (bcmc_vertex_format)(int)num(env, argv[1])));
+    return NULL;
+}
+
+static napi_value eng_stats(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(1);
+    EXT(0, bcmc_engine, eng);
+    bcmc_engine_info s;
+    BCMC_CALL(env, bcmc_engine_stats(eng, &s));
+    napi_value o;
+    napi_create_object(env, &o);
+    set_num(env, o, "numVolumes", s.num_volumes);
+    set_num(env, o, "poolSlots", s.pool_slots);
+    set_num(env, o, "poolUsed", s.pool_used);
+    set_num(env, o, "epoch", s.epoch);
+    set_num(env, o, "poolBytes", (double)s.pool_bytes);
+    set_num(env, o, "deviceBytes", (double)s.device_bytes);
+    set_num(env, o, "steps", s.steps);
+    set_num(env, o, "graphSteps", s.graph_steps);
+    return o;
+}
+
+/* engServe(eng, Float64Array arrivals, Uint32Array ids, Float32Array isos,
+ * maxBatch, vertexBudgetBytes) -> [{arrival, start, finish, numVertices,
+ * batchSize, step}] (seconds from the call) */
+static napi_value eng_serve(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(6);
+    EXT(0, bcmc_engine, eng);
+    void *arr, *ids, *isos;
+    size_t na, ni, nf;
+    if (bytes_of(env, argv[1], &arr, &na) || bytes_of(env, argv[2], &ids, &ni) ||
+        bytes_of(env, argv[3], &isos, &nf) || na / 8 != ni / 4 || ni / 4 != nf / 4) {
+        napi_throw_type_error(env, NULL,
+                              "expected (engine, Float64Array, Uint32Array, Float32Array) of one length");
+        return NULL;
+    }
+    const uint32_t n = (uint32_t)(ni / 4);
+    bcmc_serve_opts opts = {(uint32_t)num(env, argv[4]), (uint64_t)num(env, argv[5])};
+    bcmc_serve_result *r = (bcmc_serve_result *)calloc(n ? n : 1, sizeof(*r));
+    if (!r) {
+        napi_throw_error(env, NULL, "out of host memory");
+        return NULL;
+    }
+    const int rc = bcmc_serve_trace(eng, (const double *)arr, (const uint32_t *)ids, (const float *)isos, n,
+                                    &opts, r);
+    if (rc != BCMC_OK) {
+        free(r);
+        BCMC_CALL(env, rc);
+    }
+    napi_value out;
+    napi_create_array_with_length(env, n, &out);
+    for (uint32_t i = 0; i < n; ++i) {
+        napi_value o;
+        napi_create_object(env, &o);
+        set_num(env, o, "arrival", r[i].arrival_s);
+        set_num(env, o, "start", r[i].start_s);
+        set_num(env, o, "finish", r[i].finish_s);
+        set_num(env, o, "numVertices", (double)r[i].num_vertices);
+        set_num(env, o, "batchSize", r[i].batch_size);
+        set_num(env, o, "step", r[i].step);
+        napi_set_element(env, out, i, o);
+    }
+    free(r);
+    return out;
+}
+
 static napi_value version(napi_env env, napi_callback_info info)
 {
     napi_value s;
@@ -551,6 +704,9 @@ static napi_value init(napi_env env, napi_value exports)
         FN("mcSetFormat", mc_set_format), FN("mcComputeSurface", mc_compute_surface),
         FN("mcInfo", mc_info), FN("mcVertexBuffer", mc_vertex_buffer), FN("mcLru", mc_lru),
         FN("mcArray", mc_array), FN("mcMemoryUse", mc_memory_use),
+        FN("engCreate", eng_create), FN("engDestroy", eng_destroy), FN("engAddVolume", eng_add_volume),
+        FN("engRun", eng_run), FN("engVertexBuffer", eng_vertex_buffer), FN("engSetFormat", eng_set_format),
+        FN("engStats", eng_stats), FN("engServe", eng_serve),
     };
     napi_define_properties(env, exports, sizeof(d) / sizeof(d[0]), d);
     return exports;

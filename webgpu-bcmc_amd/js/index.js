@@ -279,6 +279,45 @@ class CompressedMarchingCubes {
     }
 }
 
+// Batched and served forms (no reference counterpart: the reference answers
+// one isovalue per computeSurface).  Several compressed volumes and one
+// decoded-block pool stay resident; run() answers many (volume, isovalue)
+// queries in one step, serve() replays timestamped requests with continuous
+// batching.  Each query's vertices equal a single computeSurface's.
+class Engine {
+    constructor(device, poolBytes) {
+        this.device = device;
+        this.handle = native.engCreate(device.handle, poolBytes);
+    }
+    addVolume(volume, compressionRate, volumeDims) {
+        return native.engAddVolume(this.handle, volume, compressionRate, volumeDims);
+    }
+    setVertexFormat(fmt) {
+        native.engSetFormat(this.handle, fmt);
+    }
+    // resolves to [{numVertices, vertexOffset, numActiveBlocks, stepDecodedBlocks}]
+    async run(volumeIds, isovalues) {
+        return native.engRun(this.handle, Uint32Array.from(volumeIds), Float32Array.from(isovalues));
+    }
+    // the last step's vertices; query i's run from results[i].vertexOffset
+    get vertexBuffer() {
+        return new DeviceBuffer(this.device, native.engVertexBuffer(this.handle), false);
+    }
+    stats() {
+        return native.engStats(this.handle);
+    }
+    // arrivals in seconds from the call; resolves to per-request
+    // {arrival, start, finish, numVertices, batchSize, step}
+    async serve(arrivals, volumeIds, isovalues, {maxBatch = 0, vertexBudgetBytes = 0} = {}) {
+        return native.engServe(this.handle, Float64Array.from(arrivals), Uint32Array.from(volumeIds),
+                               Float32Array.from(isovalues), maxBatch, vertexBudgetBytes);
+    }
+    destroy() {
+        if (this.handle) native.engDestroy(this.handle);
+        this.handle = null;
+    }
+}
+
 module.exports = {
     requestDevice,
     Device,
@@ -290,6 +329,7 @@ module.exports = {
     RadixSorter,
     LRUCache,
     CompressedMarchingCubes,
+    Engine,
     VertexFormat,
     McArray,
     version: native.version,
