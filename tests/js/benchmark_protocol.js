@@ -1,5 +1,5 @@
 // Drives the isovalue benchmarks.  argv[2]: output directory.  argv[3]:
-// 'mock' runs the driver on a stand-in pipeline (no GPU); 'gpu' runs a
+// 'mock' runs the benchmark loop on a stand-in pipeline (no GPU); 'gpu' runs a
 // sweepUp on a real CompressedMarchingCubes over <dir>/stream.bin.
 'use strict';
 const fs = require('fs');

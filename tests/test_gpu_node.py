@@ -106,9 +106,13 @@ def test_engine_from_node_matches_oracle(tmp_path):
     qv = rng.integers(0, len(vols), 9).tolist()
     qi = rng.uniform(0.2, 0.8, 9).astype(np.float32).tolist()
     arrivals = np.cumsum(rng.exponential(1e-4, 9)).tolist()
+    fdims, frate = (21, 18, 11), 1.5
+    fld = synth.field(synth.GAUSS, 34, fdims, 4, (3, 10, 0.5, 1))
+    (tmp_path / "field.bin").write_bytes(fld.tobytes())
+    scale = [1.0, 0.5, 2.0]
     (tmp_path / "params.json").write_text(json.dumps({
         "poolBytes": 50000 * 256, "vols": [{"dims": d, "rate": r} for d, r, _ in vols],
-        "qv": qv, "qi": qi, "arrivals": arrivals}))
+        "qv": qv, "qi": qi, "arrivals": arrivals, "scale": scale, "fieldDims": fdims, "fieldRate": frate}))
     r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "engine_facade.js"), str(tmp_path)],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -126,3 +130,18 @@ def test_engine_from_node_matches_oracle(tmp_path):
         assert out["served"][k]["numVertices"] == nv, k
         off += nv
     assert out["stats"]["numVolumes"] == 3 and out["stats"]["steps"] >= 2
+    # world positions of query 0 (the renderer's placement, float32 steps)
+    dims0 = vols[qv[0]][0]
+    n0 = out["res"][0]["numVertices"]
+    loc = verts[:n0]
+    bid = loc[:, 3].view(np.uint32)
+    nbx, nby = (dims0[0] + 3) // 4, (dims0[1] + 3) // 4
+    bpos = np.stack([bid % nbx, (bid // nbx) % nby, bid // nbx // nby], 1)
+    want = np.float32(scale) * (((4 * bpos).astype(np.float32) + loc[:, :3]) / np.float32(dims0)
+                                - np.float32(0.5))
+    world = np.fromfile(tmp_path / "world.bin", np.float32).reshape(-1, 3)
+    assert np.array_equal(world.view(np.uint32), want.view(np.uint32))
+    # the GPU encoder from Node: the host encoder's bytes, then two zero words
+    enc = np.fromfile(tmp_path / "encoded.bin", np.uint8)
+    host = synth.encode(fld, int(64 * frate))
+    assert enc[:len(host)].tobytes() == host.tobytes() and not enc[len(host):].any()

@@ -677,6 +677,42 @@ static napi_value eng_serve(napi_env env, napi_callback_info info)
     return out;
 }
 
+/* zfpEncode(ctx, fieldBuf, dims[3], maxBits, streamBuf) */
+static napi_value zfp_encode(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(5);
+    EXT(0, bcmc_ctx, ctx);
+    EXT(1, bcmc_buf, field);
+    EXT(4, bcmc_buf, stream);
+    double dd[3];
+    if (three(env, argv[2], dd)) {
+        napi_throw_type_error(env, NULL, "expected dims[3]");
+        return NULL;
+    }
+    const uint32_t dims[3] = {(uint32_t)dd[0], (uint32_t)dd[1], (uint32_t)dd[2]};
+    BCMC_CALL(env, bcmc_zfp_encode(ctx, field, dims, (uint32_t)num(env, argv[3]), stream));
+    return NULL;
+}
+
+/* verticesWorld(ctx, vertsBuf, first, n, dims[3], scale[3], outBuf) */
+static napi_value vertices_world(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(7);
+    EXT(0, bcmc_ctx, ctx);
+    EXT(1, bcmc_buf, verts);
+    EXT(6, bcmc_buf, out);
+    double dd[3], sc[3];
+    if (three(env, argv[4], dd) || three(env, argv[5], sc)) {
+        napi_throw_type_error(env, NULL, "expected dims[3] and scale[3]");
+        return NULL;
+    }
+    const uint32_t dims[3] = {(uint32_t)dd[0], (uint32_t)dd[1], (uint32_t)dd[2]};
+    const float scale[3] = {(float)sc[0], (float)sc[1], (float)sc[2]};
+    BCMC_CALL(env, bcmc_vertices_world(ctx, verts, (uint64_t)num(env, argv[2]), (uint64_t)num(env, argv[3]),
+                                       dims, scale, out));
+    return NULL;
+}
+
 static napi_value version(napi_env env, napi_callback_info info)
 {
     napi_value s;
@@ -707,6 +743,7 @@ static napi_value init(napi_env env, napi_value exports)
         FN("engCreate", eng_create), FN("engDestroy", eng_destroy), FN("engAddVolume", eng_add_volume),
         FN("engRun", eng_run), FN("engVertexBuffer", eng_vertex_buffer), FN("engSetFormat", eng_set_format),
         FN("engStats", eng_stats), FN("engServe", eng_serve),
+        FN("zfpEncode", zfp_encode), FN("verticesWorld", vertices_world),
     };
     napi_define_properties(env, exports, sizeof(d) / sizeof(d[0]), d);
     return exports;

@@ -3,7 +3,7 @@
 // loop that drives them (js/bcmc.js:319-370), without the page.  The
 // isovalue "slider" is any object with a numeric `value` (the page's range
 // input snaps values to its step; this one does not).  Each benchmark sets
-// 100 isovalues; the driver resets the LRU cache first, recomputes the surface
+// 100 isovalues; the loop resets the LRU cache first, recomputes the surface
 // whenever the value changed, and collects the perfTracker the page saves as
 // `bcmc-explicit-<dataset>-<benchmark>.json`.
 

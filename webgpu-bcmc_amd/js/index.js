@@ -318,6 +318,27 @@ class Engine {
     }
 }
 
+// Fixed-rate zfp encoding on the GPU (no reference counterpart: the reference
+// loads streams compressed offline).  field: DeviceBuffer of float32, x
+// fastest; resolves to a DeviceBuffer with the stream plus two zero words.
+async function zfpEncode(device, field, dims, compressionRate) {
+    const maxBits = Math.round(64 * compressionRate);
+    const nb = Math.ceil(dims[0] / 4) * Math.ceil(dims[1] / 4) * Math.ceil(dims[2] / 4);
+    const bytes = Math.ceil((nb * maxBits) / 64) * 8;
+    const out = device.createBuffer({size: bytes + 16});
+    native.zfpEncode(device.handle, field.handle, dims, maxBits, out.handle);
+    return out;
+}
+
+// World-space float xyz of vertices [first, first + n) of a block-local
+// vertex buffer, placed as the reference's vertex shader places them
+// (mc_isosurface.vert:51-55); resolves to a DeviceBuffer of 12 n bytes.
+async function worldPositions(device, vertexBuffer, n, dims, scale = [1, 1, 1], first = 0) {
+    const out = device.createBuffer({size: Math.max(12 * n, 4)});
+    native.verticesWorld(device.handle, vertexBuffer.handle, first, n, dims, scale, out.handle);
+    return out;
+}
+
 module.exports = {
     requestDevice,
     Device,
@@ -330,6 +351,8 @@ module.exports = {
     LRUCache,
     CompressedMarchingCubes,
     Engine,
+    zfpEncode,
+    worldPositions,
     VertexFormat,
     McArray,
     version: native.version,
