@@ -10,7 +10,7 @@
  *   - 64-bit stream bit offsets (zfp_decompress.comp:199 multiplies in u32);
  *   - total block count nbx*nby*nbz (zfp_compute_block_range.comp:22 computes
  *     (px*py)/64*pz, which truncates when px*py is not a multiple of 64);
- *   - LRU `needed` entries past N read as 0 (lru_cache_mark_new_items.comp:86
+ *   - LRU `needed` entries past N read as 0 (lru_cache_mark_new_items.comp:32,42
  *     reads past the end of the flags buffer);
  *   - vertex totals are 64-bit (exclusive_scan.js:291 reads back a u32).
  */
@@ -335,7 +335,7 @@ static void sort_chunk(uint32_t *k, uint32_t *v)
     }
 }
 
-/* upper_bound / lower_bound of merge_sorted_chunks.comp:137-163. */
+/* upper_bound / lower_bound of merge_sorted_chunks.comp:35-61. */
 static uint32_t ub(const uint32_t *keys, uint32_t start, uint32_t count, uint32_t e)
 {
     while (count > 0) {
@@ -366,11 +366,11 @@ static uint32_t lb(const uint32_t *keys, uint32_t start, uint32_t count, uint32_
 
 /* RadixSorter.sort, radix_sort_by_key.js:161-400: pad to the aligned size
  * with UINT_MAX (radix_sort_chunk.comp:32-39), sort 64-element chunks, merge
- * pairs of runs log2(chunks) times (merge_sorted_chunks.comp:165-196: an A
+ * pairs of runs log2(chunks) times (merge_sorted_chunks.comp:63-94: an A
  * element lands after every B element with key <= its own, a B element after
  * every A element with key < its own, so equal keys from B precede those from
  * A), optionally reverse the values over the aligned size
- * (reverse_buffer.comp:212-226), then copy n values from offset
+ * (reverse_buffer.comp:15-29), then copy n values from offset
  * aligned-n (reverse) or 0 (radix_sort_by_key.js:382-391).  The key buffer's
  * final contents are unspecified by the reference; here it receives the
  * ascending keys. */
@@ -476,10 +476,10 @@ void orc_lru_destroy(orc_lru *c)
 /* LRUCache.update, lru_cache.js:312-843. */
 uint32_t orc_lru_update(orc_lru *c, const uint32_t *needed, uint32_t *new_items)
 {
-    /* lru_cache_age_slots.comp:45-52 */
+    /* lru_cache_age_slots.comp:14-21 */
     for (uint32_t s = 0; s < c->size; ++s)
         c->age[s] += 1;
-    /* lru_cache_mark_new_items.comp:80-98 over alignTo(N,32) items; entries
+    /* lru_cache_mark_new_items.comp:26-44 over alignTo(N,32) items; entries
      * past N count as not needed. */
     for (uint32_t i = 0; i < c->aligned_total; ++i) {
         const uint32_t nd = i < c->total ? needed[i] : 0;
@@ -547,7 +547,7 @@ uint32_t orc_lru_update(orc_lru *c, const uint32_t *needed, uint32_t *new_items)
             keys[j] = c->age[ids[j]];
     }
     orc_sort_by_key(keys, ids, num_avail, 1);
-    /* lru_cache_update.comp:173-191 */
+    /* lru_cache_update.comp:19-37 */
     for (uint32_t w = 0; w < num_new; ++w) {
         const uint32_t it = new_items[w], s = ids[w];
         const int32_t prev = c->item[s];
@@ -563,7 +563,7 @@ uint32_t orc_lru_update(orc_lru *c, const uint32_t *needed, uint32_t *new_items)
     return num_new;
 }
 
-/* LRUCache.reset, lru_cache.js:846-903 with lru_cache_init.comp:211-222. */
+/* LRUCache.reset, lru_cache.js:846-903 with lru_cache_init.comp:19-30. */
 void orc_lru_reset(orc_lru *c)
 {
     for (uint32_t s = 0; s < c->size; ++s) {
@@ -829,7 +829,7 @@ int orc_mc_compute_surface(orc_mc *m, float iso, orc_mc_stats *st)
     s.cache_size = orc_lru_size(m->lru);
     double t2 = now_s();
     s.t_lru = t2 - t1;
-    /* decompressBlocks (:831-913; zfp_decompress_block.comp:92-102) */
+    /* decompressBlocks (:831-913; zfp_decompress_block.comp:29-39) */
     {
         float *cache = (float *)orc_lru_data(m->lru);
         const int32_t *slot = m->lru->item_slot;

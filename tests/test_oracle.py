@@ -373,3 +373,201 @@ def test_mc_no_active_skips_cache_update():
     age0 = mc.lru_state()[0].copy()
     assert mc.compute_surface(2.0) == 0 and mc.stats["num_active"] == 0
     assert np.array_equal(mc.lru_state()[0], age0)
+
+
+# ---------------------------------------------------------------- LRU tie order
+def _post_reset_slot_order(avail_slots, ages):
+    """Order in which a post-reset update hands out slots, derived from the
+    sort the reference runs on (age, available slot id) pairs: 64-element
+    chunks sorted stably (radix_sort_chunk.comp:41-95), pairs of runs merged
+    with equal keys of the later run placed first (merge_sorted_chunks.comp:
+    82-86), values reversed (reverse_buffer.comp:15-29) and read from the end
+    (radix_sort_by_key.js:382-391).  So: ages descending, then the sort-input
+    chunk ascending, then the input position descending inside a chunk.  The
+    sort input is the ascending list of available slots
+    (lru_cache.js:684-717)."""
+    pos = np.arange(len(avail_slots))
+    key = sorted(pos, key=lambda j: (-int(ages[j]), j // 64, -j))
+    return [int(avail_slots[j]) for j in key]
+
+
+def _fresh_after_reset(total, initial, seed):
+    rng = np.random.default_rng(seed)
+    lru = oracle.LRU(initial, 256, total)
+    # a few updates that fit the cache, so the state before the reset is not
+    # the constructor's but the size is unchanged
+    for _ in range(3):
+        needed = np.zeros(total, np.uint32)
+        needed[rng.permutation(total)[:initial // 2]] = 1
+        lru.update(needed)
+    lru.reset()
+    return lru, rng
+
+
+@pytest.mark.parametrize("initial,d", [(224, 150), (224, 224), (256, 1), (256, 200),
+                                       (96, 40), (32, 32), (480, 479)])
+def test_lru_after_reset_misses_take_slots_63_down_then_127_down(initial, d):
+    """Closed form of the tie order: after reset() every slot has the same
+    age, so D misses (ascending item ids) take slots 63..0, then 127..64, ...,
+    with the last, partial chunk starting at its highest real slot."""
+    total = 4096
+    lru, rng = _fresh_after_reset(total, initial, initial + d)
+    c = lru.size
+    assert c == initial and d <= c
+    items = np.sort(rng.permutation(total)[:d])
+    needed = np.zeros(total, np.uint32)
+    needed[items] = 1
+    n, new_items = lru.update(needed)
+    assert n == d and np.array_equal(new_items, items)
+    want = []
+    for ch in range(0, c, 64):
+        want.extend(range(min(ch + 64, c) - 1, ch - 1, -1))
+    _, _, item, slot = lru.state()
+    assert list(slot[items]) == want[:d]
+    assert all(item[s] == it for s, it in zip(want[:d], items))
+
+
+def test_lru_after_reset_growth_hands_out_old_slots_before_new():
+    """D > C after a reset: the cache grows by the reference's rule
+    (lru_cache.js:518-522); the old slots (age 100001 after aging) come first
+    in the chunk order above, then the new slots (age 100000), whose chunks
+    are counted in the same available-slot list."""
+    total, initial, d = 4096, 224, 300
+    lru, rng = _fresh_after_reset(total, initial, 9)
+    items = np.sort(rng.permutation(total)[:d])
+    needed = np.zeros(total, np.uint32)
+    needed[items] = 1
+    n, _ = lru.update(needed)
+    assert n == d
+    grown = (min(initial + -(-(d - initial) * 3 // 2), total) + 31) // 32 * 32
+    assert lru.size == grown == 352
+    # old slots 192..223 share chunk 3 with new slots 224..255
+    want = (list(range(63, -1, -1)) + list(range(127, 63, -1)) + list(range(191, 127, -1)) +
+            list(range(223, 191, -1)) + list(range(255, 223, -1)) + list(range(319, 255, -1)))
+    _, _, _, slot = lru.state()
+    assert list(slot[items]) == want[:d]
+    ages = np.array([100001] * initial + [100000] * (grown - initial))
+    assert want[:d] == _post_reset_slot_order(np.arange(grown), ages)[:d]
+
+
+def test_lru_second_update_after_reset_follows_the_tie_order():
+    """A second update after the reset: the slots taken by the first update
+    and not needed again become available with age 1, the untouched slots
+    keep age 100002, so the untouched slots go first (chunk order over the
+    available list, which no longer holds every slot), then the released
+    ones."""
+    total, initial = 2048, 256
+    lru, rng = _fresh_after_reset(total, initial, 21)
+    first = np.sort(rng.permutation(total)[:100])
+    needed = np.zeros(total, np.uint32)
+    needed[first] = 1
+    lru.update(needed)
+    keep = first[::3]
+    fresh = np.sort(np.setdiff1d(rng.permutation(total)[:250], first))[:180]
+    needed = np.zeros(total, np.uint32)
+    needed[keep] = 1
+    needed[fresh] = 1
+    age_b, _, item_b, slot_b = lru.state()
+    kept_slots = set(int(slot_b[i]) for i in keep)
+    avail = np.array([s for s in range(lru.size) if s not in kept_slots])
+    # ages as the sort sees them: one aging step after the state read above
+    ages = age_b[avail].astype(np.int64) + 1
+    want = _post_reset_slot_order(avail, ages)
+    n, new_items = lru.update(needed)
+    assert n == len(fresh) and np.array_equal(new_items, fresh)
+    _, _, _, slot = lru.state()
+    assert list(slot[fresh]) == want[:len(fresh)]
+    assert all(slot[i] == slot_b[i] for i in keep)
+
+
+# ---------------------------------------------------------------- MC vertex order
+# Cube corners and edge endpoints of the classic marching-cubes convention the
+# reference uses (compute_vertex_values.comp:48-57, compute_block_vertices.comp:
+# 10-23; its edge 5 runs from corner 6 to corner 5).
+_CORNERS = np.array([(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0),
+                     (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)])
+_EDGES = [(0, 1), (1, 2), (2, 3), (3, 0), (4, 5), (6, 5), (6, 7), (7, 4),
+          (0, 4), (1, 5), (2, 6), (3, 7)]
+
+
+def _bruteforce_vertex_sequence(vol, dims, iso):
+    """Every vertex of a full-volume marching cubes, listed block by block
+    (ascending block id), cells x-fastest inside a block, vertices in
+    triangle-table order inside a cell (compute_block_vertices.comp:141-173),
+    with block-local FP64 positions: corner a + t (corner b - corner a) +
+    cell + 0.5, t = (iso - fa) / (fb - fa), or 0 where |fa - fb| < 0.001 in
+    FP32 (lerp_verts, compute_block_vertices.comp:62-74).  dims = 0 mod 4, so
+    every cell whose 8 corners lie in the volume is emitted by the block that
+    holds its lowest corner."""
+    table = oracle.tri_table_ints()
+    iso32 = np.float32(iso)
+    nb = [d // 4 for d in dims]
+    out_pos, out_bid, flat = [], [], 0
+    # corner values of every cell, shape (z, y, x, 8)
+    nz, ny, nx = dims[2] - 1, dims[1] - 1, dims[0] - 1
+    cv = np.stack([vol[c[2]:c[2] + nz, c[1]:c[1] + ny, c[0]:c[0] + nx] for c in _CORNERS], -1)
+    case = np.zeros((nz, ny, nx), np.int64)
+    for i in range(8):
+        case |= (cv[..., i] <= iso32).astype(np.int64) << i
+    for bz in range(nb[2]):
+        for by in range(nb[1]):
+            for bx in range(nb[0]):
+                bid = bx + nb[0] * (by + nb[1] * bz)
+                for lz in range(4):
+                    for ly in range(4):
+                        for lx in range(4):
+                            x, y, z = 4 * bx + lx, 4 * by + ly, 4 * bz + lz
+                            if x >= nx or y >= ny or z >= nz:
+                                continue
+                            c = int(case[z, y, x])
+                            if c == 0 or c == 255:
+                                continue
+                            vals = cv[z, y, x]
+                            for k in range(16):
+                                e = table[16 * c + k]
+                                if e < 0:
+                                    break
+                                a, b = _EDGES[e]
+                                fa, fb = vals[a], vals[b]
+                                if np.abs(fa - fb) < np.float32(0.001):
+                                    t = 0.0
+                                    flat += 1
+                                else:
+                                    t = (float(iso32) - float(fa)) / (float(fb) - float(fa))
+                                p = _CORNERS[a] + t * (_CORNERS[b] - _CORNERS[a])
+                                out_pos.append(p + (lx + 0.5, ly + 0.5, lz + 0.5))
+                                out_bid.append(bid)
+    return np.array(out_pos).reshape(-1, 3), np.array(out_bid, np.uint32), flat
+
+
+@pytest.mark.parametrize("dims,rate,iso,seed", [((32, 24, 20), 4, 0.4, 12),
+                                                ((16, 16, 16), 8, 0.55, 3),
+                                                ((24, 16, 12), 2, 0.35, 8)])
+def test_mc_vertex_sequence_equals_bruteforce(dims, rate, iso, seed):
+    _, s = _mc_case(dims, synth.GAUSS, seed, rate, 8, (3, 10, 0.5, 1))
+    mc = oracle.MC(s, 64 * rate, dims, initial_cache=32)
+    v = mc.compute_surface(iso)
+    res = mc.results()
+    vol = oracle.decode_volume(s, 64 * rate, dims)
+    pos, bid, _ = _bruteforce_vertex_sequence(vol, dims, iso)
+    assert v == len(pos) and v > 0
+    assert np.array_equal(res["verts_f32"][:, 3].view(np.uint32), bid)
+    assert np.max(np.abs(res["verts_f64"] - pos)) <= 1e-12
+
+
+def test_mc_vertex_sequence_on_flat_edges():
+    """A field of small amplitude (0.5 + 0.01 g) at rate 16: many crossed
+    edges are flat (|fa - fb| < 0.001), where the vertex sits on the edge's
+    first corner, so the edge orientation of the table is pinned too."""
+    dims, rate, iso = (16, 16, 20), 16, 0.505
+    g = synth.field(synth.GAUSS, 4, dims, 6, (3, 8, 0.5, 1))
+    f = (np.float32(0.5) + np.float32(0.01) * g).astype(np.float32)
+    s = synth.encode(f, 64 * rate)
+    mc = oracle.MC(s, 64 * rate, dims, initial_cache=32)
+    v = mc.compute_surface(iso)
+    res = mc.results()
+    vol = oracle.decode_volume(s, 64 * rate, dims)
+    pos, bid, flat = _bruteforce_vertex_sequence(vol, dims, iso)
+    assert v == len(pos) and flat > 100, (v, flat)
+    assert np.array_equal(res["verts_f32"][:, 3].view(np.uint32), bid)
+    assert np.max(np.abs(res["verts_f64"] - pos)) <= 1e-12

@@ -72,8 +72,8 @@ struct MissAllOp {
     BCMC_DEV void total(const unsigned long long (&t)[1]) const { q->num_new = (uint32_t)t[0]; }
 };
 
-// Age + mark over the C slots (lru_cache_age_slots.comp:45-52,
-// lru_cache_mark_new_items.comp:80-98) fused with the extraction of the
+// Age + mark over the C slots (lru_cache_age_slots.comp:14-21,
+// lru_cache_mark_new_items.comp:26-44) fused with the extraction of the
 // available slots (lru_cache.js:501, :684-710): a slot ages by one; a slot
 // whose item is needed becomes {age 0, unavailable}, a slot whose item is
 // cached but not needed becomes available (the item->slot map is a bijection
@@ -165,7 +165,7 @@ __global__ void lru_grow_kernel(LruView v, QState *q, const uint32_t *gate, uint
     }
 }
 
-// lru_cache_update.comp:173-191; thread 0 also commits the new size.
+// lru_cache_update.comp:19-37; thread 0 also commits the new size.
 __global__ void lru_assign_kernel(LruView v, const uint32_t *new_items, QState *q) {
     const uint32_t n = q->num_avail_total ? q->num_new : 0u;
     for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) {

@@ -77,7 +77,7 @@ struct McTables {
     EdgeInfo edge[12];
 };
 
-// compute_block_dims_with_ghost (compute_vertex_values.comp:96-170): the
+// compute_block_dims_with_ghost (compute_vertex_values.comp:96-156): the
 // ghost planes are part of the tile only where the +x/+y/+z neighbours (and,
 // for the shared edges and corner, the diagonal neighbours) are active.
 // idx bits 0..6: neighbour +x, +y, +z, +xy, +xz, +yz, +xyz active; bits 7..9:

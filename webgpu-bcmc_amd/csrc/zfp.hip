@@ -302,7 +302,7 @@ __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
                 dst = (uint64_t)(uint32_t)slot * 64;
             }
             if constexpr (MODE == kModeLruAssign) {
-                // lru_cache_update.comp:24-36: evict the slot's previous item,
+                // lru_cache_update.comp:25-36: evict the slot's previous item,
                 // install this one (new items are never cached ones, so no two
                 // threads write the same item entry)
                 const uint32_t s = a.order[i];
