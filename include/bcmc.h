@@ -82,6 +82,18 @@ BCMC_API uint32_t bcmc_scan_aligned_size(uint32_t n);
 BCMC_API int bcmc_scan_exclusive_u32(bcmc_ctx *ctx, bcmc_buf *data, uint32_t aligned_n, uint32_t n,
                                      uint32_t *total);
 
+/* The same scan as bcmc_scan_exclusive_u32, allocating nothing: no reference
+ * counterpart.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its look-back state lives in a fixed 1 MiB + 256 B device array that the
+ * library holds on each device from bcmc_ctx_create until the process ends,
+ * so n is limited to 2^30 (BCMC_ERR_INVALID beyond).  All calls on one device
+ * share that array, whichever context issues them: calls that overlap on one
+ * device (two contexts or threads at once) race and must be serialised by the
+ * caller. */
+BCMC_API int bcmc_scan_exclusive_u32_noalloc(bcmc_ctx *ctx, bcmc_buf *data, uint32_t aligned_n,
+                                             uint32_t n, uint32_t *total);
+
 /* ---- StreamCompact / StreamCompactor (js/stream_compact.js:56-165) -------- */
 /* out[offsets[i]] = i for every i < n with flags[i] != 0 (stream_compact.comp). */
 BCMC_API int bcmc_compact_ids(bcmc_ctx *ctx, uint32_t n, bcmc_buf *flags, bcmc_buf *offsets,

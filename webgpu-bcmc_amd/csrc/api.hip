@@ -7,6 +7,8 @@
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 namespace bcmc {
 
@@ -194,11 +196,58 @@ __global__ void world_positions_kernel(const float4 *__restrict__ v, uint64_t n,
 
 }  // namespace bcmc
 
+// ---- look-back state of the scan that allocates nothing ---------------------
+// bcmc_scan_exclusive_u32_noalloc keeps its scan state here instead of in
+// memory the context allocates: bytes 0..11 the tile counter, the
+// finished-workgroup counter and the epoch (bcmc::ScanSite::ctrl), bytes
+// 16..23 the scan total, and from byte 256 one status word per 8192-element
+// tile.  Every device the library runs on holds its own copy for the life of
+// the process.  In host code `scratch` names a host-side stand-in, never the
+// device copy: host code uses the address hipGetSymbolAddress returns for the
+// current device (noalloc_setup), kept per device.
+constexpr uint32_t kNoAllocTiles = 131072;
+constexpr uint64_t kNoAllocMaxElems = (uint64_t)kNoAllocTiles * bcmc::kScanTile;  // 2^30
+constexpr size_t kNoAllocTotalOff = 16, kNoAllocStatusOff = 256;
+constexpr size_t kNoAllocBytes = kNoAllocStatusOff + (size_t)kNoAllocTiles * 8;  // 1 MiB + 256 B
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static void *g_noalloc_ptr[kNoAllocMaxDevices];
+
+// Once per device, with that device current (bcmc_ctx_create has just made it
+// so): look the device's copy of `scratch` up and zero it on the context's
+// stream, so the first scan sees clear counters, epoch 0 and unpublished
+// tiles.  Later scans leave the counters at zero themselves (the last
+// workgroup resets them) and tell stale status words by their epoch.
+static int noalloc_setup(bcmc_ctx *ctx) {
+    if (ctx->device < 0 || ctx->device >= kNoAllocMaxDevices)
+        return BCMC_OK;  // the no-allocation scan then reports BCMC_ERR_STATE
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[ctx->device])
+        return BCMC_OK;
+    void *p = nullptr;
+    BCMC_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    BCMC_CHECK_ARG(p != nullptr && ((uintptr_t)p & 255) == 0, "scratch symbol address");
+    const uint64_t words = kNoAllocBytes / 4;
+    bcmc::fill_u32_kernel<<<(uint32_t)bcmc::div_up(words, 256), 256, 0, ctx->stream>>>(
+        (uint32_t *)p, 0u, words);
+    BCMC_HIP_CHECK(hipGetLastError());
+    BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    g_noalloc_ptr[ctx->device] = p;
+    return BCMC_OK;
+}
+
+static void *noalloc_ptr(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices)
+        return nullptr;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    return g_noalloc_ptr[device];
+}
+
 using namespace bcmc;
 
 // Context-owned scratch, kept in a side table so bcmc_ctx stays a plain struct.
-#include <map>
-#include <mutex>
 static std::mutex g_scratch_mu;
 static std::map<bcmc_ctx *, CtxScratch *> g_scratch;
 
@@ -261,6 +310,15 @@ BCMC_API int bcmc_ctx_create(int device, bcmc_ctx **out) {
         return BCMC_ERR_HIP;
     }
     c->pinned_bytes = 4096;
+    const int rc = noalloc_setup(c);
+    if (rc != BCMC_OK) {
+        (void)hipEventDestroy(c->ev0);
+        (void)hipEventDestroy(c->ev1);
+        (void)hipHostFree(c->pinned);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
     *out = c;
     return BCMC_OK;
 }
@@ -430,6 +488,41 @@ BCMC_API int bcmc_scan_exclusive_u32(bcmc_ctx *ctx, bcmc_buf *data, uint32_t ali
     BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     if (total)
         *total = (uint32_t)t;
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_scan_exclusive_u32_noalloc(bcmc_ctx *ctx, bcmc_buf *data, uint32_t aligned_n,
+                                             uint32_t n, uint32_t *total) {
+    BCMC_CHECK_ARG(ctx && data, "null argument");
+    BCMC_CHECK_ARG(bcmc_scan_aligned_size(aligned_n) == aligned_n,
+                   "GPU input must be aligned to getAlignedSize");
+    if ((uint64_t)n > kNoAllocMaxElems) {
+        set_error("invalid argument: n = %u exceeds the no-allocation scan's maximum of %llu "
+                  "elements", n, (unsigned long long)kNoAllocMaxElems);
+        return BCMC_ERR_INVALID;
+    }
+    BCMC_CHECK_ARG(n <= aligned_n && (uint64_t)aligned_n * 4 <= data->bytes,
+                   "scan size exceeds the buffer");
+    void *p = noalloc_ptr(ctx->device);
+    if (!p) {
+        set_error("no scan scratch on device %d", ctx->device);
+        return BCMC_ERR_STATE;
+    }
+    ScanSite site;
+    site.ctrl = (uint32_t *)p;
+    site.status = (unsigned long long *)((char *)p + kNoAllocStatusOff);
+    site.max_tiles = kNoAllocTiles;
+    unsigned long long *tot = (unsigned long long *)((char *)p + kNoAllocTotalOff);
+    InplaceScanOp op{(uint32_t *)data->ptr, n, tot};
+    launch_scan<1>(ctx, site, site, op, n);
+    if (n < aligned_n)
+        write_total_kernel<<<1, 1, 0, ctx->stream>>>((uint32_t *)data->ptr, n, tot);
+    BCMC_HIP_CHECK(hipGetLastError());
+    // the total comes back through the context's pinned staging words
+    BCMC_HIP_CHECK(hipMemcpyAsync(ctx->pinned, tot, 8, hipMemcpyDeviceToHost, ctx->stream));
+    BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (total)
+        *total = (uint32_t)*(const unsigned long long *)ctx->pinned;
     return BCMC_OK;
 }
 

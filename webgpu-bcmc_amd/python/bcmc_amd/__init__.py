@@ -136,6 +136,7 @@ def lib():
         "bcmc_buf_device_ptr": (vp, [vp]),
         "bcmc_scan_aligned_size": (u32, [u32]),
         "bcmc_scan_exclusive_u32": (i32, [vp, vp, u32, u32, P(u32)]),
+        "bcmc_scan_exclusive_u32_noalloc": (i32, [vp, vp, u32, u32, P(u32)]),
         "bcmc_compact_ids": (i32, [vp, u32, vp, vp, vp]),
         "bcmc_compact_flags": (i32, [vp, u32, vp, vp, P(u32)]),
         "bcmc_sort_aligned_size": (u32, [u32]),
