@@ -51,6 +51,25 @@ def profiled_traffic(workload, stage):
     return (int(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
 
 
+def host_cpu():
+    """The host's CPU model and the threads the CPU baseline uses: the CPUs
+    this process may run on (nproc), capped by OMP_NUM_THREADS when set."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(nproc, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else nproc
+    return {"model": model, "nproc": nproc, "cpu_count": os.cpu_count(),
+            "omp_num_threads": omp, "threads": threads}
+
+
 def stage_bytes(s, rate, n_blocks):
     """Algorithmic HBM bytes per stage of one query: what each stage must read
     and write at least once (ids, ranges, slot words, block data, ghost voxels,
@@ -81,24 +100,6 @@ SERVED_SIZES = [(128, 128, 128), (192, 192, 192), (256, 256, 256), (320, 320, 32
 SERVED_RATES = [2, 4, 8, 2, 4, 8, 2, 2]   # cycling 2/4/8; the 1024x1024x1080 volume at 2
 BATCH_SIZES = [(128, 128, 128), (256, 256, 256), (384, 384, 384), (512, 512, 512)]
 BATCH_RATES = [8, 4, 4, 2]
-
-
-def make_trace(rng, n, nvol, repeat=0.5, window=16):
-    """Volume ids uniform over the volumes; isovalues U[0.1, 0.9], of which a
-    fraction `repeat` re-use one of the last `window` isovalues of the same volume."""
-    vols = rng.integers(0, nvol, n)
-    hist = [[] for _ in range(nvol)]
-    isos = np.empty(n, np.float32)
-    for i, v in enumerate(vols):
-        h = hist[v]
-        if h and rng.random() < repeat:
-            iso = h[int(rng.integers(0, len(h)))]
-        else:
-            iso = np.float32(rng.uniform(0.1, 0.9))
-        isos[i] = iso
-        h.append(iso)
-        del h[:-window]
-    return vols.astype(np.uint32), isos
 
 
 def plain_replay(bcmc_amd, dev, vols, ids, isos, cache_frac=0.25):
@@ -132,7 +133,7 @@ def served_bench(bcmc_amd, synth, dev, args):
         vols.append((s, rate, dims))
     log(f"bench: served volumes generated in {time.time() - t0:.1f}s")
     rng = np.random.default_rng(20240607)
-    ids, isos = make_trace(rng, args.served_queries, len(vols))
+    ids, isos = bcmc_amd.request_trace(rng, args.served_queries, len(vols))
     t_plain, nv_plain = plain_replay(bcmc_amd, dev, vols, ids, isos)
     plain_qps = len(ids) / t_plain.sum()
     log(f"bench: plain replay {len(ids)} queries: {plain_qps:.1f} q/s")
@@ -409,10 +410,30 @@ def main():
                                    "GBps": round(2 * nb / (cms * 1e-3) / 1e9, 1)}
         del src, dst
 
+    # the 256^3 16-Gaussian field at rate 8 with every block cached, one
+    # isovalue 0.3 after an LRU reset (the CPU reference configuration), on the GPU
+    gw = synth.WORKLOADS["gauss256_r8"]
+    gstream, _ = synth.make_stream(gw)
+    g_nb = int(np.prod([(d + 3) // 4 for d in gw["dims"]]))
+    gmc = bcmc_amd.CompressedMarchingCubes(dev)
+    gmc.setCompressedVolume(gstream, gw["rate"], gw["dims"], (1, 1, 1), g_nb)
+    g_t = []
+    for _ in range(11):
+        gmc.lruCache.reset()
+        dev.sync()
+        t = time.perf_counter()
+        g_nv = gmc.computeSurface(0.3)
+        g_t.append(time.perf_counter() - t)
+    del gmc
+    out["gauss256_r8"] = {"workload": "256^3 sum of 16 Gaussians (seed 0), zfp rate 8, every block "
+                                      "cached, isovalue 0.3 after an LRU reset",
+                          "gpu_qps": round(1.0 / float(np.median(g_t[1:])), 2), "vertices": int(g_nv)}
+
     if args.cpu_sweeps > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        host = host_cpu()
+        threads = args.cpu_threads or host["threads"]
         t = time.time()
         ref = oracle.MC(stream, 64 * rate, dims, initial_cache=cache0, threads=threads)
         t_load = time.time() - t
@@ -430,16 +451,40 @@ def main():
         for iso in isos:
             ref1.compute_surface(float(iso))
         t_1 = time.time() - t1
+        # the CPU reference configuration: cold queries after a reset, T threads and 1
+        gref = {}
+        for nt in (threads, 1):
+            r = oracle.MC(gstream, 64 * gw["rate"], gw["dims"], initial_cache=g_nb, threads=nt)
+            reps, tg = 0, 0.0
+            while reps < 3 or (tg < 3.0 and reps < 50):
+                r.lru_reset()
+                t = time.time()
+                nv_c = r.compute_surface(0.3)
+                tg += time.time() - t
+                reps += 1
+            assert nv_c == g_nv, (nv_c, g_nv)
+            gref[nt] = (reps / tg, reps, tg)
+            del r
+        out["gauss256_r8"]["cpu_qps"] = round(gref[threads][0], 3)
+        out["gauss256_r8"]["cpu_qps_1_thread"] = round(gref[1][0], 3)
+        out["gauss256_r8"]["gpu_vs_cpu"] = round(out["gauss256_r8"]["gpu_qps"] / gref[threads][0], 1)
         out["cpu_baseline"] = {
             "value": round(nq / t_q, 4), "unit": "iso-queries/s", "cores": threads,
             "kind": "port",
             "sample": f"{args.cpu_sweeps} sweeps x 8 isovalues ({nq} queries, {t_q:.1f}s) of the same "
                       f"workload on the same stream; C oracle with OpenMP over blocks, {threads} "
                       f"threads; load-time range pass ({t_load:.2f}s) excluded",
+            "host": host,
             "single_thread": {"value": round(len(isos) / t_1, 4), "cores": 1,
-                              "sample": f"1 sweep x 8 isovalues, {t_1:.1f}s"}}
+                              "sample": f"1 sweep x 8 isovalues, {t_1:.1f}s"},
+            "gauss256_r8": {"value": round(gref[threads][0], 3), "cores": threads,
+                            "single_thread": round(gref[1][0], 3), "unit": "iso-queries/s",
+                            "sample": f"{gref[threads][1]} cold queries at isovalue 0.3 after an LRU "
+                                      f"reset ({gref[threads][2]:.1f}s); 1 thread: "
+                                      f"{gref[1][1]} queries ({gref[1][2]:.1f}s)"}}
         log(f"bench: cpu oracle {nq} queries in {t_q:.2f}s with {threads} threads; "
-            f"1 thread: 8 queries in {t_1:.2f}s")
+            f"1 thread: 8 queries in {t_1:.2f}s; gauss256 r8 {gref[threads][0]:.2f} q/s "
+            f"({threads} threads), {gref[1][0]:.2f} q/s (1 thread)")
     print(json.dumps(out), flush=True)
 
 

@@ -2,9 +2,10 @@
 the oracle: a 256^3 rate-8 Gaussian field with every block cached (one
 isovalue), a 512^3 rate-4 Fourier-mode field with the cache at 25% of the
 blocks (a reset, then the 0.2..0.8 sweep of 8 isovalues) and a
-1024x1024x1080 rate-2 blob field with 10% white noise and the cache at 10%
-(one isovalue).  Per query: counts, active and decoded ids, blocks with
-vertices, vertex offsets and the cache state bit-exact; vertices bit-exact
+1024x1024x1080 rate-2 blob field with 5% white noise and the cache at 10%
+(one isovalue, also in the global FP32 xyz vertex format).  Per query:
+counts, active and decoded ids, blocks with vertices, vertex offsets and the
+cache state bit-exact; vertices bit-exact
 with the oracle's FP32 evaluation and within 1e-5 voxel units of its FP64
 interpolation.  The volumes come from the same seeded generators as the
 benchmark."""
@@ -43,7 +44,7 @@ def _check(mc, ref, iso, tag):
     for a, b, what in zip(mc.lruCache.state(), ref.lru_state(), ("age", "avail", "item", "slot")):
         assert np.array_equal(a, b), (tag, what)
     say("  iso", iso, "vertices", nv)
-    return nv
+    return nv, r
 
 
 def _setup(dev, name, cache_frac):
@@ -61,7 +62,7 @@ def _setup(dev, name, cache_frac):
 
 def test_gauss256_rate8_all_cached(dev):
     mc, ref = _setup(dev, "gauss256_r8", 1.0)
-    assert _check(mc, ref, 0.3, "gauss256") > 0
+    assert _check(mc, ref, 0.3, "gauss256")[0] > 0
 
 
 def test_fourier512_rate4_sweep_after_reset(dev):
@@ -74,4 +75,34 @@ def test_fourier512_rate4_sweep_after_reset(dev):
 
 def test_blobs1024x1024x1080_rate2_one_query(dev):
     mc, ref = _setup(dev, "blobs1024x1024x1080_r2", 0.10)
-    assert _check(mc, ref, 0.5, "blobs1024") > 0
+    nv, r = _check(mc, ref, 0.5, "blobs1024")
+    assert nv > 0
+    # the same query with FP32 global xyz output (12 B per vertex): exactly
+    # 4 * block_pos + local in FP32, within 1e-5 + one FP32 ulp of |p| of
+    # the FP64 position; compared in slices to bound host memory
+    dims = synth.WORKLOADS["blobs1024x1024x1080_r2"]["dims"]
+    nb = [(d + 3) // 4 for d in dims]
+    del mc
+    mg = bcmc_amd.CompressedMarchingCubes(dev)
+    w = synth.WORKLOADS["blobs1024x1024x1080_r2"]
+    s, _ = synth.make_stream(w, threads=THREADS)
+    mg.setCompressedVolume(s, w["rate"], dims, (1, 1, 1), int(np.ceil(oracle.nblocks(dims) * 0.10)))
+    mg.setVertexFormat(bcmc_amd.VERTS_GLOBAL_F32)
+    assert mg.computeSurface(0.5) == nv
+    buf = mg.vertexBuffer
+    step = 50_000_000
+    worst = 0.0
+    for a in range(0, nv, step):
+        b = min(nv, a + step)
+        got = buf.download(np.float32, 3 * (b - a), 12 * a).reshape(-1, 3)
+        loc = r["verts_f32"][a:b]
+        bid = loc[:, 3].view(np.uint32).astype(np.int64)
+        bpos = np.stack([bid % nb[0], (bid // nb[0]) % nb[1], bid // (nb[0] * nb[1])], 1)
+        want = (4 * bpos).astype(np.float32) + loc[:, :3]
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), a
+        exact = 4.0 * bpos + r["verts_f64"][a:b]
+        tol = 1e-5 + np.spacing(np.abs(exact).astype(np.float32)).astype(np.float64)
+        err = np.abs(got.astype(np.float64) - exact)
+        assert np.all(err <= tol), a
+        worst = max(worst, float(err.max()))
+    say("  global FP32 vertices", nv, "max |fp32 - fp64|", worst)

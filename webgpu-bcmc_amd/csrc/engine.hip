@@ -14,15 +14,17 @@
 //            compaction -> (query, block) pairs in query order, ascending
 //            blocks
 //   mark     pairs whose block is pooled stamp its slot with the step epoch;
-//            missing blocks are claimed once per volume (CAS) and listed
+//            a missing block is listed once, by the first query of the step
+//            (in query order) that has it active
 //   alloc    misses take free pool slots, then the least recently stamped
 //            slots not used by this or the previous step (radix-sorted by
 //            stamp), evicting their blocks
 //   decode   one launch per volume of the step (its zfp rate selects the
 //            LDS-staged decoder variant)
-//   mc       the chunked count / look-back / emit pass over all pairs; each
-//            query's vertices come out contiguous, in the same order as a
-//            single-query computeSurface
+//   mc       count, chunk-prefix scan and emit launches over all pairs in
+//            chunks that never span two queries; each query's vertices come
+//            out contiguous, in the same order as a single-query
+//            computeSurface
 // Query results do not depend on which other queries share the step or on
 // the pool state: the decoded values of a block are a pure function of the
 // stream.

@@ -585,6 +585,25 @@ def run_isovalue_benchmark(mc, kind, value_range, dataset_name="volume", random=
     return perf, f"bcmc-explicit-{dataset_name}-{kind}.json"
 
 
+def request_trace(rng, n, num_volumes, repeat=0.5, window=16):
+    """Synthetic served-form requests: volume ids uniform over the volumes,
+    isovalues U[0.1, 0.9] of which a fraction `repeat` re-use one of the last
+    `window` isovalues of the same volume.  Returns (volume ids, isovalues)."""
+    vols = rng.integers(0, num_volumes, n)
+    hist = [[] for _ in range(num_volumes)]
+    isos = np.empty(n, np.float32)
+    for i, v in enumerate(vols):
+        h = hist[v]
+        if h and rng.random() < repeat:
+            iso = h[int(rng.integers(0, len(h)))]
+        else:
+            iso = np.float32(rng.uniform(0.1, 0.9))
+        isos[i] = iso
+        h.append(iso)
+        del h[:-window]
+    return vols.astype(np.uint32), isos
+
+
 class Engine:
     """Batched / served form: resident volumes + a shared decoded-block pool;
     run() answers many (volume, isovalue) queries in one step."""
