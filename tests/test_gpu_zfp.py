@@ -63,3 +63,46 @@ def test_decode_synthetic_volume(dev, rate):
     out2 = bcmc_amd.decode_blocks(dev, sb, 64 * rate, len(ids), dev.buffer_from(ids))
     got = out2.download(np.float32, len(ids) * 64).reshape(-1, 64)
     assert np.array_equal(got.view(np.uint32), want.reshape(nb, 64)[ids].view(np.uint32))
+
+
+def _random_stream(rng, nblocks, maxbits, densities):
+    """Streams of random bits (every block's flag bit set) at several densities
+    of ones: sparse ones give long zero runs (up to and past the run that
+    reaches coefficient 63, and runs cut by the bit budget), dense ones give
+    group tests every few bits.  Any bit sequence is a valid fixed-rate stream."""
+    per = nblocks // len(densities)
+    bits = np.concatenate([(rng.random(per * maxbits) < d).astype(np.uint8) for d in densities])
+    bits[::maxbits] = 1
+    return np.packbits(bits, bitorder="little"), per * len(densities)
+
+
+@pytest.mark.parametrize("maxbits", [64, 128, 256, 512, 1024, 100, 37])
+def test_decode_random_bitstreams(dev, maxbits):
+    say("zfp random bitstreams, maxbits", maxbits)
+    rng = np.random.default_rng(maxbits)
+    s, nb = _random_stream(rng, 6000, maxbits, (0.005, 0.02, 0.08, 0.3, 0.6, 0.95))
+    out = bcmc_amd.decode_blocks(dev, _padded_stream(dev, s), maxbits, nb)
+    got = out.download(np.float32, nb * 64).reshape(nb, 64)
+    want = np.stack([oracle.decode_block(s, maxbits, b) for b in range(nb)])
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_decode_past_2_pow_32_bits(dev):
+    """Blocks whose first bit lies at or beyond bit 2^32 of the stream (the
+    reference's u32 offset block_index * max_bits wraps there,
+    zfp_decompress.comp:199): rate 8, blocks around 2^32 / 512 = 8,388,608."""
+    maxbits = 512
+    first = (1 << 32) // maxbits - 40
+    nb = first + 80
+    s = np.zeros(nb * maxbits // 8, np.uint8)
+    f = synth.field(synth.GAUSS, 17, (16, 16, 20), 6, (3, 8, 0.5, 1))
+    enc = synth.encode(f, maxbits)  # 80 blocks
+    s[first * maxbits // 8:] = enc[:80 * maxbits // 8]
+    say("zfp decode past 2^32 bits:", len(s), "bytes")
+    ids = np.arange(first, nb, dtype=np.uint32)
+    out = bcmc_amd.decode_blocks(dev, _padded_stream(dev, s), maxbits, len(ids), dev.buffer_from(ids))
+    got = out.download(np.float32, len(ids) * 64).reshape(-1, 64)
+    tail = s[first * maxbits // 8:]
+    want = np.stack([oracle.decode_block(tail, maxbits, b) for b in range(80)])
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert np.count_nonzero(want) > 0

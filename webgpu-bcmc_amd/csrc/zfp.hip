@@ -174,6 +174,186 @@ BCMC_DEV void decode_block(const BitWindow<kGuard> &bw, uint32_t maxbits, float 
 }
 
 // ---------------------------------------------------------------------------
+// Decoder for blocks staged in LDS (64 | maxbits: W = maxbits / 64 words).
+//
+// A wave stages its 64 blocks' fixed-rate bits as dwords, block b at
+// [b * kStride, b * kStride + 2W), followed by two all-ones dwords.  The ones
+// stand for the budget's end: a run of zeros that reaches it stops there, as
+// the reference's bit counter stops it (zfp_decompress.comp:282-287), so the
+// run needs no budget test.  Every read is the 64-bit window at bit p: three
+// dword loads and two v_alignbit.
+//
+// The bit planes (k = 31 .. 0, the same k in every lane at each step) are
+// kept as 64-bit words X[k] in registers -- the verbatim bits of the n
+// coefficients already significant, plus a 1 at each coefficient that becomes
+// significant in the plane -- and turned into the 64 coefficients at the end
+// by two 32x32 bit-matrix transposes, instead of depositing every plane's
+// bits into 64 registers.
+// ---------------------------------------------------------------------------
+template <int W>
+struct Staged {
+    static constexpr int kStride = 2 * W + 3;  // odd: lanes' windows spread over banks
+    static constexpr uint32_t kEnd = 64u * W;  // maxbits
+};
+
+BCMC_DEV void window64(const uint32_t *s, uint32_t p, uint32_t &lo, uint32_t &hi) {
+    const uint32_t *q = s + __builtin_amdgcn_ubfe(p, 5u, 27u);
+    const uint32_t a = q[0], b = q[1], c = q[2];
+    lo = __builtin_amdgcn_alignbit(b, a, p);  // the shift uses p mod 32
+    hi = __builtin_amdgcn_alignbit(c, b, p);
+}
+
+// In-place transpose of a 32x32 bit matrix: on entry bit i of a[k] is element
+// (k, i), on exit bit k of a[i] is.  Quadrant swaps at distance 16 and 8 are
+// byte permutes, those at 4, 2, 1 two shifts and two bit-field inserts.
+BCMC_DEV void transpose32(uint32_t (&a)[32]) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const uint32_t x = a[k], y = a[k + 16];
+        a[k] = __builtin_amdgcn_perm(y, x, 0x05040100u);
+        a[k + 16] = __builtin_amdgcn_perm(y, x, 0x07060302u);
+    }
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+        if (k & 8)
+            continue;
+        const uint32_t x = a[k], y = a[k + 8];
+        a[k] = __builtin_amdgcn_perm(y, x, 0x06020400u);
+        a[k + 8] = __builtin_amdgcn_perm(y, x, 0x07030501u);
+    }
+    constexpr uint32_t kMask[3] = {0x0F0F0F0Fu, 0x33333333u, 0x55555555u};
+#pragma unroll
+    for (int st = 0; st < 3; ++st) {
+        const int j = 4 >> st;
+        const uint32_t m = kMask[st];
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            if (k & j)
+                continue;
+            const uint32_t x = a[k], y = a[k + j];
+            a[k] = (x & m) | ((y << j) & ~m);
+            a[k + j] = ((x >> j) & m) | (y & ~m);
+        }
+    }
+}
+
+template <int W>
+BCMC_DEV void decode_block_staged(const uint32_t *s, float (&v)[64]) {
+    constexpr uint32_t E = Staged<W>::kEnd;
+    const uint32_t head = s[0];
+    if ((head & 1u) == 0) {
+#pragma unroll
+        for (int i = 0; i < 64; ++i)
+            v[i] = 0.0f;
+        return;
+    }
+    const int emax = (int)((head >> 1) & 0xFFu) - 127;
+    uint32_t xl[32], xh[32];  // bit plane k: coefficients 0..31 in xl[k], 32..63 in xh[k]
+    uint32_t p = 9, n = 0;
+#pragma unroll
+    for (int k = 31; k >= 0; --k) {
+        uint64_t x = 0;
+        if (p < E) {
+            // the n significant coefficients' bits, verbatim
+            const uint32_t m = min(n, E - p);
+            uint32_t lo, hi;
+            window64(s, p, lo, hi);
+            const uint64_t w0 = (uint64_t)hi << 32 | lo;
+            x = w0 & (m >= 64 ? ~0ull : (1ull << m) - 1ull);
+            p += m;
+            // group tests (zfp_decompress.comp:282-287): a group bit 1, then
+            // zeros up to the next significant coefficient and its 1; a run
+            // reaching coefficient 63 implies it without a terminating bit.
+            // Each group bit is read from the window that ends the step
+            // before it (here: m < 64 whenever n < 64), so a step runs only
+            // for a group bit 1, and a 0 ends the plane without a step.
+            bool more = n < 64 && p < E;
+            if (more && ((w0 >> m) & 1u) == 0) {
+                ++p;
+                more = false;
+            }
+            while (more) {
+                uint32_t wl, wh;
+                window64(s, p, wl, wh);
+                // zeros after the group bit: ctz of the window shifted by one
+                // (bit 63 forced to 1: a window of zeros is a run of >= 63)
+                const uint64_t t = ((uint64_t)__builtin_amdgcn_alignbit(1u, wh, 1u) << 32) |
+                                   __builtin_amdgcn_alignbit(wh, wl, 1u);
+                const uint32_t r = (uint32_t)__builtin_ctzll(t);
+                const uint32_t adv = min(r + 2u, 64u - n);
+                n = min(n + r + 1u, 64u);  // one past the coefficient found
+                x |= 1ull << (n - 1u);
+                // the next group bit: bit adv of this window (adv == 64 only
+                // after a run of 62 zeros from n = 0: read it)
+                const uint64_t w = (uint64_t)wh << 32 | wl;
+                uint32_t g = (uint32_t)(w >> (adv & 63u)) & 1u;
+                p += adv;
+                more = n < 64 && p < E;
+                if (more && adv == 64u)
+                    g = (s[p >> 5] >> (p & 31u)) & 1u;
+                if (more && g == 0) {
+                    ++p;
+                    more = false;
+                }
+            }
+        }
+        // negabinary: the planes of 0xaaaaaaaa (the odd ones) are inverted
+        xl[k] = (k & 1) ? ~(uint32_t)x : (uint32_t)x;
+        xh[k] = (k & 1) ? ~(uint32_t)(x >> 32) : (uint32_t)(x >> 32);
+    }
+    transpose32(xl);
+    transpose32(xh);
+    int32_t q[64];
+#pragma unroll
+    for (int i = 0; i < 64; ++i)
+        q[perm3d(i)] = (int32_t)((i < 32 ? xl[i] : xh[i - 32]) - 0xaaaaaaaau);
+    // inverse lifts along z, y, x (zfp_decompress.comp:296-354), wrapping
+    // int32; each step pair such as y += w; w <<= 1; w -= y is formed as
+    // (y + w, w - y), which is the same value modulo 2^32
+    auto lift = [&](int i0, int st) {
+        uint32_t x = (uint32_t)q[i0], y = (uint32_t)q[i0 + st], z = (uint32_t)q[i0 + 2 * st],
+                 w = (uint32_t)q[i0 + 3 * st];
+        y += (uint32_t)((int32_t)w >> 1);
+        w -= (uint32_t)((int32_t)y >> 1);
+        uint32_t t = y;
+        y = t + w;
+        w = w - t;
+        t = z;
+        z = t + x;
+        x = x - t;
+        t = y;
+        y = t + z;
+        z = z - t;
+        t = w;
+        w = t + x;
+        x = x - t;
+        q[i0] = (int32_t)x;
+        q[i0 + st] = (int32_t)y;
+        q[i0 + 2 * st] = (int32_t)z;
+        q[i0 + 3 * st] = (int32_t)w;
+    };
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+            lift(x + 4 * y, 16);
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int z = 0; z < 4; ++z)
+            lift(16 * z + x, 4);
+#pragma unroll
+    for (int z = 0; z < 4; ++z)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+            lift(4 * y + 16 * z, 1);
+    const float inv_w = __builtin_ldexpf(1.0f, emax - 30);
+#pragma unroll
+    for (int i = 0; i < 64; ++i)
+        v[i] = inv_w * (float)q[i];
+}
+
+// ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
 constexpr int kZfpThreads = 256;  // 4 waves = 256 blocks per workgroup pass
@@ -204,8 +384,9 @@ struct ZfpArgs {
 // W = u64 words per block when 64 | maxbits (LDS-staged), 0 = generic.
 template <int W, int MODE>
 __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
-    constexpr int kStage = W > 0 ? 64 * W + 2 : 1;
-    __shared__ uint64_t stage[kZfpThreads / 64][kStage];
+    constexpr int kStride = Staged<W>::kStride;
+    constexpr int kStage = W > 0 ? 64 * kStride : 1;
+    __shared__ uint32_t stage[kZfpThreads / 64][kStage];
     const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
     uint32_t n = a.n_dev ? *a.n_dev : a.n_host;
     if constexpr (MODE == kModeLruAssign) {
@@ -222,29 +403,67 @@ __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
         const uint32_t i = chunk * 64 + lane;
         const bool live = i < n;
         const uint32_t blk = live ? (a.ids ? a.ids[i] : i) : 0u;
+        // destination lookups, issued before the decode so that their
+        // latency hides behind it
+        int32_t slot = -1, prev = -1;
+        uint32_t s_new = 0xFFFFFFFFu;
+        if constexpr (MODE == kModeLruAssign) {
+            if (live)
+                s_new = a.order[i];
+        }
         float v[64];
         if constexpr (W > 0) {
-            uint64_t *sw = stage[wave];
-            // coalesced copy of the wave's fixed-rate words into LDS
-            for (uint32_t j = lane; j < 64u * W; j += 64) {
-                const uint32_t e = chunk * 64 + j / W;
-                uint64_t word = 0;
-                if (e < n) {
-                    const uint64_t b = a.ids ? a.ids[e] : e;
-                    const uint64_t wi = b * W + j % W;
-                    word = wi < a.nwords ? a.stream[wi] : 0ull;
-                }
-                sw[j] = word;
+            uint32_t *sw = stage[wave];
+            // copy of the wave's fixed-rate words into LDS, block by block at
+            // a stride of kStride dwords, each followed by two all-ones dwords
+            // (the budget's end).  Word j of the wave is word j % W of the
+            // chunk's block j / W, whose id lane j / W holds; all W loads of a
+            // lane are issued before any is stored.
+            uint32_t bsrc[W];
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                const uint32_t src = ((uint32_t)t * 64u + lane) / W;
+                bsrc[t] = a.ids ? (uint32_t)__shfl((int)blk, (int)src, 64) : chunk * 64 + src;
             }
-            if (lane < 2)
-                sw[64 * W + lane] = 0;
+            uint64_t words[W];
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                const uint32_t j = (uint32_t)t * 64u + lane;
+                const uint64_t wi = (uint64_t)bsrc[t] * W + j % W;
+                words[t] = (chunk * 64 + j / W < n && wi < a.nwords) ? a.stream[wi] : 0ull;
+            }
+            if constexpr (MODE == kModeCache) {
+                if (live)
+                    slot = a.item_slot[blk];
+            }
+            if constexpr (MODE == kModeLruAssign) {
+                if (s_new < a.capacity)
+                    prev = a.slot_item[s_new];
+            }
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                const uint32_t j = (uint32_t)t * 64u + lane;
+                uint32_t *d = sw + (j / W) * kStride + 2 * (j % W);
+                d[0] = (uint32_t)words[t];
+                d[1] = (uint32_t)(words[t] >> 32);
+            }
+            sw[lane * kStride + 2 * W] = ~0u;
+            sw[lane * kStride + 2 * W + 1] = ~0u;
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             if (live)
-                decode_block(BitWindow<false>{sw + lane * W, 0u, 0ull}, a.maxbits, v);
+                decode_block_staged<W>(sw + lane * kStride, v);
             __builtin_amdgcn_wave_barrier();
         } else {
+            if constexpr (MODE == kModeCache) {
+                if (live)
+                    slot = a.item_slot[blk];
+            }
+            if constexpr (MODE == kModeLruAssign) {
+                if (s_new < a.capacity)
+                    prev = a.slot_item[s_new];
+            }
             const uint64_t bit0 = (uint64_t)blk * a.maxbits, w0 = bit0 >> 6;
             if (live)
                 decode_block(BitWindow<true>{a.stream + w0, (uint32_t)(bit0 & 63), a.nwords > w0 ? a.nwords - w0 : 0ull},
@@ -296,7 +515,6 @@ __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
         } else {
             uint64_t dst = (uint64_t)i * 64;
             if constexpr (MODE == kModeCache) {
-                const int32_t slot = a.item_slot[blk];
                 if (slot < 0 || (uint32_t)slot >= a.capacity)
                     continue;
                 dst = (uint64_t)(uint32_t)slot * 64;
@@ -305,10 +523,9 @@ __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
                 // lru_cache_update.comp:25-36: evict the slot's previous item,
                 // install this one (new items are never cached ones, so no two
                 // threads write the same item entry)
-                const uint32_t s = a.order[i];
+                const uint32_t s = s_new;
                 if (s >= a.capacity || blk >= a.total)
                     continue;  // cannot happen for a consistent state
-                const int32_t prev = a.slot_item[s];
                 if (prev >= 0 && (uint32_t)prev < a.total)
                     a.item_slot_w[prev] = -1;
                 a.slot_age[s] = 0;
