@@ -263,10 +263,16 @@ BCMC_API bcmc_buf *bcmc_engine_vertex_buffer(bcmc_engine *e);
 BCMC_API int bcmc_engine_stats(bcmc_engine *e, bcmc_engine_info *info);
 
 /* Served form: replay timestamped requests in real time through one engine
- * with continuous batching (each step takes every request that has arrived,
- * FIFO, up to max_batch and an output-vertex budget).  out[i] receives request
- * i's arrival, step start and finish times (seconds from the call) and its
- * vertex count; the vertices of the last step stay in the engine buffer. */
+ * with continuous batching (each step takes the requests that have arrived,
+ * FIFO, up to max_batch).  Admission waits for room rather than failing:
+ * a step takes requests while their bound on touched blocks fits the pool
+ * slots the previous step left evictable (an empty step frees the rest when
+ * the head request needs them), and emits only the leading requests whose
+ * exact vertex counts fit vertex_budget_bytes; the others stay at the head of
+ * the queue.  Fails only when one request alone exceeds the pool or the
+ * budget.  out[i] receives request i's arrival, step start and finish times
+ * (seconds from the call) and its vertex count; the vertices of the last step
+ * stay in the engine buffer. */
 typedef struct {
     uint32_t max_batch;           /* 0: 256 */
     uint64_t vertex_budget_bytes; /* HBM for one step's vertices; 0: 64 GiB */

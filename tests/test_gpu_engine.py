@@ -90,6 +90,107 @@ def test_served_trace_results_match_single_queries(dev):
         assert out["num_vertices"][i] == cache[key], i
 
 
+def _piled_trace(streams, n, seed):
+    """n requests at 4000 req/s (they pile up behind each step) over every
+    volume, isovalues where the volumes have surface."""
+    rng = np.random.default_rng(seed)
+    arrival = np.cumsum(rng.exponential(1.0 / 4000.0, n))
+    vol = rng.integers(0, len(streams), n).astype(np.uint32)
+    iso = rng.uniform(0.3, 0.7, n).astype(np.float32)
+    return arrival, vol, iso
+
+
+def _oracle_active(s, rate, dims, iso):
+    ref = oracle.MC(s, 64 * rate, dims)
+    nv = ref.compute_surface(iso)
+    return nv, ref.results()["active_ids"].copy()
+
+
+def test_served_trace_waits_for_pool_room(dev):
+    """A pool that holds every single request's active blocks but not those
+    of a step of piled-up requests: the served loop admits what fits (an
+    empty step frees the previous step's slots when the head needs them)
+    instead of failing, and every request still equals the oracle."""
+    streams = _streams()
+    n = 48
+    arrival, vol, iso = _piled_trace(streams, n, 31)
+    act = {}
+    for i in range(n):
+        key = (int(vol[i]), float(iso[i]))
+        if key not in act:
+            s, r, d = streams[vol[i]]
+            act[key] = _oracle_active(s, r, d, float(iso[i]))
+    one = max(len(a) for _, a in act.values())
+    pool = one + 16
+    # the union of active blocks of the first 16 requests, per volume
+    union = sum(len(set().union(*[set(act[(int(vol[i]), float(iso[i]))][1].tolist())
+                                  for i in range(16) if vol[i] == v] or [set()]))
+                for v in range(len(streams)))
+    say("pool", pool, "largest request", one, "first 16 requests", union)
+    assert union > pool
+    eng = bcmc_amd.Engine(dev, pool * 256)
+    ids = [eng.add_volume(s, r, d) for s, r, d in streams]
+    assert ids == list(range(len(streams)))
+    with pytest.raises(bcmc_amd.BcmcError) as e:  # as one step it does not fit
+        eng.run(vol[:16], iso[:16])
+    assert e.value.code == -3
+    eng = bcmc_amd.Engine(dev, pool * 256)
+    for s, r, d in streams:
+        eng.add_volume(s, r, d)
+    out = eng.serve(arrival, vol, iso, max_batch=16)
+    for i in range(n):
+        assert out["num_vertices"][i] == act[(int(vol[i]), float(iso[i]))][0], i
+    assert np.all(np.diff(out["step"]) >= 0)  # FIFO
+    assert out["batch_size"].max() > 1
+    st = eng.stats()
+    assert st["pool_used"] <= st["pool_slots"] == pool
+    # the vertices of the last step are there and exact
+    last = out["step"] == out["step"].max()
+    k = int(np.nonzero(last)[0][0])
+    s, r, d = streams[vol[k]]
+    res = eng.run([vol[k]], [iso[k]])[0]
+    nv, verts = _oracle(s, r, d, float(iso[k]))
+    assert res["num_vertices"] == nv
+    assert np.array_equal(eng.vertices(res).view(np.uint32), verts.view(np.uint32))
+
+
+def test_served_trace_defers_past_the_vertex_budget(dev):
+    """An output budget that holds the largest single request but not a
+    step's worth: each step emits the leading requests that fit (exact vertex
+    counts), the rest wait at the head of the queue; a request larger than
+    the budget alone is an error."""
+    streams = _streams()
+    n = 40
+    arrival, vol, iso = _piled_trace(streams, n, 37)
+    nv = {}
+    for i in range(n):
+        key = (int(vol[i]), float(iso[i]))
+        if key not in nv:
+            s, r, d = streams[vol[i]]
+            nv[key] = _oracle(s, r, d, float(iso[i]))[0]
+    counts = np.array([nv[(int(vol[i]), float(iso[i]))] for i in range(n)], np.uint64)
+    budget = int(counts.max()) * 16 + 16
+    say("budget", budget, "bytes; all requests", int(counts.sum()) * 16)
+    assert counts.sum() * 16 > 2 * budget
+    eng = bcmc_amd.Engine(dev, 100000 * 256)
+    for s, r, d in streams:
+        eng.add_volume(s, r, d)
+    out = eng.serve(arrival, vol, iso, max_batch=16, vertex_budget_bytes=budget)
+    assert np.array_equal(out["num_vertices"].astype(np.uint64), counts)
+    steps = out["step"]
+    assert np.all(np.diff(steps) >= 0)
+    for st_ in np.unique(steps):
+        sel = steps == st_
+        assert int(counts[sel].sum()) * 16 <= budget, st_
+        assert np.all(out["batch_size"][sel] == sel.sum()), st_
+    assert out["batch_size"].max() > 1
+    # one request larger than the budget alone
+    big = int(np.argmax(counts))
+    with pytest.raises(bcmc_amd.BcmcError) as e:
+        eng.serve([0.0], [vol[big]], [iso[big]], vertex_budget_bytes=int(counts[big]) * 16 - 16)
+    assert e.value.code == -3
+
+
 def test_result_is_the_same_alone_and_in_the_mix(dev):
     """A query's vertices are bit-identical whether it runs alone or among
     other queries of other volumes (and after the pool has been churned)."""

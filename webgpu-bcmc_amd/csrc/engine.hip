@@ -84,7 +84,7 @@ struct EngCounters {
     uint32_t short_slots;       // misses that found no slot (admission failed)
     uint32_t num_rows;          // candidate row segments of all queries
     uint32_t num_chunks;        // marching-cubes chunks of all queries
-    uint32_t pad0;
+    uint32_t emit_chunks;       // chunks the emit launch writes (set by the host per step)
     unsigned long long num_verts;
     uint32_t vol_miss[64];      // misses per volume
 };
@@ -598,6 +598,13 @@ struct EngChunkPrefixOp {
     BCMC_DEV void total(const unsigned long long (&t)[1]) const { c->num_verts = t[0]; }
 };
 
+// Where each query's vertices start (its first chunk's prefix); ~0 for a
+// query without active blocks.
+__global__ void eng_vfirst_kernel(EngMcArgs a) {
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < a.nq; q += gridDim.x * blockDim.x)
+        a.q_vfirst[q] = a.q_act_start[q] == kNoPairs ? ~0ull : a.chunk_verts_pre[a.q_chunk_base[q]];
+}
+
 template <int FMT>
 __global__ __launch_bounds__(kEngThreads) __attribute__((amdgpu_waves_per_eu(5))) void eng_emit_kernel(EngMcArgs a) {
     constexpr int NW = kEngThreads / 64;
@@ -606,7 +613,9 @@ __global__ __launch_bounds__(kEngThreads) __attribute__((amdgpu_waves_per_eu(5))
     __shared__ McTables tab;
     __shared__ EmitScratch es[NW];
     load_mc_tables(&tab);
-    const uint32_t nchunks = a.c->num_chunks, epoch = a.c->epoch;
+    // the host admits the step's leading queries whose vertices fit the
+    // output budget: only their chunks are written
+    const uint32_t nchunks = min(a.c->num_chunks, a.c->emit_chunks), epoch = a.c->epoch;
     const uint32_t wave = threadIdx.x / 64;
     for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
         const uint32_t q = eng_query_of_chunk(a, chunk);
@@ -619,9 +628,6 @@ __global__ __launch_bounds__(kEngThreads) __attribute__((amdgpu_waves_per_eu(5))
         uint32_t ex_h, ex_v, agg_v;
         const uint32_t num_with = emit_prologue(cs, c, bid, q, lds_w, &ex_h, &ex_v, &agg_v);
         const unsigned long long pv = a.chunk_verts_pre[chunk];
-        // where the query's vertices start
-        if (threadIdx.x == 0 && chunk == a.q_chunk_base[q])
-            a.q_vfirst[q] = pv;
         if (num_with == 0)
             continue;
         RowBlock ub;
@@ -654,7 +660,18 @@ struct EngVolume {
     float2 *ranges = nullptr, *dilated = nullptr, *seg_range = nullptr, *tile = nullptr;
     int32_t *slot_map = nullptr;
     uint32_t *miss_list = nullptr;  // per-step block ids to decode (EngVol::miss_list)
+    // the row segments' ranges, lower and upper ends each sorted (host): the
+    // segments whose range holds an isovalue bound its active blocks
+    std::vector<float> seg_lo, seg_hi;
 };
+
+// At most this many blocks of volume v are active at iso: every active block
+// lies in a row segment whose range holds iso (kSegBlocks blocks each).
+static uint64_t eng_block_bound(const EngVolume &v, float iso) {
+    const uint64_t lo_le = std::upper_bound(v.seg_lo.begin(), v.seg_lo.end(), iso) - v.seg_lo.begin();
+    const uint64_t hi_lt = std::lower_bound(v.seg_hi.begin(), v.seg_hi.end(), iso) - v.seg_hi.begin();
+    return std::min<uint64_t>(v.n, (lo_le - std::min(lo_le, hi_lt)) * kSegBlocks);
+}
 
 using EngKey = std::array<uint64_t, 6>;
 struct EngGraph {
@@ -697,6 +714,7 @@ struct bcmc_engine {
     EngQuery *h_q = nullptr;
     uint32_t *h_same = nullptr, *h_act = nullptr;
     unsigned long long *h_vfirst = nullptr;
+    uint32_t *h_chunk_base = nullptr;
     unsigned long long *q_vfirst = nullptr;
     ScanSite site_active, site_cand;
     void *verts = nullptr;
@@ -814,10 +832,12 @@ static int eng_ensure_queries(bcmc_engine *e, uint32_t nq) {
     eng_release_host(e->h_q);
     eng_release_host(e->h_act);
     eng_release_host(e->h_vfirst);
+    eng_release_host(e->h_chunk_base);
     const uint32_t cap = std::max<uint32_t>(nq, 64);
     BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_q, (size_t)cap * sizeof(EngQuery), hipHostMallocDefault));
     BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_act, (size_t)cap * 4, hipHostMallocDefault));
     BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_vfirst, (size_t)cap * 8, hipHostMallocDefault));
+    BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_chunk_base, (size_t)cap * 4, hipHostMallocDefault));
     BCMC_TRY(dev_alloc((void **)&e->d_q, (size_t)cap * sizeof(EngQuery)));
     BCMC_TRY(dev_alloc((void **)&e->q_act_start, (size_t)cap * 4));
     BCMC_TRY(dev_alloc((void **)&e->q_pair_end, (size_t)cap * 4));
@@ -890,9 +910,9 @@ static int eng_ensure_chunks(bcmc_engine *e, uint64_t max_chunks) {
     return BCMC_OK;
 }
 
-// count -> chunk prefixes -> emit over the step's active pairs (chunk table
-// built before the mark)
-static int eng_launch_mc(bcmc_engine *e, uint32_t nq, uint64_t total) {
+// count -> chunk prefixes -> each query's first vertex (chunk table built
+// before the mark); the emit is launched after the host has seen the counts
+static int eng_launch_count(bcmc_engine *e, uint32_t nq, uint64_t total) {
     const uint64_t max_chunks = eng_max_chunks(total, nq);
     const EngMcArgs a = eng_mc_args(e, nq);
     const uint32_t grid = (uint32_t)std::max<uint64_t>(
@@ -900,7 +920,9 @@ static int eng_launch_mc(bcmc_engine *e, uint32_t nq, uint64_t total) {
     eng_count_kernel<<<grid, kEngThreads, 0, e->ctx->stream>>>(a);
     launch_scan<1>(e->ctx, e->site_chunks, e->site_chunks,
                    EngChunkPrefixOp{a.chunk_verts, a.chunk_verts_pre, a.c}, max_chunks);
-    return eng_launch_emit(e, nq, total);
+    eng_vfirst_kernel<<<div_up(nq, 256), 256, 0, e->ctx->stream>>>(a);
+    BCMC_HIP_CHECK(hipGetLastError());
+    return BCMC_OK;
 }
 
 // What a step's kernel chain bakes in besides the engine's buffers.
@@ -911,10 +933,12 @@ struct EngPlan {
     uint32_t sort_passes;  // eviction-order sort passes; 0: no eviction chain
 };
 
-// The step's kernel chain, from the epoch bump to the emit.  It reads the
-// step's queries, tile table and same-volume lists from the device tables
-// uploaded before it, and makes no allocation, copy or memset, so it can be
-// captured into a graph and replayed.
+// The step's kernel chain, from the epoch bump to the count pass and the
+// vertex offsets (the emit follows in a launch of its own, once the host has
+// sized the output from the counts).  It reads the step's queries, tile table
+// and same-volume lists from the device tables uploaded before it, and makes
+// no allocation, copy or memset, so it can be captured into a graph and
+// replayed.
 static int eng_enqueue_step(bcmc_engine *e, const EngPlan &p) {
     bcmc_ctx *ctx = e->ctx;
     hipStream_t st = ctx->stream;
@@ -949,7 +973,7 @@ static int eng_enqueue_step(bcmc_engine *e, const EngPlan &p) {
         launch_decode_to_cache(ctx, v.stream, v.nwords, v.maxbits, v.miss_list, &e->c->vol_miss[vi],
                                v.n, v.slot_map, e->pool, e->pool_cap);
     }
-    return eng_launch_mc(e, nq, p.total);
+    return eng_launch_count(e, nq, p.total);
 }
 
 extern "C" {
@@ -1041,7 +1065,7 @@ BCMC_API int bcmc_engine_destroy(bcmc_engine *e) {
     dev_free(e->same_list);
     dev_free(e->tile_q);
     for (void *h : {(void *)e->h_q, (void *)e->h_same, (void *)e->h_act, (void *)e->h_vfirst,
-                    (void *)e->h_tile_q})
+                    (void *)e->h_tile_q, (void *)e->h_chunk_base})
         if (h)
             (void)hipHostFree(h);
     dev_free(e->verts);
@@ -1109,7 +1133,20 @@ BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t
     eng_seg_range_kernel<<<eng_grid(ctx, nsegs), 256, 0, ctx->stream>>>(v.dilated, v.nb[0], nsegx, nsegs,
                                                                          v.seg_range);
     BCMC_HIP_CHECK(hipGetLastError());
-    BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    {
+        std::vector<float2> sr(nsegs);
+        BCMC_HIP_CHECK(hipMemcpyAsync(sr.data(), v.seg_range, (size_t)nsegs * 8, hipMemcpyDeviceToHost,
+                                      ctx->stream));
+        BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        v.seg_lo.resize(nsegs);
+        v.seg_hi.resize(nsegs);
+        for (uint32_t i = 0; i < nsegs; ++i) {
+            v.seg_lo[i] = sr[i].x;
+            v.seg_hi[i] = sr[i].y;
+        }
+        std::sort(v.seg_lo.begin(), v.seg_lo.end());
+        std::sort(v.seg_hi.begin(), v.seg_hi.end());
+    }
     dev_free(v.ranges);  // only the dilated and tile ranges are used from here on
     dev_free(parts);
     parts = nullptr;
@@ -1156,9 +1193,82 @@ BCMC_API int bcmc_engine_set_vertex_format(bcmc_engine *e, bcmc_vertex_format fm
     return BCMC_OK;
 }
 
-BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const float *isovalues,
-                             uint32_t nq, bcmc_query_result *results) {
+}  // extern "C"
+
+// Replays the graph captured for `key`, capturing it the second time the key
+// is seen among the last kEngSeen parts, or runs enqueue() directly.
+template <class F>
+static int eng_run_part(bcmc_engine *e, const EngKey &key, F &&enqueue, bool *replayed) {
+    hipStream_t st = e->ctx->stream;
+    EngGraph *hit = nullptr;
+    if (e->use_graph) {
+        for (EngGraph &g : e->graphs)
+            hit = g.key == key ? &g : hit;
+        const bool seen = std::find(e->seen.begin(), e->seen.end(), key) != e->seen.end();
+        if (!hit && seen) {
+            // the stream holds only the staging copies of this step: nothing
+            // was reallocated (gen is in the key)
+            BCMC_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue();
+            hipGraph_t g = nullptr;
+            hipGraphExec_t x = nullptr;
+            const hipError_t ec = hipStreamEndCapture(st, &g);
+            if (rc == BCMC_OK && ec == hipSuccess && g && hipGraphInstantiate(&x, g, nullptr, nullptr, 0) == hipSuccess) {
+                if (e->graphs.size() == kEngGraphs) {
+                    auto lru = std::min_element(e->graphs.begin(), e->graphs.end(),
+                                                [](const EngGraph &u, const EngGraph &v) { return u.last_use < v.last_use; });
+                    eng_drop_graph(*lru);
+                    e->graphs.erase(lru);
+                }
+                e->graphs.push_back(EngGraph{key, g, x, 0});
+                hit = &e->graphs.back();
+            } else {
+                if (g)
+                    (void)hipGraphDestroy(g);
+                eng_drop_graphs(e);
+                e->use_graph = false;  // replay is an optimisation: fall back to direct launches
+                (void)hipGetLastError();
+                if (rc != BCMC_OK)
+                    return rc;
+            }
+        } else if (!hit) {
+            if (e->seen.size() < kEngSeen)
+                e->seen.push_back(key);
+            else
+                e->seen[e->seen_next] = key;
+            e->seen_next = (e->seen_next + 1) % kEngSeen;
+        }
+        if (hit)
+            hit->last_use = ++e->clock;
+    }
+    *replayed = hit != nullptr;
+    if (hit) {
+        BCMC_HIP_CHECK(hipGraphLaunch(hit->exec, st));
+        return BCMC_OK;
+    }
+    return enqueue();
+}
+
+// An empty step: advances the epoch, so that the blocks the last step stamped
+// may be evicted by the next one.
+static int eng_idle_step(bcmc_engine *e) {
+    eng_begin_kernel<<<1, 256, 0, e->ctx->stream>>>(e->c, e->q_act_start, 0);
+    BCMC_HIP_CHECK(hipGetLastError());
+    BCMC_HIP_CHECK(hipMemcpyAsync(e->ch, e->c, sizeof(EngCounters), hipMemcpyDeviceToHost, e->ctx->stream));
+    BCMC_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
+    return BCMC_OK;
+}
+
+// One step over nq queries.  The chain runs up to the count pass; from the
+// exact vertex counts the host admits the leading queries whose vertices fit
+// `budget` bytes (all of them if they fit), sizes the vertex buffer for
+// exactly those, and launches the emit for their chunks.  *n_done receives
+// the number admitted (0: the first query alone exceeds the budget); their
+// results go to results[0 .. *n_done).
+static int eng_run(bcmc_engine *e, const uint32_t *volume_ids, const float *isovalues, uint32_t nq,
+                   uint64_t budget, uint32_t *n_done, bcmc_query_result *results) {
     BCMC_CHECK_ARG(e && (nq == 0 || (volume_ids && isovalues)), "null argument");
+    *n_done = 0;
     if (nq == 0)
         return BCMC_OK;
     if (!e->miss) {
@@ -1252,48 +1362,9 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
     BCMC_TRY(eng_ensure_chunks(e, eng_max_chunks(total, nq)));
     // the chain reads the step through the uploaded tables; everything the
     // launches bake in (sizes, volumes touched, sort passes, buffers, vertex
-    // format, tuned grids) is in the key
-    const EngKey key = {nq | (uint64_t)p.sort_passes << 32, rows, total, p.used_mask,
-                        (uint64_t)e->fmt | ctx->tune_gen << 8, e->gen};
-    EngGraph *hit = nullptr;
-    if (e->use_graph) {
-        for (EngGraph &g : e->graphs)
-            hit = g.key == key ? &g : hit;
-        const bool seen = std::find(e->seen.begin(), e->seen.end(), key) != e->seen.end();
-        if (!hit && seen) {
-            // the stream is idle here: nothing was reallocated (gen is in the
-            // key) and the previous step ended with a synchronisation
-            BCMC_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            const int rc = eng_enqueue_step(e, p);
-            hipGraph_t g = nullptr;
-            hipGraphExec_t x = nullptr;
-            const hipError_t ec = hipStreamEndCapture(st, &g);
-            if (rc == BCMC_OK && ec == hipSuccess && g && hipGraphInstantiate(&x, g, nullptr, nullptr, 0) == hipSuccess) {
-                if (e->graphs.size() == kEngGraphs) {
-                    auto lru = std::min_element(e->graphs.begin(), e->graphs.end(),
-                                                [](const EngGraph &u, const EngGraph &v) { return u.last_use < v.last_use; });
-                    eng_drop_graph(*lru);
-                    e->graphs.erase(lru);
-                }
-                e->graphs.push_back(EngGraph{key, g, x, 0});
-                hit = &e->graphs.back();
-            } else {
-                if (g)
-                    (void)hipGraphDestroy(g);
-                eng_drop_graphs(e);
-                e->use_graph = false;  // replay is an optimisation: fall back to direct launches
-                (void)hipGetLastError();
-            }
-        } else if (!hit) {
-            if (e->seen.size() < kEngSeen)
-                e->seen.push_back(key);
-            else
-                e->seen[e->seen_next] = key;
-            e->seen_next = (e->seen_next + 1) % kEngSeen;
-        }
-        if (hit)
-            hit->last_use = ++e->clock;
-    }
+    // format, tuned grids) is in the keys
+    const EngKey front_key = {nq | (uint64_t)p.sort_passes << 32, rows, total, p.used_mask,
+                              (uint64_t)ctx->tune_gen << 8, e->gen};
     // pinned staging: the copies run asynchronously, ahead of the chain on the
     // same stream (the step ends with a stream synchronisation before the
     // staging is written again)
@@ -1304,61 +1375,92 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
         std::memcpy(e->h_same, same.data(), same.size() * 4);
         BCMC_HIP_CHECK(hipMemcpyAsync(e->same_list, e->h_same, same.size() * 4, hipMemcpyHostToDevice, st));
     }
-    if (hit)
-        BCMC_HIP_CHECK(hipGraphLaunch(hit->exec, st));
-    else
-        BCMC_TRY(eng_enqueue_step(e, p));
+    bool replayed = false;
+    BCMC_TRY(eng_run_part(e, front_key, [&]() { return eng_enqueue_step(e, p); }, &replayed));
     ++e->steps;
-    e->graph_steps += hit ? 1 : 0;
+    e->graph_steps += replayed ? 1 : 0;
     BCMC_HIP_CHECK(hipMemcpyAsync(e->ch, e->c, sizeof(EngCounters), hipMemcpyDeviceToHost, st));
     const unsigned long long *qv = e->h_vfirst;
     const uint32_t *qa = e->h_act;
     BCMC_HIP_CHECK(hipMemcpyAsync(e->h_vfirst, e->q_vfirst, nq * 8, hipMemcpyDeviceToHost, st));
     BCMC_HIP_CHECK(hipMemcpyAsync(e->h_act, e->q_act_start, nq * 4, hipMemcpyDeviceToHost, st));
+    BCMC_HIP_CHECK(hipMemcpyAsync(e->h_chunk_base, e->q_chunk_base, nq * 4, hipMemcpyDeviceToHost, st));
     BCMC_HIP_CHECK(hipStreamSynchronize(st));
     if (e->ch->short_slots) {
         set_error("decoded-block pool too small for this step: %u blocks found no slot",
                   e->ch->short_slots);
         return BCMC_ERR_NOMEM;
     }
+    // each query's vertex range: from its first vertex to the next non-empty
+    // query's (or the step total)
     const uint64_t nv = e->ch->num_verts;
-    if (nv > e->vert_cap) {
-        const uint64_t want = std::max<uint64_t>(nv, e->vert_cap * 2);
+    std::vector<uint64_t> vstart(nq + 1);
+    vstart[nq] = nv;
+    for (uint32_t i = nq; i-- > 0;)
+        vstart[i] = qa[i] == kNoPairs ? vstart[i + 1] : qv[i];
+    // admission by the output budget: the longest prefix of queries whose
+    // vertices fit
+    const uint32_t vb = vbytes(e->fmt);
+    uint32_t nadm = nq;
+    while (nadm > 0 && vstart[nadm] * vb > budget)
+        --nadm;
+    if (nadm == 0) {
+        set_error("output budget of %llu bytes is smaller than one query's %llu vertices",
+                  (unsigned long long)budget, (unsigned long long)vstart[1]);
+        return BCMC_ERR_NOMEM;
+    }
+    const uint64_t nv_adm = vstart[nadm];
+    // the vertex buffer, sized from the exact counts
+    if (nv_adm > e->vert_cap) {
+        const uint64_t want = std::max<uint64_t>(nv_adm, e->vert_cap + e->vert_cap / 2);
         eng_changed(e);
         eng_release(e->verts);
         e->vert_cap = 0;
-        BCMC_TRY(dev_alloc(&e->verts, want * vbytes(e->fmt)));
+        BCMC_TRY(dev_alloc(&e->verts, want * vb));
         e->vert_cap = want;
-        BCMC_TRY(eng_launch_emit(e, nq, total));
-        BCMC_HIP_CHECK(hipStreamSynchronize(st));
     }
-    // a query's vertices run from its first active pair's offset to the next
-    // non-empty query's (or the step total)
-    e->last.resize(nq);
-    // q_act_start[i]: index of the query's first active pair (kNoPairs: none)
-    uint32_t next_a = e->ch->num_active;
-    uint64_t next = nv;
-    for (uint32_t i = nq; i-- > 0;) {
+    e->ch->emit_chunks = nadm == nq ? e->ch->num_chunks : e->h_chunk_base[nadm];
+    BCMC_HIP_CHECK(hipMemcpyAsync(&e->c->emit_chunks, &e->ch->emit_chunks, 4, hipMemcpyHostToDevice, st));
+    const EngKey back_key = {nq | 1ull << 40, total, 0, 0, (uint64_t)e->fmt | ctx->tune_gen << 8, e->gen};
+    BCMC_TRY(eng_run_part(e, back_key, [&]() { return eng_launch_emit(e, nq, total); }, &replayed));
+    BCMC_HIP_CHECK(hipStreamSynchronize(st));
+    e->last.resize(nadm);
+    const uint32_t na = e->ch->num_active;
+    for (uint32_t i = 0; i < nadm; ++i) {
         bcmc_query_result r;
-        if (qa[i] == kNoPairs || qa[i] == next_a) {
-            r.num_active_blocks = 0;
-            r.vertex_offset = next;
-        } else {
-            r.num_active_blocks = next_a - qa[i];
-            r.vertex_offset = qv[i];
-            next_a = qa[i];
-        }
-        r.num_vertices = next - r.vertex_offset;
+        // active pairs run from the query's first to the next non-empty query's
+        uint32_t next_a = na;
+        for (uint32_t j = i + 1; j < nq; ++j)
+            if (qa[j] != kNoPairs) {
+                next_a = qa[j];
+                break;
+            }
+        r.num_active_blocks = qa[i] == kNoPairs ? 0u : next_a - qa[i];
+        r.vertex_offset = vstart[i];
+        r.num_vertices = vstart[i + 1] - vstart[i];
         r.step_decoded_blocks = e->ch->num_miss;
-        next = r.vertex_offset;
         e->last[i] = r;
         if (results)
             results[i] = r;
     }
     e->vert_view.ctx = ctx;
     e->vert_view.ptr = e->verts;
-    e->vert_view.bytes = nv * vbytes(e->fmt);
+    e->vert_view.bytes = nv_adm * vb;
     e->vert_view.owned = false;
+    *n_done = nadm;
+    return BCMC_OK;
+}
+
+extern "C" {
+
+BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const float *isovalues,
+                             uint32_t nq, bcmc_query_result *results) {
+    uint32_t done = 0;
+    BCMC_TRY(eng_run(e, volume_ids, isovalues, nq, ~0ull, &done, results));
+    if (done != nq) {
+        set_error("step output exceeds the address space");
+        return BCMC_ERR_NOMEM;
+    }
     return BCMC_OK;
 }
 
@@ -1389,8 +1491,9 @@ BCMC_API int bcmc_engine_stats(bcmc_engine *e, bcmc_engine_info *info) {
 // ---------------------------------------------------------------------------
 // Requests arrive at given times (seconds from the start of the call).  The
 // loop keeps one step in flight: every request that has arrived by the time a
-// step starts joins it (FIFO, up to max_batch and the output-vertex budget);
-// requests arriving while a step runs wait for the next one.  Latency is
+// step starts joins it (FIFO, up to max_batch, the pool's free slots and the
+// output-vertex budget); requests arriving while a step runs, or left out by
+// a bound, wait for the next one.  Latency is
 // completion time - arrival time.  The decoded-block pool (the engine's LRU)
 // and the compressed volumes stay resident across requests.
 extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s,
@@ -1403,7 +1506,6 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
     const uint32_t max_batch = opts && opts->max_batch ? opts->max_batch : 256;
     const uint64_t vert_budget = opts && opts->vertex_budget_bytes ? opts->vertex_budget_bytes
                                                                     : (uint64_t)64 << 30;
-    const uint32_t vb = vbytes(e->fmt);
     // capacity planning before the clock starts: a step takes consecutive
     // requests (FIFO), at most max_batch of them, so the largest window of
     // max_batch consecutive requests bounds every step's block and row counts
@@ -1432,19 +1534,30 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
         BCMC_TRY(eng_ensure_queries(e, nq));
         BCMC_TRY(eng_ensure_chunks(e, eng_max_chunks(best_b, nq)));
     }
-    // per-volume running estimate of vertices per query (largest seen)
-    std::vector<uint64_t> est(e->vols.size(), 0);
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     auto now_s = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
+    // Pool admission: a step's requests may touch at most the pool's slots
+    // minus those the previous step stamped (which its successor may not
+    // evict), counted by the block bound of each request
+    // (eng_block_bound: blocks in row segments whose range holds the
+    // isovalue).  Output admission: the step's run admits the longest prefix
+    // whose exact vertex counts fit the budget; the rest wait at the head of
+    // the queue for the next step.
+    std::vector<uint64_t> bound(n);
+    for (uint32_t i = 0; i < n; ++i)
+        bound[i] = eng_block_bound(e->vols[volume_ids[i]], isovalues[i]);
+    const uint64_t pool = e->pool_cap;
+    uint64_t prev_blocks = 0;
     uint32_t next = 0, done = 0, steps = 0;
     std::vector<uint32_t> pending;
     size_t head = 0;
     std::vector<uint32_t> ids;
     std::vector<float> isos;
+    std::vector<uint32_t> batch;
     std::vector<bcmc_query_result> res;
     while (done < n) {
-        double t = now_s();
+        const double t = now_s();
         while (next < n && arrival_s[next] <= t)
             pending.push_back(next++);
         if (head == pending.size()) {
@@ -1454,40 +1567,45 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
                 std::this_thread::sleep_for(std::chrono::duration<double>(wait));
             continue;
         }
-        // admission: FIFO up to max_batch, the vertex budget (estimate) and the
-        // per-step block limit of the concatenated block arrays
+        if (bound[pending[head]] + prev_blocks > pool && prev_blocks > 0) {
+            // the head request needs the previous step's slots: one empty
+            // step makes them evictable
+            BCMC_TRY(eng_idle_step(e));
+            prev_blocks = 0;
+        }
         ids.clear();
         isos.clear();
-        std::vector<uint32_t> batch;
-        uint64_t bytes = 0, blocks = 0;
-        while (head < pending.size() && batch.size() < max_batch) {
-            const uint32_t r = pending[head];
-            BCMC_CHECK_ARG(volume_ids[r] < e->vols.size(), "unknown volume id");
-            const uint64_t need = est[volume_ids[r]] * vb, nblk = e->vols[volume_ids[r]].n;
-            if (!batch.empty() && (bytes + need > vert_budget || blocks + nblk > kMaxStepBlocks))
+        batch.clear();
+        uint64_t blocks = 0, step_blocks = 0;
+        for (size_t h = head; h < pending.size() && batch.size() < max_batch; ++h) {
+            const uint32_t r = pending[h];
+            const uint64_t nblk = e->vols[volume_ids[r]].n;
+            if (!batch.empty() && (blocks + bound[r] + prev_blocks > pool || step_blocks + nblk > kMaxStepBlocks))
                 break;
-            bytes += need;
-            blocks += nblk;
+            blocks += bound[r];
+            step_blocks += nblk;
             batch.push_back(r);
             ids.push_back(volume_ids[r]);
             isos.push_back(isovalues[r]);
-            ++head;
         }
         res.resize(batch.size());
         const double start = now_s();
-        BCMC_TRY(bcmc_engine_run(e, ids.data(), isos.data(), (uint32_t)batch.size(), res.data()));
+        uint32_t ndone = 0;
+        BCMC_TRY(eng_run(e, ids.data(), isos.data(), (uint32_t)batch.size(), vert_budget, &ndone, res.data()));
         const double end = now_s();
-        for (size_t k = 0; k < batch.size(); ++k) {
+        // every request of the batch stamped its blocks, the deferred ones too
+        prev_blocks = blocks;
+        for (uint32_t k = 0; k < ndone; ++k) {
             const uint32_t r = batch[k];
             out[r].arrival_s = arrival_s[r];
             out[r].start_s = start;
             out[r].finish_s = end;
             out[r].num_vertices = res[k].num_vertices;
-            out[r].batch_size = (uint32_t)batch.size();
+            out[r].batch_size = ndone;
             out[r].step = steps;
-            est[ids[k]] = std::max<uint64_t>(est[ids[k]], res[k].num_vertices);
         }
-        done += (uint32_t)batch.size();
+        head += ndone;
+        done += ndone;
         ++steps;
     }
     return BCMC_OK;
