@@ -141,9 +141,15 @@ def served_bench(bcmc_amd, synth, dev, args):
     eng = bcmc_amd.Engine(dev, min(args.served_pool_gib << 30, 2 * decoded))
     for s, rate, dims in vols:
         eng.add_volume(s, rate, dims)
-    budget = 200 << 30
-    eng.serve(np.zeros(len(vols)), np.arange(len(vols), dtype=np.uint32),
-              np.full(len(vols), 0.5, np.float32), vertex_budget_bytes=budget)  # warm
+    # admission by a 200 GB HBM budget for decoded blocks and output
+    # vertices: the pool takes its share, the vertex budget the rest
+    hbm_budget = 200 * 10**9
+    budget = hbm_budget - eng.stats()["pool_bytes"]
+    # warm-up: a saturation pass over requests of their own draw (buffer
+    # growth, graph capture and first decodes happen here, not in the timed
+    # trace)
+    wid, wiso = bcmc_amd.request_trace(np.random.default_rng(99), args.served_batch * 4, len(vols))
+    eng.serve(np.zeros(len(wid)), wid, wiso, max_batch=args.served_batch, vertex_budget_bytes=budget)
     dev.sync()
     sat = eng.serve(np.zeros(len(ids)), ids, isos, max_batch=args.served_batch,
                     vertex_budget_bytes=budget)
@@ -165,21 +171,45 @@ def served_bench(bcmc_amd, synth, dev, args):
            "counts_equal_plain": exact, "steps": int(sat["step"].max()) + 1,
            "vertices_total": int(nv_plain.sum()),
            "mean_batch": round(float(sat["batch_size"].mean()), 1),
-           "hbm_budget_bytes": budget, "pool_bytes": eng.stats()["pool_bytes"], "poisson": []}
+           "hbm_budget_bytes": hbm_budget, "vertex_budget_bytes": budget,
+           "pool_bytes": eng.stats()["pool_bytes"], "max_batch": args.served_batch, "poisson": []}
+    st = eng.stats()
+    out["engine_steps"], out["graph_replayed_steps"] = st["steps"], st["graph_steps"]
+    del eng
+    # Poisson legs: fresh requests (their own draws, disjoint from the
+    # saturation trace), on an engine whose pool holds what the plain
+    # replay's caches hold (25% of every volume's blocks), cold at the first
+    # leg, so that requests miss and evict as they would in service
+    pool25 = sum(max(32, int(0.25 * int(np.prod([(d + 3) // 4 for d in v[2]])))) for v in vols) * 256
+    eng = bcmc_amd.Engine(dev, pool25)
+    for s, rate, dims in vols:
+        eng.add_volume(s, rate, dims)
+    out["poisson_pool_bytes"] = eng.stats()["pool_bytes"]
+    budget = hbm_budget - out["poisson_pool_bytes"]
+    leg_rng = np.random.default_rng(77)
     for rate in args.served_rates:
-        n = min(len(ids), max(64, int(rate * args.served_seconds)))
-        arr = np.cumsum(rng.exponential(1.0 / rate, n))
-        r = eng.serve(arr, ids[:n], isos[:n], max_batch=args.served_batch, vertex_budget_bytes=budget)
+        n = min(args.served_queries, max(64, int(rate * args.served_seconds)))
+        lid, liso = bcmc_amd.request_trace(leg_rng, n, len(vols))
+        arr = np.cumsum(leg_rng.exponential(1.0 / rate, n))
+        r = eng.serve(arr, lid, liso, max_batch=args.served_batch, vertex_budget_bytes=budget)
         lat = r["finish_s"] - r["arrival_s"]
         span = float(r["finish_s"].max() - arr[0])
         out["poisson"].append({"rate_rps": rate, "queries": n, "qps": round(n / span, 2),
                                "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
                                "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
-                               "mean_batch": round(float(r["batch_size"].mean()), 2)})
+                               "mean_batch": round(float(r["batch_size"].mean()), 2),
+                               "steps": int(r["step"].max()) + 1})
         log(f"bench: served poisson {rate} req/s: p50 {out['poisson'][-1]['p50_ms']} ms "
-            f"p99 {out['poisson'][-1]['p99_ms']} ms")
-    st = eng.stats()
-    out["engine_steps"], out["graph_replayed_steps"] = st["steps"], st["graph_steps"]
+            f"p99 {out['poisson'][-1]['p99_ms']} ms, mean batch {out['poisson'][-1]['mean_batch']}")
+    # the saturation trace again, through the pool of the plain replay's size
+    sat = eng.serve(np.zeros(len(ids)), ids, isos, max_batch=args.served_batch,
+                    vertex_budget_bytes=budget)
+    q25 = len(ids) / float(sat["finish_s"].max())
+    out["pool25"] = {"served_qps": round(q25, 2), "served_vs_plain": round(q25 / plain_qps, 3),
+                     "steps": int(sat["step"].max()) + 1,
+                     "counts_equal_plain": bool(np.array_equal(sat["num_vertices"].astype(np.uint64),
+                                                               nv_plain))}
+    log(f"bench: served saturation, pool of the plain caches' size: {q25:.1f} q/s")
     del eng
     return out
 
@@ -230,9 +260,9 @@ def main():
                     help="also run one device copy kernel of this size (HBM counter calibration)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--served-queries", type=int, default=2048)
-    ap.add_argument("--served-batch", type=int, default=64)
+    ap.add_argument("--served-batch", type=int, default=256)
     ap.add_argument("--served-pool-gib", type=int, default=32)
-    ap.add_argument("--served-rates", type=float, nargs="*", default=[10, 100, 300, 1000, 2000])
+    ap.add_argument("--served-rates", type=float, nargs="*", default=[10, 30, 100, 300, 1000, 2000, 5000, 10000])
     ap.add_argument("--served-seconds", type=float, default=1.0)
     ap.add_argument("--batched-reps", type=int, default=5)
     ap.add_argument("--no-served", action="store_true")

@@ -109,6 +109,18 @@ inline uint32_t persistent_grid(const bcmc_ctx *ctx, uint32_t per_cu) {
     return (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * per_cu;
 }
 
+// Workgroups of `kernel` that one CU holds at once (its registers and LDS);
+// a persistent grid larger than that would leave workgroups waiting for
+// others to finish.  Callers cache it in a function-local static.
+inline uint32_t resident_per_cu(const void *kernel, int threads) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess || n < 1) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return (uint32_t)n;
+}
+
 }  // namespace bcmc
 
 // ---------------------------------------------------------------------------

@@ -880,8 +880,11 @@ static EngMcArgs eng_mc_args(bcmc_engine *e, uint32_t nq) {
 static uint64_t eng_max_chunks(uint64_t total, uint32_t nq) { return div_up(total, kChunk) + nq; }
 
 static int eng_launch_emit(bcmc_engine *e, uint32_t nq, uint64_t total) {
+    static const uint32_t resident =
+        resident_per_cu((const void *)eng_emit_kernel<BCMC_VERTS_LOCAL_F32>, kEngThreads);
     const uint32_t grid = (uint32_t)std::max<uint64_t>(
-        1, std::min<uint64_t>(eng_max_chunks(total, nq), persistent_grid(e->ctx, e->ctx->tune.mc)));
+        1, std::min<uint64_t>(eng_max_chunks(total, nq),
+                              persistent_grid(e->ctx, std::min(e->ctx->tune.mc, resident))));
     hipStream_t st = e->ctx->stream;
     if (e->fmt == BCMC_VERTS_LOCAL_F32)
         eng_emit_kernel<BCMC_VERTS_LOCAL_F32><<<grid, kEngThreads, 0, st>>>(eng_mc_args(e, nq));
@@ -915,8 +918,9 @@ static int eng_ensure_chunks(bcmc_engine *e, uint64_t max_chunks) {
 static int eng_launch_count(bcmc_engine *e, uint32_t nq, uint64_t total) {
     const uint64_t max_chunks = eng_max_chunks(total, nq);
     const EngMcArgs a = eng_mc_args(e, nq);
+    static const uint32_t resident = resident_per_cu((const void *)eng_count_kernel, kEngThreads);
     const uint32_t grid = (uint32_t)std::max<uint64_t>(
-        1, std::min<uint64_t>(max_chunks, persistent_grid(e->ctx, e->ctx->tune.count)));
+        1, std::min<uint64_t>(max_chunks, persistent_grid(e->ctx, std::min(e->ctx->tune.count, resident))));
     eng_count_kernel<<<grid, kEngThreads, 0, e->ctx->stream>>>(a);
     launch_scan<1>(e->ctx, e->site_chunks, e->site_chunks,
                    EngChunkPrefixOp{a.chunk_verts, a.chunk_verts_pre, a.c}, max_chunks);
@@ -1556,6 +1560,7 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
     std::vector<float> isos;
     std::vector<uint32_t> batch;
     std::vector<bcmc_query_result> res;
+    std::vector<uint64_t> vsum(e->vols.size());
     while (done < n) {
         const double t = now_s();
         while (next < n && arrival_s[next] <= t)
@@ -1576,13 +1581,17 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
         ids.clear();
         isos.clear();
         batch.clear();
+        // a volume's requests together touch at most all of its blocks
+        std::fill(vsum.begin(), vsum.end(), 0);
         uint64_t blocks = 0, step_blocks = 0;
         for (size_t h = head; h < pending.size() && batch.size() < max_batch; ++h) {
-            const uint32_t r = pending[h];
-            const uint64_t nblk = e->vols[volume_ids[r]].n;
-            if (!batch.empty() && (blocks + bound[r] + prev_blocks > pool || step_blocks + nblk > kMaxStepBlocks))
+            const uint32_t r = pending[h], v = volume_ids[r];
+            const uint64_t nblk = e->vols[v].n;
+            const uint64_t grown = blocks - std::min(nblk, vsum[v]) + std::min(nblk, vsum[v] + bound[r]);
+            if (!batch.empty() && (grown + prev_blocks > pool || step_blocks + nblk > kMaxStepBlocks))
                 break;
-            blocks += bound[r];
+            blocks = grown;
+            vsum[v] += bound[r];
             step_blocks += nblk;
             batch.push_back(r);
             ids.push_back(volume_ids[r]);

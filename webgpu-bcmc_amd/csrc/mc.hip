@@ -375,8 +375,11 @@ static McArgs mc_args(bcmc_mc *m) {
 }
 
 static int mc_launch_emit(bcmc_mc *m) {
-    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(m->total, kChunk),
-                                                                   persistent_grid(m->ctx, m->ctx->tune.mc)));
+    static const uint32_t resident =
+        resident_per_cu((const void *)mc_emit_kernel<BCMC_VERTS_LOCAL_F32>, kMcThreads);
+    const uint32_t grid = std::max<uint32_t>(
+        1, std::min<uint32_t>(div_up(m->total, kChunk),
+                              persistent_grid(m->ctx, std::min(m->ctx->tune.mc, resident))));
     hipStream_t st = m->ctx->stream;
     if (m->fmt == BCMC_VERTS_LOCAL_F32)
         mc_emit_kernel<BCMC_VERTS_LOCAL_F32><<<grid, kMcThreads, 0, st>>>(mc_args(m));
@@ -393,8 +396,9 @@ static int mc_launch_emit(bcmc_mc *m) {
 static int mc_launch_pass(bcmc_mc *m, hipEvent_t *ev) {
     const McArgs a = mc_args(m);
     const uint32_t max_chunks = div_up(m->total, kChunk);
-    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(max_chunks,
-                                                                   persistent_grid(m->ctx, m->ctx->tune.count)));
+    static const uint32_t resident = resident_per_cu((const void *)mc_count_kernel, kMcThreads);
+    const uint32_t grid = std::max<uint32_t>(
+        1, std::min<uint32_t>(max_chunks, persistent_grid(m->ctx, std::min(m->ctx->tune.count, resident))));
     mc_count_kernel<<<grid, kMcThreads, 0, m->ctx->stream>>>(a);
     if (ev)
         BCMC_HIP_CHECK(hipEventRecord(ev[0], m->ctx->stream));
