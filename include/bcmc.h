@@ -315,7 +315,7 @@ BCMC_API double bcmc_ctx_last_kernel_ms(bcmc_ctx *ctx);
  * the marching-cubes emit and count launches.  No reference counterpart (WebGPU dispatch
  * sizes are fixed in compressed_marching_cubes.js / exclusive_scan.js /
  * radix_sort.js); results are identical for every shape, only speed changes.
- * Defaults: scan 4, sort 4, decode 8, mc (emit) 5, count 8.  The *_ms fields are filled by
+ * Defaults: scan 2, sort 8, decode 8, mc (emit) 5, count 8; tiles 16, 16, 256.  The *_ms fields are filled by
  * bcmc_ctx_autotune (device time of the chosen shape on its probe workload)
  * and ignored by bcmc_ctx_set_tuning. */
 typedef struct bcmc_tuning {
@@ -328,11 +328,18 @@ typedef struct bcmc_tuning {
     float sort_ms;   /* 2M pairs, 24-bit keys (3 digit passes) */
     float decode_ms; /* 256K blocks of a synthetic rate-4 stream into a block cache */
     float mc_ms;     /* not measured: the MC launches keep their occupancy-derived shapes */
+    /* Tile shapes (replace the reference's build-time BLOCK_SIZE and
+     * SORT_CHUNK_SIZE, shaders/embed_shaders.py:30-31); 0 in
+     * bcmc_ctx_set_tuning keeps the current one. */
+    uint32_t scan_items;     /* scan tile = 512 x scan_items elements: 8, 16 or 32 */
+    uint32_t sort_items;     /* radix-sort tile = 256 x sort_items: 8, 16 or 32 */
+    uint32_t decode_threads; /* zfp decoder blocks per workgroup: 64, 128 or 256 */
 } bcmc_tuning;
 BCMC_API int bcmc_ctx_get_tuning(bcmc_ctx *ctx, bcmc_tuning *out);
 /* Each *_wg_per_cu must be in 1..64; pipelines re-capture their graphs. */
 BCMC_API int bcmc_ctx_set_tuning(bcmc_ctx *ctx, const bcmc_tuning *tuning);
-/* Times the scan, sort and decode candidates on synthetic device data
+/* Times the scan, sort and decode candidates (tile shape x workgroups per
+ * CU) on synthetic device data
  * (about 200 MiB of temporary device memory, well under a second) and keeps
  * the fastest shape of each family in the context; out (nullable) receives
  * the result.  Call it before the first query: it synchronises the stream. */

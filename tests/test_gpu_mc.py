@@ -154,11 +154,19 @@ def test_launch_shapes_leave_results_unchanged(dev):
     f = synth.field(synth.FOURIER, 2, dims, 128, (1, 24, 0, 0))
     s = synth.encode(f, 64 * rate)
     n = oracle.nblocks(dims)
-    for shape in (dict(scan=1, sort=1, decode=1, mc=1, count=1),
-                  dict(scan=16, sort=8, decode=32, mc=8, count=16)):
+    assert t["tiles"]["scan_items"] in (8, 16, 32) and t["tiles"]["sort_items"] in (8, 16, 32)
+    assert t["tiles"]["decode_threads"] in (64, 128, 256)
+    with pytest.raises(bcmc_amd.BcmcError):
+        d.set_tuning(scan_items=12)
+    with pytest.raises(bcmc_amd.BcmcError):
+        d.set_tuning(decode_threads=96)
+    for shape in (dict(scan=1, sort=1, decode=1, mc=1, count=1, scan_items=8, sort_items=8, decode_threads=64),
+                  dict(scan=16, sort=8, decode=32, mc=8, count=16, scan_items=32, sort_items=32,
+                       decode_threads=256)):
         say("mc launch shape", shape)
         d.set_tuning(**shape)
-        assert {k: d.tuning()[k] for k in shape} == shape
+        got = d.tuning()
+        assert {k: got[k] if k in got else got["tiles"][k] for k in shape} == shape
         mc = bcmc_amd.CompressedMarchingCubes(d)
         mc.setCompressedVolume(s, rate, dims, (1, 1, 1), n // 4)
         ref = oracle.MC(s, 64 * rate, dims, initial_cache=n // 4)
@@ -166,7 +174,7 @@ def test_launch_shapes_leave_results_unchanged(dev):
             _check_query(mc, ref, iso)
             v = mc.vertices()
             assert np.array_equal(v.view(np.uint32), ref.results()["verts_f32"].view(np.uint32))
-    d.set_tuning(**{k: t[k] for k in ("scan", "sort", "decode", "mc", "count")})
+    d.set_tuning(**{k: t[k] for k in ("scan", "sort", "decode", "mc", "count")}, **t["tiles"])
 
 
 def test_world_positions_follow_the_renderer_formula(dev):

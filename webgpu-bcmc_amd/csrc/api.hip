@@ -76,8 +76,9 @@ int make_buf(bcmc_ctx *ctx, uint64_t bytes, bcmc_buf **out) {
     return BCMC_OK;
 }
 
-int scan_site_alloc(ScanSite *s, uint64_t max_elems, uint32_t tile) {
-    s->max_tiles = std::max<uint32_t>(1, div_up(max_elems, tile));
+int scan_site_alloc(ScanSite *s, uint64_t max_elems) {
+    // one status word per tile of the smallest shape
+    s->max_tiles = std::max<uint32_t>(1, div_up(max_elems, kScanTileMin));
     BCMC_TRY(dev_alloc((void **)&s->ctrl, 16));
     BCMC_TRY(dev_alloc((void **)&s->status, (size_t)s->max_tiles * 8));
     BCMC_HIP_CHECK(hipMemset(s->ctrl, 0, 16));
@@ -206,7 +207,8 @@ __global__ void world_positions_kernel(const float4 *__restrict__ v, uint64_t n,
 // device copy: host code uses the address hipGetSymbolAddress returns for the
 // current device (noalloc_setup), kept per device.
 constexpr uint32_t kNoAllocTiles = 131072;
-constexpr uint64_t kNoAllocMaxElems = (uint64_t)kNoAllocTiles * bcmc::kScanTile;  // 2^30
+constexpr int kNoAllocItems = 16;  // its tile shape, whatever the context's tuning
+constexpr uint64_t kNoAllocMaxElems = (uint64_t)kNoAllocTiles * bcmc::kScanThreads * kNoAllocItems;  // 2^30
 constexpr size_t kNoAllocTotalOff = 16, kNoAllocStatusOff = 256;
 constexpr size_t kNoAllocBytes = kNoAllocStatusOff + (size_t)kNoAllocTiles * 8;  // 1 MiB + 256 B
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
@@ -268,8 +270,8 @@ static int ensure_site(bcmc_ctx *ctx, CtxScratch *s, uint64_t elems) {
         return BCMC_OK;
     BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     scan_site_free(&s->site);
-    BCMC_TRY(scan_site_alloc(&s->site, std::max<uint64_t>(elems, kScanTile), kScanTile));
-    s->site_elems = std::max<uint64_t>(elems, kScanTile);
+    BCMC_TRY(scan_site_alloc(&s->site, std::max<uint64_t>(elems, kScanTileMax)));
+    s->site_elems = std::max<uint64_t>(elems, kScanTileMax);
     return BCMC_OK;
 }
 
@@ -514,7 +516,7 @@ BCMC_API int bcmc_scan_exclusive_u32_noalloc(bcmc_ctx *ctx, bcmc_buf *data, uint
     site.max_tiles = kNoAllocTiles;
     unsigned long long *tot = (unsigned long long *)((char *)p + kNoAllocTotalOff);
     InplaceScanOp op{(uint32_t *)data->ptr, n, tot};
-    launch_scan<1>(ctx, site, site, op, n);
+    launch_scan_items<1, kNoAllocItems>(ctx, site, site, op, n);
     if (n < aligned_n)
         write_total_kernel<<<1, 1, 0, ctx->stream>>>((uint32_t *)data->ptr, n, tot);
     BCMC_HIP_CHECK(hipGetLastError());

@@ -62,6 +62,10 @@ struct bcmc_ctx {
     // (bcmc_ctx_autotune measures and sets them; results never depend on them)
     struct {
         uint32_t scan = 2, sort = 8, decode = 8, mc = 5, count = 8;
+        // tile shapes: elements per thread of the scan (tile = 512 x items)
+        // and of the radix sort (tile = 256 x items), blocks per workgroup
+        // of the zfp decoders (one block per lane)
+        uint32_t scan_items = 16, sort_items = 16, decode_threads = 256;
     } tune;
     // bumped on every change of `tune`: captured graphs keyed on it re-capture
     uint64_t tune_gen = 0;
@@ -95,12 +99,14 @@ struct ScanSite {
     uint32_t max_tiles = 0;
 };
 
-int scan_site_alloc(ScanSite *s, uint64_t max_elems, uint32_t tile);
+// Look-back state for scans of up to max_elems elements at any tile shape.
+int scan_site_alloc(ScanSite *s, uint64_t max_elems);
 void scan_site_free(ScanSite *s);
 
+// Scan tiles: 512 threads x 8, 16 or 32 elements (4096, 8192, 16384).
 constexpr int kScanThreads = 512;
-constexpr int kScanItems = 16;
-constexpr uint32_t kScanTile = kScanThreads * kScanItems;  // 8192 elements
+constexpr uint32_t kScanTileMin = kScanThreads * 8, kScanTileMax = kScanThreads * 32;
+inline uint32_t scan_tile(const bcmc_ctx *ctx) { return kScanThreads * ctx->tune.scan_items; }
 
 __host__ __device__ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 

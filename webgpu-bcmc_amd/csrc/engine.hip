@@ -803,7 +803,7 @@ static int eng_ensure_rows(bcmc_engine *e, uint64_t rows, uint64_t row_elems) {
         eng_release(e->cand_rows);
         scan_site_free(&e->site_rows);
         BCMC_TRY(dev_alloc((void **)&e->cand_rows, cap * sizeof(CandRow)));
-        BCMC_TRY(scan_site_alloc(&e->site_rows, cap, kScanTile));
+        BCMC_TRY(scan_site_alloc(&e->site_rows, cap));
         e->rows_cap = cap;
     }
     if (row_elems > e->row_elems_cap) {
@@ -812,7 +812,7 @@ static int eng_ensure_rows(bcmc_engine *e, uint64_t rows, uint64_t row_elems) {
         const uint64_t cap = std::max<uint64_t>(row_elems, e->row_elems_cap * 3 / 2);
         e->row_elems_cap = 0;
         scan_site_free(&e->site_active);
-        BCMC_TRY(scan_site_alloc(&e->site_active, cap, kScanTile));
+        BCMC_TRY(scan_site_alloc(&e->site_active, cap));
         e->row_elems_cap = cap;
     }
     return BCMC_OK;
@@ -907,7 +907,7 @@ static int eng_ensure_chunks(bcmc_engine *e, uint64_t max_chunks) {
         scan_site_free(&e->site_chunks);
         BCMC_TRY(dev_alloc((void **)&e->chunk_verts, cap * 4));
         BCMC_TRY(dev_alloc((void **)&e->chunk_verts_pre, cap * 8));
-        BCMC_TRY(scan_site_alloc(&e->site_chunks, cap, kScanTile));
+        BCMC_TRY(scan_site_alloc(&e->site_chunks, cap));
         e->chunks_cap = cap;
     }
     return BCMC_OK;
@@ -952,7 +952,7 @@ static int eng_enqueue_step(bcmc_engine *e, const EngPlan &p) {
     launch_scan<1>(ctx, e->site_rows, e->site_rows,
                    CandSegOp{e->d_q, e->tile_q, e->cand_rows, e->c, (uint32_t)p.rows}, p.rows);
     SegActiveOp aop{e->cand_rows, e->flags, e->act_q, e->act_b, e->q_act_start, e->c};
-    launch_scan<1>(ctx, e->site_active, e->site_active, aop, p.row_elems);
+    launch_scan<1, SegActiveOp, 16>(ctx, e->site_active, e->site_active, aop, p.row_elems);
     // marching-cubes chunks of every query (also the work units of the mark)
     eng_chunk_table_kernel<<<1, 256, 0, st>>>(e->c, e->q_act_start, nq, e->q_pair_end, e->q_chunk_base);
     eng_mark_kernel<<<(uint32_t)std::max<uint64_t>(
@@ -1011,7 +1011,7 @@ BCMC_API int bcmc_engine_create(bcmc_ctx *ctx, uint64_t pool_bytes, bcmc_engine 
     if (rc == BCMC_OK)
         rc = sort_scratch_alloc(&e->sort, e->pool_cap);
     if (rc == BCMC_OK)
-        rc = scan_site_alloc(&e->site_cand, e->pool_cap, kScanTile);
+        rc = scan_site_alloc(&e->site_cand, e->pool_cap);
     if (rc == BCMC_OK && hipHostMalloc(&e->ch, sizeof(EngCounters), hipHostMallocDefault) != hipSuccess)
         rc = BCMC_ERR_NOMEM;
     if (rc != BCMC_OK) {
@@ -1286,6 +1286,8 @@ static int eng_run(bcmc_engine *e, const uint32_t *volume_ids, const float *isov
     std::vector<std::vector<uint32_t>> earlier(e->vols.size());
     std::vector<uint32_t> same;
     uint64_t total = 0, rows = 0;
+    // every scan tile of the row segments lies in one query
+    const uint32_t tile = scan_tile(ctx);
     for (uint32_t i = 0; i < nq; ++i) {
         BCMC_CHECK_ARG(volume_ids[i] < e->vols.size(), "unknown volume id");
         const EngVolume &v = e->vols[volume_ids[i]];
@@ -1308,11 +1310,11 @@ static int eng_run(bcmc_engine *e, const uint32_t *volume_ids, const float *isov
         earlier[q.vol].push_back(q.flag_base);
         used[volume_ids[i]] = true;
         total += v.n;
-        rows += (uint64_t)div_up(q.nrows, kScanTile) * kScanTile;  // tile-aligned
+        rows += (uint64_t)div_up(q.nrows, tile) * tile;  // tile-aligned
     }
     BCMC_CHECK_ARG(total <= kMaxStepBlocks, "too many blocks in one step");
     const uint64_t row_elems = rows * kSegBlocks;
-    BCMC_CHECK_ARG(row_elems < (1ull << 32) - kScanTile, "too many row segments in one step");
+    BCMC_CHECK_ARG(row_elems < (1ull << 32) - kScanTileMax, "too many row segments in one step");
     BCMC_TRY(eng_ensure_blocks(e, total));
     BCMC_TRY(eng_ensure_rows(e, rows, row_elems));
     BCMC_TRY(eng_ensure_queries(e, nq));
@@ -1328,7 +1330,7 @@ static int eng_run(bcmc_engine *e, const uint32_t *volume_ids, const float *isov
         e->same_cap = cap;
     }
     // query of every scan tile of the row segments (+ a last entry)
-    const uint64_t ntiles = div_up(rows, kScanTile) + 1;
+    const uint64_t ntiles = div_up(rows, tile) + 1;
     if (ntiles > e->tile_q_cap) {
         BCMC_HIP_CHECK(hipStreamSynchronize(st));
         eng_changed(e);
@@ -1341,7 +1343,7 @@ static int eng_run(bcmc_engine *e, const uint32_t *volume_ids, const float *isov
         e->tile_q_cap = cap;
     }
     for (uint64_t t = 0, q = 0; t < ntiles; ++t) {
-        const uint64_t r = std::min<uint64_t>(t * kScanTile, rows ? rows - 1 : 0);
+        const uint64_t r = std::min<uint64_t>(t * tile, rows ? rows - 1 : 0);
         while (q + 1 < nq && qs[q + 1].row_base <= r)
             ++q;
         e->h_tile_q[t] = (uint32_t)q;
@@ -1518,7 +1520,8 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
         auto vol_rows = [&](uint32_t r) -> uint64_t {
             const EngVolume &v = e->vols[volume_ids[r]];
             const uint64_t nrows = (uint64_t)div_up(v.nb[0], kSegBlocks) * v.nb[1] * v.nb[2];
-            return (uint64_t)div_up(nrows, kScanTile) * kScanTile;  // tile-aligned, as in a step
+            const uint32_t tile = scan_tile(e->ctx);
+            return (uint64_t)div_up(nrows, tile) * tile;  // tile-aligned, as in a step
         };
         for (uint32_t i = 0; i < n; ++i) {
             BCMC_CHECK_ARG(volume_ids[i] < e->vols.size(), "unknown volume id");
@@ -1533,7 +1536,7 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
         }
         const uint32_t nq = std::min<uint32_t>(n, max_batch);
         BCMC_TRY(eng_ensure_blocks(e, best_b));
-        if (best_r * kSegBlocks < (1ull << 32) - kScanTile)
+        if (best_r * kSegBlocks < (1ull << 32) - kScanTileMax)
             BCMC_TRY(eng_ensure_rows(e, best_r, best_r * kSegBlocks));
         BCMC_TRY(eng_ensure_queries(e, nq));
         BCMC_TRY(eng_ensure_chunks(e, eng_max_chunks(best_b, nq)));

@@ -41,25 +41,27 @@ struct QState {
 //             const unsigned long long excl[NCH]) const
 //   void total(const unsigned long long t[NCH]) const  (one thread, once)
 // value() only reads memory (no stores) and takes no branch on what it reads,
-// so the loads of all 16 elements of a thread are in flight together; it is
+// so the loads of all ITEMS elements of a thread are in flight together; it is
 // also evaluated for the positions past the end (at element n - 1, with the
 // values then zeroed).  Stores belong in emit(), which the payload feeds.
-// Each wave scans a contiguous 1024-element run of its tile in 16 coalesced
-// rounds with DPP; wave totals combine through LDS; tiles chain through the
-// look-back status words of site[c].
+// Each wave scans a contiguous run of 64 x ITEMS elements of its tile in
+// ITEMS coalesced rounds with DPP; wave totals combine through LDS; tiles
+// chain through the look-back status words of site[c].  Results do not depend
+// on ITEMS (the tile shape the autotuner picks).
 // ---------------------------------------------------------------------------
 struct NoTile {};
 struct NoItem {};
 
-template <int NCH, class Op>
+template <int NCH, class Op, int ITEMS>
 __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanSite s0, ScanSite s1, Op op) {
+    constexpr uint32_t kTile = kScanThreads * ITEMS;
     constexpr int NW = kScanThreads / 64;
     __shared__ uint32_t lds_w[NCH][NW + 1];
     __shared__ LookbackLds<kScanThreads> lds_lb;
     __shared__ uint32_t lds_tile;
     const ScanSite *sites[2] = {&s0, &s1};
     const uint32_t n = op.size();
-    const uint32_t ntiles = div_up(n, kScanTile);
+    const uint32_t ntiles = div_up(n, kTile);
     uint32_t epoch[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c)
@@ -79,17 +81,17 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanSite s0, ScanSit
         if (tile >= ntiles)
             break;
         const typename Op::Tile tc = op.tile(tile);
-        const uint32_t wbase = tile * kScanTile + wave * (64 * kScanItems);
-        uint32_t vals[kScanItems][NCH], excl[kScanItems][NCH], run[NCH];
-        typename Op::Item items[kScanItems];
+        const uint32_t wbase = tile * kTile + wave * (64 * ITEMS);
+        uint32_t vals[ITEMS][NCH], excl[ITEMS][NCH], run[NCH];
+        typename Op::Item items[ITEMS];
         // all loads of the tile first (independent), then the scans
 #pragma unroll
-        for (int j = 0; j < kScanItems; ++j) {
+        for (int j = 0; j < ITEMS; ++j) {
             const uint32_t i = wbase + j * 64 + lane;
             op.value(tc, min(i, n - 1), vals[j], items[j]);
         }
 #pragma unroll
-        for (int j = 0; j < kScanItems; ++j) {
+        for (int j = 0; j < ITEMS; ++j) {
             const bool in = wbase + j * 64 + lane < n;
 #pragma unroll
             for (int c = 0; c < NCH; ++c)
@@ -99,7 +101,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanSite s0, ScanSit
         for (int c = 0; c < NCH; ++c)
             run[c] = 0;
 #pragma unroll
-        for (int j = 0; j < kScanItems; ++j) {
+        for (int j = 0; j < ITEMS; ++j) {
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
                 const uint32_t v = vals[j][c];
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanSite s0, ScanSit
                 prefix[c] = q[c];
         }
 #pragma unroll
-        for (int j = 0; j < kScanItems; ++j) {
+        for (int j = 0; j < ITEMS; ++j) {
             const uint32_t i = wbase + j * 64 + lane;
             if (i < n) {
                 unsigned long long e[NCH];
@@ -164,12 +166,26 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanSite s0, ScanSit
         scan_site_finish(*sites[c], epoch[c]);
 }
 
-template <int NCH, class Op>
+template <int NCH, int ITEMS, class Op>
+inline void launch_scan_items(bcmc_ctx *ctx, const ScanSite &s0, const ScanSite &s1, const Op &op,
+                              uint64_t n_max) {
+    const uint32_t grid = std::max<uint32_t>(
+        1, std::min<uint32_t>(div_up(n_max, kScanThreads * ITEMS), persistent_grid(ctx, ctx->tune.scan)));
+    scan_kernel<NCH, Op, ITEMS><<<grid, kScanThreads, 0, ctx->stream>>>(s0, s1, op);
+}
+
+// The scan at the context's tile shape, at most MAX_ITEMS elements per
+// thread (an Op with a large per-element payload would spill registers at 32).
+template <int NCH, class Op, int MAX_ITEMS = 32>
 inline void launch_scan(bcmc_ctx *ctx, const ScanSite &s0, const ScanSite &s1, const Op &op,
                         uint64_t n_max) {
-    uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(n_max, kScanTile),
-                                                             persistent_grid(ctx, ctx->tune.scan)));
-    scan_kernel<NCH, Op><<<grid, kScanThreads, 0, ctx->stream>>>(s0, s1, op);
+    const uint32_t items = std::min<uint32_t>(ctx->tune.scan_items, MAX_ITEMS);
+    if (items == 8)
+        launch_scan_items<NCH, 8>(ctx, s0, s1, op, n_max);
+    else if (items == 32)
+        launch_scan_items<NCH, (MAX_ITEMS >= 32 ? 32 : 16)>(ctx, s0, s1, op, n_max);
+    else
+        launch_scan_items<NCH, 16>(ctx, s0, s1, op, n_max);
 }
 
 // ---------------------------------------------------------------------------

@@ -211,8 +211,8 @@ int lru_state_alloc(bcmc_ctx *ctx, LruState *L, uint32_t initial, uint32_t elem_
     BCMC_HIP_CHECK(hipMemsetAsync((char *)L->data + cap * elem_bytes, 0, 256, ctx->stream));
     BCMC_TRY(dev_alloc((void **)&L->new_items, (size_t)L->aligned_total * 4 + 4));
     BCMC_TRY(dev_alloc((void **)&L->ctl, 8));
-    BCMC_TRY(scan_site_alloc(&L->site_miss, L->aligned_total, kScanTile));
-    BCMC_TRY(scan_site_alloc(&L->site_avail, cap, kScanTile));
+    BCMC_TRY(scan_site_alloc(&L->site_miss, L->aligned_total));
+    BCMC_TRY(scan_site_alloc(&L->site_avail, cap));
     BCMC_TRY(sort_scratch_alloc(&L->sort, L->capacity));
     L->v.total = total;
     L->v.capacity = L->capacity;

@@ -530,15 +530,15 @@ BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *m, const void *stream, uint6
     BCMC_TRY(dev_alloc((void **)&m->tile, n * 8));
     BCMC_TRY(dev_alloc((void **)&m->with_verts, n * 4));
     BCMC_TRY(dev_alloc((void **)&m->vert_offsets, n * 8));
-    BCMC_TRY(scan_site_alloc(&m->site_active, n, kScanTile));
+    BCMC_TRY(scan_site_alloc(&m->site_active, n));
     const size_t nch = div_up((uint64_t)n, kChunk);
     BCMC_TRY(dev_alloc((void **)&m->counts, n * 4));
     BCMC_TRY(dev_alloc((void **)&m->chunk_with, nch * 4));
     BCMC_TRY(dev_alloc((void **)&m->chunk_verts, nch * 4));
     BCMC_TRY(dev_alloc((void **)&m->chunk_with_pre, nch * 4));
     BCMC_TRY(dev_alloc((void **)&m->chunk_verts_pre, nch * 8));
-    BCMC_TRY(scan_site_alloc(&m->site_c0, nch, kScanTile));
-    BCMC_TRY(scan_site_alloc(&m->site_c1, nch, kScanTile));
+    BCMC_TRY(scan_site_alloc(&m->site_c0, nch));
+    BCMC_TRY(scan_site_alloc(&m->site_c1, nch));
     const uint32_t init = initial_cache_slots ? initial_cache_slots
                                               : (uint32_t)std::ceil((double)total * 0.05);
     BCMC_TRY(lru_state_alloc(ctx, &m->lru.st, init, 256, m->total));

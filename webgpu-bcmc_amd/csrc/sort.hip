@@ -22,15 +22,18 @@
 
 namespace bcmc {
 
+// Tiles of 256 threads x ITEMS elements, ITEMS = 8, 16 or 32 (2048, 4096,
+// 8192 elements; the autotuner picks one).  The sort is stable at every tile
+// shape, so the output does not depend on it.
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
 constexpr int kSortWaves = kSortThreads / 64;
-constexpr uint32_t kSortTile = kSortThreads * kSortItems;  // 4096
+constexpr uint32_t kSortTileMin = kSortThreads * 8;
 
 // Position of item j of a lane in tile t: each wave owns a contiguous run of
-// 64 * kSortItems positions, read in kSortItems coalesced rounds.
+// 64 * ITEMS positions, read in ITEMS coalesced rounds.
+template <int ITEMS>
 BCMC_DEV uint32_t sort_pos(uint32_t tile, uint32_t wave, int j, uint32_t lane) {
-    return tile * kSortTile + wave * (64u * kSortItems) + (uint32_t)j * 64u + lane;
+    return tile * (kSortThreads * ITEMS) + wave * (64u * ITEMS) + (uint32_t)j * 64u + lane;
 }
 
 // Lanes of the wave holding the same digit as this lane (among `valid` ones).
@@ -92,25 +95,25 @@ struct GenericSource {
 // Elements are read for every position of a tile (positions past n read
 // element n - 1 and are ignored), so all loads of a thread are in flight
 // together.
-template <class Src>
+template <class Src, int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void sort_hist_kernel(Src src, uint32_t shift,
                                                                  uint32_t *hist) {
     __shared__ uint32_t h[256];
-    const uint32_t n = src.size(), ntiles = div_up(n, kSortTile);
+    const uint32_t n = src.size(), ntiles = div_up(n, kSortThreads * ITEMS);
     const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
     const uint64_t lt = (1ull << lane) - 1ull;
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        uint32_t key[kSortItems];
+        uint32_t key[ITEMS];
 #pragma unroll
-        for (int j = 0; j < kSortItems; ++j) {
+        for (int j = 0; j < ITEMS; ++j) {
             uint32_t v;
-            src.get(min(sort_pos(tile, wave, j, lane), n - 1), key[j], v);
+            src.get(min(sort_pos<ITEMS>(tile, wave, j, lane), n - 1), key[j], v);
         }
         h[threadIdx.x] = 0;
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < kSortItems; ++j) {
-            const bool valid = sort_pos(tile, wave, j, lane) < n;
+        for (int j = 0; j < ITEMS; ++j) {
+            const bool valid = sort_pos<ITEMS>(tile, wave, j, lane) < n;
             const uint32_t d = (key[j] >> shift) & 255u;
             const uint64_t peers = digit_peers(d, valid);
             if (valid && (peers & lt) == 0)
@@ -126,9 +129,10 @@ struct HistScanOp {
     uint32_t *hist;
     const uint32_t *n_dev;
     uint32_t n_host;
+    uint32_t tile_elems;  // elements per sort tile
     BCMC_DEV uint32_t size() const {
         const uint32_t n = n_dev ? *n_dev : n_host;
-        return 256u * div_up(n, kSortTile);
+        return 256u * div_up(n, tile_elems);
     }
     BCMC_DEV void begin() const {}
     using Tile = NoTile;
@@ -147,28 +151,28 @@ struct HistScanOp {
 // that the digit's lowest lane advances) plus its lower lanes of the same
 // digit in this round; then every digit's start in the tile (from the scanned
 // histogram table) is split among the four waves by their counts.
-template <class Src>
+template <class Src, int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(Src src, uint32_t shift,
                                                                     const uint32_t *hist,
                                                                     uint32_t *okeys, uint32_t *ovals,
                                                                     uint32_t out_skip, uint32_t out_cap) {
     __shared__ uint32_t cnt[kSortWaves][256];
-    const uint32_t n = src.size(), ntiles = div_up(n, kSortTile);
+    const uint32_t n = src.size(), ntiles = div_up(n, kSortThreads * ITEMS);
     const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
     const uint64_t lt = (1ull << lane) - 1ull;
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+        uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
 #pragma unroll
-        for (int j = 0; j < kSortItems; ++j)
-            src.get(min(sort_pos(tile, wave, j, lane), n - 1), key[j], val[j]);
+        for (int j = 0; j < ITEMS; ++j)
+            src.get(min(sort_pos<ITEMS>(tile, wave, j, lane), n - 1), key[j], val[j]);
         const uint32_t start = hist[threadIdx.x * ntiles + tile];  // digit threadIdx.x
 #pragma unroll
         for (int w = 0; w < kSortWaves; ++w)
             cnt[w][threadIdx.x] = 0;
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < kSortItems; ++j) {
-            const bool valid = sort_pos(tile, wave, j, lane) < n;
+        for (int j = 0; j < ITEMS; ++j) {
+            const bool valid = sort_pos<ITEMS>(tile, wave, j, lane) < n;
             const uint32_t d = (key[j] >> shift) & 255u;
             const uint64_t peers = digit_peers(d, valid);
             const uint32_t before = cnt[wave][d];
@@ -188,8 +192,8 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(Src src, uin
         }
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < kSortItems; ++j) {
-            if (sort_pos(tile, wave, j, lane) >= n)
+        for (int j = 0; j < ITEMS; ++j) {
+            if (sort_pos<ITEMS>(tile, wave, j, lane) >= n)
                 continue;
             const uint32_t dst = cnt[wave][(key[j] >> shift) & 255u] + rank[j];
             if (dst >= out_skip && dst - out_skip < out_cap) {
@@ -205,14 +209,15 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(Src src, uin
 // `npasses` 8-bit digit passes, LSB first; the final pass writes only the
 // values (to final_vals, from position out_skip on).  Pass p reads the source
 // (p = 0) or buffer (p - 1) & 1, and writes buffer p & 1 or the final output.
-template <class FirstSrc>
-static void sort_passes(bcmc_ctx *ctx, SortScratch *s, uint32_t n_max, const uint32_t *n_dev,
-                        uint32_t n_host, const FirstSrc &first_src, uint32_t npasses,
-                        uint32_t *final_vals, uint32_t out_skip) {
-    const uint32_t tiles = std::max<uint32_t>(1, div_up(n_max, kSortTile));
+template <int ITEMS, class FirstSrc>
+static void sort_passes_items(bcmc_ctx *ctx, SortScratch *s, uint32_t n_max, const uint32_t *n_dev,
+                              uint32_t n_host, const FirstSrc &first_src, uint32_t npasses,
+                              uint32_t *final_vals, uint32_t out_skip) {
+    constexpr uint32_t kTile = kSortThreads * ITEMS;
+    const uint32_t tiles = std::max<uint32_t>(1, div_up(n_max, kTile));
     const uint32_t grid = std::min<uint32_t>(tiles, persistent_grid(ctx, ctx->tune.sort));
     hipStream_t st = ctx->stream;
-    HistScanOp hop{s->hist, n_dev, n_host};
+    HistScanOp hop{s->hist, n_dev, n_host, kTile};
     for (uint32_t pass = 0; pass < npasses; ++pass) {
         const uint32_t shift = 8 * pass;
         const bool last = pass + 1 == npasses;
@@ -220,29 +225,42 @@ static void sort_passes(bcmc_ctx *ctx, SortScratch *s, uint32_t n_max, const uin
         uint32_t *ov = last ? final_vals : s->vals[pass & 1];
         const uint32_t skip = last ? out_skip : 0;
         if (pass == 0) {
-            sort_hist_kernel<<<grid, kSortThreads, 0, st>>>(first_src, shift, s->hist);
+            sort_hist_kernel<FirstSrc, ITEMS><<<grid, kSortThreads, 0, st>>>(first_src, shift, s->hist);
             launch_scan<1>(ctx, s->site, s->site, hop, 256ull * tiles);
-            sort_scatter_kernel<<<grid, kSortThreads, 0, st>>>(first_src, shift, s->hist, ok, ov, skip,
-                                                               s->max_elems);
+            sort_scatter_kernel<FirstSrc, ITEMS><<<grid, kSortThreads, 0, st>>>(first_src, shift, s->hist, ok,
+                                                                                ov, skip, s->max_elems);
         } else {
             BufSource b{s->keys[(pass - 1) & 1], s->vals[(pass - 1) & 1], n_dev, n_host};
-            sort_hist_kernel<<<grid, kSortThreads, 0, st>>>(b, shift, s->hist);
+            sort_hist_kernel<BufSource, ITEMS><<<grid, kSortThreads, 0, st>>>(b, shift, s->hist);
             launch_scan<1>(ctx, s->site, s->site, hop, 256ull * tiles);
-            sort_scatter_kernel<<<grid, kSortThreads, 0, st>>>(b, shift, s->hist, ok, ov, skip,
-                                                               s->max_elems);
+            sort_scatter_kernel<BufSource, ITEMS><<<grid, kSortThreads, 0, st>>>(b, shift, s->hist, ok, ov,
+                                                                                 skip, s->max_elems);
         }
     }
 }
 
+// The passes at the context's tile shape.
+template <class FirstSrc>
+static void sort_passes(bcmc_ctx *ctx, SortScratch *s, uint32_t n_max, const uint32_t *n_dev,
+                        uint32_t n_host, const FirstSrc &first_src, uint32_t npasses,
+                        uint32_t *final_vals, uint32_t out_skip) {
+    if (ctx->tune.sort_items == 8)
+        sort_passes_items<8>(ctx, s, n_max, n_dev, n_host, first_src, npasses, final_vals, out_skip);
+    else if (ctx->tune.sort_items == 32)
+        sort_passes_items<32>(ctx, s, n_max, n_dev, n_host, first_src, npasses, final_vals, out_skip);
+    else
+        sort_passes_items<16>(ctx, s, n_max, n_dev, n_host, first_src, npasses, final_vals, out_skip);
+}
+
 int sort_scratch_alloc(SortScratch *s, uint32_t max_elems) {
     s->max_elems = std::max<uint32_t>(max_elems, 64);
-    s->max_tiles = div_up(s->max_elems, kSortTile);
+    s->max_tiles = div_up(s->max_elems, kSortTileMin);  // tiles of the smallest shape
     for (int i = 0; i < 2; ++i) {
         BCMC_TRY(dev_alloc((void **)&s->keys[i], (size_t)s->max_elems * 4));
         BCMC_TRY(dev_alloc((void **)&s->vals[i], (size_t)s->max_elems * 4));
     }
     BCMC_TRY(dev_alloc((void **)&s->hist, (size_t)256 * s->max_tiles * 4));
-    BCMC_TRY(scan_site_alloc(&s->site, (uint64_t)256 * s->max_tiles, kScanTile));
+    BCMC_TRY(scan_site_alloc(&s->site, (uint64_t)256 * s->max_tiles));
     return BCMC_OK;
 }
 

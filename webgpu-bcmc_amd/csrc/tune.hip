@@ -1,12 +1,18 @@
-// Launch-shape autotuner: times the persistent-grid candidates of the scan,
-// radix-sort and zfp-decode kernel families on synthetic device data and keeps
-// the fastest in the context.  Results of every kernel are independent of the
-// grid (tiles are fixed-size and claimed in order, per-tile histograms and
-// look-back status words are indexed by tile), so tuning changes speed only.
+// Launch-shape autotuner: times the tile shapes and persistent-grid
+// candidates of the scan, radix-sort and zfp-decode kernel families on
+// synthetic device data and keeps the fastest in the context.  Tiles: the
+// scan's elements per thread (tiles of 4096, 8192 or 16384 elements), the
+// sort's (2048, 4096 or 8192) and the decoders' blocks per workgroup (64, 128
+// or 256).  Results of every kernel are independent of both (tiles are
+// claimed in order, per-tile histograms and look-back status words are
+// indexed by tile, the sort is stable, a decoded block is a pure function of
+// its bits), so tuning changes speed only.
 #include "common.h"
 #include "kernels.h"
 
 #include <algorithm>
+#include <initializer_list>
+#include <utility>
 #include <vector>
 
 namespace bcmc {
@@ -94,14 +100,16 @@ int time_median(bcmc_ctx *ctx, Events &ev, int reps, F &&f, float *ms) {
     return BCMC_OK;
 }
 
-// Tries each candidate for `slot`, leaves the fastest in it; best time in *best_ms.
+// Tries each candidate (tile, workgroups per CU) for the pair of slots,
+// leaves the fastest in them; best time in *best_ms.
 template <class F>
-int pick(bcmc_ctx *ctx, Events &ev, uint32_t &slot, const std::vector<uint32_t> &cands, F &&f,
-         float *best_ms) {
-    uint32_t best = slot;
+int pick(bcmc_ctx *ctx, Events &ev, uint32_t &tile_slot, uint32_t &grid_slot,
+         const std::vector<std::pair<uint32_t, uint32_t>> &cands, F &&f, float *best_ms) {
+    std::pair<uint32_t, uint32_t> best{tile_slot, grid_slot};
     float best_t = 0.f;
     for (size_t i = 0; i < cands.size(); ++i) {
-        slot = cands[i];
+        tile_slot = cands[i].first;
+        grid_slot = cands[i].second;
         float t = 0.f;
         BCMC_TRY(time_median(ctx, ev, 5, f, &t));
         if (i == 0 || t < best_t) {
@@ -109,9 +117,20 @@ int pick(bcmc_ctx *ctx, Events &ev, uint32_t &slot, const std::vector<uint32_t> 
             best = cands[i];
         }
     }
-    slot = best;
+    tile_slot = best.first;
+    grid_slot = best.second;
     *best_ms = best_t;
     return BCMC_OK;
+}
+
+// Every tile shape with every grid size.
+std::vector<std::pair<uint32_t, uint32_t>> grid_of(std::initializer_list<uint32_t> tiles,
+                                                   std::initializer_list<uint32_t> grids) {
+    std::vector<std::pair<uint32_t, uint32_t>> v;
+    for (uint32_t t : tiles)
+        for (uint32_t g : grids)
+            v.push_back({t, g});
+    return v;
 }
 
 }  // namespace
@@ -125,7 +144,12 @@ static void to_public(const bcmc_ctx *ctx, bcmc_tuning *t) {
     t->decode_wg_per_cu = ctx->tune.decode;
     t->mc_wg_per_cu = ctx->tune.mc;
     t->count_wg_per_cu = ctx->tune.count;
+    t->scan_items = ctx->tune.scan_items;
+    t->sort_items = ctx->tune.sort_items;
+    t->decode_threads = ctx->tune.decode_threads;
 }
+
+static bool one_of(uint32_t x, uint32_t a, uint32_t b, uint32_t c) { return x == a || x == b || x == c; }
 
 extern "C" {
 
@@ -142,6 +166,16 @@ BCMC_API int bcmc_ctx_set_tuning(bcmc_ctx *ctx, const bcmc_tuning *t) {
                            t->mc_wg_per_cu, t->count_wg_per_cu};
     for (uint32_t x : v)
         BCMC_CHECK_ARG(x >= 1 && x <= 64, "workgroups per CU must be in 1..64");
+    BCMC_CHECK_ARG(t->scan_items == 0 || one_of(t->scan_items, 8, 16, 32), "scan_items must be 8, 16 or 32");
+    BCMC_CHECK_ARG(t->sort_items == 0 || one_of(t->sort_items, 8, 16, 32), "sort_items must be 8, 16 or 32");
+    BCMC_CHECK_ARG(t->decode_threads == 0 || one_of(t->decode_threads, 64, 128, 256),
+                   "decode_threads must be 64, 128 or 256");
+    if (t->scan_items)
+        ctx->tune.scan_items = t->scan_items;
+    if (t->sort_items)
+        ctx->tune.sort_items = t->sort_items;
+    if (t->decode_threads)
+        ctx->tune.decode_threads = t->decode_threads;
     ctx->tune.scan = v[0];
     ctx->tune.sort = v[1];
     ctx->tune.decode = v[2];
@@ -168,14 +202,14 @@ BCMC_API int bcmc_ctx_autotune(bcmc_ctx *ctx, bcmc_tuning *out) {
     BCMC_TRY(B.alloc(&scan_data, (size_t)kScanN * 4));
     BCMC_TRY(B.alloc(&total, 8));
     ScanSite site;
-    BCMC_TRY(scan_site_alloc(&site, kScanN, kScanTile));
+    BCMC_TRY(scan_site_alloc(&site, kScanN));
     struct SiteGuard {
         ScanSite *s;
         ~SiteGuard() { scan_site_free(s); }
     } sg{&site};
     fill(ctx, scan_data, kScanN, 1, 0xFF, 0);
     const TuneScanOp sop{scan_data, kScanN, total};
-    BCMC_TRY(pick(ctx, ev, ctx->tune.scan, {1, 2, 4, 8},
+    BCMC_TRY(pick(ctx, ev, ctx->tune.scan_items, ctx->tune.scan, grid_of({8, 16, 32}, {1, 2, 4, 8}),
                   [&] { launch_scan<1>(ctx, site, site, sop, kScanN); }, &res.scan_ms));
 
     // radix sort: 2M pairs with 24-bit keys (3 digit passes, as an LRU order)
@@ -194,7 +228,7 @@ BCMC_API int bcmc_ctx_autotune(bcmc_ctx *ctx, bcmc_tuning *out) {
         ~SortGuard() { sort_scratch_free(s); }
     } sog{&ss};
     BCMC_TRY(sort_scratch_alloc(&ss, kSortN));
-    BCMC_TRY(pick(ctx, ev, ctx->tune.sort, {2, 4, 8, 16},
+    BCMC_TRY(pick(ctx, ev, ctx->tune.sort_items, ctx->tune.sort, grid_of({8, 16, 32}, {2, 4, 8, 16}),
                   [&] { launch_sort_pairs(ctx, &ss, keys, vals, count, kSortN, sorted, 3); },
                   &res.sort_ms));
 
@@ -208,7 +242,12 @@ BCMC_API int bcmc_ctx_autotune(bcmc_ctx *ctx, bcmc_tuning *out) {
     BCMC_TRY(B.alloc(&cache, (size_t)kDecN * 64 * 4));
     fill(ctx, stream, nwords * 2, 3, 0xFFFFFFFFu, 0);
     fill(ctx, slots, kDecN, 0, 0, 1);
-    BCMC_TRY(pick(ctx, ev, ctx->tune.decode, {2, 4, 8, 16},
+    // workgroups per CU scaled with the workgroup size: the same waves per CU
+    std::vector<std::pair<uint32_t, uint32_t>> dec;
+    for (uint32_t threads : {64u, 128u, 256u})
+        for (uint32_t waves : {8u, 16u, 32u, 64u})
+            dec.push_back({threads, waves / (threads / 64)});
+    BCMC_TRY(pick(ctx, ev, ctx->tune.decode_threads, ctx->tune.decode, dec,
                   [&] {
                       launch_decode_to_cache(ctx, (const uint64_t *)stream, nwords, kMaxBits,
                                              nullptr, nullptr, kDecN, (const int32_t *)slots,

@@ -356,7 +356,10 @@ BCMC_DEV void decode_block_staged(const uint32_t *s, float (&v)[64]) {
 // ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
-constexpr int kZfpThreads = 256;  // 4 waves = 256 blocks per workgroup pass
+// Workgroups of 64, 128 or 256 threads (one block per lane; the autotuner
+// picks the size, which changes nothing but speed); each wave stages its
+// chunk's words in its own part of the dynamic LDS.
+constexpr int kZfpThreads = 256;  // largest workgroup
 
 enum ZfpMode { kModeRanges = 0, kModeCache = 1, kModeDense = 2, kModeLruAssign = 3 };
 
@@ -382,12 +385,16 @@ struct ZfpArgs {
 };
 
 // W = u64 words per block when 64 | maxbits (LDS-staged), 0 = generic.
+// Dwords of LDS staging per wave.
+__host__ __device__ constexpr int zfp_stage_dwords(int w) { return w > 0 ? 64 * (2 * w + 3) : 1; }
+
 template <int W, int MODE>
 __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
     constexpr int kStride = Staged<W>::kStride;
-    constexpr int kStage = W > 0 ? 64 * kStride : 1;
-    __shared__ uint32_t stage[kZfpThreads / 64][kStage];
-    const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
+    constexpr int kStage = zfp_stage_dwords(W);
+    static_assert(W == 0 || kStage == 64 * kStride, "staging stride");
+    extern __shared__ uint32_t zfp_stage[];
+    const uint32_t lane = lane_id(), wave = threadIdx.x / 64, wpb = blockDim.x / 64;
     uint32_t n = a.n_dev ? *a.n_dev : a.n_host;
     if constexpr (MODE == kModeLruAssign) {
         // no assignment without available slots (lru_cache.js:717-769)
@@ -397,9 +404,8 @@ __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
             a.q->cache_size = a.q->cache_size_next;
         }
     }
-    const uint32_t waves_total = gridDim.x * (kZfpThreads / 64);
-    for (uint32_t chunk = blockIdx.x * (kZfpThreads / 64) + wave; (uint64_t)chunk * 64 < n;
-         chunk += waves_total) {
+    const uint32_t waves_total = gridDim.x * wpb;
+    for (uint32_t chunk = blockIdx.x * wpb + wave; (uint64_t)chunk * 64 < n; chunk += waves_total) {
         const uint32_t i = chunk * 64 + lane;
         const bool live = i < n;
         const uint32_t blk = live ? (a.ids ? a.ids[i] : i) : 0u;
@@ -413,7 +419,7 @@ __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
         }
         float v[64];
         if constexpr (W > 0) {
-            uint32_t *sw = stage[wave];
+            uint32_t *sw = zfp_stage + wave * kStage;
             // copy of the wave's fixed-rate words into LDS, block by block at
             // a stride of kStride dwords, each followed by two all-ones dwords
             // (the budget's end).  Word j of the wave is word j % W of the
@@ -542,16 +548,18 @@ __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
     }
 }
 
+// `threads` per workgroup: 64, 128 or 256.
 template <int MODE>
-static void launch_mode(const ZfpArgs &a, uint32_t grid, hipStream_t s) {
+static void launch_mode(const ZfpArgs &a, uint32_t grid, uint32_t threads, hipStream_t s) {
     const uint32_t W = a.maxbits % 64 == 0 ? a.maxbits / 64 : 0;
+    const size_t lds = (size_t)(threads / 64) * zfp_stage_dwords((int)W) * 4;
     switch (W) {
-    case 1: zfp_kernel<1, MODE><<<grid, kZfpThreads, 0, s>>>(a); break;
-    case 2: zfp_kernel<2, MODE><<<grid, kZfpThreads, 0, s>>>(a); break;
-    case 4: zfp_kernel<4, MODE><<<grid, kZfpThreads, 0, s>>>(a); break;
-    case 8: zfp_kernel<8, MODE><<<grid, kZfpThreads, 0, s>>>(a); break;
-    case 16: zfp_kernel<16, MODE><<<grid, kZfpThreads, 0, s>>>(a); break;
-    default: zfp_kernel<0, MODE><<<grid, kZfpThreads, 0, s>>>(a); break;
+    case 1: zfp_kernel<1, MODE><<<grid, threads, lds, s>>>(a); break;
+    case 2: zfp_kernel<2, MODE><<<grid, threads, lds, s>>>(a); break;
+    case 4: zfp_kernel<4, MODE><<<grid, threads, lds, s>>>(a); break;
+    case 8: zfp_kernel<8, MODE><<<grid, threads, lds, s>>>(a); break;
+    case 16: zfp_kernel<16, MODE><<<grid, threads, lds, s>>>(a); break;
+    default: zfp_kernel<0, MODE><<<grid, threads, lds, s>>>(a); break;
     }
 }
 
@@ -575,7 +583,7 @@ void launch_block_ranges(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords,
     a.part_ranges = part_ranges;
     const uint32_t grid = div_up(a.n_host, kZfpThreads);
     if (grid)
-        launch_mode<kModeRanges>(a, grid, ctx->stream);
+        launch_mode<kModeRanges>(a, grid, kZfpThreads, ctx->stream);
 }
 
 // Per block: [min lo, max hi] over its in-bounds 3x3x3 neighbourhood.
@@ -658,9 +666,10 @@ void launch_decode_to_cache(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwor
     a.item_slot = item_slot;
     a.capacity = capacity;
     a.out = cache;
-    const uint32_t grid = std::min(div_up(n_max, kZfpThreads), persistent_grid(ctx, ctx->tune.decode));
+    const uint32_t threads = ctx->tune.decode_threads;
+    const uint32_t grid = std::min(div_up(n_max, threads), persistent_grid(ctx, ctx->tune.decode));
     if (grid)
-        launch_mode<kModeCache>(a, grid, ctx->stream);
+        launch_mode<kModeCache>(a, grid, threads, ctx->stream);
 }
 
 void launch_decode_assign(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords, uint32_t maxbits,
@@ -680,8 +689,9 @@ void launch_decode_assign(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords
     a.capacity = capacity;
     a.out = cache;
     a.q = q;
-    const uint32_t grid = std::max(1u, std::min(div_up(n_max, kZfpThreads), persistent_grid(ctx, ctx->tune.decode)));
-    launch_mode<kModeLruAssign>(a, grid, ctx->stream);
+    const uint32_t threads = ctx->tune.decode_threads;
+    const uint32_t grid = std::max(1u, std::min(div_up(n_max, threads), persistent_grid(ctx, ctx->tune.decode)));
+    launch_mode<kModeLruAssign>(a, grid, threads, ctx->stream);
 }
 
 void launch_decode_dense(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords, uint32_t maxbits,
@@ -692,7 +702,7 @@ void launch_decode_dense(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords,
     a.out = out;
     const uint32_t grid = div_up(n, kZfpThreads);
     if (grid)
-        launch_mode<kModeDense>(a, grid, ctx->stream);
+        launch_mode<kModeDense>(a, grid, kZfpThreads, ctx->stream);
 }
 
 // ---------------------------------------------------------------------------

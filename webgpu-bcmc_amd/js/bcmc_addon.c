@@ -164,7 +164,8 @@ static napi_value ctx_info(napi_env env, napi_callback_info info)
     return o;
 }
 
-/* ctxAutotune(ctx) -> {scan, sort, decode, mc, scanMs, sortMs, decodeMs} */
+/* ctxAutotune(ctx) -> {scan, sort, decode, mc, count, scanItems, sortItems,
+ * decodeThreads, scanMs, sortMs, decodeMs} */
 static napi_value ctx_autotune(napi_env env, napi_callback_info info)
 {
     GET_ARGS(1);
@@ -178,6 +179,9 @@ static napi_value ctx_autotune(napi_env env, napi_callback_info info)
     set_num(env, o, "decode", t.decode_wg_per_cu);
     set_num(env, o, "mc", t.mc_wg_per_cu);
     set_num(env, o, "count", t.count_wg_per_cu);
+    set_num(env, o, "scanItems", t.scan_items);
+    set_num(env, o, "sortItems", t.sort_items);
+    set_num(env, o, "decodeThreads", t.decode_threads);
     set_num(env, o, "scanMs", t.scan_ms);
     set_num(env, o, "sortMs", t.sort_ms);
     set_num(env, o, "decodeMs", t.decode_ms);

@@ -81,7 +81,9 @@ class _Tuning(ctypes.Structure):
                 ("decode_wg_per_cu", ctypes.c_uint32), ("mc_wg_per_cu", ctypes.c_uint32),
                 ("count_wg_per_cu", ctypes.c_uint32),
                 ("scan_ms", ctypes.c_float), ("sort_ms", ctypes.c_float),
-                ("decode_ms", ctypes.c_float), ("mc_ms", ctypes.c_float)]
+                ("decode_ms", ctypes.c_float), ("mc_ms", ctypes.c_float),
+                ("scan_items", ctypes.c_uint32), ("sort_items", ctypes.c_uint32),
+                ("decode_threads", ctypes.c_uint32)]
 
 
 class _ServeOpts(ctypes.Structure):
@@ -214,26 +216,35 @@ class Device:
         return {"num_cus": n.value, "name": name.value.decode()}
 
     _TUNE_KEYS = ("scan", "sort", "decode", "mc", "count")
+    _TILE_KEYS = ("scan_items", "sort_items", "decode_threads")
 
     @staticmethod
     def _tuning_dict(t):
         d = {k: int(getattr(t, f"{k}_wg_per_cu")) for k in Device._TUNE_KEYS}
+        d["tiles"] = {k: int(getattr(t, k)) for k in Device._TILE_KEYS}
         d["ms"] = {k: float(getattr(t, f"{k}_ms")) for k in Device._TUNE_KEYS[:3]}
         return d
 
     def tuning(self):
-        """Workgroups per CU of the scan / sort / decode / mc kernel families."""
+        """Workgroups per CU of the scan / sort / decode / mc kernel families
+        and the tile shapes (scan and sort elements per thread, decoder
+        blocks per workgroup)."""
         t = _Tuning()
         _check(lib().bcmc_ctx_get_tuning(self.h, ctypes.byref(t)))
         return self._tuning_dict(t)
 
-    def set_tuning(self, **wg_per_cu):
+    def set_tuning(self, **shapes):
+        """Keys: scan, sort, decode, mc, count (workgroups per CU) and
+        scan_items, sort_items (8, 16, 32), decode_threads (64, 128, 256)."""
         t = _Tuning()
         _check(lib().bcmc_ctx_get_tuning(self.h, ctypes.byref(t)))
-        for k, v in wg_per_cu.items():
-            if k not in self._TUNE_KEYS:
-                raise ValueError(f"unknown kernel family {k!r}")
-            setattr(t, f"{k}_wg_per_cu", int(v))
+        for k, v in shapes.items():
+            if k in self._TUNE_KEYS:
+                setattr(t, f"{k}_wg_per_cu", int(v))
+            elif k in self._TILE_KEYS:
+                setattr(t, k, int(v))
+            else:
+                raise ValueError(f"unknown tuning key {k!r}")
         _check(lib().bcmc_ctx_set_tuning(self.h, ctypes.byref(t)))
 
     def autotune(self):
