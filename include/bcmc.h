@@ -339,10 +339,12 @@ BCMC_API int bcmc_ctx_get_tuning(bcmc_ctx *ctx, bcmc_tuning *out);
 /* Each *_wg_per_cu must be in 1..64; pipelines re-capture their graphs. */
 BCMC_API int bcmc_ctx_set_tuning(bcmc_ctx *ctx, const bcmc_tuning *tuning);
 /* Times the scan, sort and decode candidates (tile shape x workgroups per
- * CU) on synthetic device data
- * (about 200 MiB of temporary device memory, well under a second) and keeps
- * the fastest shape of each family in the context; out (nullable) receives
- * the result.  Call it before the first query: it synchronises the stream. */
+ * CU) on synthetic device data at several sizes (scans of 256K, 2M and 16M
+ * elements, sorts of 64K and 2M pairs, decodes of 32K and 256K blocks; about
+ * 200 MiB of temporary device memory, about a second) and keeps per family
+ * the shape with the least time per element summed over the sizes; out
+ * (nullable) receives the result, its *_ms at the largest size.  Call it
+ * before the first query: it synchronises the stream. */
 BCMC_API int bcmc_ctx_autotune(bcmc_ctx *ctx, bcmc_tuning *out);
 
 #ifdef __cplusplus

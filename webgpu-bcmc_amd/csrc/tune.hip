@@ -100,19 +100,28 @@ int time_median(bcmc_ctx *ctx, Events &ev, int reps, F &&f, float *ms) {
     return BCMC_OK;
 }
 
-// Tries each candidate (tile, workgroups per CU) for the pair of slots,
-// leaves the fastest in them; best time in *best_ms.
+// Tries each candidate (tile, workgroups per CU) for the pair of slots on
+// every problem size (f(n) runs one), leaves the one with the least summed
+// time per element in them (each size weighs the same, so a shape that only
+// suits the largest size loses); *best_ms receives its time at the last size.
 template <class F>
 int pick(bcmc_ctx *ctx, Events &ev, uint32_t &tile_slot, uint32_t &grid_slot,
-         const std::vector<std::pair<uint32_t, uint32_t>> &cands, F &&f, float *best_ms) {
+         const std::vector<std::pair<uint32_t, uint32_t>> &cands, const std::vector<uint32_t> &sizes, F &&f,
+         float *best_ms) {
     std::pair<uint32_t, uint32_t> best{tile_slot, grid_slot};
+    double best_score = 0.0;
     float best_t = 0.f;
     for (size_t i = 0; i < cands.size(); ++i) {
         tile_slot = cands[i].first;
         grid_slot = cands[i].second;
+        double score = 0.0;
         float t = 0.f;
-        BCMC_TRY(time_median(ctx, ev, 5, f, &t));
-        if (i == 0 || t < best_t) {
+        for (uint32_t n : sizes) {
+            BCMC_TRY(time_median(ctx, ev, 5, [&] { f(n); }, &t));
+            score += (double)t / n;
+        }
+        if (i == 0 || score < best_score) {
+            best_score = score;
             best_t = t;
             best = cands[i];
         }
@@ -208,9 +217,12 @@ BCMC_API int bcmc_ctx_autotune(bcmc_ctx *ctx, bcmc_tuning *out) {
         ~SiteGuard() { scan_site_free(s); }
     } sg{&site};
     fill(ctx, scan_data, kScanN, 1, 0xFF, 0);
-    const TuneScanOp sop{scan_data, kScanN, total};
+    // the sizes of a per-query active scan (a 512^3 volume has 2M blocks),
+    // of small ones and of a batched step
     BCMC_TRY(pick(ctx, ev, ctx->tune.scan_items, ctx->tune.scan, grid_of({8, 16, 32}, {1, 2, 4, 8}),
-                  [&] { launch_scan<1>(ctx, site, site, sop, kScanN); }, &res.scan_ms));
+                  {1u << 18, 1u << 21, kScanN},
+                  [&](uint32_t n) { launch_scan<1>(ctx, site, site, TuneScanOp{scan_data, n, total}, n); },
+                  &res.scan_ms));
 
     // radix sort: 2M pairs with 24-bit keys (3 digit passes, as an LRU order)
     constexpr uint32_t kSortN = 1u << 21;
@@ -218,10 +230,12 @@ BCMC_API int bcmc_ctx_autotune(bcmc_ctx *ctx, bcmc_tuning *out) {
     BCMC_TRY(B.alloc(&keys, (size_t)kSortN * 4));
     BCMC_TRY(B.alloc(&vals, (size_t)kSortN * 4));
     BCMC_TRY(B.alloc(&sorted, (size_t)kSortN * 4));
-    BCMC_TRY(B.alloc(&count, 4));
+    BCMC_TRY(B.alloc(&count, 8));
     fill(ctx, keys, kSortN, 2, 0xFFFFFF, 0);
     fill(ctx, vals, kSortN, 0, 0, 1);
-    BCMC_HIP_CHECK(hipMemcpyAsync(count, &kSortN, 4, hipMemcpyHostToDevice, ctx->stream));
+    constexpr uint32_t kSortSmall = 1u << 16;
+    const uint32_t counts[2] = {kSortSmall, kSortN};
+    BCMC_HIP_CHECK(hipMemcpyAsync(count, counts, 8, hipMemcpyHostToDevice, ctx->stream));
     SortScratch ss;
     struct SortGuard {
         SortScratch *s;
@@ -229,7 +243,10 @@ BCMC_API int bcmc_ctx_autotune(bcmc_ctx *ctx, bcmc_tuning *out) {
     } sog{&ss};
     BCMC_TRY(sort_scratch_alloc(&ss, kSortN));
     BCMC_TRY(pick(ctx, ev, ctx->tune.sort_items, ctx->tune.sort, grid_of({8, 16, 32}, {2, 4, 8, 16}),
-                  [&] { launch_sort_pairs(ctx, &ss, keys, vals, count, kSortN, sorted, 3); },
+                  {kSortSmall, kSortN},
+                  [&](uint32_t n) {
+                      launch_sort_pairs(ctx, &ss, keys, vals, count + (n == kSortN ? 1 : 0), n, sorted, 3);
+                  },
                   &res.sort_ms));
 
     // zfp decode: 256K blocks of a synthetic rate-4 stream into a block cache
@@ -247,11 +264,11 @@ BCMC_API int bcmc_ctx_autotune(bcmc_ctx *ctx, bcmc_tuning *out) {
     for (uint32_t threads : {64u, 128u, 256u})
         for (uint32_t waves : {8u, 16u, 32u, 64u})
             dec.push_back({threads, waves / (threads / 64)});
-    BCMC_TRY(pick(ctx, ev, ctx->tune.decode_threads, ctx->tune.decode, dec,
-                  [&] {
-                      launch_decode_to_cache(ctx, (const uint64_t *)stream, nwords, kMaxBits,
-                                             nullptr, nullptr, kDecN, (const int32_t *)slots,
-                                             cache, kDecN);
+    // the misses of a query with a cold cache, and of a mostly warm one
+    BCMC_TRY(pick(ctx, ev, ctx->tune.decode_threads, ctx->tune.decode, dec, {1u << 15, kDecN},
+                  [&](uint32_t n) {
+                      launch_decode_to_cache(ctx, (const uint64_t *)stream, nwords, kMaxBits, nullptr,
+                                             nullptr, n, (const int32_t *)slots, cache, kDecN);
                   },
                   &res.decode_ms));
     BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
