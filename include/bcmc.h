@@ -129,6 +129,21 @@ BCMC_API int bcmc_lru_destroy(bcmc_lru *lru);
  * ids in ascending order (caller frees); *new_items = NULL when zero. */
 BCMC_API int bcmc_lru_update(bcmc_lru *lru, bcmc_buf *needed, uint32_t *num_new,
                              bcmc_buf **new_items);
+/* LRUCache.update with the reference's perfTracker entries
+ * (lru_cache.js:320-333): device milliseconds of each stage and the two
+ * counts.  Stages the reference runs apart run fused here: the new-item pass
+ * marks, counts and compacts (lruMarkNewItems; lruScanNeededSlots and
+ * lruCompactNewItems are 0), the aging pass scans and compacts the available
+ * slots (lruScanSlotsAvailable; lruCompactAvailableSlots 0), and the sort
+ * forms its keys in its first pass (lruPrepKeyValue 0).  With no new items
+ * every stage after lruNumNewItems is 0, as in the reference. */
+typedef struct {
+    double lruMarkNewItems, lruScanNeededSlots, lruNumNewItems, lruCompactNewItems;
+    double lruScanSlotsAvailable, lruNumSlotsAvailable, lruGrowCache, lruCompactAvailableSlots;
+    double lruPrepKeyValue, lruTotalSortTime, lruWriteNewItems;
+} bcmc_lru_perf;
+BCMC_API int bcmc_lru_update_timed(bcmc_lru *lru, bcmc_buf *needed, uint32_t *num_new,
+                                   bcmc_buf **new_items, bcmc_lru_perf *perf);
 /* LRUCache.reset (lru_cache.js:846-903). */
 BCMC_API int bcmc_lru_reset(bcmc_lru *lru);
 typedef struct {

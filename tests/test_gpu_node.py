@@ -145,3 +145,58 @@ def test_engine_from_node_matches_oracle(tmp_path):
     enc = np.fromfile(tmp_path / "encoded.bin", np.uint8)
     host = synth.encode(fld, int(64 * frate))
     assert enc[:len(host)].tobytes() == host.tobytes() and not enc[len(host):].any()
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+def test_async_facade_keeps_the_event_loop_running(tmp_path):
+    """Engine.serve of requests arriving over a second runs on a worker
+    thread: a 5 ms setInterval timer keeps ticking while it is pending.  The
+    LRUCache.update perfTracker gets one entry per update under each of the
+    reference's 11 lru* keys (counts equal to the oracle's), and operations
+    submitted without awaiting complete in call order with the oracle's
+    counts."""
+    assert os.path.exists(ADDON), "N-API addon not built"
+    dims, rate, cache = (48, 40, 36), 4, 200
+    f = synth.field(synth.BLOBS, 51, dims, 6, (3, 10, 0.5, 1))
+    s = synth.encode(f, 64 * rate)
+    (tmp_path / "vol.bin").write_bytes(s.tobytes())
+    rng = np.random.default_rng(52)
+    n = 40
+    arrivals = np.linspace(0.0, 1.0, n).tolist()
+    isos = rng.uniform(0.3, 0.7, n).astype(np.float32).tolist()
+    lru_total, lru_initial = 3000, 200
+    (tmp_path / "params.json").write_text(json.dumps({
+        "dims": dims, "rate": rate, "cache": cache, "poolBytes": 20000 * 256, "arrivals": arrivals,
+        "isos": isos, "lruTotal": lru_total, "lruInitial": lru_initial}))
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "async_timer.js"), str(tmp_path)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads((tmp_path / "out.json").read_text())
+    assert out["serveMs"] >= 900
+    assert out["serveTicks"] >= 0.3 * out["serveMs"] / 5, out["serveTicks"]
+    assert len(out["served"]) == n
+    for k in range(n):
+        ref = oracle.MC(s, 64 * rate, dims)
+        assert out["served"][k]["numVertices"] == ref.compute_surface(np.float32(isos[k])), k
+    keys = ["lruMarkNewItems", "lruScanNeededSlots", "lruNumNewItems", "lruCompactNewItems",
+            "lruScanSlotsAvailable", "lruNumSlotsAvailable", "lruGrowCache", "lruCompactAvailableSlots",
+            "lruPrepKeyValue", "lruTotalSortTime", "lruWriteNewItems"]
+    perf = out["perf"]
+    assert sorted(perf) == sorted(keys)
+    assert all(len(perf[k]) == 2 for k in keys)
+    cpu = oracle.LRU(lru_initial, 256, lru_total)
+    size0 = len(cpu.state()[0])  # the constructor's slots, all available
+    needed = np.zeros(lru_total, np.uint32)
+    needed[::3] = 1
+    n1, _ = cpu.update(needed)
+    n2, _ = cpu.update(needed)
+    assert perf["lruNumNewItems"] == [n1, n2] and n1 > 0 and n2 == 0
+    assert out["updateNew"] == n1
+    assert perf["lruMarkNewItems"][0] > 0 and perf["lruTotalSortTime"][0] > 0
+    assert perf["lruGrowCache"][0] > 0  # 1000 new items into 200 slots
+    assert perf["lruNumSlotsAvailable"][0] == size0 >= lru_initial
+    assert all(perf[k][1] == 0 for k in keys[3:])  # no new items: as the reference
+    ref = oracle.MC(s, 64 * rate, dims, initial_cache=cache)
+    counts = [ref.compute_surface(np.float32(iso)) for iso in isos[:3]]
+    assert out["surfaceVertices"] == counts[0]
+    assert out["order"] == [["a", counts[1]], ["b", counts[2]]]

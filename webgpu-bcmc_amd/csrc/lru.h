@@ -50,8 +50,11 @@ void lru_enqueue_update_mc(bcmc_ctx *ctx, LruState *L, const uint32_t *flags, QS
 // Counts one more update (age bound, updates since the reset) and returns the
 // radix passes its eviction order needs.
 uint32_t lru_begin_update(LruState *L);
-// LRUCache.update for arbitrary 0/1 `needed` flags over all items.
-void lru_enqueue_update_all(bcmc_ctx *ctx, LruState *L, const uint32_t *needed, QState *q);
+// LRUCache.update for arbitrary 0/1 `needed` flags over all items.  With `ev`
+// (6 events), ev[k] is recorded before stage k and ev[5] at the end: new-item
+// pass, aging + available-slot pass, growth, eviction order, assignment.
+void lru_enqueue_update_all(bcmc_ctx *ctx, LruState *L, const uint32_t *needed, QState *q,
+                            hipEvent_t *ev = nullptr);
 void lru_enqueue_reset(bcmc_ctx *ctx, LruState *L);
 int lru_copy_state(bcmc_ctx *ctx, LruState *L, uint32_t *age, uint32_t *available, int32_t *item,
                    int32_t *item_slot);
@@ -67,4 +70,5 @@ struct bcmc_lru {
     bcmc::QState *qh = nullptr;  // pinned host mirror
     bcmc_buf cache_view;         // non-owning view of the cache contents
     bool owns = true;
+    hipEvent_t ev[6] = {};       // stage events of timed updates (created on first use)
 };

@@ -264,14 +264,21 @@ uint32_t lru_begin_update(LruState *L) {
 }
 
 // Age/mark + available list, growth and eviction order, once q->num_new and
-// the new items are on the device.
+// the new items are on the device; `ev` (nullable) receives an event after
+// each of the three stages.
 static void enqueue_tail(bcmc_ctx *ctx, LruState *L, const uint32_t *needed, QState *q,
-                         const uint32_t *gate, uint32_t npasses) {
+                         const uint32_t *gate, uint32_t npasses, hipEvent_t *ev = nullptr) {
     hipStream_t st = ctx->stream;
     launch_scan<1>(ctx, L->site_avail, L->site_avail, AgeAvailOp{L->v, needed, q, gate, L->ctl}, L->capacity);
+    if (ev)
+        (void)hipEventRecord(ev[0], st);
     lru_grow_kernel<<<grid_for(ctx, L->capacity, 2), 256, 0, st>>>(L->v, q, gate, L->ctl);
+    if (ev)
+        (void)hipEventRecord(ev[1], st);
     launch_lru_order(ctx, &L->sort, L->v.avail_ids, L->v.avail_keys, &q->num_avail_total, L->capacity,
                      L->v.order, npasses);
+    if (ev)
+        (void)hipEventRecord(ev[2], st);
 }
 
 void lru_enqueue_update_mc(bcmc_ctx *ctx, LruState *L, const uint32_t *flags, QState *q,
@@ -279,11 +286,18 @@ void lru_enqueue_update_mc(bcmc_ctx *ctx, LruState *L, const uint32_t *flags, QS
     enqueue_tail(ctx, L, flags, q, &q->num_active, npasses);
 }
 
-void lru_enqueue_update_all(bcmc_ctx *ctx, LruState *L, const uint32_t *needed, QState *q) {
+void lru_enqueue_update_all(bcmc_ctx *ctx, LruState *L, const uint32_t *needed, QState *q,
+                            hipEvent_t *ev) {
+    if (ev)
+        (void)hipEventRecord(ev[0], ctx->stream);
     launch_scan<1>(ctx, L->site_miss, L->site_miss, MissAllOp{L->v, needed, L->new_items, q},
                    L->aligned_total);
-    enqueue_tail(ctx, L, needed, q, nullptr, lru_begin_update(L));
+    if (ev)
+        (void)hipEventRecord(ev[1], ctx->stream);
+    enqueue_tail(ctx, L, needed, q, nullptr, lru_begin_update(L), ev ? ev + 2 : nullptr);
     lru_assign_kernel<<<grid_for(ctx, L->capacity, 2), 256, 0, ctx->stream>>>(L->v, L->new_items, q);
+    if (ev)
+        (void)hipEventRecord(ev[5], ctx->stream);
 }
 
 void lru_enqueue_reset(bcmc_ctx *ctx, LruState *L) {
@@ -357,21 +371,54 @@ BCMC_API int bcmc_lru_destroy(bcmc_lru *lru) {
     lru_state_free(&lru->st);
     dev_free(lru->q);
     (void)hipHostFree(lru->qh);
+    for (hipEvent_t e : lru->ev)
+        if (e)
+            (void)hipEventDestroy(e);
     delete lru;
     return BCMC_OK;
 }
 
 BCMC_API int bcmc_lru_update(bcmc_lru *lru, bcmc_buf *needed, uint32_t *num_new,
                              bcmc_buf **new_items) {
+    return bcmc_lru_update_timed(lru, needed, num_new, new_items, nullptr);
+}
+
+BCMC_API int bcmc_lru_update_timed(bcmc_lru *lru, bcmc_buf *needed, uint32_t *num_new,
+                                   bcmc_buf **new_items, bcmc_lru_perf *perf) {
     BCMC_CHECK_ARG(lru && needed && num_new, "null argument");
     BCMC_CHECK_ARG(needed->bytes >= (uint64_t)lru->st.total * 4,
                    "itemNeeded must hold totalElements u32 flags");
     if (new_items)
         *new_items = nullptr;
-    lru_enqueue_update_all(lru->ctx, &lru->st, (const uint32_t *)needed->ptr, lru->q);
+    if (perf && !lru->ev[0])
+        for (hipEvent_t &e : lru->ev)
+            BCMC_HIP_CHECK(hipEventCreate(&e));
+    const uint32_t size_before = lru->st.cache_size;
+    lru_enqueue_update_all(lru->ctx, &lru->st, (const uint32_t *)needed->ptr, lru->q,
+                           perf ? lru->ev : nullptr);
     BCMC_HIP_CHECK(hipGetLastError());
     BCMC_TRY(lru_sync_state(lru));
     *num_new = lru->qh->num_new;
+    if (perf) {
+        float t[5];
+        for (int k = 0; k < 5; ++k)
+            BCMC_HIP_CHECK(hipEventElapsedTime(&t[k], lru->ev[k], lru->ev[k + 1]));
+        *perf = bcmc_lru_perf{};
+        perf->lruNumNewItems = *num_new;
+        if (*num_new == 0) {
+            // as the reference: the stages after the count did not run (here
+            // they ran as no-ops and aged the slots): their time goes to the
+            // mark stage, which also ages in the reference
+            perf->lruMarkNewItems = t[0] + t[1] + t[2] + t[3] + t[4];
+        } else {
+            perf->lruMarkNewItems = t[0];
+            perf->lruScanSlotsAvailable = t[1];
+            perf->lruNumSlotsAvailable = lru->qh->num_avail;
+            perf->lruGrowCache = lru->st.cache_size != size_before ? t[2] : 0.0;
+            perf->lruTotalSortTime = t[3];
+            perf->lruWriteNewItems = t[4];
+        }
+    }
     if (*num_new && new_items) {
         bcmc_buf *b = nullptr;
         BCMC_TRY(make_buf(lru->ctx, (uint64_t)*num_new * 4, &b));

@@ -6,14 +6,27 @@
  * (index.js) builds the reference's classes (CompressedMarchingCubes,
  * LRUCache, ExclusiveScanPipeline/ExclusiveScanner, StreamCompact(or),
  * RadixSorter) on top of these functions.
+ *
+ * The GPU work of the reference's async methods (computeSurface,
+ * LRUCache.update, scan, compaction, sort, Engine.run / serve) also has a
+ * *Async form: a napi_async_work runs the bcmc_* call on a libuv worker
+ * thread and resolves a Promise on the main thread, so the event loop keeps
+ * running meanwhile (the reference awaits onSubmittedWorkDone /
+ * mapAsync the same way).  Every library call, synchronous or not, holds one
+ * mutex: the library's objects are not meant for concurrent calls, and the
+ * facade issues each device's operations in order.  The typed arrays an
+ * async call reads are kept alive by references until it completes.
  */
 #define NAPI_VERSION 6
 #include <node_api.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "bcmc.h"
+
+static pthread_mutex_t g_lib_mu = PTHREAD_MUTEX_INITIALIZER;
 
 #define NAPI_CALL(env, call)                                                                    \
     do {                                                                                        \
@@ -23,12 +36,16 @@
         }                                                                                       \
     } while (0)
 
+/* One library call under the mutex; a negative status throws. */
 #define BCMC_CALL(env, call)                                                                    \
     do {                                                                                        \
+        char msg_[640];                                                                         \
+        pthread_mutex_lock(&g_lib_mu);                                                          \
         int rc_ = (call);                                                                       \
-        if (rc_ != BCMC_OK) {                                                                   \
-            char msg_[640];                                                                     \
+        if (rc_ != BCMC_OK)                                                                     \
             snprintf(msg_, sizeof(msg_), "bcmc error %d: %s", rc_, bcmc_last_error());          \
+        pthread_mutex_unlock(&g_lib_mu);                                                        \
+        if (rc_ != BCMC_OK) {                                                                   \
             napi_throw_error((env), NULL, msg_);                                                \
             return NULL;                                                                        \
         }                                                                                       \
@@ -318,19 +335,46 @@ static napi_value lru_destroy(napi_env env, napi_callback_info info)
     return NULL;
 }
 
-static napi_value lru_update(napi_env env, napi_callback_info info)
+/* [numNew, idsBuffer | null, perf | undefined] */
+static napi_value lru_update_result(napi_env env, uint32_t n, bcmc_buf *ids, const bcmc_lru_perf *p)
 {
-    GET_ARGS(2);
-    EXT(0, bcmc_lru, lru);
-    EXT(1, bcmc_buf, needed);
-    uint32_t n = 0;
-    bcmc_buf *ids = NULL;
-    BCMC_CALL(env, bcmc_lru_update(lru, needed, &n, &ids));
     napi_value arr;
-    NAPI_CALL(env, napi_create_array_with_length(env, 2, &arr));
+    NAPI_CALL(env, napi_create_array_with_length(env, 3, &arr));
     napi_set_element(env, arr, 0, mk_num(env, n));
     napi_set_element(env, arr, 1, mk_ext(env, ids));
+    if (p) {
+        napi_value o;
+        NAPI_CALL(env, napi_create_object(env, &o));
+        set_num(env, o, "lruMarkNewItems", p->lruMarkNewItems);
+        set_num(env, o, "lruScanNeededSlots", p->lruScanNeededSlots);
+        set_num(env, o, "lruNumNewItems", p->lruNumNewItems);
+        set_num(env, o, "lruCompactNewItems", p->lruCompactNewItems);
+        set_num(env, o, "lruScanSlotsAvailable", p->lruScanSlotsAvailable);
+        set_num(env, o, "lruNumSlotsAvailable", p->lruNumSlotsAvailable);
+        set_num(env, o, "lruGrowCache", p->lruGrowCache);
+        set_num(env, o, "lruCompactAvailableSlots", p->lruCompactAvailableSlots);
+        set_num(env, o, "lruPrepKeyValue", p->lruPrepKeyValue);
+        set_num(env, o, "lruTotalSortTime", p->lruTotalSortTime);
+        set_num(env, o, "lruWriteNewItems", p->lruWriteNewItems);
+        napi_set_element(env, arr, 2, o);
+    }
     return arr;
+}
+
+/* lruUpdate(lru, needed, timed) */
+static napi_value lru_update(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(3);
+    EXT(0, bcmc_lru, lru);
+    EXT(1, bcmc_buf, needed);
+    bool timed = false;
+    if (argc > 2)
+        napi_get_value_bool(env, argv[2], &timed);
+    uint32_t n = 0;
+    bcmc_buf *ids = NULL;
+    bcmc_lru_perf p;
+    BCMC_CALL(env, bcmc_lru_update_timed(lru, needed, &n, &ids, timed ? &p : NULL));
+    return lru_update_result(env, n, ids, timed ? &p : NULL);
 }
 
 static napi_value lru_reset(napi_env env, napi_callback_info info)
@@ -436,9 +480,45 @@ static napi_value mc_set_format(napi_env env, napi_callback_info info)
     return NULL;
 }
 
-/* mcComputeSurface(mc, iso, timed): with timed the per-stage device times are
- * recorded (direct launches); otherwise the captured query graph is replayed
- * and only the counts are returned. */
+/* computeSurface: with `timed` the per-stage device times are recorded
+ * (direct launches); otherwise the captured query graph is replayed and only
+ * the counts are filled in (from bcmc_mc_get_info). */
+static int mc_surface_call(bcmc_mc *mc, float iso, int timed, bcmc_perf *p, uint64_t *nv)
+{
+    memset(p, 0, sizeof(*p));
+    int rc = bcmc_mc_compute_surface(mc, iso, timed ? p : NULL, nv);
+    if (rc == BCMC_OK && !timed) {
+        bcmc_mc_info mi;
+        rc = bcmc_mc_get_info(mc, &mi);
+        p->numActiveBlocks = mi.num_active_blocks;
+        p->numBlocksDecompressed = mi.new_decompressed;
+        p->numBlocksWithVertices = mi.num_blocks_with_vertices;
+        p->numVertices = mi.num_vertices;
+    }
+    return rc;
+}
+
+static napi_value mc_surface_result(napi_env env, const bcmc_perf *p, uint64_t nv)
+{
+    napi_value o;
+    NAPI_CALL(env, napi_create_object(env, &o));
+    set_num(env, o, "numVertices", (double)nv);
+    set_num(env, o, "computeActiveBlocks", p->computeActiveBlocks);
+    set_num(env, o, "numActiveBlocks", p->numActiveBlocks);
+    set_num(env, o, "cacheUpdate", p->cacheUpdate);
+    set_num(env, o, "numBlocksDecompressed", p->numBlocksDecompressed);
+    set_num(env, o, "decompression", p->decompression);
+    set_num(env, o, "compactActiveIDs", p->compactActiveIDs);
+    set_num(env, o, "computeBlockHasVertices", p->computeBlockHasVertices);
+    set_num(env, o, "numBlocksWithVertices", p->numBlocksWithVertices);
+    set_num(env, o, "compactBlocksWithVertices", p->compactBlocksWithVertices);
+    set_num(env, o, "computeVertices", p->computeVertices);
+    set_num(env, o, "cacheSize", p->cacheSize);
+    set_num(env, o, "totalMs", p->total_ms);
+    return o;
+}
+
+/* mcComputeSurface(mc, iso, timed) */
 static napi_value mc_compute_surface(napi_env env, napi_callback_info info)
 {
     GET_ARGS(3);
@@ -447,33 +527,9 @@ static napi_value mc_compute_surface(napi_env env, napi_callback_info info)
     if (argc > 2)
         napi_get_value_bool(env, argv[2], &timed);
     bcmc_perf p;
-    memset(&p, 0, sizeof(p));
     uint64_t nv = 0;
-    BCMC_CALL(env, bcmc_mc_compute_surface(mc, (float)num(env, argv[1]), timed ? &p : NULL, &nv));
-    if (!timed) {
-        bcmc_mc_info mi;
-        BCMC_CALL(env, bcmc_mc_get_info(mc, &mi));
-        p.numActiveBlocks = mi.num_active_blocks;
-        p.numBlocksDecompressed = mi.new_decompressed;
-        p.numBlocksWithVertices = mi.num_blocks_with_vertices;
-        p.numVertices = mi.num_vertices;
-    }
-    napi_value o;
-    NAPI_CALL(env, napi_create_object(env, &o));
-    set_num(env, o, "numVertices", (double)nv);
-    set_num(env, o, "computeActiveBlocks", p.computeActiveBlocks);
-    set_num(env, o, "numActiveBlocks", p.numActiveBlocks);
-    set_num(env, o, "cacheUpdate", p.cacheUpdate);
-    set_num(env, o, "numBlocksDecompressed", p.numBlocksDecompressed);
-    set_num(env, o, "decompression", p.decompression);
-    set_num(env, o, "compactActiveIDs", p.compactActiveIDs);
-    set_num(env, o, "computeBlockHasVertices", p.computeBlockHasVertices);
-    set_num(env, o, "numBlocksWithVertices", p.numBlocksWithVertices);
-    set_num(env, o, "compactBlocksWithVertices", p.compactBlocksWithVertices);
-    set_num(env, o, "computeVertices", p.computeVertices);
-    set_num(env, o, "cacheSize", p.cacheSize);
-    set_num(env, o, "totalMs", p.total_ms);
-    return o;
+    BCMC_CALL(env, mc_surface_call(mc, (float)num(env, argv[1]), timed, &p, &nv));
+    return mc_surface_result(env, &p, nv);
 }
 
 static napi_value mc_info(napi_env env, napi_callback_info info)
@@ -565,30 +621,10 @@ static napi_value eng_add_volume(napi_env env, napi_callback_info info)
     return mk_num(env, id);
 }
 
-/* engRun(eng, Uint32Array volumeIds, Float32Array isovalues) -> [{...}] */
-static napi_value eng_run(napi_env env, napi_callback_info info)
+static napi_value eng_run_result(napi_env env, const bcmc_query_result *r, uint32_t nq)
 {
-    GET_ARGS(3);
-    EXT(0, bcmc_engine, eng);
-    void *ids, *isos;
-    size_t ni, nf;
-    if (bytes_of(env, argv[1], &ids, &ni) || bytes_of(env, argv[2], &isos, &nf) || ni / 4 != nf / 4) {
-        napi_throw_type_error(env, NULL, "expected (engine, Uint32Array, Float32Array) of one length");
-        return NULL;
-    }
-    const uint32_t nq = (uint32_t)(ni / 4);
-    bcmc_query_result *r = (bcmc_query_result *)calloc(nq ? nq : 1, sizeof(*r));
-    if (!r) {
-        napi_throw_error(env, NULL, "out of host memory");
-        return NULL;
-    }
-    const int rc = bcmc_engine_run(eng, (const uint32_t *)ids, (const float *)isos, nq, r);
-    if (rc != BCMC_OK) {
-        free(r);
-        BCMC_CALL(env, rc);
-    }
     napi_value arr;
-    napi_create_array_with_length(env, nq, &arr);
+    NAPI_CALL(env, napi_create_array_with_length(env, nq, &arr));
     for (uint32_t i = 0; i < nq; ++i) {
         napi_value o;
         napi_create_object(env, &o);
@@ -598,8 +634,49 @@ static napi_value eng_run(napi_env env, napi_callback_info info)
         set_num(env, o, "stepDecodedBlocks", r[i].step_decoded_blocks);
         napi_set_element(env, arr, i, o);
     }
-    free(r);
     return arr;
+}
+
+/* (engine, Uint32Array, Float32Array) of one length -> ids, isos, nq */
+static int eng_run_args(napi_env env, napi_value *argv, void **ids, void **isos, uint32_t *nq)
+{
+    size_t ni, nf;
+    if (bytes_of(env, argv[1], ids, &ni) || bytes_of(env, argv[2], isos, &nf) || ni / 4 != nf / 4) {
+        napi_throw_type_error(env, NULL, "expected (engine, Uint32Array, Float32Array) of one length");
+        return -1;
+    }
+    *nq = (uint32_t)(ni / 4);
+    return 0;
+}
+
+/* engRun(eng, Uint32Array volumeIds, Float32Array isovalues) -> [{...}] */
+static napi_value eng_run(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(3);
+    EXT(0, bcmc_engine, eng);
+    void *ids, *isos;
+    uint32_t nq;
+    if (eng_run_args(env, argv, &ids, &isos, &nq))
+        return NULL;
+    bcmc_query_result *r = (bcmc_query_result *)calloc(nq ? nq : 1, sizeof(*r));
+    if (!r) {
+        napi_throw_error(env, NULL, "out of host memory");
+        return NULL;
+    }
+    char msg[640];
+    pthread_mutex_lock(&g_lib_mu);
+    const int rc = bcmc_engine_run(eng, (const uint32_t *)ids, (const float *)isos, nq, r);
+    if (rc != BCMC_OK)
+        snprintf(msg, sizeof(msg), "bcmc error %d: %s", rc, bcmc_last_error());
+    pthread_mutex_unlock(&g_lib_mu);
+    if (rc != BCMC_OK) {
+        free(r);
+        napi_throw_error(env, NULL, msg);
+        return NULL;
+    }
+    napi_value out = eng_run_result(env, r, nq);
+    free(r);
+    return out;
 }
 
 static napi_value eng_vertex_buffer(napi_env env, napi_callback_info info)
@@ -636,36 +713,10 @@ static napi_value eng_stats(napi_env env, napi_callback_info info)
     return o;
 }
 
-/* engServe(eng, Float64Array arrivals, Uint32Array ids, Float32Array isos,
- * maxBatch, vertexBudgetBytes) -> [{arrival, start, finish, numVertices,
- * batchSize, step}] (seconds from the call) */
-static napi_value eng_serve(napi_env env, napi_callback_info info)
+static napi_value eng_serve_result(napi_env env, const bcmc_serve_result *r, uint32_t n)
 {
-    GET_ARGS(6);
-    EXT(0, bcmc_engine, eng);
-    void *arr, *ids, *isos;
-    size_t na, ni, nf;
-    if (bytes_of(env, argv[1], &arr, &na) || bytes_of(env, argv[2], &ids, &ni) ||
-        bytes_of(env, argv[3], &isos, &nf) || na / 8 != ni / 4 || ni / 4 != nf / 4) {
-        napi_throw_type_error(env, NULL,
-                              "expected (engine, Float64Array, Uint32Array, Float32Array) of one length");
-        return NULL;
-    }
-    const uint32_t n = (uint32_t)(ni / 4);
-    bcmc_serve_opts opts = {(uint32_t)num(env, argv[4]), (uint64_t)num(env, argv[5])};
-    bcmc_serve_result *r = (bcmc_serve_result *)calloc(n ? n : 1, sizeof(*r));
-    if (!r) {
-        napi_throw_error(env, NULL, "out of host memory");
-        return NULL;
-    }
-    const int rc = bcmc_serve_trace(eng, (const double *)arr, (const uint32_t *)ids, (const float *)isos, n,
-                                    &opts, r);
-    if (rc != BCMC_OK) {
-        free(r);
-        BCMC_CALL(env, rc);
-    }
     napi_value out;
-    napi_create_array_with_length(env, n, &out);
+    NAPI_CALL(env, napi_create_array_with_length(env, n, &out));
     for (uint32_t i = 0; i < n; ++i) {
         napi_value o;
         napi_create_object(env, &o);
@@ -677,8 +728,351 @@ static napi_value eng_serve(napi_env env, napi_callback_info info)
         set_num(env, o, "step", r[i].step);
         napi_set_element(env, out, i, o);
     }
+    return out;
+}
+
+/* (engine, Float64Array, Uint32Array, Float32Array, maxBatch, budget) */
+static int eng_serve_args(napi_env env, napi_value *argv, void **arr, void **ids, void **isos, uint32_t *n,
+                          bcmc_serve_opts *opts)
+{
+    size_t na, ni, nf;
+    if (bytes_of(env, argv[1], arr, &na) || bytes_of(env, argv[2], ids, &ni) ||
+        bytes_of(env, argv[3], isos, &nf) || na / 8 != ni / 4 || ni / 4 != nf / 4) {
+        napi_throw_type_error(env, NULL,
+                              "expected (engine, Float64Array, Uint32Array, Float32Array) of one length");
+        return -1;
+    }
+    *n = (uint32_t)(ni / 4);
+    opts->max_batch = (uint32_t)num(env, argv[4]);
+    opts->vertex_budget_bytes = (uint64_t)num(env, argv[5]);
+    return 0;
+}
+
+/* engServe(eng, Float64Array arrivals, Uint32Array ids, Float32Array isos,
+ * maxBatch, vertexBudgetBytes) -> [{arrival, start, finish, numVertices,
+ * batchSize, step}] (seconds from the call) */
+static napi_value eng_serve(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(6);
+    EXT(0, bcmc_engine, eng);
+    void *arr, *ids, *isos;
+    uint32_t n;
+    bcmc_serve_opts opts;
+    if (eng_serve_args(env, argv, &arr, &ids, &isos, &n, &opts))
+        return NULL;
+    bcmc_serve_result *r = (bcmc_serve_result *)calloc(n ? n : 1, sizeof(*r));
+    if (!r) {
+        napi_throw_error(env, NULL, "out of host memory");
+        return NULL;
+    }
+    char msg[640];
+    pthread_mutex_lock(&g_lib_mu);
+    const int rc = bcmc_serve_trace(eng, (const double *)arr, (const uint32_t *)ids, (const float *)isos, n,
+                                    &opts, r);
+    if (rc != BCMC_OK)
+        snprintf(msg, sizeof(msg), "bcmc error %d: %s", rc, bcmc_last_error());
+    pthread_mutex_unlock(&g_lib_mu);
+    if (rc != BCMC_OK) {
+        free(r);
+        napi_throw_error(env, NULL, msg);
+        return NULL;
+    }
+    napi_value out = eng_serve_result(env, r, n);
     free(r);
     return out;
+}
+
+/* ---- asynchronous forms ----------------------------------------------------- */
+enum { J_SCAN, J_COMPACT_IDS, J_COMPACT_FLAGS, J_SORT, J_LRU_UPDATE, J_MC, J_ENG_RUN, J_ENG_SERVE };
+
+typedef struct {
+    napi_async_work work;
+    napi_deferred deferred;
+    napi_ref keep[4];
+    int nkeep;
+    int kind;
+    int rc;
+    char err[640];
+    /* arguments */
+    void *h;
+    void *p[4];
+    uint32_t u[3];
+    int flag;
+    float f;
+    bcmc_serve_opts opts;
+    /* results */
+    uint32_t count;
+    uint64_t nv;
+    bcmc_buf *buf;
+    void *res;
+    bcmc_perf perf;
+    bcmc_lru_perf lperf;
+} job;
+
+static void job_exec(napi_env env, void *data)
+{
+    (void)env;
+    job *j = (job *)data;
+    pthread_mutex_lock(&g_lib_mu);
+    switch (j->kind) {
+    case J_SCAN:
+        j->rc = bcmc_scan_exclusive_u32((bcmc_ctx *)j->h, (bcmc_buf *)j->p[0], j->u[0], j->u[1], &j->count);
+        break;
+    case J_COMPACT_IDS:
+        j->rc = bcmc_compact_ids((bcmc_ctx *)j->h, j->u[0], (bcmc_buf *)j->p[0], (bcmc_buf *)j->p[1],
+                                 (bcmc_buf *)j->p[2]);
+        break;
+    case J_COMPACT_FLAGS:
+        j->rc = bcmc_compact_flags((bcmc_ctx *)j->h, j->u[0], (bcmc_buf *)j->p[0], (bcmc_buf *)j->p[1],
+                                   &j->count);
+        break;
+    case J_SORT:
+        j->rc = bcmc_sort_u32_by_key((bcmc_ctx *)j->h, (bcmc_buf *)j->p[0], (bcmc_buf *)j->p[1], j->u[0],
+                                     j->flag);
+        break;
+    case J_LRU_UPDATE:
+        j->rc = bcmc_lru_update_timed((bcmc_lru *)j->h, (bcmc_buf *)j->p[0], &j->count, &j->buf,
+                                      j->flag ? &j->lperf : NULL);
+        break;
+    case J_MC:
+        j->rc = mc_surface_call((bcmc_mc *)j->h, j->f, j->flag, &j->perf, &j->nv);
+        break;
+    case J_ENG_RUN:
+        j->rc = bcmc_engine_run((bcmc_engine *)j->h, (const uint32_t *)j->p[0], (const float *)j->p[1], j->u[0],
+                                (bcmc_query_result *)j->res);
+        break;
+    case J_ENG_SERVE:
+        j->rc = bcmc_serve_trace((bcmc_engine *)j->h, (const double *)j->p[0], (const uint32_t *)j->p[1],
+                                 (const float *)j->p[2], j->u[0], &j->opts, (bcmc_serve_result *)j->res);
+        break;
+    default:
+        j->rc = BCMC_ERR_INVALID;
+    }
+    if (j->rc != BCMC_OK)
+        snprintf(j->err, sizeof(j->err), "bcmc error %d: %s", j->rc, bcmc_last_error());
+    pthread_mutex_unlock(&g_lib_mu);
+}
+
+static void job_done(napi_env env, napi_status status, void *data)
+{
+    job *j = (job *)data;
+    if (status != napi_ok && j->rc == BCMC_OK) {
+        j->rc = BCMC_ERR_STATE;
+        snprintf(j->err, sizeof(j->err), "async work did not run (status %d)", (int)status);
+    }
+    if (j->rc != BCMC_OK) {
+        napi_value msg, e;
+        napi_create_string_utf8(env, j->err, NAPI_AUTO_LENGTH, &msg);
+        napi_create_error(env, NULL, msg, &e);
+        napi_reject_deferred(env, j->deferred, e);
+    } else {
+        napi_value v = NULL;
+        switch (j->kind) {
+        case J_SCAN: case J_COMPACT_FLAGS: v = mk_num(env, j->count); break;
+        case J_LRU_UPDATE: v = lru_update_result(env, j->count, j->buf, j->flag ? &j->lperf : NULL); break;
+        case J_MC: v = mc_surface_result(env, &j->perf, j->nv); break;
+        case J_ENG_RUN: v = eng_run_result(env, (const bcmc_query_result *)j->res, j->u[0]); break;
+        case J_ENG_SERVE: v = eng_serve_result(env, (const bcmc_serve_result *)j->res, j->u[0]); break;
+        default: break;
+        }
+        if (!v)
+            napi_get_undefined(env, &v);
+        napi_resolve_deferred(env, j->deferred, v);
+    }
+    for (int i = 0; i < j->nkeep; ++i)
+        napi_delete_reference(env, j->keep[i]);
+    napi_delete_async_work(env, j->work);
+    free(j->res);
+    free(j);
+}
+
+/* Queues j; `keep` are the JS values whose memory the work reads. */
+static napi_value job_start(napi_env env, job *j, napi_value *keep, int nkeep)
+{
+    napi_value promise, name;
+    if (napi_create_promise(env, &j->deferred, &promise) != napi_ok) {
+        free(j->res);
+        free(j);
+        napi_throw_error(env, NULL, "napi_create_promise failed");
+        return NULL;
+    }
+    for (int i = 0; i < nkeep; ++i)
+        napi_create_reference(env, keep[i], 1, &j->keep[j->nkeep++]);
+    napi_create_string_utf8(env, "bcmc", NAPI_AUTO_LENGTH, &name);
+    napi_create_async_work(env, NULL, name, job_exec, job_done, j, &j->work);
+    napi_queue_async_work(env, j->work);
+    return promise;
+}
+
+static job *job_new(napi_env env, int kind)
+{
+    job *j = (job *)calloc(1, sizeof(job));
+    if (!j)
+        napi_throw_error(env, NULL, "out of host memory");
+    else
+        j->kind = kind;
+    return j;
+}
+
+/* scanExclusiveAsync(ctx, buf, alignedSize, dataSize) -> Promise<total> */
+static napi_value scan_exclusive_async(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(4);
+    EXT(0, bcmc_ctx, ctx);
+    EXT(1, bcmc_buf, buf);
+    job *j = job_new(env, J_SCAN);
+    if (!j)
+        return NULL;
+    j->h = ctx;
+    j->p[0] = buf;
+    j->u[0] = (uint32_t)num(env, argv[2]);
+    j->u[1] = (uint32_t)num(env, argv[3]);
+    return job_start(env, j, NULL, 0);
+}
+
+/* compactIdsAsync(ctx, n, flags, offsets, out) -> Promise */
+static napi_value compact_ids_async(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(5);
+    EXT(0, bcmc_ctx, ctx);
+    EXT(2, bcmc_buf, flags);
+    EXT(3, bcmc_buf, offsets);
+    EXT(4, bcmc_buf, out);
+    job *j = job_new(env, J_COMPACT_IDS);
+    if (!j)
+        return NULL;
+    j->h = ctx;
+    j->u[0] = (uint32_t)num(env, argv[1]);
+    j->p[0] = flags;
+    j->p[1] = offsets;
+    j->p[2] = out;
+    return job_start(env, j, NULL, 0);
+}
+
+/* compactFlagsAsync(ctx, n, flags, out) -> Promise<count> */
+static napi_value compact_flags_async(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(4);
+    EXT(0, bcmc_ctx, ctx);
+    EXT(2, bcmc_buf, flags);
+    EXT(3, bcmc_buf, out);
+    job *j = job_new(env, J_COMPACT_FLAGS);
+    if (!j)
+        return NULL;
+    j->h = ctx;
+    j->u[0] = (uint32_t)num(env, argv[1]);
+    j->p[0] = flags;
+    j->p[1] = out;
+    return job_start(env, j, NULL, 0);
+}
+
+/* sortByKeyAsync(ctx, keys, vals, n, reverse) -> Promise */
+static napi_value sort_by_key_async(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(5);
+    EXT(0, bcmc_ctx, ctx);
+    EXT(1, bcmc_buf, keys);
+    EXT(2, bcmc_buf, vals);
+    bool rev = false;
+    napi_get_value_bool(env, argv[4], &rev);
+    job *j = job_new(env, J_SORT);
+    if (!j)
+        return NULL;
+    j->h = ctx;
+    j->p[0] = keys;
+    j->p[1] = vals;
+    j->u[0] = (uint32_t)num(env, argv[3]);
+    j->flag = rev ? 1 : 0;
+    return job_start(env, j, NULL, 0);
+}
+
+/* lruUpdateAsync(lru, needed, timed) -> Promise<[n, ids, perf]> */
+static napi_value lru_update_async(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(3);
+    EXT(0, bcmc_lru, lru);
+    EXT(1, bcmc_buf, needed);
+    bool timed = false;
+    if (argc > 2)
+        napi_get_value_bool(env, argv[2], &timed);
+    job *j = job_new(env, J_LRU_UPDATE);
+    if (!j)
+        return NULL;
+    j->h = lru;
+    j->p[0] = needed;
+    j->flag = timed ? 1 : 0;
+    return job_start(env, j, NULL, 0);
+}
+
+/* mcComputeSurfaceAsync(mc, iso, timed) -> Promise<{...}> */
+static napi_value mc_compute_surface_async(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(3);
+    EXT(0, bcmc_mc, mc);
+    bool timed = true;
+    if (argc > 2)
+        napi_get_value_bool(env, argv[2], &timed);
+    job *j = job_new(env, J_MC);
+    if (!j)
+        return NULL;
+    j->h = mc;
+    j->f = (float)num(env, argv[1]);
+    j->flag = timed ? 1 : 0;
+    return job_start(env, j, NULL, 0);
+}
+
+/* engRunAsync(eng, Uint32Array, Float32Array) -> Promise<[{...}]> */
+static napi_value eng_run_async(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(3);
+    EXT(0, bcmc_engine, eng);
+    void *ids, *isos;
+    uint32_t nq;
+    if (eng_run_args(env, argv, &ids, &isos, &nq))
+        return NULL;
+    job *j = job_new(env, J_ENG_RUN);
+    if (!j)
+        return NULL;
+    j->res = calloc(nq ? nq : 1, sizeof(bcmc_query_result));
+    if (!j->res) {
+        free(j);
+        napi_throw_error(env, NULL, "out of host memory");
+        return NULL;
+    }
+    j->h = eng;
+    j->p[0] = ids;
+    j->p[1] = isos;
+    j->u[0] = nq;
+    return job_start(env, j, argv + 1, 2);
+}
+
+/* engServeAsync(eng, Float64Array, Uint32Array, Float32Array, maxBatch,
+ * vertexBudgetBytes) -> Promise<[{...}]> */
+static napi_value eng_serve_async(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(6);
+    EXT(0, bcmc_engine, eng);
+    void *arr, *ids, *isos;
+    uint32_t n;
+    bcmc_serve_opts opts;
+    if (eng_serve_args(env, argv, &arr, &ids, &isos, &n, &opts))
+        return NULL;
+    job *j = job_new(env, J_ENG_SERVE);
+    if (!j)
+        return NULL;
+    j->res = calloc(n ? n : 1, sizeof(bcmc_serve_result));
+    if (!j->res) {
+        free(j);
+        napi_throw_error(env, NULL, "out of host memory");
+        return NULL;
+    }
+    j->h = eng;
+    j->p[0] = arr;
+    j->p[1] = ids;
+    j->p[2] = isos;
+    j->u[0] = n;
+    j->opts = opts;
+    return job_start(env, j, argv + 1, 3);
 }
 
 /* zfpEncode(ctx, fieldBuf, dims[3], maxBits, streamBuf) */
@@ -748,6 +1142,10 @@ static napi_value init(napi_env env, napi_value exports)
         FN("engRun", eng_run), FN("engVertexBuffer", eng_vertex_buffer), FN("engSetFormat", eng_set_format),
         FN("engStats", eng_stats), FN("engServe", eng_serve),
         FN("zfpEncode", zfp_encode), FN("verticesWorld", vertices_world),
+        FN("scanExclusiveAsync", scan_exclusive_async), FN("compactIdsAsync", compact_ids_async),
+        FN("compactFlagsAsync", compact_flags_async), FN("sortByKeyAsync", sort_by_key_async),
+        FN("lruUpdateAsync", lru_update_async), FN("mcComputeSurfaceAsync", mc_compute_surface_async),
+        FN("engRunAsync", eng_run_async), FN("engServeAsync", eng_serve_async),
     };
     napi_define_properties(env, exports, sizeof(d) / sizeof(d[0]), d);
     return exports;

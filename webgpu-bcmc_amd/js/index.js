@@ -10,7 +10,10 @@
  * over the N-API addon (bcmc_addon.node), which calls the gfx950 HIP library
  * through its C ABI.  Methods keep the reference's names, arguments and
  * Promise-returning shape; `device` is a Device from requestDevice() instead
- * of a GPUDevice and buffers are DeviceBuffers instead of GPUBuffers.
+ * of a GPUDevice and buffers are DeviceBuffers instead of GPUBuffers.  The
+ * GPU work of every async method runs on a worker thread (the addon's *Async
+ * calls), so the event loop keeps running while it is pending; each device
+ * runs its operations in the order they were called, as a GPU queue does.
  */
 const path = require('path');
 
@@ -49,6 +52,14 @@ class Device {
     constructor(index = 0) {
         this.handle = native.ctxCreate(index);
         this.info = native.ctxInfo(this.handle);
+        this._tail = Promise.resolve();
+    }
+    // Runs start() (which returns the addon's Promise) after every operation
+    // submitted before it on this device; resolves to its result.
+    _submit(start) {
+        const p = this._tail.then(start);
+        this._tail = p.catch(() => {});
+        return p;
     }
     createBuffer({size}) {
         return new DeviceBuffer(this, native.bufAlloc(this.handle, Math.max(size, 4)));
@@ -59,6 +70,7 @@ class Device {
         return b;
     }
     async onSubmittedWorkDone() {
+        await this._tail;
         native.ctxSync(this.handle);
     }
     // Times launch-shape candidates of the scan/sort/decode kernels and keeps
@@ -107,8 +119,9 @@ class ExclusiveScanner {
         this.inputBuf = gpuBuffer;
     }
     async scan(dataSize) {
-        return native.scanExclusive(
-            this.scanPipeline.device.handle, this.inputBuf.handle, this.inputSize, dataSize);
+        const dev = this.scanPipeline.device;
+        return dev._submit(
+            () => native.scanExclusiveAsync(dev.handle, this.inputBuf.handle, this.inputSize, dataSize));
     }
 }
 
@@ -117,13 +130,14 @@ class StreamCompact {
         this.device = device;
     }
     async compactActiveIDs(numElements, isActiveBuffer, offsetsBuffer, outputBuffer) {
-        native.compactIds(this.device.handle, numElements, isActiveBuffer.handle,
-                          offsetsBuffer.handle, outputBuffer.handle);
+        await this.device._submit(() => native.compactIdsAsync(
+            this.device.handle, numElements, isActiveBuffer.handle, offsetsBuffer.handle,
+            outputBuffer.handle));
     }
     // Fused scan + compaction of 0/1 flags in one pass; resolves to the count.
     async compactFlags(numElements, flagsBuffer, outputBuffer) {
-        return native.compactFlags(
-            this.device.handle, numElements, flagsBuffer.handle, outputBuffer.handle);
+        return this.device._submit(() => native.compactFlagsAsync(
+            this.device.handle, numElements, flagsBuffer.handle, outputBuffer.handle));
     }
 }
 
@@ -135,7 +149,8 @@ class RadixSorter {
         return native.sortAlignedSize(size);
     }
     async sort(keys, values, size, reverse) {
-        native.sortByKey(this.device.handle, keys.handle, values.handle, size, !!reverse);
+        await this.device._submit(() => native.sortByKeyAsync(
+            this.device.handle, keys.handle, values.handle, size, !!reverse));
     }
 }
 
@@ -170,24 +185,25 @@ class LRUCache {
     get cache() {
         return new DeviceBuffer(this.device, native.lruCacheBuffer(this.handle), false);
     }
-    // Returns [numNewItems, newItemIDs (DeviceBuffer, caller destroys) | undefined]
+    // Returns [numNewItems, newItemIDs (DeviceBuffer, caller destroys) | undefined].
+    // With a perfTracker, each of the reference's 11 lru* keys gets one entry
+    // per update: device milliseconds of the stage, or the count.
     async update(itemNeeded, perfTracker) {
         if (!(itemNeeded instanceof DeviceBuffer)) {
             throw new TypeError('itemNeeded info must be a DeviceBuffer');
         }
-        const start = Date.now();
-        const [n, ids] = native.lruUpdate(this.handle, itemNeeded.handle);
+        const [n, ids, perf] = await this.device._submit(
+            () => native.lruUpdateAsync(this.handle, itemNeeded.handle, !!perfTracker));
         if (perfTracker) {
             if (perfTracker.lruMarkNewItems === undefined) {
                 for (const k of LRU_PERF_KEYS) perfTracker[k] = [];
             }
-            perfTracker.lruNumNewItems.push(n);
-            perfTracker.lruMarkNewItems.push(Date.now() - start);
+            for (const k of LRU_PERF_KEYS) perfTracker[k].push(perf[k]);
         }
         return [n, ids ? new DeviceBuffer(this.device, ids) : undefined];
     }
     async reset() {
-        native.lruReset(this.handle);
+        await this.device._submit(async () => native.lruReset(this.handle));
     }
     // Slot state for inspection: {age, available, item, itemSlot}
     state() {
@@ -241,7 +257,8 @@ class CompressedMarchingCubes {
     }
     async computeSurface(isovalue, perfTracker) {
         // without a perfTracker the captured query graph is replayed (no stage times)
-        const r = native.mcComputeSurface(this.handle, isovalue, !!perfTracker);
+        const r = await this.device._submit(
+            () => native.mcComputeSurfaceAsync(this.handle, isovalue, !!perfTracker));
         if (perfTracker) {
             if (perfTracker.computeActiveBlocks === undefined) {
                 perfTracker.totalBlocks = this.totalBlocks;
@@ -297,7 +314,8 @@ class Engine {
     }
     // resolves to [{numVertices, vertexOffset, numActiveBlocks, stepDecodedBlocks}]
     async run(volumeIds, isovalues) {
-        return native.engRun(this.handle, Uint32Array.from(volumeIds), Float32Array.from(isovalues));
+        const ids = Uint32Array.from(volumeIds), isos = Float32Array.from(isovalues);
+        return this.device._submit(() => native.engRunAsync(this.handle, ids, isos));
     }
     // the last step's vertices; query i's run from results[i].vertexOffset
     get vertexBuffer() {
@@ -309,8 +327,10 @@ class Engine {
     // arrivals in seconds from the call; resolves to per-request
     // {arrival, start, finish, numVertices, batchSize, step}
     async serve(arrivals, volumeIds, isovalues, {maxBatch = 0, vertexBudgetBytes = 0} = {}) {
-        return native.engServe(this.handle, Float64Array.from(arrivals), Uint32Array.from(volumeIds),
-                               Float32Array.from(isovalues), maxBatch, vertexBudgetBytes);
+        const a = Float64Array.from(arrivals), ids = Uint32Array.from(volumeIds);
+        const isos = Float32Array.from(isovalues);
+        return this.device._submit(
+            () => native.engServeAsync(this.handle, a, ids, isos, maxBatch, vertexBudgetBytes));
     }
     destroy() {
         if (this.handle) native.engDestroy(this.handle);
