@@ -118,9 +118,12 @@ BCMC_API int bcmc_sort_u32_by_key(bcmc_ctx *ctx, bcmc_buf *keys, bcmc_buf *vals,
 
 /* ---- LRUCache (js/lru_cache.js) -------------------------------------------- */
 /* new LRUCache(device, scan, compact, initialSize, elementSize, totalElements)
- * (lru_cache.js:4-305).  Slot storage for alignTo(total, 32) elements is
- * reserved up front so the cache grows without reallocation; the logical size
- * follows the reference exactly. */
+ * (lru_cache.js:4-305).  The slot contents are stored for the logical size
+ * plus room for the next growth and moved to a 1.5x larger allocation when
+ * an update grows the cache past it (the reference re-creates its buffers the
+ * same way, lru_cache.js:509-672); the per-slot bookkeeping (age,
+ * availability, item: 24 bytes a slot) is sized for totalElements once.  The
+ * logical size follows the reference exactly. */
 BCMC_API int bcmc_lru_create(bcmc_ctx *ctx, uint32_t initial_size, uint32_t elem_bytes,
                              uint32_t total, bcmc_lru **out);
 BCMC_API int bcmc_lru_destroy(bcmc_lru *lru);
@@ -151,7 +154,9 @@ typedef struct {
     uint32_t total_elements;  /* this.totalElements */
     uint32_t element_size;    /* this.elementSize (bytes) */
     uint32_t slots_available; /* this.displayNumSlotsAvailable after the last update */
-    uint64_t reserved_bytes;  /* physical slot storage */
+    uint64_t reserved_bytes;  /* device memory of the slot contents: follows cacheSize (plus
+                                 room for growth), not totalElements */
+    uint64_t logical_bytes;   /* cacheSize * elementSize */
 } bcmc_lru_info;
 BCMC_API int bcmc_lru_get_info(bcmc_lru *lru, bcmc_lru_info *info);
 /* Slot state for inspection: age[C], available[C], item[C] (-1 = empty) and

@@ -77,6 +77,14 @@ def test_blobs1024x1024x1080_rate2_one_query(dev):
     mc, ref = _setup(dev, "blobs1024x1024x1080_r2", 0.10)
     nv, r = _check(mc, ref, 0.5, "blobs1024")
     assert nv > 0
+    # the cache's storage follows its logical size (the query grows it from
+    # 10% of the blocks to what it needs, plus room for the next growth), not
+    # the 4.6 GB of all blocks decoded
+    info = mc.lruCache.info()
+    say("  cache slots", info["cache_size"], "storage", info["reserved_bytes"])
+    all_blocks = oracle.nblocks(synth.WORKLOADS["blobs1024x1024x1080_r2"]["dims"]) * 256
+    assert info["logical_bytes"] <= info["reserved_bytes"] <= info["logical_bytes"] * 3 // 2 + (64 << 20)
+    assert info["reserved_bytes"] < all_blocks
     # the same query with FP32 global xyz output (12 B per vertex): exactly
     # 4 * block_pos + local in FP32, within 1e-5 + one FP32 ulp of |p| of
     # the FP64 position; compared in slices to bound host memory

@@ -82,3 +82,26 @@ def test_lru_long_run_after_reset_matches_oracle(dev):
         if k % 10 == 0 or 125 <= k <= 130:
             same_state(k)
     same_state("end")
+
+
+def test_lru_perf_tracker_has_the_reference_keys(dev):
+    """LRUCache.update(needed, perfTracker): one entry per update under each
+    of the reference's 11 lru* keys; counts equal the oracle's; with no new
+    items every stage after the count is 0 (lru_cache.js:468-480)."""
+    total, initial = 5000, 100
+    gpu = bcmc_amd.LRUCache(dev, None, None, initial, 256, total)
+    cpu = oracle.LRU(initial, 256, total)
+    size0 = len(cpu.state()[0])
+    needed = np.zeros(total, np.uint32)
+    needed[1000:2500] = 1
+    perf = {}
+    for _ in range(2):
+        n_g, _ = gpu.update(dev.buffer_from(needed), perf)
+        assert n_g == cpu.update(needed)[0]
+    assert list(perf) == list(bcmc_amd.LRU_PERF_KEYS)
+    assert all(len(v) == 2 for v in perf.values())
+    assert perf["lruNumNewItems"] == [1500, 0]
+    assert perf["lruNumSlotsAvailable"][0] == size0
+    assert perf["lruGrowCache"][0] > 0 and perf["lruTotalSortTime"][0] > 0 and perf["lruWriteNewItems"][0] > 0
+    assert perf["lruMarkNewItems"][1] > 0
+    assert all(perf[k][1] == 0 for k in bcmc_amd.LRU_PERF_KEYS[3:])

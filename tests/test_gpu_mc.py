@@ -208,3 +208,33 @@ def test_world_positions_follow_the_renderer_formula(dev):
                           want[-5:].view(np.uint32))
     with pytest.raises(bcmc_amd.BcmcError):
         bcmc_amd.world_positions(dev, mc.vertexBuffer, 10, dims, scale, first=nv - 5)
+
+
+def test_cache_memory_follows_the_logical_size(dev):
+    """The LRU's slot contents are stored for its logical size plus room for
+    growth, not for every block.  A first query that grows a 64-slot cache
+    past its storage assigns nothing, enlarges the storage (contents copied)
+    and runs its assignment, decode and MC pass again: counts, cache state and
+    vertices still equal the oracle's, and so do the queries after it."""
+    dims, rate = (320, 320, 320), 4
+    f = synth.field(synth.BLOBS, 4, dims, 64, (8, 40, 0.5, 1))
+    s = synth.encode(f, 64 * rate)
+    n = oracle.nblocks(dims)
+    mc = bcmc_amd.CompressedMarchingCubes(dev)
+    mc.setCompressedVolume(s, rate, dims, (1, 1, 1), 64)
+    info0 = mc.lruCache.info()
+    say("cache storage before the first query", info0)
+    assert info0["logical_bytes"] == 64 * 256 and info0["reserved_bytes"] < 128 * 256 + 256
+    ref = oracle.MC(s, 64 * rate, dims, initial_cache=64)
+    for iso in (0.5, 0.45, 0.55, 0.52):
+        nv, r = _check_query(mc, ref, iso)
+        assert np.array_equal(mc.vertices().view(np.uint32), r["verts_f32"].view(np.uint32)), iso
+        info = mc.lruCache.info()
+        say("  iso", iso, "cache slots", info["cache_size"], "storage", info["reserved_bytes"])
+        assert info["logical_bytes"] == info["cache_size"] * 256 < info["reserved_bytes"]
+    # the first query grew the cache far past its 64 slots (the re-run
+    # path); the storage stays well below all blocks decoded
+    assert ref.stats["cache_size"] > 10000
+    assert info["reserved_bytes"] < n * 256 // 2
+    rep = mc.reportMemoryUse()
+    assert rep["cache"] == info["reserved_bytes"] and rep["total"] > rep["cache"]

@@ -86,6 +86,7 @@ int dev_alloc(void **p, size_t bytes);
 void dev_free(void *p);
 int make_buf(bcmc_ctx *ctx, uint64_t bytes, bcmc_buf **out);
 
+
 // ---------------------------------------------------------------------------
 // Single-pass scan state (decoupled look-back).  One per call site; the
 // kernels reset it themselves, so launches need no memset (graph friendly).

@@ -26,7 +26,8 @@ struct QState {
     unsigned long long vert_capacity; // vertices the output buffer holds
     uint32_t avail_display;    // S' - D (displayNumSlotsAvailable)
     uint32_t num_avail_total;  // S': available slots after growth
-    uint32_t pad[4];
+    uint32_t cache_short;      // the assignment did not run: the cache grew past its storage
+    uint32_t pad[3];
 };
 
 // ---------------------------------------------------------------------------
@@ -200,6 +201,9 @@ void launch_decode_to_cache(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwor
 // LRU assignment fused with the decode: new item w (w < D, 0 when no slot is
 // available) evicts the previous item of slot order[w], takes the slot
 // {age 0, unavailable} and is decoded into it; commits the cache size.
+// `capacity` is the number of slots `cache` holds: when the grown cache size
+// exceeds it nothing is assigned and q->cache_short is set (the host enlarges
+// the storage and launches it again).
 void launch_decode_assign(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords, uint32_t maxbits,
                           const uint32_t *new_items, uint32_t n_max, const uint32_t *order,
                           uint32_t *slot_age, uint32_t *slot_avail, int32_t *slot_item,

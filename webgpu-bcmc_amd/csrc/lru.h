@@ -20,7 +20,12 @@ struct LruView {
 
 struct LruState {
     LruView v;
-    uint8_t *data = nullptr;      // cache contents, capacity * elem_bytes
+    // cache contents: `slots` slots of elem_bytes and a zero slot after them
+    // (absent blocks read it).  The storage follows the logical size: it
+    // holds the cache size plus room for growth, and is enlarged (contents
+    // copied) when a query grows the cache past it, up to `capacity`.
+    uint8_t *data = nullptr;
+    uint32_t slots = 0;
     uint32_t *new_items = nullptr;// [aligned_total] ids to decode, ascending
     uint32_t total = 0, aligned_total = 0, elem_bytes = 0, capacity = 0;
     uint32_t cache_size = 0;      // host mirror of the logical size
@@ -37,6 +42,10 @@ struct LruState {
 
 int lru_state_alloc(bcmc_ctx *ctx, LruState *L, uint32_t initial, uint32_t elem_bytes,
                     uint32_t total);
+// Makes the storage hold at least `slots` slots (at most the capacity):
+// a larger one, 1.5x at least, with the contents copied.  Synchronises the
+// stream when it moves; callers refresh the pointers they keep.
+int lru_reserve(bcmc_ctx *ctx, LruState *L, uint64_t slots);
 void lru_state_free(LruState *L);
 // LRUCache.update for a marching-cubes query whose misses the active pass has
 // already listed (new_items, q->num_new): `flags` are the N active flags; does

@@ -206,9 +206,7 @@ int lru_state_alloc(bcmc_ctx *ctx, LruState *L, uint32_t initial, uint32_t elem_
     BCMC_TRY(dev_alloc((void **)&L->v.avail_keys, cap * 4));
     BCMC_TRY(dev_alloc((void **)&L->v.order, cap * 4));
     BCMC_TRY(dev_alloc((void **)&L->v.item_slot, (size_t)L->aligned_total * 4 + 4));
-    // one extra zeroed 256-byte slot past the end: absent blocks read it
-    BCMC_TRY(dev_alloc((void **)&L->data, cap * elem_bytes + 256));
-    BCMC_HIP_CHECK(hipMemsetAsync((char *)L->data + cap * elem_bytes, 0, 256, ctx->stream));
+
     BCMC_TRY(dev_alloc((void **)&L->new_items, (size_t)L->aligned_total * 4 + 4));
     BCMC_TRY(dev_alloc((void **)&L->ctl, 8));
     BCMC_TRY(scan_site_alloc(&L->site_miss, L->aligned_total));
@@ -221,6 +219,25 @@ int lru_state_alloc(bcmc_ctx *ctx, LruState *L, uint32_t initial, uint32_t elem_
     lru_init_kernel<<<grid_for(ctx, cap), 256, 0, ctx->stream>>>(L->v, 0, L->cache_size, 0, 1, L->ctl,
                                                                   kUpdatesUnknown);
     BCMC_HIP_CHECK(hipGetLastError());
+    return lru_reserve(ctx, L, L->cache_size);
+}
+
+int lru_reserve(bcmc_ctx *ctx, LruState *L, uint64_t slots) {
+    slots = std::min<uint64_t>(slots, L->capacity);
+    if (L->data && slots <= L->slots)
+        return BCMC_OK;
+    uint64_t want = std::max<uint64_t>(slots, (uint64_t)L->slots + L->slots / 2);
+    want = std::min<uint64_t>(L->capacity, (want + 31) / 32 * 32);
+    uint8_t *d = nullptr;
+    BCMC_TRY(dev_alloc((void **)&d, want * L->elem_bytes + 256));
+    if (L->data && L->slots)
+        BCMC_HIP_CHECK(hipMemcpyAsync(d, L->data, (size_t)L->slots * L->elem_bytes, hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+    BCMC_HIP_CHECK(hipMemsetAsync(d + want * L->elem_bytes, 0, 256, ctx->stream));
+    BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    dev_free(L->data);
+    L->data = d;
+    L->slots = (uint32_t)want;
     return BCMC_OK;
 }
 
@@ -399,6 +416,10 @@ BCMC_API int bcmc_lru_update_timed(bcmc_lru *lru, bcmc_buf *needed, uint32_t *nu
     BCMC_HIP_CHECK(hipGetLastError());
     BCMC_TRY(lru_sync_state(lru));
     *num_new = lru->qh->num_new;
+    // storage for every slot of the (possibly grown) cache: its contents
+    // belong to the caller, who may write any of them
+    BCMC_TRY(lru_reserve(lru->ctx, &lru->st, lru->st.cache_size));
+    lru->cache_view.ptr = lru->st.data;
     if (perf) {
         float t[5];
         for (int k = 0; k < 5; ++k)
@@ -445,7 +466,8 @@ BCMC_API int bcmc_lru_get_info(bcmc_lru *lru, bcmc_lru_info *info) {
     info->total_elements = lru->st.total;
     info->element_size = lru->st.elem_bytes;
     info->slots_available = lru->qh->avail_display;
-    info->reserved_bytes = (uint64_t)lru->st.capacity * lru->st.elem_bytes;
+    info->reserved_bytes = (uint64_t)lru->st.slots * lru->st.elem_bytes + 256;
+    info->logical_bytes = (uint64_t)lru->st.cache_size * lru->st.elem_bytes;
     return BCMC_OK;
 }
 

@@ -47,6 +47,7 @@ struct ActiveOp {
         q->num_with = 0;
         q->num_verts = 0;
         q->overflow = 0;
+        q->cache_short = 0;
     }
     struct Tile {
         float iso;
@@ -280,7 +281,7 @@ struct bcmc_mc {
     bool use_graph = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    uint64_t graph_key[4] = {0, 0, 0, 0};
+    uint64_t graph_key[6] = {0, 0, 0, 0, 0, 0};
 };
 
 static void mc_drop_graph(bcmc_mc *m) {
@@ -358,7 +359,7 @@ static McArgs mc_args(bcmc_mc *m) {
     a.nb[2] = m->nb[2];
     a.div_x = make_fast_div(m->nb[0]);
     a.div_y = make_fast_div(m->nb[1]);
-    a.cache_cap = m->lru.st.capacity;
+    a.cache_cap = m->lru.st.slots;  // slot `slots` is the zero slot
     a.active_ids = m->active_ids;
     a.q = m->q;
     a.counts = m->counts;
@@ -410,6 +411,25 @@ static int mc_launch_pass(bcmc_mc *m, hipEvent_t *ev) {
     return mc_launch_emit(m);
 }
 
+// The query from the slot assignment on: assignment + decode of the misses
+// into their slots, the MC pass (counts, offsets, vertices) and the read-back
+// of the counters; `timed` records events 3..6.
+static int mc_enqueue_tail(bcmc_mc *m, bool timed) {
+    hipStream_t st = m->ctx->stream;
+    LruState &L = m->lru.st;
+    launch_decode_assign(m->ctx, m->stream, m->stream_words, m->maxbits, L.new_items, m->total, L.v.order,
+                         L.v.age, L.v.avail, L.v.item, L.v.item_slot, L.total, (float *)L.data, L.slots,
+                         m->q);
+    if (timed)
+        BCMC_HIP_CHECK(hipEventRecord(m->ev[3], st));
+    BCMC_TRY(mc_launch_pass(m, timed ? &m->ev[4] : nullptr));
+    if (timed)
+        BCMC_HIP_CHECK(hipEventRecord(m->ev[6], st));
+    qstate_out_kernel<<<1, 64, 0, st>>>(m->q, m->qh_d);
+    BCMC_HIP_CHECK(hipGetLastError());
+    return BCMC_OK;
+}
+
 // One query as a kernel chain on the context's stream (capturable when not
 // timed): the isovalue copy, active scan, LRU update, assignment + decode, MC
 // pass, and the read-back of the counters; `timed` records the perfTracker
@@ -430,19 +450,7 @@ static int mc_enqueue_query(bcmc_mc *m, uint32_t npasses, bool timed) {
     lru_enqueue_update_mc(ctx, &L, m->flags, m->q, npasses);
     if (timed)
         BCMC_HIP_CHECK(hipEventRecord(m->ev[2], st));
-    // slot assignment and decode of the misses into their slots
-    launch_decode_assign(ctx, m->stream, m->stream_words, m->maxbits, L.new_items, m->total, L.v.order,
-                         L.v.age, L.v.avail, L.v.item, L.v.item_slot, L.total, (float *)L.data,
-                         L.capacity, m->q);
-    if (timed)
-        BCMC_HIP_CHECK(hipEventRecord(m->ev[3], st));
-    // counts, offsets and vertices
-    BCMC_TRY(mc_launch_pass(m, timed ? &m->ev[4] : nullptr));
-    if (timed)
-        BCMC_HIP_CHECK(hipEventRecord(m->ev[6], st));
-    qstate_out_kernel<<<1, 64, 0, st>>>(m->q, m->qh_d);
-    BCMC_HIP_CHECK(hipGetLastError());
-    return BCMC_OK;
+    return mc_enqueue_tail(m, timed);
 }
 
 extern "C" {
@@ -592,8 +600,9 @@ BCMC_API int bcmc_mc_compute_surface(bcmc_mc *m, float isovalue, bcmc_perf *perf
     *m->iso_h = isovalue;
     // counted before the launch, so the host's update count never lags the device's
     const uint32_t npasses = lru_begin_update(&m->lru.st);
-    const uint64_t key[4] = {(uint64_t)(uintptr_t)m->verts, m->vert_cap,
-                             (uint64_t)m->fmt | ctx->tune_gen << 8, npasses};
+    const uint64_t key[6] = {(uint64_t)(uintptr_t)m->verts, m->vert_cap,
+                             (uint64_t)m->fmt | ctx->tune_gen << 8, npasses,
+                             (uint64_t)(uintptr_t)m->lru.st.data, m->lru.st.slots};
     bool launched = false;
     if (m->use_graph && !timed) {
         if (!m->gexec || std::memcmp(key, m->graph_key, sizeof(key)) != 0) {
@@ -623,6 +632,26 @@ BCMC_API int bcmc_mc_compute_surface(bcmc_mc *m, float isovalue, bcmc_perf *perf
         BCMC_TRY(mc_enqueue_query(m, npasses, timed));
     BCMC_HIP_CHECK(hipStreamSynchronize(st));
     QState qs = *m->qh;
+    float ms[6] = {};
+    if (timed)
+        for (int i = 0; i < 6; ++i)
+            BCMC_HIP_CHECK(hipEventElapsedTime(&ms[i], m->ev[i], m->ev[i + 1]));
+    if (qs.cache_short) {
+        // the cache grew past its storage, so nothing was assigned: enlarge
+        // the storage, then assign, decode, count and emit again (the LRU
+        // state up to the assignment is complete)
+        BCMC_TRY(lru_reserve(ctx, &m->lru.st, (uint64_t)qs.cache_size_next + qs.num_new / 2 + 32));
+        mc_refresh_views(m);
+        BCMC_HIP_CHECK(hipMemsetAsync(&m->q->cache_short, 0, 4, st));
+        if (timed)
+            BCMC_HIP_CHECK(hipEventRecord(m->ev[2], st));
+        BCMC_TRY(mc_enqueue_tail(m, timed));
+        BCMC_HIP_CHECK(hipStreamSynchronize(st));
+        qs = *m->qh;
+        if (timed)
+            for (int i = 2; i < 6; ++i)
+                BCMC_HIP_CHECK(hipEventElapsedTime(&ms[i], m->ev[i], m->ev[i + 1]));
+    }
     if (qs.num_verts > m->vert_cap) {
         // grow like compressed_marching_cubes.js:774-785, then run the pass again
         const uint64_t want = std::max<uint64_t>(qs.num_verts, (uint64_t)std::floor(m->vert_cap * 1.5));
@@ -639,8 +668,14 @@ BCMC_API int bcmc_mc_compute_surface(bcmc_mc *m, float isovalue, bcmc_perf *perf
         BCMC_HIP_CHECK(hipMemcpyAsync(m->qh, m->q, sizeof(QState), hipMemcpyDeviceToHost, st));
         BCMC_HIP_CHECK(hipStreamSynchronize(st));
         qs = *m->qh;
+        if (timed)
+            BCMC_HIP_CHECK(hipEventElapsedTime(&ms[5], m->ev[5], m->ev[6]));
     }
     m->lru.st.cache_size = qs.cache_size;
+    // storage for some growth of the next query (misses a third of this
+    // one's, which a sweep's later queries stay under); a larger growth takes
+    // the path above
+    BCMC_TRY(lru_reserve(ctx, &m->lru.st, (uint64_t)qs.cache_size + qs.num_new / 2 + 32));
     m->last_active = qs.num_active;
     m->last_with = qs.num_with;
     m->last_new = qs.num_new;
@@ -649,9 +684,6 @@ BCMC_API int bcmc_mc_compute_surface(bcmc_mc *m, float isovalue, bcmc_perf *perf
     if (num_vertices)
         *num_vertices = qs.num_verts;
     if (perf) {
-        float ms[6];
-        for (int i = 0; i < 6; ++i)
-            BCMC_HIP_CHECK(hipEventElapsedTime(&ms[i], m->ev[i], m->ev[i + 1]));
         perf->computeActiveBlocks = ms[0];
         perf->cacheUpdate = ms[1];
         perf->decompression = ms[2];
@@ -702,7 +734,7 @@ BCMC_API int bcmc_mc_memory_use(bcmc_mc *m, uint64_t *total_out, uint64_t *cache
     BCMC_CHECK_ARG(m, "null argument");
     const uint64_t n = m->total;
     const LruState &L = m->lru.st;
-    const uint64_t cache = (uint64_t)L.capacity * L.elem_bytes;
+    const uint64_t cache = (uint64_t)L.slots * L.elem_bytes + 256;  // the storage, not all blocks
     const uint64_t lru_meta = (uint64_t)L.capacity * 4 * 5 + (uint64_t)L.aligned_total * 8;
     const uint64_t mc = m->stream_words * 8 + n * (8 + 4 + 4 + 4 + 4 + 8) +
                         m->vert_cap * vertex_bytes(m->fmt);

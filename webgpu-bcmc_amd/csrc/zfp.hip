@@ -403,6 +403,13 @@ __global__ __launch_bounds__(kZfpThreads) void zfp_kernel(ZfpArgs a) {
             a.q->avail_display = a.q->num_avail_total - n;
             a.q->cache_size = a.q->cache_size_next;
         }
+        // the grown cache has more slots than the storage holds: assign
+        // nothing (the host enlarges the storage and launches this again)
+        if (a.q->cache_size_next > a.capacity) {
+            if (blockIdx.x == 0 && threadIdx.x == 0)
+                a.q->cache_short = 1;
+            n = 0;
+        }
     }
     const uint32_t waves_total = gridDim.x * wpb;
     for (uint32_t chunk = blockIdx.x * wpb + wave; (uint64_t)chunk * 64 < n; chunk += waves_total) {

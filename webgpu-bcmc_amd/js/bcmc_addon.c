@@ -398,6 +398,7 @@ static napi_value lru_info(napi_env env, napi_callback_info info)
     set_num(env, o, "elementSize", li.element_size);
     set_num(env, o, "displayNumSlotsAvailable", li.slots_available);
     set_num(env, o, "reservedBytes", (double)li.reserved_bytes);
+    set_num(env, o, "logicalBytes", (double)li.logical_bytes);
     return o;
 }
 

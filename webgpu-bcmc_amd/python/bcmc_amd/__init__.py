@@ -52,7 +52,7 @@ class _Perf(ctypes.Structure):
 class _LruInfo(ctypes.Structure):
     _fields_ = [("cache_size", ctypes.c_uint32), ("total_elements", ctypes.c_uint32),
                 ("element_size", ctypes.c_uint32), ("slots_available", ctypes.c_uint32),
-                ("reserved_bytes", ctypes.c_uint64)]
+                ("reserved_bytes", ctypes.c_uint64), ("logical_bytes", ctypes.c_uint64)]
 
 
 class _McInfo(ctypes.Structure):
@@ -74,6 +74,15 @@ class _EngineInfo(ctypes.Structure):
                 ("pool_used", ctypes.c_uint32), ("epoch", ctypes.c_uint32),
                 ("pool_bytes", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
                 ("steps", ctypes.c_uint32), ("graph_steps", ctypes.c_uint32)]
+
+
+LRU_PERF_KEYS = ("lruMarkNewItems", "lruScanNeededSlots", "lruNumNewItems", "lruCompactNewItems",
+                 "lruScanSlotsAvailable", "lruNumSlotsAvailable", "lruGrowCache",
+                 "lruCompactAvailableSlots", "lruPrepKeyValue", "lruTotalSortTime", "lruWriteNewItems")
+
+
+class _LruPerf(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in LRU_PERF_KEYS]
 
 
 class _Tuning(ctypes.Structure):
@@ -146,6 +155,7 @@ def lib():
         "bcmc_lru_create": (i32, [vp, u32, u32, u32, P(vp)]),
         "bcmc_lru_destroy": (i32, [vp]),
         "bcmc_lru_update": (i32, [vp, vp, P(u32), P(vp)]),
+        "bcmc_lru_update_timed": (i32, [vp, vp, P(u32), P(vp), P(_LruPerf)]),
         "bcmc_lru_reset": (i32, [vp]),
         "bcmc_lru_get_info": (i32, [vp, P(_LruInfo)]),
         "bcmc_lru_read_state": (i32, [vp, vp, vp, vp, vp]),
@@ -408,9 +418,16 @@ class LRUCache:
             self.h = None
 
     def update(self, item_needed, perf_tracker=None):
+        """LRUCache.update; with a perf_tracker dict, each of the reference's
+        11 lru* keys gets one entry (stage device ms or count)."""
         n = ctypes.c_uint32()
         ids = ctypes.c_void_p()
-        _check(lib().bcmc_lru_update(self.h, item_needed.h, ctypes.byref(n), ctypes.byref(ids)))
+        perf = _LruPerf() if perf_tracker is not None else None
+        _check(lib().bcmc_lru_update_timed(self.h, item_needed.h, ctypes.byref(n), ctypes.byref(ids),
+                                           ctypes.byref(perf) if perf is not None else None))
+        if perf is not None:
+            for k in LRU_PERF_KEYS:
+                perf_tracker.setdefault(k, []).append(getattr(perf, k))
         return n.value, (Buffer(self.device, ids) if ids.value else None)
 
     def reset(self):
