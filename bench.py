@@ -95,9 +95,6 @@ def stage_bytes(s, rate, n_blocks):
     }
 
 
-SERVED_SIZES = [(128, 128, 128), (192, 192, 192), (256, 256, 256), (320, 320, 320),
-                (384, 384, 384), (512, 512, 512), (768, 768, 768), (1024, 1024, 1080)]
-SERVED_RATES = [2, 4, 8, 2, 4, 8, 2, 2]   # cycling 2/4/8; the 1024x1024x1080 volume at 2
 BATCH_SIZES = [(128, 128, 128), (256, 256, 256), (384, 384, 384), (512, 512, 512)]
 BATCH_RATES = [8, 4, 4, 2]
 
@@ -127,10 +124,7 @@ def plain_replay(bcmc_amd, dev, vols, ids, isos, cache_frac=0.25):
 
 def served_bench(bcmc_amd, synth, dev, args):
     t0 = time.time()
-    vols = []
-    for i, (dims, rate) in enumerate(zip(SERVED_SIZES, SERVED_RATES)):
-        s, sha = synth.make_stream(synth.blob_workload(dims, 7 + i, rate))
-        vols.append((s, rate, dims))
+    vols = synth.served_volumes()
     log(f"bench: served volumes generated in {time.time() - t0:.1f}s")
     rng = np.random.default_rng(20240607)
     ids, isos = bcmc_amd.request_trace(rng, args.served_queries, len(vols))

@@ -23,10 +23,7 @@ pytestmark = pytest.mark.gpu
 THREADS = 16
 BATCH = [((128, 128, 128), 8, 3), ((256, 256, 256), 4, 4), ((384, 384, 384), 4, 5),
          ((512, 512, 512), 2, 6)]
-SERVED_SIZES = [(128, 128, 128), (192, 192, 192), (256, 256, 256), (320, 320, 320),
-                (384, 384, 384), (512, 512, 512), (768, 768, 768), (1024, 1024, 1080)]
-SERVED_RATES = [2, 4, 8, 2, 4, 8, 2, 2]
-SERVED = [(d, r, 7 + i) for i, (d, r) in enumerate(zip(SERVED_SIZES, SERVED_RATES))]
+SERVED = [(d, r, 7 + i) for i, (d, r) in enumerate(zip(synth.SERVED_SIZES, synth.SERVED_RATES))]
 
 
 def _volumes(spec):

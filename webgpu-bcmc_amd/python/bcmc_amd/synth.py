@@ -102,6 +102,22 @@ def blob_workload(dims, seed, rate, count=256):
                 params=(16.0, 96.0, 0.5, 1.0), rate=rate, noise=0.0)
 
 
+# The served set: 8 device-resident blob volumes of uneven size, seeds 7-14,
+# zfp rates cycling 2/4/8 (the 1024x1024x1080 volume at 2).
+SERVED_SIZES = [(128, 128, 128), (192, 192, 192), (256, 256, 256), (320, 320, 320),
+                (384, 384, 384), (512, 512, 512), (768, 768, 768), (1024, 1024, 1080)]
+SERVED_RATES = [2, 4, 8, 2, 4, 8, 2, 2]
+
+
+def served_volumes(threads=None):
+    """[(stream, rate, dims)] of the served set."""
+    out = []
+    for i, (dims, rate) in enumerate(zip(SERVED_SIZES, SERVED_RATES)):
+        s, _ = make_stream(blob_workload(dims, 7 + i, rate), threads=threads)
+        out.append((s, rate, dims))
+    return out
+
+
 def make_stream(w, threads=None):
     """Compressed stream for a workload dict; returns (uint8 stream, sha256 hex)."""
     s = stream(w["kind"], w["seed"], w["dims"], w["count"], w["params"], 64 * w["rate"],
