@@ -21,10 +21,12 @@
 //            stamp), evicting their blocks
 //   decode   one launch per volume of the step (its zfp rate selects the
 //            LDS-staged decoder variant)
-//   mc       count, chunk-prefix scan and emit launches over all pairs in
-//            chunks that never span two queries; each query's vertices come
-//            out contiguous, in the same order as a single-query
-//            computeSurface
+//   mc       count and chunk-prefix launches over all pairs in chunks that
+//            never span two queries; the host reads the exact vertex counts,
+//            admits the leading queries whose vertices fit the output budget
+//            and sizes the vertex buffer; then the emit launch over their
+//            chunks.  Each query's vertices come out contiguous, in the same
+//            order as a single-query computeSurface
 // Query results do not depend on which other queries share the step or on
 // the pool state: the decoded values of a block are a pure function of the
 // stream.
