@@ -211,6 +211,33 @@ BCMC_API int bcmc_mc_destroy(bcmc_mc *mc);
 BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *mc, const void *stream, uint64_t bytes,
                                            double rate, const uint32_t dims[3],
                                            const float scale[3], uint32_t initial_cache_slots);
+/* Block value ranges persisted beside a stream, so that a load need not decode
+ * the whole volume (the reference always does, computeBlockRanges,
+ * compressed_marching_cubes.js:522-578, and notes at :523-524 that the server
+ * could compute them).
+ * BCMC_RANGES_BLOCK: float2 (min, max) per block over its voxels inside the
+ *   volume, x-fastest block order: the reference's blockRangesBuffer layout,
+ *   as bcmc_zfp_block_ranges writes it.
+ * BCMC_RANGES_BLOCK_TILE: that array followed by a second float2 per block,
+ *   the range of every decoded voxel the block's cells read (its own 64,
+ *   padding included, and the ghost faces, edges and corner of its +x/+y/+z
+ *   neighbours). */
+typedef enum { BCMC_RANGES_BLOCK = 0, BCMC_RANGES_BLOCK_TILE = 1 } bcmc_ranges_format;
+/* setCompressedVolume with persisted ranges instead of the full-volume decode.
+ * `stream` and `ranges` may be host or device memory; ranges_bytes must be 8
+ * (BLOCK) or 16 (BLOCK_TILE) times the block count.  With BLOCK the tile
+ * ranges are bounded by the union of the block's and its +x/+y/+z neighbours'
+ * ranges ((-inf, inf) for blocks that read padding voxels): results are the
+ * same, the count pass examines more blocks.  The ranges must be this
+ * stream's: others give another active set, as they would in the reference. */
+BCMC_API int bcmc_mc_set_compressed_volume_ranges(bcmc_mc *mc, const void *stream, uint64_t bytes,
+                                                  double rate, const uint32_t dims[3],
+                                                  const float scale[3], uint32_t initial_cache_slots,
+                                                  const void *ranges, bcmc_ranges_format fmt,
+                                                  uint64_t ranges_bytes);
+/* The loaded volume's ranges in `fmt` (host or device memory of at least
+ * 8 or 16 bytes per block), for persisting beside the stream. */
+BCMC_API int bcmc_mc_get_block_ranges(bcmc_mc *mc, bcmc_ranges_format fmt, void *out, uint64_t bytes);
 BCMC_API int bcmc_mc_set_vertex_format(bcmc_mc *mc, bcmc_vertex_format fmt);
 /* computeSurface(isovalue, perfTracker) (compressed_marching_cubes.js:580-809).
  * The isovalue is used as FP32 (:613-619).  perf may be NULL: the query chain
@@ -273,6 +300,12 @@ BCMC_API int bcmc_engine_create(bcmc_ctx *ctx, uint64_t pool_bytes, bcmc_engine 
 BCMC_API int bcmc_engine_destroy(bcmc_engine *e);
 BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t bytes, double rate,
                                     const uint32_t dims[3], uint32_t *volume_id);
+/* add_volume with persisted block ranges (bcmc_ranges_format, as
+ * bcmc_mc_set_compressed_volume_ranges takes them). */
+BCMC_API int bcmc_engine_add_volume_ranges(bcmc_engine *e, const void *stream, uint64_t bytes, double rate,
+                                           const uint32_t dims[3], const void *ranges,
+                                           bcmc_ranges_format fmt, uint64_t ranges_bytes,
+                                           uint32_t *volume_id);
 BCMC_API int bcmc_engine_set_vertex_format(bcmc_engine *e, bcmc_vertex_format fmt);
 /* One step over nq queries; results[i] for query i (may be NULL).  Fails with
  * BCMC_ERR_NOMEM when the step's blocks do not fit the pool. */

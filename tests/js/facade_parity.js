@@ -61,6 +61,24 @@ async function main() {
     out.perfKeys = Object.keys(perf).sort();
     out.perfLen = perf.numVertices.length;
     out.memory = mc.reportMemoryUse()[3];
+
+    // persisted block ranges: without and with the tile ranges, loaded into
+    // new objects in place of the load-time decode
+    const persisted = {block: mc.blockRanges(false), tile: mc.blockRanges(true)};
+    fs.writeFileSync(path.join(dir, 'ranges.bin'), Buffer.from(persisted.block.buffer));
+    out.persisted = {};
+    for (const name of Object.keys(persisted)) {
+        const m2 = new bcmc.CompressedMarchingCubes(device);
+        await m2.setCompressedVolume(stream, p.rate, p.dims, [1, 1, 1], p.cache, persisted[name]);
+        out.persisted[name] = [];
+        for (let i = 0; i < p.isos.length; ++i) {
+            const nv = await m2.computeSurface(p.isos[i]);
+            const verts = m2.vertexBuffer.read(Float32Array, 4 * nv);
+            fs.writeFileSync(path.join(dir, `pverts_${name}_${i}.bin`), Buffer.from(verts.buffer));
+            out.persisted[name].push({numVertices: nv, newDecompressed: m2.newDecompressed});
+        }
+        m2.destroy();
+    }
     await mc.lruCache.reset();
     out.after_reset_age0 = mc.lruCache.state().age[0];
     mc.destroy();

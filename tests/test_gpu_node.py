@@ -55,6 +55,18 @@ def test_facade_sweep(tmp_path):
         assert np.array_equal(slots, ref.lru_state()[3])
     assert out["perfLen"] == 3 and "computeVertices" in out["perfKeys"]
     assert out["after_reset_age0"] == 100000
+    # blockRanges() persists the oracle's block ranges; objects loaded from
+    # them (with and without tile ranges) give the oracle's results
+    ranges = np.fromfile(tmp_path / "ranges.bin", np.float32).reshape(-1, 2)
+    assert np.array_equal(ranges.view(np.uint32), oracle.block_ranges(s, 64 * rate, dims).view(np.uint32))
+    for name in ("block", "tile"):
+        ref = oracle.MC(s, 64 * rate, dims, initial_cache=cache)
+        for i, iso in enumerate(params["isos"]):
+            c = out["persisted"][name][i]
+            assert c["numVertices"] == ref.compute_surface(iso), (name, iso)
+            assert c["newDecompressed"] == ref.stats["num_decoded"], (name, iso)
+            v = np.fromfile(tmp_path / f"pverts_{name}_{i}.bin", np.float32).reshape(-1, 4)
+            assert np.array_equal(v.view(np.uint32), ref.results()["verts_f32"].view(np.uint32)), (name, iso)
 
 
 @pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")

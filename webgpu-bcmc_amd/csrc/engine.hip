@@ -1084,8 +1084,10 @@ BCMC_API int bcmc_engine_destroy(bcmc_engine *e) {
     return BCMC_OK;
 }
 
-BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t bytes, double rate,
-                                    const uint32_t dims[3], uint32_t *volume_id) {
+// add_volume with the ranges from the range pass (ranges == nullptr) or
+// persisted ones.
+static int eng_add_volume(bcmc_engine *e, const void *stream, uint64_t bytes, double rate, const uint32_t dims[3],
+                          const void *ranges, bcmc_ranges_format fmt, uint64_t ranges_bytes, uint32_t *volume_id) {
     BCMC_CHECK_ARG(e && stream && dims && volume_id, "null argument");
     BCMC_CHECK_ARG(e->vols.size() < kMaxVols, "too many volumes (max 64)");
     const double mb = 64.0 * rate;
@@ -1102,6 +1104,11 @@ BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t
     v.n = (uint32_t)n;
     v.maxbits = (uint32_t)mb;
     BCMC_CHECK_ARG(bytes >= (n * v.maxbits + 7) / 8, "compressed stream shorter than the volume");
+    if (ranges) {
+        BCMC_CHECK_ARG(fmt == BCMC_RANGES_BLOCK || fmt == BCMC_RANGES_BLOCK_TILE, "bad ranges format");
+        BCMC_CHECK_ARG(ranges_bytes == n * (fmt == BCMC_RANGES_BLOCK ? 8u : 16u),
+                       "ranges must hold 8 (block) or 16 (block and tile) bytes per block");
+    }
     bcmc_ctx *ctx = e->ctx;
     float2 *parts = nullptr;  // 8 part ranges per block, transient
     // frees the volume's arrays on an error return; disarmed once it is registered
@@ -1123,15 +1130,24 @@ BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t
     v.nwords = (bytes + 7) / 8 + 2;
     BCMC_TRY(dev_alloc((void **)&v.stream, v.nwords * 8));
     BCMC_HIP_CHECK(hipMemsetAsync(v.stream, 0, v.nwords * 8, ctx->stream));
-    BCMC_HIP_CHECK(hipMemcpyAsync(v.stream, stream, bytes, hipMemcpyHostToDevice, ctx->stream));
+    BCMC_HIP_CHECK(hipMemcpyAsync(v.stream, stream, bytes, hipMemcpyDefault, ctx->stream));
     BCMC_TRY(dev_alloc((void **)&v.ranges, n * 8));
     BCMC_TRY(dev_alloc((void **)&v.slot_map, n * 4));
     BCMC_TRY(dev_alloc((void **)&v.miss_list, n * 4));
     BCMC_HIP_CHECK(hipMemsetAsync(v.slot_map, 0xFF, n * 4, ctx->stream));
-    BCMC_TRY(dev_alloc((void **)&parts, n * 64));
     BCMC_TRY(dev_alloc((void **)&v.tile, n * 8));
-    launch_block_ranges(ctx, v.stream, v.nwords, v.maxbits, v.dims, v.ranges, parts);
-    launch_tile_ranges(ctx, parts, v.nb, v.tile);
+    if (ranges) {
+        BCMC_HIP_CHECK(hipMemcpyAsync(v.ranges, ranges, n * 8, hipMemcpyDefault, ctx->stream));
+        if (fmt == BCMC_RANGES_BLOCK_TILE)
+            BCMC_HIP_CHECK(hipMemcpyAsync(v.tile, (const uint8_t *)ranges + n * 8, n * 8, hipMemcpyDefault,
+                                          ctx->stream));
+        else
+            launch_tile_bound_ranges(ctx, v.ranges, v.dims, v.tile);
+    } else {
+        BCMC_TRY(dev_alloc((void **)&parts, n * 64));
+        launch_block_ranges(ctx, v.stream, v.nwords, v.maxbits, v.dims, v.ranges, parts);
+        launch_tile_ranges(ctx, parts, v.nb, v.tile);
+    }
     const uint32_t nsegx = div_up(v.nb[0], kSegBlocks), nsegs = nsegx * v.nb[1] * v.nb[2];
     BCMC_TRY(dev_alloc((void **)&v.dilated, n * 8));
     BCMC_TRY(dev_alloc((void **)&v.seg_range, (size_t)nsegs * 8));
@@ -1184,6 +1200,18 @@ BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t
     e->miss_cap = (uint32_t)std::min<uint64_t>(all, 0xFFFFFFFFu);
     *volume_id = id;
     return BCMC_OK;
+}
+
+BCMC_API int bcmc_engine_add_volume(bcmc_engine *e, const void *stream, uint64_t bytes, double rate,
+                                    const uint32_t dims[3], uint32_t *volume_id) {
+    return eng_add_volume(e, stream, bytes, rate, dims, nullptr, BCMC_RANGES_BLOCK, 0, volume_id);
+}
+
+BCMC_API int bcmc_engine_add_volume_ranges(bcmc_engine *e, const void *stream, uint64_t bytes, double rate,
+                                           const uint32_t dims[3], const void *ranges, bcmc_ranges_format fmt,
+                                           uint64_t ranges_bytes, uint32_t *volume_id) {
+    BCMC_CHECK_ARG(ranges, "null ranges");
+    return eng_add_volume(e, stream, bytes, rate, dims, ranges, fmt, ranges_bytes, volume_id);
 }
 
 BCMC_API int bcmc_engine_set_vertex_format(bcmc_engine *e, bcmc_vertex_format fmt) {

@@ -209,14 +209,21 @@ void launch_decode_assign(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords
                           uint32_t *slot_age, uint32_t *slot_avail, int32_t *slot_item,
                           int32_t *item_slot, uint32_t total, float *cache, uint32_t capacity,
                           QState *q);
-// Per block the range [min lo, max hi] over its in-bounds 3x3x3 neighbourhood
-// (the active test's interval, see block_active).
 // Per block its tile range: [min, max] of every decoded voxel its cells can
 // read (its own 64, padding included, and the ghost faces, edges and corner of
 // its in-grid +x/+y/+z neighbours), from the range pass's 8 part ranges per
 // block.  A block has vertices only if lo <= iso < hi: otherwise all its cells
 // have case 0 or 255.
 void launch_tile_ranges(bcmc_ctx *ctx, const float2 *part_ranges, const uint32_t nb[3], float2 *out);
+// A wider tile range from block ranges alone (no part ranges): the union of the
+// ranges of the block and its in-grid +x/+y/+z, edge and corner neighbours, or
+// (-inf, inf) for a block whose cells read padding voxels (last on an axis
+// whose size is not a multiple of 4).  Every other voxel a cell reads lies
+// inside the volume in one of those blocks, so the bound skips only blocks
+// without vertices, as the tile range does.
+void launch_tile_bound_ranges(bcmc_ctx *ctx, const float2 *ranges, const uint32_t dims[3], float2 *out);
+// Per block the range [min lo, max hi] over its in-bounds 3x3x3 neighbourhood
+// (the active test's interval, see block_active).
 void launch_dilate_ranges(bcmc_ctx *ctx, const float2 *ranges, const uint32_t nb[3], float2 *out);
 void launch_decode_dense(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords, uint32_t maxbits,
                          const uint32_t *ids, uint32_t n, float *out);

@@ -498,9 +498,11 @@ BCMC_API int bcmc_mc_destroy(bcmc_mc *m) {
     return BCMC_OK;
 }
 
-BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *m, const void *stream, uint64_t bytes,
-                                           double rate, const uint32_t dims[3],
-                                           const float scale[3], uint32_t initial_cache_slots) {
+// setCompressedVolume with the block ranges computed by the full-volume range
+// pass (ranges == nullptr) or taken from the caller (persisted ranges).
+static int mc_set_volume(bcmc_mc *m, const void *stream, uint64_t bytes, double rate, const uint32_t dims[3],
+                         const float scale[3], uint32_t initial_cache_slots, const void *ranges,
+                         bcmc_ranges_format fmt, uint64_t ranges_bytes) {
     BCMC_CHECK_ARG(m && stream && dims, "null argument");
     BCMC_CHECK_ARG(dims[0] && dims[1] && dims[2], "volume dims must be positive");
     const double mb = 64.0 * rate;
@@ -514,6 +516,11 @@ BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *m, const void *stream, uint6
     const uint32_t maxbits = (uint32_t)mb;
     const uint64_t need = (total * maxbits + 7) / 8;
     BCMC_CHECK_ARG(bytes >= need, "compressed stream shorter than the volume at this rate");
+    if (ranges) {
+        BCMC_CHECK_ARG(fmt == BCMC_RANGES_BLOCK || fmt == BCMC_RANGES_BLOCK_TILE, "bad ranges format");
+        BCMC_CHECK_ARG(ranges_bytes == total * (fmt == BCMC_RANGES_BLOCK ? 8u : 16u),
+                       "ranges must hold 8 (block) or 16 (block and tile) bytes per block");
+    }
     mc_free_volume(m);
     bcmc_ctx *ctx = m->ctx;
     for (int i = 0; i < 3; ++i) {
@@ -529,7 +536,7 @@ BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *m, const void *stream, uint6
     m->stream_words = (bytes + 7) / 8 + 2;
     BCMC_TRY(dev_alloc((void **)&m->stream, m->stream_words * 8));
     BCMC_HIP_CHECK(hipMemsetAsync(m->stream, 0, m->stream_words * 8, ctx->stream));
-    BCMC_HIP_CHECK(hipMemcpyAsync(m->stream, stream, bytes, hipMemcpyHostToDevice, ctx->stream));
+    BCMC_HIP_CHECK(hipMemcpyAsync(m->stream, stream, bytes, hipMemcpyDefault, ctx->stream));
     const size_t n = m->total;
     BCMC_TRY(dev_alloc((void **)&m->ranges, n * 8));
     BCMC_TRY(dev_alloc((void **)&m->flags, n * 4));
@@ -555,19 +562,65 @@ BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *m, const void *stream, uint6
     qi.cache_size_next = qi.cache_size;
     *m->qh = qi;
     BCMC_HIP_CHECK(hipMemcpyAsync(m->q, m->qh, sizeof(QState), hipMemcpyHostToDevice, ctx->stream));
-    // computeBlockRanges (:522-578): full-volume decode, min/max per block
-    float2 *parts = nullptr;  // 8 part ranges per block, transient
-    BCMC_TRY(dev_alloc((void **)&parts, n * 64));
-    launch_block_ranges(ctx, m->stream, m->stream_words, maxbits, m->dims, m->ranges, parts);
-    launch_dilate_ranges(ctx, m->ranges, m->nb, m->dilated);
-    launch_tile_ranges(ctx, parts, m->nb, m->tile);
-    BCMC_HIP_CHECK(hipGetLastError());
-    BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    dev_free(parts);
+    if (ranges) {
+        // persisted ranges (the server-side computation :523-524 suggests)
+        BCMC_HIP_CHECK(hipMemcpyAsync(m->ranges, ranges, n * 8, hipMemcpyDefault, ctx->stream));
+        if (fmt == BCMC_RANGES_BLOCK_TILE)
+            BCMC_HIP_CHECK(hipMemcpyAsync(m->tile, (const uint8_t *)ranges + n * 8, n * 8, hipMemcpyDefault,
+                                          ctx->stream));
+        else
+            launch_tile_bound_ranges(ctx, m->ranges, m->dims, m->tile);
+        launch_dilate_ranges(ctx, m->ranges, m->nb, m->dilated);
+        BCMC_HIP_CHECK(hipGetLastError());
+        BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    } else {
+        // computeBlockRanges (:522-578): full-volume decode, min/max per block
+        float2 *parts = nullptr;  // 8 part ranges per block, transient
+        BCMC_TRY(dev_alloc((void **)&parts, n * 64));
+        launch_block_ranges(ctx, m->stream, m->stream_words, maxbits, m->dims, m->ranges, parts);
+        launch_dilate_ranges(ctx, m->ranges, m->nb, m->dilated);
+        launch_tile_ranges(ctx, parts, m->nb, m->tile);
+        const hipError_t err = hipGetLastError();
+        const hipError_t serr = hipStreamSynchronize(ctx->stream);
+        dev_free(parts);
+        BCMC_HIP_CHECK(err);
+        BCMC_HIP_CHECK(serr);
+    }
     m->has_volume = true;
     m->last_active = m->last_with = m->last_new = 0;
     m->last_verts = 0;
     mc_refresh_views(m);
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_mc_set_compressed_volume(bcmc_mc *m, const void *stream, uint64_t bytes, double rate,
+                                           const uint32_t dims[3], const float scale[3],
+                                           uint32_t initial_cache_slots) {
+    return mc_set_volume(m, stream, bytes, rate, dims, scale, initial_cache_slots, nullptr, BCMC_RANGES_BLOCK, 0);
+}
+
+BCMC_API int bcmc_mc_set_compressed_volume_ranges(bcmc_mc *m, const void *stream, uint64_t bytes, double rate,
+                                                  const uint32_t dims[3], const float scale[3],
+                                                  uint32_t initial_cache_slots, const void *ranges,
+                                                  bcmc_ranges_format fmt, uint64_t ranges_bytes) {
+    BCMC_CHECK_ARG(ranges, "null ranges");
+    return mc_set_volume(m, stream, bytes, rate, dims, scale, initial_cache_slots, ranges, fmt, ranges_bytes);
+}
+
+BCMC_API int bcmc_mc_get_block_ranges(bcmc_mc *m, bcmc_ranges_format fmt, void *out, uint64_t bytes) {
+    BCMC_CHECK_ARG(m && out, "null argument");
+    if (!m->has_volume) {
+        set_error("getBlockRanges called before setCompressedVolume");
+        return BCMC_ERR_STATE;
+    }
+    BCMC_CHECK_ARG(fmt == BCMC_RANGES_BLOCK || fmt == BCMC_RANGES_BLOCK_TILE, "bad ranges format");
+    const uint64_t n = m->total;
+    BCMC_CHECK_ARG(bytes >= n * (fmt == BCMC_RANGES_BLOCK ? 8u : 16u), "output shorter than the ranges");
+    hipStream_t st = m->ctx->stream;
+    BCMC_HIP_CHECK(hipMemcpyAsync(out, m->ranges, n * 8, hipMemcpyDefault, st));
+    if (fmt == BCMC_RANGES_BLOCK_TILE)
+        BCMC_HIP_CHECK(hipMemcpyAsync((uint8_t *)out + n * 8, m->tile, n * 8, hipMemcpyDefault, st));
+    BCMC_HIP_CHECK(hipStreamSynchronize(st));
     return BCMC_OK;
 }
 

@@ -593,21 +593,32 @@ void launch_block_ranges(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords,
         launch_mode<kModeRanges>(a, grid, kZfpThreads, ctx->stream);
 }
 
-// Per block: [min lo, max hi] over its in-bounds 3x3x3 neighbourhood.
-__global__ void dilate_ranges_kernel(const float2 *ranges, uint32_t nbx, uint32_t nby, uint32_t nbz,
-                                     float2 *out) {
+// Per block: [min lo, max hi] over the in-grid blocks at offsets LO..1 on
+// each axis: LO = -1 is the 3x3x3 neighbourhood (the active test's interval),
+// LO = 0 the block and its +x/+y/+z, edge and corner neighbours (a bound on the
+// tile range from block ranges alone).  Block ranges cover voxels inside the
+// volume only, while the cells of a block last on an axis whose size is not a
+// multiple of 4 also read its padding voxels: bit k of `padded` set marks axis
+// k so, and such blocks get (-inf, inf).
+template <int LO>
+__global__ void union_ranges_kernel(const float2 *ranges, uint32_t nbx, uint32_t nby, uint32_t nbz,
+                                    uint32_t padded, float2 *out) {
     const uint64_t n = (uint64_t)nbx * nby * nbz;
     for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n;
          b += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t x = (uint32_t)(b % nbx), y = (uint32_t)((b / nbx) % nby), z = (uint32_t)(b / ((uint64_t)nbx * nby));
+        if (((padded & 1u) && x + 1 == nbx) || ((padded & 2u) && y + 1 == nby) || ((padded & 4u) && z + 1 == nbz)) {
+            out[b] = make_float2(-INFINITY, INFINITY);
+            continue;
+        }
         float lo = ranges[b].x, hi = ranges[b].y;
-        for (int dz = -1; dz <= 1; ++dz)
-            for (int dy = -1; dy <= 1; ++dy) {
+        for (int dz = LO; dz <= 1; ++dz)
+            for (int dy = LO; dy <= 1; ++dy) {
                 const int yy = (int)y + dy, zz = (int)z + dz;
                 if (yy < 0 || zz < 0 || yy >= (int)nby || zz >= (int)nbz)
                     continue;
                 const float2 *row = ranges + (uint64_t)nbx * ((uint32_t)yy + (uint64_t)nby * (uint32_t)zz);
-                for (int dx = -1; dx <= 1; ++dx) {
+                for (int dx = LO; dx <= 1; ++dx) {
                     const int xx = (int)x + dx;
                     if (xx < 0 || xx >= (int)nbx)
                         continue;
@@ -620,9 +631,6 @@ __global__ void dilate_ranges_kernel(const float2 *ranges, uint32_t nbx, uint32_
     }
 }
 
-// Per block: range over all decoded voxels (padding included) of the block and
-// of its in-grid +x/+y/+z, edge and corner neighbours: every voxel a cell of
-// the block can read (compute_vertex_values.comp:96-252).
 // Per block: [min, max] over its whole 5^3 tile, i.e. every decoded voxel its
 // cells can read: its own 64 (padding included) and the ghost voxels of its
 // in-grid +x / +y / +z / +xy / +xz / +yz / +xyz neighbours, which are their
@@ -659,7 +667,15 @@ void launch_tile_ranges(bcmc_ctx *ctx, const float2 *part_ranges, const uint32_t
 void launch_dilate_ranges(bcmc_ctx *ctx, const float2 *ranges, const uint32_t nb[3], float2 *out) {
     const uint64_t n = (uint64_t)nb[0] * nb[1] * nb[2];
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(n, 256), persistent_grid(ctx, 8)));
-    dilate_ranges_kernel<<<grid, 256, 0, ctx->stream>>>(ranges, nb[0], nb[1], nb[2], out);
+    union_ranges_kernel<-1><<<grid, 256, 0, ctx->stream>>>(ranges, nb[0], nb[1], nb[2], 0u, out);
+}
+
+void launch_tile_bound_ranges(bcmc_ctx *ctx, const float2 *ranges, const uint32_t dims[3], float2 *out) {
+    const uint32_t nb[3] = {(dims[0] + 3) / 4, (dims[1] + 3) / 4, (dims[2] + 3) / 4};
+    const uint32_t padded = (dims[0] % 4 ? 1u : 0u) | (dims[1] % 4 ? 2u : 0u) | (dims[2] % 4 ? 4u : 0u);
+    const uint64_t n = (uint64_t)nb[0] * nb[1] * nb[2];
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(n, 256), persistent_grid(ctx, 8)));
+    union_ranges_kernel<0><<<grid, 256, 0, ctx->stream>>>(ranges, nb[0], nb[1], nb[2], padded, out);
 }
 
 void launch_decode_to_cache(bcmc_ctx *ctx, const uint64_t *stream, uint64_t nwords,

@@ -455,21 +455,69 @@ static int three(napi_env env, napi_value arr, double out[3])
 }
 
 /* mcSetVolume(mc, bytes, rate, dims[3], scale[3], initialCacheSlots) */
+/* Optional persisted block ranges (a typed array of 8 or 16 bytes per block):
+ * *data = NULL when v is undefined or null.  Returns -1 for anything else
+ * that is not a typed array. */
+static int ranges_of(napi_env env, napi_value v, const uint32_t dims[3], void **data, size_t *len,
+                     bcmc_ranges_format *fmt)
+{
+    napi_valuetype t;
+    *data = NULL;
+    *len = 0;
+    *fmt = BCMC_RANGES_BLOCK;
+    if (napi_typeof(env, v, &t) != napi_ok)
+        return -1;
+    if (t == napi_undefined || t == napi_null)
+        return 0;
+    if (bytes_of(env, v, data, len))
+        return -1;
+    const uint64_t nb = (uint64_t)((dims[0] + 3) / 4) * ((dims[1] + 3) / 4) * ((dims[2] + 3) / 4);
+    *fmt = *len == 16 * nb ? BCMC_RANGES_BLOCK_TILE : BCMC_RANGES_BLOCK;
+    return 0;
+}
+
+/* mcSetVolume(mc, bytes, rate, dims[3], scale[3], initialSlots, ranges?) */
 static napi_value mc_set_volume(napi_env env, napi_callback_info info)
 {
-    GET_ARGS(6);
+    GET_ARGS(7);
     EXT(0, bcmc_mc, mc);
-    void *d;
-    size_t n;
+    void *d, *r;
+    size_t n, rn;
+    bcmc_ranges_format fmt;
     double dd[3], sc[3];
     if (bytes_of(env, argv[1], &d, &n) || three(env, argv[3], dd) || three(env, argv[4], sc)) {
-        napi_throw_type_error(env, NULL, "expected (mc, Uint8Array, rate, dims[3], scale[3], n)");
+        napi_throw_type_error(env, NULL, "expected (mc, Uint8Array, rate, dims[3], scale[3], n, ranges?)");
         return NULL;
     }
     const uint32_t dims[3] = {(uint32_t)dd[0], (uint32_t)dd[1], (uint32_t)dd[2]};
     const float scale[3] = {(float)sc[0], (float)sc[1], (float)sc[2]};
-    BCMC_CALL(env, bcmc_mc_set_compressed_volume(mc, d, n, num(env, argv[2]), dims, scale,
-                                                 (uint32_t)num(env, argv[5])));
+    if (ranges_of(env, argv[6], dims, &r, &rn, &fmt)) {
+        napi_throw_type_error(env, NULL, "block ranges must be a typed array");
+        return NULL;
+    }
+    if (r)
+        BCMC_CALL(env, bcmc_mc_set_compressed_volume_ranges(mc, d, n, num(env, argv[2]), dims, scale,
+                                                            (uint32_t)num(env, argv[5]), r, fmt, rn));
+    else
+        BCMC_CALL(env, bcmc_mc_set_compressed_volume(mc, d, n, num(env, argv[2]), dims, scale,
+                                                     (uint32_t)num(env, argv[5])));
+    return NULL;
+}
+
+/* mcGetRanges(mc, withTile, out Float32Array) */
+static napi_value mc_get_ranges(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(3);
+    EXT(0, bcmc_mc, mc);
+    bool tile = false;
+    void *d;
+    size_t n;
+    NAPI_CALL(env, napi_get_value_bool(env, argv[1], &tile));
+    if (bytes_of(env, argv[2], &d, &n)) {
+        napi_throw_type_error(env, NULL, "expected (mc, withTile, Float32Array)");
+        return NULL;
+    }
+    BCMC_CALL(env, bcmc_mc_get_block_ranges(mc, tile ? BCMC_RANGES_BLOCK_TILE : BCMC_RANGES_BLOCK, d, n));
     return NULL;
 }
 
@@ -604,21 +652,29 @@ static napi_value eng_destroy(napi_env env, napi_callback_info info)
     return NULL;
 }
 
-/* engAddVolume(eng, bytes, rate, dims[3]) -> volume id */
+/* engAddVolume(eng, bytes, rate, dims[3], ranges?) -> volume id */
 static napi_value eng_add_volume(napi_env env, napi_callback_info info)
 {
-    GET_ARGS(4);
+    GET_ARGS(5);
     EXT(0, bcmc_engine, eng);
-    void *d;
-    size_t n;
+    void *d, *r;
+    size_t n, rn;
+    bcmc_ranges_format fmt;
     double dd[3];
     if (bytes_of(env, argv[1], &d, &n) || three(env, argv[3], dd)) {
-        napi_throw_type_error(env, NULL, "expected (engine, Uint8Array, rate, dims[3])");
+        napi_throw_type_error(env, NULL, "expected (engine, Uint8Array, rate, dims[3], ranges?)");
         return NULL;
     }
     const uint32_t dims[3] = {(uint32_t)dd[0], (uint32_t)dd[1], (uint32_t)dd[2]};
+    if (ranges_of(env, argv[4], dims, &r, &rn, &fmt)) {
+        napi_throw_type_error(env, NULL, "block ranges must be a typed array");
+        return NULL;
+    }
     uint32_t id = 0;
-    BCMC_CALL(env, bcmc_engine_add_volume(eng, d, n, num(env, argv[2]), dims, &id));
+    if (r)
+        BCMC_CALL(env, bcmc_engine_add_volume_ranges(eng, d, n, num(env, argv[2]), dims, r, fmt, rn, &id));
+    else
+        BCMC_CALL(env, bcmc_engine_add_volume(eng, d, n, num(env, argv[2]), dims, &id));
     return mk_num(env, id);
 }
 
@@ -1135,7 +1191,8 @@ static napi_value init(napi_env env, napi_value exports)
         FN("lruCreate", lru_create), FN("lruDestroy", lru_destroy), FN("lruUpdate", lru_update),
         FN("lruReset", lru_reset), FN("lruInfo", lru_info), FN("lruState", lru_state),
         FN("lruCacheBuffer", lru_cache_buffer),
-        FN("mcCreate", mc_create), FN("mcDestroy", mc_destroy), FN("mcSetVolume", mc_set_volume),
+        FN("mcCreate", mc_create), FN("mcDestroy", mc_destroy), FN("mcGetRanges", mc_get_ranges),
+        FN("mcSetVolume", mc_set_volume),
         FN("mcSetFormat", mc_set_format), FN("mcComputeSurface", mc_compute_surface),
         FN("mcInfo", mc_info), FN("mcVertexBuffer", mc_vertex_buffer), FN("mcLru", mc_lru),
         FN("mcArray", mc_array), FN("mcMemoryUse", mc_memory_use),

@@ -240,17 +240,27 @@ class CompressedMarchingCubes {
         this.newDecompressed = 0;
         this.totalBlocks = 0;
     }
+    // blockRanges (optional): ranges persisted from blockRanges() -- a
+    // Float32Array of 2 (min, max) or 4 (and the tile range) values per block
+    // -- in place of the full-volume decode at load.
     async setCompressedVolume(volume, compressionRate, volumeDims, volumeScale,
-                              initialCacheSlots = 0) {
+                              initialCacheSlots = 0, blockRanges = undefined) {
         this.volumeDims = volumeDims;
         this.paddedDims = [alignTo(volumeDims[0], 4), alignTo(volumeDims[1], 4),
                            alignTo(volumeDims[2], 4)];
         this.totalBlocks = (this.paddedDims[0] * this.paddedDims[1] * this.paddedDims[2]) / 64;
         native.mcSetVolume(this.handle, volume, compressionRate, volumeDims,
-                           volumeScale || [1, 1, 1], initialCacheSlots);
+                           volumeScale || [1, 1, 1], initialCacheSlots, blockRanges);
         this.compressedDataSize = volume.byteLength;
         this.lruCache = new LRUCache(this.device, this.scanPipeline, this.streamCompact, 0, 0, 0,
                                      native.mcLru(this.handle));
+    }
+    // the loaded volume's block ranges for persisting beside the stream:
+    // (min, max) per block, followed by the tile ranges when withTile
+    blockRanges(withTile = true) {
+        const out = new Float32Array(this.totalBlocks * (withTile ? 4 : 2));
+        native.mcGetRanges(this.handle, withTile, out);
+        return out;
     }
     setVertexFormat(fmt) {
         native.mcSetFormat(this.handle, fmt);
@@ -306,8 +316,8 @@ class Engine {
         this.device = device;
         this.handle = native.engCreate(device.handle, poolBytes);
     }
-    addVolume(volume, compressionRate, volumeDims) {
-        return native.engAddVolume(this.handle, volume, compressionRate, volumeDims);
+    addVolume(volume, compressionRate, volumeDims, blockRanges = undefined) {
+        return native.engAddVolume(this.handle, volume, compressionRate, volumeDims, blockRanges);
     }
     setVertexFormat(fmt) {
         native.engSetFormat(this.handle, fmt);

@@ -30,6 +30,7 @@ HEADER_PATH = os.path.join(os.path.dirname(_ROOT), "include", "bcmc.h")
 VERTS_LOCAL_F32, VERTS_GLOBAL_F32, VERTS_PACKED_U32X2 = 0, 1, 2
 ARRAY_RANGES, ARRAY_ACTIVE_FLAGS, ARRAY_ACTIVE_IDS, ARRAY_NEW_ITEMS, ARRAY_WITH_VERTS, \
     ARRAY_VERT_OFFSETS = range(6)
+RANGES_BLOCK, RANGES_BLOCK_TILE = 0, 1
 STATUS = {0: "OK", -1: "INVALID", -2: "HIP", -3: "NOMEM", -4: "OVERFLOW", -5: "STATE",
           -6: "NO_DEVICE"}
 
@@ -163,6 +164,9 @@ def lib():
         "bcmc_mc_create": (i32, [vp, P(vp)]),
         "bcmc_mc_destroy": (i32, [vp]),
         "bcmc_mc_set_compressed_volume": (i32, [vp, vp, u64, ctypes.c_double, P(u32), P(ctypes.c_float), u32]),
+        "bcmc_mc_set_compressed_volume_ranges": (i32, [vp, vp, u64, ctypes.c_double, P(u32), P(ctypes.c_float),
+                                                       u32, vp, i32, u64]),
+        "bcmc_mc_get_block_ranges": (i32, [vp, i32, vp, u64]),
         "bcmc_mc_set_vertex_format": (i32, [vp, i32]),
         "bcmc_mc_compute_surface": (i32, [vp, ctypes.c_float, P(_Perf), P(u64)]),
         "bcmc_mc_get_info": (i32, [vp, P(_McInfo)]),
@@ -173,6 +177,7 @@ def lib():
         "bcmc_engine_create": (i32, [vp, u64, P(vp)]),
         "bcmc_engine_destroy": (i32, [vp]),
         "bcmc_engine_add_volume": (i32, [vp, vp, u64, ctypes.c_double, P(u32), P(u32)]),
+        "bcmc_engine_add_volume_ranges": (i32, [vp, vp, u64, ctypes.c_double, P(u32), vp, i32, u64, P(u32)]),
         "bcmc_engine_set_vertex_format": (i32, [vp, i32]),
         "bcmc_engine_run": (i32, [vp, vp, vp, u32, vp]),
         "bcmc_engine_vertex_buffer": (vp, [vp]),
@@ -194,6 +199,26 @@ def lib():
 def _check(rc):
     if rc != 0:
         raise BcmcError(rc, lib().bcmc_last_error().decode(errors="replace"))
+
+
+def _source(data):
+    """(pointer, bytes, keep-alive) of host bytes or a device Buffer."""
+    if isinstance(data, Buffer):
+        return lib().bcmc_buf_device_ptr(data.h), data.nbytes, data
+    a = np.ascontiguousarray(np.frombuffer(memoryview(data), np.uint8))
+    return a.ctypes.data, a.nbytes, a
+
+
+def _ranges_source(ranges, dims):
+    """(pointer, format, bytes, keep-alive) of persisted block ranges: 8 bytes
+    a block (RANGES_BLOCK) or 16 (RANGES_BLOCK_TILE), host array or Buffer."""
+    nb = int(np.prod([(int(d) + 3) // 4 for d in dims]))
+    if not isinstance(ranges, Buffer):
+        ranges = np.ascontiguousarray(ranges, np.float32)
+    ptr, nbytes, keep = _source(ranges)
+    if nbytes not in (8 * nb, 16 * nb):
+        raise ValueError(f"block ranges of {nbytes} bytes for {nb} blocks: expected {8 * nb} or {16 * nb}")
+    return ptr, RANGES_BLOCK if nbytes == 8 * nb else RANGES_BLOCK_TILE, nbytes, keep
 
 
 def _u32arr(vals):
@@ -478,12 +503,31 @@ class CompressedMarchingCubes:
             self.h = None
 
     def setCompressedVolume(self, volume, compression_rate, volume_dims, volume_scale=(1, 1, 1),
-                            initial_cache_slots=0):
-        v = np.ascontiguousarray(np.frombuffer(memoryview(volume), np.uint8))
+                            initial_cache_slots=0, block_ranges=None):
+        """volume: host bytes or a device Buffer.  block_ranges (optional):
+        persisted ranges from blockRanges() -- float32, 2 (min, max) or 4
+        (and the tile range) values per block, host array or Buffer -- in
+        place of the full-volume decode."""
+        ptr, nbytes, keep = _source(volume)
         scale = (ctypes.c_float * 3)(*[float(s) for s in volume_scale])
-        _check(lib().bcmc_mc_set_compressed_volume(self.h, v.ctypes.data, v.nbytes,
-                                                   float(compression_rate), _u32arr(volume_dims),
-                                                   scale, int(initial_cache_slots)))
+        if block_ranges is None:
+            _check(lib().bcmc_mc_set_compressed_volume(self.h, ptr, nbytes, float(compression_rate),
+                                                       _u32arr(volume_dims), scale, int(initial_cache_slots)))
+        else:
+            rp, fmt, rbytes, rkeep = _ranges_source(block_ranges, volume_dims)
+            _check(lib().bcmc_mc_set_compressed_volume_ranges(self.h, ptr, nbytes, float(compression_rate),
+                                                              _u32arr(volume_dims), scale,
+                                                              int(initial_cache_slots), rp, fmt, rbytes))
+        del keep
+
+    def blockRanges(self, with_tile=True):
+        """The loaded volume's block ranges for persisting: float32 (N, 2)
+        (min, max) per block, followed by the N tile ranges when with_tile."""
+        n = self.info()["total_blocks"]
+        out = np.empty((2 * n if with_tile else n, 2), np.float32)
+        _check(lib().bcmc_mc_get_block_ranges(self.h, RANGES_BLOCK_TILE if with_tile else RANGES_BLOCK,
+                                              out.ctypes.data, out.nbytes))
+        return out
 
     def setVertexFormat(self, fmt):
         _check(lib().bcmc_mc_set_vertex_format(self.h, fmt))
@@ -648,11 +692,19 @@ class Engine:
             lib().bcmc_engine_destroy(self.h)
             self.h = None
 
-    def add_volume(self, stream, rate, dims):
-        v = np.ascontiguousarray(np.frombuffer(memoryview(stream), np.uint8))
+    def add_volume(self, stream, rate, dims, block_ranges=None):
+        """stream: host bytes or a device Buffer; block_ranges as
+        CompressedMarchingCubes.setCompressedVolume takes them."""
+        ptr, nbytes, keep = _source(stream)
         vid = ctypes.c_uint32()
-        _check(lib().bcmc_engine_add_volume(self.h, v.ctypes.data, v.nbytes, float(rate),
-                                            _u32arr(dims), ctypes.byref(vid)))
+        if block_ranges is None:
+            _check(lib().bcmc_engine_add_volume(self.h, ptr, nbytes, float(rate), _u32arr(dims),
+                                                ctypes.byref(vid)))
+        else:
+            rp, fmt, rbytes, rkeep = _ranges_source(block_ranges, dims)
+            _check(lib().bcmc_engine_add_volume_ranges(self.h, ptr, nbytes, float(rate), _u32arr(dims), rp,
+                                                       fmt, rbytes, ctypes.byref(vid)))
+        del keep
         return vid.value
 
     def set_vertex_format(self, fmt):
