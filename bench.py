@@ -151,8 +151,23 @@ def served_bench(bcmc_amd, synth, dev, args):
     served_qps = len(ids) / makespan
     exact = bool(np.array_equal(sat["num_vertices"].astype(np.uint64), nv_plain))
     lat = sat["finish_s"] - sat["arrival_s"]
-    log(f"bench: served saturation {served_qps:.1f} q/s in {int(sat['step'].max()) + 1} steps; "
-        f"counts equal to plain: {exact}")
+    # requests answered by an equal request's query (same step, volume and
+    # isovalue)
+    seen, shared = set(), 0
+    for st_, v_, x_ in zip(sat["step"], ids, isos):
+        key = (int(st_), int(v_), float(x_))
+        shared += key in seen
+        seen.add(key)
+    log(f"bench: served saturation {served_qps:.1f} q/s in {int(sat['step'].max()) + 1} steps "
+        f"({shared} requests coalesced); counts equal to plain: {exact}")
+    # the same trace with every request a query of its own
+    sep = eng.serve(np.zeros(len(ids)), ids, isos, max_batch=args.served_batch,
+                    vertex_budget_bytes=budget, coalesce=False)
+    sep_qps = len(ids) / float(sep["finish_s"].max())
+    sep_exact = bool(np.array_equal(sep["num_vertices"].astype(np.uint64), nv_plain))
+    sep_lat = sep["finish_s"] - sep["arrival_s"]
+    log(f"bench: served saturation without coalescing {sep_qps:.1f} q/s; counts equal to plain: "
+        f"{sep_exact}")
     out = {"workload": "8 resident blob volumes 128^3..1024x1024x1080 (seeds 7-14, zfp rates "
                        "2/4/8/2/4/8/2/2), isovalues U[0.1,0.9] with 50% repeats of the last 16 of "
                        "the same volume",
@@ -163,6 +178,13 @@ def served_bench(bcmc_amd, synth, dev, args):
            "saturation_latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                                      "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
            "counts_equal_plain": exact, "steps": int(sat["step"].max()) + 1,
+           "coalesced_requests": int(shared),
+           "no_coalesce": {"served_qps": round(sep_qps, 2),
+                           "served_vs_plain": round(sep_qps / plain_qps, 3),
+                           "saturation_latency_ms": {
+                               "p50": round(float(np.percentile(sep_lat, 50)) * 1e3, 3),
+                               "p99": round(float(np.percentile(sep_lat, 99)) * 1e3, 3)},
+                           "counts_equal_plain": sep_exact, "steps": int(sep["step"].max()) + 1},
            "vertices_total": int(nv_plain.sum()),
            "mean_batch": round(float(sat["batch_size"].mean()), 1),
            "hbm_budget_bytes": hbm_budget, "vertex_budget_bytes": budget,

@@ -325,10 +325,16 @@ BCMC_API int bcmc_engine_stats(bcmc_engine *e, bcmc_engine_info *info);
  * the queue.  Fails only when one request alone exceeds the pool or the
  * budget.  out[i] receives request i's arrival, step start and finish times
  * (seconds from the call) and its vertex count; the vertices of the last step
- * stay in the engine buffer. */
+ * stay in the engine buffer.  Equal requests in one step are answered by one
+ * query unless opts->no_coalesce is set; they never add to the pool bound. */
 typedef struct {
     uint32_t max_batch;           /* 0: 256 */
     uint64_t vertex_budget_bytes; /* HBM for one step's vertices; 0: 64 GiB */
+    /* 0: requests of one step with the same volume and isovalue (bitwise) are
+     * coalesced: the step computes that query once and every such request is
+     * answered by its result (same vertex count and vertex range).
+     * 1: every request is a query of its own. */
+    uint32_t no_coalesce;
 } bcmc_serve_opts;
 typedef struct {
     double arrival_s, start_s, finish_s;

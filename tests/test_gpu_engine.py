@@ -90,6 +90,35 @@ def test_served_trace_results_match_single_queries(dev):
         assert out["num_vertices"][i] == cache[key], i
 
 
+@pytest.mark.parametrize("pool_blocks", [200000, 1200])
+def test_coalesced_serving_matches_separate_queries(dev, pool_blocks):
+    """Requests all waiting at once, most of them exact repeats: served with
+    coalescing (equal requests of a step answered by one query) and with
+    every request a query of its own, every request gets the oracle's count;
+    repeats take no pool room, so coalescing needs no more steps.  A pool of
+    1200 blocks holds one step of a few distinct requests only."""
+    say("coalesced serving, pool", pool_blocks)
+    streams = _streams()
+    eng = bcmc_amd.Engine(dev, pool_blocks * 256)
+    ids = [eng.add_volume(s, r, d) for s, r, d in streams]
+    rng = np.random.default_rng(pool_blocks + 5)
+    n = 96
+    vol = rng.integers(0, len(ids), n).astype(np.uint32)
+    choices = rng.uniform(0.2, 0.8, (len(ids), 5)).astype(np.float32)
+    iso = choices[vol, rng.integers(0, 5, n)]
+    out_c = eng.serve(np.zeros(n), vol, iso, max_batch=32)
+    out_s = eng.serve(np.zeros(n), vol, iso, max_batch=32, coalesce=False)
+    cache = {}
+    for i in range(n):
+        key = (int(vol[i]), float(iso[i]))
+        if key not in cache:
+            s, r, d = streams[vol[i]]
+            cache[key] = _oracle(s, r, d, float(iso[i]))[0]
+        assert out_c["num_vertices"][i] == cache[key], ("coalesced", i)
+        assert out_s["num_vertices"][i] == cache[key], ("separate", i)
+    assert out_c["step"].max() <= out_s["step"].max()
+
+
 def _piled_trace(streams, n, seed):
     """n requests at 4000 req/s (they pile up behind each step) over every
     volume, isovalues where the volumes have surface."""

@@ -43,6 +43,7 @@
 #include <cstring>
 #include <initializer_list>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 namespace bcmc {
@@ -1487,6 +1488,42 @@ static int eng_run(bcmc_engine *e, const uint32_t *volume_ids, const float *isov
     return BCMC_OK;
 }
 
+// A step over the distinct (volume, isovalue bits) queries of a request list:
+// request i is answered by the result of the first request equal to it, so
+// equal requests share one vertex range.  Admission stays a prefix of the
+// requests: the distinct queries are in order of first occurrence, and the
+// requests before the first occurrence of an unadmitted query only use
+// admitted ones.
+static int eng_run_coalesced(bcmc_engine *e, const uint32_t *volume_ids, const float *isovalues,
+                             uint32_t nq, uint64_t budget, uint32_t *n_done,
+                             bcmc_query_result *results) {
+    std::vector<uint32_t> map(nq), first, uid;
+    std::vector<float> uiso;
+    std::unordered_map<uint64_t, uint32_t> seen;
+    seen.reserve(nq * 2u);
+    for (uint32_t i = 0; i < nq; ++i) {
+        uint32_t bits;
+        std::memcpy(&bits, &isovalues[i], 4);
+        const uint64_t key = (uint64_t)volume_ids[i] << 32 | bits;
+        const auto ins = seen.emplace(key, (uint32_t)uid.size());
+        if (ins.second) {
+            first.push_back(i);
+            uid.push_back(volume_ids[i]);
+            uiso.push_back(isovalues[i]);
+        }
+        map[i] = ins.first->second;
+    }
+    const uint32_t nu = (uint32_t)uid.size();
+    std::vector<bcmc_query_result> ures(nu);
+    uint32_t udone = 0;
+    BCMC_TRY(eng_run(e, uid.data(), uiso.data(), nu, budget, &udone, ures.data()));
+    const uint32_t nadm = udone == nu ? nq : first[udone];
+    for (uint32_t i = 0; i < nadm; ++i)
+        results[i] = ures[map[i]];
+    *n_done = nadm;
+    return BCMC_OK;
+}
+
 extern "C" {
 
 BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const float *isovalues,
@@ -1542,6 +1579,7 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
     const uint32_t max_batch = opts && opts->max_batch ? opts->max_batch : 256;
     const uint64_t vert_budget = opts && opts->vertex_budget_bytes ? opts->vertex_budget_bytes
                                                                     : (uint64_t)64 << 30;
+    const bool coalesce = !(opts && opts->no_coalesce);
     // capacity planning before the clock starts: a step takes consecutive
     // requests (FIFO), at most max_batch of them, so the largest window of
     // max_batch consecutive requests bounds every step's block and row counts
@@ -1594,6 +1632,7 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
     std::vector<uint32_t> batch;
     std::vector<bcmc_query_result> res;
     std::vector<uint64_t> vsum(e->vols.size());
+    std::unordered_map<uint64_t, uint32_t> in_step;  // (volume, isovalue bits) of the step's requests
     while (done < n) {
         const double t = now_s();
         while (next < n && arrival_s[next] <= t)
@@ -1616,16 +1655,24 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
         batch.clear();
         // a volume's requests together touch at most all of its blocks
         std::fill(vsum.begin(), vsum.end(), 0);
+        in_step.clear();
         uint64_t blocks = 0, step_blocks = 0;
         for (size_t h = head; h < pending.size() && batch.size() < max_batch; ++h) {
             const uint32_t r = pending[h], v = volume_ids[r];
-            const uint64_t nblk = e->vols[v].n;
-            const uint64_t grown = blocks - std::min(nblk, vsum[v]) + std::min(nblk, vsum[v] + bound[r]);
+            uint32_t bits;
+            std::memcpy(&bits, &isovalues[r], 4);
+            // a request equal to one already in the step adds no query
+            const bool dup = coalesce && in_step.count((uint64_t)v << 32 | bits) != 0;
+            const uint64_t nblk = dup ? 0 : e->vols[v].n, add = dup ? 0 : bound[r];
+            const uint64_t grown = blocks - std::min<uint64_t>(e->vols[v].n, vsum[v]) +
+                                   std::min<uint64_t>(e->vols[v].n, vsum[v] + add);
             if (!batch.empty() && (grown + prev_blocks > pool || step_blocks + nblk > kMaxStepBlocks))
                 break;
             blocks = grown;
-            vsum[v] += bound[r];
+            vsum[v] += add;
             step_blocks += nblk;
+            if (coalesce)
+                in_step.emplace((uint64_t)v << 32 | bits, r);
             batch.push_back(r);
             ids.push_back(volume_ids[r]);
             isos.push_back(isovalues[r]);
@@ -1633,7 +1680,12 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
         res.resize(batch.size());
         const double start = now_s();
         uint32_t ndone = 0;
-        BCMC_TRY(eng_run(e, ids.data(), isos.data(), (uint32_t)batch.size(), vert_budget, &ndone, res.data()));
+        if (coalesce)
+            BCMC_TRY(eng_run_coalesced(e, ids.data(), isos.data(), (uint32_t)batch.size(), vert_budget,
+                                       &ndone, res.data()));
+        else
+            BCMC_TRY(eng_run(e, ids.data(), isos.data(), (uint32_t)batch.size(), vert_budget, &ndone,
+                             res.data()));
         const double end = now_s();
         // every request of the batch stamped its blocks, the deferred ones too
         prev_blocks = blocks;

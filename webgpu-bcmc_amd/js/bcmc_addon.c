@@ -788,7 +788,7 @@ static napi_value eng_serve_result(napi_env env, const bcmc_serve_result *r, uin
     return out;
 }
 
-/* (engine, Float64Array, Uint32Array, Float32Array, maxBatch, budget) */
+/* (engine, Float64Array, Uint32Array, Float32Array, maxBatch, budget[, noCoalesce]) */
 static int eng_serve_args(napi_env env, napi_value *argv, void **arr, void **ids, void **isos, uint32_t *n,
                           bcmc_serve_opts *opts)
 {
@@ -802,15 +802,16 @@ static int eng_serve_args(napi_env env, napi_value *argv, void **arr, void **ids
     *n = (uint32_t)(ni / 4);
     opts->max_batch = (uint32_t)num(env, argv[4]);
     opts->vertex_budget_bytes = (uint64_t)num(env, argv[5]);
+    opts->no_coalesce = num(env, argv[6]) != 0 ? 1u : 0u; /* undefined: 0 */
     return 0;
 }
 
 /* engServe(eng, Float64Array arrivals, Uint32Array ids, Float32Array isos,
- * maxBatch, vertexBudgetBytes) -> [{arrival, start, finish, numVertices,
- * batchSize, step}] (seconds from the call) */
+ * maxBatch, vertexBudgetBytes[, noCoalesce]) -> [{arrival, start, finish,
+ * numVertices, batchSize, step}] (seconds from the call) */
 static napi_value eng_serve(napi_env env, napi_callback_info info)
 {
-    GET_ARGS(6);
+    GET_ARGS(7);
     EXT(0, bcmc_engine, eng);
     void *arr, *ids, *isos;
     uint32_t n;
@@ -1104,10 +1105,10 @@ static napi_value eng_run_async(napi_env env, napi_callback_info info)
 }
 
 /* engServeAsync(eng, Float64Array, Uint32Array, Float32Array, maxBatch,
- * vertexBudgetBytes) -> Promise<[{...}]> */
+ * vertexBudgetBytes[, noCoalesce]) -> Promise<[{...}]> */
 static napi_value eng_serve_async(napi_env env, napi_callback_info info)
 {
-    GET_ARGS(6);
+    GET_ARGS(7);
     EXT(0, bcmc_engine, eng);
     void *arr, *ids, *isos;
     uint32_t n;

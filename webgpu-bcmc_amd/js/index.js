@@ -335,12 +335,14 @@ class Engine {
         return native.engStats(this.handle);
     }
     // arrivals in seconds from the call; resolves to per-request
-    // {arrival, start, finish, numVertices, batchSize, step}
-    async serve(arrivals, volumeIds, isovalues, {maxBatch = 0, vertexBudgetBytes = 0} = {}) {
+    // {arrival, start, finish, numVertices, batchSize, step}.  Requests of one
+    // step with the same volume and isovalue share one query unless
+    // coalesce is false.
+    async serve(arrivals, volumeIds, isovalues, {maxBatch = 0, vertexBudgetBytes = 0, coalesce = true} = {}) {
         const a = Float64Array.from(arrivals), ids = Uint32Array.from(volumeIds);
         const isos = Float32Array.from(isovalues);
         return this.device._submit(
-            () => native.engServeAsync(this.handle, a, ids, isos, maxBatch, vertexBudgetBytes));
+            () => native.engServeAsync(this.handle, a, ids, isos, maxBatch, vertexBudgetBytes, coalesce ? 0 : 1));
     }
     destroy() {
         if (this.handle) native.engDestroy(this.handle);

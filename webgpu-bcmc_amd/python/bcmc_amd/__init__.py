@@ -97,7 +97,8 @@ class _Tuning(ctypes.Structure):
 
 
 class _ServeOpts(ctypes.Structure):
-    _fields_ = [("max_batch", ctypes.c_uint32), ("vertex_budget_bytes", ctypes.c_uint64)]
+    _fields_ = [("max_batch", ctypes.c_uint32), ("vertex_budget_bytes", ctypes.c_uint64),
+                ("no_coalesce", ctypes.c_uint32)]
 
 
 class _ServeResult(ctypes.Structure):
@@ -727,15 +728,17 @@ class Engine:
             return b.download(np.float32, 3 * n, 12 * off).reshape(-1, 3)
         return b.download(np.uint32, 2 * n, 8 * off).reshape(-1, 2)
 
-    def serve(self, arrival_s, volume_ids, isovalues, max_batch=0, vertex_budget_bytes=0):
+    def serve(self, arrival_s, volume_ids, isovalues, max_batch=0, vertex_budget_bytes=0, coalesce=True):
         """Replay a request trace in real time (continuous batching); returns a
-        structured array with arrival/start/finish times, vertices, batch size."""
+        structured array with arrival/start/finish times, vertices, batch size.
+        With coalesce, requests of one step with the same volume and isovalue
+        are answered by one query."""
         arr = np.ascontiguousarray(arrival_s, np.float64)
         ids = np.ascontiguousarray(volume_ids, np.uint32)
         isos = np.ascontiguousarray(isovalues, np.float32)
         n = len(arr)
         out = (_ServeResult * max(n, 1))()
-        opts = _ServeOpts(max_batch, vertex_budget_bytes)
+        opts = _ServeOpts(max_batch, vertex_budget_bytes, 0 if coalesce else 1)
         _check(lib().bcmc_serve_trace(self.h, arr.ctypes.data, ids.ctypes.data, isos.ctypes.data, n,
                                       ctypes.byref(opts), out))
         return {f: np.array([getattr(r, f) for r in out[:n]]) for f, _ in _ServeResult._fields_}
