@@ -5,13 +5,14 @@ trace replayed through Engine.serve (continuous batching), throughput and
 latency printed as one JSON line per replay.
 
     PYTHONPATH=webgpu-bcmc_amd/python python3 -m bcmc_amd.serve \\
-        [--requests N] [--rate R] [--max-batch B] [--pool-gib G] [--reps K]
+        [--requests N] [--rate R] [--max-batch B] [--pool-gib G] [--reps K] [--no-coalesce]
 
 Requests draw a volume uniformly and an isovalue U[0.1, 0.9], half of them
 repeating one of the last 16 isovalues of the same volume; --rate R > 0 spaces
 them by Poisson arrivals at R per second, --rate 0 has them all arrive at
-t = 0 (saturation).  A first replay of requests of their own draw warms the
-engine (buffer growth, graph capture, first decodes)."""
+t = 0 (saturation).  Requests of one step with the same volume and isovalue
+share one query unless --no-coalesce.  A first replay of requests of their own
+draw warms the engine (buffer growth, graph capture, first decodes)."""
 import argparse
 import json
 import sys
@@ -32,6 +33,7 @@ def main(argv=None):
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0, help="pool + output vertices")
     ap.add_argument("--reps", type=int, default=1)
     ap.add_argument("--seed", type=int, default=20240607)
+    ap.add_argument("--no-coalesce", action="store_true", help="every request a query of its own")
     args = ap.parse_args(argv)
     t0 = time.time()
     vols = synth.served_volumes()
@@ -50,10 +52,12 @@ def main(argv=None):
         ids, isos = bcmc_amd.request_trace(rng, args.requests, len(vols))
         arr = (np.cumsum(rng.exponential(1.0 / args.rate, args.requests)) if args.rate > 0
                else np.zeros(args.requests))
-        out = eng.serve(arr, ids, isos, max_batch=args.max_batch, vertex_budget_bytes=budget)
+        out = eng.serve(arr, ids, isos, max_batch=args.max_batch, vertex_budget_bytes=budget,
+                        coalesce=not args.no_coalesce)
         lat = out["finish_s"] - out["arrival_s"]
         span = float(out["finish_s"].max() - arr[0])
         print(json.dumps({"rep": rep, "requests": args.requests, "rate_rps": args.rate,
+                          "coalesce": not args.no_coalesce,
                           "qps": round(args.requests / span, 2),
                           "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
                           "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
