@@ -425,6 +425,15 @@ def main():
             "range_pass_ms": round(range_t, 4),
             "range_pass_GBps": round(decode_bytes_range / (range_t * 1e-3) / 1e9, 1),
             "range_pass_decoded_equiv_GBps": round(256 * n_blocks / (range_t * 1e-3) / 1e9, 1),
+            # per isovalue of the sweep: decoded blocks and device time of the
+            # cache-decode launch (mean over the timed steps), where the
+            # launch's fixed latency shows in the small miss lists
+            "per_query": [{"iso": round(iso, 4),
+                           "blocks": int(np.mean([r["numBlocksDecompressed"] for r in records
+                                                  if r["iso"] == iso])),
+                           "us": round(1e3 * float(np.mean([r["decompression"] for r in records
+                                                            if r["iso"] == iso])), 1)}
+                          for iso in sorted({r["iso"] for r in records})],
         },
         "zfp_encode": encode_out,
         "roofline": {"bound": "hbm", "kernel": dom,
