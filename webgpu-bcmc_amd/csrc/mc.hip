@@ -106,6 +106,7 @@ struct McArgs {
     void *verts;
     unsigned long long capacity;
     int fmt;
+    QState *q_out;  // host-coherent pinned copy of *q, written by the emit launch
 };
 
 BCMC_DEV RowBlock mc_row_base(const McArgs &a) {
@@ -161,13 +162,6 @@ __global__ __launch_bounds__(kMcThreads) void mc_count_kernel(McArgs a) {
     }
 }
 
-// The query's counters into host-coherent pinned memory.
-__global__ void qstate_out_kernel(const QState *q, QState *out) {
-    static_assert(sizeof(QState) % 4 == 0 && sizeof(QState) <= 64 * 4, "one word per lane");
-    if (threadIdx.x < sizeof(QState) / 4)
-        reinterpret_cast<uint32_t *>(out)[threadIdx.x] = reinterpret_cast<const uint32_t *>(q)[threadIdx.x];
-}
-
 // Exclusive prefixes of the chunk totals; the grand totals are the query's
 // numBlocksWithVertices and numVertices.
 struct ChunkPrefixOp {
@@ -202,6 +196,12 @@ __global__ __launch_bounds__(kMcThreads) __attribute__((amdgpu_waves_per_eu(5)))
     __shared__ uint32_t lds_w[NW + 1];
     __shared__ McTables tab;
     __shared__ EmitScratch es[NW];
+    // the query's counters into host-coherent pinned memory for the host's
+    // one read-back: every field is final before this launch (the emit
+    // writes none), so the first workgroup copies them at once
+    static_assert(sizeof(QState) % 4 == 0 && sizeof(QState) <= kMcThreads * 4, "one word per thread");
+    if (blockIdx.x == 0 && threadIdx.x < sizeof(QState) / 4)
+        reinterpret_cast<uint32_t *>(a.q_out)[threadIdx.x] = reinterpret_cast<const uint32_t *>(a.q)[threadIdx.x];
     load_mc_tables(&tab);
     const uint32_t n = a.q->num_active;
     const uint32_t nchunks = div_up(n, kChunk);
@@ -274,7 +274,7 @@ struct bcmc_mc {
     // captured again when anything it bakes in (buffers, format, sort passes)
     // changes.  The isovalue is read by the active scan straight from
     // host-coherent pinned memory, and the counters are written back into
-    // such memory by a one-wave kernel (no copy nodes).
+    // such memory by the emit launch's first workgroup (no copy nodes).
     float *iso_h = nullptr;
     const float *iso_d = nullptr;  // device address of iso_h
     QState *qh_d = nullptr;        // device address of qh
@@ -372,6 +372,7 @@ static McArgs mc_args(bcmc_mc *m) {
     a.verts = m->verts;
     a.capacity = m->vert_cap;
     a.fmt = m->fmt;
+    a.q_out = m->qh_d;
     return a;
 }
 
@@ -425,8 +426,6 @@ static int mc_enqueue_tail(bcmc_mc *m, bool timed) {
     BCMC_TRY(mc_launch_pass(m, timed ? &m->ev[4] : nullptr));
     if (timed)
         BCMC_HIP_CHECK(hipEventRecord(m->ev[6], st));
-    qstate_out_kernel<<<1, 64, 0, st>>>(m->q, m->qh_d);
-    BCMC_HIP_CHECK(hipGetLastError());
     return BCMC_OK;
 }
 
