@@ -142,6 +142,35 @@ __global__ void eng_seg_range_kernel(const float2 *dilated, uint32_t nbx, uint32
     }
 }
 
+// Per volume, once at load: a 2-D histogram of the blocks' dilated ranges,
+// bin(lo) * kRangeBins + bin(hi), where bin(x) is the number of the kRangeBins
+// - 1 ascending edges that are <= x.  Only comparisons, so the host's bin of
+// an isovalue agrees with it exactly (eng_union_bound).
+constexpr uint32_t kRangeBins = 1024;
+
+__global__ void eng_range_hist_kernel(const float2 *dilated, uint64_t n, const float *edges, uint32_t *hist) {
+    __shared__ float e[kRangeBins - 1];
+    for (uint32_t i = threadIdx.x; i < kRangeBins - 1; i += blockDim.x)
+        e[i] = edges[i];
+    __syncthreads();
+    auto bin = [&](float x) {
+        uint32_t lo = 0, hi = kRangeBins - 1;  // the answer lies in [lo, hi]
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) / 2;  // edges[mid] <= x: at least mid + 1 edges
+            if (e[mid] <= x)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        return lo;
+    };
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n;
+         b += (uint64_t)gridDim.x * blockDim.x) {
+        const float2 r = dilated[b];
+        atomicAdd(&hist[bin(r.x) * kRangeBins + bin(r.y)], 1u);
+    }
+}
+
 // A candidate row segment with everything its active test and output need,
 // so that the per-block work reads one 32-byte descriptor and one range.
 // SegActiveOp reads it as 8 words: dilated (0, 1), b0, qi, iso, nx, flag_base.
@@ -663,18 +692,54 @@ struct EngVolume {
     float2 *ranges = nullptr, *dilated = nullptr, *seg_range = nullptr, *tile = nullptr;
     int32_t *slot_map = nullptr;
     uint32_t *miss_list = nullptr;  // per-step block ids to decode (EngVol::miss_list)
-    // the row segments' ranges, lower and upper ends each sorted (host): the
-    // segments whose range holds an isovalue bound its active blocks
-    std::vector<float> seg_lo, seg_hi;
+    // bins of the dilated ranges (eng_range_hist_kernel): the kRangeBins - 1
+    // edges and below[a * (kRangeBins + 1) + b] = blocks with bin(lo) >= a and
+    // bin(hi) < b, a in [0, kRangeBins], b in [0, kRangeBins]
+    std::vector<float> edges;
+    std::vector<uint32_t> below;
+    uint32_t bin(float x) const {
+        return (uint32_t)(std::upper_bound(edges.begin(), edges.end(), x) - edges.begin());
+    }
+    // blocks strictly between the isovalues of bins a and c (a = -1: below
+    // every isovalue; c = kRangeBins: above): bin(lo) > a and bin(hi) < c, so
+    // lo and hi lie strictly between the two isovalues and no isovalue in
+    // bins <= a or >= c makes them active
+    uint64_t gap(int a, int c) const {
+        if (c - a < 2)
+            return 0;
+        return below[(uint64_t)(a + 1) * (kRangeBins + 1) + (uint32_t)c];
+    }
 };
 
-// At most this many blocks of volume v are active at iso: every active block
-// lies in a row segment whose range holds iso (kSegBlocks blocks each).
-static uint64_t eng_block_bound(const EngVolume &v, float iso) {
-    const uint64_t lo_le = std::upper_bound(v.seg_lo.begin(), v.seg_lo.end(), iso) - v.seg_lo.begin();
-    const uint64_t hi_lt = std::lower_bound(v.seg_hi.begin(), v.seg_hi.end(), iso) - v.seg_hi.begin();
-    return std::min<uint64_t>(v.n, (lo_le - std::min(lo_le, hi_lt)) * kSegBlocks);
-}
+// The blocks that queries at a set of isovalues on one volume can touch
+// together: every block but those whose dilated range lies strictly between
+// two neighbouring isovalues of the set, or strictly below or above all of
+// them (counted from the range bins, so only blocks that are inactive for
+// every isovalue are left out).  Kept per bin: isovalues in one bin leave the
+// same blocks out.
+struct EngUnion {
+    std::vector<int> bins;  // ascending, distinct
+    uint64_t inactive = 0;  // blocks left out
+    bool empty() const { return bins.empty(); }
+    // blocks touched after adding isovalue bin b (the set itself unchanged)
+    uint64_t with(const EngVolume &v, int b, uint64_t *inactive_after) const {
+        const auto it = std::lower_bound(bins.begin(), bins.end(), b);
+        uint64_t in = bins.empty() ? v.n : inactive;
+        if (it == bins.end() || *it != b) {
+            const int a = it == bins.begin() ? -1 : *(it - 1);
+            const int c = it == bins.end() ? (int)kRangeBins : *it;
+            in = in - (bins.empty() ? v.n : v.gap(a, c)) + v.gap(a, b) + v.gap(b, c);
+        }
+        *inactive_after = in;
+        return v.n - std::min<uint64_t>(v.n, in);
+    }
+    void add(int b, uint64_t inactive_after) {
+        const auto it = std::lower_bound(bins.begin(), bins.end(), b);
+        if (it == bins.end() || *it != b)
+            bins.insert(it, b);
+        inactive = inactive_after;
+    }
+};
 
 using EngKey = std::array<uint64_t, 6>;
 struct EngGraph {
@@ -1157,18 +1222,67 @@ static int eng_add_volume(bcmc_engine *e, const void *stream, uint64_t bytes, do
                                                                          v.seg_range);
     BCMC_HIP_CHECK(hipGetLastError());
     {
+        // range bins: edges spread evenly over the volume's finite dilated
+        // ranges (the row segments' ranges span them); any ascending edges
+        // give exact bounds, even ones make them tight
         std::vector<float2> sr(nsegs);
         BCMC_HIP_CHECK(hipMemcpyAsync(sr.data(), v.seg_range, (size_t)nsegs * 8, hipMemcpyDeviceToHost,
                                       ctx->stream));
         BCMC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-        v.seg_lo.resize(nsegs);
-        v.seg_hi.resize(nsegs);
-        for (uint32_t i = 0; i < nsegs; ++i) {
-            v.seg_lo[i] = sr[i].x;
-            v.seg_hi[i] = sr[i].y;
+        float gmin = INFINITY, gmax = -INFINITY;
+        for (const float2 &r : sr) {
+            if (std::isfinite(r.x))
+                gmin = std::min(gmin, r.x);
+            if (std::isfinite(r.y))
+                gmax = std::max(gmax, r.y);
         }
-        std::sort(v.seg_lo.begin(), v.seg_lo.end());
-        std::sort(v.seg_hi.begin(), v.seg_hi.end());
+        if (!std::isfinite(gmin))
+            gmin = 0.0f;
+        if (!(gmax > gmin))
+            gmax = gmin + 1.0f;
+        v.edges.resize(kRangeBins - 1);
+        for (uint32_t i = 0; i + 1 < kRangeBins; ++i)
+            v.edges[i] = (float)((double)gmin + ((double)gmax - (double)gmin) * (i + 1) / kRangeBins);
+        for (uint32_t i = 1; i + 1 < kRangeBins; ++i)
+            v.edges[i] = std::max(v.edges[i], v.edges[i - 1]);
+        float *d_edges = nullptr;
+        uint32_t *d_hist = nullptr;
+        const size_t hbytes = (size_t)kRangeBins * kRangeBins * 4;
+        int rc = dev_alloc((void **)&d_edges, (kRangeBins - 1) * 4);
+        if (rc == BCMC_OK)
+            rc = dev_alloc((void **)&d_hist, hbytes);
+        std::vector<uint32_t> hist;
+        if (rc == BCMC_OK) {
+            hist.resize((size_t)kRangeBins * kRangeBins);
+            if (hipMemcpyAsync(d_edges, v.edges.data(), (kRangeBins - 1) * 4, hipMemcpyHostToDevice,
+                               ctx->stream) != hipSuccess ||
+                hipMemsetAsync(d_hist, 0, hbytes, ctx->stream) != hipSuccess) {
+                rc = BCMC_ERR_HIP;
+            } else {
+                eng_range_hist_kernel<<<eng_grid(ctx, n), 256, 0, ctx->stream>>>(v.dilated, n, d_edges, d_hist);
+                if (hipGetLastError() != hipSuccess ||
+                    hipMemcpyAsync(hist.data(), d_hist, hbytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                    hipStreamSynchronize(ctx->stream) != hipSuccess)
+                    rc = BCMC_ERR_HIP;
+            }
+        }
+        dev_free(d_edges);
+        dev_free(d_hist);
+        if (rc != BCMC_OK) {
+            set_error("range histogram of the volume failed");
+            return rc;
+        }
+        // below[a][b] = blocks with bin(lo) >= a and bin(hi) < b
+        constexpr uint32_t W = kRangeBins + 1;
+        v.below.assign((size_t)W * W, 0u);
+        for (int a = (int)kRangeBins - 1; a >= 0; --a) {
+            uint32_t row = 0;
+            for (uint32_t b = 0; b < kRangeBins; ++b) {
+                v.below[(size_t)a * W + b] = v.below[(size_t)(a + 1) * W + b] + row;
+                row += hist[(size_t)a * kRangeBins + b];
+            }
+            v.below[(size_t)a * W + kRangeBins] = v.below[(size_t)(a + 1) * W + kRangeBins] + row;
+        }
     }
     dev_free(v.ranges);  // only the dilated and tile ranges are used from here on
     dev_free(parts);
@@ -1609,19 +1723,29 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
         BCMC_TRY(eng_ensure_queries(e, nq));
         BCMC_TRY(eng_ensure_chunks(e, eng_max_chunks(best_b, nq)));
     }
+    // the slots stamped by the engine's last step (an earlier serve call or
+    // bcmc_engine_run) may not be evicted by the next one; an empty step
+    // before the clock makes them evictable, so the first step of this call
+    // has no predecessor's blocks to keep
+    BCMC_TRY(eng_idle_step(e));
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     auto now_s = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
     // Pool admission: a step's requests may touch at most the pool's slots
     // minus those the previous step stamped (which its successor may not
-    // evict), counted by the block bound of each request
-    // (eng_block_bound: blocks in row segments whose range holds the
-    // isovalue).  Output admission: the step's run admits the longest prefix
-    // whose exact vertex counts fit the budget; the rest wait at the head of
-    // the queue for the next step.
-    std::vector<uint64_t> bound(n);
-    for (uint32_t i = 0; i < n; ++i)
-        bound[i] = eng_block_bound(e->vols[volume_ids[i]], isovalues[i]);
+    // evict), counted per volume by the union bound of the step's isovalues
+    // on it (EngUnion: every block but those whose dilated range lies between
+    // two neighbouring isovalues or outside all of them).  Output admission:
+    // the step's run admits the longest prefix whose exact vertex counts fit
+    // the budget; the rest wait at the head of the queue for the next step.
+    std::vector<uint32_t> rbin(n);
+    std::vector<uint64_t> alone(n);  // a request's bound on its own
+    for (uint32_t i = 0; i < n; ++i) {
+        const EngVolume &v = e->vols[volume_ids[i]];
+        rbin[i] = v.bin(isovalues[i]);
+        uint64_t in;
+        alone[i] = EngUnion{}.with(v, (int)rbin[i], &in);
+    }
     const uint64_t pool = e->pool_cap;
     uint64_t prev_blocks = 0;
     uint32_t next = 0, done = 0, steps = 0;
@@ -1631,7 +1755,7 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
     std::vector<float> isos;
     std::vector<uint32_t> batch;
     std::vector<bcmc_query_result> res;
-    std::vector<uint64_t> vsum(e->vols.size());
+    std::vector<EngUnion> uni(e->vols.size());
     std::unordered_map<uint64_t, uint32_t> in_step;  // (volume, isovalue bits) of the step's requests
     while (done < n) {
         const double t = now_s();
@@ -1644,7 +1768,7 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
                 std::this_thread::sleep_for(std::chrono::duration<double>(wait));
             continue;
         }
-        if (bound[pending[head]] + prev_blocks > pool && prev_blocks > 0) {
+        if (alone[pending[head]] + prev_blocks > pool && prev_blocks > 0) {
             // the head request needs the previous step's slots: one empty
             // step makes them evictable
             BCMC_TRY(eng_idle_step(e));
@@ -1653,8 +1777,8 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
         ids.clear();
         isos.clear();
         batch.clear();
-        // a volume's requests together touch at most all of its blocks
-        std::fill(vsum.begin(), vsum.end(), 0);
+        for (auto &u : uni)
+            u = EngUnion{};
         in_step.clear();
         uint64_t blocks = 0, step_blocks = 0;
         for (size_t h = head; h < pending.size() && batch.size() < max_batch; ++h) {
@@ -1663,13 +1787,15 @@ extern "C" BCMC_API int bcmc_serve_trace(bcmc_engine *e, const double *arrival_s
             std::memcpy(&bits, &isovalues[r], 4);
             // a request equal to one already in the step adds no query
             const bool dup = coalesce && in_step.count((uint64_t)v << 32 | bits) != 0;
-            const uint64_t nblk = dup ? 0 : e->vols[v].n, add = dup ? 0 : bound[r];
-            const uint64_t grown = blocks - std::min<uint64_t>(e->vols[v].n, vsum[v]) +
-                                   std::min<uint64_t>(e->vols[v].n, vsum[v] + add);
+            const uint64_t nblk = dup ? 0 : e->vols[v].n;
+            const uint64_t before = uni[v].empty() ? 0 : e->vols[v].n - std::min<uint64_t>(e->vols[v].n, uni[v].inactive);
+            uint64_t in_after;
+            const uint64_t after = uni[v].with(e->vols[v], (int)rbin[r], &in_after);
+            const uint64_t grown = blocks - before + after;
             if (!batch.empty() && (grown + prev_blocks > pool || step_blocks + nblk > kMaxStepBlocks))
                 break;
             blocks = grown;
-            vsum[v] += add;
+            uni[v].add((int)rbin[r], in_after);
             step_blocks += nblk;
             if (coalesce)
                 in_step.emplace((uint64_t)v << 32 | bits, r);
