@@ -314,6 +314,14 @@ BCMC_API int bcmc_engine_run(bcmc_engine *e, const uint32_t *volume_ids, const f
 /* The last step's vertices (results[i].vertex_offset .. + num_vertices). */
 BCMC_API bcmc_buf *bcmc_engine_vertex_buffer(bcmc_engine *e);
 BCMC_API int bcmc_engine_stats(bcmc_engine *e, bcmc_engine_info *info);
+/* Pool slots that one step of queries at isovalues[0..n) on volume_id may
+ * need (the served loop's admission bound): the volume's blocks except those
+ * whose dilated range (the active test's) lies strictly between two of the
+ * isovalues or strictly below or above all of them, counted from a histogram
+ * of the ranges built at add_volume.  Never below the blocks a step of these
+ * queries makes active; 0 for n == 0. */
+BCMC_API int bcmc_engine_block_bound(bcmc_engine *e, uint32_t volume_id, const float *isovalues, uint32_t n,
+                                     uint64_t *blocks);
 
 /* Served form: replay timestamped requests in real time through one engine
  * with continuous batching (each step takes the requests that have arrived,

@@ -135,6 +135,32 @@ def _oracle_active(s, rate, dims, iso):
     return nv, ref.results()["active_ids"].copy()
 
 
+def test_admission_bound_covers_active_union(dev):
+    """The served loop's pool bound for a set of isovalues on a volume is at
+    least the union of their active blocks (the oracle's) and at most the
+    volume's blocks; it never shrinks as isovalues are added, repeats add
+    nothing and no isovalue means no block."""
+    streams = _streams()
+    eng = bcmc_amd.Engine(dev, 100000 * 256)
+    ids = [eng.add_volume(s, r, d) for s, r, d in streams]
+    rng = np.random.default_rng(41)
+    for k, (s, r, d) in enumerate(streams):
+        say("admission bound, volume", k)
+        nb = oracle.nblocks(d)
+        isos = rng.uniform(0.05, 0.95, 12).astype(np.float32)
+        act = [set(_oracle_active(s, r, d, float(x))[1].tolist()) for x in isos]
+        last = 0
+        for m in range(1, 13):
+            union = set().union(*act[:m])
+            b = eng.block_bound(ids[k], isos[:m])
+            assert len(union) <= b <= nb, (k, m, len(union), b)
+            assert b >= last, (k, m)
+            last = b
+        assert eng.block_bound(ids[k], np.concatenate([isos[:3], isos[:3]])) == \
+            eng.block_bound(ids[k], isos[:3])
+        assert eng.block_bound(ids[k], []) == 0
+
+
 def test_served_trace_waits_for_pool_room(dev):
     """A pool that holds every single request's active blocks but not those
     of a step of piled-up requests: the served loop admits what fits (an

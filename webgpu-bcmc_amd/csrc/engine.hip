@@ -1671,6 +1671,23 @@ BCMC_API int bcmc_engine_stats(bcmc_engine *e, bcmc_engine_info *info) {
     return BCMC_OK;
 }
 
+BCMC_API int bcmc_engine_block_bound(bcmc_engine *e, uint32_t volume_id, const float *isovalues, uint32_t n,
+                                     uint64_t *blocks) {
+    BCMC_CHECK_ARG(e && blocks && (n == 0 || isovalues), "null argument");
+    BCMC_CHECK_ARG(volume_id < e->vols.size(), "unknown volume id");
+    const EngVolume &v = e->vols[volume_id];
+    EngUnion u;
+    uint64_t touched = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const int b = (int)v.bin(isovalues[i]);
+        uint64_t in;
+        touched = u.with(v, b, &in);
+        u.add(b, in);
+    }
+    *blocks = touched;
+    return BCMC_OK;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

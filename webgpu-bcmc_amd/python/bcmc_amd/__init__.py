@@ -183,6 +183,7 @@ def lib():
         "bcmc_engine_run": (i32, [vp, vp, vp, u32, vp]),
         "bcmc_engine_vertex_buffer": (vp, [vp]),
         "bcmc_engine_stats": (i32, [vp, P(_EngineInfo)]),
+        "bcmc_engine_block_bound": (i32, [vp, u32, vp, u32, P(u64)]),
         "bcmc_serve_trace": (i32, [vp, vp, vp, vp, u32, P(_ServeOpts), vp]),
         "bcmc_zfp_decode_blocks": (i32, [vp, vp, u32, vp, u32, vp]),
         "bcmc_zfp_block_ranges": (i32, [vp, vp, u32, P(u32), vp]),
@@ -747,6 +748,15 @@ class Engine:
         i = _EngineInfo()
         _check(lib().bcmc_engine_stats(self.h, ctypes.byref(i)))
         return {f: getattr(i, f) for f, _ in _EngineInfo._fields_}
+
+    def block_bound(self, volume_id, isovalues):
+        """Pool slots one step of queries at these isovalues on the volume may
+        need (the served loop's admission bound)."""
+        isos = np.ascontiguousarray(isovalues, np.float32)
+        out = ctypes.c_uint64()
+        _check(lib().bcmc_engine_block_bound(self.h, volume_id, isos.ctypes.data, len(isos),
+                                             ctypes.byref(out)))
+        return int(out.value)
 
 
 # ---------------------------------------------------------------------------
