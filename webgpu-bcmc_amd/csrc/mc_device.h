@@ -355,7 +355,10 @@ struct alignas(16) EmitScratch {
 };
 
 typedef float V4f __attribute__((ext_vector_type(4)));
+typedef uint32_t V4u __attribute__((ext_vector_type(4)));
 typedef uint32_t V2u __attribute__((ext_vector_type(2)));
+// cache-policy operand of the raw buffer store builtins: non-temporal
+constexpr int kStoreNontemporal = 2;
 
 BCMC_DEV void row_store(float *dst, const float (&r)[5]) {
     *reinterpret_cast<float4 *>(dst) = make_float4(r[0], r[1], r[2], r[3]);
@@ -427,6 +430,18 @@ BCMC_DEV void emit_row_blocks(const RowBlock &b, const RowTile &t, const McTable
     // (64 r + l) / 3 (ds_bpermute), so the stores stay one contiguous run.
     const uint32_t ntri = (limit + 2) / 3;
     uint32_t carry = 0;
+    // block-local format: stores through a buffer resource over the group's
+    // first `limit` vertices, which drops those past it, so the three
+    // sub-rounds of a round are straight-line code (gfx9 raw buffer
+    // descriptor, word 3 = 0x00020000: 32-bit data format, no swizzle)
+    // (out0 and limit are the same in every lane: the descriptor is built in
+    // scalar registers)
+    const unsigned long long out0_s =
+        ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(out0 >> 32)) << 32) |
+        __builtin_amdgcn_readfirstlane((uint32_t)out0);
+    const uint32_t limit_s = __builtin_amdgcn_readfirstlane(limit);
+    [[maybe_unused]] const __amdgpu_buffer_rsrc_t vrs = __builtin_amdgcn_make_buffer_rsrc(
+        reinterpret_cast<V4f *>(verts) + out0_s, 0, (int)(limit_s * 16u), 0x00020000);
     for (uint32_t t0 = 0; t0 < ntri; t0 += 64) {
         const uint32_t t = t0 + lane;
         const uint32_t g = max(carry, wave_incl_max(t < ntri ? (uint32_t)es.start[t] : 0u));
@@ -466,15 +481,17 @@ BCMC_DEV void emit_row_blocks(const RowBlock &b, const RowTile &t, const McTable
             const float py = (ea.y + tt * ed.y + (float)cy) + 0.5f;
             const float pz = (ea.z + tt * ed.z + (float)cz) + 0.5f;
             const unsigned long long o = out0 + v;
-            if (v >= limit)
-                continue;
             // vertices are written once and not read back by the pass:
             // non-temporal stores keep them from displacing the block data
             // in the caches
             if constexpr (FMT == BCMC_VERTS_LOCAL_F32) {
-                const V4f vv = {px, py, pz, __uint_as_float(bid)};
-                __builtin_nontemporal_store(vv, reinterpret_cast<V4f *>(verts) + o);
-            } else if constexpr (FMT == BCMC_VERTS_GLOBAL_F32) {
+                const V4u vv = {__float_as_uint(px), __float_as_uint(py), __float_as_uint(pz), bid};
+                __builtin_amdgcn_raw_buffer_store_b128(vv, vrs, (int)(v * 16u), 0, kStoreNontemporal);
+                continue;
+            }
+            if (v >= limit)
+                continue;
+            if constexpr (FMT == BCMC_VERTS_GLOBAL_F32) {
                 float *gp = reinterpret_cast<float *>(verts) + 3 * o;
                 __builtin_nontemporal_store((float)(4 * es.bpos[j][0]) + px, gp);
                 __builtin_nontemporal_store((float)(4 * es.bpos[j][1]) + py, gp + 1);
