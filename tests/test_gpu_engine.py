@@ -246,6 +246,42 @@ def test_served_trace_defers_past_the_vertex_budget(dev):
     assert e.value.code == -3
 
 
+def test_coalesced_requests_under_the_vertex_budget(dev):
+    """Coalesced requests under an output budget: the budget counts each
+    distinct query of a step once, admission stays a prefix of the requests
+    (a repeat of an admitted query is admitted with it), and every request
+    gets the oracle's count."""
+    streams = _streams()
+    rng = np.random.default_rng(43)
+    n = 48
+    vol = rng.integers(0, len(streams), n).astype(np.uint32)
+    choices = rng.uniform(0.3, 0.7, (len(streams), 4)).astype(np.float32)
+    iso = choices[vol, rng.integers(0, 4, n)]
+    nv = {}
+    for i in range(n):
+        key = (int(vol[i]), float(iso[i]))
+        if key not in nv:
+            s, r, d = streams[vol[i]]
+            nv[key] = _oracle(s, r, d, float(iso[i]))[0]
+    counts = np.array([nv[(int(vol[i]), float(iso[i]))] for i in range(n)], np.uint64)
+    budget = 2 * int(counts.max()) * 16 + 16
+    eng = bcmc_amd.Engine(dev, 100000 * 256)
+    for s, r, d in streams:
+        eng.add_volume(s, r, d)
+    out = eng.serve(np.zeros(n), vol, iso, max_batch=16, vertex_budget_bytes=budget)
+    assert np.array_equal(out["num_vertices"].astype(np.uint64), counts)
+    steps = out["step"]
+    assert np.all(np.diff(steps) >= 0)
+    shared = False
+    for st_ in np.unique(steps):
+        sel = np.flatnonzero(steps == st_)
+        keys = {(int(vol[i]), float(iso[i])) for i in sel}
+        assert sum(nv[k] for k in keys) * 16 <= budget, st_
+        shared |= len(keys) < len(sel)
+    assert shared  # some step answered a repeat by an earlier request's query
+    assert steps.max() + 1 < n  # steps took several requests
+
+
 def test_result_is_the_same_alone_and_in_the_mix(dev):
     """A query's vertices are bit-identical whether it runs alone or among
     other queries of other volumes (and after the pool has been churned)."""
