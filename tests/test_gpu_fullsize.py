@@ -114,3 +114,45 @@ def test_blobs1024x1024x1080_rate2_one_query(dev):
         assert np.all(err <= tol), a
         worst = max(worst, float(err.max()))
     say("  global FP32 vertices", nv, "max |fp32 - fp64|", worst)
+
+
+def test_vertex_indices_past_2_pow_31(dev):
+    """A query with more than 2^31 vertices: a 640x640x600 field of uniform
+    noise at rate 8 and the median isovalue (about 9.6 vertices per cell,
+    2.35G in all).  The block-local vertices agree with the global FP32
+    format at indices on both sides of 2^31 and at the end (4 * block_pos +
+    local, evaluated as the emit evaluates it), and the batched engine's
+    vertices of the same query equal the plain pipeline's there."""
+    dims = (640, 640, 600)
+    nb = [(d + 3) // 4 for d in dims]
+    n = int(np.prod(nb))
+    field = np.random.default_rng(12).random((dims[2], dims[1], dims[0]), dtype=np.float32)
+    fbuf = bcmc_amd.Buffer.alloc(dev, field.nbytes)
+    fbuf.upload(field)
+    del field
+    stream = bcmc_amd.encode(dev, fbuf, dims, 64 * 8)
+    fbuf.destroy()
+    mc = bcmc_amd.CompressedMarchingCubes(dev)
+    mc.setCompressedVolume(stream, 8, dims, (1, 1, 1), n)
+    nv = mc.computeSurface(0.5)
+    say("  noise 640x640x600 r8 at 0.5:", nv, "vertices")
+    assert nv > 2**31 + (1 << 20)
+    k = 4096
+    starts = [0, 2**31 - k, 2**31 - 64, nv - k]
+    local = [mc.vertexBuffer.download(np.float32, 4 * k, 16 * s).reshape(-1, 4) for s in starts]
+    mc.setVertexFormat(bcmc_amd.VERTS_GLOBAL_F32)
+    assert mc.computeSurface(0.5) == nv
+    glob = [mc.vertexBuffer.download(np.float32, 3 * k, 12 * s).reshape(-1, 3) for s in starts]
+    for s, lv, gv in zip(starts, local, glob):
+        bid = lv[:, 3].view(np.uint32).astype(np.int64)
+        pos = np.stack([bid % nb[0], (bid // nb[0]) % nb[1], bid // (nb[0] * nb[1])], 1)
+        want = (4 * pos).astype(np.float32) + lv[:, :3]
+        assert np.array_equal(gv.view(np.uint32), want.view(np.uint32)), s
+    del mc
+    eng = bcmc_amd.Engine(dev, n * 256)
+    vid = eng.add_volume(stream, 8, dims)
+    res = eng.run([vid], [0.5])[0]
+    assert res["num_vertices"] == nv and res["vertex_offset"] == 0
+    for s, lv in zip(starts, local):
+        got = eng.vertices(res, s, k)
+        assert np.array_equal(got.view(np.uint32), lv.view(np.uint32)), s

@@ -435,11 +435,13 @@ BCMC_DEV void emit_row_blocks(const RowBlock &b, const RowTile &t, const McTable
     // sub-rounds of a round are straight-line code (gfx9 raw buffer
     // descriptor, word 3 = 0x00020000: 32-bit data format, no swizzle)
     // (out0 and limit are the same in every lane: the descriptor is built in
-    // scalar registers)
+    // scalar registers; readfirstlane returns int, so each word goes through
+    // uint32_t before it is widened -- a low word with bit 31 set would
+    // otherwise sign-extend into the high word of the address)
     const unsigned long long out0_s =
-        ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(out0 >> 32)) << 32) |
-        __builtin_amdgcn_readfirstlane((uint32_t)out0);
-    const uint32_t limit_s = __builtin_amdgcn_readfirstlane(limit);
+        ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(out0 >> 32)) << 32) |
+        (unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)out0);
+    const uint32_t limit_s = (uint32_t)__builtin_amdgcn_readfirstlane(limit);
     [[maybe_unused]] const __amdgpu_buffer_rsrc_t vrs = __builtin_amdgcn_make_buffer_rsrc(
         reinterpret_cast<V4f *>(verts) + out0_s, 0, (int)(limit_s * 16u), 0x00020000);
     for (uint32_t t0 = 0; t0 < ntri; t0 += 64) {

@@ -720,9 +720,12 @@ class Engine:
         _check(lib().bcmc_engine_run(self.h, ids.ctypes.data, isos.ctypes.data, len(ids), res))
         return [{f: getattr(r, f) for f, _ in _QueryResult._fields_} for r in res[:len(ids)]]
 
-    def vertices(self, result):
+    def vertices(self, result, first=0, count=None):
+        """Vertices first .. first + count (default: all) of a query's range
+        in the last step's vertex buffer."""
         b = _view(self.device, lib().bcmc_engine_vertex_buffer(self.h))
-        n, off = result["num_vertices"], result["vertex_offset"]
+        n = result["num_vertices"] - first if count is None else count
+        off = result["vertex_offset"] + first
         if self.fmt == VERTS_LOCAL_F32:
             return b.download(np.float32, 4 * n, 16 * off).reshape(-1, 4)
         if self.fmt == VERTS_GLOBAL_F32:
