@@ -365,3 +365,45 @@ def test_repeated_step_shapes_replay_a_graph(dev, pool_blocks):
     assert st["steps"] == len(plan)
     # vertex-buffer growth re-captures, so not every repeat replays
     assert st["graph_steps"] >= 2, st
+
+
+def test_nonfinite_and_tied_isovalues_in_steps(dev):
+    """NaN and +-inf isovalues (every comparison of the active test false: no
+    active block, no vertex) and isovalues equal to voxel values (a block's
+    minimum, another block's maximum), beside ordinary ones on every volume:
+    each query of a batched step, in two orders, equals the oracle's single
+    query (counts and vertices bit-exact); served (coalesced and not), every
+    request gets the oracle's count; the admission bound of such a set still
+    covers the union of its active blocks."""
+    say("engine non-finite and tied isovalues")
+    streams = _streams()
+    eng = bcmc_amd.Engine(dev, 100000 * 256)
+    ids = [eng.add_volume(s, r, d) for s, r, d in streams]
+    qv, qi, want = [], [], []
+    for v, (s, r, d) in zip(ids, streams):
+        rg = np.asarray(oracle.MC(s, 64 * r, d).ranges(), np.float32)
+        ties = [float(rg[len(rg) // 2, 0]), float(rg[len(rg) // 3, 1])]
+        for iso in [float("nan"), float("inf"), -float("inf"), *ties, 0.5]:
+            qv.append(v)
+            qi.append(iso)
+            want.append(_oracle(s, r, d, iso))
+            if not np.isfinite(iso):
+                assert want[-1][0] == 0
+        active = set()
+        for iso in ties:
+            active |= set(_oracle_active(s, r, d, iso)[1].tolist())
+        assert eng.block_bound(v, [float("nan"), float("inf"), -float("inf"), *ties]) >= len(active)
+    qv = np.array(qv, np.uint32)
+    qi = np.array(qi, np.float32)
+    order = np.random.default_rng(17).permutation(len(qv))
+    for perm in (np.arange(len(qv)), order):
+        res = eng.run(qv[perm], qi[perm])
+        for k, j in enumerate(perm):
+            nv, verts = want[j]
+            assert res[k]["num_vertices"] == nv, (k, qi[j])
+            got = eng.vertices(res[k])
+            assert np.array_equal(got.view(np.uint32), verts.view(np.uint32)), (k, qi[j])
+    for coalesce in (True, False):
+        out = eng.serve(np.zeros(len(qv)), qv[order], qi[order], max_batch=8, coalesce=coalesce)
+        for k, j in enumerate(order):
+            assert out["num_vertices"][k] == want[j][0], (coalesce, k, qi[j])

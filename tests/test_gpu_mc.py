@@ -126,6 +126,37 @@ def test_empty_and_full_isovalues(dev):
         _check_query(mc, ref, iso)
 
 
+def _tie_isovalues(ranges):
+    """Isovalues equal to decoded voxel values: a block's minimum and another
+    block's maximum (both taken from the block ranges, which are voxel values)."""
+    r = np.asarray(ranges, np.float32)
+    return [float(r[len(r) // 2, 0]), float(r[len(r) // 3, 1])]
+
+
+def test_nonfinite_and_tied_isovalues(dev):
+    """NaN and +-inf isovalues make every comparison of the active test false
+    (no active block, no vertex: compute_block_active.comp:30-60); isovalues
+    equal to voxel values exercise the comparisons at equality in the active
+    test, the case index and the interpolation.  A ragged volume, so partial
+    blocks take part; the cache state carries across the queries."""
+    dims, rate = (30, 21, 13), 4
+    say("mc non-finite and tied isovalues", dims)
+    s = _volume(dims, synth.GAUSS, 12, rate, 6, (3, 8, 0.5, 1))
+    mc = bcmc_amd.CompressedMarchingCubes(dev)
+    mc.setCompressedVolume(s, rate, dims)
+    ref = oracle.MC(s, 64 * rate, dims)
+    ties = _tie_isovalues(ref.ranges())
+    for iso in [float("nan"), float("inf"), -float("inf"), *ties, 0.5, float("nan"), ties[0]]:
+        nv, r = _check_query(mc, ref, iso)
+        v = mc.vertices()
+        assert np.array_equal(v.view(np.uint32), r["verts_f32"].view(np.uint32)), iso
+        if not np.isfinite(iso):
+            assert nv == 0 and mc.info()["num_active_blocks"] == 0, iso
+        else:
+            assert nv > 0, iso
+            assert np.max(np.abs(v[:, :3].astype(np.float64) - r["verts_f64"])) <= TOL
+
+
 def test_perf_tracker_keys(dev):
     dims = (32, 32, 32)
     s = _volume(dims, synth.GAUSS, 9, 4, 6, (3, 8, 0.5, 1))
