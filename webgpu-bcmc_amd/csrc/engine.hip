@@ -87,7 +87,7 @@ struct EngCounters {
     uint32_t short_slots;       // misses that found no slot (admission failed)
     uint32_t num_rows;          // candidate row segments of all queries
     uint32_t num_chunks;        // marching-cubes chunks of all queries
-    uint32_t emit_chunks;       // chunks the emit launch writes (set by the host per step)
+    uint32_t pad;
     unsigned long long num_verts;
     uint32_t vol_miss[64];      // misses per volume
 };
@@ -375,7 +375,9 @@ struct EngMcArgs {
     uint32_t *counts;                    // per pair: vertices
     uint32_t *chunk_verts;               // per chunk: vertices
     unsigned long long *chunk_verts_pre; // exclusive prefixes of chunk_verts
-    unsigned long long *q_vfirst;        // per query: vertex offset of its first active pair
+    // chunks the emit launch writes: set by the host per step, in
+    // host-coherent pinned memory (device alias)
+    const uint32_t *emit_chunks;
     void *verts;
     unsigned long long vert_cap;
     int fmt;
@@ -630,11 +632,29 @@ struct EngChunkPrefixOp {
     BCMC_DEV void total(const unsigned long long (&t)[1]) const { c->num_verts = t[0]; }
 };
 
-// Where each query's vertices start (its first chunk's prefix); ~0 for a
-// query without active blocks.
-__global__ void eng_vfirst_kernel(EngMcArgs a) {
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < a.nq; q += gridDim.x * blockDim.x)
-        a.q_vfirst[q] = a.q_act_start[q] == kNoPairs ? ~0ull : a.chunk_verts_pre[a.q_chunk_base[q]];
+// What the host reads after the step's front chain, written by its last
+// launch straight into host-coherent pinned memory (device aliases of the
+// host buffers), so one stream synchronisation and no copy launches bring it
+// to the host.
+struct EngReadback {
+    EngCounters *c;                   // the step's counters
+    uint32_t *act_start;              // per query: first active pair (kNoPairs: none)
+    unsigned long long *vfirst;       // per query: its first vertex (~0: no active blocks)
+    uint32_t *chunk_base;             // per query: its first chunk
+};
+
+// Where each query's vertices start (its first chunk's prefix), with the
+// query's first pair and chunk and the step's counters, to the host.
+__global__ void eng_vfirst_kernel(EngMcArgs a, EngReadback rb) {
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < a.nq; q += gridDim.x * blockDim.x) {
+        const uint32_t s = a.q_act_start[q], cb = a.q_chunk_base[q];
+        rb.vfirst[q] = s == kNoPairs ? ~0ull : a.chunk_verts_pre[cb];
+        rb.act_start[q] = s;
+        rb.chunk_base[q] = cb;
+    }
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < sizeof(EngCounters) / 4; i += blockDim.x)
+            reinterpret_cast<uint32_t *>(rb.c)[i] = reinterpret_cast<const uint32_t *>(a.c)[i];
 }
 
 template <int FMT>
@@ -647,7 +667,7 @@ __global__ __launch_bounds__(kEngThreads) __attribute__((amdgpu_waves_per_eu(5))
     load_mc_tables(&tab);
     // the host admits the step's leading queries whose vertices fit the
     // output budget: only their chunks are written
-    const uint32_t nchunks = min(a.c->num_chunks, a.c->emit_chunks), epoch = a.c->epoch;
+    const uint32_t nchunks = min(a.c->num_chunks, *a.emit_chunks), epoch = a.c->epoch;
     const uint32_t wave = threadIdx.x / 64;
     for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
         const uint32_t q = eng_query_of_chunk(a, chunk);
@@ -760,7 +780,7 @@ struct bcmc_engine {
     unsigned long long *owner = nullptr, *miss = nullptr;
     uint32_t miss_cap = 0;
     SortScratch sort;
-    EngCounters *c = nullptr, *ch = nullptr;
+    EngCounters *c = nullptr, *ch = nullptr, *ch_d = nullptr;
     // per-step arrays (grow)
     uint64_t blocks_cap = 0;
     uint32_t *flags = nullptr, *act_q = nullptr, *act_b = nullptr;
@@ -778,12 +798,14 @@ struct bcmc_engine {
     uint64_t same_cap = 0;
     uint32_t *tile_q = nullptr, *h_tile_q = nullptr;  // query of each scan tile's first row segment
     uint64_t tile_q_cap = 0;
-    // pinned host staging of the per-step query tables and results
+    // pinned host staging of the per-step query tables; the per-query
+    // results and the counters (ch) are host-coherent, written by the device
+    // through their aliases (_d), as is the emit's chunk count (h_emit)
     EngQuery *h_q = nullptr;
-    uint32_t *h_same = nullptr, *h_act = nullptr;
-    unsigned long long *h_vfirst = nullptr;
-    uint32_t *h_chunk_base = nullptr;
-    unsigned long long *q_vfirst = nullptr;
+    uint32_t *h_same = nullptr, *h_act = nullptr, *h_act_d = nullptr;
+    unsigned long long *h_vfirst = nullptr, *h_vfirst_d = nullptr;
+    uint32_t *h_chunk_base = nullptr, *h_chunk_base_d = nullptr;
+    uint32_t *h_emit = nullptr, *h_emit_d = nullptr;
     ScanSite site_active, site_cand;
     void *verts = nullptr;
     uint64_t vert_cap = 0;
@@ -896,21 +918,22 @@ static int eng_ensure_queries(bcmc_engine *e, uint32_t nq) {
     eng_release(e->q_act_start);
     eng_release(e->q_pair_end);
     eng_release(e->q_chunk_base);
-    eng_release(e->q_vfirst);
     eng_release_host(e->h_q);
     eng_release_host(e->h_act);
     eng_release_host(e->h_vfirst);
     eng_release_host(e->h_chunk_base);
     const uint32_t cap = std::max<uint32_t>(nq, 64);
     BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_q, (size_t)cap * sizeof(EngQuery), hipHostMallocDefault));
-    BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_act, (size_t)cap * 4, hipHostMallocDefault));
-    BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_vfirst, (size_t)cap * 8, hipHostMallocDefault));
-    BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_chunk_base, (size_t)cap * 4, hipHostMallocDefault));
+    BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_act, (size_t)cap * 4, hipHostMallocCoherent));
+    BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_vfirst, (size_t)cap * 8, hipHostMallocCoherent));
+    BCMC_HIP_CHECK(hipHostMalloc((void **)&e->h_chunk_base, (size_t)cap * 4, hipHostMallocCoherent));
+    BCMC_HIP_CHECK(hipHostGetDevicePointer((void **)&e->h_act_d, e->h_act, 0));
+    BCMC_HIP_CHECK(hipHostGetDevicePointer((void **)&e->h_vfirst_d, e->h_vfirst, 0));
+    BCMC_HIP_CHECK(hipHostGetDevicePointer((void **)&e->h_chunk_base_d, e->h_chunk_base, 0));
     BCMC_TRY(dev_alloc((void **)&e->d_q, (size_t)cap * sizeof(EngQuery)));
     BCMC_TRY(dev_alloc((void **)&e->q_act_start, (size_t)cap * 4));
     BCMC_TRY(dev_alloc((void **)&e->q_pair_end, (size_t)cap * 4));
     BCMC_TRY(dev_alloc((void **)&e->q_chunk_base, (size_t)cap * 4));
-    BCMC_TRY(dev_alloc((void **)&e->q_vfirst, (size_t)cap * 8));
     e->q_cap = cap;
     return BCMC_OK;
 }
@@ -937,7 +960,7 @@ static EngMcArgs eng_mc_args(bcmc_engine *e, uint32_t nq) {
     a.counts = e->counts;
     a.chunk_verts = e->chunk_verts;
     a.chunk_verts_pre = e->chunk_verts_pre;
-    a.q_vfirst = e->q_vfirst;
+    a.emit_chunks = e->h_emit_d;
     a.verts = e->verts;
     a.vert_cap = e->vert_cap;
     a.fmt = e->fmt;
@@ -992,7 +1015,8 @@ static int eng_launch_count(bcmc_engine *e, uint32_t nq, uint64_t total) {
     eng_count_kernel<<<grid, kEngThreads, 0, e->ctx->stream>>>(a);
     launch_scan<1>(e->ctx, e->site_chunks, e->site_chunks,
                    EngChunkPrefixOp{a.chunk_verts, a.chunk_verts_pre, a.c}, max_chunks);
-    eng_vfirst_kernel<<<div_up(nq, 256), 256, 0, e->ctx->stream>>>(a);
+    eng_vfirst_kernel<<<div_up(nq, 256), 256, 0, e->ctx->stream>>>(
+        a, EngReadback{e->ch_d, e->h_act_d, e->h_vfirst_d, e->h_chunk_base_d});
     BCMC_HIP_CHECK(hipGetLastError());
     return BCMC_OK;
 }
@@ -1080,13 +1104,18 @@ BCMC_API int bcmc_engine_create(bcmc_ctx *ctx, uint64_t pool_bytes, bcmc_engine 
         rc = sort_scratch_alloc(&e->sort, e->pool_cap);
     if (rc == BCMC_OK)
         rc = scan_site_alloc(&e->site_cand, e->pool_cap);
-    if (rc == BCMC_OK && hipHostMalloc(&e->ch, sizeof(EngCounters), hipHostMallocDefault) != hipSuccess)
+    if (rc == BCMC_OK && (hipHostMalloc(&e->ch, sizeof(EngCounters), hipHostMallocCoherent) != hipSuccess ||
+                          hipHostMalloc(&e->h_emit, 4, hipHostMallocCoherent) != hipSuccess))
         rc = BCMC_ERR_NOMEM;
+    if (rc == BCMC_OK && (hipHostGetDevicePointer((void **)&e->ch_d, e->ch, 0) != hipSuccess ||
+                          hipHostGetDevicePointer((void **)&e->h_emit_d, e->h_emit, 0) != hipSuccess))
+        rc = BCMC_ERR_HIP;
     if (rc != BCMC_OK) {
         bcmc_engine_destroy(e);
         return rc;
     }
     *e->ch = EngCounters{};
+    *e->h_emit = 0;
     const char *ng = std::getenv("BCMC_NO_GRAPH");
     e->use_graph = !(ng && ng[0] == '1');
     BCMC_HIP_CHECK(hipMemsetAsync(e->c, 0, sizeof(EngCounters), ctx->stream));
@@ -1133,7 +1162,6 @@ BCMC_API int bcmc_engine_destroy(bcmc_engine *e) {
     dev_free(e->q_act_start);
     dev_free(e->q_pair_end);
     dev_free(e->q_chunk_base);
-    dev_free(e->q_vfirst);
     dev_free(e->same_list);
     dev_free(e->tile_q);
     for (void *h : {(void *)e->h_q, (void *)e->h_same, (void *)e->h_act, (void *)e->h_vfirst,
@@ -1146,6 +1174,8 @@ BCMC_API int bcmc_engine_destroy(bcmc_engine *e) {
     scan_site_free(&e->site_chunks);
     if (e->ch)
         (void)hipHostFree(e->ch);
+    if (e->h_emit)
+        (void)hipHostFree(e->h_emit);
     delete e;
     return BCMC_OK;
 }
@@ -1530,12 +1560,10 @@ static int eng_run(bcmc_engine *e, const uint32_t *volume_ids, const float *isov
     BCMC_TRY(eng_run_part(e, front_key, [&]() { return eng_enqueue_step(e, p); }, &replayed));
     ++e->steps;
     e->graph_steps += replayed ? 1 : 0;
-    BCMC_HIP_CHECK(hipMemcpyAsync(e->ch, e->c, sizeof(EngCounters), hipMemcpyDeviceToHost, st));
+    // the chain's last launch wrote the counters and the per-query results
+    // into host-coherent memory
     const unsigned long long *qv = e->h_vfirst;
     const uint32_t *qa = e->h_act;
-    BCMC_HIP_CHECK(hipMemcpyAsync(e->h_vfirst, e->q_vfirst, nq * 8, hipMemcpyDeviceToHost, st));
-    BCMC_HIP_CHECK(hipMemcpyAsync(e->h_act, e->q_act_start, nq * 4, hipMemcpyDeviceToHost, st));
-    BCMC_HIP_CHECK(hipMemcpyAsync(e->h_chunk_base, e->q_chunk_base, nq * 4, hipMemcpyDeviceToHost, st));
     BCMC_HIP_CHECK(hipStreamSynchronize(st));
     if (e->ch->short_slots) {
         set_error("decoded-block pool too small for this step: %u blocks found no slot",
@@ -1570,8 +1598,9 @@ static int eng_run(bcmc_engine *e, const uint32_t *volume_ids, const float *isov
         BCMC_TRY(dev_alloc(&e->verts, want * vb));
         e->vert_cap = want;
     }
-    e->ch->emit_chunks = nadm == nq ? e->ch->num_chunks : e->h_chunk_base[nadm];
-    BCMC_HIP_CHECK(hipMemcpyAsync(&e->c->emit_chunks, &e->ch->emit_chunks, 4, hipMemcpyHostToDevice, st));
+    // read by the emit launch from host-coherent memory (the previous step's
+    // emit has finished: every step ends with a stream synchronisation)
+    *e->h_emit = nadm == nq ? e->ch->num_chunks : e->h_chunk_base[nadm];
     const EngKey back_key = {nq | 1ull << 40, total, 0, 0, (uint64_t)e->fmt | ctx->tune_gen << 8, e->gen};
     BCMC_TRY(eng_run_part(e, back_key, [&]() { return eng_launch_emit(e, nq, total); }, &replayed));
     BCMC_HIP_CHECK(hipStreamSynchronize(st));
