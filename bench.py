@@ -285,6 +285,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes instead of timing candidates first")
+    ap.add_argument("--graph-replay", action="store_true",
+                    help="replay captured HIP graphs of the query chains instead of launching "
+                         "their kernels directly (the default, measured faster)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -304,6 +307,7 @@ def main():
         f"in {time.time() - t0:.1f}s")
 
     dev = bcmc_amd.Device(0)
+    dev.set_graph_replay(args.graph_replay)
     info = dev.info()
     tuning = dev.tuning() if args.no_autotune else dev.autotune()
     log(f"bench: launch shapes (workgroups per CU) {tuning}")
@@ -376,7 +380,7 @@ def main():
     qps = queries / elapsed
     lat = np.array([r["wall_ms"] for r in records])
     # per-stage device times: the same steps again, timed with events (direct
-    # launches; the timed region above replays the captured query graph)
+    # launches with events between the stages)
     records = []
     for i in range(args.steps):
         step(records, timed=True)
@@ -446,6 +450,7 @@ def main():
                               "numVertices")},
         "cpu_baseline": None,
         "tuning": tuning,
+        "launch": "graph replay" if args.graph_replay else "direct",
     }
 
     if not args.no_batched:

@@ -241,7 +241,8 @@ BCMC_API int bcmc_mc_get_block_ranges(bcmc_mc *mc, bcmc_ranges_format fmt, void 
 BCMC_API int bcmc_mc_set_vertex_format(bcmc_mc *mc, bcmc_vertex_format fmt);
 /* computeSurface(isovalue, perfTracker) (compressed_marching_cubes.js:580-809).
  * The isovalue is used as FP32 (:613-619).  perf may be NULL: the query chain
- * is then replayed from a HIP graph captured on first use (and captured again
+ * is then launched in the context's launch mode (bcmc_ctx_set_graph_replay:
+ * direct launches, or a HIP graph captured on first use and captured again
  * when its buffers change); with perf the stages are launched directly and
  * timed with events.  Results are identical either way. */
 BCMC_API int bcmc_mc_compute_surface(bcmc_mc *mc, float isovalue, bcmc_perf *perf,
@@ -376,6 +377,17 @@ BCMC_API int bcmc_zfp_encode(bcmc_ctx *ctx, bcmc_buf *field, const uint32_t dims
  * stream); bcmc_ctx_last_kernel_ms returns it in ms (0 if none recorded). */
 BCMC_API int bcmc_ctx_enable_timing(bcmc_ctx *ctx, int enable);
 BCMC_API double bcmc_ctx_last_kernel_ms(bcmc_ctx *ctx);
+
+/* How the query chains of the pipelines and engines on this context are
+ * launched.  0 (the default): every kernel is launched directly.  1: a chain
+ * is captured into a HIP graph the second time its shape is seen (a
+ * pipeline's untimed computeSurface; an engine step's front and emit parts)
+ * and replayed after that.  Results are identical either way; on MI355X with
+ * ROCm 7.2 the direct launches measured faster (plain queries 2%, a served
+ * request alone in its step 5%).  No reference counterpart (WebGPU command
+ * buffers are recorded per call in compressed_marching_cubes.js). */
+BCMC_API int bcmc_ctx_set_graph_replay(bcmc_ctx *ctx, int enable);
+BCMC_API int bcmc_ctx_get_graph_replay(bcmc_ctx *ctx, int *enabled);
 
 /* Launch shapes of the context's persistent kernels: workgroups per compute
  * unit for the single-pass scans, the radix-sort passes, the zfp decoders and

@@ -25,5 +25,14 @@ def dev():
     return device()
 
 
+@pytest.fixture
+def replay_dev():
+    """The device with graph replay of the query chains on for one test."""
+    d = device()
+    d.set_graph_replay(True)
+    yield d
+    d.set_graph_replay(False)
+
+
 def say(*a):
     print(*a, flush=True)

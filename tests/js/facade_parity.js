@@ -63,11 +63,13 @@ async function main() {
     out.memory = mc.reportMemoryUse()[3];
 
     // persisted block ranges: without and with the tile ranges, loaded into
-    // new objects in place of the load-time decode
+    // new objects in place of the load-time decode; the second object's
+    // untimed queries replay a captured graph of the query chain
     const persisted = {block: mc.blockRanges(false), tile: mc.blockRanges(true)};
     fs.writeFileSync(path.join(dir, 'ranges.bin'), Buffer.from(persisted.block.buffer));
     out.persisted = {};
     for (const name of Object.keys(persisted)) {
+        device.setGraphReplay(name === 'tile');
         const m2 = new bcmc.CompressedMarchingCubes(device);
         await m2.setCompressedVolume(stream, p.rate, p.dims, [1, 1, 1], p.cache, persisted[name]);
         out.persisted[name] = [];
@@ -79,6 +81,7 @@ async function main() {
         }
         m2.destroy();
     }
+    device.setGraphReplay(false);
     await mc.lruCache.reset();
     out.after_reset_age0 = mc.lruCache.state().age[0];
     mc.destroy();

@@ -7,7 +7,7 @@ import pytest
 import bcmc_amd
 import oracle
 from bcmc_amd import synth
-from gpu_common import dev, say  # noqa: F401
+from gpu_common import dev, replay_dev, say  # noqa: F401
 
 pytestmark = pytest.mark.gpu
 
@@ -334,7 +334,7 @@ def test_many_queries_in_one_step(dev):
 
 
 @pytest.mark.parametrize("pool_blocks", [100000, 3000])
-def test_repeated_step_shapes_replay_a_graph(dev, pool_blocks):
+def test_repeated_step_shapes_replay_a_graph(replay_dev, pool_blocks):
     """A step shape (volumes in query order) seen a second time is captured
     into a HIP graph and replayed whenever it comes again; replayed steps with
     new isovalues, after a step of another shape and (small pool) with the
@@ -342,7 +342,7 @@ def test_repeated_step_shapes_replay_a_graph(dev, pool_blocks):
     queries."""
     say("engine graph replay, pool", pool_blocks)
     streams = _streams()
-    eng = bcmc_amd.Engine(dev, pool_blocks * 256)
+    eng = bcmc_amd.Engine(replay_dev, pool_blocks * 256)
     ids = [eng.add_volume(s, r, d) for s, r, d in streams]
     shape_a = np.array([ids[0], ids[1], ids[1], ids[2], ids[3], ids[0]], np.uint32)
     shape_b = shape_a[1:4]
@@ -365,6 +365,20 @@ def test_repeated_step_shapes_replay_a_graph(dev, pool_blocks):
     assert st["steps"] == len(plan)
     # vertex-buffer growth re-captures, so not every repeat replays
     assert st["graph_steps"] >= 2, st
+
+
+def test_steps_launch_directly_by_default(dev):
+    """Without graph replay on the device, no step replays a graph."""
+    assert not dev.graph_replay
+    streams = _streams()
+    eng = bcmc_amd.Engine(dev, 100000 * 256)
+    ids = [eng.add_volume(s, r, d) for s, r, d in streams]
+    for _ in range(3):
+        res = eng.run([ids[0], ids[2]], [0.4, 0.6])
+        for k, (v, iso) in enumerate(((0, 0.4), (2, 0.6))):
+            assert res[k]["num_vertices"] == _oracle(*streams[v], iso)[0]
+    st = eng.stats()
+    assert st["steps"] == 3 and st["graph_steps"] == 0, st
 
 
 def test_nonfinite_and_tied_isovalues_in_steps(dev):

@@ -16,7 +16,7 @@ import pytest
 import bcmc_amd
 import oracle
 from bcmc_amd import synth
-from gpu_common import dev, say  # noqa: F401
+from gpu_common import dev, replay_dev, say  # noqa: F401
 
 pytestmark = pytest.mark.gpu
 
@@ -60,7 +60,10 @@ def _decoded_bytes(vols):
     return sum(oracle.nblocks(d) for _, _, d in vols) * 256
 
 
-def test_batched_64_queries_over_uneven_volumes(dev):
+def test_batched_64_queries_over_uneven_volumes(replay_dev):
+    """With graph replay on: the step is captured the second time and
+    replayed the third, and every run equals the oracle."""
+    dev = replay_dev
     vols = _volumes(BATCH)
     rng = np.random.default_rng(3)
     ids = rng.integers(0, len(vols), 64).astype(np.uint32)

@@ -11,7 +11,7 @@ import pytest
 import bcmc_amd
 import oracle
 from bcmc_amd import synth
-from gpu_common import dev, say  # noqa: F401
+from gpu_common import dev, replay_dev, say  # noqa: F401
 
 pytestmark = pytest.mark.gpu
 
@@ -90,6 +90,34 @@ def test_turbulent_sweep_with_eviction(dev):
         assert np.array_equal(v.view(np.uint32), r["verts_f32"].view(np.uint32))
         for a, b, what in zip(mc.lruCache.state(), ref.lru_state(), ("age", "avail", "item", "slot")):
             assert np.array_equal(a, b), (iso, what)
+
+
+def test_graph_replay_matches_the_oracle(replay_dev):
+    """With graph replay on, untimed queries replay a captured graph of the
+    query chain (captured again after the vertex buffer or the cache storage
+    grows); a sweep with eviction, a reset and the sweep again equal the
+    oracle query for query, and so do timed queries mixed in."""
+    dims, rate = (64, 48, 40), 4
+    say("mc graph replay", dims)
+    f = synth.field(synth.FOURIER, 3, dims, 96, (1, 16, 0, 0))
+    s = synth.encode(f, 64 * rate)
+    n = oracle.nblocks(dims)
+    mc = bcmc_amd.CompressedMarchingCubes(replay_dev)
+    mc.setCompressedVolume(s, rate, dims, (1, 1, 1), n // 4)
+    ref = oracle.MC(s, 64 * rate, dims, initial_cache=n // 4)
+    isos = np.linspace(0.2, 0.8, 6).astype(np.float32)
+    for sweep in range(2):
+        for k, iso in enumerate(isos):
+            if k == 3:
+                nv = mc.computeSurface(float(iso), {})
+                assert nv == ref.compute_surface(float(iso))
+                r = ref.results()
+            else:
+                nv, r = _check_query(mc, ref, float(iso))
+            v = mc.vertices()
+            assert np.array_equal(v.view(np.uint32), r["verts_f32"].view(np.uint32)), (sweep, k)
+        mc.lruCache.reset()
+        ref.lru_reset()
 
 
 def test_vertex_formats(dev):

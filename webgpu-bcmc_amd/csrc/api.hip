@@ -370,6 +370,18 @@ BCMC_API int bcmc_ctx_enable_timing(bcmc_ctx *ctx, int enable) {
     return BCMC_OK;
 }
 
+BCMC_API int bcmc_ctx_set_graph_replay(bcmc_ctx *ctx, int enable) {
+    BCMC_CHECK_ARG(ctx, "null context");
+    ctx->graph_replay = enable != 0;
+    return BCMC_OK;
+}
+
+BCMC_API int bcmc_ctx_get_graph_replay(bcmc_ctx *ctx, int *enabled) {
+    BCMC_CHECK_ARG(ctx && enabled, "null argument");
+    *enabled = ctx->graph_replay ? 1 : 0;
+    return BCMC_OK;
+}
+
 BCMC_API double bcmc_ctx_last_kernel_ms(bcmc_ctx *ctx) { return ctx ? ctx->last_kernel_ms : 0.0; }
 
 BCMC_API int bcmc_buf_alloc(bcmc_ctx *ctx, uint64_t bytes, bcmc_buf **out) {

@@ -69,6 +69,8 @@ struct bcmc_ctx {
     } tune;
     // bumped on every change of `tune`: captured graphs keyed on it re-capture
     uint64_t tune_gen = 0;
+    // query chains replayed from captured HIP graphs (bcmc_ctx_set_graph_replay)
+    bool graph_replay = false;
 };
 
 struct bcmc_buf {

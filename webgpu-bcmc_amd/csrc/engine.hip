@@ -812,11 +812,13 @@ struct bcmc_engine {
     int fmt = BCMC_VERTS_LOCAL_F32;
     bcmc_buf vert_view;
     uint64_t budget = 0;
-    // replay of the step's kernel chain from captured HIP graphs: a step
-    // shape (key in bcmc_engine_run) is captured the second time it is seen
-    // among the last kEngSeen steps, and replayed whenever it comes again; the
-    // kEngGraphs most recently used graphs are kept.  gen is bumped whenever a
-    // buffer the chain reads is replaced, which also drops every graph.
+    // replay of the step's kernel chain from captured HIP graphs, with the
+    // context's graph replay on: a step shape (key in bcmc_engine_run) is
+    // captured the second time it is seen among the last kEngSeen steps, and
+    // replayed whenever it comes again; the kEngGraphs most recently used
+    // graphs are kept.  gen is bumped whenever a buffer the chain reads is
+    // replaced, which also drops every graph.  use_graph turns off when a
+    // capture fails.
     bool use_graph = true;
     uint64_t gen = 0;
     std::vector<EngGraph> graphs;
@@ -1116,8 +1118,6 @@ BCMC_API int bcmc_engine_create(bcmc_ctx *ctx, uint64_t pool_bytes, bcmc_engine 
     }
     *e->ch = EngCounters{};
     *e->h_emit = 0;
-    const char *ng = std::getenv("BCMC_NO_GRAPH");
-    e->use_graph = !(ng && ng[0] == '1');
     BCMC_HIP_CHECK(hipMemsetAsync(e->c, 0, sizeof(EngCounters), ctx->stream));
     BCMC_HIP_CHECK(hipMemsetAsync(e->stamp, 0, (size_t)e->pool_cap * 4, ctx->stream));
     BCMC_HIP_CHECK(hipMemsetAsync(e->owner, 0xFF, (size_t)e->pool_cap * 8, ctx->stream));
@@ -1380,7 +1380,7 @@ template <class F>
 static int eng_run_part(bcmc_engine *e, const EngKey &key, F &&enqueue, bool *replayed) {
     hipStream_t st = e->ctx->stream;
     EngGraph *hit = nullptr;
-    if (e->use_graph) {
+    if (e->ctx->graph_replay && e->use_graph) {
         for (EngGraph &g : e->graphs)
             hit = g.key == key ? &g : hit;
         const bool seen = std::find(e->seen.begin(), e->seen.end(), key) != e->seen.end();

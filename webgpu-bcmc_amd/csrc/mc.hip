@@ -270,9 +270,10 @@ struct bcmc_mc {
     hipEvent_t ev[7] = {};
     uint32_t last_active = 0, last_with = 0, last_new = 0;
     uint64_t last_verts = 0;
-    // the query chain is captured once into a HIP graph and replayed; it is
-    // captured again when anything it bakes in (buffers, format, sort passes)
-    // changes.  The isovalue is read by the active scan straight from
+    // with the context's graph replay on, the query chain is captured once
+    // into a HIP graph and replayed; it is captured again when anything it
+    // bakes in (buffers, format, sort passes) changes.  use_graph turns off
+    // when a capture fails.  The isovalue is read by the active scan straight from
     // host-coherent pinned memory, and the counters are written back into
     // such memory by the emit launch's first workgroup (no copy nodes).
     float *iso_h = nullptr;
@@ -478,8 +479,6 @@ BCMC_API int bcmc_mc_create(bcmc_ctx *ctx, bcmc_mc **out) {
     m->lru.q = m->q;
     m->lru.qh = m->qh;
     m->lru.owns = false;
-    const char *ng = std::getenv("BCMC_NO_GRAPH");
-    m->use_graph = !(ng && ng[0] == '1');
     *out = m;
     return BCMC_OK;
 }
@@ -656,7 +655,7 @@ BCMC_API int bcmc_mc_compute_surface(bcmc_mc *m, float isovalue, bcmc_perf *perf
                              (uint64_t)m->fmt | ctx->tune_gen << 8, npasses,
                              (uint64_t)(uintptr_t)m->lru.st.data, m->lru.st.slots};
     bool launched = false;
-    if (m->use_graph && !timed) {
+    if (ctx->graph_replay && m->use_graph && !timed) {
         if (!m->gexec || std::memcmp(key, m->graph_key, sizeof(key)) != 0) {
             mc_drop_graph(m);
             BCMC_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));

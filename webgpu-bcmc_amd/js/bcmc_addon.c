@@ -205,6 +205,18 @@ static napi_value ctx_autotune(napi_env env, napi_callback_info info)
     return o;
 }
 
+/* ctxSetGraphReplay(ctx, enable): query chains launched directly (false) or
+ * replayed from captured HIP graphs (true) */
+static napi_value ctx_set_graph_replay(napi_env env, napi_callback_info info)
+{
+    GET_ARGS(2);
+    EXT(0, bcmc_ctx, ctx);
+    bool on = false;
+    NAPI_CALL(env, napi_get_value_bool(env, argv[1], &on));
+    BCMC_CALL(env, bcmc_ctx_set_graph_replay(ctx, on ? 1 : 0));
+    return NULL;
+}
+
 static napi_value buf_alloc(napi_env env, napi_callback_info info)
 {
     GET_ARGS(2);
@@ -530,8 +542,9 @@ static napi_value mc_set_format(napi_env env, napi_callback_info info)
 }
 
 /* computeSurface: with `timed` the per-stage device times are recorded
- * (direct launches); otherwise the captured query graph is replayed and only
- * the counts are filled in (from bcmc_mc_get_info). */
+ * (direct launches); otherwise the query chain runs in the context's launch
+ * mode (ctxSetGraphReplay) and only the counts are filled in (from
+ * bcmc_mc_get_info). */
 static int mc_surface_call(bcmc_mc *mc, float iso, int timed, bcmc_perf *p, uint64_t *nv)
 {
     memset(p, 0, sizeof(*p));
@@ -1184,6 +1197,7 @@ static napi_value init(napi_env env, napi_value exports)
         FN("version", version),
         FN("ctxCreate", ctx_create), FN("ctxDestroy", ctx_destroy), FN("ctxSync", ctx_sync),
         FN("ctxInfo", ctx_info), FN("ctxAutotune", ctx_autotune),
+        FN("ctxSetGraphReplay", ctx_set_graph_replay),
         FN("bufAlloc", buf_alloc), FN("bufFree", buf_free), FN("bufSize", buf_size),
         FN("bufUpload", buf_upload), FN("bufDownload", buf_download),
         FN("scanAlignedSize", scan_aligned_size), FN("scanExclusive", scan_exclusive),

@@ -78,6 +78,11 @@ class Device {
     autotune() {
         return native.ctxAutotune(this.handle);
     }
+    // Query chains launched directly (false, the default) or replayed from
+    // captured HIP graphs (true); results are identical either way.
+    setGraphReplay(enable) {
+        native.ctxSetGraphReplay(this.handle, !!enable);
+    }
     destroy() {
         if (this.handle) native.ctxDestroy(this.handle);
         this.handle = null;
@@ -266,7 +271,7 @@ class CompressedMarchingCubes {
         native.mcSetFormat(this.handle, fmt);
     }
     async computeSurface(isovalue, perfTracker) {
-        // without a perfTracker the captured query graph is replayed (no stage times)
+        // without a perfTracker the chain runs in the device's launch mode (no stage times)
         const r = await this.device._submit(
             () => native.mcComputeSurfaceAsync(this.handle, isovalue, !!perfTracker));
         if (perfTracker) {

@@ -134,6 +134,8 @@ def lib():
         "bcmc_ctx_sync": (i32, [vp]),
         "bcmc_ctx_info": (i32, [vp, P(i32), ctypes.c_char_p, ctypes.c_size_t]),
         "bcmc_ctx_enable_timing": (i32, [vp, i32]),
+        "bcmc_ctx_set_graph_replay": (i32, [vp, i32]),
+        "bcmc_ctx_get_graph_replay": (i32, [vp, P(i32)]),
         "bcmc_ctx_last_kernel_ms": (ctypes.c_double, [vp]),
         "bcmc_ctx_get_tuning": (i32, [vp, P(_Tuning)]),
         "bcmc_ctx_set_tuning": (i32, [vp, P(_Tuning)]),
@@ -290,6 +292,18 @@ class Device:
         t = _Tuning()
         _check(lib().bcmc_ctx_autotune(self.h, ctypes.byref(t)))
         return self._tuning_dict(t)
+
+    def set_graph_replay(self, enable):
+        """Launch the query chains of pipelines and engines on this device
+        directly (False, the default) or replay them from captured HIP graphs
+        (True); results are identical either way."""
+        _check(lib().bcmc_ctx_set_graph_replay(self.h, 1 if enable else 0))
+
+    @property
+    def graph_replay(self):
+        v = ctypes.c_int()
+        _check(lib().bcmc_ctx_get_graph_replay(self.h, ctypes.byref(v)))
+        return bool(v.value)
 
     def create_buffer(self, nbytes):
         return Buffer.alloc(self, nbytes)
@@ -538,8 +552,9 @@ class CompressedMarchingCubes:
         """Returns the vertex count; fills perf_tracker (dict of lists) like the reference.
 
         timed (default: whether perf_tracker is given) records the per-stage
-        device times in lastPerf; untimed queries replay the captured graph of
-        the query chain and lastPerf holds the counts only."""
+        device times in lastPerf; untimed queries run in the device's launch
+        mode (direct launches, or the replayed graph of the query chain:
+        Device.set_graph_replay) and lastPerf holds the counts only."""
         if timed is None:
             timed = perf_tracker is not None
         perf = _Perf()
